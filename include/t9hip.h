@@ -1,0 +1,140 @@
+/*
+ * t9hip.h — C ABI of the MI355X (gfx950) AMOEBA electrostatics path.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns T9_OK (0) or a T9_ERR_* code; t9_last_error() describes the last
+ * failure on the calling thread.  A context (t9_ctx) owns all device memory
+ * for one system on one GPU and is not thread-safe.  Host arrays are laid out
+ * [n][3] (xyz, dipoles, fields, gradients) or [n][10] (multipoles) in the
+ * caller's atom order, FP64, units as in the reference: Å, e·Å (dipoles),
+ * e/Å² (fields), kcal/mol, kcal/mol/Å.
+ *
+ * Each device entry point names the reference interface it replaces
+ * (paths relative to the TinkerTools/tinker-gpu tree).
+ */
+#ifndef T9HIP_H
+#define T9HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  T9_OK = 0,
+  T9_ERR_HIP = 1,            /* a HIP runtime call failed (device missing, fault, OOM) */
+  T9_ERR_NOT_CONVERGED = 2,  /* induce: iteration limit reached
+                                (src/cu/amoeba/pcg.cu:180-184 throws here) */
+  T9_ERR_LIST_OVERFLOW = 3,  /* neighbor list exceeded its capacity
+                                (src/cu/spatial.cu:708-716 throws here) */
+  T9_ERR_ARG = 4,            /* bad argument or unsupported option */
+  T9_ERR_SETUP = 5           /* parameter / coordinate / keyword setup failed */
+};
+
+const char* t9_last_error(void);
+
+/* ----------------------------------------------------------- host system */
+/* A host-side system: coordinates, force-field parameters, keywords and the
+ * derived topology (1-2..1-5 connectivity, polarization groups, local frames,
+ * pair scale factors).  Replaces the reference's Fortran setup chain
+ * (test/testrt.cpp:184-193: initial, getxyz, mechanic) whose sources are not
+ * in the reference tree. */
+typedef struct t9_system t9_system;
+
+/* Tinker .xyz + .prm + keyword text (the contents of a .key file). */
+int t9_system_from_files(const char* xyz_path, const char* prm_path, const char* key_text,
+                         t9_system** out);
+/* Seeded AMOEBA water box: nmol rigid waters (O-H 0.9572 Å, 104.52°) on an
+ * nside^3 cubic lattice in a cube of edge box_len, random orientation and
+ * ±0.1 Å jitter from `seed`, plus n_na Na+ and n_cl Cl- on unused sites at
+ * least 3 Å from any oxygen. */
+int t9_system_water_box(int nmol, int nside, double box_len, uint64_t seed, int n_na, int n_cl,
+                        const char* prm_path, const char* key_text, t9_system** out);
+void t9_system_destroy(t9_system* sys);
+int t9_system_natoms(const t9_system* sys);
+/* Named per-atom arrays.  f64: "xyz" [n][3], "pole" [n][10] (local frame),
+ * "polarity", "pdamp", "thole", "mass" [n], "thlval" [nj*nj], "box" [3].
+ * i32: "type", "frame", "zaxis", "xaxis", "yaxis", "ysign", "jpolar" [n],
+ * "i12" [n][4] (-1 padded). */
+int t9_system_get_f64(const t9_system* sys, const char* name, double* out);
+int t9_system_get_i32(const t9_system* sys, const char* name, int* out);
+/* Scalars: "aewald", "ewald_cut", "ewald", "grid1", "grid2", "grid3",
+ * "pme_order", "njpolar", "nscale", "polar_eps", "polar_iter", "pcg_prec",
+ * "pcg_guess", "pcg_peek", "usolve_cut", "udiag", "list_buffer", "periodic",
+ * "use_mpole", "use_polar", "mpole_cut", "vdw_cut". */
+int t9_system_get_scalar(const t9_system* sys, const char* name, double* out);
+/* Pairs with any scale factor != 1: ik [nscale][2] (i < k), mdpu [nscale][4]. */
+int t9_system_get_scales(const t9_system* sys, int* ik, double* mdpu);
+int t9_system_set_xyz(t9_system* sys, const double* xyz);
+/* Override a keyword after setup (same names as t9_system_get_scalar). */
+int t9_system_set_scalar(t9_system* sys, const char* name, double value);
+
+/* ---------------------------------------------------------------- context */
+typedef struct t9_ctx t9_ctx;
+
+enum { T9_FP32 = 32, T9_FP64 = 64 };
+
+/* Allocate device state for `sys` on the current HIP device.  precision
+ * selects the pair/PME/vector arithmetic (T9_FP32 or T9_FP64); coordinates,
+ * dot products, residual norms and energy/virial sums are FP64 in both.
+ * Replaces the resource chain initialize() -> epolarData / pmeData /
+ * nblistData (src/rcman.cpp:49-80, src/amoeba/epolar.cpp:25-511,
+ * src/pme.cpp:119-217, src/nblist.cpp:317-650). */
+int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out);
+int t9_ctx_destroy(t9_ctx* ctx);
+
+/* Host -> device coordinates [n][3] (replaces the x/y/z device upload of
+ * copyPosToXyz, src/atom.cpp:153-166).  Does not rebuild the list. */
+int t9_set_coords(t9_ctx* ctx, const double* xyz);
+/* Device -> host coordinates. */
+int t9_get_coords(t9_ctx* ctx, double* xyz);
+
+/* Neighbor list refresh: rebuild when any atom moved more than half the list
+ * buffer since the last build, else keep.  *rebuilt (may be NULL) reports
+ * which.  Replaces nblistRefresh() (src/nblist.cpp:534) and
+ * spatialCheck_cu / spatialDataInit_cu (src/cu/spatial.cu:728-960). */
+int t9_nblist_refresh(t9_ctx* ctx, int* rebuilt);
+/* Force a rebuild (nblistData(RcOp::INIT)). */
+int t9_nblist_build(t9_ctx* ctx);
+/* List statistics: [0] list pairs (directed), [1] atoms, [2] waves,
+ * [3] padded slots, [4] builds so far. */
+int t9_nblist_stats(t9_ctx* ctx, int64_t* out5);
+
+/* Local frame -> global multipoles.  Replaces mpoleInit's chkpole/rotpole
+ * (src/amoeba/mpole.cpp:28-58, src/cu/amoeba/rotpole.cu:7-42). */
+int t9_mpole_init(t9_ctx* ctx);
+int t9_get_rpole(t9_ctx* ctx, double* rpole);
+
+/* Direct field of the permanent multipoles, d and p schemes, [n][3] each.
+ * Replaces dfield(field, fieldp) -> dfieldEwald / dfieldNonEwald
+ * (include/ff/amoeba/induce.h:8-20, src/amoeba/field.cpp:17-63). */
+int t9_dfield(t9_ctx* ctx, double* field, double* fieldp);
+/* Mutual field -T·u of two dipole sets.  Replaces ufield(uind, uinp, field,
+ * fieldp) -> ufieldEwald / ufieldNonEwald (src/amoeba/field.cpp:76-117). */
+int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field, double* fieldp);
+
+/* Induced dipole solve statistics. */
+typedef struct {
+  int iterations;       /* PCG iterations */
+  int predictor;        /* 0 direct guess, 1 ASPC, 2 learned */
+  double rms_debye;     /* final RMS residual (Debye) */
+  double guess_rms;     /* RMS residual of the initial guess (Debye) */
+} t9_induce_stats;
+
+/* Mutual induced dipoles by preconditioned CG to polar-eps.  uind/uinp may be
+ * NULL: the result then stays on the device (read it with t9_get_uind).
+ * Replaces induce(uind, uinp) (include/ff/amoeba/induce.h:34,
+ * src/amoeba/induce.cpp:108-113 -> src/cu/amoeba/pcg.cu:14-185). */
+int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats);
+int t9_get_uind(t9_ctx* ctx, double* uind, double* uinp);
+int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
+
+/* Wait for all work queued on the context's stream. */
+int t9_sync(t9_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* T9HIP_H */

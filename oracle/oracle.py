@@ -1,0 +1,166 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/oracle.cpp) — TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this module;
+it is the checker, never the thing measured or shipped.  The oracle takes the
+per-atom arrays produced by the host setup (local-frame multipoles, frames,
+polarizabilities, Thole table, pair scale list) and recomputes the physics in
+double precision.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+def lib_path():
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def build():
+    """Compile the oracle with g++ (no GPU code)."""
+    src = os.path.join(_HERE, "oracle.cpp")
+    out = lib_path()
+    cmd = f"g++ -std=c++17 -O2 -fopenmp -fPIC -shared -o {out}.tmp {src} && mv {out}.tmp {out}"
+    if os.system(cmd) != 0:
+        raise RuntimeError("oracle build failed")
+    return out
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(lib_path()):
+            build()
+        L = ctypes.CDLL(lib_path())
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, _d, _d, _i, _i, _i, _i, _i, _d, _d, _i, ctypes.c_int, _d,
+                                 ctypes.c_int, _i, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i,
+                                 ctypes.c_int]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_xyz.argtypes = [ctypes.c_void_p, _d]
+        L.orc_get_rpole.argtypes = [ctypes.c_void_p, _d]
+        L.orc_get_bsmod.argtypes = [ctypes.c_void_p, ctypes.c_int, _d]
+        L.orc_dfield.argtypes = [ctypes.c_void_p, _d, _d]
+        L.orc_ufield.argtypes = [ctypes.c_void_p, _d, _d, _d, _d]
+        L.orc_recip_dipole.argtypes = [ctypes.c_void_p, _d, _d, _d, _d]
+        L.orc_recip_mpole.argtypes = [ctypes.c_void_p, _d, _d]
+        L.orc_recip_mpole.restype = ctypes.c_double
+        L.orc_dft3.argtypes = [ctypes.c_void_p, _d, ctypes.c_int]
+        L.orc_induce.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, _d, _d, _i, _d]
+        L.orc_induce.restype = ctypes.c_int
+        if hasattr(L, "orc_energy"):
+            L.orc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d, _d, _d]
+            L.orc_energy.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """FP64 reference computation for one system (see module docstring)."""
+
+    def __init__(self, sysd, cut=None):
+        """sysd: dict from t9hip.System.arrays() (numpy arrays + scalars)."""
+        L = _lib()
+        self.n = n = int(sysd["n"])
+        self._keep = []
+
+        def f64(x):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return a
+
+        def i32(x):
+            a = np.ascontiguousarray(x, dtype=np.int32)
+            self._keep.append(a)
+            return a
+
+        xyz = f64(sysd["xyz"])
+        box = f64(sysd["box"]) if sysd["periodic"] else None
+        grid = i32(sysd["grid"])
+        ik = i32(sysd["scale_ik"]) if len(sysd["scale_ik"]) else i32(np.zeros((1, 2)))
+        mdpu = f64(sysd["scale_mdpu"]) if len(sysd["scale_mdpu"]) else f64(np.zeros((1, 4)))
+        if cut is None:
+            cut = sysd["ewald_cut"] if sysd["ewald"] else sysd["mpole_cut"]
+        self.sysd = sysd
+        self.h = L.orc_create(
+            n, _p(xyz), _p(f64(sysd["pole"])), _p(i32(sysd["frame"]), _i), _p(i32(sysd["zaxis"]), _i),
+            _p(i32(sysd["xaxis"]), _i), _p(i32(sysd["yaxis"]), _i), _p(i32(sysd["ysign"]), _i),
+            _p(f64(sysd["polarity"])), _p(f64(sysd["pdamp"])), _p(i32(sysd["jpolar"]), _i), int(sysd["njpolar"]),
+            _p(f64(sysd["thlval"])), int(sysd["nscale"]), _p(ik, _i), _p(mdpu), _p(box) if box is not None else None,
+            int(sysd["ewald"]), float(sysd["aewald"]), float(cut), _p(grid, _i), int(sysd["pme_order"]))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_destroy(self.h)
+            self.h = None
+
+    def set_xyz(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        _lib().orc_set_xyz(self.h, _p(a))
+
+    def rpole(self):
+        out = np.zeros((self.n, 10))
+        _lib().orc_get_rpole(self.h, _p(out))
+        return out
+
+    def bsmod(self, dim):
+        out = np.zeros(int(self.sysd["grid"][dim]))
+        _lib().orc_get_bsmod(self.h, dim, _p(out))
+        return out
+
+    def dfield(self):
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _lib().orc_dfield(self.h, _p(fd), _p(fp))
+        return fd, fp
+
+    def ufield(self, ud, up):
+        ud = np.ascontiguousarray(ud, dtype=np.float64)
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _lib().orc_ufield(self.h, _p(ud), _p(up), _p(fd), _p(fp))
+        return fd, fp
+
+    def recip_dipole(self, ud, up):
+        ud = np.ascontiguousarray(ud, dtype=np.float64)
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _lib().orc_recip_dipole(self.h, _p(ud), _p(up), _p(fd), _p(fp))
+        return fd, fp
+
+    def recip_mpole(self):
+        f = np.zeros((self.n, 3))
+        fphi = np.zeros((self.n, 20))
+        e = _lib().orc_recip_mpole(self.h, _p(f), _p(fphi))
+        return e, f, fphi
+
+    def dft3(self, grid_c, sign):
+        g = np.ascontiguousarray(np.stack([grid_c.real, grid_c.imag], axis=-1), dtype=np.float64)
+        _lib().orc_dft3(self.h, _p(g), sign)
+        return g[..., 0] + 1j * g[..., 1]
+
+    def induce(self, eps=None, maxiter=None, prec=None, guess=None, peek=None, usolve=None, udiag=None):
+        s = self.sysd
+        eps = s["polar_eps"] if eps is None else eps
+        maxiter = int(s["polar_iter"]) if maxiter is None else maxiter
+        prec = int(s["pcg_prec"]) if prec is None else int(prec)
+        guess = int(s["pcg_guess"]) if guess is None else int(guess)
+        peek = s["pcg_peek"] if peek is None else peek
+        usolve = s["usolve_cut"] if usolve is None else usolve
+        udiag = s["udiag"] if udiag is None else udiag
+        ud, up = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        it = np.zeros(1, dtype=np.int32)
+        rms = np.zeros(1)
+        rc = _lib().orc_induce(self.h, eps, maxiter, prec, guess, peek, usolve, udiag, _p(ud), _p(up),
+                               _p(it, _i), _p(rms))
+        return ud, up, int(it[0]), float(rms[0]), rc

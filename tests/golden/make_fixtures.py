@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Write the committed golden fixtures from the reference tree (run once, in a
+container where the reference is mounted; the fixtures are then data in git).
+
+What it writes (numbers and data files only, no reference source text):
+  * tests/golden/local_frame.xyz, local_frame2.xyz  — inputs of
+    test/localframe.cpp and test/localframe3.cpp (test/file/local_frame/)
+  * tests/golden/localframe_ref.json — the reference values of those tests:
+    every `ref_*` array/scalar of test/localframe.cpp:24-536 and
+    test/localframe3.cpp:29-230, keyed "<TEST_CASE>/<SECTION>/<name>"
+  * tests/golden/tinkernist.npz — test/ref/tinkernist.{arc,uind,udir,def,tef}:
+    2 frames x 2,684 atoms (coordinates Å, dipoles Debye, fields as dumped),
+    plus types and 1-2 connectivity of test/file/tinkernist/water30.xyz
+  * tinker-gpu_amd/params/amoeba09.prm — the subset of
+    test/file/commit_6fe8e913/amoeba09.prm this path uses: the scale-factor
+    header and the atom/vdw/bond/angle/ureybrad/multipole/polarize records of
+    the atom types in the test systems and the water/ion boxes.
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+
+TYPES = {7, 15, 36, 37, 61, 62, 210, 211, 212, 213}
+HEADER_KEYS = {"forcefield", "vdwtype", "radiusrule", "radiustype", "radiussize", "epsilonrule", "dielectric",
+               "polarization", "opbendtype", "torsionunit", "angle-trigonal"}
+
+
+def write_prm_subset():
+    src = os.path.join(REF, "test/file/commit_6fe8e913/amoeba09.prm")
+    lines = open(src).read().splitlines()
+    out = ["# Subset of the AMOEBA-2009 parameter set used by the electrostatics path", ""]
+    classes = set()
+    for ln in lines:
+        t = ln.split()
+        if len(t) >= 3 and t[0] == "atom" and int(t[1]) in TYPES:
+            classes.add(int(t[2]))
+    header = True
+    i = 0
+    while i < len(lines):
+        ln = lines[i]
+        t = ln.split()
+        if header:
+            if t and t[0] == "atom":
+                header = False
+            else:
+                if t and len(t) == 2 and (t[0] in HEADER_KEYS or t[0].endswith(("-scale", "-intra", "-cubic",
+                                                                                   "-quartic", "-pentic", "-sextic"))):
+                    out.append(ln)
+                i += 1
+                continue
+        if not t:
+            i += 1
+            continue
+        k = t[0]
+        keep = False
+        if k == "atom" and int(t[1]) in TYPES:
+            keep = True
+        elif k == "vdw" and int(t[1]) in classes:
+            keep = True
+        elif k == "bond" and int(t[1]) in classes and int(t[2]) in classes:
+            keep = True
+        elif k in ("angle", "ureybrad") and all(int(x) in classes for x in t[1:4]):
+            keep = True
+        elif k == "polarize" and int(t[1]) in TYPES:
+            keep = True
+        elif k == "multipole" and int(t[1]) in TYPES:
+            out.append(ln)
+            out.extend(lines[i + 1:i + 5])
+            i += 5
+            continue
+        if keep:
+            out.append(ln)
+        i += 1
+    dst = os.path.join(REPO, "tinker-gpu_amd/params/amoeba09.prm")
+    os.makedirs(os.path.dirname(dst), exist_ok=True)
+    open(dst, "w").write("\n".join(out) + "\n")
+    return dst
+
+
+def parse_test_arrays(path):
+    """Extract `ref_*` numeric arrays/scalars, tracking TEST_CASE / SECTION."""
+    text = open(path).read()
+    out = {}
+    case, section = "", ""
+    pos = 0
+    tok = re.compile(r'TEST_CASE\("([^"]+)"|SECTION\("([^"]+)"\)|const\s+(?:double|int)\s+(ref_\w+)((?:\[\])?(?:\[\d+\])?)\s*=\s*')
+    while True:
+        m = tok.search(text, pos)
+        if not m:
+            break
+        if m.group(1):
+            case, section = m.group(1), ""
+            pos = m.end()
+            continue
+        if m.group(2):
+            section = m.group(2).strip()
+            pos = m.end()
+            continue
+        name = m.group(3)
+        j = m.end()
+        if text[j] == "{":
+            depth, k = 0, j
+            while True:
+                if text[k] == "{":
+                    depth += 1
+                elif text[k] == "}":
+                    depth -= 1
+                    if depth == 0:
+                        break
+                k += 1
+            body = text[j:k + 1]
+            nums = [float(x) for x in re.findall(r"-?\d+\.?\d*(?:[eE][-+]?\d+)?", body)]
+            rows = body.count("{") - 1
+            val = np.array(nums).reshape(rows, -1).tolist() if rows > 0 else nums
+            pos = k + 1
+        else:
+            k = text.index(";", j)
+            expr = text[j:k]
+            parts = [p.strip() for p in expr.split("+")]
+            try:
+                val = sum(float(p) for p in parts)
+            except ValueError:
+                val = None
+                for p in parts:
+                    key = f"{case}/{section}/{p}"
+                    if key in out:
+                        val = (val or 0.0) + out[key]
+            pos = k + 1
+        if val is not None:
+            out[f"{case}/{section}/{name}"] = val
+    return out
+
+
+def read_dump(path):
+    frames, lines = [], open(path).read().splitlines()
+    i = 0
+    while i < len(lines):
+        n = int(lines[i].split()[0])
+        i += 2  # title, box
+        xyz = np.array([[float(v) for v in lines[i + a].split()[2:5]] for a in range(n)])
+        frames.append(xyz)
+        i += n
+    return np.array(frames)
+
+
+def write_tinkernist():
+    base = os.path.join(REF, "test/ref/tinkernist")
+    d = {k: read_dump(base + "." + k) for k in ("arc", "uind", "udir", "def", "tef")}
+    lines = open(os.path.join(REF, "test/file/tinkernist/water30.xyz")).read().splitlines()
+    n = int(lines[0].split()[0])
+    types = np.zeros(n, dtype=np.int32)
+    i12 = -np.ones((n, 4), dtype=np.int32)
+    for a, ln in enumerate(lines[2:2 + n]):
+        t = ln.split()
+        types[a] = int(t[5])
+        for j, b in enumerate(t[6:]):
+            i12[a, j] = int(b) - 1
+    np.savez_compressed(os.path.join(HERE, "tinkernist.npz"), types=types, i12=i12, **d)
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("reference tree not mounted")
+    print("prm subset:", write_prm_subset())
+    for f in ("local_frame.xyz", "local_frame2.xyz"):
+        src = open(os.path.join(REF, "test/file/local_frame", f)).read()
+        open(os.path.join(HERE, f), "w").write(src)
+    ref = {}
+    ref.update(parse_test_arrays(os.path.join(REF, "test/localframe.cpp")))
+    ref.update(parse_test_arrays(os.path.join(REF, "test/localframe3.cpp")))
+    json.dump(ref, open(os.path.join(HERE, "localframe_ref.json"), "w"), indent=0, sort_keys=True)
+    print("golden values:", len(ref))
+    write_tinkernist()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,203 @@
+// C ABI for the host-side system (t9_system_*): setup from Tinker files or
+// the seeded water-box generator, and read access to the derived arrays.
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/t9hip.h"
+#include "errors.hpp"
+#include "system.hpp"
+
+struct t9_system {
+  t9::System s;
+};
+
+namespace {
+
+std::string dirname_of(const std::string& p) {
+  auto k = p.find_last_of('/');
+  return k == std::string::npos ? std::string(".") : p.substr(0, k);
+}
+
+int finish_setup(t9_system* out, const char* prm_path, const char* key_text) {
+  std::string prm_name;
+  if (key_text) t9::read_key_text(key_text, out->s, &prm_name);
+  if (!prm_path || !*prm_path) throw std::runtime_error("parameter file path required");
+  t9::read_prm(prm_path, out->s.ff);
+  t9::setup_system(out->s);
+  return T9_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int t9_system_from_files(const char* xyz_path, const char* prm_path, const char* key_text, t9_system** out) {
+  return t9::guarded([&] {
+    auto* sys = new t9_system;
+    try {
+      t9::read_xyz(xyz_path, sys->s);
+      finish_setup(sys, prm_path, key_text);
+    } catch (...) {
+      delete sys;
+      throw;
+    }
+    *out = sys;
+    return T9_OK;
+  }, T9_ERR_SETUP);
+}
+
+int t9_system_water_box(int nmol, int nside, double box_len, uint64_t seed, int n_na, int n_cl,
+                        const char* prm_path, const char* key_text, t9_system** out) {
+  return t9::guarded([&] {
+    auto* sys = new t9_system;
+    try {
+      t9::build_water_box(sys->s, nmol, nside, box_len, seed, n_na, n_cl);
+      finish_setup(sys, prm_path, key_text);
+    } catch (...) {
+      delete sys;
+      throw;
+    }
+    *out = sys;
+    return T9_OK;
+  }, T9_ERR_SETUP);
+}
+
+void t9_system_destroy(t9_system* sys) { delete sys; }
+
+int t9_system_natoms(const t9_system* sys) { return sys ? sys->s.n : 0; }
+
+int t9_system_get_f64(const t9_system* sys, const char* name, double* out) {
+  return t9::guarded([&] {
+    const t9::System& s = sys->s;
+    std::string k = name;
+    const std::vector<double>* v = nullptr;
+    if (k == "xyz") v = &s.xyz;
+    else if (k == "pole") v = &s.pole;
+    else if (k == "polarity") v = &s.polarity;
+    else if (k == "pdamp") v = &s.pdamp;
+    else if (k == "thole") v = &s.thole;
+    else if (k == "mass") v = &s.mass;
+    else if (k == "thlval") v = &s.thlval;
+    else if (k == "vdw_rad") v = &s.vdw_rad;
+    else if (k == "vdw_eps") v = &s.vdw_eps;
+    else if (k == "vdw_red") v = &s.vdw_red;
+    else if (k == "box") {
+      std::memcpy(out, s.box, 3 * sizeof(double));
+      return T9_OK;
+    } else {
+      throw t9::ArgError("unknown f64 array " + k);
+    }
+    std::memcpy(out, v->data(), v->size() * sizeof(double));
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_get_i32(const t9_system* sys, const char* name, int* out) {
+  return t9::guarded([&] {
+    const t9::System& s = sys->s;
+    std::string k = name;
+    const std::vector<int>* v = nullptr;
+    if (k == "type") v = &s.type;
+    else if (k == "frame") v = &s.frame;
+    else if (k == "zaxis") v = &s.zaxis;
+    else if (k == "xaxis") v = &s.xaxis;
+    else if (k == "yaxis") v = &s.yaxis;
+    else if (k == "ysign") v = &s.ysign;
+    else if (k == "jpolar") v = &s.jpolar;
+    else if (k == "vdw_parent") v = &s.vdw_parent;
+    else if (k == "i12") {
+      for (int i = 0; i < s.n; ++i)
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = j < (int)s.i12[i].size() ? s.i12[i][j] : -1;
+      return T9_OK;
+    } else {
+      throw t9::ArgError("unknown i32 array " + k);
+    }
+    std::memcpy(out, v->data(), v->size() * sizeof(int));
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_get_scalar(const t9_system* sys, const char* name, double* out) {
+  return t9::guarded([&] {
+    const t9::System& s = sys->s;
+    const t9::Options& o = s.opt;
+    std::string k = name;
+    double v;
+    if (k == "aewald") v = o.ewald_alpha;
+    else if (k == "ewald_cut") v = o.ewald_cut;
+    else if (k == "ewald") v = o.ewald;
+    else if (k == "grid1") v = o.pme_grid[0];
+    else if (k == "grid2") v = o.pme_grid[1];
+    else if (k == "grid3") v = o.pme_grid[2];
+    else if (k == "pme_order") v = o.pme_order;
+    else if (k == "njpolar") v = s.njpolar;
+    else if (k == "nscale") v = (double)s.scales.size();
+    else if (k == "polar_eps") v = o.polar_eps;
+    else if (k == "polar_iter") v = o.polar_iter;
+    else if (k == "pcg_prec") v = o.pcg_prec;
+    else if (k == "pcg_guess") v = o.pcg_guess;
+    else if (k == "pcg_peek") v = o.pcg_peek;
+    else if (k == "usolve_cut") v = o.usolve_cut;
+    else if (k == "udiag") v = o.udiag;
+    else if (k == "list_buffer") v = o.list_buffer;
+    else if (k == "periodic") v = s.periodic;
+    else if (k == "use_mpole") v = o.use_mpole;
+    else if (k == "use_polar") v = o.use_polar;
+    else if (k == "mpole_cut") v = o.mpole_cut;
+    else if (k == "vdw_cut") v = o.vdw_cut;
+    else throw t9::ArgError("unknown scalar " + k);
+    *out = v;
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_set_scalar(t9_system* sys, const char* name, double value) {
+  return t9::guarded([&] {
+    t9::Options& o = sys->s.opt;
+    std::string k = name;
+    if (k == "aewald") o.ewald_alpha = value;
+    else if (k == "ewald_cut") o.ewald_cut = value;
+    else if (k == "polar_eps") o.polar_eps = value;
+    else if (k == "polar_iter") o.polar_iter = (int)value;
+    else if (k == "pcg_prec") o.pcg_prec = value != 0;
+    else if (k == "pcg_guess") o.pcg_guess = value != 0;
+    else if (k == "pcg_peek") o.pcg_peek = value;
+    else if (k == "usolve_cut") o.usolve_cut = value;
+    else if (k == "udiag") o.udiag = value;
+    else if (k == "list_buffer") o.list_buffer = value;
+    else if (k == "mpole_cut") o.mpole_cut = value;
+    else if (k == "vdw_cut") o.vdw_cut = value;
+    else throw t9::ArgError("scalar not settable: " + k);
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_get_scales(const t9_system* sys, int* ik, double* mdpu) {
+  return t9::guarded([&] {
+    const auto& sc = sys->s.scales;
+    for (size_t j = 0; j < sc.size(); ++j) {
+      ik[2 * j] = sc[j].i;
+      ik[2 * j + 1] = sc[j].k;
+      mdpu[4 * j] = sc[j].m;
+      mdpu[4 * j + 1] = sc[j].d;
+      mdpu[4 * j + 2] = sc[j].p;
+      mdpu[4 * j + 3] = sc[j].u;
+    }
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_set_xyz(t9_system* sys, const double* xyz) {
+  return t9::guarded([&] {
+    std::memcpy(sys->s.xyz.data(), xyz, sys->s.xyz.size() * sizeof(double));
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+}  // extern "C"
+
+// Accessor used by the device context (ctx.cpp).
+const t9::System& t9_system_ref(const t9_system* sys) { return sys->s; }

@@ -1,0 +1,259 @@
+"""Python binding of libt9hip.so (the C ABI declared in include/t9hip.h).
+
+Host-side mirror of the reference's entry points for the AMOEBA electrostatics
+path: `System` is the setup (keywords, parameters, topology), `Context` holds
+the device state and exposes mpole_init / dfield / ufield / induce /
+nblist_refresh with the reference's argument meaning.  Errors from the C ABI
+raise `T9Error` (with `.code`), mirroring the reference's TINKER_THROW paths.
+This module never falls back to any CPU computation: without the built
+library, or without a GPU, device calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libt9hip.so")
+PARAMS = os.path.join(HERE, "params", "amoeba09.prm")
+
+T9_OK, T9_ERR_HIP, T9_ERR_NOT_CONVERGED, T9_ERR_LIST_OVERFLOW, T9_ERR_ARG, T9_ERR_SETUP = range(6)
+T9_FP32, T9_FP64 = 32, 64
+
+# Every symbol include/t9hip.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "t9_last_error", "t9_system_from_files", "t9_system_water_box", "t9_system_destroy", "t9_system_natoms",
+    "t9_system_get_f64", "t9_system_get_i32", "t9_system_get_scalar", "t9_system_get_scales",
+    "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
+    "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
+    "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
+]
+
+
+class T9Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"t9hip error {code}: {msg}")
+        self.code = code
+
+
+class InduceStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("predictor", ctypes.c_int), ("rms_debye", ctypes.c_double),
+                ("guess_rms", ctypes.c_double)]
+
+
+_LIB = None
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.t9_last_error.restype = ctypes.c_char_p
+        L.t9_system_from_files.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]
+        L.t9_system_water_box.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.POINTER(_vp)]
+        L.t9_system_destroy.argtypes = [_vp]
+        L.t9_system_destroy.restype = None
+        L.t9_system_natoms.argtypes = [_vp]
+        L.t9_system_get_f64.argtypes = [_vp, ctypes.c_char_p, _d]
+        L.t9_system_get_i32.argtypes = [_vp, ctypes.c_char_p, _i]
+        L.t9_system_get_scalar.argtypes = [_vp, ctypes.c_char_p, _d]
+        L.t9_system_get_scales.argtypes = [_vp, _i, _d]
+        L.t9_system_set_xyz.argtypes = [_vp, _d]
+        L.t9_system_set_scalar.argtypes = [_vp, ctypes.c_char_p, ctypes.c_double]
+        for name, args in [
+            ("t9_ctx_create", [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+            ("t9_ctx_destroy", [_vp]),
+            ("t9_set_coords", [_vp, _d]),
+            ("t9_get_coords", [_vp, _d]),
+            ("t9_nblist_refresh", [_vp, _i]),
+            ("t9_nblist_build", [_vp]),
+            ("t9_nblist_stats", [_vp, ctypes.POINTER(ctypes.c_int64)]),
+            ("t9_mpole_init", [_vp]),
+            ("t9_get_rpole", [_vp, _d]),
+            ("t9_dfield", [_vp, _d, _d]),
+            ("t9_ufield", [_vp, _d, _d, _d, _d]),
+            ("t9_induce", [_vp, _d, _d, ctypes.POINTER(InduceStats)]),
+            ("t9_get_uind", [_vp, _d, _d]),
+            ("t9_set_uind", [_vp, _d, _d]),
+            ("t9_sync", [_vp]),
+        ]:
+            if hasattr(L, name):  # device entry points are absent from a host-only build
+                getattr(L, name).argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != T9_OK:
+        raise T9Error(rc, lib().t9_last_error().decode())
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class System:
+    """Host-side system: coordinates, parameters, keywords and topology."""
+
+    def __init__(self, handle):
+        self.h = handle
+        self.n = lib().t9_system_natoms(handle)
+
+    @classmethod
+    def from_files(cls, xyz, prm=PARAMS, key=""):
+        h = _vp()
+        check(lib().t9_system_from_files(xyz.encode(), prm.encode(), key.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def water_box(cls, nmol, nside, box_len, seed, n_na=0, n_cl=0, prm=PARAMS, key=""):
+        h = _vp()
+        check(lib().t9_system_water_box(nmol, nside, box_len, seed, n_na, n_cl, prm.encode(), key.encode(),
+                                        ctypes.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.t9_system_destroy(self.h)
+            self.h = None
+
+    def f64(self, name, shape):
+        out = np.zeros(shape, dtype=np.float64)
+        check(lib().t9_system_get_f64(self.h, name.encode(), _p(out)))
+        return out
+
+    def i32(self, name, shape):
+        out = np.zeros(shape, dtype=np.int32)
+        check(lib().t9_system_get_i32(self.h, name.encode(), _p(out, _i)))
+        return out
+
+    def scalar(self, name):
+        v = ctypes.c_double()
+        check(lib().t9_system_get_scalar(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_scalar(self, name, value):
+        check(lib().t9_system_set_scalar(self.h, name.encode(), float(value)))
+
+    def set_xyz(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        check(lib().t9_system_set_xyz(self.h, _p(a)))
+
+    def scales(self):
+        ns = int(self.scalar("nscale"))
+        ik = np.zeros((ns, 2), dtype=np.int32)
+        mdpu = np.zeros((ns, 4))
+        if ns:
+            check(lib().t9_system_get_scales(self.h, _p(ik, _i), _p(mdpu)))
+        return ik, mdpu
+
+    def arrays(self):
+        """Everything the oracle needs, as numpy arrays and scalars."""
+        n = self.n
+        nj = int(self.scalar("njpolar"))
+        ik, mdpu = self.scales()
+        d = dict(n=n, xyz=self.f64("xyz", (n, 3)), pole=self.f64("pole", (n, 10)),
+                 polarity=self.f64("polarity", n), pdamp=self.f64("pdamp", n), thole=self.f64("thole", n),
+                 thlval=self.f64("thlval", nj * nj), njpolar=nj, frame=self.i32("frame", n),
+                 zaxis=self.i32("zaxis", n), xaxis=self.i32("xaxis", n), yaxis=self.i32("yaxis", n),
+                 ysign=self.i32("ysign", n), jpolar=self.i32("jpolar", n), type=self.i32("type", n),
+                 box=self.f64("box", 3), scale_ik=ik, scale_mdpu=mdpu, nscale=len(ik))
+        for k in ("aewald", "ewald_cut", "ewald", "pme_order", "polar_eps", "polar_iter", "pcg_prec", "pcg_guess",
+                  "pcg_peek", "usolve_cut", "udiag", "list_buffer", "periodic", "mpole_cut", "use_mpole",
+                  "use_polar"):
+            d[k] = self.scalar(k)
+        d["grid"] = np.array([int(self.scalar("grid%d" % j)) for j in (1, 2, 3)], dtype=np.int32)
+        return d
+
+
+class Context:
+    """Device state of one system on the current GPU (t9_ctx)."""
+
+    def __init__(self, system, precision=T9_FP32):
+        self.n = system.n
+        self.system = system
+        h = _vp()
+        check(lib().t9_ctx_create(system.h, precision, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().t9_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_coords(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        check(lib().t9_set_coords(self.h, _p(a)))
+
+    def coords(self):
+        out = np.zeros((self.n, 3))
+        check(lib().t9_get_coords(self.h, _p(out)))
+        return out
+
+    def nblist_refresh(self):
+        r = ctypes.c_int()
+        check(lib().t9_nblist_refresh(self.h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def nblist_build(self):
+        check(lib().t9_nblist_build(self.h))
+
+    def nblist_stats(self):
+        out = np.zeros(5, dtype=np.int64)
+        check(lib().t9_nblist_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def mpole_init(self):
+        check(lib().t9_mpole_init(self.h))
+
+    def rpole(self):
+        out = np.zeros((self.n, 10))
+        check(lib().t9_get_rpole(self.h, _p(out)))
+        return out
+
+    def dfield(self):
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(lib().t9_dfield(self.h, _p(fd), _p(fp)))
+        return fd, fp
+
+    def ufield(self, ud, up):
+        ud = np.ascontiguousarray(ud, dtype=np.float64)
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(lib().t9_ufield(self.h, _p(ud), _p(up), _p(fd), _p(fp)))
+        return fd, fp
+
+    def induce(self, copy_out=True):
+        st = InduceStats()
+        if copy_out:
+            ud, up = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+            check(lib().t9_induce(self.h, _p(ud), _p(up), ctypes.byref(st)))
+            return ud, up, st
+        check(lib().t9_induce(self.h, None, None, ctypes.byref(st)))
+        return None, None, st
+
+    def uind(self):
+        ud, up = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(lib().t9_get_uind(self.h, _p(ud), _p(up)))
+        return ud, up
+
+    def set_uind(self, ud, up):
+        ud = np.ascontiguousarray(ud, dtype=np.float64)
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        check(lib().t9_set_uind(self.h, _p(ud), _p(up)))
+
+    def sync(self):
+        check(lib().t9_sync(self.h))
