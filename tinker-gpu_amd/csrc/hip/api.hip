@@ -1,0 +1,473 @@
+// C ABI of the device context (t9_ctx_*, t9_nblist_*, t9_mpole_init,
+// t9_dfield, t9_ufield, t9_induce): context creation from a host system,
+// caller-order <-> spatial-order transfers, and the "prepare" step that keeps
+// the neighbor list, sorted coordinates and global multipoles current.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "../../../include/t9hip.h"
+#include "ctx.hpp"
+
+const t9::System& t9_system_ref(const t9_system* sys);
+
+struct t9_ctx {
+  t9::Ctx c;
+};
+
+namespace t9 {
+
+std::vector<double> pme_bsmod(int K, int order);
+int pme_tile_edge(int K);
+
+namespace {
+
+template <class real>
+__global__ void k_sorted_to_orig3(int n, const int* __restrict__ perm, const real* __restrict__ src,
+                                  double* __restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int i = perm[s];
+  for (int d = 0; d < 3; ++d) dst[3 * i + d] = (double)src[3 * s + d];
+}
+
+template <class real>
+__global__ void k_orig_to_sorted3(int n, const int* __restrict__ perm, const double* __restrict__ src,
+                                  real* __restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int i = perm[s];
+  for (int d = 0; d < 3; ++d) dst[3 * s + d] = (real)src[3 * i + d];
+}
+
+template <class real>
+__global__ void k_sorted_to_orig10(int n, const int* __restrict__ perm, const real* __restrict__ src,
+                                   double* __restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int i = perm[s];
+  for (int d = 0; d < 10; ++d) dst[10 * i + d] = (double)src[10 * s + d];
+}
+
+template <class T>
+void upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
+  b.alloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) T9_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+void alloc_vectors(Ctx& c) {
+  const size_t vb = rsize(c) * 3 * (size_t)c.n;
+  for (DBuf* b : {&c.field, &c.fieldp, &c.udir, &c.udirp, &c.uind, &c.uinp, &c.rsd, &c.rsdp, &c.zrsd, &c.zrsdp,
+                  &c.conj, &c.conjp, &c.vec, &c.vecp}) {
+    b->alloc(vb);
+    T9_HIP(hipMemsetAsync(b->p, 0, vb, c.stream));
+  }
+  c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
+  c.scalars.alloc(sizeof(double) * 32);
+  T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 32, c.stream));
+  T9_HIP(hipHostMalloc((void**)&c.h_scalars, sizeof(double) * 32, hipHostMallocDefault));
+}
+
+}  // namespace
+
+void ctx_init(Ctx& c, const System& s, int precision) {
+  if (precision != 32 && precision != 64) throw ArgError("precision must be 32 or 64");
+  c.prec = precision;
+  c.n = s.n;
+  c.nwave = ceil_div(s.n, 64);
+  c.opt = s.opt;
+  c.periodic = s.periodic;
+  for (int d = 0; d < 3; ++d) c.box[d] = s.box[d];
+  c.ewald = s.opt.ewald;
+  c.aewald = s.opt.ewald_alpha;
+  c.cut = c.ewald ? s.opt.ewald_cut : s.opt.mpole_cut;
+  if (c.periodic && !c.ewald && c.cut > 1e20)
+    throw ArgError("periodic systems need ewald or a finite mpole-cutoff");
+  if (!c.periodic && c.ewald) throw ArgError("ewald requires a periodic box");
+  if (c.periodic && c.cut < 1e20 && !c.ewald)
+    throw ArgError("non-Ewald periodic multipole cutoffs (switching taper) are not supported");
+  for (int d = 0; d < 3; ++d) c.grid[d] = s.opt.pme_grid[d];
+  T9_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  hipStream_t st = c.stream;
+  const int n = s.n;
+
+  // ---- static per-atom data, caller order
+  upload(c.xyz, s.xyz, st);
+  c.xyz_last.alloc(sizeof(double) * 3 * n);
+  upload(c.pole, s.pole, st);
+  upload(c.frame, s.frame, st);
+  upload(c.zaxis, s.zaxis, st);
+  upload(c.xaxis, s.xaxis, st);
+  upload(c.yaxis, s.yaxis, st);
+  upload(c.ysign, s.ysign, st);
+  upload(c.polarity_o, s.polarity, st);
+  upload(c.pdamp_o, s.pdamp, st);
+  upload(c.jpolar_o, s.jpolar, st);
+  c.nj = s.njpolar;
+  if (c.prec == 32) {
+    std::vector<float> t(s.thlval.begin(), s.thlval.end());
+    upload(c.thlval, t, st);
+  } else {
+    upload(c.thlval, s.thlval, st);
+  }
+
+  // ---- scale classes and the exclusion CSR (both directions)
+  std::map<std::tuple<double, double, double, double>, int> classes;
+  classes[{1.0, 1.0, 1.0, 1.0}] = 0;
+  std::vector<std::vector<std::pair<int, int>>> ex(n);
+  for (const auto& p : s.scales) {
+    auto key = std::make_tuple(p.m, p.d, p.p, p.u);
+    auto it = classes.find(key);
+    int code;
+    if (it == classes.end()) {
+      code = (int)classes.size();
+      if (code >= kMaxCodes) throw ArgError("too many distinct pair scale classes");
+      classes[key] = code;
+    } else {
+      code = it->second;
+    }
+    ex[p.i].push_back({p.k, code});
+    ex[p.k].push_back({p.i, code});
+  }
+  c.ncodes = (int)classes.size();
+  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = 1.0f;
+  for (auto& kv : classes) {
+    int k = kv.second;
+    c.scale.m[k] = (float)std::get<0>(kv.first);
+    c.scale.d[k] = (float)std::get<1>(kv.first);
+    c.scale.p[k] = (float)std::get<2>(kv.first);
+    c.scale.u[k] = (float)std::get<3>(kv.first);
+  }
+  std::vector<int> eoff(n + 1, 0), ej, ecode;
+  for (int i = 0; i < n; ++i) {
+    eoff[i + 1] = eoff[i] + (int)ex[i].size();
+    for (auto& pr : ex[i]) {
+      ej.push_back(pr.first);
+      ecode.push_back(pr.second);
+    }
+  }
+  upload(c.excl_off, eoff, st);
+  upload(c.excl_j, ej, st);
+  upload(c.excl_code, ecode, st);
+
+  // ---- sorted-order arrays
+  const size_t rs = rsize(c);
+  c.perm.alloc(sizeof(int) * n);
+  c.iperm.alloc(sizeof(int) * n);
+  c.cellc.alloc(sizeof(int) * n);
+  c.pos.alloc(rs * 4 * n);
+  c.rpole.alloc(rs * 10 * n);
+  c.polarity.alloc(rs * n);
+  c.polinv.alloc(rs * n);
+  c.pdamp.alloc(rs * n);
+  c.jpolar.alloc(sizeof(int) * n);
+
+  // ---- neighbor-list geometry
+  const double buf = s.opt.list_buffer;
+  c.mlist.cutoff = c.cut < 1e20 ? c.cut + buf : 1e30;
+  const bool sparse = s.opt.pcg_prec && s.opt.usolve_cut > 0;
+  c.ulist.cutoff = sparse ? std::min(s.opt.usolve_cut + buf, c.mlist.cutoff) : 0;
+  c.brute = true;
+  c.morton_bits = 0;
+  if (c.periodic) {
+    const double rl = c.mlist.cutoff;
+    for (int d = 0; d < 3; ++d)
+      if (rl > 0.5 * c.box[d] + 1e-9)
+        throw ArgError("cutoff plus list buffer exceeds half the box edge");
+    bool ok = true;
+    int mx = 1;
+    for (int d = 0; d < 3; ++d) {
+      c.ncell[d] = std::min(1024, (int)std::floor(c.box[d] / (0.5 * rl)));
+      if (c.ncell[d] < 5) ok = false;
+      mx = std::max(mx, c.ncell[d]);
+    }
+    if (ok) {
+      c.brute = false;
+      int bits = 0;
+      while ((1 << bits) < mx) ++bits;
+      c.morton_bits = bits;
+    } else {
+      c.ncell[0] = c.ncell[1] = c.ncell[2] = 1;
+    }
+  }
+
+  // ---- PME
+  if (c.ewald) {
+    std::vector<double> bs;
+    std::vector<double> tw;
+    for (int d = 0; d < 3; ++d) {
+      const int K = c.grid[d];
+      if (K < 5) throw ArgError("PME grid too small");
+      auto m = pme_bsmod(K, s.opt.pme_order);
+      bs.insert(bs.end(), m.begin(), m.end());
+      for (int k = 0; k < K; ++k) {
+        tw.push_back(std::cos(2 * M_PI * k / K));
+        tw.push_back(-std::sin(2 * M_PI * k / K));
+      }
+      c.tsz[d] = pme_tile_edge(K);
+      c.ntile[d] = K / c.tsz[d];
+    }
+    upload(c.bsmod, bs, st);
+    if (c.prec == 32) {
+      std::vector<float> t(tw.begin(), tw.end());
+      upload(c.twiddle, t, st);
+    } else {
+      upload(c.twiddle, tw, st);
+    }
+    const size_t ng = (size_t)c.grid[0] * c.grid[1] * c.grid[2];
+    c.qgrid.alloc(2 * rs * ng);
+    c.pme_w.alloc(rs * 4 * n);
+    c.pme_base.alloc(sizeof(int4) * n);
+    const int nt = c.ntile[0] * c.ntile[1] * c.ntile[2];
+    c.tile_count.alloc(sizeof(int) * nt);
+    c.tile_start.alloc(sizeof(int) * nt);
+    c.tile_cursor.alloc(sizeof(int) * nt);
+    c.tile_atoms.alloc(sizeof(int) * n);
+  }
+  alloc_vectors(c);
+  T9_HIP(hipStreamSynchronize(st));
+}
+
+void gather_to_host(Ctx& c, const void* dev_sorted, double* host_orig) {
+  DBuf tmp;
+  tmp.alloc(sizeof(double) * 3 * c.n);
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_sorted_to_orig3<float>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), (const float*)dev_sorted, tmp.as<double>());
+  else
+    hipLaunchKernelGGL(k_sorted_to_orig3<double>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), (const double*)dev_sorted, tmp.as<double>());
+  T9_LAUNCH_CHECK();
+  T9_HIP(hipMemcpyAsync(host_orig, tmp.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+void scatter_from_host(Ctx& c, const double* host_orig, void* dev_sorted) {
+  DBuf tmp;
+  tmp.alloc(sizeof(double) * 3 * c.n);
+  T9_HIP(hipMemcpyAsync(tmp.p, host_orig, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_orig_to_sorted3<float>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), tmp.as<double>(), (float*)dev_sorted);
+  else
+    hipLaunchKernelGGL(k_orig_to_sorted3<double>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), tmp.as<double>(), (double*)dev_sorted);
+  T9_LAUNCH_CHECK();
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Make the list, sorted coordinates and global multipoles current.
+void prepare(Ctx& c) {
+  if (!c.list_valid) {
+    nblist_build(c);
+  } else if (c.coords_dirty) {
+    if (nblist_needs_rebuild(c)) nblist_build(c);
+    else refresh_sorted_coords(c);
+  }
+  if (!c.rpole_valid) mpole_init(c);
+}
+
+void rpole_to_host(Ctx& c, double* out) {
+  DBuf tmp;
+  tmp.alloc(sizeof(double) * 10 * c.n);
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_sorted_to_orig10<float>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), c.rpole.as<float>(), tmp.as<double>());
+  else
+    hipLaunchKernelGGL(k_sorted_to_orig10<double>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), c.rpole.as<double>(), tmp.as<double>());
+  T9_LAUNCH_CHECK();
+  T9_HIP(hipMemcpyAsync(out, tmp.p, sizeof(double) * 10 * c.n, hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace t9
+
+using t9::guarded;
+
+extern "C" {
+
+int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out) {
+  return guarded([&] {
+    if (!sys || !out) throw t9::ArgError("null argument");
+    auto* h = new t9_ctx;
+    try {
+      t9::ctx_init(h->c, t9_system_ref(sys), precision);
+    } catch (...) {
+      if (h->c.stream) (void)hipStreamDestroy(h->c.stream);
+      if (h->c.h_scalars) (void)hipHostFree(h->c.h_scalars);
+      delete h;
+      throw;
+    }
+    *out = h;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_ctx_destroy(t9_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return T9_OK;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    hipStream_t st = ctx->c.stream;
+    double* hs = ctx->c.h_scalars;
+    delete ctx;
+    if (hs) (void)hipHostFree(hs);
+    if (st) (void)hipStreamDestroy(st);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_set_coords(t9_ctx* ctx, const double* xyz) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    T9_HIP(hipMemcpyAsync(c.xyz.p, xyz, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    c.coords_dirty = true;
+    c.rpole_valid = false;
+    c.bins_valid = false;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_get_coords(t9_ctx* ctx, double* xyz) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    T9_HIP(hipMemcpyAsync(xyz, c.xyz.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_nblist_refresh(t9_ctx* ctx, int* rebuilt) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    bool rb = false;
+    if (!c.list_valid || nblist_needs_rebuild(c)) {
+      t9::nblist_build(c);
+      rb = true;
+    } else if (c.coords_dirty) {
+      t9::refresh_sorted_coords(c);
+    }
+    if (rebuilt) *rebuilt = rb ? 1 : 0;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_nblist_build(t9_ctx* ctx) {
+  return guarded([&] {
+    t9::nblist_build(ctx->c);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_nblist_stats(t9_ctx* ctx, int64_t* out5) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!c.list_valid) t9::nblist_build(c);
+    std::vector<int> nl(c.n);
+    T9_HIP(hipMemcpyAsync(nl.data(), c.mlist.nlist.p, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    long long pairs = 0;
+    for (int v : nl) pairs += v;
+    out5[0] = pairs;
+    out5[1] = c.n;
+    out5[2] = c.nwave;
+    out5[3] = c.mlist.total;
+    out5[4] = c.builds;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_mpole_init(t9_ctx* ctx) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    t9::mpole_init(c);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_get_rpole(t9_ctx* ctx, double* rpole) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    t9::rpole_to_host(c, rpole);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_dfield(t9_ctx* ctx, double* field, double* fieldp) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    t9::dfield_all(c);
+    if (field) t9::gather_to_host(c, c.field.p, field);
+    if (fieldp) t9::gather_to_host(c, c.fieldp.p, fieldp);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field, double* fieldp) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
+    t9::prepare(c);
+    // use the CG scratch vectors: zrsd/zrsdp hold the inputs, vec/vecp the fields
+    t9::scatter_from_host(c, uind, c.zrsd.p);
+    t9::scatter_from_host(c, uinp, c.zrsdp.p);
+    t9::ufield_all(c, c.zrsd.p, c.zrsdp.p, c.vec.p, c.vecp.p);
+    if (field) t9::gather_to_host(c, c.vec.p, field);
+    if (fieldp) t9::gather_to_host(c, c.vecp.p, fieldp);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    int iters = 0;
+    double rms = 0;
+    int rc = t9::induce_pcg(c, &iters, &rms);
+    if (stats) {
+      stats->iterations = iters;
+      stats->predictor = 0;
+      stats->rms_debye = rms;
+      stats->guess_rms = 0;
+    }
+    if (uind) t9::gather_to_host(c, c.uind.p, uind);
+    if (uinp) t9::gather_to_host(c, c.uinp.p, uinp);
+    if (rc != 0) throw t9::NotConverged("INDUCE -- induced dipoles are not converged");
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_get_uind(t9_ctx* ctx, double* uind, double* uinp) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (uind) t9::gather_to_host(c, c.uind.p, uind);
+    if (uinp) t9::gather_to_host(c, c.uinp.p, uinp);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    if (uind) t9::scatter_from_host(c, uind, c.uind.p);
+    if (uinp) t9::scatter_from_host(c, uinp, c.uinp.p);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_sync(t9_ctx* ctx) {
+  return guarded([&] {
+    T9_HIP(hipStreamSynchronize(ctx->c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+}  // extern "C"

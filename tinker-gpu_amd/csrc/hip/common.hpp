@@ -1,0 +1,132 @@
+// Device-side building blocks shared by the gfx950 kernels: precision
+// traits, minimum image, Thole/Ewald radial functions, deterministic block
+// reductions, and the launch-error check.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "host/errors.hpp"
+
+namespace t9 {
+
+#define T9_HIP(call)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::t9::HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " at " __FILE__ \
+                           ":" + std::to_string(__LINE__));                                     \
+  } while (0)
+
+#define T9_LAUNCH_CHECK() T9_HIP(hipGetLastError())
+
+constexpr int kWave = 64;
+constexpr int kCodeBits = 5;  // neighbor entry = (j << kCodeBits) | scale class
+constexpr int kMaxCodes = 1 << kCodeBits;
+
+template <class real>
+struct RealTraits;
+template <>
+struct RealTraits<float> {
+  using real3 = float3;
+  using real4 = float4;
+  static constexpr double fixed_scale = 4294967296.0;  // 2^32 for PME grid accumulation
+};
+template <>
+struct RealTraits<double> {
+  using real3 = double3;
+  using real4 = double4;
+  static constexpr double fixed_scale = 17592186044416.0;  // 2^44
+};
+
+__device__ __forceinline__ float t9_rint(float x) { return rintf(x); }
+__device__ __forceinline__ double t9_rint(double x) { return rint(x); }
+__device__ __forceinline__ float t9_exp(float x) { return expf(x); }
+__device__ __forceinline__ double t9_exp(double x) { return exp(x); }
+__device__ __forceinline__ float t9_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double t9_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float t9_rsqrt(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double t9_rsqrt(double x) { return rsqrt(x); }
+
+// erfc: the device library's erfcf / erfc.  The reference's FP32 build uses
+// the 5-term A&S 7.1.26 form (include/math/libfunc.h:8-29), whose relative
+// error reaches 2e-3 near the 7 Å cutoff (x ~ 3.8); that bias is systematic
+// over millions of pairs, so the accurate routine is used instead.
+__device__ __forceinline__ float t9_erfc_ex(float x, float /*expmx2*/) { return erfcf(x); }
+__device__ __forceinline__ double t9_erfc_ex(double x, double /*expmx2*/) { return erfc(x); }
+
+// Minimum image in an orthorhombic box.
+template <class real>
+__device__ __forceinline__ void min_image(real& x, real& y, real& z, real lx, real ly, real lz, real ilx, real ily,
+                                          real ilz) {
+  x -= lx * t9_rint(x * ilx);
+  y -= ly * t9_rint(y * ily);
+  z -= lz * t9_rint(z * ilz);
+}
+
+// Box description passed by value to kernels.
+template <class real>
+struct BoxT {
+  real l[3];
+  real il[3];
+  int periodic;
+};
+
+// Per scale-class factors (m, d, p, u); class 0 is the all-ones class.
+struct ScaleTable {
+  float m[kMaxCodes], d[kMaxCodes], p[kMaxCodes], u[kMaxCodes];
+};
+
+// Thole damping of the 1/r^3 and 1/r^5 (and 1/r^7) tensors:
+// lambda3 = 1 - e^{-a u^3}, lambda5 = 1 - (1 + a u^3) e^{-a u^3},
+// lambda7 = 1 - (1 + a u^3 + 0.6 a^2 u^6) e^{-a u^3}, u = r / (pdi pdk)^{...}
+// (include/seq/damp.h:8-60).
+template <class real>
+__device__ __forceinline__ void thole_damp(real r, real pdik, real pga, real& s3, real& s5, real& s7) {
+  if (pdik == 0) {
+    s3 = s5 = s7 = 1;
+    return;
+  }
+  real ratio = r / pdik;
+  real damp = -pga * ratio * ratio * ratio;
+  real e = t9_exp(damp);
+  s3 = 1 - e;
+  s5 = 1 - e * (1 - damp);
+  s7 = 1 - e * (1 - damp + (real)0.6 * damp * damp);
+}
+
+// Ewald real-space radial functions b0..b{NB-1} (include/seq/damp.h:137-151).
+template <int NB, class real>
+__device__ __forceinline__ void ewald_bn(real* bn, real r, real rinv, real rr2, real aewald) {
+  real ra = aewald * r;
+  real ex = t9_exp(-ra * ra);
+  bn[0] = t9_erfc_ex(ra, ex) * rinv;
+  real alsq2 = 2 * aewald * aewald;
+  real alsq2n = (real)0.56418958354775628695 / aewald;  // 1/(sqrt(pi) a)
+#pragma unroll
+  for (int j = 1; j < NB; ++j) {
+    alsq2n *= alsq2;
+    bn[j] = ((2 * j - 1) * bn[j - 1] + alsq2n * ex) * rr2;
+  }
+}
+
+// Deterministic block sum of doubles: fixed tree over a fixed block size.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace t9

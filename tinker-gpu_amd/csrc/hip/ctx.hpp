@@ -1,0 +1,148 @@
+// Device context: everything one system needs on one GPU.
+//
+// Layout in HBM.  Atoms live in two orders:
+//  * caller order ("orig"): FP64 coordinates d_xyz[n][3] and the static
+//    parameters exactly as the host setup produced them;
+//  * spatial order ("sorted"): the order of the last neighbor-list build
+//    (cells in Morton order, original index inside a cell).  All per-step
+//    working arrays — wrapped coordinates, global multipoles, dipoles,
+//    fields, CG vectors — are in this order, [n][3] or [n][10] of `real`,
+//    so the pair kernels' lanes read consecutive atoms.
+// perm[s] = original index of sorted slot s; iperm is its inverse.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "host/system.hpp"
+
+namespace t9 {
+
+// A device allocation that frees itself.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t nbytes) {
+    if (nbytes <= bytes && p) return;
+    release();
+    if (nbytes == 0) nbytes = 16;
+    T9_HIP(hipMalloc(&p, nbytes));
+    bytes = nbytes;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Neighbor list in "wave slabs": atoms are grouped 64 to a wavefront in
+// sorted order; wave w owns entries [off[w], off[w] + 64*maxlen[w]) laid out
+// k-major (entry k of lane l at off[w] + 64*k + l) so a wave's lanes read one
+// contiguous 256-byte row per neighbor step.  Entry = (j << 5) | scale class.
+struct NbList {
+  double cutoff = 0;   // list radius (interaction cutoff + buffer)
+  DBuf nlist;          // int [n]     neighbors of each sorted atom
+  DBuf maxlen;         // int [nwave]
+  DBuf off;            // int64 [nwave + 1]
+  DBuf entries;        // uint32 [capacity]
+  size_t capacity = 0;
+  long long total = 0;  // slots used (including padding)
+  long long pairs = 0;  // directed pairs stored
+};
+
+struct Ctx {
+  int n = 0, nwave = 0;
+  int prec = 32;
+  hipStream_t stream = nullptr;
+  Options opt;
+  double box[3] = {0, 0, 0};
+  bool periodic = false;
+  bool ewald = false;
+  double aewald = 0;
+  double cut = 0;        // real-space cutoff of the multipole/polar pair terms
+  int grid[3] = {0, 0, 0};
+  int nj = 0;            // polarize types (Thole table is nj x nj)
+  ScaleTable scale{};    // m, d, p, u per scale class
+  int ncodes = 1;
+
+  // ---- caller order, FP64
+  DBuf xyz, xyz_last;            // [n][3]
+  DBuf pole;                     // [n][10] local frame
+  DBuf frame, zaxis, xaxis, yaxis, ysign;  // int [n]
+  DBuf polarity_o, pdamp_o;      // double [n]
+  DBuf jpolar_o;                 // int [n]
+  DBuf excl_off, excl_j, excl_code;  // CSR of scaled partners (both directions)
+  DBuf thlval;                   // real [nj*nj]
+
+  // ---- spatial order
+  DBuf perm, iperm;              // int [n]
+  DBuf cellc;                    // int [n] packed cell coordinates of sorted atoms
+  DBuf pos;                      // real4 [n] wrapped coordinates
+  DBuf rpole;                    // real [n][10] global multipoles
+  DBuf polarity, polinv, pdamp;  // real [n]
+  DBuf jpolar;                   // int [n]
+
+  // ---- cell binning
+  int ncell[3] = {1, 1, 1};
+  int morton_bits = 0;
+  bool brute = true;             // all-pairs search (small or non-periodic systems)
+  DBuf cell_key, cell_count, cell_start, cell_cursor, unsorted;
+  NbList mlist, ulist;
+  int builds = 0;
+  bool list_valid = false;
+
+  // ---- CG / field vectors, real [n][3] each, sorted order
+  DBuf field, fieldp, udir, udirp, uind, uinp, rsd, rsdp, zrsd, zrsdp, conj, conjp, vec, vecp;
+  DBuf partials;                 // double [nblocks * 8]
+  DBuf scalars;                  // double [32] on device
+  DBuf hscalars_pinned;          // host-pinned mirror
+  double* h_scalars = nullptr;
+
+  // ---- PME
+  int tsz[3] = {8, 8, 8};       // spreading tile edge per axis (divides the grid size)
+  int ntile[3] = {1, 1, 1};
+  DBuf qgrid;                    // real2 [K1*K2*K3]
+  DBuf bsmod;                    // real [K1 + K2 + K3]
+  DBuf twiddle;                  // real2 [K1 + K2 + K3] forward tables per axis
+  DBuf pme_w, pme_base;          // real4 / int4 [n] per sorted atom
+  DBuf tile_count, tile_start, tile_cursor, tile_atoms;
+  bool bins_valid = false;
+
+  // ---- state
+  bool coords_dirty = true;      // coordinates changed since the sorted copy
+  bool rpole_valid = false;
+  bool uind_valid = false;
+};
+
+// ----- entry points implemented across the .hip files
+void ctx_init(Ctx& c, const System& s, int precision);
+void nblist_build(Ctx& c);
+bool nblist_needs_rebuild(Ctx& c);
+void refresh_sorted_coords(Ctx& c);
+void mpole_init(Ctx& c);
+void dfield_all(Ctx& c);                  // c.field/fieldp = full direct field
+void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp);  // sorted real vectors
+void pme_bin(Ctx& c);
+void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp);  // adds recip+self
+void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
+int induce_pcg(Ctx& c, int* iters, double* rms);
+void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp);
+
+// sorted <-> caller order transfers of [n][3] real vectors
+void gather_to_host(Ctx& c, const void* dev_sorted_real3, double* host_orig);
+void scatter_from_host(Ctx& c, const double* host_orig, void* dev_sorted_real3);
+
+inline size_t rsize(const Ctx& c) { return c.prec == 32 ? sizeof(float) : sizeof(double); }
+
+}  // namespace t9

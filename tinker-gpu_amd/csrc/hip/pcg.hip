@@ -1,0 +1,251 @@
+// Preconditioned conjugate gradient for the mutual induced dipoles.
+//
+// Algorithm of the reference (include/ff/cuinduce.h:8-38,
+// src/acc/amoeba/induce.cpp:41-292, src/cu/amoeba/pcg.cu:14-185): solve
+// (alpha^-1 + T) u = E for the d and p dipole sets simultaneously, direct-field
+// guess, zeroed residual on unpolarizable sites, min 3 iterations, stop when
+// debye * sqrt(max(r_d.r_d, r_p.r_p) / n) < polar-eps, then one "peek" step
+// u += peek * alpha * r.  Differences by design: the CG scalars (dot products,
+// step lengths, residual norm) are FP64 and live on the device — each dot is
+// a fixed-order two-level reduction, so iterates are bit-reproducible — and
+// the host reads back one residual per iteration to decide convergence.
+#include "ctx.hpp"
+
+namespace t9 {
+namespace {
+
+constexpr int kB = 256;
+// device scalar slots
+enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_NSLOT = 16 };
+
+template <class real>
+__global__ __launch_bounds__(kB) void k_udir(int n, const real* __restrict__ pol, const real* __restrict__ fd,
+                                            const real* __restrict__ fp, real* __restrict__ udir,
+                                            real* __restrict__ udirp, real* __restrict__ ud, real* __restrict__ up,
+                                            int guess) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const real a = pol[s];
+  for (int d = 0; d < 3; ++d) {
+    const real x = a * fd[3 * s + d], y = a * fp[3 * s + d];
+    udir[3 * s + d] = x;
+    udirp[3 * s + d] = y;
+    ud[3 * s + d] = guess ? x : 0;
+    up[3 * s + d] = guess ? y : 0;
+  }
+}
+
+template <class real>
+__global__ __launch_bounds__(kB) void k_copy3_zero_nopol(int n, const real* __restrict__ pol,
+                                                        const real* __restrict__ a, const real* __restrict__ b,
+                                                        real* __restrict__ ra, real* __restrict__ rb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const bool zero = pol[s] == 0;
+  for (int d = 0; d < 3; ++d) {
+    ra[3 * s + d] = zero ? (real)0 : a[3 * s + d];
+    rb[3 * s + d] = zero ? (real)0 : b[3 * s + d];
+  }
+}
+
+// partial sums of two dot products a1.b1 and a2.b2 (slot0, slot1) and optionally
+// a3.b3, a4.b4 (slot2, slot3); partials[blk * 4 + slot]
+template <class real>
+__global__ __launch_bounds__(kB) void k_dots(int n, int ndots, const real* __restrict__ a1,
+                                            const real* __restrict__ b1, const real* __restrict__ a2,
+                                            const real* __restrict__ b2, const real* __restrict__ a3,
+                                            const real* __restrict__ b3, const real* __restrict__ a4,
+                                            const real* __restrict__ b4, double* __restrict__ partials) {
+  __shared__ double sh[kB];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const real* A[4] = {a1, a2, a3, a4};
+  const real* Bv[4] = {b1, b2, b3, b4};
+  for (int q = 0; q < ndots; ++q) {
+    double v = 0;
+    if (s < n)
+      for (int d = 0; d < 3; ++d) v += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
+    double t = block_sum<kB>(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x * 4 + q] = t;
+  }
+}
+
+// Finalize: sum partials in fixed order, then update the CG scalars.
+// stage 0: initial sum = r.z          (slots SUM, SUMP)
+// stage 1: a = sum / p.Tp             (slots A, AP)
+// stage 2: b = sum1 / sum; sum = sum1; eps from r.r
+__global__ __launch_bounds__(kB) void k_finalize(int nblk, int ndots, int stage, int n, double debye,
+                                                const double* __restrict__ partials, double* __restrict__ sc) {
+  __shared__ double sh[kB];
+  double tot[4] = {0, 0, 0, 0};
+  for (int q = 0; q < ndots; ++q) {
+    double v = 0;
+    for (int b = threadIdx.x; b < nblk; b += kB) v += partials[b * 4 + q];
+    tot[q] = block_sum<kB>(v, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if (stage == 0) {
+    sc[S_SUM] = tot[0];
+    sc[S_SUMP] = tot[1];
+  } else if (stage == 1) {
+    sc[S_A] = tot[0] != 0 ? sc[S_SUM] / tot[0] : 0;
+    sc[S_AP] = tot[1] != 0 ? sc[S_SUMP] / tot[1] : 0;
+  } else {
+    sc[S_B] = sc[S_SUM] != 0 ? tot[0] / sc[S_SUM] : 0;
+    sc[S_BP] = sc[S_SUMP] != 0 ? tot[1] / sc[S_SUMP] : 0;
+    sc[S_SUM] = tot[0];
+    sc[S_SUMP] = tot[1];
+    sc[S_EPSD] = tot[2];
+    sc[S_EPSP] = tot[3];
+    sc[S_EPS] = debye * sqrt(fmax(tot[2], tot[3]) / n);
+  }
+}
+
+// vec = conj / alpha - field   (reference pcgP1)
+template <class real>
+__global__ __launch_bounds__(kB) void k_p1(int n, const real* __restrict__ polinv, const real* __restrict__ cd,
+                                          const real* __restrict__ cp, const real* __restrict__ fd,
+                                          const real* __restrict__ fp, real* __restrict__ vd,
+                                          real* __restrict__ vp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const real pi = polinv[s];
+  for (int d = 0; d < 3; ++d) {
+    vd[3 * s + d] = pi * cd[3 * s + d] - fd[3 * s + d];
+    vp[3 * s + d] = pi * cp[3 * s + d] - fp[3 * s + d];
+  }
+}
+
+// u += a p; r -= a vec; r = 0 on unpolarizable sites   (reference pcgP2)
+template <class real>
+__global__ __launch_bounds__(kB) void k_p2(int n, const double* __restrict__ sc, const real* __restrict__ pol,
+                                          const real* __restrict__ cd, const real* __restrict__ cp,
+                                          const real* __restrict__ vd, const real* __restrict__ vp,
+                                          real* __restrict__ ud, real* __restrict__ up, real* __restrict__ rd,
+                                          real* __restrict__ rp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const real a = (real)sc[S_A], ap = (real)sc[S_AP];
+  const bool zero = pol[s] == 0;
+  for (int d = 0; d < 3; ++d) {
+    const int k = 3 * s + d;
+    ud[k] += a * cd[k];
+    up[k] += ap * cp[k];
+    rd[k] = zero ? (real)0 : rd[k] - a * vd[k];
+    rp[k] = zero ? (real)0 : rp[k] - ap * vp[k];
+  }
+}
+
+// p = z + b p   (reference pcgP3)
+template <class real>
+__global__ __launch_bounds__(kB) void k_p3(int n, const double* __restrict__ sc, const real* __restrict__ zd,
+                                          const real* __restrict__ zp, real* __restrict__ cd,
+                                          real* __restrict__ cp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const real b = (real)sc[S_B], bp = (real)sc[S_BP];
+  for (int d = 0; d < 3; ++d) {
+    const int k = 3 * s + d;
+    cd[k] = zd[k] + b * cd[k];
+    cp[k] = zp[k] + bp * cp[k];
+  }
+}
+
+// u += peek * alpha * r   (reference pcgPeek)
+template <class real>
+__global__ __launch_bounds__(kB) void k_peek(int n, real peek, const real* __restrict__ pol,
+                                            const real* __restrict__ rd, const real* __restrict__ rp,
+                                            real* __restrict__ ud, real* __restrict__ up) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const real t = peek * pol[s];
+  for (int d = 0; d < 3; ++d) {
+    ud[3 * s + d] += t * rd[3 * s + d];
+    up[3 * s + d] += t * rp[3 * s + d];
+  }
+}
+
+template <class real>
+int induce_t(Ctx& c, int* iters, double* rms) {
+  const int n = c.n;
+  const int nblk = ceil_div(n, kB);
+  const dim3 G(nblk), Bk(kB);
+  hipStream_t st = c.stream;
+  real* field = c.field.as<real>();
+  real* fieldp = c.fieldp.as<real>();
+  real* ud = c.uind.as<real>();
+  real* up = c.uinp.as<real>();
+  real* rd = c.rsd.as<real>();
+  real* rp = c.rsdp.as<real>();
+  real* zd = c.zrsd.as<real>();
+  real* zp = c.zrsdp.as<real>();
+  real* cd = c.conj.as<real>();
+  real* cp = c.conjp.as<real>();
+  real* vd = c.vec.as<real>();
+  real* vp = c.vecp.as<real>();
+  double* part = c.partials.as<double>();
+  double* sc = c.scalars.as<double>();
+  const double debye = kDebye;
+  const bool guess = c.opt.pcg_guess;
+
+  dfield_all(c);
+  hipLaunchKernelGGL(k_udir<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, c.udir.as<real>(),
+                     c.udirp.as<real>(), ud, up, guess ? 1 : 0);
+  T9_LAUNCH_CHECK();
+  if (guess) {
+    ufield_all(c, ud, up, rd, rp);
+    hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), rd, rp, rd, rp);
+  } else {
+    hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, rd, rp);
+  }
+  T9_LAUNCH_CHECK();
+  precond_apply(c, rd, rp, zd, zp);
+  T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
+  T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, zd, rp, zp, (const real*)nullptr, (const real*)nullptr,
+                     (const real*)nullptr, (const real*)nullptr, part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 0, n, debye, part, sc);
+  T9_LAUNCH_CHECK();
+
+  const int maxiter = 100;
+  const int miniter = n < 3 ? n : 3;
+  int iter = 0;
+  double eps = 100;
+  bool done = false;
+  while (!done) {
+    ++iter;
+    ufield_all(c, cd, cp, field, fieldp);
+    hipLaunchKernelGGL(k_p1<real>, G, Bk, 0, st, n, c.polinv.as<real>(), cd, cp, field, fieldp, vd, vp);
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, cd, vd, cp, vp, (const real*)nullptr, (const real*)nullptr,
+                       (const real*)nullptr, (const real*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 1, n, debye, part, sc);
+    hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up, rd, rp);
+    T9_LAUNCH_CHECK();
+    precond_apply(c, rd, rp, zd, zp);
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 4, 2, n, debye, part, sc);
+    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, sc, zd, zp, cd, cp);
+    T9_LAUNCH_CHECK();
+    T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
+    T9_HIP(hipStreamSynchronize(st));
+    eps = c.h_scalars[S_EPS];
+    if (eps < c.opt.polar_eps) done = true;
+    if (iter < miniter) done = false;
+    if (iter >= c.opt.polar_iter) done = true;
+    if (done) {
+      hipLaunchKernelGGL(k_peek<real>, G, Bk, 0, st, n, (real)c.opt.pcg_peek, c.polarity.as<real>(), rd, rp, ud, up);
+      T9_LAUNCH_CHECK();
+    }
+  }
+  c.uind_valid = true;
+  if (iters) *iters = iter;
+  if (rms) *rms = eps;
+  return iter >= maxiter ? 1 : 0;
+}
+
+}  // namespace
+
+int induce_pcg(Ctx& c, int* iters, double* rms) {
+  return c.prec == 32 ? induce_t<float>(c, iters, rms) : induce_t<double>(c, iters, rms);
+}
+
+}  // namespace t9

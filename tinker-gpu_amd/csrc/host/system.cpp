@@ -683,31 +683,40 @@ void build_water_box(System& s, int nmol, int nside, double L, uint64_t seed, in
       int j = (int)(rng.next() % (uint64_t)(i + 1));
       std::swap(sites[i], sites[j]);
     }
-    int placed = 0;
+    // Sites at least ion_min from every oxygen are taken in shuffled order;
+    // if there are too few of them, the remaining ions go to the free sites
+    // farthest from any oxygen (descending distance, ties by shuffled order).
     const int want = n_na + n_cl;
-    for (int site : sites) {
-      if (placed == want) break;
+    std::vector<std::pair<double, int>> far;  // (-min distance, shuffled rank)
+    std::vector<int> chosen;
+    for (size_t r = 0; r < sites.size(); ++r) {
       double c[3];
-      center(site, c);
-      bool ok = true;
-      for (int m = 0; m < nmol && ok; ++m) {
+      center(sites[r], c);
+      double md2 = 1e300;
+      for (int m = 0; m < nmol; ++m) {
         double r2 = 0;
         for (int d = 0; d < 3; ++d) {
           double dx = s.xyz[9 * m + d] - c[d];
           dx -= L * std::nearbyint(dx / L);
           r2 += dx * dx;
         }
-        if (r2 < ion_min * ion_min) ok = false;
+        md2 = std::min(md2, r2);
       }
-      if (!ok) continue;
+      if (md2 >= ion_min * ion_min && (int)chosen.size() < want) chosen.push_back((int)r);
+      else far.push_back({-md2, (int)r});
+    }
+    std::sort(far.begin(), far.end());
+    for (size_t q = 0; q < far.size() && (int)chosen.size() < want; ++q) chosen.push_back(far[q].second);
+    if ((int)chosen.size() != want) throw std::runtime_error("not enough free lattice sites for the ions");
+    for (int placed = 0; placed < want; ++placed) {
+      double c[3];
+      center(sites[chosen[placed]], c);
       const int idx = 3 * nmol + placed;
       for (int d = 0; d < 3; ++d) s.xyz[3 * idx + d] = c[d];
       const bool na = placed < n_na;
       s.type[idx] = na ? 7 : 15;
       s.name[idx] = na ? "Na+" : "Cl-";
-      ++placed;
     }
-    if (placed != want) throw std::runtime_error("could not place all ions");
   }
   s.box[0] = s.box[1] = s.box[2] = L;
   s.periodic = true;
