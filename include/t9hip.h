@@ -134,6 +134,21 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
 /* Wait for all work queued on the context's stream. */
 int t9_sync(t9_ctx* ctx);
 
+/* ------------------------------------------------------------ measurement */
+/* No counterpart in the reference (its only timing is the ns/day printout,
+ * src/xdynamic.cpp:175-190).  Live HIP-event timing of one kernel family on
+ * the context's stream: mode 1 clears and starts, 0 stops, 2 reads.  Ids:
+ * 0 ufield real-space pairs, 1 dfield real-space pairs, 2 PME spread,
+ * 3 FFT passes, 4 PME gathers, 5 preconditioner. */
+enum { T9_K_UFIELD = 0, T9_K_DFIELD, T9_K_SPREAD, T9_K_FFT, T9_K_GATHER, T9_K_PRECOND };
+int t9_kernel_timing(t9_ctx* ctx, int kernel, int mode, double* total_ms, int64_t* launches);
+/* Unique atom pairs within the real-space cutoff on the current coordinates
+ * (the algorithmic unit of the pair kernels). */
+int t9_cutoff_pairs(t9_ctx* ctx, int64_t* pairs);
+/* Plain device-to-device copy kernel of nbytes, repeated `reps` times
+ * (calibrates memory-traffic counters against a known byte count). */
+int t9_copy_probe(int64_t nbytes, int reps, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

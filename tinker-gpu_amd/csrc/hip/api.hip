@@ -269,6 +269,76 @@ void prepare(Ctx& c) {
   if (!c.rpole_valid) mpole_init(c);
 }
 
+void ktimer_collect(Ctx& c, int id) {
+  KTimer& t = c.timers[id];
+  T9_HIP(hipStreamSynchronize(c.stream));
+  for (size_t k = 0; k + 1 < t.used; k += 2) {
+    float ms = 0;
+    T9_HIP(hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]));
+    t.ms += ms;
+    t.count += 1;
+  }
+  t.used = 0;
+}
+
+void ktimer_release(Ctx& c) {
+  for (auto& t : c.timers)
+    for (auto e : t.ev) (void)hipEventDestroy(e);
+}
+
+namespace {
+
+template <class real>
+__global__ void k_count_cut(int n, const int* __restrict__ nlist, const long long* __restrict__ woff,
+                            const uint32_t* __restrict__ entries, const typename RealTraits<real>::real4* __restrict__ pos,
+                            BoxT<real> box, real cut2, unsigned long long* __restrict__ out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const auto pi = pos[s];
+  const uint32_t* e = entries + woff[s >> 6] + (s & 63);
+  unsigned long long c = 0;
+  for (int k = 0; k < nlist[s]; ++k) {
+    const int j = (int)(e[(long long)k * 64] >> kCodeBits);
+    const auto pj = pos[j];
+    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
+    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    if (xr * xr + yr * yr + zr * zr <= cut2) ++c;
+  }
+  atomicAdd(out, c);
+}
+
+template <class real>
+long long count_cut_t(Ctx& c) {
+  DBuf cnt;
+  cnt.alloc(sizeof(unsigned long long));
+  T9_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), c.stream));
+  BoxT<real> b;
+  for (int d = 0; d < 3; ++d) {
+    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
+    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
+  }
+  b.periodic = c.periodic ? 1 : 0;
+  double c2 = c.cut * c.cut;
+  hipLaunchKernelGGL(k_count_cut<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                     c.mlist.nlist.as<int>(), c.mlist.off.as<long long>(), c.mlist.entries.as<uint32_t>(),
+                     c.pos.as<typename RealTraits<real>::real4>(), b, (real)(c2 > 1e30 ? 1e30 : c2),
+                     cnt.as<unsigned long long>());
+  T9_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  T9_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  return (long long)h;
+}
+
+__global__ void k_copy_probe(long long n4, const float4* __restrict__ a, float4* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+}  // namespace
+
+long long count_cutoff_pairs(Ctx& c) { return c.prec == 32 ? count_cut_t<float>(c) : count_cut_t<double>(c); }
+
 void rpole_to_host(Ctx& c, double* out) {
   DBuf tmp;
   tmp.alloc(sizeof(double) * 10 * c.n);
@@ -312,6 +382,7 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->c.stream);
     hipStream_t st = ctx->c.stream;
     double* hs = ctx->c.h_scalars;
+    t9::ktimer_release(ctx->c);
     delete ctx;
     if (hs) (void)hipHostFree(hs);
     if (st) (void)hipStreamDestroy(st);
@@ -385,6 +456,7 @@ int t9_mpole_init(t9_ctx* ctx) {
     auto& c = ctx->c;
     t9::prepare(c);
     t9::mpole_init(c);
+    c.bins_valid = false;  // a new step: PME spline bins follow the coordinates
     return T9_OK;
   }, T9_ERR_HIP);
 }
@@ -466,6 +538,63 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp) {
 int t9_sync(t9_ctx* ctx) {
   return guarded([&] {
     T9_HIP(hipStreamSynchronize(ctx->c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_kernel_timing(t9_ctx* ctx, int kernel, int mode, double* total_ms, int64_t* launches) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (kernel < 0 || kernel >= t9::KT_COUNT) throw t9::ArgError("unknown kernel id");
+    auto& t = c.timers[kernel];
+    if (mode == 1) {  // start: clear and enable
+      t9::ktimer_collect(c, kernel);
+      t.ms = 0;
+      t.count = 0;
+      t.on = true;
+    } else if (mode == 0) {  // stop: disable, keep totals
+      t9::ktimer_collect(c, kernel);
+      t.on = false;
+    } else {
+      t9::ktimer_collect(c, kernel);
+    }
+    if (total_ms) *total_ms = t.ms;
+    if (launches) *launches = t.count;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_cutoff_pairs(t9_ctx* ctx, int64_t* pairs) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    t9::prepare(c);
+    *pairs = t9::count_cutoff_pairs(c) / 2;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_copy_probe(int64_t nbytes, int reps, double* ms_per_copy) {
+  return guarded([&] {
+    if (nbytes < 16 || reps < 1) throw t9::ArgError("bad copy probe size");
+    t9::DBuf a, b;
+    a.alloc((size_t)nbytes);
+    b.alloc((size_t)nbytes);
+    T9_HIP(hipMemset(a.p, 0, (size_t)nbytes));
+    hipEvent_t e0, e1;
+    T9_HIP(hipEventCreate(&e0));
+    T9_HIP(hipEventCreate(&e1));
+    const long long n4 = nbytes / 16;
+    hipLaunchKernelGGL(t9::k_copy_probe, dim3(2048), dim3(256), 0, nullptr, n4, a.as<float4>(), b.as<float4>());
+    T9_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(t9::k_copy_probe, dim3(2048), dim3(256), 0, nullptr, n4, a.as<float4>(), b.as<float4>());
+    T9_HIP(hipEventRecord(e1, nullptr));
+    T9_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    T9_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
     return T9_OK;
   }, T9_ERR_HIP);
 }

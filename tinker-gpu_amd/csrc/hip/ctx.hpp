@@ -61,8 +61,30 @@ struct NbList {
   long long pairs = 0;  // directed pairs stored
 };
 
+// Live kernel timing with HIP events recorded on the context stream around
+// selected launches (enabled per kernel id; see t9_kernel_timing).
+enum KernelId : int {
+  KT_UFIELD_REAL = 0,
+  KT_DFIELD_REAL,
+  KT_SPREAD,
+  KT_FFT,
+  KT_GATHER,
+  KT_PRECOND,
+  KT_PCG_VEC,
+  KT_COUNT
+};
+
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  size_t used = 0;             // events in use
+  double ms = 0;
+  long long count = 0;
+};
+
 struct Ctx {
   int n = 0, nwave = 0;
+  KTimer timers[KT_COUNT];
   int prec = 32;
   hipStream_t stream = nullptr;
   Options opt;
@@ -144,5 +166,33 @@ void gather_to_host(Ctx& c, const void* dev_sorted_real3, double* host_orig);
 void scatter_from_host(Ctx& c, const double* host_orig, void* dev_sorted_real3);
 
 inline size_t rsize(const Ctx& c) { return c.prec == 32 ? sizeof(float) : sizeof(double); }
+
+// Records start/stop events around a launch when the timer is on.
+struct KScope {
+  KTimer* t;
+  hipStream_t st;
+  hipEvent_t stop = nullptr;
+  KScope(Ctx& c, int id) : t(c.timers[id].on ? &c.timers[id] : nullptr), st(c.stream) {
+    if (!t) return;
+    if (t->used + 2 > t->ev.size()) {
+      hipEvent_t a, b;
+      T9_HIP(hipEventCreate(&a));
+      T9_HIP(hipEventCreate(&b));
+      t->ev.push_back(a);
+      t->ev.push_back(b);
+    }
+    T9_HIP(hipEventRecord(t->ev[t->used], st));
+    stop = t->ev[t->used + 1];
+    t->used += 2;
+  }
+  ~KScope() {
+    if (t) (void)hipEventRecord(stop, st);
+  }
+};
+
+// Sum the recorded intervals into ms/count and recycle the events.
+void ktimer_collect(Ctx& c, int id);
+void ktimer_release(Ctx& c);
+long long count_cutoff_pairs(Ctx& c);
 
 }  // namespace t9

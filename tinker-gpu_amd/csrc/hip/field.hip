@@ -252,6 +252,7 @@ template <class real>
 void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   const auto& L = c.mlist;
   auto kern = c.ewald ? k_ufield_real<real, 1> : k_ufield_real<real, 0>;
+  KScope ks_(c, KT_UFIELD_REAL);
   hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.nlist.as<int>(),
                      L.off.as<long long>(), L.entries.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
                      (const real*)up, c.pdamp.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, c.scale,
@@ -263,6 +264,7 @@ template <class real>
 void dfield_real_t(Ctx& c) {
   const auto& L = c.mlist;
   auto kern = c.ewald ? k_dfield_real<real, 1> : k_dfield_real<real, 0>;
+  KScope ks_(c, KT_DFIELD_REAL);
   hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.nlist.as<int>(),
                      L.off.as<long long>(), L.entries.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
                      c.pdamp.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, c.scale, box_of<real>(c),
@@ -274,6 +276,7 @@ template <class real>
 void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
   const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.ulist.cutoff > 0;
   const auto& L = sparse ? c.ulist : c.mlist;
+  KScope ks_(c, KT_PRECOND);
   hipLaunchKernelGGL(k_precond<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, sparse ? 1 : 0,
                      (real)c.opt.udiag, L.nlist.as<int>(), L.off.as<long long>(), L.entries.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.pdamp.as<real>(), c.jpolar.as<int>(),

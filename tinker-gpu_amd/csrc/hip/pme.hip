@@ -630,6 +630,7 @@ void fft_roundtrip(Ctx& c) {
   auto launch = [&](int axis, int sign, const FftPlan& p, const cplx_t<real>* t, const ConvArgs& cv) {
     int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
     size_t sh = 2 * (size_t)B * p.n * sizeof(cplx_t<real>);
+    KScope ks_(c, KT_FFT);
     hipLaunchKernelGGL((k_fft_pass<real, B>), dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign, p, t,
                        cv);
     T9_LAUNCH_CHECK();
@@ -667,6 +668,7 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   const size_t sh = 2 * sizeof(long long) * c.tsz[0] * c.tsz[1] * c.tsz[2];
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   auto kern = mode == 0 ? k_spread<real, 0> : k_spread<real, 1>;
+  KScope ks_(c, KT_SPREAD);
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
                      c.tile_count.as<int>(), c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
                      (const real*)s1, (const real*)s2, a1, a2, a3, RealTraits<real>::fixed_scale,
@@ -681,6 +683,7 @@ void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) 
   fft_roundtrip<real>(c);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
+  KScope ks_(c, KT_GATHER);
   hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
                      c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
                      (const real*)ud, (const real*)up, (real*)fd, (real*)fp);
@@ -694,6 +697,7 @@ void mpole_field_t(Ctx& c, void* fd, void* fp) {
   fft_roundtrip<real>(c);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
+  KScope ks_(c, KT_GATHER);
   hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
                      c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
                      c.rpole.as<real>(), (real*)fd, (real*)fp);

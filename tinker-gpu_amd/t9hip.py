@@ -27,7 +27,9 @@ EXPORTS = [
     "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
     "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
+    "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe",
 ]
+K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND = range(6)
 
 
 class T9Error(RuntimeError):
@@ -83,6 +85,9 @@ def lib():
             ("t9_get_uind", [_vp, _d, _d]),
             ("t9_set_uind", [_vp, _d, _d]),
             ("t9_sync", [_vp]),
+            ("t9_kernel_timing", [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int64)]),
+            ("t9_cutoff_pairs", [_vp, ctypes.POINTER(ctypes.c_int64)]),
+            ("t9_copy_probe", [ctypes.c_int64, ctypes.c_int, _d]),
         ]:
             if hasattr(L, name):  # device entry points are absent from a host-only build
                 getattr(L, name).argtypes = args
@@ -257,3 +262,21 @@ class Context:
 
     def sync(self):
         check(lib().t9_sync(self.h))
+
+    def kernel_timing(self, kernel, mode):
+        """mode 1: clear+start, 0: stop, 2: read.  Returns (total_ms, launches)."""
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        check(lib().t9_kernel_timing(self.h, kernel, mode, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def cutoff_pairs(self):
+        v = ctypes.c_int64()
+        check(lib().t9_cutoff_pairs(self.h, ctypes.byref(v)))
+        return v.value
+
+
+def copy_probe(nbytes, reps=20):
+    ms = ctypes.c_double()
+    check(lib().t9_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
