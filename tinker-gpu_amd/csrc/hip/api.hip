@@ -106,6 +106,10 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   upload(c.pdamp_o, s.pdamp, st);
   upload(c.jpolar_o, s.jpolar, st);
   c.nj = s.njpolar;
+  c.uniform_thole = !s.thlval.empty();
+  c.thole_value = s.thlval.empty() ? 0.0 : s.thlval[0];
+  for (double v : s.thlval)
+    if (v != c.thole_value) c.uniform_thole = false;
   if (c.prec == 32) {
     std::vector<float> t(s.thlval.begin(), s.thlval.end());
     upload(c.thlval, t, st);
@@ -169,6 +173,8 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   c.mlist.cutoff = c.cut < 1e20 ? c.cut + buf : 1e30;
   const bool sparse = s.opt.pcg_prec && s.opt.usolve_cut > 0;
   c.ulist.cutoff = sparse ? std::min(s.opt.usolve_cut + buf, c.mlist.cutoff) : 0;
+  c.mlist.cut = c.cut < 1e20 ? c.cut : 1e30;
+  c.ulist.cut = s.opt.usolve_cut;
   c.brute = true;
   c.morton_bits = 0;
   if (c.periodic) {
@@ -209,6 +215,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
       c.tsz[d] = pme_tile_edge(K);
       c.ntile[d] = K / c.tsz[d];
     }
+    pme_check_fft_sizes(c);
     upload(c.bsmod, bs, st);
     if (c.prec == 32) {
       std::vector<float> t(tw.begin(), tw.end());
@@ -266,6 +273,7 @@ void prepare(Ctx& c) {
     if (nblist_needs_rebuild(c)) nblist_build(c);
     else refresh_sorted_coords(c);
   }
+  if (!c.mlist.cut_valid) nblist_compact(c);
   if (!c.rpole_valid) mpole_init(c);
 }
 
@@ -454,9 +462,12 @@ int t9_nblist_stats(t9_ctx* ctx, int64_t* out5) {
 int t9_mpole_init(t9_ctx* ctx) {
   return guarded([&] {
     auto& c = ctx->c;
+    // a new evaluation: every coordinate-derived cache (in-cutoff lists, PME
+    // spline bins, global multipoles) is rebuilt from the current coordinates
+    c.mlist.cut_valid = c.ulist.cut_valid = false;
+    c.bins_valid = false;
+    c.rpole_valid = false;
     t9::prepare(c);
-    t9::mpole_init(c);
-    c.bins_valid = false;  // a new step: PME spline bins follow the coordinates
     return T9_OK;
   }, T9_ERR_HIP);
 }

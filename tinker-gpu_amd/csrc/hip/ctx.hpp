@@ -59,6 +59,12 @@ struct NbList {
   size_t capacity = 0;
   long long total = 0;  // slots used (including padding)
   long long pairs = 0;  // directed pairs stored
+  // Per-step compaction: the entries of each atom that lie within the
+  // interaction cutoff on the current coordinates, same slab layout (off).
+  double cut = 0;      // interaction cutoff used for the compaction
+  DBuf ncut;           // int [n]
+  DBuf ecut;           // uint32 [capacity]
+  bool cut_valid = false;
 };
 
 // Live kernel timing with HIP events recorded on the context stream around
@@ -95,6 +101,8 @@ struct Ctx {
   double cut = 0;        // real-space cutoff of the multipole/polar pair terms
   int grid[3] = {0, 0, 0};
   int nj = 0;            // polarize types (Thole table is nj x nj)
+  bool uniform_thole = false;  // every pair shares one Thole value
+  double thole_value = 0;
   ScaleTable scale{};    // m, d, p, u per scale class
   int ncodes = 1;
 
@@ -151,11 +159,14 @@ struct Ctx {
 void ctx_init(Ctx& c, const System& s, int precision);
 void nblist_build(Ctx& c);
 bool nblist_needs_rebuild(Ctx& c);
+void nblist_compact(Ctx& c);              // refresh the in-cutoff lists for the current coordinates
 void refresh_sorted_coords(Ctx& c);
 void mpole_init(Ctx& c);
 void dfield_all(Ctx& c);                  // c.field/fieldp = full direct field
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp);  // sorted real vectors
 void pme_bin(Ctx& c);
+void pme_fft_roundtrip(Ctx& c);           // forward FFT, influence function, inverse FFT of c.qgrid
+void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);

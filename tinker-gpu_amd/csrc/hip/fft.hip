@@ -1,0 +1,167 @@
+// 3D FFT of the PME grid for gfx950 (replaces cuFFT C2C in place,
+// src/cudart/fft.cpp:16-99).
+//
+// Five passes per reciprocal-space round trip: x forward, y forward,
+// z forward + influence function + z inverse (one kernel, data stays in
+// LDS), y inverse, x inverse.  Each workgroup loads 16 (FP32) or 8 (FP64)
+// lines — contiguous lines for x, lines at consecutive x for y and z so every
+// global row access is a contiguous 128-byte segment — transforms them in
+// LDS with the Stockham stages of fft_core.hpp and writes them back.
+// The influence function is the reference's pmeConv
+// (src/acc/pme.cpp:220-303): exp(-pi^2 h^2 / a^2) / (pi V h^2 b1 b2 b3),
+// k = 0 set to zero, terms with exponent below -50 dropped; it is evaluated
+// in FP64 on the fly.
+#include "ctx.hpp"
+#include "fft_core.hpp"
+
+namespace t9 {
+namespace {
+
+struct ConvArgs {
+  int enabled;
+  double pterm;    // (pi / alpha)^2
+  double volterm;  // pi * V
+  double il[3];    // 1 / box edge
+  const double* bsmod;
+};
+
+struct DevBarrier {
+  __device__ void operator()() const { __syncthreads(); }
+};
+
+__device__ __forceinline__ long long elem_addr(int axis, int K1, int K2, int lb, int e, long long l0, int x0,
+                                               int other) {
+  if (axis == 0) return (l0 + lb) * K1 + e;
+  if (axis == 1) return (x0 + lb) + (long long)K1 * (e + (long long)K2 * other);
+  return (x0 + lb) + (long long)K1 * (other + (long long)K2 * e);
+}
+
+template <class real, int B>
+__global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid, int K1, int K2, int K3, int axis,
+                                                  int sign, FftPlan plan, const Cplx<real>* __restrict__ tw,
+                                                  ConvArgs conv) {
+  extern __shared__ __align__(16) char smem[];
+  const int N = plan.n;
+  Cplx<real>* buf0 = reinterpret_cast<Cplx<real>*>(smem);
+  const int LP = N + 1;  // padded LDS line pitch: adjacent lines start on different banks
+  Cplx<real>* buf1 = buf0 + B * LP;
+  int nl, x0 = 0, other = 0;
+  long long l0 = 0;
+  if (axis == 0) {
+    const int nlines = K2 * K3;
+    l0 = (long long)blockIdx.x * B;
+    nl = min(B, nlines - (int)l0);
+  } else {
+    const int xb = (K1 + B - 1) / B;
+    x0 = (blockIdx.x % xb) * B;
+    other = blockIdx.x / xb;
+    nl = min(B, K1 - x0);
+  }
+  const int tot = nl * N;
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+    int lb, e;
+    if (axis == 0) {
+      lb = q / N;
+      e = q - lb * N;
+    } else {
+      e = q / nl;
+      lb = q - e * nl;
+    }
+    buf0[lb * LP + e] = grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)];
+  }
+  __syncthreads();
+  Cplx<real>* res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
+  if (conv.enabled) {
+    // z pass: lines are (x0 + lb, y = other), element index is k3
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+      const int lb = q / N, k3 = q - lb * N;
+      const int k1 = x0 + lb, k2 = other;
+      double f = 0;
+      if (k1 != 0 || k2 != 0 || k3 != 0) {
+        const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
+        const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
+        const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
+        const double h1 = r1 * conv.il[0], h2 = r2 * conv.il[1], h3 = r3 * conv.il[2];
+        const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
+        const double term = -conv.pterm * hsq;
+        if (term > -50)
+          f = exp(term) / (hsq * conv.volterm * conv.bsmod[k1] * conv.bsmod[K1 + k2] * conv.bsmod[K1 + K2 + k3]);
+      }
+      Cplx<real> v = res[lb * LP + k3];
+      v.x = (real)(v.x * f);
+      v.y = (real)(v.y * f);
+      res[lb * LP + k3] = v;
+    }
+    __syncthreads();
+    Cplx<real>* spare = res == buf0 ? buf1 : buf0;
+    res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
+  }
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+    int lb, e;
+    if (axis == 0) {
+      lb = q / N;
+      e = q - lb * N;
+    } else {
+      e = q / nl;
+      lb = q - e * nl;
+    }
+    grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)] = res[lb * LP + e];
+  }
+}
+
+template <class real>
+constexpr int lines_per_block() {
+  return sizeof(real) == 4 ? 16 : 8;
+}
+
+template <class real>
+void roundtrip_t(Ctx& c) {
+  constexpr int B = lines_per_block<real>();
+  const int K1 = c.grid[0], K2 = c.grid[1], K3 = c.grid[2];
+  auto* grid = c.qgrid.as<Cplx<real>>();
+  const auto* tw = c.twiddle.as<Cplx<real>>();
+  FftPlan p[3] = {make_fft_plan(K1), make_fft_plan(K2), make_fft_plan(K3)};
+  ConvArgs none{};
+  ConvArgs conv{};
+  conv.enabled = 1;
+  const double pi = 3.14159265358979323846;
+  conv.pterm = (pi / c.aewald) * (pi / c.aewald);
+  conv.volterm = pi * c.box[0] * c.box[1] * c.box[2];
+  for (int d = 0; d < 3; ++d) conv.il[d] = 1.0 / c.box[d];
+  conv.bsmod = c.bsmod.as<double>();
+  const int xb = (K1 + B - 1) / B;
+  const Cplx<real>* twa[3] = {tw, tw + K1, tw + K1 + K2};
+  auto launch = [&](int axis, int sign, const ConvArgs& cv) {
+    const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
+    const size_t sh = 2 * (size_t)B * (p[axis].n + 1) * sizeof(Cplx<real>);
+    KScope ks_(c, KT_FFT);
+    hipLaunchKernelGGL((k_fft_pass<real, B>), dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign,
+                       p[axis], twa[axis], cv);
+    T9_LAUNCH_CHECK();
+  };
+  launch(0, -1, none);
+  launch(1, -1, none);
+  launch(2, -1, conv);
+  launch(1, +1, none);
+  launch(0, +1, none);
+}
+
+}  // namespace
+
+void pme_fft_roundtrip(Ctx& c) {
+  if (c.prec == 32) roundtrip_t<float>(c);
+  else roundtrip_t<double>(c);
+}
+
+void pme_check_fft_sizes(const Ctx& c) {
+  const size_t esize = c.prec == 32 ? 8 : 16;
+  const int B = c.prec == 32 ? 16 : 8;
+  for (int d = 0; d < 3; ++d) {
+    if (make_fft_plan(c.grid[d]).nstage < 0)
+      throw ArgError("PME grid size " + std::to_string(c.grid[d]) + " is not a product of 2, 3 and 5");
+    if (2 * (size_t)B * (c.grid[d] + 1) * esize > 65536)
+      throw ArgError("PME grid dimension too large for the LDS FFT");
+  }
+}
+
+}  // namespace t9

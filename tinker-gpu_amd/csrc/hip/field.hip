@@ -8,9 +8,18 @@
 // the reference's "full pair + (s-1) non-Ewald correction" in one term).
 //
 // Execution: one lane per sorted atom, 64 consecutive atoms per wavefront,
-// each lane walking its own full neighbor list (see nblist.hip).  A lane
+// each lane walking its own in-cutoff neighbor list (compacted once per
+// coordinate update from the buffered list, see nblist.hip), so no lane
+// idles on out-of-range entries and no distance test is repeated.  A lane
 // accumulates its atom's field in FP64 registers and stores it once, so the
 // result is independent of scheduling (no atomics).
+//
+// Per-pair data layout: pos[j] = (x, y, z, 1/pdamp) so one 16-byte load
+// brings coordinates and the Thole radius; when every pair shares one Thole
+// value (water, ions) the UTH variants skip the type-table lookup.  Square
+// roots and reciprocals use the hardware rsqrt; the Ewald constants
+// 2^n a^(2n-1)/sqrt(pi) are precomputed on the host; erfc shares exp(-x^2)
+// with the Ewald recursion (t9_erfc_pair).
 #include "ctx.hpp"
 
 namespace t9 {
@@ -19,103 +28,152 @@ namespace {
 template <class real>
 using R4 = typename RealTraits<real>::real4;
 
-template <class real, int EWALD>
-__global__ __launch_bounds__(256) void k_ufield_real(int n, const int* __restrict__ nlist,
+template <class real>
+struct EwaldK {
+  real a;           // Ewald coefficient
+  real e1, e2, e3;  // 2a/sqrt(pi), 4a^3/sqrt(pi), 8a^5/sqrt(pi)
+};
+
+// Thole damping with the reciprocal radius product rpdik = 1/(pd_i pd_k)
+// (0 when either site is unpolarizable: no damping, as the reference's
+// damp == 0 branch in include/seq/damp.h:8-16).
+template <class real>
+__device__ __forceinline__ void thole_r(real r, real rpdik, real pga, real& s3, real& s5, real& s7) {
+  const real ratio = r * rpdik;
+  const real damp = rpdik == 0 ? (real)-1e16 : -pga * ratio * ratio * ratio;
+  const real e = t9_exp(damp);
+  s3 = 1 - e;
+  s5 = 1 - e * (1 - damp);
+  s7 = 1 - e * (1 - damp + (real)0.6 * damp * damp);
+}
+
+template <class real, int NB>
+__device__ __forceinline__ void ewald_b(real* bn, real r, real rinv, real rr2, const EwaldK<real>& ek) {
+  real ex;
+  bn[0] = t9_erfc_pair(ek.a * r, ex) * rinv;
+  bn[1] = (bn[0] + ek.e1 * ex) * rr2;
+  if (NB > 2) bn[2 < NB ? 2 : 0] = (3 * bn[1] + ek.e2 * ex) * rr2;
+  if (NB > 3) bn[3 < NB ? 3 : 0] = (5 * bn[2 < NB ? 2 : 0] + ek.e3 * ex) * rr2;
+}
+
+// Field of induced dipole pair (ud_k, up_k) at site i, accumulated into
+// a (d) and b (p).  (xr, yr, zr) = x_k - x_i.
+template <class real, int EWALD, int UTH>
+__device__ __forceinline__ void ufield_pair(uint32_t ent, const R4<real>& pi, const R4<real>& pj, real dx, real dy,
+                                            real dz, real px, real py, real pz, int jpi, int jpk,
+                                            const real* __restrict__ thlval, real pga_u, const ScaleTable& sc,
+                                            const BoxT<real>& box, const EwaldK<real>& ek, double* a, double* b) {
+  real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
+  if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+  const real r2 = xr * xr + yr * yr + zr * zr;
+  const real rinv = t9_rsqrt(r2);
+  const real r = r2 * rinv;
+  const real rr2 = rinv * rinv;
+  const real rr3 = rinv * rr2;
+  const real rr5 = 3 * rr3 * rr2;
+  const real pga = UTH ? pga_u : thlval[jpi + jpk];
+  real s3, s5, s7;
+  thole_r(r, pi.w * pj.w, pga, s3, s5, s7);
+  const int code = (int)(ent & (kMaxCodes - 1));
+  const real us = code ? (real)sc.u[code] : (real)1;
+  real c1, c2;
+  if (EWALD) {
+    real bn[3];
+    ewald_b<real, 3>(bn, r, rinv, rr2, ek);
+    c1 = bn[1] - (1 - us * s3) * rr3;
+    c2 = bn[2] - (1 - us * s5) * rr5;
+  } else {
+    c1 = us * s3 * rr3;
+    c2 = us * s5 * rr5;
+  }
+  const real cd = c2 * (dx * xr + dy * yr + dz * zr);
+  const real cp = c2 * (px * xr + py * yr + pz * zr);
+  a[0] += (double)(cd * xr - c1 * dx);
+  a[1] += (double)(cd * yr - c1 * dy);
+  a[2] += (double)(cd * zr - c1 * dz);
+  b[0] += (double)(cp * xr - c1 * px);
+  b[1] += (double)(cp * yr - c1 * py);
+  b[2] += (double)(cp * zr - c1 * pz);
+}
+
+template <class real, int EWALD, int UTH>
+__global__ __launch_bounds__(256) void k_ufield_real(int n, const int* __restrict__ ncut,
                                                      const long long* __restrict__ woff,
-                                                     const uint32_t* __restrict__ entries,
+                                                     const uint32_t* __restrict__ ecut,
                                                      const R4<real>* __restrict__ pos, const real* __restrict__ ud,
-                                                     const real* __restrict__ up, const real* __restrict__ pdamp,
-                                                     const int* __restrict__ jpolar, const real* __restrict__ thlval,
-                                                     int nj, ScaleTable sc, BoxT<real> box, real cut2, real aewald,
+                                                     const real* __restrict__ up, const int* __restrict__ jpolar,
+                                                     const real* __restrict__ thlval, int nj, real pga_u,
+                                                     ScaleTable sc, BoxT<real> box, EwaldK<real> ek,
                                                      real* __restrict__ fd, real* __restrict__ fp) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const int len = nlist[s];
-  const uint32_t* e = entries + woff[s >> 6] + (s & 63);
+  const int len = ncut[s];
+  const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
   const R4<real> pi = pos[s];
-  const real pdi = pdamp[s];
-  const real* trow = thlval + jpolar[s] * nj;
-  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-  for (int k = 0; k < len; ++k) {
-    const uint32_t ent = e[(long long)k * 64];
-    const int j = (int)(ent >> kCodeBits);
-    const int code = (int)(ent & (kMaxCodes - 1));
-    const R4<real> pj = pos[j];
-    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
-    const real r2 = xr * xr + yr * yr + zr * zr;
-    if (r2 > cut2) continue;
-    const real r = t9_sqrt(r2);
-    const real rinv = 1 / r;
-    const real rr2 = rinv * rinv;
-    const real rr3 = rinv * rr2;
-    const real rr5 = 3 * rr3 * rr2;
-    real s3, s5, s7;
-    thole_damp(r, pdi * pdamp[j], trow[jpolar[j]], s3, s5, s7);
-    const real us = (real)sc.u[code];
-    real c1, c2;
-    if (EWALD) {
-      real bn[3];
-      ewald_bn<3>(bn, r, rinv, rr2, aewald);
-      c1 = bn[1] - (1 - us * s3) * rr3;
-      c2 = bn[2] - (1 - us * s5) * rr5;
-    } else {
-      c1 = us * s3 * rr3;
-      c2 = us * s5 * rr5;
-    }
-    const real dx = ud[3 * j], dy = ud[3 * j + 1], dz = ud[3 * j + 2];
-    const real px = up[3 * j], py = up[3 * j + 1], pz = up[3 * j + 2];
-    const real cd = c2 * (dx * xr + dy * yr + dz * zr);
-    const real cp = c2 * (px * xr + py * yr + pz * zr);
-    a0 += (double)(cd * xr - c1 * dx);
-    a1 += (double)(cd * yr - c1 * dy);
-    a2 += (double)(cd * zr - c1 * dz);
-    b0 += (double)(cp * xr - c1 * px);
-    b1 += (double)(cp * yr - c1 * py);
-    b2 += (double)(cp * zr - c1 * pz);
+  const int jpi = UTH ? 0 : jpolar[s] * nj;
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  int k = 0;
+  // two independent pairs per iteration: their gathers overlap in flight
+  for (; k + 1 < len; k += 2) {
+    const uint32_t e0 = e[(long long)k * 64], e1 = e[(long long)(k + 1) * 64];
+    const int j0 = (int)(e0 >> kCodeBits), j1 = (int)(e1 >> kCodeBits);
+    const R4<real> p0 = pos[j0], p1 = pos[j1];
+    const real d0x = ud[3 * j0], d0y = ud[3 * j0 + 1], d0z = ud[3 * j0 + 2];
+    const real q0x = up[3 * j0], q0y = up[3 * j0 + 1], q0z = up[3 * j0 + 2];
+    const real d1x = ud[3 * j1], d1y = ud[3 * j1 + 1], d1z = ud[3 * j1 + 2];
+    const real q1x = up[3 * j1], q1y = up[3 * j1 + 1], q1z = up[3 * j1 + 2];
+    const int t0 = UTH ? 0 : jpolar[j0], t1 = UTH ? 0 : jpolar[j1];
+    ufield_pair<real, EWALD, UTH>(e0, pi, p0, d0x, d0y, d0z, q0x, q0y, q0z, jpi, t0, thlval, pga_u, sc, box, ek, a, b);
+    ufield_pair<real, EWALD, UTH>(e1, pi, p1, d1x, d1y, d1z, q1x, q1y, q1z, jpi, t1, thlval, pga_u, sc, box, ek, a, b);
   }
-  fd[3 * s] = (real)a0;
-  fd[3 * s + 1] = (real)a1;
-  fd[3 * s + 2] = (real)a2;
-  fp[3 * s] = (real)b0;
-  fp[3 * s + 1] = (real)b1;
-  fp[3 * s + 2] = (real)b2;
+  if (k < len) {
+    const uint32_t e0 = e[(long long)k * 64];
+    const int j0 = (int)(e0 >> kCodeBits);
+    const int t0 = UTH ? 0 : jpolar[j0];
+    ufield_pair<real, EWALD, UTH>(e0, pi, pos[j0], ud[3 * j0], ud[3 * j0 + 1], ud[3 * j0 + 2], up[3 * j0],
+                                  up[3 * j0 + 1], up[3 * j0 + 2], jpi, t0, thlval, pga_u, sc, box, ek, a, b);
+  }
+  fd[3 * s] = (real)a[0];
+  fd[3 * s + 1] = (real)a[1];
+  fd[3 * s + 2] = (real)a[2];
+  fp[3 * s] = (real)b[0];
+  fp[3 * s + 1] = (real)b[1];
+  fp[3 * s + 2] = (real)b[2];
 }
 
-template <class real, int EWALD>
-__global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restrict__ nlist,
+template <class real, int EWALD, int UTH>
+__global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restrict__ ncut,
                                                      const long long* __restrict__ woff,
-                                                     const uint32_t* __restrict__ entries,
+                                                     const uint32_t* __restrict__ ecut,
                                                      const R4<real>* __restrict__ pos, const real* __restrict__ rpole,
-                                                     const real* __restrict__ pdamp, const int* __restrict__ jpolar,
-                                                     const real* __restrict__ thlval, int nj, ScaleTable sc,
-                                                     BoxT<real> box, real cut2, real aewald, real* __restrict__ fd,
-                                                     real* __restrict__ fp) {
+                                                     const int* __restrict__ jpolar, const real* __restrict__ thlval,
+                                                     int nj, real pga_u, ScaleTable sc, BoxT<real> box,
+                                                     EwaldK<real> ek, real* __restrict__ fd, real* __restrict__ fp) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const int len = nlist[s];
-  const uint32_t* e = entries + woff[s >> 6] + (s & 63);
+  const int len = ncut[s];
+  const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
   const R4<real> pi = pos[s];
-  const real pdi = pdamp[s];
-  const real* trow = thlval + jpolar[s] * nj;
+  const int jpi = UTH ? 0 : jpolar[s] * nj;
   double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  uint32_t nxt = len > 0 ? e[0] : 0;
   for (int k = 0; k < len; ++k) {
-    const uint32_t ent = e[(long long)k * 64];
+    const uint32_t ent = nxt;
+    if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
     const int j = (int)(ent >> kCodeBits);
-    const int code = (int)(ent & (kMaxCodes - 1));
     const R4<real> pj = pos[j];
     real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
     if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
     const real r2 = xr * xr + yr * yr + zr * zr;
-    if (r2 > cut2) continue;
-    const real r = t9_sqrt(r2);
-    const real rinv = 1 / r;
+    const real rinv = t9_rsqrt(r2);
+    const real r = r2 * rinv;
     const real rr2 = rinv * rinv;
     const real rr3 = rinv * rr2;
     const real rr5 = 3 * rr3 * rr2;
     const real rr7 = 5 * rr5 * rr2;
+    const real pga = UTH ? pga_u : thlval[jpi + jpolar[j]];
     real s3, s5, s7;
-    thole_damp(r, pdi * pdamp[j], trow[jpolar[j]], s3, s5, s7);
+    thole_r(r, pi.w * pj.w, pga, s3, s5, s7);
     const real* pk = rpole + 10 * (size_t)j;
     const real ck = pk[0], dkx = pk[1], dky = pk[2], dkz = pk[3];
     const real qxx = pk[4], qyy = pk[5], qzz = pk[6], qxy = pk[7], qxz = pk[8], qyz = pk[9];
@@ -125,8 +183,9 @@ __global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restric
     const real qkz = qxz * xr + qyz * yr + qzz * zr;
     const real qkr = qkx * xr + qky * yr + qkz * zr;
     real bn[4] = {0, 0, 0, 0};
-    if (EWALD) ewald_bn<4>(bn, r, rinv, rr2, aewald);
-    const real ds = (real)sc.d[code], ps = (real)sc.p[code];
+    if (EWALD) ewald_b<real, 4>(bn, r, rinv, rr2, ek);
+    const int code = (int)(ent & (kMaxCodes - 1));
+    const real ds = code ? (real)sc.d[code] : (real)1, ps = code ? (real)sc.p[code] : (real)1;
     real c1, c2, c3;
     if (EWALD) {
       c1 = bn[1] - (1 - ds * s3) * rr3;
@@ -173,16 +232,15 @@ __global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restric
 
 // z = udiag * alpha * r + sum_{r_ik <= ucut} alpha_i alpha_k T_ik r_k
 // (sparse preconditioner, src/acc/amoeba/induce.cpp:310-420).
-template <class real>
-__global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, const int* __restrict__ nlist,
+template <class real, int UTH>
+__global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, const int* __restrict__ ncut,
                                                  const long long* __restrict__ woff,
-                                                 const uint32_t* __restrict__ entries,
+                                                 const uint32_t* __restrict__ ecut,
                                                  const R4<real>* __restrict__ pos, const real* __restrict__ polarity,
-                                                 const real* __restrict__ pdamp, const int* __restrict__ jpolar,
-                                                 const real* __restrict__ thlval, int nj, ScaleTable sc,
-                                                 BoxT<real> box, real cut2, const real* __restrict__ rd,
-                                                 const real* __restrict__ rp, real* __restrict__ zd,
-                                                 real* __restrict__ zp) {
+                                                 const int* __restrict__ jpolar, const real* __restrict__ thlval,
+                                                 int nj, real pga_u, ScaleTable sc, BoxT<real> box,
+                                                 const real* __restrict__ rd, const real* __restrict__ rp,
+                                                 real* __restrict__ zd, real* __restrict__ zp) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const real poli = polarity[s];
@@ -190,27 +248,30 @@ __global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, 
   double a0 = f * rd[3 * s], a1 = f * rd[3 * s + 1], a2 = f * rd[3 * s + 2];
   double b0 = f * rp[3 * s], b1 = f * rp[3 * s + 1], b2 = f * rp[3 * s + 2];
   if (sparse) {
-    const int len = nlist[s];
-    const uint32_t* e = entries + woff[s >> 6] + (s & 63);
+    const int len = ncut[s];
+    const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
     const R4<real> pi = pos[s];
-    const real pdi = pdamp[s];
-    const real* trow = thlval + jpolar[s] * nj;
+    const int jpi = UTH ? 0 : jpolar[s] * nj;
+    uint32_t nxt = len > 0 ? e[0] : 0;
     for (int k = 0; k < len; ++k) {
-      const uint32_t ent = e[(long long)k * 64];
+      const uint32_t ent = nxt;
+      if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
       const int j = (int)(ent >> kCodeBits);
-      const int code = (int)(ent & (kMaxCodes - 1));
       const R4<real> pj = pos[j];
       real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
       if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
       const real r2 = xr * xr + yr * yr + zr * zr;
-      if (r2 > cut2) continue;
-      const real r = t9_sqrt(r2);
+      const real rinv = t9_rsqrt(r2);
+      const real r = r2 * rinv;
+      const real pga = UTH ? pga_u : thlval[jpi + jpolar[j]];
       real s3, s5, s7;
-      thole_damp(r, pdi * pdamp[j], trow[jpolar[j]], s3, s5, s7);
-      const real us = (real)sc.u[code];
+      thole_r(r, pi.w * pj.w, pga, s3, s5, s7);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const real us = code ? (real)sc.u[code] : (real)1;
       const real polik = poli * polarity[j];
-      const real rr3 = us * s3 * polik / (r * r2);
-      const real rr5 = 3 * us * s5 * polik / (r * r2 * r2);
+      const real ri3 = rinv * rinv * rinv;
+      const real rr3 = us * s3 * polik * ri3;
+      const real rr5 = 3 * us * s5 * polik * ri3 * rinv * rinv;
       const real m0 = rr5 * xr * xr - rr3, m1 = rr5 * xr * yr, m2 = rr5 * xr * zr;
       const real m3 = rr5 * yr * yr - rr3, m4 = rr5 * yr * zr, m5 = rr5 * zr * zr - rr3;
       const real x0 = rd[3 * j], x1 = rd[3 * j + 1], x2 = rd[3 * j + 2];
@@ -243,32 +304,45 @@ BoxT<real> box_of(const Ctx& c) {
 }
 
 template <class real>
-real cut2_of(double cut) {
-  double c2 = cut * cut;
-  return c2 > 1e30 ? (real)1e30 : (real)c2;
+EwaldK<real> ewald_k(const Ctx& c) {
+  const double a = c.aewald, isp = 0.56418958354775628695;  // 1/sqrt(pi)
+  EwaldK<real> k;
+  k.a = (real)a;
+  k.e1 = (real)(2 * a * isp);
+  k.e2 = (real)(4 * a * a * a * isp);
+  k.e3 = (real)(8 * a * a * a * a * a * isp);
+  return k;
+}
+
+void require_cut(const NbList& L) {
+  if (!L.cut_valid) throw ArgError("in-cutoff neighbor list is stale (internal ordering error)");
 }
 
 template <class real>
 void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   const auto& L = c.mlist;
-  auto kern = c.ewald ? k_ufield_real<real, 1> : k_ufield_real<real, 0>;
+  require_cut(L);
+  auto kern = c.ewald ? (c.uniform_thole ? k_ufield_real<real, 1, 1> : k_ufield_real<real, 1, 0>)
+                      : (c.uniform_thole ? k_ufield_real<real, 0, 1> : k_ufield_real<real, 0, 0>);
   KScope ks_(c, KT_UFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.nlist.as<int>(),
-                     L.off.as<long long>(), L.entries.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
-                     (const real*)up, c.pdamp.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, c.scale,
-                     box_of<real>(c), cut2_of<real>(c.cut), (real)c.aewald, (real*)fd, (real*)fp);
+  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+                     L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
+                     (const real*)up, c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale,
+                     box_of<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp);
   T9_LAUNCH_CHECK();
 }
 
 template <class real>
 void dfield_real_t(Ctx& c) {
   const auto& L = c.mlist;
-  auto kern = c.ewald ? k_dfield_real<real, 1> : k_dfield_real<real, 0>;
+  require_cut(L);
+  auto kern = c.ewald ? (c.uniform_thole ? k_dfield_real<real, 1, 1> : k_dfield_real<real, 1, 0>)
+                      : (c.uniform_thole ? k_dfield_real<real, 0, 1> : k_dfield_real<real, 0, 0>);
   KScope ks_(c, KT_DFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.nlist.as<int>(),
-                     L.off.as<long long>(), L.entries.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
-                     c.pdamp.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, c.scale, box_of<real>(c),
-                     cut2_of<real>(c.cut), (real)c.aewald, c.field.as<real>(), c.fieldp.as<real>());
+  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+                     L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
+                     c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, box_of<real>(c),
+                     ewald_k<real>(c), c.field.as<real>(), c.fieldp.as<real>());
   T9_LAUNCH_CHECK();
 }
 
@@ -276,12 +350,14 @@ template <class real>
 void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
   const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.ulist.cutoff > 0;
   const auto& L = sparse ? c.ulist : c.mlist;
+  if (sparse) require_cut(L);
+  auto kern = c.uniform_thole ? k_precond<real, 1> : k_precond<real, 0>;
   KScope ks_(c, KT_PRECOND);
-  hipLaunchKernelGGL(k_precond<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, sparse ? 1 : 0,
-                     (real)c.opt.udiag, L.nlist.as<int>(), L.off.as<long long>(), L.entries.as<uint32_t>(),
-                     c.pos.as<R4<real>>(), c.polarity.as<real>(), c.pdamp.as<real>(), c.jpolar.as<int>(),
-                     c.thlval.as<real>(), c.nj, c.scale, box_of<real>(c), cut2_of<real>(c.opt.usolve_cut),
-                     (const real*)rd, (const real*)rp, (real*)zd, (real*)zp);
+  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, sparse ? 1 : 0,
+                     (real)c.opt.udiag, L.ncut.as<int>(), L.off.as<long long>(), L.ecut.as<uint32_t>(),
+                     c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
+                     (real)c.thole_value, c.scale, box_of<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
+                     (real*)zp);
   T9_LAUNCH_CHECK();
 }
 

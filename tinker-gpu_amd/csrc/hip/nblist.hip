@@ -164,7 +164,7 @@ __global__ void k_build_sorted(int n, const int* __restrict__ perm, int* __restr
   p.x = wrap1<real>(x, lx, periodic);
   p.y = wrap1<real>(y, ly, periodic);
   p.z = wrap1<real>(z, lz, periodic);
-  p.w = 0;
+  p.w = (real)(pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0);  // reciprocal Thole radius
   pos[s] = p;
   cellc[s] = cellc_orig[i];
   double a = polarity_o[i];
@@ -176,7 +176,8 @@ __global__ void k_build_sorted(int n, const int* __restrict__ perm, int* __restr
 
 template <class real>
 __global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double* __restrict__ xyz, double lx,
-                              double ly, double lz, int periodic, typename RealTraits<real>::real4* __restrict__ pos) {
+                              double ly, double lz, int periodic, const double* __restrict__ pdamp_o,
+                              typename RealTraits<real>::real4* __restrict__ pos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
@@ -184,7 +185,7 @@ __global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double*
   p.x = wrap1<real>(xyz[3 * i], lx, periodic);
   p.y = wrap1<real>(xyz[3 * i + 1], ly, periodic);
   p.z = wrap1<real>(xyz[3 * i + 2], lz, periodic);
-  p.w = 0;
+  p.w = (real)(pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0);
   pos[s] = p;
 }
 
@@ -379,6 +380,48 @@ void nblist_build_t(Ctx& c) {
   c.coords_dirty = false;
   c.rpole_valid = false;
   c.bins_valid = false;
+  c.mlist.cut_valid = c.ulist.cut_valid = false;
+}
+
+// Keep the entries within the interaction cutoff (same FP arithmetic as the
+// pair kernels, so those kernels need no distance test of their own).
+template <class real>
+__global__ void k_compact(int n, const int* __restrict__ nlist, const long long* __restrict__ woff,
+                          const uint32_t* __restrict__ entries, const typename RealTraits<real>::real4* __restrict__ pos,
+                          BoxT<real> box, real cut2, int* __restrict__ ncut, uint32_t* __restrict__ ecut) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const long long base = woff[s >> 6] + (s & 63);
+  const uint32_t* e = entries + base;
+  uint32_t* o = ecut + base;
+  const auto pi = pos[s];
+  int cnt = 0;
+  const int len = nlist[s];
+  for (int k = 0; k < len; ++k) {
+    const uint32_t ent = e[(long long)k * 64];
+    const auto pj = pos[ent >> kCodeBits];
+    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
+    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    if (xr * xr + yr * yr + zr * zr <= cut2) o[(long long)(cnt++) * 64] = ent;
+  }
+  ncut[s] = cnt;
+}
+
+template <class real>
+void compact_t(Ctx& c) {
+  BoxT<real> box = make_box<real>(c);
+  for (NbList* L : {&c.mlist, &c.ulist}) {
+    if (L->cutoff <= 0 || !L->entries.p) continue;
+    L->ncut.alloc(sizeof(int) * c.n);
+    L->ecut.alloc(sizeof(uint32_t) * L->capacity);
+    const double c2 = L->cut * L->cut;
+    hipLaunchKernelGGL(k_compact<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L->nlist.as<int>(),
+                       L->off.as<long long>(), L->entries.as<uint32_t>(),
+                       c.pos.as<typename RealTraits<real>::real4>(), box, (real)(c2 > 1e30 ? 1e30 : c2),
+                       L->ncut.as<int>(), L->ecut.as<uint32_t>());
+    T9_LAUNCH_CHECK();
+    L->cut_valid = true;
+  }
 }
 
 template <class real>
@@ -402,11 +445,13 @@ template <class real>
 void refresh_t(Ctx& c) {
   const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0, lz = c.periodic ? c.box[2] : 1.0;
   hipLaunchKernelGGL(k_refresh_pos<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
-                     c.xyz.as<double>(), lx, ly, lz, c.periodic ? 1 : 0, c.pos.as<typename RealTraits<real>::real4>());
+                     c.xyz.as<double>(), lx, ly, lz, c.periodic ? 1 : 0, c.pdamp_o.as<double>(),
+                     c.pos.as<typename RealTraits<real>::real4>());
   T9_LAUNCH_CHECK();
   c.coords_dirty = false;
   c.rpole_valid = false;
   c.bins_valid = false;
+  c.mlist.cut_valid = c.ulist.cut_valid = false;
 }
 
 }  // namespace
@@ -417,6 +462,11 @@ void nblist_build(Ctx& c) {
 }
 
 bool nblist_needs_rebuild(Ctx& c) { return c.prec == 32 ? needs_rebuild_t<float>(c) : needs_rebuild_t<double>(c); }
+
+void nblist_compact(Ctx& c) {
+  if (c.prec == 32) compact_t<float>(c);
+  else compact_t<double>(c);
+}
 
 void refresh_sorted_coords(Ctx& c) {
   if (c.prec == 32) refresh_t<float>(c);

@@ -12,10 +12,10 @@
 //
 // Spreading ("owner computes"): the grid is cut into tiles whose edge divides
 // K; each workgroup owns one tile, takes the atoms whose support base lies in
-// its own tile or in the adjacent lower tiles, accumulates their
-// contributions to its own points in LDS as 64-bit fixed point, and stores
-// each point once.  Integer sums are order-independent, so the grid — and
-// everything downstream — is bit-reproducible without global atomics.
+// its own tile or in the adjacent lower tiles, and each thread sums, in a
+// fixed atom order, the contributions to one row of the tile's own points.
+// Every grid point is written once by one thread: the grid — and everything
+// downstream — is bit-reproducible with no atomics.
 //
 // FFT: one workgroup transforms 16 (FP32) or 8 (FP64) grid lines in LDS with
 // a Stockham auto-sort over radices 4, 2, 3, 5; the influence-function
@@ -128,6 +128,25 @@ __global__ void k_tile_scatter(int n, const int4* __restrict__ pbase, const int*
   atoms[start[t] + atomicAdd(&cursor[t], 1)] = s;
 }
 
+// Order each bin's atoms by sorted index (insertion sort per bin), which
+// fixes the summation order of the spreading gather.
+__global__ void k_tile_sort(int nbin, const int* __restrict__ start, const int* __restrict__ count,
+                            int* __restrict__ atoms) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbin) return;
+  int* a = atoms + start[t];
+  const int c = count[t];
+  for (int i = 1; i < c; ++i) {
+    const int v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int m) {
   __shared__ int sh[1024];
   __shared__ int carry;
@@ -157,328 +176,201 @@ struct TileGeom {
   int K[3], T[3], nt[3];
 };
 
-// Local index (0..T-1) of grid point base + m inside tile [lo, lo + T), or -1.
-__device__ __forceinline__ int local_idx(int base, int m, int K, int lo, int T) {
-  int g = base + m;
-  if (g >= K) g -= K;
-  int l = g - lo;
-  return (l >= 0 && l < T) ? l : -1;
-}
-
 // MODE 0: induced dipoles d (real part) and p (imaginary part).
 // MODE 1: permanent multipoles (real part; imaginary part zero).
+//
+// Owner-computes gather: the workgroup owns one tile.  Phase 1 stages the
+// B-spline weights of the atoms of its own bin and of the lower neighbour
+// bins (chunks of at most kSpreadChunk atoms) in LDS.  Phase 2 gives each
+// thread one grid row of the tile (fixed y, z) and one of Q interleaved
+// slices of the staged atoms; the thread adds the contributions of its atoms
+// to its row in registers.  The Q partial rows are then summed in slice
+// order.  Bins are sorted by atom index, so every grid point is a sum in a
+// fixed order: the grid is bit-reproducible with no atomics at all.
+constexpr int kSpreadChunk = 256;
+constexpr int kMaxTile = 16;
+
+template <class real, int MODE>
+struct SpreadAtom {
+  static constexpr int LV = MODE == 0 ? 2 : 3;
+  real th[3][LV][5];
+  real f[MODE == 0 ? 6 : 10];
+  int off[3];
+};
+
+// offset of grid index l from the atom's support base, wrapped; valid if < 5
+__device__ __forceinline__ int support_m(int l, int off, int K) {
+  int m = l - off;
+  if (m < 0) m += K;
+  return m;
+}
+
 template <class real, int MODE>
 __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restrict__ tile_start,
                                                 const int* __restrict__ tile_count,
                                                 const int* __restrict__ tile_atoms, const R4<real>* __restrict__ pw,
                                                 const int4* __restrict__ pbase, const real* __restrict__ src1,
-                                                const real* __restrict__ src2, real a1, real a2, real a3,
-                                                double scale, cplx_t<real>* __restrict__ grid) {
-  extern __shared__ long long acc[];
+                                                const real* __restrict__ src2, real a1, real a2, real a3, int chunk,
+                                                cplx_t<real>* __restrict__ grid) {
+  using SA = SpreadAtom<real, MODE>;
+  constexpr int LV = SA::LV;
+  constexpr int NACC = MODE == 0 ? 2 : 1;
+  extern __shared__ __align__(16) char smem[];
+  SA* atoms = reinterpret_cast<SA*>(smem);
+  __shared__ int bin_start[8], bin_count[8], bin_pref[9];
   const int T1 = g.T[0], T2 = g.T[1], T3 = g.T[2];
-  const int npt = T1 * T2 * T3;
   const int tt = blockIdx.x;
   const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
-  const int lo1 = t1 * T1, lo2 = t2 * T2, lo3 = t3 * T3;
-  for (int q = threadIdx.x; q < 2 * npt; q += blockDim.x) acc[q] = 0;
-  __syncthreads();
-  for (int c = 0; c < 8; ++c) {
+  const int lo[3] = {t1 * T1, t2 * T2, t3 * T3};
+  const int rows = T2 * T3;
+  const int Q = 256 / rows;  // host guarantees rows <= 256
+  const int row = threadIdx.x % rows, slice = threadIdx.x / rows;
+  const bool worker = slice < Q;
+  const int l2 = row % T2, l3 = row / T2;
+  if (threadIdx.x < 8) {
+    const int c = threadIdx.x;
     const int da = c & 1, db = (c >> 1) & 1, dc = c >> 2;
-    if ((da && g.nt[0] == 1) || (db && g.nt[1] == 1) || (dc && g.nt[2] == 1)) continue;
-    int b1 = t1 - da, b2 = t2 - db, b3 = t3 - dc;
-    if (b1 < 0) b1 += g.nt[0];
-    if (b2 < 0) b2 += g.nt[1];
-    if (b3 < 0) b3 += g.nt[2];
-    const int bin = b1 + g.nt[0] * (b2 + g.nt[1] * b3);
-    const int st = tile_start[bin], ct = tile_count[bin];
-    for (int q = threadIdx.x; q < ct; q += blockDim.x) {
-      const int s = tile_atoms[st + q];
-      const R4<real> w = pw[s];
-      const int4 b = pbase[s];
-      int l1[5], l2[5], l3[5];
-      bool any1 = false, any2 = false, any3 = false;
+    int st = 0, ct = 0;
+    if (!((da && g.nt[0] == 1) || (db && g.nt[1] == 1) || (dc && g.nt[2] == 1))) {
+      int b1 = t1 - da, b2 = t2 - db, b3 = t3 - dc;
+      if (b1 < 0) b1 += g.nt[0];
+      if (b2 < 0) b2 += g.nt[1];
+      if (b3 < 0) b3 += g.nt[2];
+      const int bin = b1 + g.nt[0] * (b2 + g.nt[1] * b3);
+      st = tile_start[bin];
+      ct = tile_count[bin];
+    }
+    bin_start[c] = st;
+    bin_count[c] = ct;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bin_pref[0] = 0;
+    for (int c = 0; c < 8; ++c) bin_pref[c + 1] = bin_pref[c] + bin_count[c];
+  }
+  __syncthreads();
+  real acc[NACC][kMaxTile];
 #pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        l1[m] = local_idx(b.x, m, g.K[0], lo1, T1);
-        l2[m] = local_idx(b.y, m, g.K[1], lo2, T2);
-        l3[m] = local_idx(b.z, m, g.K[2], lo3, T3);
-        any1 |= l1[m] >= 0;
-        any2 |= l2[m] >= 0;
-        any3 |= l3[m] >= 0;
-      }
-      if (!(any1 && any2 && any3)) continue;
-      constexpr int LV = MODE == 0 ? 2 : 3;
-      real th1[LV][5], th2[LV][5], th3[LV][5];
-      bspline5<LV>(w.x, th1);
-      bspline5<LV>(w.y, th2);
-      bspline5<LV>(w.z, th3);
-      real f[10];
+  for (int q = 0; q < NACC; ++q)
+#pragma unroll
+    for (int l = 0; l < kMaxTile; ++l) acc[q][l] = 0;
+  const int total = bin_pref[8];
+  for (int base = 0; base < total; base += chunk) {
+    const int nchunk = min(chunk, total - base);
+    // ---- phase 1: stage weights
+    for (int a = threadIdx.x; a < nchunk; a += blockDim.x) {
+      const int idx = base + a;
+      int c = 0;
+      while (idx >= bin_pref[c + 1]) ++c;
+      const int s = tile_atoms[bin_start[c] + idx - bin_pref[c]];
+      SA& A = atoms[a];
+      const int4 b = pbase[s];
+      A.off[0] = b.x - lo[0];
+      A.off[1] = b.y - lo[1];
+      A.off[2] = b.z - lo[2];
+      const R4<real> w = pw[s];
+      real th[LV][5];
+      bspline5<LV>(w.x, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[0][lv][m] = th[lv][m];
+      bspline5<LV>(w.y, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[1][lv][m] = th[lv][m];
+      bspline5<LV>(w.z, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[2][lv][m] = th[lv][m];
       if (MODE == 0) {
-        f[0] = src1[3 * s] * a1;
-        f[1] = src1[3 * s + 1] * a2;
-        f[2] = src1[3 * s + 2] * a3;
-        f[3] = src2[3 * s] * a1;
-        f[4] = src2[3 * s + 1] * a2;
-        f[5] = src2[3 * s + 2] * a3;
+        A.f[0] = src1[3 * s] * a1;
+        A.f[1] = src1[3 * s + 1] * a2;
+        A.f[2] = src1[3 * s + 2] * a3;
+        A.f[3] = src2[3 * s] * a1;
+        A.f[4] = src2[3 * s + 1] * a2;
+        A.f[5] = src2[3 * s + 2] * a3;
       } else {
         const real* p = src1 + 10 * (size_t)s;
-        f[0] = p[0];
-        f[1] = p[1] * a1;
-        f[2] = p[2] * a2;
-        f[3] = p[3] * a3;
-        f[4] = p[4] * a1 * a1;
-        f[5] = p[5] * a2 * a2;
-        f[6] = p[6] * a3 * a3;
-        f[7] = 2 * p[7] * a1 * a2;
-        f[8] = 2 * p[8] * a1 * a3;
-        f[9] = 2 * p[9] * a2 * a3;
+        A.f[0] = p[0];
+        A.f[1] = p[1] * a1;
+        A.f[2] = p[2] * a2;
+        A.f[3] = p[3] * a3;
+        A.f[4] = p[4] * a1 * a1;
+        A.f[5] = p[5] * a2 * a2;
+        A.f[6] = p[6] * a3 * a3;
+        A.f[7] = 2 * p[7] * a1 * a2;
+        A.f[8] = 2 * p[8] * a1 * a3;
+        A.f[9] = 2 * p[9] * a2 * a3;
       }
-      for (int m3 = 0; m3 < 5; ++m3) {
-        if (l3[m3] < 0) continue;
-        const real v0 = th3[0][m3], v1 = th3[1][m3];
-        const real v2 = LV > 2 ? th3[LV > 2 ? 2 : 0][m3] : 0;
-        for (int m2 = 0; m2 < 5; ++m2) {
-          if (l2[m2] < 0) continue;
-          const real u0 = th2[0][m2], u1 = th2[1][m2];
-          const real u2 = LV > 2 ? th2[LV > 2 ? 2 : 0][m2] : 0;
-          const int row = T1 * (l2[m2] + T2 * l3[m3]);
-          for (int m1 = 0; m1 < 5; ++m1) {
-            if (l1[m1] < 0) continue;
-            const real t0 = th1[0][m1], t1v = th1[1][m1];
-            const real t2v = LV > 2 ? th1[LV > 2 ? 2 : 0][m1] : 0;
-            const int idx = row + l1[m1];
-            if (MODE == 0) {
-              real vd = f[0] * t1v * u0 * v0 + f[1] * t0 * u1 * v0 + f[2] * t0 * u0 * v1;
-              real vp = f[3] * t1v * u0 * v0 + f[4] * t0 * u1 * v0 + f[5] * t0 * u0 * v1;
-              atomicAdd((unsigned long long*)&acc[idx], (unsigned long long)llrint((double)vd * scale));
-              atomicAdd((unsigned long long*)&acc[npt + idx], (unsigned long long)llrint((double)vp * scale));
-            } else {
-              real val = f[0] * t0 * u0 * v0 + f[1] * t1v * u0 * v0 + f[2] * t0 * u1 * v0 + f[3] * t0 * u0 * v1 +
-                         f[4] * t2v * u0 * v0 + f[5] * t0 * u2 * v0 + f[6] * t0 * u0 * v2 + f[7] * t1v * u1 * v0 +
-                         f[8] * t1v * u0 * v1 + f[9] * t0 * u1 * v1;
-              atomicAdd((unsigned long long*)&acc[idx], (unsigned long long)llrint((double)val * scale));
-            }
+    }
+    __syncthreads();
+    // ---- phase 2: this thread's row, its slice of the staged atoms
+    if (worker) {
+      for (int a = slice; a < nchunk; a += Q) {
+        const SA& A = atoms[a];
+        const int m2 = support_m(lo[1] + l2, A.off[1] + lo[1], g.K[1]);
+        const int m3 = support_m(lo[2] + l3, A.off[2] + lo[2], g.K[2]);
+        if (m2 >= 5 || m3 >= 5) continue;
+        const real u0 = A.th[1][0][m2], u1 = A.th[1][1][m2];
+        const real v0 = A.th[2][0][m3], v1 = A.th[2][1][m3];
+        const int offx = A.off[0] + lo[0];
+        if (MODE == 0) {
+          const real Ad = A.f[0] * u0 * v0, Bd = A.f[1] * u1 * v0 + A.f[2] * u0 * v1;
+          const real Ap = A.f[3] * u0 * v0, Bp = A.f[4] * u1 * v0 + A.f[5] * u0 * v1;
+#pragma unroll
+          for (int l1 = 0; l1 < kMaxTile; ++l1) {
+            if (l1 >= T1) break;
+            const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
+            if (m1 >= 5) continue;
+            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1];
+            acc[0][l1] += Ad * t1v + Bd * t0;
+            acc[NACC - 1][l1] += Ap * t1v + Bp * t0;
+          }
+        } else {
+          const real u2 = A.th[1][LV > 2 ? 2 : 0][m2], v2 = A.th[2][LV > 2 ? 2 : 0][m3];
+          const real* f = A.f;
+          const real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
+                          f[9] * u1 * v1;
+          const real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+          const real R2 = f[4] * u0 * v0;
+#pragma unroll
+          for (int l1 = 0; l1 < kMaxTile; ++l1) {
+            if (l1 >= T1) break;
+            const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
+            if (m1 >= 5) continue;
+            acc[0][l1] += R0 * A.th[0][0][m1] + R1 * A.th[0][1][m1] + R2 * A.th[0][LV > 2 ? 2 : 0][m1];
           }
         }
       }
     }
+    __syncthreads();
   }
+  // ---- combine the Q slices of each row in slice order
+  real* part = reinterpret_cast<real*>(smem);  // [slice][row][NACC][T1], reuses the staging area
+  if (worker)
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+      for (int l1 = 0; l1 < kMaxTile; ++l1)
+        if (l1 < T1) part[((slice * rows + row) * NACC + q) * T1 + l1] = acc[q][l1];
   __syncthreads();
-  const double inv = 1.0 / scale;
-  for (int q = threadIdx.x; q < npt; q += blockDim.x) {
-    const int l1 = q % T1, l2 = (q / T1) % T2, l3 = q / (T1 * T2);
-    const long long gi = (lo1 + l1) + (long long)g.K[0] * ((lo2 + l2) + (long long)g.K[1] * (lo3 + l3));
+  const int npt = T1 * rows;
+  for (int pnt = threadIdx.x; pnt < npt; pnt += blockDim.x) {
+    const int l1 = pnt % T1, r = pnt / T1;
+    real sd = 0, sp = 0;
+    for (int q = 0; q < Q; ++q) {
+      sd += part[((q * rows + r) * NACC + 0) * T1 + l1];
+      if (NACC == 2) sp += part[((q * rows + r) * NACC + NACC - 1) * T1 + l1];
+    }
+    const int rl2 = r % T2, rl3 = r / T2;
+    const long long gi = (lo[0] + l1) + (long long)g.K[0] * ((lo[1] + rl2) + (long long)g.K[1] * (lo[2] + rl3));
     cplx_t<real> v;
-    v.x = (real)((double)acc[q] * inv);
-    v.y = (real)((double)acc[npt + q] * inv);
+    v.x = sd;
+    v.y = NACC == 2 ? sp : (real)0;
     grid[gi] = v;
-  }
-}
-
-// ------------------------------------------------------------ FFT
-
-struct FftPlan {
-  int n;
-  int nstage;
-  int radix[16];
-};
-
-struct ConvArgs {
-  int enabled;
-  double pterm;        // (pi / alpha)^2
-  double volterm;      // pi * V
-  double il[3];        // 1 / box edge
-  const double* bsmod;  // K1 + K2 + K3 moduli (FP64)
-};
-
-template <class real>
-__device__ __forceinline__ cplx_t<real> cmul(cplx_t<real> a, cplx_t<real> b) {
-  cplx_t<real> r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
-  return r;
-}
-
-// Small DFT y_q = sum_r v_r exp(sign 2 pi i r q / R), R in {2,3,4,5}.
-template <class real>
-__device__ __forceinline__ void small_dft(int R, int sign, cplx_t<real>* v, cplx_t<real>* y) {
-  if (R == 2) {
-    y[0].x = v[0].x + v[1].x;
-    y[0].y = v[0].y + v[1].y;
-    y[1].x = v[0].x - v[1].x;
-    y[1].y = v[0].y - v[1].y;
-    return;
-  }
-  if (R == 4) {
-    // exp(sign i pi/2) = sign * i
-    cplx_t<real> a, b, c, d;
-    a.x = v[0].x + v[2].x;
-    a.y = v[0].y + v[2].y;
-    b.x = v[0].x - v[2].x;
-    b.y = v[0].y - v[2].y;
-    c.x = v[1].x + v[3].x;
-    c.y = v[1].y + v[3].y;
-    d.x = v[1].x - v[3].x;
-    d.y = v[1].y - v[3].y;
-    // i*sign*d = (-sign d.y, sign d.x)
-    const real sg = (real)sign;
-    y[0].x = a.x + c.x;
-    y[0].y = a.y + c.y;
-    y[2].x = a.x - c.x;
-    y[2].y = a.y - c.y;
-    y[1].x = b.x - sg * d.y;
-    y[1].y = b.y + sg * d.x;
-    y[3].x = b.x + sg * d.y;
-    y[3].y = b.y - sg * d.x;
-    return;
-  }
-  // R = 3 or 5: direct sum with exact constants
-  const double c3[3] = {1.0, -0.5, -0.5};
-  const double s3[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
-  const double c5[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
-                        0.30901699437494742410};
-  const double s5[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
-                        -0.95105651629515357212};
-  const double* cc = R == 3 ? c3 : c5;
-  const double* ss = R == 3 ? s3 : s5;
-  for (int q = 0; q < R; ++q) {
-    real yx = 0, yy = 0;
-    for (int r = 0; r < R; ++r) {
-      const int k = (r * q) % R;
-      const real cr = (real)cc[k], sr = (real)(sign * ss[k]);
-      yx += v[r].x * cr - v[r].y * sr;
-      yy += v[r].x * sr + v[r].y * cr;
-    }
-    y[q].x = yx;
-    y[q].y = yy;
-  }
-}
-
-template <class real>
-__device__ void fft_lines_lds(cplx_t<real>* buf0, cplx_t<real>* buf1, int nlines, const FftPlan& plan,
-                              const cplx_t<real>* __restrict__ tw, int sign, cplx_t<real>** result) {
-  const int N = plan.n;
-  cplx_t<real>* src = buf0;
-  cplx_t<real>* dst = buf1;
-  int Ns = 1;
-  for (int st = 0; st < plan.nstage; ++st) {
-    const int R = plan.radix[st];
-    const int nb = N / R;
-    const int tstep = N / (Ns * R);
-    for (int q = threadIdx.x; q < nlines * nb; q += blockDim.x) {
-      const int lb = q / nb, j = q - lb * nb;
-      const int k = j % Ns;
-      cplx_t<real> v[5], y[5];
-      for (int r = 0; r < R; ++r) {
-        cplx_t<real> x = src[lb * N + j + r * nb];
-        int ti = (k * r * tstep) % N;
-        cplx_t<real> t = tw[ti];
-        if (sign > 0) t.y = -t.y;
-        v[r] = r == 0 ? x : cmul<real>(x, t);
-      }
-      small_dft<real>(R, sign, v, y);
-      const int ob = lb * N + (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) dst[ob + r * Ns] = y[r];
-    }
-    __syncthreads();
-    cplx_t<real>* t = src;
-    src = dst;
-    dst = t;
-    Ns *= R;
-  }
-  *result = src;
-}
-
-// One pass of 1D FFTs along `axis` over all grid lines.  With conv.enabled
-// (z axis only), the forward transform is followed by the influence-function
-// multiply and the inverse transform, all in LDS.
-template <class real, int B>
-__global__ __launch_bounds__(256) void k_fft_pass(cplx_t<real>* __restrict__ grid, int K1, int K2, int K3, int axis,
-                                                  int sign, FftPlan plan, const cplx_t<real>* __restrict__ tw,
-                                                  ConvArgs conv) {
-  extern __shared__ char smem[];
-  const int N = plan.n;
-  cplx_t<real>* buf0 = (cplx_t<real>*)smem;
-  cplx_t<real>* buf1 = buf0 + B * N;
-  // line geometry
-  long long base[B];
-  long long stride;
-  int nl = 0;
-  if (axis == 0) {
-    const int nlines = K2 * K3;
-    const int l0 = blockIdx.x * B;
-    nl = min(B, nlines - l0);
-    stride = 1;
-    for (int b = 0; b < B; ++b) base[b] = (long long)(l0 + b) * K1;
-  } else {
-    const int xb = (K1 + B - 1) / B;
-    const int x0 = (blockIdx.x % xb) * B;
-    const int other = blockIdx.x / xb;  // z for axis 1, y for axis 2
-    nl = min(B, K1 - x0);
-    if (axis == 1) {
-      stride = K1;
-      for (int b = 0; b < B; ++b) base[b] = (x0 + b) + (long long)K1 * K2 * other;
-    } else {
-      stride = (long long)K1 * K2;
-      for (int b = 0; b < B; ++b) base[b] = (x0 + b) + (long long)K1 * other;
-    }
-  }
-  // load: consecutive threads take consecutive lines of the same element for
-  // axes 1/2 (adjacent x), consecutive elements for axis 0
-  for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-    int lb, e;
-    if (axis == 0) {
-      lb = q / N;
-      e = q - lb * N;
-    } else {
-      e = q / nl;
-      lb = q - e * nl;
-    }
-    buf0[lb * N + e] = grid[base[lb] + e * stride];
-  }
-  __syncthreads();
-  cplx_t<real>* res;
-  fft_lines_lds<real>(buf0, buf1, nl, plan, tw, sign, &res);
-  if (conv.enabled) {
-    // res holds the z-direction forward transform of lines (x0 + lb, y)
-    const int xb = (K1 + B - 1) / B;
-    const int x0 = (blockIdx.x % xb) * B;
-    const int y = blockIdx.x / xb;
-    for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-      const int lb = q / N, k3 = q - lb * N;
-      const int k1 = x0 + lb, k2 = y;
-      double f = 0;
-      if (k1 != 0 || k2 != 0 || k3 != 0) {
-        const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
-        const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
-        const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
-        const double h1 = r1 * conv.il[0], h2 = r2 * conv.il[1], h3 = r3 * conv.il[2];
-        const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
-        const double term = -conv.pterm * hsq;
-        if (term > -50) {
-          const double denom = hsq * conv.volterm * conv.bsmod[k1] * conv.bsmod[K1 + k2] * conv.bsmod[K1 + K2 + k3];
-          f = exp(term) / denom;
-        }
-      }
-      cplx_t<real> v = res[q];
-      v.x = (real)(v.x * f);
-      v.y = (real)(v.y * f);
-      res[q] = v;
-    }
-    __syncthreads();
-    cplx_t<real>* other = res == buf0 ? buf1 : buf0;
-    fft_lines_lds<real>(res, other, nl, plan, tw, -sign, &res);
-  }
-  for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-    int lb, e;
-    if (axis == 0) {
-      lb = q / N;
-      e = q - lb * N;
-    } else {
-      e = q / nl;
-      lb = q - e * nl;
-    }
-    grid[base[lb] + e * stride] = res[lb * N + e];
   }
 }
 
@@ -583,20 +475,6 @@ __global__ __launch_bounds__(256) void k_gather_mpole_field(int n, TileGeom g, c
 
 // ------------------------------------------------------------ host side
 
-FftPlan make_plan(int N) {
-  FftPlan p{};
-  p.n = N;
-  int m = N;
-  for (int R : {4, 2, 3, 5})
-    while (m % R == 0) {
-      if (p.nstage >= 16) throw ArgError("FFT plan too long");
-      p.radix[p.nstage++] = R;
-      m /= R;
-    }
-  if (m != 1) throw ArgError("PME grid size " + std::to_string(N) + " is not a product of 2, 3 and 5");
-  return p;
-}
-
 TileGeom geom(const Ctx& c) {
   TileGeom g;
   for (int d = 0; d < 3; ++d) {
@@ -605,41 +483,6 @@ TileGeom geom(const Ctx& c) {
     g.nt[d] = c.ntile[d];
   }
   return g;
-}
-
-template <class real>
-constexpr int lines_per_block() {
-  return sizeof(real) == 4 ? 16 : 8;
-}
-
-template <class real>
-void fft_roundtrip(Ctx& c) {
-  constexpr int B = lines_per_block<real>();
-  const int K1 = c.grid[0], K2 = c.grid[1], K3 = c.grid[2];
-  auto* grid = c.qgrid.as<cplx_t<real>>();
-  const auto* tw = c.twiddle.as<cplx_t<real>>();
-  FftPlan p1 = make_plan(K1), p2 = make_plan(K2), p3 = make_plan(K3);
-  ConvArgs none{};
-  ConvArgs conv{};
-  conv.enabled = 1;
-  conv.pterm = (kPi / c.aewald) * (kPi / c.aewald);
-  conv.volterm = kPi * c.box[0] * c.box[1] * c.box[2];
-  for (int d = 0; d < 3; ++d) conv.il[d] = 1.0 / c.box[d];
-  conv.bsmod = c.bsmod.as<double>();
-  const int xb = (K1 + B - 1) / B;
-  auto launch = [&](int axis, int sign, const FftPlan& p, const cplx_t<real>* t, const ConvArgs& cv) {
-    int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
-    size_t sh = 2 * (size_t)B * p.n * sizeof(cplx_t<real>);
-    KScope ks_(c, KT_FFT);
-    hipLaunchKernelGGL((k_fft_pass<real, B>), dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign, p, t,
-                       cv);
-    T9_LAUNCH_CHECK();
-  };
-  launch(0, -1, p1, tw, none);
-  launch(1, -1, p2, tw + K1, none);
-  launch(2, -1, p3, tw + K1 + K2, conv);
-  launch(1, +1, p2, tw + K1, none);
-  launch(0, +1, p1, tw, none);
 }
 
 template <class real>
@@ -659,19 +502,27 @@ void pme_bin_t(Ctx& c) {
   hipLaunchKernelGGL(k_tile_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.pme_base.as<int4>(),
                      c.tile_start.as<int>(), c.tile_cursor.as<int>(), c.tile_atoms.as<int>());
   T9_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tile_sort, dim3(ceil_div(ntot, 256)), dim3(256), 0, c.stream, ntot, c.tile_start.as<int>(),
+                     c.tile_count.as<int>(), c.tile_atoms.as<int>());
+  T9_LAUNCH_CHECK();
   c.bins_valid = true;
 }
 
 template <class real>
 void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   const int ntot = c.ntile[0] * c.ntile[1] * c.ntile[2];
-  const size_t sh = 2 * sizeof(long long) * c.tsz[0] * c.tsz[1] * c.tsz[2];
+  const int rows = c.tsz[1] * c.tsz[2];
+  if (rows > 256 || c.tsz[0] > kMaxTile) throw ArgError("spreading tile too large");
+  const size_t per = mode == 0 ? sizeof(SpreadAtom<real, 0>) : sizeof(SpreadAtom<real, 1>);
+  const int chunk = (int)std::min<size_t>(kSpreadChunk, 65536 / per);
+  const size_t part = sizeof(real) * (size_t)256 * (mode == 0 ? 2 : 1) * c.tsz[0];
+  const size_t sh = std::max(per * chunk, part);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   auto kern = mode == 0 ? k_spread<real, 0> : k_spread<real, 1>;
   KScope ks_(c, KT_SPREAD);
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
                      c.tile_count.as<int>(), c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
-                     (const real*)s1, (const real*)s2, a1, a2, a3, RealTraits<real>::fixed_scale,
+                     (const real*)s1, (const real*)s2, a1, a2, a3, chunk,
                      c.qgrid.as<cplx_t<real>>());
   T9_LAUNCH_CHECK();
 }
@@ -680,7 +531,7 @@ template <class real>
 void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 0, ud, up);
-  fft_roundtrip<real>(c);
+  pme_fft_roundtrip(c);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
@@ -694,7 +545,7 @@ template <class real>
 void mpole_field_t(Ctx& c, void* fd, void* fp) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 1, c.rpole.p, nullptr);
-  fft_roundtrip<real>(c);
+  pme_fft_roundtrip(c);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
