@@ -11,6 +11,16 @@
 //   * reciprocal + self field                src/cu/amoeba/field.cu:12-118
 //   * preconditioned CG                      src/acc/amoeba/induce.cpp:41-292
 //   * diagonal / sparse preconditioner       src/acc/amoeba/induce.cpp:296-420
+//   * multipole + polarization energy,      oracle_energy.inc (own formulation,
+//     gradient, virial, torques             checked against empoleewald.cpp,
+//                                           epolarewald.cpp, torque.cu results)
+// Pinning: every function above is checked in tests/test_oracle_golden.py and
+// tests/test_oracle_energy.py against the reference's golden values
+// (test/localframe.cpp, test/localframe3.cpp, test/ref/tinkernist.*) and
+// against closed forms (numpy FFT, Ewald-split invariance, finite-difference
+// gradients, strain-derivative virial).  Parity unpinned: none of the
+// restated functions; the reference's FP32 round-off and iteration counts are
+// not reproducible without a run of its GPU build.
 // Pairs are enumerated exhaustively (minimum image) or through a cell list;
 // every pair is visited from both atoms, so each atom's sums are written by one
 // thread only.  The 3D DFT is evaluated as three separable naive DFTs.
@@ -39,6 +49,7 @@ struct Scale { int k; double m, d, p, u; };
 struct Oracle {
   int n = 0;
   std::vector<double> xyz, pole, rpole;
+  std::vector<double> pole_eff;  // local-frame pole after the chirality check
   std::vector<int> frame, zaxis, xaxis, yaxis, ysign;
   std::vector<double> polarity, pdamp, thlval;
   std::vector<int> jpolar;
@@ -186,6 +197,7 @@ void rotpole(Oracle& o) {
       pole[10 * i + 9] = -pole[10 * i + 9];
     }
   }
+  o.pole_eff = pole;
   for (int i = 0; i < n; ++i) {
     double a[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     const int fr = o.frame[i];
@@ -731,6 +743,8 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
   return s;
 }
 
+#include "oracle_energy.inc"
+
 }  // namespace
 
 // ==================================================================== C ABI
@@ -910,6 +924,20 @@ int orc_induce(void* h, double poleps, int politer, int prec, int guess, double 
   if (iters_out) *iters_out = iter;
   if (rms_out) *rms_out = eps;
   return iter >= 100 ? 1 : 0;
+}
+
+// Energy (e5 = {em, ep, em_real, em_recip, em_self}, kcal/mol), gradient
+// (dE/dx, [n][3]) and virial ([3][3]) of the multipole (terms & 1) and
+// polarization (terms & 2) energies at the given induced dipoles.
+int orc_energy(void* h, int terms, const double* ud, const double* up, double* e5, double* grad, double* vir) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  EnergyOut e = energy_grad(o, terms & 1, terms & 2, ud, up, grad, vir);
+  e5[0] = e.em;
+  e5[1] = e.ep;
+  e5[2] = e.em_real;
+  e5[3] = e.em_recip;
+  e5[4] = e.em_self;
+  return 0;
 }
 
 }  // extern "C"

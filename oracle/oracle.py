@@ -149,6 +149,18 @@ class Oracle:
         _lib().orc_dft3(self.h, _p(g), sign)
         return g[..., 0] + 1j * g[..., 1]
 
+    def energy(self, ud=None, up=None, mpole=True, polar=True):
+        """Energies (dict), gradient dE/dx [n,3] and virial [3,3] of the
+        multipole and/or polarization terms at the given induced dipoles."""
+        n = self.n
+        ud = np.zeros((n, 3)) if ud is None else np.ascontiguousarray(ud, dtype=np.float64)
+        up = np.zeros((n, 3)) if up is None else np.ascontiguousarray(up, dtype=np.float64)
+        e5, g, v = np.zeros(5), np.zeros((n, 3)), np.zeros((3, 3))
+        terms = (1 if mpole else 0) | (2 if polar else 0)
+        _lib().orc_energy(self.h, terms, _p(ud), _p(up), _p(e5), _p(g), _p(v))
+        e = dict(em=e5[0], ep=e5[1], em_real=e5[2], em_recip=e5[3], em_self=e5[4])
+        return e, g, v
+
     def induce(self, eps=None, maxiter=None, prec=None, guess=None, peek=None, usolve=None, udiag=None):
         s = self.sysd
         eps = s["polar_eps"] if eps is None else eps

@@ -272,6 +272,9 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
       s.box[2] = to_d(tok[1]);
     } else if (k == "ewald") {
       o.ewald = true;
+    } else if (k == "cutoff") {
+      // one distance for every nonbonded term (Tinker's CUTOFF keyword)
+      o.ewald_cut = o.mpole_cut = o.vdw_cut = to_d(tok[1]);
     } else if (k == "ewald-cutoff") {
       o.ewald_cut = to_d(tok[1]);
     } else if (k == "ewald-alpha") {
