@@ -131,6 +131,25 @@ int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats);
 int t9_get_uind(t9_ctx* ctx, double* uind, double* uinp);
 int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
 
+/* Energy, gradient and virial of the multipole and polarization terms (the
+ * terms enabled by the system's keywords).  Solves for the induced dipoles
+ * first, as the reference's emplar does.  vers is a bit set: T9_CALC_ENERGY,
+ * T9_CALC_GRAD, T9_CALC_VIRIAL (the reference's calc::energy/grad/virial,
+ * include/tool/rcman.h:105-155).  grad receives dE/dx [n][3] in kcal/mol/Å in
+ * caller order when non-NULL.  Replaces energy(vers) for the AMOEBA
+ * electrostatics terms (src/energy.cpp:319-450 -> emplar(vers),
+ * src/amoeba/emplar.cpp:10-28 and src/cu/amoeba/emplar.cu:546-605, with
+ * torque(vers, gx, gy, gz), src/cu/amoeba/torque.cu:9-380). */
+enum { T9_CALC_ENERGY = 1, T9_CALC_GRAD = 2, T9_CALC_VIRIAL = 4 };
+typedef struct {
+  double esum;          /* em + ep, kcal/mol */
+  double em, ep;        /* permanent multipole, polarization */
+  double em_real, em_recip, em_self;  /* parts of em (recip/self zero without Ewald) */
+  double vir[9];        /* dE/d(strain), kcal/mol, symmetric */
+  int induce_iterations;
+} t9_energy_out;
+int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
+
 /* Wait for all work queued on the context's stream. */
 int t9_sync(t9_ctx* ctx);
 
@@ -139,8 +158,9 @@ int t9_sync(t9_ctx* ctx);
  * src/xdynamic.cpp:175-190).  Live HIP-event timing of one kernel family on
  * the context's stream: mode 1 clears and starts, 0 stops, 2 reads.  Ids:
  * 0 ufield real-space pairs, 1 dfield real-space pairs, 2 PME spread,
- * 3 FFT passes, 4 PME gathers, 5 preconditioner. */
-enum { T9_K_UFIELD = 0, T9_K_DFIELD, T9_K_SPREAD, T9_K_FFT, T9_K_GATHER, T9_K_PRECOND };
+ * 3 FFT passes, 4 PME gathers, 5 preconditioner, 6 PCG vector kernels,
+ * 7 fused multipole + polarization real-space energy/gradient kernel. */
+enum { T9_K_UFIELD = 0, T9_K_DFIELD, T9_K_SPREAD, T9_K_FFT, T9_K_GATHER, T9_K_PRECOND, T9_K_PCG_VEC, T9_K_EMPLAR };
 int t9_kernel_timing(t9_ctx* ctx, int kernel, int mode, double* total_ms, int64_t* launches);
 /* Unique atom pairs within the real-space cutoff on the current coordinates
  * (the algorithmic unit of the pair kernels). */

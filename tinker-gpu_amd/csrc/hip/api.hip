@@ -136,13 +136,13 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     ex[p.k].push_back({p.i, code});
   }
   c.ncodes = (int)classes.size();
-  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = 1.0f;
+  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = 1.0;
   for (auto& kv : classes) {
     int k = kv.second;
-    c.scale.m[k] = (float)std::get<0>(kv.first);
-    c.scale.d[k] = (float)std::get<1>(kv.first);
-    c.scale.p[k] = (float)std::get<2>(kv.first);
-    c.scale.u[k] = (float)std::get<3>(kv.first);
+    c.scale.m[k] = std::get<0>(kv.first);
+    c.scale.d[k] = std::get<1>(kv.first);
+    c.scale.p[k] = std::get<2>(kv.first);
+    c.scale.u[k] = std::get<3>(kv.first);
   }
   std::vector<int> eoff(n + 1, 0), ej, ecode;
   for (int i = 0; i < n; ++i) {
@@ -234,6 +234,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.tile_atoms.alloc(sizeof(int) * n);
   }
   alloc_vectors(c);
+  energy_setup(c, s);
   T9_HIP(hipStreamSynchronize(st));
 }
 
@@ -542,6 +543,38 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp) {
     t9::prepare(c);
     if (uind) t9::scatter_from_host(c, uind, c.uind.p);
     if (uinp) t9::scatter_from_host(c, uinp, c.uinp.p);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL)) throw t9::ArgError("unknown vers bits");
+    // a new evaluation on the current coordinates
+    c.mlist.cut_valid = c.ulist.cut_valid = false;
+    c.bins_valid = false;
+    c.rpole_valid = false;
+    t9::prepare(c);
+    int iters = 0;
+    if (c.opt.use_polar) {
+      double rms = 0;
+      if (t9::induce_pcg(c, &iters, &rms) != 0) throw t9::NotConverged("INDUCE -- induced dipoles are not converged");
+    }
+    double e[16];
+    t9::energy_run(c, vers, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
+    if (out) {
+      out->em_real = e[t9::EP_MREAL];
+      out->em_recip = e[t9::EP_MRECIP];
+      out->em_self = e[t9::EP_MSELF];
+      out->em = out->em_real + out->em_recip + out->em_self;
+      out->ep = e[t9::EP_POLAR];
+      out->esum = out->em + out->ep;
+      const double v6[6] = {e[t9::EP_VXX], e[t9::EP_VYY], e[t9::EP_VZZ], e[t9::EP_VXY], e[t9::EP_VXZ], e[t9::EP_VYZ]};
+      const double v9[9] = {v6[0], v6[3], v6[4], v6[3], v6[1], v6[5], v6[4], v6[5], v6[2]};
+      for (int k = 0; k < 9; ++k) out->vir[k] = (vers & T9_CALC_VIRIAL) ? v9[k] : 0.0;
+      out->induce_iterations = iters;
+    }
     return T9_OK;
   }, T9_ERR_HIP);
 }

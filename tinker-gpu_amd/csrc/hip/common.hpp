@@ -104,7 +104,7 @@ struct BoxT {
 
 // Per scale-class factors (m, d, p, u); class 0 is the all-ones class.
 struct ScaleTable {
-  float m[kMaxCodes], d[kMaxCodes], p[kMaxCodes], u[kMaxCodes];
+  double m[kMaxCodes], d[kMaxCodes], p[kMaxCodes], u[kMaxCodes];
 };
 
 // Thole damping of the 1/r^3 and 1/r^5 (and 1/r^7) tensors:
@@ -153,6 +153,29 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   double r = sh[0];
   __syncthreads();
   return r;
+}
+
+// Deterministic sum of NV per-thread doubles over a 256-thread block (wave
+// butterflies, then the four waves in a fixed order); thread 0 writes
+// row[0..NV).  Every thread of the block must call it.
+template <int NV>
+__device__ __forceinline__ void block_row(double* v, double* row) {
+  __shared__ double sh[4][NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[q] = x;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[w][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) row[q] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
 }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

@@ -77,6 +77,7 @@ enum KernelId : int {
   KT_GATHER,
   KT_PRECOND,
   KT_PCG_VEC,
+  KT_EMPLAR,
   KT_COUNT
 };
 
@@ -149,11 +150,25 @@ struct Ctx {
   DBuf tile_count, tile_start, tile_cursor, tile_atoms;
   bool bins_valid = false;
 
+  // ---- energy / gradient / virial (sorted order, FP64)
+  DBuf egrad;                    // double [n][3] dE/dx
+  DBuf etq;                      // double [n][9] grad phi, grad grad phi (xx yy zz xy xz yz) = dE/dM
+  DBuf eframe;                   // double [n][4][3] torque share of the atom and its frame atoms
+  DBuf ecphim;                   // double [n][10] reciprocal potential of the multipoles (orders 0-2)
+  DBuf escr;                     // real [n][13] packed-grid sources for the reciprocal virial
+  DBuf rev_off, rev_ent;         // caller order: frames naming each atom, (owner << 2) | slot
+  DBuf epart;                    // double [ecap][kEnergyRow] per-block partial sums
+  int erows = 0, ecap = 0;
+
   // ---- state
   bool coords_dirty = true;      // coordinates changed since the sorted copy
   bool rpole_valid = false;
   bool uind_valid = false;
 };
+
+// Per-block partial rows of the energy/virial reductions.
+enum { EP_MREAL = 0, EP_MRECIP, EP_MSELF, EP_POLAR, EP_VXX, EP_VYY, EP_VZZ, EP_VXY, EP_VXZ, EP_VYZ, EP_N };
+constexpr int kEnergyRow = 16;
 
 // ----- entry points implemented across the .hip files
 void ctx_init(Ctx& c, const System& s, int precision);
@@ -171,6 +186,17 @@ void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp
 void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp);
+// spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
+// 2: multipoles src1 (real) + dipoles src2 (imaginary)
+void pme_spread(Ctx& c, int mode, const void* s1, const void* s2);
+// FFT of c.qgrid: roundtrip != 0 runs forward, influence function, inverse;
+// roundtrip == 0 only the forward transforms.  vscale != 0 adds vscale times
+// the convolution virial sums of the transformed grid to the energy rows.
+void pme_fft_virial(Ctx& c, int roundtrip, double vscale);
+double* energy_rows(Ctx& c, int nrows);   // reserve partial rows for one launch
+void energy_setup(Ctx& c, const System& s);
+// vers bit 2: virial; out16[EP_*] energies / virial; grad_orig: caller-order dE/dx or null
+void energy_run(Ctx& c, int vers, double* out16, double* grad_orig);
 
 // sorted <-> caller order transfers of [n][3] real vectors
 void gather_to_host(Ctx& c, const void* dev_sorted_real3, double* host_orig);

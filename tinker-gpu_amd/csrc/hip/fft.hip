@@ -23,6 +23,9 @@ struct ConvArgs {
   double volterm;  // pi * V
   double il[3];    // 1 / box edge
   const double* bsmod;
+  int fwd_only;    // stop after the forward transform (virial sums only)
+  double vscale;   // weight of the convolution virial sums
+  double* vrows;   // per-block partial rows, or null
 };
 
 struct DevBarrier {
@@ -72,26 +75,50 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
   __syncthreads();
   Cplx<real>* res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
   if (conv.enabled) {
-    // z pass: lines are (x0 + lb, y = other), element index is k3
+    // z pass: lines are (x0 + lb, y = other), element index is k3.  The
+    // strain derivative of 1/2 sum_k G(k) |S(k)|^2 (G the influence function)
+    // is accumulated from the transformed values before they are scaled.
+    double vs[6] = {0, 0, 0, 0, 0, 0};
     for (int q = threadIdx.x; q < tot; q += blockDim.x) {
       const int lb = q / N, k3 = q - lb * N;
       const int k1 = x0 + lb, k2 = other;
-      double f = 0;
+      double f = 0, term = 0, hsq = 1, h1 = 0, h2 = 0, h3 = 0;
       if (k1 != 0 || k2 != 0 || k3 != 0) {
         const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
         const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
         const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
-        const double h1 = r1 * conv.il[0], h2 = r2 * conv.il[1], h3 = r3 * conv.il[2];
-        const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
-        const double term = -conv.pterm * hsq;
+        h1 = r1 * conv.il[0];
+        h2 = r2 * conv.il[1];
+        h3 = r3 * conv.il[2];
+        hsq = h1 * h1 + h2 * h2 + h3 * h3;
+        term = -conv.pterm * hsq;
         if (term > -50)
           f = exp(term) / (hsq * conv.volterm * conv.bsmod[k1] * conv.bsmod[K1 + k2] * conv.bsmod[K1 + K2 + k3]);
       }
       Cplx<real> v = res[lb * LP + k3];
-      v.x = (real)(v.x * f);
-      v.y = (real)(v.y * f);
-      res[lb * LP + k3] = v;
+      if (conv.vrows && f != 0) {
+        const double et = 0.5 * f * ((double)v.x * v.x + (double)v.y * v.y);
+        const double vt = 2.0 * (1.0 - term) / hsq * et;
+        vs[0] += h1 * h1 * vt - et;
+        vs[1] += h2 * h2 * vt - et;
+        vs[2] += h3 * h3 * vt - et;
+        vs[3] += h1 * h2 * vt;
+        vs[4] += h1 * h3 * vt;
+        vs[5] += h2 * h3 * vt;
+      }
+      if (!conv.fwd_only) {
+        v.x = (real)(v.x * f);
+        v.y = (real)(v.y * f);
+        res[lb * LP + k3] = v;
+      }
     }
+    if (conv.vrows) {
+      double red[EP_N];
+      for (int q = 0; q < EP_N; ++q) red[q] = 0;
+      for (int q = 0; q < 6; ++q) red[EP_VXX + q] = conv.vscale * vs[q];
+      block_row<EP_N>(red, conv.vrows + (size_t)blockIdx.x * kEnergyRow);
+    }
+    if (conv.fwd_only) return;
     __syncthreads();
     Cplx<real>* spare = res == buf0 ? buf1 : buf0;
     res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
@@ -115,7 +142,7 @@ constexpr int lines_per_block() {
 }
 
 template <class real>
-void roundtrip_t(Ctx& c) {
+void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
   constexpr int B = lines_per_block<real>();
   const int K1 = c.grid[0], K2 = c.grid[1], K3 = c.grid[2];
   auto* grid = c.qgrid.as<Cplx<real>>();
@@ -129,6 +156,9 @@ void roundtrip_t(Ctx& c) {
   conv.volterm = pi * c.box[0] * c.box[1] * c.box[2];
   for (int d = 0; d < 3; ++d) conv.il[d] = 1.0 / c.box[d];
   conv.bsmod = c.bsmod.as<double>();
+  conv.fwd_only = fwd_only ? 1 : 0;
+  conv.vscale = vscale;
+  conv.vrows = vrows;
   const int xb = (K1 + B - 1) / B;
   const Cplx<real>* twa[3] = {tw, tw + K1, tw + K1 + K2};
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
@@ -142,15 +172,30 @@ void roundtrip_t(Ctx& c) {
   launch(0, -1, none);
   launch(1, -1, none);
   launch(2, -1, conv);
+  if (fwd_only) return;
   launch(1, +1, none);
   launch(0, +1, none);
+}
+
+template <class real>
+int zpass_blocks(const Ctx& c) {
+  constexpr int B = lines_per_block<real>();
+  return ((c.grid[0] + B - 1) / B) * c.grid[1];
 }
 
 }  // namespace
 
 void pme_fft_roundtrip(Ctx& c) {
-  if (c.prec == 32) roundtrip_t<float>(c);
-  else roundtrip_t<double>(c);
+  if (c.prec == 32) roundtrip_t<float>(c, false, nullptr, 0.0);
+  else roundtrip_t<double>(c, false, nullptr, 0.0);
+}
+
+void pme_fft_virial(Ctx& c, int roundtrip, double vscale) {
+  if (!roundtrip && vscale == 0) return;
+  const int nz = c.prec == 32 ? zpass_blocks<float>(c) : zpass_blocks<double>(c);
+  double* rows = vscale != 0 ? energy_rows(c, nz) : nullptr;
+  if (c.prec == 32) roundtrip_t<float>(c, !roundtrip, rows, vscale);
+  else roundtrip_t<double>(c, !roundtrip, rows, vscale);
 }
 
 void pme_check_fft_sizes(const Ctx& c) {

@@ -20,6 +20,7 @@
 // FFT: one workgroup transforms 16 (FP32) or 8 (FP64) grid lines in LDS with
 // a Stockham auto-sort over radices 4, 2, 3, 5; the influence-function
 // multiply is fused between the forward and inverse transforms of the z pass.
+#include "bspline.hpp"
 #include "ctx.hpp"
 
 namespace t9 {
@@ -42,48 +43,6 @@ using cplx_t = typename C2<real>::T;
 
 template <class real>
 using R4 = typename RealTraits<real>::real4;
-
-// Order-5 cardinal B-spline weights and derivatives for a point at fractional
-// offset w in [0,1): th[lv][m] = d^lv/dx^lv M5(x) at x = w + 4 - m,
-// m = 0..4 (grid point base + m).
-template <int LV, class real>
-__device__ __forceinline__ void bspline5(real w, real th[LV][5]) {
-  real A2[2] = {w, 1 - w};
-  real A3[3], A4[4], A5[5];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    real lo = k >= 1 ? A2[k - 1] : 0, hi = k <= 1 ? A2[k] : 0;
-    A3[k] = ((w + k) * hi + (3 - w - k) * lo) * (real)0.5;
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    real lo = k >= 1 ? A3[k - 1] : 0, hi = k <= 2 ? A3[k] : 0;
-    A4[k] = ((w + k) * hi + (4 - w - k) * lo) * (real)(1.0 / 3.0);
-  }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    real lo = k >= 1 ? A4[k - 1] : 0, hi = k <= 3 ? A4[k] : 0;
-    A5[k] = ((w + k) * hi + (5 - w - k) * lo) * (real)0.25;
-  }
-#pragma unroll
-  for (int m = 0; m < 5; ++m) {
-    const int k = 4 - m;
-    th[0][m] = A5[k];
-    if (LV > 1) {
-      real a = k <= 3 ? A4[k] : 0, b = k >= 1 ? A4[k - 1] : 0;
-      th[LV > 1 ? 1 : 0][m] = a - b;
-    }
-    if (LV > 2) {
-      real a = k <= 2 ? A3[k] : 0, b = (k >= 1 && k - 1 <= 2) ? A3[k - 1] : 0, c = k >= 2 ? A3[k - 2] : 0;
-      th[LV > 2 ? 2 : 0][m] = a - 2 * b + c;
-    }
-    if (LV > 3) {
-      real a = k <= 1 ? A2[k] : 0, b = (k >= 1 && k - 1 <= 1) ? A2[k - 1] : 0;
-      real c = (k >= 2 && k - 2 <= 1) ? A2[k - 2] : 0, d = k >= 3 ? A2[k - 3] : 0;
-      th[LV > 3 ? 3 : 0][m] = a - 3 * b + 3 * c - d;
-    }
-  }
-}
 
 // ------------------------------------------------------------ binning
 
@@ -178,6 +137,7 @@ struct TileGeom {
 
 // MODE 0: induced dipoles d (real part) and p (imaginary part).
 // MODE 1: permanent multipoles (real part; imaginary part zero).
+// MODE 2: multipoles (real part) and dipoles (imaginary part).
 //
 // Owner-computes gather: the workgroup owns one tile.  Phase 1 stages the
 // B-spline weights of the atoms of its own bin and of the lower neighbour
@@ -194,7 +154,7 @@ template <class real, int MODE>
 struct SpreadAtom {
   static constexpr int LV = MODE == 0 ? 2 : 3;
   real th[3][LV][5];
-  real f[MODE == 0 ? 6 : 10];
+  real f[MODE == 0 ? 6 : (MODE == 1 ? 10 : 13)];
   int off[3];
 };
 
@@ -214,7 +174,7 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
                                                 cplx_t<real>* __restrict__ grid) {
   using SA = SpreadAtom<real, MODE>;
   constexpr int LV = SA::LV;
-  constexpr int NACC = MODE == 0 ? 2 : 1;
+  constexpr int NACC = MODE == 1 ? 1 : 2;
   extern __shared__ __align__(16) char smem[];
   SA* atoms = reinterpret_cast<SA*>(smem);
   __shared__ int bin_start[8], bin_count[8], bin_pref[9];
@@ -304,6 +264,11 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
         A.f[7] = 2 * p[7] * a1 * a2;
         A.f[8] = 2 * p[8] * a1 * a3;
         A.f[9] = 2 * p[9] * a2 * a3;
+        if (MODE == 2) {
+          A.f[MODE == 2 ? 10 : 0] = src2[3 * s] * a1;
+          A.f[MODE == 2 ? 11 : 0] = src2[3 * s + 1] * a2;
+          A.f[MODE == 2 ? 12 : 0] = src2[3 * s + 2] * a3;
+        }
       }
     }
     __syncthreads();
@@ -342,6 +307,17 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
             const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
             if (m1 >= 5) continue;
             acc[0][l1] += R0 * A.th[0][0][m1] + R1 * A.th[0][1][m1] + R2 * A.th[0][LV > 2 ? 2 : 0][m1];
+          }
+          if (MODE == 2) {
+            const real Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
+            const real Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
+#pragma unroll
+            for (int l1 = 0; l1 < kMaxTile; ++l1) {
+              if (l1 >= T1) break;
+              const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
+              if (m1 >= 5) continue;
+              acc[NACC - 1][l1] += Ap * A.th[0][1][m1] + Bp * A.th[0][0][m1];
+            }
           }
         }
       }
@@ -513,12 +489,13 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   const int ntot = c.ntile[0] * c.ntile[1] * c.ntile[2];
   const int rows = c.tsz[1] * c.tsz[2];
   if (rows > 256 || c.tsz[0] > kMaxTile) throw ArgError("spreading tile too large");
-  const size_t per = mode == 0 ? sizeof(SpreadAtom<real, 0>) : sizeof(SpreadAtom<real, 1>);
+  const size_t per = mode == 0 ? sizeof(SpreadAtom<real, 0>)
+                               : (mode == 1 ? sizeof(SpreadAtom<real, 1>) : sizeof(SpreadAtom<real, 2>));
   const int chunk = (int)std::min<size_t>(kSpreadChunk, 65536 / per);
-  const size_t part = sizeof(real) * (size_t)256 * (mode == 0 ? 2 : 1) * c.tsz[0];
+  const size_t part = sizeof(real) * (size_t)256 * (mode == 1 ? 1 : 2) * c.tsz[0];
   const size_t sh = std::max(per * chunk, part);
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
-  auto kern = mode == 0 ? k_spread<real, 0> : k_spread<real, 1>;
+  auto kern = mode == 0 ? k_spread<real, 0> : (mode == 1 ? k_spread<real, 1> : k_spread<real, 2>);
   KScope ks_(c, KT_SPREAD);
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
                      c.tile_count.as<int>(), c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
@@ -560,6 +537,16 @@ void mpole_field_t(Ctx& c, void* fd, void* fp) {
 void pme_bin(Ctx& c) {
   if (c.prec == 32) pme_bin_t<float>(c);
   else pme_bin_t<double>(c);
+}
+
+void pme_spread(Ctx& c, int mode, const void* s1, const void* s2) {
+  if (c.prec == 32) {
+    if (!c.bins_valid) pme_bin_t<float>(c);
+    spread_t<float>(c, mode, s1, s2);
+  } else {
+    if (!c.bins_valid) pme_bin_t<double>(c);
+    spread_t<double>(c, mode, s1, s2);
+  }
 }
 
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {

@@ -27,9 +27,10 @@ EXPORTS = [
     "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
     "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
-    "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe",
+    "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy",
 ]
-K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND = range(6)
+K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR = range(8)
+CALC_ENERGY, CALC_GRAD, CALC_VIRIAL = 1, 2, 4
 
 
 class T9Error(RuntimeError):
@@ -41,6 +42,12 @@ class T9Error(RuntimeError):
 class InduceStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("predictor", ctypes.c_int), ("rms_debye", ctypes.c_double),
                 ("guess_rms", ctypes.c_double)]
+
+
+class EnergyOut(ctypes.Structure):
+    _fields_ = [("esum", ctypes.c_double), ("em", ctypes.c_double), ("ep", ctypes.c_double),
+                ("em_real", ctypes.c_double), ("em_recip", ctypes.c_double), ("em_self", ctypes.c_double),
+                ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int)]
 
 
 _LIB = None
@@ -88,6 +95,7 @@ def lib():
             ("t9_kernel_timing", [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int64)]),
             ("t9_cutoff_pairs", [_vp, ctypes.POINTER(ctypes.c_int64)]),
             ("t9_copy_probe", [ctypes.c_int64, ctypes.c_int, _d]),
+            ("t9_energy", [_vp, ctypes.c_int, ctypes.POINTER(EnergyOut), _d]),
         ]:
             if hasattr(L, name):  # device entry points are absent from a host-only build
                 getattr(L, name).argtypes = args
@@ -259,6 +267,16 @@ class Context:
         ud = np.ascontiguousarray(ud, dtype=np.float64)
         up = np.ascontiguousarray(up, dtype=np.float64)
         check(lib().t9_set_uind(self.h, _p(ud), _p(up)))
+
+    def energy(self, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL, grad=True):
+        """Multipole + polarization energy (dict), gradient dE/dx [n,3]
+        (caller order, or None) and virial [3,3]."""
+        out = EnergyOut()
+        g = np.zeros((self.n, 3)) if grad and (vers & CALC_GRAD) else None
+        check(lib().t9_energy(self.h, vers, ctypes.byref(out), _p(g) if g is not None else None))
+        e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
+                 em_self=out.em_self, iterations=out.induce_iterations)
+        return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     def sync(self):
         check(lib().t9_sync(self.h))
