@@ -158,10 +158,31 @@ void ctx_init(Ctx& c, const System& s, int precision) {
 
   // ---- sorted-order arrays
   const size_t rs = rsize(c);
+  // fixed-point frame: the box, or a virtual box four times the extent
+  if (c.periodic) {
+    for (int d = 0; d < 3; ++d) {
+      c.vorg[d] = 0;
+      c.vlen[d] = c.box[d];
+    }
+  } else {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], s.xyz[3 * i + d]);
+        hi[d] = std::max(hi[d], s.xyz[3 * i + d]);
+      }
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, n > 0 ? hi[d] - lo[d] : 0.0);
+    const double len = 4 * ext + 64;
+    for (int d = 0; d < 3; ++d) {
+      c.vlen[d] = len;
+      c.vorg[d] = (n > 0 ? 0.5 * (lo[d] + hi[d]) : 0.0) - 0.5 * len;
+    }
+  }
   c.perm.alloc(sizeof(int) * n);
   c.iperm.alloc(sizeof(int) * n);
   c.cellc.alloc(sizeof(int) * n);
-  c.pos.alloc(rs * 4 * n);
+  c.pos.alloc(rs == 4 ? sizeof(PosRec<float>) * n : sizeof(PosRec<double>) * n);
   c.rpole.alloc(rs * 10 * n);
   c.polarity.alloc(rs * n);
   c.polinv.alloc(rs * n);
@@ -299,7 +320,7 @@ namespace {
 
 template <class real>
 __global__ void k_count_cut(int n, const int* __restrict__ nlist, const long long* __restrict__ woff,
-                            const uint32_t* __restrict__ entries, const typename RealTraits<real>::real4* __restrict__ pos,
+                            const uint32_t* __restrict__ entries, const PosRec<real>* __restrict__ pos,
                             BoxT<real> box, real cut2, unsigned long long* __restrict__ out) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -309,8 +330,8 @@ __global__ void k_count_cut(int n, const int* __restrict__ nlist, const long lon
   for (int k = 0; k < nlist[s]; ++k) {
     const int j = (int)(e[(long long)k * 64] >> kCodeBits);
     const auto pj = pos[j];
-    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    real xr, yr, zr;
+    pair_disp(pi, pj, box, xr, yr, zr);
     if (xr * xr + yr * yr + zr * zr <= cut2) ++c;
   }
   atomicAdd(out, c);
@@ -321,16 +342,11 @@ long long count_cut_t(Ctx& c) {
   DBuf cnt;
   cnt.alloc(sizeof(unsigned long long));
   T9_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), c.stream));
-  BoxT<real> b;
-  for (int d = 0; d < 3; ++d) {
-    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
-    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
-  }
-  b.periodic = c.periodic ? 1 : 0;
+  BoxT<real> b = make_boxT<real>(c);
   double c2 = c.cut * c.cut;
   hipLaunchKernelGGL(k_count_cut<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
                      c.mlist.nlist.as<int>(), c.mlist.off.as<long long>(), c.mlist.entries.as<uint32_t>(),
-                     c.pos.as<typename RealTraits<real>::real4>(), b, (real)(c2 > 1e30 ? 1e30 : c2),
+                     c.pos.as<PosRec<real>>(), b, (real)(c2 > 1e30 ? 1e30 : c2),
                      cnt.as<unsigned long long>());
   T9_LAUNCH_CHECK();
   unsigned long long h = 0;

@@ -94,13 +94,97 @@ __device__ __forceinline__ void min_image(real& x, real& y, real& z, real lx, re
   z -= lz * t9_rint(z * ilz);
 }
 
-// Box description passed by value to kernels.
+// Box description passed by value to kernels.  q + qlo = l / 2^32: the
+// length of one unit of the FP32 path's fixed-point coordinates, split so
+// that the product with a displacement carries no systematic scale error.
 template <class real>
 struct BoxT {
   real l[3];
   real il[3];
+  real q[3], qlo[3];
   int periodic;
 };
+
+// Sorted-order position record of the pair kernels.
+//  * FP32: fixed-point fractional coordinates, (x - origin) / l mod 1 in
+//    units of 2^-32.  A displacement is the exact integer difference; its
+//    32-bit wrap-around is the minimum image (orthorhombic box; a virtual box
+//    several times the system's extent for non-periodic systems), and only
+//    the final conversion to float rounds.
+//  * FP64: wrapped Cartesian coordinates, minimum image by rint.
+// w = 1 / Thole radius (0 for sites without damping).
+template <class real>
+struct PosRec;
+template <>
+struct alignas(16) PosRec<float> {
+  int x, y, z;
+  float w;
+};
+template <>
+struct alignas(32) PosRec<double> {
+  double x, y, z, w;
+};
+
+// (x, y, z) = minimum-image displacement b - a.
+__device__ __forceinline__ void pair_disp(const PosRec<float>& a, const PosRec<float>& b, const BoxT<float>& box,
+                                          float& x, float& y, float& z) {
+  const float dx = (float)(int)((unsigned)b.x - (unsigned)a.x);
+  const float dy = (float)(int)((unsigned)b.y - (unsigned)a.y);
+  const float dz = (float)(int)((unsigned)b.z - (unsigned)a.z);
+  x = fmaf(dx, box.q[0], dx * box.qlo[0]);
+  y = fmaf(dy, box.q[1], dy * box.qlo[1]);
+  z = fmaf(dz, box.q[2], dz * box.qlo[2]);
+}
+__device__ __forceinline__ void pair_disp(const PosRec<double>& a, const PosRec<double>& b, const BoxT<double>& box,
+                                          double& x, double& y, double& z) {
+  x = b.x - a.x;
+  y = b.y - a.y;
+  z = b.z - a.z;
+  if (box.periodic) min_image(x, y, z, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+}
+
+// Position record of a site from FP64 coordinates; (org, len) is the box
+// (periodic) or the virtual box (non-periodic).
+template <class real>
+__device__ __forceinline__ PosRec<real> make_pos(double x, double y, double z, double w, const double* org,
+                                                 const double* len, int periodic);
+template <>
+__device__ __forceinline__ PosRec<float> make_pos<float>(double x, double y, double z, double w, const double* org,
+                                                         const double* len, int /*periodic*/) {
+  const double v[3] = {x, y, z};
+  int o[3];
+  for (int d = 0; d < 3; ++d) {
+    double f = (v[d] - org[d]) / len[d];
+    f -= floor(f);
+    o[d] = (int)(unsigned)(unsigned long long)(f * 4294967296.0 + 0.5);
+  }
+  PosRec<float> p;
+  p.x = o[0];
+  p.y = o[1];
+  p.z = o[2];
+  p.w = (float)w;
+  return p;
+}
+template <>
+__device__ __forceinline__ PosRec<double> make_pos<double>(double x, double y, double z, double w, const double* org,
+                                                           const double* len, int periodic) {
+  const double v[3] = {x, y, z};
+  double o[3];
+  for (int d = 0; d < 3; ++d) {
+    double f = v[d];
+    if (periodic) {
+      f = v[d] - len[d] * floor(v[d] / len[d]);
+      if (f >= len[d]) f -= len[d];
+    }
+    o[d] = f;
+  }
+  PosRec<double> p;
+  p.x = o[0];
+  p.y = o[1];
+  p.z = o[2];
+  p.w = w;
+  return p;
+}
 
 // Per scale-class factors (m, d, p, u); class 0 is the all-ones class.
 struct ScaleTable {

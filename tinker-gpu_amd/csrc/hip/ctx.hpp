@@ -97,6 +97,8 @@ struct Ctx {
   Options opt;
   double box[3] = {0, 0, 0};
   bool periodic = false;
+  double vorg[3] = {0, 0, 0};    // origin and edges of the fixed-point frame: the box,
+  double vlen[3] = {1, 1, 1};    // or a virtual box around a non-periodic system
   bool ewald = false;
   double aewald = 0;
   double cut = 0;        // real-space cutoff of the multipole/polar pair terms
@@ -119,7 +121,7 @@ struct Ctx {
   // ---- spatial order
   DBuf perm, iperm;              // int [n]
   DBuf cellc;                    // int [n] packed cell coordinates of sorted atoms
-  DBuf pos;                      // real4 [n] wrapped coordinates
+  DBuf pos;                      // PosRec<real> [n] (fixed-point FP32 / wrapped FP64) + 1/Thole radius
   DBuf rpole;                    // real [n][10] global multipoles
   DBuf polarity, polinv, pdamp;  // real [n]
   DBuf jpolar;                   // int [n]
@@ -203,6 +205,21 @@ void gather_to_host(Ctx& c, const void* dev_sorted_real3, double* host_orig);
 void scatter_from_host(Ctx& c, const double* host_orig, void* dev_sorted_real3);
 
 inline size_t rsize(const Ctx& c) { return c.prec == 32 ? sizeof(float) : sizeof(double); }
+
+// Box description of the pair kernels (see PosRec).
+template <class real>
+BoxT<real> make_boxT(const Ctx& c) {
+  BoxT<real> b;
+  for (int d = 0; d < 3; ++d) {
+    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
+    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
+    const double q = c.vlen[d] / 4294967296.0;
+    b.q[d] = (real)q;
+    b.qlo[d] = (real)(q - (double)b.q[d]);
+  }
+  b.periodic = c.periodic ? 1 : 0;
+  return b;
+}
 
 // Records start/stop events around a launch when the timer is on.
 struct KScope {

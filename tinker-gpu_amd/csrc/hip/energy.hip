@@ -235,7 +235,7 @@ struct EmplarArgs {
   const int* ncut;
   const long long* woff;
   const uint32_t* ecut;
-  const R4<real>* pos;
+  const PosRec<real>* pos;
   const real* rpole;
   const real* ud;
   const real* up;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
   Acc<real> acc;
   acc.zero();
   if (s < A.n) {
-    const R4<real> pi = A.pos[s];
+    const PosRec<real> pi = A.pos[s];
     const Mp<real> mi = load_mp(A.rpole + 10 * (size_t)s);
     real udi[3] = {0, 0, 0}, upi[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
     if (DP)
@@ -283,10 +283,9 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
       const uint32_t ent = nxt;
       if (kk + 1 < len) nxt = e[(long long)(kk + 1) * 64];
       const int j = (int)(ent >> kCodeBits);
-      const R4<real> pj = A.pos[j];
-      real R[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-      if (A.box.periodic)
-        min_image(R[0], R[1], R[2], A.box.l[0], A.box.l[1], A.box.l[2], A.box.il[0], A.box.il[1], A.box.il[2]);
+      const PosRec<real> pj = A.pos[j];
+      real R[3];
+      pair_disp(pj, pi, A.box, R[0], R[1], R[2]);
       const real r2 = dot3(R, R);
       const real rinv = t9_rsqrt(r2);
       const real r = r2 * rinv;
@@ -332,11 +331,10 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
                  wu = (real)(A.sc.u[code] - 1);
       if ((!DM || wm == 0) && (!DP || (wd == 0 && wp == 0 && wu == 0))) continue;
       const int j = A.iperm[A.excl_j[q]];
-      const R4<real> pj = A.pos[j];
+      const PosRec<real> pj = A.pos[j];
       // same displacement and test as the in-cutoff compaction
-      real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-      if (A.box.periodic)
-        min_image(xr, yr, zr, A.box.l[0], A.box.l[1], A.box.l[2], A.box.il[0], A.box.il[1], A.box.il[2]);
+      real xr, yr, zr;
+      pair_disp(pi, pj, A.box, xr, yr, zr);
       if (!(xr * xr + yr * yr + zr * zr <= A.cut2)) continue;
       real R[3] = {-xr, -yr, -zr};
       const real r2 = dot3(R, R);
@@ -835,16 +833,6 @@ EwK6<real> ewk6(const Ctx& c) {
   return k;
 }
 
-template <class real>
-BoxT<real> boxT(const Ctx& c) {
-  BoxT<real> b;
-  for (int d = 0; d < 3; ++d) {
-    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
-    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
-  }
-  b.periodic = c.periodic ? 1 : 0;
-  return b;
-}
 
 template <class real>
 void real_space_t(Ctx& c, bool use_m, bool use_p) {
@@ -855,7 +843,7 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   A.ncut = L.ncut.as<int>();
   A.woff = L.off.as<long long>();
   A.ecut = L.ecut.as<uint32_t>();
-  A.pos = c.pos.as<R4<real>>();
+  A.pos = c.pos.as<PosRec<real>>();
   A.rpole = c.rpole.as<real>();
   A.ud = c.uind.as<real>();
   A.up = c.uinp.as<real>();
@@ -870,7 +858,7 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   A.excl_j = c.excl_j.as<int>();
   A.excl_code = c.excl_code.as<int>();
   A.sc = c.scale;
-  A.box = boxT<real>(c);
+  A.box = make_boxT<real>(c);
   const double c2 = L.cut * L.cut;
   A.cut2 = (real)(c2 > 1e30 ? 1e30 : c2);
   A.ek = ewk6<real>(c);

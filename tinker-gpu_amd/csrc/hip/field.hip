@@ -26,7 +26,7 @@ namespace t9 {
 namespace {
 
 template <class real>
-using R4 = typename RealTraits<real>::real4;
+using R4 = PosRec<real>;
 
 template <class real>
 struct EwaldK {
@@ -63,8 +63,8 @@ __device__ __forceinline__ void ufield_pair(uint32_t ent, const R4<real>& pi, co
                                             real dz, real px, real py, real pz, int jpi, int jpk,
                                             const real* __restrict__ thlval, real pga_u, const ScaleTable& sc,
                                             const BoxT<real>& box, const EwaldK<real>& ek, double* a, double* b) {
-  real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-  if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+  real xr, yr, zr;
+  pair_disp(pi, pj, box, xr, yr, zr);
   const real r2 = xr * xr + yr * yr + zr * zr;
   const real rinv = t9_rsqrt(r2);
   const real r = r2 * rinv;
@@ -162,8 +162,8 @@ __global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restric
     if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
     const int j = (int)(ent >> kCodeBits);
     const R4<real> pj = pos[j];
-    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    real xr, yr, zr;
+    pair_disp(pi, pj, box, xr, yr, zr);
     const real r2 = xr * xr + yr * yr + zr * zr;
     const real rinv = t9_rsqrt(r2);
     const real r = r2 * rinv;
@@ -258,8 +258,8 @@ __global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, 
       if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
       const int j = (int)(ent >> kCodeBits);
       const R4<real> pj = pos[j];
-      real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-      if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+      real xr, yr, zr;
+      pair_disp(pi, pj, box, xr, yr, zr);
       const real r2 = xr * xr + yr * yr + zr * zr;
       const real rinv = t9_rsqrt(r2);
       const real r = r2 * rinv;
@@ -292,16 +292,6 @@ __global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, 
   zp[3 * s + 2] = (real)b2;
 }
 
-template <class real>
-BoxT<real> box_of(const Ctx& c) {
-  BoxT<real> b;
-  for (int d = 0; d < 3; ++d) {
-    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
-    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
-  }
-  b.periodic = c.periodic ? 1 : 0;
-  return b;
-}
 
 template <class real>
 EwaldK<real> ewald_k(const Ctx& c) {
@@ -328,7 +318,7 @@ void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
                      (const real*)up, c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale,
-                     box_of<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp);
+                     make_boxT<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp);
   T9_LAUNCH_CHECK();
 }
 
@@ -341,7 +331,7 @@ void dfield_real_t(Ctx& c) {
   KScope ks_(c, KT_DFIELD_REAL);
   hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
-                     c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, box_of<real>(c),
+                     c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, make_boxT<real>(c),
                      ewald_k<real>(c), c.field.as<real>(), c.fieldp.as<real>());
   T9_LAUNCH_CHECK();
 }
@@ -356,7 +346,7 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
   hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, sparse ? 1 : 0,
                      (real)c.opt.udiag, L.ncut.as<int>(), L.off.as<long long>(), L.ecut.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
-                     (real)c.thole_value, c.scale, box_of<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
+                     (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
                      (real*)zp);
   T9_LAUNCH_CHECK();
 }

@@ -137,17 +137,9 @@ __global__ void k_identity(int n, int* __restrict__ perm) {
 }
 
 template <class real>
-__device__ __forceinline__ real wrap1(double x, double l, int periodic) {
-  if (!periodic) return (real)x;
-  double f = x - l * floor(x / l);
-  if (f >= l) f -= l;
-  return (real)f;
-}
-
-template <class real>
 __global__ void k_build_sorted(int n, const int* __restrict__ perm, int* __restrict__ iperm,
-                               const double* __restrict__ xyz, double* __restrict__ xyz_last, double lx, double ly,
-                               double lz, int periodic, typename RealTraits<real>::real4* __restrict__ pos,
+                               const double* __restrict__ xyz, double* __restrict__ xyz_last, double3 vorg,
+                               double3 vlen, int periodic, PosRec<real>* __restrict__ pos,
                                const int* __restrict__ cellc_orig, int* __restrict__ cellc,
                                const double* __restrict__ polarity_o, const double* __restrict__ pdamp_o,
                                const int* __restrict__ jpolar_o, real* __restrict__ polarity,
@@ -160,12 +152,8 @@ __global__ void k_build_sorted(int n, const int* __restrict__ perm, int* __restr
   xyz_last[3 * i] = x;
   xyz_last[3 * i + 1] = y;
   xyz_last[3 * i + 2] = z;
-  typename RealTraits<real>::real4 p;
-  p.x = wrap1<real>(x, lx, periodic);
-  p.y = wrap1<real>(y, ly, periodic);
-  p.z = wrap1<real>(z, lz, periodic);
-  p.w = (real)(pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0);  // reciprocal Thole radius
-  pos[s] = p;
+  const double org[3] = {vorg.x, vorg.y, vorg.z}, len[3] = {vlen.x, vlen.y, vlen.z};
+  pos[s] = make_pos<real>(x, y, z, pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0, org, len, periodic);
   cellc[s] = cellc_orig[i];
   double a = polarity_o[i];
   polarity[s] = (real)a;
@@ -175,23 +163,20 @@ __global__ void k_build_sorted(int n, const int* __restrict__ perm, int* __restr
 }
 
 template <class real>
-__global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double* __restrict__ xyz, double lx,
-                              double ly, double lz, int periodic, const double* __restrict__ pdamp_o,
-                              typename RealTraits<real>::real4* __restrict__ pos) {
+__global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double* __restrict__ xyz, double3 vorg,
+                              double3 vlen, int periodic, const double* __restrict__ pdamp_o,
+                              PosRec<real>* __restrict__ pos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
-  typename RealTraits<real>::real4 p;
-  p.x = wrap1<real>(xyz[3 * i], lx, periodic);
-  p.y = wrap1<real>(xyz[3 * i + 1], ly, periodic);
-  p.z = wrap1<real>(xyz[3 * i + 2], lz, periodic);
-  p.w = (real)(pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0);
-  pos[s] = p;
+  const double org[3] = {vorg.x, vorg.y, vorg.z}, len[3] = {vlen.x, vlen.y, vlen.z};
+  pos[s] = make_pos<real>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], pdamp_o[i] > 0 ? 1.0 / pdamp_o[i] : 0.0, org,
+                          len, periodic);
 }
 
 // Count (fill == 0) or write (fill == 1) each sorted atom's neighbors.
 template <class real>
-__global__ void k_nb_list(int n, int fill, const typename RealTraits<real>::real4* __restrict__ pos,
+__global__ void k_nb_list(int n, int fill, const PosRec<real>* __restrict__ pos,
                           const int* __restrict__ cellc, int ncx, int ncy, int ncz, int brute,
                           const int* __restrict__ cell_start, const int* __restrict__ cell_count, BoxT<real> box,
                           real rl2, const int* __restrict__ perm, const int* __restrict__ excl_off,
@@ -212,8 +197,8 @@ __global__ void k_nb_list(int n, int fill, const typename RealTraits<real>::real
   auto test = [&](int j) {
     if (j == s) return;
     const auto pj = pos[j];
-    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    real xr, yr, zr;
+    pair_disp(pi, pj, box, xr, yr, zr);
     real r2 = xr * xr + yr * yr + zr * zr;
     if (r2 < rl2 && cnt < limit) {
       if (fill) {
@@ -280,16 +265,6 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
   if (dx * dx + dy * dy + dz * dz > lim2) atomicOr(flag, 1);
 }
 
-template <class real>
-BoxT<real> make_box(const Ctx& c) {
-  BoxT<real> b;
-  for (int d = 0; d < 3; ++d) {
-    b.l[d] = (real)(c.periodic ? c.box[d] : 1.0);
-    b.il[d] = (real)(c.periodic ? 1.0 / c.box[d] : 1.0);
-  }
-  b.periodic = c.periodic ? 1 : 0;
-  return b;
-}
 
 template <class real>
 void build_list(Ctx& c, NbList& L) {
@@ -300,11 +275,11 @@ void build_list(Ctx& c, NbList& L) {
   L.off.alloc(sizeof(long long) * (nw + 1));
   DBuf slots;
   slots.alloc(sizeof(long long) * nw);
-  BoxT<real> box = make_box<real>(c);
+  BoxT<real> box = make_boxT<real>(c);
   real rl2 = (real)(L.cutoff * L.cutoff);
   const int nkeys = 1 << (3 * c.morton_bits);
   hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 0,
-                     c.pos.as<typename RealTraits<real>::real4>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
+                     c.pos.as<PosRec<real>>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
                      c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(), c.cell_count.as<int>(), box, rl2,
                      c.perm.as<int>(), c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>(),
                      L.nlist.as<int>(), (const long long*)nullptr, (uint32_t*)nullptr);
@@ -327,7 +302,7 @@ void build_list(Ctx& c, NbList& L) {
   }
   L.total = total;
   hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 1,
-                     c.pos.as<typename RealTraits<real>::real4>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
+                     c.pos.as<PosRec<real>>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
                      c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(), c.cell_count.as<int>(), box, rl2,
                      c.perm.as<int>(), c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>(),
                      L.nlist.as<int>(), L.off.as<long long>(), L.entries.as<uint32_t>());
@@ -368,8 +343,9 @@ void nblist_build_t(Ctx& c) {
     T9_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_build_sorted<real>, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.iperm.as<int>(),
-                     c.xyz.as<double>(), c.xyz_last.as<double>(), lx, ly, lz, c.periodic ? 1 : 0,
-                     c.pos.as<typename RealTraits<real>::real4>(), cellc_orig.as<int>(), c.cellc.as<int>(),
+                     c.xyz.as<double>(), c.xyz_last.as<double>(), make_double3(c.vorg[0], c.vorg[1], c.vorg[2]),
+                     make_double3(c.vlen[0], c.vlen[1], c.vlen[2]), c.periodic ? 1 : 0,
+                     c.pos.as<PosRec<real>>(), cellc_orig.as<int>(), c.cellc.as<int>(),
                      c.polarity_o.as<double>(), c.pdamp_o.as<double>(), c.jpolar_o.as<int>(), c.polarity.as<real>(),
                      c.polinv.as<real>(), c.pdamp.as<real>(), c.jpolar.as<int>());
   T9_LAUNCH_CHECK();
@@ -387,7 +363,7 @@ void nblist_build_t(Ctx& c) {
 // pair kernels, so those kernels need no distance test of their own).
 template <class real>
 __global__ void k_compact(int n, const int* __restrict__ nlist, const long long* __restrict__ woff,
-                          const uint32_t* __restrict__ entries, const typename RealTraits<real>::real4* __restrict__ pos,
+                          const uint32_t* __restrict__ entries, const PosRec<real>* __restrict__ pos,
                           BoxT<real> box, real cut2, int* __restrict__ ncut, uint32_t* __restrict__ ecut) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -400,8 +376,8 @@ __global__ void k_compact(int n, const int* __restrict__ nlist, const long long*
   for (int k = 0; k < len; ++k) {
     const uint32_t ent = e[(long long)k * 64];
     const auto pj = pos[ent >> kCodeBits];
-    real xr = pj.x - pi.x, yr = pj.y - pi.y, zr = pj.z - pi.z;
-    if (box.periodic) min_image(xr, yr, zr, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
+    real xr, yr, zr;
+    pair_disp(pi, pj, box, xr, yr, zr);
     if (xr * xr + yr * yr + zr * zr <= cut2) o[(long long)(cnt++) * 64] = ent;
   }
   ncut[s] = cnt;
@@ -409,7 +385,7 @@ __global__ void k_compact(int n, const int* __restrict__ nlist, const long long*
 
 template <class real>
 void compact_t(Ctx& c) {
-  BoxT<real> box = make_box<real>(c);
+  BoxT<real> box = make_boxT<real>(c);
   for (NbList* L : {&c.mlist, &c.ulist}) {
     if (L->cutoff <= 0 || !L->entries.p) continue;
     L->ncut.alloc(sizeof(int) * c.n);
@@ -417,7 +393,7 @@ void compact_t(Ctx& c) {
     const double c2 = L->cut * L->cut;
     hipLaunchKernelGGL(k_compact<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L->nlist.as<int>(),
                        L->off.as<long long>(), L->entries.as<uint32_t>(),
-                       c.pos.as<typename RealTraits<real>::real4>(), box, (real)(c2 > 1e30 ? 1e30 : c2),
+                       c.pos.as<PosRec<real>>(), box, (real)(c2 > 1e30 ? 1e30 : c2),
                        L->ncut.as<int>(), L->ecut.as<uint32_t>());
     T9_LAUNCH_CHECK();
     L->cut_valid = true;
@@ -443,10 +419,10 @@ bool needs_rebuild_t(Ctx& c) {
 
 template <class real>
 void refresh_t(Ctx& c) {
-  const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0, lz = c.periodic ? c.box[2] : 1.0;
   hipLaunchKernelGGL(k_refresh_pos<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
-                     c.xyz.as<double>(), lx, ly, lz, c.periodic ? 1 : 0, c.pdamp_o.as<double>(),
-                     c.pos.as<typename RealTraits<real>::real4>());
+                     c.xyz.as<double>(), make_double3(c.vorg[0], c.vorg[1], c.vorg[2]),
+                     make_double3(c.vlen[0], c.vlen[1], c.vlen[2]), c.periodic ? 1 : 0, c.pdamp_o.as<double>(),
+                     c.pos.as<PosRec<real>>());
   T9_LAUNCH_CHECK();
   c.coords_dirty = false;
   c.rpole_valid = false;
