@@ -62,6 +62,16 @@ struct Oracle {
   int grid[3] = {0, 0, 0};
   int order = 5;
   std::vector<double> bsmod[3];
+  // vdW and valence (MD terms)
+  bool has_md = false;
+  std::vector<int> jvdw, vparent;
+  int njvdw = 0;
+  std::vector<double> vrmin, veps, vred;
+  std::vector<std::vector<std::pair<int, double>>> vscale;  // both directions
+  double vcut = 9, vtaper = 8.1;
+  std::vector<int> bond_ij, angle_abc, urey_ac;
+  std::vector<double> bond_kr, angle_kt, urey_kr;
+  double cbnd = 0, qbnd = 0, cang = 0, qang = 0, pang = 0, sang = 0;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -83,7 +93,7 @@ const Scale* find_scale(const Oracle& o, int i, int k) {
 // rc, calling f(i, k, xr, yr, zr, r2) where (xr,yr,zr) = x_k - x_i.  The call for
 // a given i happens on one thread.
 template <class F>
-void for_pairs(const Oracle& o, double rc, F f) {
+void for_pairs_at(const Oracle& o, const double* X, double rc, F f) {
   const int n = o.n;
   const double rc2 = rc * rc;
   bool use_cells = o.periodic && n > 3000;
@@ -99,9 +109,9 @@ void for_pairs(const Oracle& o, double rc, F f) {
     for (int i = 0; i < n; ++i) {
       for (int k = 0; k < n; ++k) {
         if (k == i) continue;
-        double xr = o.xyz[3 * k] - o.xyz[3 * i];
-        double yr = o.xyz[3 * k + 1] - o.xyz[3 * i + 1];
-        double zr = o.xyz[3 * k + 2] - o.xyz[3 * i + 2];
+        double xr = X[3 * k] - X[3 * i];
+        double yr = X[3 * k + 1] - X[3 * i + 1];
+        double zr = X[3 * k + 2] - X[3 * i + 2];
         image(o, xr, yr, zr);
         double r2 = xr * xr + yr * yr + zr * zr;
         if (r2 <= rc2) f(i, k, xr, yr, zr, r2);
@@ -114,7 +124,7 @@ void for_pairs(const Oracle& o, double rc, F f) {
   for (int i = n - 1; i >= 0; --i) {
     int c[3];
     for (int d = 0; d < 3; ++d) {
-      double f0 = o.xyz[3 * i + d] / o.box[d];
+      double f0 = X[3 * i + d] / o.box[d];
       f0 -= std::floor(f0);
       c[d] = std::min(nc[d] - 1, (int)(f0 * nc[d]));
     }
@@ -133,15 +143,20 @@ void for_pairs(const Oracle& o, double rc, F f) {
           int x0 = (c0 + a + nc[0]) % nc[0], x1 = (c1 + b + nc[1]) % nc[1], x2 = (c2 + c + nc[2]) % nc[2];
           for (int k = head[x0 + nc[0] * (x1 + nc[1] * x2)]; k >= 0; k = next[k]) {
             if (k == i) continue;
-            double xr = o.xyz[3 * k] - o.xyz[3 * i];
-            double yr = o.xyz[3 * k + 1] - o.xyz[3 * i + 1];
-            double zr = o.xyz[3 * k + 2] - o.xyz[3 * i + 2];
+            double xr = X[3 * k] - X[3 * i];
+            double yr = X[3 * k + 1] - X[3 * i + 1];
+            double zr = X[3 * k + 2] - X[3 * i + 2];
             image(o, xr, yr, zr);
             double r2 = xr * xr + yr * yr + zr * zr;
             if (r2 <= rc2) f(i, k, xr, yr, zr, r2);
           }
         }
   }
+}
+
+template <class F>
+void for_pairs(const Oracle& o, double rc, F f) {
+  for_pairs_at(o, o.xyz.data(), rc, f);
 }
 
 // Thole damping of the dipole-field tensors (damp.h dmpthole3_a), gamma from
@@ -744,6 +759,7 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 }
 
 #include "oracle_energy.inc"
+#include "oracle_md.inc"
 
 }  // namespace
 
@@ -842,8 +858,22 @@ void orc_dft3(void* h, double* grid, int sign) {
 
 // Induced dipoles by preconditioned CG (the reference's PCG, FP64).  Returns
 // 0 on convergence, 1 if the iteration limit was reached.
+// PCG solve as orc_induce, started from a given guess (u0d, u0p) when non-null:
+// r0 = alpha^-1 (udir - u0) + field(u0) (src/cu/amoeba/pcg.cu:52-64).  guess_rms
+// receives the RMS residual of the initial guess in Debye.
+int orc_induce_from(void* h, double poleps, int politer, int prec, int guess, double peek, double ucut,
+                    double udiag, const double* u0d, const double* u0p, double* ud_out, double* up_out,
+                    int* iters_out, double* rms_out, double* guess_rms);
+
 int orc_induce(void* h, double poleps, int politer, int prec, int guess, double peek, double ucut,
                double udiag, double* ud_out, double* up_out, int* iters_out, double* rms_out) {
+  return orc_induce_from(h, poleps, politer, prec, guess, peek, ucut, udiag, nullptr, nullptr, ud_out, up_out,
+                         iters_out, rms_out, nullptr);
+}
+
+int orc_induce_from(void* h, double poleps, int politer, int prec, int guess, double peek, double ucut,
+                    double udiag, const double* u0d, const double* u0p, double* ud_out, double* up_out,
+                    int* iters_out, double* rms_out, double* guess_rms) {
   Oracle& o = *static_cast<Oracle*>(h);
   const int n = o.n, n3 = 3 * n;
   std::vector<double> fd(n3), fp(n3), udir(n3), udirp(n3), ud(n3), up(n3);
@@ -856,7 +886,16 @@ int orc_induce(void* h, double poleps, int politer, int prec, int guess, double 
       udir[3 * i + d] = o.polarity[i] * fd[3 * i + d];
       udirp[3 * i + d] = o.polarity[i] * fp[3 * i + d];
     }
-  if (guess) {
+  if (u0d && u0p) {
+    ud.assign(u0d, u0d + n3);
+    up.assign(u0p, u0p + n3);
+    ufield(o, ud.data(), up.data(), rd.data(), rp.data());
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        rd[3 * i + d] += pinv[i] * (udir[3 * i + d] - ud[3 * i + d]);
+        rp[3 * i + d] += pinv[i] * (udirp[3 * i + d] - up[3 * i + d]);
+      }
+  } else if (guess) {
     ud = udir;
     up = udirp;
     ufield(o, ud.data(), up.data(), rd.data(), rp.data());
@@ -869,6 +908,7 @@ int orc_induce(void* h, double poleps, int politer, int prec, int guess, double 
   for (int i = 0; i < n; ++i)
     if (o.polarity[i] == 0)
       for (int d = 0; d < 3; ++d) rd[3 * i + d] = rp[3 * i + d] = 0;
+  if (guess_rms) *guess_rms = kDebye * std::sqrt(std::max(dot(rd, rd), dot(rp, rp)) / n);
   const bool sparse = prec && ucut > 0;
   precond(o, sparse, ucut, udiag, rd.data(), rp.data(), zd.data(), zp.data());
   cd = zd;
@@ -924,6 +964,61 @@ int orc_induce(void* h, double poleps, int politer, int prec, int guess, double 
   if (iters_out) *iters_out = iter;
   if (rms_out) *rms_out = eps;
   return iter >= 100 ? 1 : 0;
+}
+
+void orc_set_md(void* h, const int* jvdw, int njvdw, const double* rmin, const double* eps, const double* red,
+                const int* parent, int nvs, const int* vs_ik, const double* vs_s, double vcut, double vtaper,
+                int nb, const int* bij, const double* bkr, int na, const int* aabc, const double* akt, int nu,
+                const int* uac, const double* ukr, const double* poly6) {
+  Oracle& o = *static_cast<Oracle*>(h);
+  const int n = o.n;
+  o.has_md = true;
+  o.jvdw.assign(jvdw, jvdw + n);
+  o.njvdw = njvdw;
+  o.vrmin.assign(rmin, rmin + njvdw * njvdw);
+  o.veps.assign(eps, eps + njvdw * njvdw);
+  o.vred.assign(red, red + n);
+  o.vparent.assign(parent, parent + n);
+  o.vscale.assign(n, {});
+  for (int q = 0; q < nvs; ++q) {
+    const int i = vs_ik[2 * q], k = vs_ik[2 * q + 1];
+    o.vscale[i].push_back({k, vs_s[q]});
+    o.vscale[k].push_back({i, vs_s[q]});
+  }
+  o.vcut = vcut;
+  o.vtaper = vtaper;
+  o.bond_ij.assign(bij, bij + 2 * nb);
+  o.bond_kr.assign(bkr, bkr + 2 * nb);
+  o.angle_abc.assign(aabc, aabc + 3 * na);
+  o.angle_kt.assign(akt, akt + 2 * na);
+  o.urey_ac.assign(uac, uac + 2 * nu);
+  o.urey_kr.assign(ukr, ukr + 2 * nu);
+  o.cbnd = poly6[0];
+  o.qbnd = poly6[1];
+  o.cang = poly6[2];
+  o.qang = poly6[3];
+  o.pang = poly6[4];
+  o.sang = poly6[5];
+}
+
+// vdW and valence energies e4 = {evdw, ebond, eangle, eurey}, gradient
+// [n][3] (added to by the selected terms) and virial [3][3].
+int orc_md_energy(void* h, int terms, double* e4, double* grad, double* vir) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  if (!o.has_md) return 1;
+  std::vector<double> g(3 * o.n, 0.0);
+  double W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  e4[0] = (terms & 1) ? vdw_energy(o, g.data(), W) : 0.0;
+  double eb = 0, ea = 0, eu = 0;
+  if (terms & 2) valence_energy(o, &eb, &ea, &eu, g.data(), W);
+  e4[1] = eb;
+  e4[2] = ea;
+  e4[3] = eu;
+  if (grad) std::copy(g.begin(), g.end(), grad);
+  if (vir)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) vir[3 * a + b] = 0.5 * (W[3 * a + b] + W[3 * b + a]);
+  return 0;
 }
 
 // Energy (e5 = {em, ep, em_real, em_recip, em_self}, kcal/mol), gradient

@@ -55,6 +55,15 @@ def _lib():
         L.orc_induce.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _d, _d, _i, _d]
         L.orc_induce.restype = ctypes.c_int
+        L.orc_induce_from.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, _d, _d, _d, _d, _i, _d,
+                                      _d]
+        L.orc_induce_from.restype = ctypes.c_int
+        L.orc_set_md.argtypes = [ctypes.c_void_p, _i, ctypes.c_int, _d, _d, _d, _i, ctypes.c_int, _i, _d,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _i, _d, ctypes.c_int, _i, _d,
+                                 ctypes.c_int, _i, _d, _d]
+        L.orc_md_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d]
+        L.orc_md_energy.restype = ctypes.c_int
         if hasattr(L, "orc_energy"):
             L.orc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d, _d, _d]
             L.orc_energy.restype = ctypes.c_int
@@ -64,6 +73,14 @@ def _lib():
 
 def _p(a, t=_d):
     return a.ctypes.data_as(t)
+
+
+def aspc_coefficients(k=14):
+    """Kolafa's always-stable predictor-corrector coefficients B_1..B_{k+2}
+    (J. Comput. Chem. 25, 335 (2004)); B[0] weights the most recent solution."""
+    from math import comb
+    den = comb(2 * k + 2, k + 1)
+    return np.array([(-1) ** (j + 1) * j * comb(2 * k + 4, k + 2 - j) / den for j in range(1, k + 3)])
 
 
 class Oracle:
@@ -99,6 +116,24 @@ class Oracle:
             _p(f64(sysd["polarity"])), _p(f64(sysd["pdamp"])), _p(i32(sysd["jpolar"]), _i), int(sysd["njpolar"]),
             _p(f64(sysd["thlval"])), int(sysd["nscale"]), _p(ik, _i), _p(mdpu), _p(box) if box is not None else None,
             int(sysd["ewald"]), float(sysd["aewald"]), float(cut), _p(grid, _i), int(sysd["pme_order"]))
+        if "jvdw" in sysd:
+            nj = int(sysd["njvdw"])
+            vs_ik = i32(sysd["vscale_ik"]) if len(sysd["vscale_s"]) else i32(np.zeros((1, 2)))
+            vs_s = f64(sysd["vscale_s"]) if len(sysd["vscale_s"]) else f64(np.zeros(1))
+
+            def arr(key, shape_t, dt):
+                a = sysd[key]
+                return (i32 if dt == "i" else f64)(a if len(a) else np.zeros(shape_t))
+            poly = f64([sysd["bond_cubic"], sysd["bond_quartic"], sysd["angle_cubic"], sysd["angle_quartic"],
+                        sysd["angle_pentic"], sysd["angle_sextic"]])
+            L.orc_set_md(self.h, _p(i32(sysd["jvdw"]), _i), nj, _p(f64(sysd["vdw_rmin"])),
+                         _p(f64(sysd["vdw_epsmix"])), _p(f64(sysd["vdw_red"])), _p(i32(sysd["vdw_parent"]), _i),
+                         len(sysd["vscale_s"]), _p(vs_ik, _i), _p(vs_s), float(sysd["vdw_cut"]),
+                         float(sysd["vdw_taper"]), len(sysd["bond_kr"]), _p(arr("bond_ij", (1, 2), "i"), _i),
+                         _p(arr("bond_kr", (1, 2), "d")), len(sysd["angle_kt"]),
+                         _p(arr("angle_abc", (1, 3), "i"), _i), _p(arr("angle_kt", (1, 2), "d")),
+                         len(sysd["urey_kr"]), _p(arr("urey_ac", (1, 2), "i"), _i), _p(arr("urey_kr", (1, 2), "d")),
+                         _p(poly))
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
@@ -148,6 +183,30 @@ class Oracle:
         g = np.ascontiguousarray(np.stack([grid_c.real, grid_c.imag], axis=-1), dtype=np.float64)
         _lib().orc_dft3(self.h, _p(g), sign)
         return g[..., 0] + 1j * g[..., 1]
+
+    def md_energy(self, vdw=True, valence=True):
+        """vdW and valence energies (dict), gradient [n,3] and virial [3,3]."""
+        e4, g, v = np.zeros(4), np.zeros((self.n, 3)), np.zeros((3, 3))
+        terms = (1 if vdw else 0) | (2 if valence else 0)
+        rc = _lib().orc_md_energy(self.h, terms, _p(e4), _p(g), _p(v))
+        assert rc == 0
+        return dict(evdw=e4[0], ebond=e4[1], eangle=e4[2], eurey=e4[3]), g, v
+
+    def induce_from(self, u0d, u0p, eps=None, maxiter=None):
+        """PCG from the initial guess (u0d, u0p); returns (ud, up, iters, rms,
+        guess_rms, rc)."""
+        s = self.sysd
+        eps = s["polar_eps"] if eps is None else eps
+        maxiter = int(s["polar_iter"]) if maxiter is None else maxiter
+        a = np.ascontiguousarray(u0d, dtype=np.float64)
+        b = np.ascontiguousarray(u0p, dtype=np.float64)
+        ud, up = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        it = np.zeros(1, dtype=np.int32)
+        rms, g = np.zeros(1), np.zeros(1)
+        rc = _lib().orc_induce_from(self.h, eps, maxiter, int(s["pcg_prec"]), int(s["pcg_guess"]), s["pcg_peek"],
+                                    s["usolve_cut"], s["udiag"], _p(a), _p(b), _p(ud), _p(up), _p(it, _i), _p(rms),
+                                    _p(g))
+        return ud, up, int(it[0]), float(rms[0]), float(g[0]), rc
 
     def energy(self, ud=None, up=None, mpole=True, polar=True):
         """Energies (dict), gradient dE/dx [n,3] and virial [3,3] of the

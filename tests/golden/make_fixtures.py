@@ -11,6 +11,10 @@ What it writes (numbers and data files only, no reference source text):
   * tests/golden/tinkernist.npz — test/ref/tinkernist.{arc,uind,udir,def,tef}:
     2 frames x 2,684 atoms (coordinates Å, dipoles Debye, fields as dumped),
     plus types and 1-2 connectivity of test/file/tinkernist/water30.xyz
+  * tests/golden/aspc_reference.json — the ASPC predictor coefficients of
+    src/cu/upredict.cu:50-65 (16 numbers)
+  * tests/golden/nacl_vdw_ref.json — test/nacl.cpp NaCl-1 vdW energies,
+    gradients, virials and the ion separations of test/file/nacl/
   * tinker-gpu_amd/params/amoeba09.prm — the subset of
     test/file/commit_6fe8e913/amoeba09.prm this path uses: the scale-factor
     header and the atom/vdw/bond/angle/ureybrad/multipole/polarize records of
@@ -165,6 +169,32 @@ def write_tinkernist():
     np.savez_compressed(os.path.join(HERE, "tinkernist.npz"), types=types, i12=i12, **d)
 
 
+def write_aspc():
+    """The reference's ASPC predictor table (src/cu/upredict.cu:50-65) as
+    numbers: {"aspc": [16 floats]}, most recent solution first."""
+    src = open(os.path.join(REF, "src/cu/upredict.cu")).read()
+    i = src.index("aspc[16] = {")
+    body = src[i:src.index("};", i)]
+    vals = [float(a) / float(b) for a, b in re.findall(r"(-?[0-9]+)\.\s*/\s*([0-9]+)\.", body)]
+    assert len(vals) == 16, vals
+    json.dump({"aspc": vals}, open(os.path.join(HERE, "aspc_reference.json"), "w"), indent=1)
+    return vals
+
+
+def write_nacl():
+    """vdW golden values of test/nacl.cpp (NaCl-1, no long-range correction)
+    and the ion separations of test/file/nacl/nacl{1,2,3}.xyz."""
+    d = parse_test_arrays(os.path.join(REF, "test/nacl.cpp"))
+    out = {}
+    for sec, f in (("no switch", "nacl1.xyz"), ("switch, near cut", "nacl2.xyz"), ("switch, near off", "nacl3.xyz")):
+        lines = open(os.path.join(REF, "test/file/nacl", f)).read().splitlines()
+        r = float(lines[2].split()[2])
+        pre = f"NaCl-1/- ehal -- {sec}/"
+        out[sec] = dict(r=r, eng=d[pre + "ref_eng"], grad=d[pre + "ref_grad"], vir=d[pre + "ref_v"])
+    json.dump(out, open(os.path.join(HERE, "nacl_vdw_ref.json"), "w"), indent=1)
+    return out
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not mounted")
@@ -178,6 +208,8 @@ def main():
     json.dump(ref, open(os.path.join(HERE, "localframe_ref.json"), "w"), indent=0, sort_keys=True)
     print("golden values:", len(ref))
     write_tinkernist()
+    print("aspc table:", write_aspc())
+    print("nacl:", write_nacl())
 
 
 if __name__ == "__main__":

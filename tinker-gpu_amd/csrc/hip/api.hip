@@ -255,6 +255,8 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.tile_atoms.alloc(sizeof(int) * n);
   }
   alloc_vectors(c);
+  if (s.opt.predictor == "ASPC") c.ualt.alloc(rs * 6 * (size_t)n * 16);
+  else if (s.opt.predictor != "NONE") throw ArgError("polar-predict " + s.opt.predictor + " is not supported");
   energy_setup(c, s);
   T9_HIP(hipStreamSynchronize(st));
 }
@@ -533,9 +535,9 @@ int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats) {
     int rc = t9::induce_pcg(c, &iters, &rms);
     if (stats) {
       stats->iterations = iters;
-      stats->predictor = 0;
+      stats->predictor = c.last_predictor;
       stats->rms_debye = rms;
-      stats->guess_rms = 0;
+      stats->guess_rms = c.last_guess_rms;
     }
     if (uind) t9::gather_to_host(c, c.uind.p, uind);
     if (uinp) t9::gather_to_host(c, c.uinp.p, uinp);

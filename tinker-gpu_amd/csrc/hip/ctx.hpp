@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -162,6 +163,12 @@ struct Ctx {
   DBuf epart;                    // double [ecap][kEnergyRow] per-block partial sums
   int erows = 0, ecap = 0;
 
+  // ---- induced-dipole predictor: the last 16 solutions, caller order
+  DBuf ualt;                     // real [16][2][n][3]
+  int nualt = 0;                 // solutions stored so far (wraps between 16 and 32)
+  int last_predictor = 0;        // 0 direct guess, 1 ASPC
+  double last_guess_rms = 0;     // RMS residual of the last initial guess (Debye)
+
   // ---- state
   bool coords_dirty = true;      // coordinates changed since the sorted copy
   bool rpole_valid = false;
@@ -187,6 +194,7 @@ void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);
+std::array<double, 16> aspc_coefficients();
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp);
 // spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
 // 2: multipoles src1 (real) + dipoles src2 (imaginary)

@@ -9,6 +9,15 @@
 // step lengths, residual norm) are FP64 and live on the device — each dot is
 // a fixed-order two-level reduction, so iterates are bit-reproducible — and
 // the host reads back one residual per iteration to decide convergence.
+//
+// Predictor (polar-predict ASPC, src/amoeba/induce.cpp:29-69,
+// src/cu/upredict.cu:34-201): after 16 converged solves the initial guess is
+// sum_j B_j u(t - j) over the last 16 solutions with Kolafa's always-stable
+// predictor coefficients (k = 14); before that, and without the keyword, the
+// direct-field guess.  The history is kept in caller order so a re-sort of
+// the atoms at a list rebuild does not disturb it.
+#include <array>
+
 #include "ctx.hpp"
 
 namespace t9 {
@@ -16,7 +25,61 @@ namespace {
 
 constexpr int kB = 256;
 // device scalar slots
-enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_NSLOT = 16 };
+enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_EPS0, S_NSLOT = 16 };
+constexpr int kMaxUalt = 16;
+
+struct Coef16 {
+  double c[kMaxUalt];
+};
+
+// u0 = sum_k c_k hist_k (caller-order history, slot k: [2][n][3])
+template <class real>
+__global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict__ perm, Coef16 cf,
+                                                  const real* __restrict__ hist, real* __restrict__ ud,
+                                                  real* __restrict__ up) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  const size_t slot = 6 * (size_t)n;
+  for (int d = 0; d < 3; ++d) {
+    double a = 0, b = 0;
+    for (int k = 0; k < kMaxUalt; ++k) {
+      a += cf.c[k] * (double)hist[k * slot + 3 * (size_t)i + d];
+      b += cf.c[k] * (double)hist[k * slot + 3 * (size_t)n + 3 * (size_t)i + d];
+    }
+    ud[3 * s + d] = (real)a;
+    up[3 * s + d] = (real)b;
+  }
+}
+
+// r0 = alpha^-1 (udir - u0) + (field of u0), zero on unpolarizable sites
+template <class real>
+__global__ __launch_bounds__(kB) void k_rsd_guess(int n, const real* __restrict__ pol, const real* __restrict__ polinv,
+                                                 const real* __restrict__ udir, const real* __restrict__ udirp,
+                                                 const real* __restrict__ ud, const real* __restrict__ up,
+                                                 real* __restrict__ rd, real* __restrict__ rp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const bool zero = pol[s] == 0;
+  const real pi = polinv[s];
+  for (int d = 0; d < 3; ++d) {
+    const int k = 3 * s + d;
+    rd[k] = zero ? (real)0 : pi * (udir[k] - ud[k]) + rd[k];
+    rp[k] = zero ? (real)0 : pi * (udirp[k] - up[k]) + rp[k];
+  }
+}
+
+template <class real>
+__global__ __launch_bounds__(kB) void k_save_hist(int n, const int* __restrict__ perm, const real* __restrict__ ud,
+                                                 const real* __restrict__ up, real* __restrict__ slot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  for (int d = 0; d < 3; ++d) {
+    slot[3 * (size_t)i + d] = ud[3 * s + d];
+    slot[3 * (size_t)n + 3 * (size_t)i + d] = up[3 * s + d];
+  }
+}
 
 template <class real>
 __global__ __launch_bounds__(kB) void k_udir(int n, const real* __restrict__ pol, const real* __restrict__ fd,
@@ -86,6 +149,8 @@ __global__ __launch_bounds__(kB) void k_finalize(int nblk, int ndots, int stage,
   if (stage == 0) {
     sc[S_SUM] = tot[0];
     sc[S_SUMP] = tot[1];
+  } else if (stage == 3) {
+    sc[S_EPS0] = debye * sqrt(fmax(tot[0], tot[1]) / n);
   } else if (stage == 1) {
     sc[S_A] = tot[0] != 0 ? sc[S_SUM] / tot[0] : 0;
     sc[S_AP] = tot[1] != 0 ? sc[S_SUMP] / tot[1] : 0;
@@ -188,15 +253,32 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   const bool guess = c.opt.pcg_guess;
 
   dfield_all(c);
+  // ASPC once 16 solutions are stored (src/cu/amoeba/pcg.cu:30-33)
+  const bool predict = c.opt.predictor == "ASPC" && c.nualt >= kMaxUalt;
   hipLaunchKernelGGL(k_udir<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, c.udir.as<real>(),
-                     c.udirp.as<real>(), ud, up, guess ? 1 : 0);
+                     c.udirp.as<real>(), ud, up, (guess || predict) ? 1 : 0);
   T9_LAUNCH_CHECK();
-  if (guess) {
+  if (predict) {
+    const auto B = aspc_coefficients();
+    Coef16 cf;
+    for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
+    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.perm.as<int>(), cf, c.ualt.as<real>(), ud, up);
+    T9_LAUNCH_CHECK();
+    ufield_all(c, ud, up, rd, rp);
+    hipLaunchKernelGGL(k_rsd_guess<real>, G, Bk, 0, st, n, c.polarity.as<real>(), c.polinv.as<real>(),
+                       c.udir.as<real>(), c.udirp.as<real>(), ud, up, rd, rp);
+  } else if (guess) {
     ufield_all(c, ud, up, rd, rp);
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), rd, rp, rd, rp);
   } else {
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, rd, rp);
   }
+  T9_LAUNCH_CHECK();
+  c.last_predictor = predict ? 1 : 0;
+  // residual norm of the initial guess (Debye), read back with the first iteration
+  hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr, (const real*)nullptr,
+                     (const real*)nullptr, (const real*)nullptr, part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
   T9_LAUNCH_CHECK();
   precond_apply(c, rd, rp, zd, zp);
   T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
@@ -237,12 +319,38 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     }
   }
   c.uind_valid = true;
+  c.last_guess_rms = c.h_scalars[S_EPS0];
+  if (c.opt.predictor == "ASPC") {  // store the solution (src/amoeba/induce.cpp:29-62)
+    const size_t slot = 6 * (size_t)n;
+    hipLaunchKernelGGL(k_save_hist<real>, G, Bk, 0, st, n, c.perm.as<int>(), ud, up,
+                       c.ualt.as<real>() + (size_t)(c.nualt % kMaxUalt) * slot);
+    T9_LAUNCH_CHECK();
+    c.nualt += 1;
+    if (c.nualt > 2 * kMaxUalt) c.nualt -= kMaxUalt;
+  }
   if (iters) *iters = iter;
   if (rms) *rms = eps;
   return iter >= maxiter ? 1 : 0;
 }
 
 }  // namespace
+
+// Kolafa's always-stable predictor-corrector coefficients for k = 14
+// (J. Kolafa, J. Comput. Chem. 25, 335 (2004)):
+//   B_j = (-1)^(j+1) j C(2k+4, k+2-j) / C(2k+2, k+1),  j = 1..k+2,
+// B[0] multiplies the most recent solution.
+std::array<double, 16> aspc_coefficients() {
+  const int k = 14;
+  auto binom = [](int nn, int r) {
+    double v = 1;
+    for (int q = 1; q <= r; ++q) v = v * (nn - r + q) / q;
+    return v;
+  };
+  std::array<double, 16> B{};
+  const double den = binom(2 * k + 2, k + 1);
+  for (int j = 1; j <= k + 2; ++j) B[j - 1] = ((j & 1) ? 1.0 : -1.0) * j * binom(2 * k + 4, k + 2 - j) / den;
+  return B;
+}
 
 int induce_pcg(Ctx& c, int* iters, double* rms) {
   return c.prec == 32 ? induce_t<float>(c, iters, rms) : induce_t<double>(c, iters, rms);

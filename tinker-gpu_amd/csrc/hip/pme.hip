@@ -89,22 +89,6 @@ __global__ void k_tile_scatter(int n, const int4* __restrict__ pbase, const int*
 
 // Order each bin's atoms by sorted index (insertion sort per bin), which
 // fixes the summation order of the spreading gather.
-__global__ void k_tile_sort(int nbin, const int* __restrict__ start, const int* __restrict__ count,
-                            int* __restrict__ atoms) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nbin) return;
-  int* a = atoms + start[t];
-  const int c = count[t];
-  for (int i = 1; i < c; ++i) {
-    const int v = a[i];
-    int j = i - 1;
-    while (j >= 0 && a[j] > v) {
-      a[j + 1] = a[j];
-      --j;
-    }
-    a[j + 1] = v;
-  }
-}
 
 __global__ __launch_bounds__(1024) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int m) {
   __shared__ int sh[1024];
@@ -139,14 +123,15 @@ struct TileGeom {
 // MODE 1: permanent multipoles (real part; imaginary part zero).
 // MODE 2: multipoles (real part) and dipoles (imaginary part).
 //
-// Owner-computes gather: the workgroup owns one tile.  Phase 1 stages the
-// B-spline weights of the atoms of its own bin and of the lower neighbour
-// bins (chunks of at most kSpreadChunk atoms) in LDS.  Phase 2 gives each
-// thread one grid row of the tile (fixed y, z) and one of Q interleaved
-// slices of the staged atoms; the thread adds the contributions of its atoms
-// to its row in registers.  The Q partial rows are then summed in slice
-// order.  Bins are sorted by atom index, so every grid point is a sum in a
-// fixed order: the grid is bit-reproducible with no atomics at all.
+// Owner-computes spreading: the workgroup owns one tile.  Phase 1 stages the
+// B-spline weights and sources of the atoms whose support base lies in its own
+// bin or in one of the seven lower-neighbour bins (chunks of at most
+// kSpreadChunk atoms) in LDS.  Phase 2 hands out (atom, grid row) items; each
+// adds the row's points that fall inside the tile into an LDS copy of the
+// tile as 64-bit fixed-point integers (scale RealTraits::fixed_scale).
+// Integer addition is associative, so the tile holds the same bits whatever
+// order the items run in; it is converted and written to the grid once.
+// Every grid point has one owner: no global atomics, bit-reproducible.
 constexpr int kSpreadChunk = 256;
 constexpr int kMaxTile = 16;
 
@@ -155,14 +140,45 @@ struct SpreadAtom {
   static constexpr int LV = MODE == 0 ? 2 : 3;
   real th[3][LV][5];
   real f[MODE == 0 ? 6 : (MODE == 1 ? 10 : 13)];
-  int off[3];
+  int off[3];   // support base relative to the tile origin
+  int mlo[3];   // first support index inside the tile, per axis
+  int cnt[3];   // support points inside the tile, per axis (0: none)
 };
 
-// offset of grid index l from the atom's support base, wrapped; valid if < 5
-__device__ __forceinline__ int support_m(int l, int off, int K) {
-  int m = l - off;
-  if (m < 0) m += K;
-  return m;
+// Support indices m (0..4) of an atom whose base sits at `off` from the tile
+// origin that land inside the tile [0, T), as a contiguous range: the point
+// index is off + m, wrapped once by K.
+__device__ __forceinline__ void support_range(int off, int T, int K, int& mlo, int& cnt) {
+  if (T == K) {  // one tile spans the axis: the whole (wrapped) support is inside
+    mlo = 0;
+    cnt = 5;
+    return;
+  }
+  // several tiles: T divides K and T >= 4, so K >= T + 4 and the in-tile
+  // points form one contiguous run of m
+  int a, b;
+  if (off >= K - 4) {  // base at the far end of the grid: the support wraps to 0..
+    a = K - off;
+    b = 4;
+  } else if (off >= 0) {
+    a = 0;
+    b = min(4, T - 1 - off);
+  } else {  // base in a lower neighbour tile
+    a = -off;
+    b = 4;
+  }
+  mlo = a;
+  cnt = b >= a ? b - a + 1 : 0;
+}
+
+__device__ __forceinline__ int support_l(int off, int m, int K) {
+  int l = off + m;
+  if (l >= K) l -= K;
+  return l;
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
+  return (unsigned long long)(long long)rint(v * scale);
 }
 
 template <class real, int MODE>
@@ -176,17 +192,19 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
   constexpr int LV = SA::LV;
   constexpr int NACC = MODE == 1 ? 1 : 2;
   extern __shared__ __align__(16) char smem[];
-  SA* atoms = reinterpret_cast<SA*>(smem);
-  __shared__ int bin_start[8], bin_count[8], bin_pref[9];
   const int T1 = g.T[0], T2 = g.T[1], T3 = g.T[2];
+  // padded pitches spread the rows of one atom's support over LDS banks
+  const int P1 = T1 + 1, P2 = T2 + 1;
+  const int nacc = P1 * P2 * T3;
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);  // [NACC][nacc]
+  int* pref = reinterpret_cast<int*>(smem + sizeof(unsigned long long) * NACC * nacc);  // [chunk + 1]
+  SA* atoms = reinterpret_cast<SA*>(smem + sizeof(unsigned long long) * NACC * nacc +
+                                    sizeof(int) * ((chunk + 2) & ~1));
+  __shared__ int bin_start[8], bin_count[8], bin_pref[9];
+  __shared__ int wsum[4];
   const int tt = blockIdx.x;
   const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
   const int lo[3] = {t1 * T1, t2 * T2, t3 * T3};
-  const int rows = T2 * T3;
-  const int Q = 256 / rows;  // host guarantees rows <= 256
-  const int row = threadIdx.x % rows, slice = threadIdx.x / rows;
-  const bool worker = slice < Q;
-  const int l2 = row % T2, l3 = row / T2;
   if (threadIdx.x < 8) {
     const int c = threadIdx.x;
     const int da = c & 1, db = (c >> 1) & 1, dc = c >> 2;
@@ -203,21 +221,20 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
     bin_start[c] = st;
     bin_count[c] = ct;
   }
+  for (int q = threadIdx.x; q < NACC * nacc; q += blockDim.x) acc[q] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     bin_pref[0] = 0;
     for (int c = 0; c < 8; ++c) bin_pref[c + 1] = bin_pref[c] + bin_count[c];
   }
   __syncthreads();
-  real acc[NACC][kMaxTile];
-#pragma unroll
-  for (int q = 0; q < NACC; ++q)
-#pragma unroll
-    for (int l = 0; l < kMaxTile; ++l) acc[q][l] = 0;
+  const double scale = RealTraits<real>::fixed_scale;
   const int total = bin_pref[8];
   for (int base = 0; base < total; base += chunk) {
     const int nchunk = min(chunk, total - base);
-    // ---- phase 1: stage weights
+    // ---- phase 1: stage weights and the in-tile support ranges; count the
+    // atom's (y, z) rows inside the tile
+    int nrow = 0;
     for (int a = threadIdx.x; a < nchunk; a += blockDim.x) {
       const int idx = base + a;
       int c = 0;
@@ -228,6 +245,10 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
       A.off[0] = b.x - lo[0];
       A.off[1] = b.y - lo[1];
       A.off[2] = b.z - lo[2];
+      support_range(A.off[0], T1, g.K[0], A.mlo[0], A.cnt[0]);
+      support_range(A.off[1], T2, g.K[1], A.mlo[1], A.cnt[1]);
+      support_range(A.off[2], T3, g.K[2], A.mlo[2], A.cnt[2]);
+      nrow = A.cnt[0] ? A.cnt[1] * A.cnt[2] : 0;
       const R4<real> w = pw[s];
       real th[LV][5];
       bspline5<LV>(w.x, th);
@@ -271,81 +292,83 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
         }
       }
     }
-    __syncthreads();
-    // ---- phase 2: this thread's row, its slice of the staged atoms
-    if (worker) {
-      for (int a = slice; a < nchunk; a += Q) {
-        const SA& A = atoms[a];
-        const int m2 = support_m(lo[1] + l2, A.off[1] + lo[1], g.K[1]);
-        const int m3 = support_m(lo[2] + l3, A.off[2] + lo[2], g.K[2]);
-        if (m2 >= 5 || m3 >= 5) continue;
-        const real u0 = A.th[1][0][m2], u1 = A.th[1][1][m2];
-        const real v0 = A.th[2][0][m3], v1 = A.th[2][1][m3];
-        const int offx = A.off[0] + lo[0];
-        if (MODE == 0) {
-          const real Ad = A.f[0] * u0 * v0, Bd = A.f[1] * u1 * v0 + A.f[2] * u0 * v1;
-          const real Ap = A.f[3] * u0 * v0, Bp = A.f[4] * u1 * v0 + A.f[5] * u0 * v1;
+    // exclusive scan of the row counts (chunk <= blockDim: one atom per thread)
+    {
+      int v = nrow;
+      const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-          for (int l1 = 0; l1 < kMaxTile; ++l1) {
-            if (l1 >= T1) break;
-            const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
-            if (m1 >= 5) continue;
-            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1];
-            acc[0][l1] += Ad * t1v + Bd * t0;
-            acc[NACC - 1][l1] += Ap * t1v + Bp * t0;
-          }
-        } else {
-          const real u2 = A.th[1][LV > 2 ? 2 : 0][m2], v2 = A.th[2][LV > 2 ? 2 : 0][m3];
-          const real* f = A.f;
-          const real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
-                          f[9] * u1 * v1;
-          const real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
-          const real R2 = f[4] * u0 * v0;
-#pragma unroll
-          for (int l1 = 0; l1 < kMaxTile; ++l1) {
-            if (l1 >= T1) break;
-            const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
-            if (m1 >= 5) continue;
-            acc[0][l1] += R0 * A.th[0][0][m1] + R1 * A.th[0][1][m1] + R2 * A.th[0][LV > 2 ? 2 : 0][m1];
-          }
-          if (MODE == 2) {
-            const real Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
-            const real Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
-#pragma unroll
-            for (int l1 = 0; l1 < kMaxTile; ++l1) {
-              if (l1 >= T1) break;
-              const int m1 = support_m(lo[0] + l1, offx, g.K[0]);
-              if (m1 >= 5) continue;
-              acc[NACC - 1][l1] += Ap * A.th[0][1][m1] + Bp * A.th[0][0][m1];
-            }
-          }
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane == 63) wsum[wv] = v;
+      __syncthreads();
+      int add = 0;
+      for (int q = 0; q < wv; ++q) add += wsum[q];
+      if ((int)threadIdx.x < nchunk) pref[threadIdx.x + 1] = v + add;
+      if (threadIdx.x == 0) pref[0] = 0;
+      __syncthreads();
+    }
+    // ---- phase 2: one item per in-tile row of a staged atom
+    const int nitem = pref[nchunk];
+    for (int it = threadIdx.x; it < nitem; it += blockDim.x) {
+      int lo_a = 0, hi_a = nchunk - 1;  // last atom with pref[a] <= it
+      while (lo_a < hi_a) {
+        const int mid = (lo_a + hi_a + 1) >> 1;
+        if (pref[mid] <= it) lo_a = mid;
+        else hi_a = mid - 1;
+      }
+      const SA& A = atoms[lo_a];
+      const int q = it - pref[lo_a];
+      const int m2 = A.mlo[1] + q % A.cnt[1], m3 = A.mlo[2] + q / A.cnt[1];
+      const int ly = support_l(A.off[1], m2, g.K[1]);
+      const int lz = support_l(A.off[2], m3, g.K[2]);
+      const real u0 = A.th[1][0][m2], u1 = A.th[1][1][m2];
+      const real v0 = A.th[2][0][m3], v1 = A.th[2][1][m3];
+      const int rowbase = P1 * (ly + P2 * lz);
+      const int m1a = A.mlo[0], m1b = A.mlo[0] + A.cnt[0];
+      if (MODE == 0) {
+        const real Ad = A.f[0] * u0 * v0, Bd = A.f[1] * u1 * v0 + A.f[2] * u0 * v1;
+        const real Ap = A.f[3] * u0 * v0, Bp = A.f[4] * u1 * v0 + A.f[5] * u0 * v1;
+        for (int m1 = m1a; m1 < m1b; ++m1) {
+          const int lx = support_l(A.off[0], m1, g.K[0]);
+          const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1];
+          atomicAdd(&acc[rowbase + lx], to_fixed((double)(Ad * t1v + Bd * t0), scale));
+          atomicAdd(&acc[nacc + rowbase + lx], to_fixed((double)(Ap * t1v + Bp * t0), scale));
+        }
+      } else {
+        const real u2 = A.th[1][LV > 2 ? 2 : 0][m2], v2 = A.th[2][LV > 2 ? 2 : 0][m3];
+        const real* f = A.f;
+        const real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
+                        f[9] * u1 * v1;
+        const real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+        const real R2 = f[4] * u0 * v0;
+        real Ap = 0, Bp = 0;
+        if (MODE == 2) {
+          Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
+          Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
+        }
+        for (int m1 = m1a; m1 < m1b; ++m1) {
+          const int lx = support_l(A.off[0], m1, g.K[0]);
+          const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1], t2v = A.th[0][LV > 2 ? 2 : 0][m1];
+          atomicAdd(&acc[rowbase + lx], to_fixed((double)(R0 * t0 + R1 * t1v + R2 * t2v), scale));
+          if (MODE == 2) atomicAdd(&acc[(NACC - 1) * nacc + rowbase + lx], to_fixed((double)(Ap * t1v + Bp * t0), scale));
         }
       }
     }
     __syncthreads();
   }
-  // ---- combine the Q slices of each row in slice order
-  real* part = reinterpret_cast<real*>(smem);  // [slice][row][NACC][T1], reuses the staging area
-  if (worker)
-#pragma unroll
-    for (int q = 0; q < NACC; ++q)
-#pragma unroll
-      for (int l1 = 0; l1 < kMaxTile; ++l1)
-        if (l1 < T1) part[((slice * rows + row) * NACC + q) * T1 + l1] = acc[q][l1];
-  __syncthreads();
-  const int npt = T1 * rows;
+  // ---- convert and write the tile once
+  const double inv = 1.0 / scale;
+  const int npt = T1 * T2 * T3;
   for (int pnt = threadIdx.x; pnt < npt; pnt += blockDim.x) {
     const int l1 = pnt % T1, r = pnt / T1;
-    real sd = 0, sp = 0;
-    for (int q = 0; q < Q; ++q) {
-      sd += part[((q * rows + r) * NACC + 0) * T1 + l1];
-      if (NACC == 2) sp += part[((q * rows + r) * NACC + NACC - 1) * T1 + l1];
-    }
     const int rl2 = r % T2, rl3 = r / T2;
+    const int ai = l1 + P1 * (rl2 + P2 * rl3);
     const long long gi = (lo[0] + l1) + (long long)g.K[0] * ((lo[1] + rl2) + (long long)g.K[1] * (lo[2] + rl3));
     cplx_t<real> v;
-    v.x = sd;
-    v.y = NACC == 2 ? sp : (real)0;
+    v.x = (real)((double)(long long)acc[ai] * inv);
+    v.y = NACC == 2 ? (real)((double)(long long)acc[(NACC - 1) * nacc + ai] * inv) : (real)0;
     grid[gi] = v;
   }
 }
@@ -478,9 +501,7 @@ void pme_bin_t(Ctx& c) {
   hipLaunchKernelGGL(k_tile_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.pme_base.as<int4>(),
                      c.tile_start.as<int>(), c.tile_cursor.as<int>(), c.tile_atoms.as<int>());
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_tile_sort, dim3(ceil_div(ntot, 256)), dim3(256), 0, c.stream, ntot, c.tile_start.as<int>(),
-                     c.tile_count.as<int>(), c.tile_atoms.as<int>());
-  T9_LAUNCH_CHECK();
+  // bins stay in arrival order: the fixed-point tile sums do not depend on it
   c.bins_valid = true;
 }
 
@@ -491,9 +512,12 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   if (rows > 256 || c.tsz[0] > kMaxTile) throw ArgError("spreading tile too large");
   const size_t per = mode == 0 ? sizeof(SpreadAtom<real, 0>)
                                : (mode == 1 ? sizeof(SpreadAtom<real, 1>) : sizeof(SpreadAtom<real, 2>));
-  const int chunk = (int)std::min<size_t>(kSpreadChunk, 65536 / per);
-  const size_t part = sizeof(real) * (size_t)256 * (mode == 1 ? 1 : 2) * c.tsz[0];
-  const size_t sh = std::max(per * chunk, part);
+  const size_t accb =
+      sizeof(unsigned long long) * (mode == 1 ? 1 : 2) * (size_t)(c.tsz[0] + 1) * (c.tsz[1] + 1) * c.tsz[2];
+  const size_t budget = 65536 - 1024;  // dynamic LDS next to the kernel's static tables
+  if (accb + 32 * (per + sizeof(int)) > budget) throw ArgError("spreading tile too large for LDS");
+  const int chunk = (int)std::min<size_t>(kSpreadChunk, (budget - accb - 2 * sizeof(int)) / (per + sizeof(int)));
+  const size_t sh = accb + sizeof(int) * ((chunk + 2) & ~1) + per * chunk;
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   auto kern = mode == 0 ? k_spread<real, 0> : (mode == 1 ? k_spread<real, 1> : k_spread<real, 2>);
   KScope ks_(c, KT_SPREAD);
@@ -608,13 +632,14 @@ std::vector<double> pme_bsmod(int K, int order) {
   return mod;
 }
 
-// Tile edge dividing K, closest to 8, between 4 and 16 (K <= 16: one tile).
+// Tile edge dividing K, closest to 8, between 4 and 12 (K <= 12: one tile);
+// the fixed-point copy of a tile must fit in LDS next to the staged atoms.
 int pme_tile_edge(int K) {
-  if (K <= 16) return K;
+  if (K <= 12) return K;
   int best = -1;
-  for (int t = 4; t <= 16; ++t)
+  for (int t = 4; t <= 12; ++t)
     if (K % t == 0 && (best < 0 || std::abs(t - 8) < std::abs(best - 8))) best = t;
-  if (best < 0) throw ArgError("PME grid size " + std::to_string(K) + " has no divisor between 4 and 16");
+  if (best < 0) throw ArgError("PME grid size " + std::to_string(K) + " has no divisor between 4 and 12");
   return best;
 }
 

@@ -84,6 +84,12 @@ int t9_system_get_f64(const t9_system* sys, const char* name, double* out) {
     else if (k == "vdw_rad") v = &s.vdw_rad;
     else if (k == "vdw_eps") v = &s.vdw_eps;
     else if (k == "vdw_red") v = &s.vdw_red;
+    else if (k == "vdw_rmin") v = &s.vdw_rmin;
+    else if (k == "vdw_epsmix") v = &s.vdw_epsmix;
+    else if (k == "vscale_s") v = &s.vscale_s;
+    else if (k == "bond_kr") v = &s.bond_kr;
+    else if (k == "angle_kt") v = &s.angle_kt;
+    else if (k == "urey_kr") v = &s.urey_kr;
     else if (k == "box") {
       std::memcpy(out, s.box, 3 * sizeof(double));
       return T9_OK;
@@ -108,6 +114,11 @@ int t9_system_get_i32(const t9_system* sys, const char* name, int* out) {
     else if (k == "ysign") v = &s.ysign;
     else if (k == "jpolar") v = &s.jpolar;
     else if (k == "vdw_parent") v = &s.vdw_parent;
+    else if (k == "jvdw") v = &s.jvdw;
+    else if (k == "vscale_ik") v = &s.vscale_ik;
+    else if (k == "bond_ij") v = &s.bond_ij;
+    else if (k == "angle_abc") v = &s.angle_abc;
+    else if (k == "urey_ac") v = &s.urey_ac;
     else if (k == "i12") {
       for (int i = 0; i < s.n; ++i)
         for (int j = 0; j < 4; ++j) out[4 * i + j] = j < (int)s.i12[i].size() ? s.i12[i][j] : -1;
@@ -148,6 +159,20 @@ int t9_system_get_scalar(const t9_system* sys, const char* name, double* out) {
     else if (k == "use_polar") v = o.use_polar;
     else if (k == "mpole_cut") v = o.mpole_cut;
     else if (k == "vdw_cut") v = o.vdw_cut;
+    else if (k == "vdw_taper") v = o.vdw_taper;
+    else if (k == "use_vdw") v = o.use_vdw;
+    else if (k == "use_valence") v = o.use_valence;
+    else if (k == "njvdw") v = s.njvdw;
+    else if (k == "nvscale") v = (double)s.vscale_s.size();
+    else if (k == "nbond") v = (double)s.bond_kr.size() / 2;
+    else if (k == "nangle") v = (double)s.angle_kt.size() / 2;
+    else if (k == "nurey") v = (double)s.urey_kr.size() / 2;
+    else if (k == "bond_cubic") v = s.ff.bond_cubic;
+    else if (k == "bond_quartic") v = s.ff.bond_quartic;
+    else if (k == "angle_cubic") v = s.ff.angle_cubic;
+    else if (k == "angle_quartic") v = s.ff.angle_quartic;
+    else if (k == "angle_pentic") v = s.ff.angle_pentic;
+    else if (k == "angle_sextic") v = s.ff.angle_sextic;
     else throw t9::ArgError("unknown scalar " + k);
     *out = v;
     return T9_OK;

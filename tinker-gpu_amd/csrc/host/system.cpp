@@ -255,7 +255,7 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
   std::istringstream in(text);
   std::string line;
   Options& o = s.opt;
-  bool mpole_only = false, polar_only = false;
+  bool mpole_only = false, polar_only = false, vdw_only = false, valence_only = false;
   while (std::getline(in, line)) {
     auto tok = split(line);
     if (tok.empty()) continue;
@@ -321,9 +321,18 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
       if (v1 == "only") polar_only = true;
       else o.use_polar = (v1 != "none");
     } else if (k == "vdwterm") {
-      o.use_vdw = (v1 != "none");
+      if (v1 == "only") vdw_only = true;
+      else o.use_vdw = (v1 != "none");
+    } else if (k == "bondterm" || k == "angleterm" || k == "ureybradterm") {
+      if (v1 == "only") valence_only = true;
+      else o.use_valence = (v1 != "none");
+    } else if (k == "vdw-taper") {
+      const double v = to_d(tok[1]);
+      o.vdw_taper = v <= 1 ? v * o.vdw_cut : v;
     }
   }
+  if (vdw_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = true; o.use_valence = false; }
+  if (valence_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = false; o.use_valence = true; }
   if (mpole_only) { o.use_mpole = true; o.use_polar = false; o.use_vdw = false; o.use_valence = false; }
   if (polar_only) { o.use_mpole = false; o.use_polar = true; o.use_vdw = false; o.use_valence = false; }
 }
@@ -585,6 +594,96 @@ void assign_vdw(System& s) {
   }
 }
 
+// vdW tables: one index per distinct (radius, epsilon) atom class, pair
+// R_min by the cubic-mean rule on R-min diameters and epsilon by the HHG rule
+// (amoeba09.prm: radiusrule CUBIC-MEAN, radiustype R-MIN, radiussize
+// DIAMETER, epsilonrule HHG).  vdW pair scales from 1-2 .. 1-5 connectivity.
+void build_vdw_tables(System& s) {
+  const int n = s.n;
+  std::map<std::pair<double, double>, int> idx;
+  s.jvdw.assign(n, 0);
+  std::vector<std::pair<double, double>> cls;
+  for (int i = 0; i < n; ++i) {
+    auto key = std::make_pair(s.vdw_rad[i], s.vdw_eps[i]);
+    auto it = idx.find(key);
+    if (it == idx.end()) {
+      it = idx.emplace(key, (int)cls.size()).first;
+      cls.push_back(key);
+    }
+    s.jvdw[i] = it->second;
+  }
+  const int m = (int)cls.size();
+  s.njvdw = m;
+  s.vdw_rmin.assign((size_t)m * m, 0.0);
+  s.vdw_epsmix.assign((size_t)m * m, 0.0);
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      const double ra = cls[a].first, rb = cls[b].first, ea = cls[a].second, eb = cls[b].second;
+      const double r2 = ra * ra + rb * rb;
+      s.vdw_rmin[a * m + b] = r2 > 0 ? (ra * ra * ra + rb * rb * rb) / r2 : 0.0;
+      const double se = std::sqrt(ea) + std::sqrt(eb);
+      s.vdw_epsmix[a * m + b] = se > 0 ? 4 * ea * eb / (se * se) : 0.0;
+    }
+  s.vscale_ik.clear();
+  s.vscale_s.clear();
+  for (int i = 0; i < n; ++i) {
+    const std::vector<int>* cpl[6] = {nullptr, nullptr, &s.i12[i], &s.i13[i], &s.i14[i], &s.i15[i]};
+    std::map<int, double> row;
+    for (int b = 2; b <= 5; ++b)
+      for (int k : *cpl[b])
+        if (k > i && s.ff.v_scale[b] != 1 && !row.count(k)) row[k] = s.ff.v_scale[b];
+    for (auto& kv : row) {
+      s.vscale_ik.push_back(i);
+      s.vscale_ik.push_back(kv.first);
+      s.vscale_s.push_back(kv.second);
+    }
+  }
+}
+
+// Bonds from the 1-2 lists, angles and Urey-Bradley terms from pairs of
+// neighbours of each vertex, parameters by atom classes.
+void build_valence(System& s) {
+  const int n = s.n;
+  const ForceField& ff = s.ff;
+  s.bond_ij.clear();
+  s.bond_kr.clear();
+  s.angle_abc.clear();
+  s.angle_kt.clear();
+  s.urey_ac.clear();
+  s.urey_kr.clear();
+  for (int i = 0; i < n; ++i)
+    for (int j : s.i12[i]) {
+      if (j <= i) continue;
+      auto it = ff.bond.find({std::min(s.cls[i], s.cls[j]), std::max(s.cls[i], s.cls[j])});
+      if (it == ff.bond.end()) {
+        if (s.opt.use_valence) throw std::runtime_error("no bond parameters for atoms " + std::to_string(i + 1));
+        continue;
+      }
+      s.bond_ij.insert(s.bond_ij.end(), {i, j});
+      s.bond_kr.insert(s.bond_kr.end(), {it->second.k, it->second.r0});
+    }
+  for (int b = 0; b < n; ++b) {
+    const auto& nb = s.i12[b];
+    for (size_t p = 0; p < nb.size(); ++p)
+      for (size_t q = p + 1; q < nb.size(); ++q) {
+        const int a = nb[p], c = nb[q];
+        const auto key = std::make_tuple(std::min(s.cls[a], s.cls[c]), s.cls[b], std::max(s.cls[a], s.cls[c]));
+        auto it = ff.angle.find(key);
+        if (it != ff.angle.end()) {
+          s.angle_abc.insert(s.angle_abc.end(), {a, b, c});
+          s.angle_kt.insert(s.angle_kt.end(), {it->second.k, it->second.t0});
+        } else if (s.opt.use_valence) {
+          throw std::runtime_error("no angle parameters at atom " + std::to_string(b + 1));
+        }
+        auto iu = ff.urey.find(key);
+        if (iu != ff.urey.end()) {
+          s.urey_ac.insert(s.urey_ac.end(), {a, c});
+          s.urey_kr.insert(s.urey_kr.end(), {iu->second.k, iu->second.r0});
+        }
+      }
+  }
+}
+
 }  // namespace
 
 void setup_system(System& s) {
@@ -606,7 +705,10 @@ void setup_system(System& s) {
   assign_polarize(s);
   build_scales(s);
   assign_vdw(s);
+  build_vdw_tables(s);
+  build_valence(s);
   Options& o = s.opt;
+  if (o.vdw_taper <= 0) o.vdw_taper = 0.9 * o.vdw_cut;
   if (o.ewald) {
     if (!s.periodic) throw std::runtime_error("ewald requires a periodic box");
     if (o.ewald_alpha <= 0) o.ewald_alpha = ewald_coeff(o.ewald_cut);

@@ -105,6 +105,7 @@ struct Options {
   double udiag = 2.0;
   double list_buffer = 2.0;
   double vdw_cut = 9.0;
+  double vdw_taper = 0;        // 0: 0.9 * vdw_cut (the reference's vdwtaper default)
   bool mutual = true;
 };
 
@@ -138,6 +139,20 @@ struct System {
   // vdW, per atom
   std::vector<double> vdw_rad, vdw_eps, vdw_red;
   std::vector<int> vdw_parent;
+  // vdW pair parameters by vdW class index (cubic-mean R_min, HHG epsilon)
+  std::vector<int> jvdw;          // [n] index into the tables
+  int njvdw = 0;
+  std::vector<double> vdw_rmin, vdw_epsmix;  // [njvdw][njvdw]
+  std::vector<int> vscale_ik;     // [nvscale][2] pairs (i < k) with vdW scale != 1
+  std::vector<double> vscale_s;   // [nvscale]
+
+  // valence terms (0-based atoms)
+  std::vector<int> bond_ij;       // [nbond][2]
+  std::vector<double> bond_kr;    // [nbond][2] force constant, ideal length
+  std::vector<int> angle_abc;     // [nangle][3], b the vertex
+  std::vector<double> angle_kt;   // [nangle][2] force constant, ideal angle (degrees)
+  std::vector<int> urey_ac;       // [nurey][2]
+  std::vector<double> urey_kr;    // [nurey][2]
 
   std::vector<ScalePair> scales;  // sorted by (i, k)
   Options opt;

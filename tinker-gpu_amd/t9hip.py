@@ -183,6 +183,23 @@ class System:
                   "use_polar"):
             d[k] = self.scalar(k)
         d["grid"] = np.array([int(self.scalar("grid%d" % j)) for j in (1, 2, 3)], dtype=np.int32)
+        # MD terms: vdW tables and valence lists
+        nj = int(self.scalar("njvdw"))
+        nvs, nb, na, nu = (int(self.scalar(k)) for k in ("nvscale", "nbond", "nangle", "nurey"))
+        d.update(jvdw=self.i32("jvdw", n), njvdw=nj, vdw_rmin=self.f64("vdw_rmin", nj * nj),
+                 vdw_epsmix=self.f64("vdw_epsmix", nj * nj), vdw_red=self.f64("vdw_red", n),
+                 vdw_parent=self.i32("vdw_parent", n), mass=self.f64("mass", n),
+                 vscale_ik=self.i32("vscale_ik", (nvs, 2)) if nvs else np.zeros((0, 2), np.int32),
+                 vscale_s=self.f64("vscale_s", nvs) if nvs else np.zeros(0),
+                 bond_ij=self.i32("bond_ij", (nb, 2)) if nb else np.zeros((0, 2), np.int32),
+                 bond_kr=self.f64("bond_kr", (nb, 2)) if nb else np.zeros((0, 2)),
+                 angle_abc=self.i32("angle_abc", (na, 3)) if na else np.zeros((0, 3), np.int32),
+                 angle_kt=self.f64("angle_kt", (na, 2)) if na else np.zeros((0, 2)),
+                 urey_ac=self.i32("urey_ac", (nu, 2)) if nu else np.zeros((0, 2), np.int32),
+                 urey_kr=self.f64("urey_kr", (nu, 2)) if nu else np.zeros((0, 2)))
+        for k in ("vdw_cut", "vdw_taper", "use_vdw", "use_valence", "bond_cubic", "bond_quartic", "angle_cubic",
+                  "angle_quartic", "angle_pentic", "angle_sextic"):
+            d[k] = self.scalar(k)
         return d
 
 
