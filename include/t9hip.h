@@ -131,24 +131,58 @@ int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats);
 int t9_get_uind(t9_ctx* ctx, double* uind, double* uinp);
 int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
 
-/* Energy, gradient and virial of the multipole and polarization terms (the
- * terms enabled by the system's keywords).  Solves for the induced dipoles
- * first, as the reference's emplar does.  vers is a bit set: T9_CALC_ENERGY,
- * T9_CALC_GRAD, T9_CALC_VIRIAL (the reference's calc::energy/grad/virial,
- * include/tool/rcman.h:105-155).  grad receives dE/dx [n][3] in kcal/mol/Å in
- * caller order when non-NULL.  Replaces energy(vers) for the AMOEBA
- * electrostatics terms (src/energy.cpp:319-450 -> emplar(vers),
+/* Energy, gradient and virial of the terms enabled by the system's keywords:
+ * the multipole and polarization terms and, when enabled, the buffered 14-7
+ * vdW and the bond / angle / Urey-Bradley valence terms.  Solves for the
+ * induced dipoles first, as the reference's emplar does.  vers is a bit set:
+ * T9_CALC_ENERGY, T9_CALC_GRAD, T9_CALC_VIRIAL (the reference's
+ * calc::energy/grad/virial, include/tool/rcman.h:105-155).  grad receives
+ * dE/dx [n][3] in kcal/mol/Å in caller order when non-NULL.  Replaces
+ * energy(vers) (src/energy.cpp:319-450) for these terms: emplar(vers),
  * src/amoeba/emplar.cpp:10-28 and src/cu/amoeba/emplar.cu:546-605, with
- * torque(vers, gx, gy, gz), src/cu/amoeba/torque.cu:9-380). */
+ * torque(vers, gx, gy, gz), src/cu/amoeba/torque.cu:9-380; evdw(vers) for
+ * AMOEBA (src/evdw.cpp, src/cu/evdw.cu ehal path); ebond / eangle / eurey
+ * (src/cu/evalence.cu). */
 enum { T9_CALC_ENERGY = 1, T9_CALC_GRAD = 2, T9_CALC_VIRIAL = 4 };
 typedef struct {
-  double esum;          /* em + ep, kcal/mol */
+  double esum;          /* sum of the terms below, kcal/mol */
   double em, ep;        /* permanent multipole, polarization */
   double em_real, em_recip, em_self;  /* parts of em (recip/self zero without Ewald) */
-  double vir[9];        /* dE/d(strain), kcal/mol, symmetric */
+  double vir[9];        /* dE/d(strain), kcal/mol, symmetric, all terms */
   int induce_iterations;
+  double evdw;          /* buffered 14-7 vdW */
+  double ebond, eangle, eurey;  /* valence */
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
+
+/* ------------------------------------------------------------ dynamics */
+/* Molecular dynamics on the device, caller-order FP64 positions and
+ * velocities (Å, Å/ps), masses from the system (g/mol).
+ * t9_md_init draws Maxwell-Boltzmann velocities at `kelvin` from `seed`
+ * (net momentum removed) and seeds the thermostat stream; it replaces the
+ * velocity setup of the reference's Fortran mdinit (not in the reference
+ * tree). */
+int t9_md_init(t9_ctx* ctx, double kelvin, uint64_t seed);
+int t9_md_set_velocities(t9_ctx* ctx, const double* vel);
+int t9_md_get_velocities(t9_ctx* ctx, double* vel);
+typedef struct {
+  double epot;          /* potential energy after the last step, kcal/mol */
+  double ekin;          /* kinetic energy after the last step, kcal/mol */
+  double temperature;   /* 2 ekin / (nfree R), nfree = 3n - 3 */
+  int64_t induce_iterations;  /* CG iterations summed over the run */
+  int list_builds;      /* neighbor-list rebuilds during the run */
+  int steps;
+} t9_md_stats;
+/* nsteps RESPA steps of outer length dt (ps) with nrespa inner steps: the
+ * slow group (multipole, polarization, vdW) kicks at the outer half steps,
+ * the fast group (valence) at the inner steps; nrespa = 1 is velocity
+ * Verlet.  bath_kelvin > 0 applies the Bussi stochastic velocity rescaling
+ * with relaxation time tau (ps) after every step; bath_kelvin <= 0 is NVE.
+ * Replaces BasicIntegrator::dynamic (src/md/integrator.cpp:70-165) with
+ * RespaDevice::velR0/velR1/velR2 (src/md/propagator.cpp:174-187) and
+ * BussiThermostat::control2 -> bussiThermostat (src/md/thermostat.cpp:77-82,
+ * src/mdpt.cpp:41-71). */
+int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin, double tau, t9_md_stats* stats);
 
 /* Wait for all work queued on the context's stream. */
 int t9_sync(t9_ctx* ctx);
@@ -159,8 +193,10 @@ int t9_sync(t9_ctx* ctx);
  * the context's stream: mode 1 clears and starts, 0 stops, 2 reads.  Ids:
  * 0 ufield real-space pairs, 1 dfield real-space pairs, 2 PME spread,
  * 3 FFT passes, 4 PME gathers, 5 preconditioner, 6 PCG vector kernels,
- * 7 fused multipole + polarization real-space energy/gradient kernel. */
-enum { T9_K_UFIELD = 0, T9_K_DFIELD, T9_K_SPREAD, T9_K_FFT, T9_K_GATHER, T9_K_PRECOND, T9_K_PCG_VEC, T9_K_EMPLAR };
+ * 7 fused multipole + polarization real-space energy/gradient kernel,
+ * 8 vdW pair kernel, 9 valence kernel. */
+enum { T9_K_UFIELD = 0, T9_K_DFIELD, T9_K_SPREAD, T9_K_FFT, T9_K_GATHER, T9_K_PRECOND, T9_K_PCG_VEC, T9_K_EMPLAR,
+       T9_K_VDW, T9_K_VALENCE };
 int t9_kernel_timing(t9_ctx* ctx, int kernel, int mode, double* total_ms, int64_t* launches);
 /* Unique atom pairs within the real-space cutoff on the current coordinates
  * (the algorithmic unit of the pair kernels). */

@@ -65,9 +65,9 @@ void alloc_vectors(Ctx& c) {
     T9_HIP(hipMemsetAsync(b->p, 0, vb, c.stream));
   }
   c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
-  c.scalars.alloc(sizeof(double) * 32);
-  T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 32, c.stream));
-  T9_HIP(hipHostMalloc((void**)&c.h_scalars, sizeof(double) * 32, hipHostMallocDefault));
+  c.scalars.alloc(sizeof(double) * 48);
+  T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 48, c.stream));
+  T9_HIP(hipHostMalloc((void**)&c.h_scalars, sizeof(double) * 48, hipHostMallocDefault));
 }
 
 }  // namespace
@@ -196,12 +196,32 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   c.ulist.cutoff = sparse ? std::min(s.opt.usolve_cut + buf, c.mlist.cutoff) : 0;
   c.mlist.cut = c.cut < 1e20 ? c.cut : 1e30;
   c.ulist.cut = s.opt.usolve_cut;
+  // vdW sites sit off their atoms by (1 - red) times the bond to the parent;
+  // the cell search around the atom's cell must cover twice that offset
+  double site_slack = 0;
+  if (s.opt.use_vdw) {
+    c.vlist.cutoff = s.opt.vdw_cut + buf;
+    c.vlist.cut = s.opt.vdw_cut;
+    double off = 0;
+    for (int i = 0; i < n; ++i) {
+      const int p = s.vdw_parent[i];
+      if (p == i) continue;
+      double r2 = 0;
+      for (int d = 0; d < 3; ++d) {
+        double x = s.xyz[3 * i + d] - s.xyz[3 * p + d];
+        if (c.periodic) x -= c.box[d] * std::nearbyint(x / c.box[d]);
+        r2 += x * x;
+      }
+      off = std::max(off, (1.0 - s.vdw_red[i]) * std::sqrt(r2));
+    }
+    site_slack = 2.0 * (1.5 * off + 0.1);
+  }
   c.brute = true;
   c.morton_bits = 0;
   if (c.periodic) {
-    const double rl = c.mlist.cutoff;
+    const double rl = std::max(c.mlist.cutoff, c.vlist.cutoff + site_slack);
     for (int d = 0; d < 3; ++d)
-      if (rl > 0.5 * c.box[d] + 1e-9)
+      if (std::max(c.mlist.cutoff, c.vlist.cutoff) > 0.5 * c.box[d] + 1e-9)
         throw ArgError("cutoff plus list buffer exceeds half the box edge");
     bool ok = true;
     int mx = 1;
@@ -258,6 +278,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   if (s.opt.predictor == "ASPC") c.ualt.alloc(rs * 6 * (size_t)n * 16);
   else if (s.opt.predictor != "NONE") throw ArgError("polar-predict " + s.opt.predictor + " is not supported");
   energy_setup(c, s);
+  md_setup(c, s);
   T9_HIP(hipStreamSynchronize(st));
 }
 
@@ -380,6 +401,41 @@ void rpole_to_host(Ctx& c, double* out) {
   T9_HIP(hipStreamSynchronize(c.stream));
 }
 
+// One evaluation on the current coordinates: list refresh, induced dipoles,
+// then the requested term groups.  Returns the CG iteration count.
+int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad_host) {
+  int iters = 0;
+  if (terms & (TERM_ELEC | TERM_VDW)) {
+    c.mlist.cut_valid = c.ulist.cut_valid = false;
+    c.bins_valid = false;
+    c.rpole_valid = false;
+    prepare(c);
+    if ((terms & TERM_ELEC) && c.opt.use_polar) {
+      double rms = 0;
+      if (induce_pcg(c, &iters, &rms) != 0) throw NotConverged("INDUCE -- induced dipoles are not converged");
+    }
+  }
+  energy_run(c, vers, terms, e16, grad_host);
+  return iters;
+}
+
+void fill_energy_out(const double* e, int vers, int iters, t9_energy_out* out) {
+  out->em_real = e[EP_MREAL];
+  out->em_recip = e[EP_MRECIP];
+  out->em_self = e[EP_MSELF];
+  out->em = out->em_real + out->em_recip + out->em_self;
+  out->ep = e[EP_POLAR];
+  out->evdw = e[EP_EVDW];
+  out->ebond = e[EP_EBOND];
+  out->eangle = e[EP_EANGLE];
+  out->eurey = e[EP_EUREY];
+  out->esum = out->em + out->ep + out->evdw + out->ebond + out->eangle + out->eurey;
+  const double v6[6] = {e[EP_VXX], e[EP_VYY], e[EP_VZZ], e[EP_VXY], e[EP_VXZ], e[EP_VYZ]};
+  const double v9[9] = {v6[0], v6[3], v6[4], v6[3], v6[1], v6[5], v6[4], v6[5], v6[2]};
+  for (int k = 0; k < 9; ++k) out->vir[k] = (vers & T9_CALC_VIRIAL) ? v9[k] : 0.0;
+  out->induce_iterations = iters;
+}
+
 }  // namespace t9
 
 using t9::guarded;
@@ -425,6 +481,7 @@ int t9_set_coords(t9_ctx* ctx, const double* xyz) {
     c.coords_dirty = true;
     c.rpole_valid = false;
     c.bins_valid = false;
+    c.md_ready = false;
     return T9_OK;
   }, T9_ERR_HIP);
 }
@@ -569,29 +626,99 @@ int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad) {
   return guarded([&] {
     auto& c = ctx->c;
     if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL)) throw t9::ArgError("unknown vers bits");
-    // a new evaluation on the current coordinates
-    c.mlist.cut_valid = c.ulist.cut_valid = false;
-    c.bins_valid = false;
-    c.rpole_valid = false;
-    t9::prepare(c);
-    int iters = 0;
-    if (c.opt.use_polar) {
-      double rms = 0;
-      if (t9::induce_pcg(c, &iters, &rms) != 0) throw t9::NotConverged("INDUCE -- induced dipoles are not converged");
-    }
     double e[16];
-    t9::energy_run(c, vers, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
-    if (out) {
-      out->em_real = e[t9::EP_MREAL];
-      out->em_recip = e[t9::EP_MRECIP];
-      out->em_self = e[t9::EP_MSELF];
-      out->em = out->em_real + out->em_recip + out->em_self;
-      out->ep = e[t9::EP_POLAR];
-      out->esum = out->em + out->ep;
-      const double v6[6] = {e[t9::EP_VXX], e[t9::EP_VYY], e[t9::EP_VZZ], e[t9::EP_VXY], e[t9::EP_VXZ], e[t9::EP_VYZ]};
-      const double v9[9] = {v6[0], v6[3], v6[4], v6[3], v6[1], v6[5], v6[4], v6[5], v6[2]};
-      for (int k = 0; k < 9; ++k) out->vir[k] = (vers & T9_CALC_VIRIAL) ? v9[k] : 0.0;
-      out->induce_iterations = iters;
+    const int iters = t9::evaluate(c, vers, t9::TERM_ALL, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
+    if (out) t9::fill_energy_out(e, vers, iters, out);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_md_init(t9_ctx* ctx, double kelvin, uint64_t seed) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!(kelvin >= 0)) throw t9::ArgError("temperature must be >= 0");
+    t9::md_velocities(c, kelvin, seed);
+    c.md_rng.seed(seed ^ 0x9e3779b97f4a7c15ULL);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_md_set_velocities(t9_ctx* ctx, const double* v) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!v) throw t9::ArgError("null velocities");
+    T9_HIP(hipMemcpyAsync(c.vel.p, v, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_md_get_velocities(t9_ctx* ctx, double* v) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!v) throw t9::ArgError("null velocities");
+    T9_HIP(hipMemcpyAsync(v, c.vel.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin, double tau, t9_md_stats* stats) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (nsteps < 0 || !(dt > 0) || nrespa < 1) throw t9::ArgError("bad nsteps, dt or nrespa");
+    if (bath_kelvin > 0 && !(tau > 0)) throw t9::ArgError("the thermostat needs tau > 0");
+    double e[16] = {0};
+    long long iters = 0;
+    const int builds0 = c.builds;
+    if (!c.md_ready) {
+      iters += t9::evaluate(c, T9_CALC_ENERGY, t9::TERM_ALL, e, nullptr);
+      t9::md_sorted_to_orig(c, c.egrad.as<double>(), c.g_slow.as<double>());
+      c.md_ready = true;
+    }
+    const double dta = dt / nrespa;
+    double* gs = c.g_slow.as<double>();
+    double* gf = c.g_fast.as<double>();
+    for (int step = 0; step < nsteps; ++step) {
+      // velR1: half kick of the slow forces and the first inner half kick
+      t9::md_kick(c, 0.5 * dta, gf, 0.5 * dt, gs);
+      for (int k = 1; k < nrespa; ++k) {
+        t9::md_drift(c, dta);
+        if (c.use_valence) {
+          c.erows = 0;
+          t9::valence_run(c, gf);
+          t9::md_kick(c, dta, gf, 0.0, nullptr);  // velR0
+        }
+      }
+      t9::md_drift(c, dta);
+      iters += t9::evaluate(c, T9_CALC_ENERGY, t9::TERM_ALL, e, nullptr);
+      t9::md_sorted_to_orig(c, c.egrad.as<double>(), gs);
+      t9::md_kick(c, 0.5 * dta, gf, 0.5 * dt, gs);  // velR2
+      if (bath_kelvin > 0) {
+        // Bussi stochastic velocity rescaling over the full step
+        const double ek = t9::md_kinetic(c);
+        double temp = 2.0 * ek / (c.nfree * t9::kGasConst);
+        if (temp == 0) temp = 0.1;
+        const double cc = std::exp(-dt / tau);
+        const double d = (1 - cc) * (bath_kelvin / temp) / c.nfree;
+        std::normal_distribution<double> nd(0.0, 1.0);
+        const double r = nd(c.md_rng);
+        std::gamma_distribution<double> gd(0.5 * (c.nfree - 1), 2.0);
+        const double s2 = c.nfree > 1 ? gd(c.md_rng) : 0.0;
+        double sc = std::sqrt(cc + (s2 + r * r) * d + 2 * r * std::sqrt(cc * d));
+        if (r + std::sqrt(cc / d) < 0) sc = -sc;
+        t9::md_scale_v(c, sc);
+      }
+    }
+    if (stats) {
+      t9_energy_out eout;
+      t9::fill_energy_out(e, T9_CALC_ENERGY, 0, &eout);
+      stats->epot = eout.esum;
+      stats->ekin = t9::md_kinetic(c);
+      stats->temperature = 2.0 * stats->ekin / (c.nfree * t9::kGasConst);
+      stats->induce_iterations = iters;
+      stats->list_builds = c.builds - builds0;
+      stats->steps = nsteps;
     }
     return T9_OK;
   }, T9_ERR_HIP);

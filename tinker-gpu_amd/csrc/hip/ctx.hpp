@@ -14,6 +14,7 @@
 
 #include <array>
 #include <cstdint>
+#include <random>
 #include <vector>
 
 #include "common.hpp"
@@ -79,6 +80,8 @@ enum KernelId : int {
   KT_PRECOND,
   KT_PCG_VEC,
   KT_EMPLAR,
+  KT_VDW,
+  KT_VALENCE,
   KT_COUNT
 };
 
@@ -132,7 +135,7 @@ struct Ctx {
   int morton_bits = 0;
   bool brute = true;             // all-pairs search (small or non-periodic systems)
   DBuf cell_key, cell_count, cell_start, cell_cursor, unsorted;
-  NbList mlist, ulist;
+  NbList mlist, ulist, vlist;    // multipole/polar, preconditioner, vdW sites
   int builds = 0;
   bool list_valid = false;
 
@@ -169,6 +172,29 @@ struct Ctx {
   int last_predictor = 0;        // 0 direct guess, 1 ASPC
   double last_guess_rms = 0;     // RMS residual of the last initial guess (Debye)
 
+  // ---- vdW and valence (MD terms)
+  bool use_vdw = false, use_valence = false;
+  double vdw_cut = 9, vdw_taper = 8.1;
+  DBuf vpos;                     // PosRec<real> [n] vdW sites, sorted order
+  DBuf vparent, vred;            // caller order: parent atom, reduction factor (double)
+  DBuf jvdw_o, jvdw;             // vdW table index, caller / sorted order
+  int njvdw = 0;
+  DBuf vrmin, veps;              // real [njvdw][njvdw]
+  DBuf vexcl_off, vexcl_j, vexcl_code;  // CSR of vdW-scaled partners, caller order
+  double vscale[kMaxCodes] = {};
+  DBuf vchild_off, vchild;       // caller order: reduced sites hanging on each atom
+  DBuf vsite_grad;               // double [n][3] sorted
+  DBuf vterm_off, vterm;         // caller order: valence terms touching each atom
+  DBuf bond_ij, angle_abc, urey_ac;    // int
+  DBuf bond_kr, angle_kt, urey_kr;     // double
+  double vpoly[6] = {0, 0, 0, 0, 0, 0};  // bond cubic, quartic; angle cubic..sextic
+  // molecular dynamics state, caller order, FP64
+  DBuf mass, vel, g_slow, g_fast;
+  std::vector<double> mass_h;
+  bool md_ready = false;        // g_slow / g_fast hold the forces of the current coordinates
+  int nfree = 0;                // degrees of freedom of the thermostat
+  std::mt19937_64 md_rng;       // thermostat random stream
+
   // ---- state
   bool coords_dirty = true;      // coordinates changed since the sorted copy
   bool rpole_valid = false;
@@ -176,7 +202,10 @@ struct Ctx {
 };
 
 // Per-block partial rows of the energy/virial reductions.
-enum { EP_MREAL = 0, EP_MRECIP, EP_MSELF, EP_POLAR, EP_VXX, EP_VYY, EP_VZZ, EP_VXY, EP_VXZ, EP_VYZ, EP_N };
+enum {
+  EP_MREAL = 0, EP_MRECIP, EP_MSELF, EP_POLAR, EP_VXX, EP_VYY, EP_VZZ, EP_VXY, EP_VXZ, EP_VYZ,
+  EP_EVDW, EP_EBOND, EP_EANGLE, EP_EUREY, EP_N
+};
 constexpr int kEnergyRow = 16;
 
 // ----- entry points implemented across the .hip files
@@ -205,8 +234,29 @@ void pme_spread(Ctx& c, int mode, const void* s1, const void* s2);
 void pme_fft_virial(Ctx& c, int roundtrip, double vscale);
 double* energy_rows(Ctx& c, int nrows);   // reserve partial rows for one launch
 void energy_setup(Ctx& c, const System& s);
-// vers bit 2: virial; out16[EP_*] energies / virial; grad_orig: caller-order dE/dx or null
-void energy_run(Ctx& c, int vers, double* out16, double* grad_orig);
+void md_setup(Ctx& c, const System& s);   // vdW / valence tables, MD buffers
+void vdw_sites(Ctx& c);                   // vdW site positions (sorted) from the coordinates
+// vdW (sorted egrad += ..) and valence (caller-order gval) terms of one
+// evaluation; partial rows reserved from the energy rows
+void vdw_run(Ctx& c);
+void valence_run(Ctx& c, double* gval_caller);
+// Term groups of one evaluation: the electrostatics (multipole + polarization)
+// and vdW are the slow group of the RESPA split, valence the fast group.
+enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_ALL = 7 };
+// vers bit 2: virial; out16[EP_*] energies / virial; sorted-order dE/dx of
+// the electrostatics and vdW in c.egrad, caller-order valence dE/dx in
+// c.g_fast; grad_orig (host, caller order, or null) receives their sum
+void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig);
+void prepare(Ctx& c);                     // list, sorted coordinates, global multipoles current
+
+// molecular dynamics (caller order, FP64)
+void md_velocities(Ctx& c, double kelvin, uint64_t seed);  // Maxwell-Boltzmann, net momentum removed
+double md_kinetic(Ctx& c);                // kcal/mol
+// v -= (h1 g1 + h2 g2) / m, g2 may be null
+void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
+void md_drift(Ctx& c, double h);          // x += h v
+void md_scale_v(Ctx& c, double a);
+void md_sorted_to_orig(Ctx& c, const double* sorted, double* orig);
 
 // sorted <-> caller order transfers of [n][3] real vectors
 void gather_to_host(Ctx& c, const void* dev_sorted_real3, double* host_orig);

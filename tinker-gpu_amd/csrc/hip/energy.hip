@@ -809,12 +809,14 @@ __global__ __launch_bounds__(256) void k_rows_reduce(int nrows, const double* __
   block_row<EP_N>(res, out);
 }
 
+// dst (caller order) = src (sorted) + add (caller order, may be null)
 __global__ void k_grad_to_orig(int n, const int* __restrict__ perm, const double* __restrict__ src,
-                               double* __restrict__ dst) {
+                               const double* __restrict__ add, double* __restrict__ dst) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
-  for (int d = 0; d < 3; ++d) dst[3 * (size_t)i + d] = src[3 * (size_t)s + d];
+  for (int d = 0; d < 3; ++d)
+    dst[3 * (size_t)i + d] = src[3 * (size_t)s + d] + (add ? add[3 * (size_t)i + d] : 0.0);
 }
 
 // ------------------------------------------------------------ host side
@@ -928,31 +930,37 @@ void recip_t(Ctx& c, bool use_m, bool use_p, bool want_v) {
 }
 
 template <class real>
-void energy_t(Ctx& c, int vers, double* out16) {
-  const bool use_m = c.opt.use_mpole, use_p = c.opt.use_polar;
+void energy_t(Ctx& c, int vers, int terms, double* out16) {
+  const bool use_m = c.opt.use_mpole && (terms & TERM_ELEC), use_p = c.opt.use_polar && (terms & TERM_ELEC);
   const bool want_v = (vers & 4) != 0;
   const int n = c.n, nb = ceil_div(n, 256);
   c.erows = 0;
-  if (!use_m && !use_p) {
+  if (use_m || use_p) {
+    real_space_t<real>(c, use_m, use_p);
+    if (c.ewald) recip_t<real>(c, use_m, use_p, want_v);
+    hipLaunchKernelGGL(k_self_polar<real>, dim3(nb), dim3(256), 0, c.stream, n, use_m ? 1 : 0, use_p ? 1 : 0,
+                       c.ewald ? 1 : 0, c.aewald, c.rpole.as<real>(), c.uind.as<real>(), c.udirp.as<real>(),
+                       c.polinv.as<real>(), energy_rows(c, nb));
+    T9_LAUNCH_CHECK();
+    const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0,
+                 lz = c.periodic ? c.box[2] : 1.0;
+    hipLaunchKernelGGL(k_torque<real>, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.xyz.as<double>(),
+                       lx, ly, lz, c.periodic ? 1 : 0, c.frame.as<int>(), c.zaxis.as<int>(), c.xaxis.as<int>(),
+                       c.yaxis.as<int>(), c.rpole.as<real>(), c.etq.as<double>(), c.eframe.as<double>(),
+                       energy_rows(c, nb));
+    T9_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_frame_gather, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.iperm.as<int>(),
+                       c.rev_off.as<int>(), c.rev_ent.as<int>(), c.eframe.as<double>(), c.egrad.as<double>());
+    T9_LAUNCH_CHECK();
+  } else {
     T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
+  }
+  if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c);
+  if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>());
+  if (c.erows == 0) {
     for (int q = 0; q < 16; ++q) out16[q] = 0;
     return;
   }
-  real_space_t<real>(c, use_m, use_p);
-  if (c.ewald) recip_t<real>(c, use_m, use_p, want_v);
-  hipLaunchKernelGGL(k_self_polar<real>, dim3(nb), dim3(256), 0, c.stream, n, use_m ? 1 : 0, use_p ? 1 : 0,
-                     c.ewald ? 1 : 0, c.aewald, c.rpole.as<real>(), c.uind.as<real>(), c.udirp.as<real>(),
-                     c.polinv.as<real>(), energy_rows(c, nb));
-  T9_LAUNCH_CHECK();
-  const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0, lz = c.periodic ? c.box[2] : 1.0;
-  hipLaunchKernelGGL(k_torque<real>, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.xyz.as<double>(), lx,
-                     ly, lz, c.periodic ? 1 : 0, c.frame.as<int>(), c.zaxis.as<int>(), c.xaxis.as<int>(),
-                     c.yaxis.as<int>(), c.rpole.as<real>(), c.etq.as<double>(), c.eframe.as<double>(),
-                     energy_rows(c, nb));
-  T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_frame_gather, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.iperm.as<int>(),
-                     c.rev_off.as<int>(), c.rev_ent.as<int>(), c.eframe.as<double>(), c.egrad.as<double>());
-  T9_LAUNCH_CHECK();
   double* dsum = c.scalars.as<double>() + 16;
   hipLaunchKernelGGL(k_rows_reduce, dim3(1), dim3(256), 0, c.stream, c.erows, c.epart.as<double>(), dsum);
   T9_LAUNCH_CHECK();
@@ -1001,24 +1009,26 @@ void energy_setup(Ctx& c, const System& s) {
   c.rev_ent.alloc(sizeof(int) * ent.size());
   T9_HIP(hipMemcpyAsync(c.rev_off.p, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c.stream));
   T9_HIP(hipMemcpyAsync(c.rev_ent.p, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice, c.stream));
-  // reduction rows: per-atom kernels (real space, 2 recip, self, torque) plus
-  // up to three convolution passes of one row per z-pass workgroup
+  // reduction rows: per-atom kernels (real space, 2 recip, self, torque, vdW,
+  // valence) plus up to three convolution passes of one row per z-pass
+  // workgroup
   const int nb = ceil_div(n, 256);
   long long lines = 0;
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
-  c.ecap = (int)(6LL * nb + lines + 16);
+  c.ecap = (int)(8LL * nb + lines + 16);
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }
 
-void energy_run(Ctx& c, int vers, double* out16, double* grad_orig) {
-  if (c.prec == 32) energy_t<float>(c, vers, out16);
-  else energy_t<double>(c, vers, out16);
+void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig) {
+  if (c.prec == 32) energy_t<float>(c, vers, terms, out16);
+  else energy_t<double>(c, vers, terms, out16);
   if (grad_orig) {
+    const bool val = (terms & TERM_VALENCE) && c.use_valence;
     DBuf tmp;
     tmp.alloc(sizeof(double) * 3 * c.n);
     hipLaunchKernelGGL(k_grad_to_orig, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
-                       c.egrad.as<double>(), tmp.as<double>());
+                       c.egrad.as<double>(), val ? c.g_fast.as<double>() : nullptr, tmp.as<double>());
     T9_LAUNCH_CHECK();
     T9_HIP(hipMemcpyAsync(grad_orig, tmp.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
     T9_HIP(hipStreamSynchronize(c.stream));

@@ -1,0 +1,593 @@
+// Molecular-dynamics terms around the electrostatics, for gfx950: AMOEBA
+// buffered 14-7 vdW on reduced hydrogen sites, the valence terms of the water
+// model, and a RESPA integrator with the Bussi thermostat.
+//
+// Reference behaviour followed: include/seq/pair_hal.h (buffered 14-7),
+// include/math/switch.h (5th-order taper), src/cu/evdw.cu / ehal (site
+// reduction), include/seq/bond.h, angle.h, urey.h (valence polynomials),
+// src/md/integrator.cpp:70-165 and src/md/propagator.cpp (RESPA splitting:
+// valence terms in the inner loop), src/md/thermostat.cpp (Bussi).
+//
+// vdW: one lane per sorted site over the site list (full, both directions),
+// pair scale by the entry's code, FP64 lane accumulation of the site
+// gradient, half the pair energy and half the pair virial; sites then hand
+// their gradient to the atoms (reduced hydrogen: red to itself, 1 - red to the
+// parent) through a fixed-order gather.  Valence: one thread per atom walks the
+// terms that contain it and keeps its own share; the term's first atom adds
+// the energy and virial.  No atomics anywhere, bit-reproducible.
+#include <algorithm>
+#include <cmath>
+#include <random>
+
+#include "ctx.hpp"
+
+namespace t9 {
+namespace {
+
+constexpr double kEkcalD = kEkcal;
+constexpr double kGasR = kGasConst;
+
+template <class real>
+__global__ void k_vsite_pos(int n, const int* __restrict__ perm, const double* __restrict__ xyz,
+                            const int* __restrict__ vparent, const double* __restrict__ vred,
+                            const int* __restrict__ jvdw_o, double3 vorg, double3 vlen, double3 box, int periodic,
+                            PosRec<real>* __restrict__ vpos, int* __restrict__ jvdw) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s], p = vparent[i];
+  const double L[3] = {box.x, box.y, box.z};
+  double x[3];
+  for (int d = 0; d < 3; ++d) {
+    x[d] = xyz[3 * i + d];
+    if (p != i) {
+      double dd = xyz[3 * i + d] - xyz[3 * p + d];
+      if (periodic) dd -= L[d] * rint(dd / L[d]);
+      x[d] = xyz[3 * p + d] + vred[i] * dd;
+    }
+  }
+  const double org[3] = {vorg.x, vorg.y, vorg.z}, len[3] = {vlen.x, vlen.y, vlen.z};
+  vpos[s] = make_pos<real>(x[0], x[1], x[2], 0.0, org, len, periodic);
+  jvdw[s] = jvdw_o[i];
+}
+
+template <class real>
+struct VdwArgs {
+  int n;
+  const int* nlist;
+  const long long* woff;
+  const uint32_t* entries;
+  const PosRec<real>* pos;
+  const int* jvdw;
+  const real* irv;  // [nj*nj][2] 1 / R_min as hi + lo
+  const real* eps;  // [nj*nj][2] epsilon as hi + lo
+  int nj;
+  double vscale[kMaxCodes];
+  BoxT<real> box;
+  real cut2, cut, taper;
+  double* gsite;  // [n][3] sorted
+  double* rows;
+};
+
+template <class real>
+__global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0, g[3] = {0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+  if (s < A.n) {
+    const PosRec<real> pi = A.pos[s];
+    const int ji = A.jvdw[s] * A.nj;
+    const int len = A.nlist[s];
+    const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
+    const real inv_tc = 1 / (A.taper - A.cut);
+    for (int kk = 0; kk < len; ++kk) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const real sc = code ? (real)A.vscale[code] : (real)1;
+      if (sc == 0) continue;
+      const PosRec<real> pk = A.pos[k];
+      real R[3];
+      pair_disp(pk, pi, A.box, R[0], R[1], R[2]);  // site i - site k
+      const real r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      if (!(r2 <= A.cut2)) continue;
+      // correctly rounded square root and division keep the per-pair
+      // rounding unbiased (the hardware reciprocal square root is not)
+      const real r = sqrt(r2);
+      const real rinv = 1 / r;
+      const int t = ji + A.jvdw[k];
+      // the pair parameters are shared by every pair of two types, so their
+      // rounding would bias the whole sum: hi + lo split, as for coordinates
+      const real ivh = A.irv[2 * t], ivl = A.irv[2 * t + 1];
+      const real eh = A.eps[2 * t] * sc, el = A.eps[2 * t + 1] * sc;
+      const real rho = fma(r, ivh, r * ivl);
+      const real rho6 = rho * rho * rho * rho * rho * rho, rho7 = rho6 * rho;
+      const real b = (real)1.07 / (rho + (real)0.07);
+      const real b2 = b * b, b3 = b2 * b;
+      const real t1 = b3 * b3 * b;
+      const real s2 = 1 / (rho7 + (real)0.12);
+      const real t2 = (real)1.12 * s2;
+      const real u = t1 * (t2 - 2);
+      real en_ = fma(u, eh, u * el);
+      real de = eh * ivh * (-7 * t1 / (rho + (real)0.07) * (t2 - 2) - 7 * t1 * rho6 * t2 * s2);
+      if (r > A.taper) {
+        const real x = (r - A.cut) * inv_tc;
+        const real S = x * x * x * (x * (6 * x - 15) + 10);
+        const real dS = 30 * (x * (1 - x)) * (x * (1 - x)) * inv_tc;
+        de = de * S + en_ * dS;
+        en_ *= S;
+      }
+      e += (double)en_;
+      const real f = de * rinv;
+      for (int d = 0; d < 3; ++d) g[d] += (double)(f * R[d]);
+      v[0] += (double)(f * R[0] * R[0]);
+      v[1] += (double)(f * R[1] * R[1]);
+      v[2] += (double)(f * R[2] * R[2]);
+      v[3] += (double)(f * R[0] * R[1]);
+      v[4] += (double)(f * R[0] * R[2]);
+      v[5] += (double)(f * R[1] * R[2]);
+    }
+    for (int d = 0; d < 3; ++d) A.gsite[3 * (size_t)s + d] = g[d];
+  }
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  red[EP_EVDW] = 0.5 * e;
+  for (int q = 0; q < 6; ++q) red[EP_VXX + q] = 0.5 * v[q];
+  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
+}
+
+// atom gradient (sorted egrad) += own site share + shares of the reduced sites
+// that hang on it
+__global__ void k_vdw_to_atoms(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
+                               const int* __restrict__ vparent, const double* __restrict__ vred,
+                               const int* __restrict__ coff, const int* __restrict__ child,
+                               const double* __restrict__ gsite, double* __restrict__ grad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int j = perm[t];
+  const double w = vparent[j] != j ? vred[j] : 1.0;
+  double g[3];
+  for (int d = 0; d < 3; ++d) g[d] = w * gsite[3 * (size_t)t + d];
+  for (int q = coff[j]; q < coff[j + 1]; ++q) {
+    const int c = child[q];
+    const int sc = iperm[c];
+    const double wc = 1.0 - vred[c];
+    for (int d = 0; d < 3; ++d) g[d] += wc * gsite[3 * (size_t)sc + d];
+  }
+  for (int d = 0; d < 3; ++d) grad[3 * (size_t)t + d] += g[d];
+}
+
+__device__ __forceinline__ void mdiff(const double* xyz, int a, int b, const double* L, int periodic, double* v) {
+  for (int d = 0; d < 3; ++d) {
+    double x = xyz[3 * a + d] - xyz[3 * b + d];
+    if (periodic) x -= L[d] * rint(x / L[d]);
+    v[d] = x;
+  }
+}
+
+struct ValArgs {
+  int n;
+  const double* xyz;
+  double L[3];
+  int periodic;
+  const int* toff;
+  const int* term;  // (kind << 28) | (index << 2) | role
+  const int* bij;
+  const double* bkr;
+  const int* aabc;
+  const double* akt;
+  const int* uac;
+  const double* ukr;
+  double cb, qb, ca, qa, pa, sa;
+  double* grad;  // caller order [n][3]
+  double* rows;
+};
+
+// Gradient of one valence term for its atoms (up to 3), energy, and the
+// relative positions used by the virial.
+__device__ void valence_term(const ValArgs& A, int kind, int idx, double* e, double g[3][3], double rel[3][3],
+                             int* at, int* nat) {
+  if (kind == 0 || kind == 2) {  // bond or Urey-Bradley: stretch polynomial
+    const int* ij = kind == 0 ? A.bij + 2 * idx : A.uac + 2 * idx;
+    const double* kr = kind == 0 ? A.bkr + 2 * idx : A.ukr + 2 * idx;
+    const double c = kind == 0 ? A.cb : 0.0, q = kind == 0 ? A.qb : 0.0;
+    at[0] = ij[0];
+    at[1] = ij[1];
+    *nat = 2;
+    double R[3];
+    mdiff(A.xyz, ij[0], ij[1], A.L, A.periodic, R);
+    const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    const double d = r - kr[1];
+    *e = kr[0] * d * d * (1 + c * d + q * d * d);
+    const double de = kr[0] * d * (2 + 3 * c * d + 4 * q * d * d) / r;
+    for (int a = 0; a < 3; ++a) {
+      g[0][a] = de * R[a];
+      g[1][a] = -de * R[a];
+      rel[0][a] = R[a];  // relative to atom 1
+      rel[1][a] = 0;
+    }
+  } else {  // angle a-b-c, b the vertex
+    const int* abc = A.aabc + 3 * idx;
+    at[0] = abc[0];
+    at[1] = abc[1];
+    at[2] = abc[2];
+    *nat = 3;
+    double u[3], v[3];
+    mdiff(A.xyz, abc[0], abc[1], A.L, A.periodic, u);
+    mdiff(A.xyz, abc[2], abc[1], A.L, A.periodic, v);
+    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
+    cs = fmin(1.0, fmax(-1.0, cs));
+    const double th = acos(cs), sn = sin(th);
+    const double deg = 57.29577951308232087680;
+    const double d = th * deg - A.akt[2 * idx + 1];
+    const double k = A.akt[2 * idx] / (deg * deg);
+    *e = k * d * d * (1 + A.ca * d + A.qa * d * d + A.pa * d * d * d + A.sa * d * d * d * d);
+    const double dedth =
+        k * d * (2 + 3 * A.ca * d + 4 * A.qa * d * d + 5 * A.pa * d * d * d + 6 * A.sa * d * d * d * d) * deg;
+    for (int q = 0; q < 3; ++q) {
+      g[0][q] = dedth * (cs * u[q] / lu - v[q] / lv) / (lu * sn);
+      g[2][q] = dedth * (cs * v[q] / lv - u[q] / lu) / (lv * sn);
+      g[1][q] = -(g[0][q] + g[2][q]);
+      rel[0][q] = u[q];  // relative to the vertex
+      rel[1][q] = 0;
+      rel[2][q] = v[q];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_valence(ValArgs A) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  if (j < A.n) {
+    double gj[3] = {0, 0, 0};
+    for (int q = A.toff[j]; q < A.toff[j + 1]; ++q) {
+      const int t = A.term[q];
+      const int kind = t >> 28, idx = (t >> 2) & 0x3ffffff, role = t & 3;
+      double e, g[3][3], rel[3][3];
+      int at[3], nat;
+      valence_term(A, kind, idx, &e, g, rel, at, &nat);
+      for (int d = 0; d < 3; ++d) gj[d] += g[role][d];
+      if (role == 0) {
+        red[kind == 0 ? EP_EBOND : (kind == 1 ? EP_EANGLE : EP_EUREY)] += e;
+        double W[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int m = 0; m < nat; ++m)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) W[a][b] += g[m][a] * rel[m][b];
+        red[EP_VXX] += W[0][0];
+        red[EP_VYY] += W[1][1];
+        red[EP_VZZ] += W[2][2];
+        red[EP_VXY] += 0.5 * (W[0][1] + W[1][0]);
+        red[EP_VXZ] += 0.5 * (W[0][2] + W[2][0]);
+        red[EP_VYZ] += 0.5 * (W[1][2] + W[2][1]);
+      }
+    }
+    for (int d = 0; d < 3; ++d) A.grad[3 * (size_t)j + d] = gj[d];
+  }
+  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
+}
+
+// ------------------------------------------------------------ integrator
+
+// v -= (h1 g1 + h2 g2) / m (h in ps, g in kcal/mol/Å, v in Å/ps); g2 may be null
+__global__ void k_kick(int n, double h1, const double* __restrict__ g1, double h2, const double* __restrict__ g2,
+                       const double* __restrict__ mass, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+  if (m <= 0) return;
+  const double c = kEkcalD / m;
+  for (int d = 0; d < 3; ++d) {
+    double a = h1 * g1[3 * i + d];
+    if (g2) a += h2 * g2[3 * i + d];
+    v[3 * i + d] -= c * a;
+  }
+}
+
+__global__ void k_drift(int n, double h, const double* __restrict__ v, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) x[3 * i + d] += h * v[3 * i + d];
+}
+
+__global__ void k_scale_v(int n, double a, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) v[3 * i + d] *= a;
+}
+
+// 2 K (g/mol Å^2/ps^2) partial sums
+__global__ __launch_bounds__(256) void k_ekin(int n, const double* __restrict__ mass, const double* __restrict__ v,
+                                              double* __restrict__ rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red[1] = {0};
+  if (i < n) red[0] = mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  block_row<1>(red, rows + blockIdx.x);
+}
+
+__global__ void k_sum_rows1(int nrows, const double* __restrict__ rows, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0;
+  for (int r = 0; r < nrows; ++r) s += rows[r];
+  out[0] = s;
+}
+
+__global__ void k_sorted_grad_to_orig(int n, const int* __restrict__ perm, const double* __restrict__ src,
+                                      double* __restrict__ dst) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  for (int d = 0; d < 3; ++d) dst[3 * (size_t)i + d] = src[3 * (size_t)s + d];
+}
+
+template <class T>
+void upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
+  b.alloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) T9_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+template <class real>
+void vdw_sites_t(Ctx& c) {
+  const double3 box = make_double3(c.periodic ? c.box[0] : 1.0, c.periodic ? c.box[1] : 1.0,
+                                   c.periodic ? c.box[2] : 1.0);
+  hipLaunchKernelGGL(k_vsite_pos<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
+                     c.xyz.as<double>(), c.vparent.as<int>(), c.vred.as<double>(), c.jvdw_o.as<int>(),
+                     make_double3(c.vorg[0], c.vorg[1], c.vorg[2]), make_double3(c.vlen[0], c.vlen[1], c.vlen[2]),
+                     box, c.periodic ? 1 : 0, c.vpos.as<PosRec<real>>(), c.jvdw.as<int>());
+  T9_LAUNCH_CHECK();
+}
+
+template <class real>
+void vdw_run_t(Ctx& c) {
+  VdwArgs<real> A;
+  A.n = c.n;
+  A.nlist = c.vlist.nlist.as<int>();
+  A.woff = c.vlist.off.as<long long>();
+  A.entries = c.vlist.entries.as<uint32_t>();
+  A.pos = c.vpos.as<PosRec<real>>();
+  A.jvdw = c.jvdw.as<int>();
+  A.irv = c.vrmin.as<real>();
+  A.eps = c.veps.as<real>();
+  A.nj = c.njvdw;
+  for (int k = 0; k < kMaxCodes; ++k) A.vscale[k] = c.vscale[k];
+  A.box = make_boxT<real>(c);
+  A.cut = (real)c.vdw_cut;
+  A.cut2 = (real)(c.vdw_cut * c.vdw_cut);
+  A.taper = (real)c.vdw_taper;
+  A.gsite = c.vsite_grad.as<double>();
+  const int nb = ceil_div(c.n, 256);
+  A.rows = energy_rows(c, nb);
+  {
+    KScope ks_(c, KT_VDW);
+    hipLaunchKernelGGL(k_vdw<real>, dim3(nb), dim3(256), 0, c.stream, A);
+    T9_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_vdw_to_atoms, dim3(nb), dim3(256), 0, c.stream, c.n, c.perm.as<int>(), c.iperm.as<int>(),
+                     c.vparent.as<int>(), c.vred.as<double>(), c.vchild_off.as<int>(), c.vchild.as<int>(),
+                     c.vsite_grad.as<double>(), c.egrad.as<double>());
+  T9_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void vdw_sites(Ctx& c) {
+  if (c.prec == 32) vdw_sites_t<float>(c);
+  else vdw_sites_t<double>(c);
+}
+
+void vdw_run(Ctx& c) {
+  if (!c.use_vdw) return;
+  if (c.prec == 32) vdw_run_t<float>(c);
+  else vdw_run_t<double>(c);
+}
+
+void valence_run(Ctx& c, double* gval) {
+  if (!c.use_valence) return;
+  ValArgs A;
+  A.n = c.n;
+  A.xyz = c.xyz.as<double>();
+  for (int d = 0; d < 3; ++d) A.L[d] = c.periodic ? c.box[d] : 1.0;
+  A.periodic = c.periodic ? 1 : 0;
+  A.toff = c.vterm_off.as<int>();
+  A.term = c.vterm.as<int>();
+  A.bij = c.bond_ij.as<int>();
+  A.bkr = c.bond_kr.as<double>();
+  A.aabc = c.angle_abc.as<int>();
+  A.akt = c.angle_kt.as<double>();
+  A.uac = c.urey_ac.as<int>();
+  A.ukr = c.urey_kr.as<double>();
+  A.cb = c.vpoly[0];
+  A.qb = c.vpoly[1];
+  A.ca = c.vpoly[2];
+  A.qa = c.vpoly[3];
+  A.pa = c.vpoly[4];
+  A.sa = c.vpoly[5];
+  A.grad = gval;
+  const int nb = ceil_div(c.n, 256);
+  A.rows = energy_rows(c, nb);
+  KScope ks_(c, KT_VALENCE);
+  hipLaunchKernelGGL(k_valence, dim3(nb), dim3(256), 0, c.stream, A);
+  T9_LAUNCH_CHECK();
+}
+
+void md_setup(Ctx& c, const System& s) {
+  const int n = c.n;
+  hipStream_t st = c.stream;
+  c.use_vdw = s.opt.use_vdw;
+  c.use_valence = s.opt.use_valence;
+  c.vdw_cut = s.opt.vdw_cut;
+  c.vdw_taper = s.opt.vdw_taper;
+  if (c.use_vdw) {
+    upload(c.vparent, s.vdw_parent, st);
+    upload(c.vred, s.vdw_red, st);
+    upload(c.jvdw_o, s.jvdw, st);
+    c.jvdw.alloc(sizeof(int) * n);
+    c.njvdw = s.njvdw;
+    // tables of (1 / R_min, epsilon), each as hi + lo in the kernel precision.
+    // In FP32 the kernel's 1.07 is float(1.07), which scales every pair energy
+    // by (float(1.07) / 1.07)^7; epsilon carries the inverse factor.
+    const size_t nt = s.vdw_rmin.size();
+    const double fix = c.prec == 32 ? std::pow(1.07 / (double)1.07f, 7) : 1.0;
+    std::vector<double> irv(2 * nt), eps(2 * nt);
+    for (size_t t = 0; t < nt; ++t) {
+      const double iv = s.vdw_rmin[t] > 0 ? 1.0 / s.vdw_rmin[t] : 0.0, ep = s.vdw_epsmix[t] * fix;
+      if (c.prec == 32) {
+        irv[2 * t] = (float)iv;
+        irv[2 * t + 1] = (float)(iv - (double)(float)iv);
+        eps[2 * t] = (float)ep;
+        eps[2 * t + 1] = (float)(ep - (double)(float)ep);
+      } else {
+        irv[2 * t] = iv;
+        eps[2 * t] = ep;
+        irv[2 * t + 1] = eps[2 * t + 1] = 0.0;
+      }
+    }
+    if (c.prec == 32) {
+      upload(c.vrmin, std::vector<float>(irv.begin(), irv.end()), st);
+      upload(c.veps, std::vector<float>(eps.begin(), eps.end()), st);
+    } else {
+      upload(c.vrmin, irv, st);
+      upload(c.veps, eps, st);
+    }
+    c.vpos.alloc(c.prec == 32 ? sizeof(PosRec<float>) * n : sizeof(PosRec<double>) * n);
+    c.vsite_grad.alloc(sizeof(double) * 3 * n);
+    // scaled vdW partners: codes over the distinct scale values
+    std::vector<double> vals = {1.0};
+    std::vector<std::vector<std::pair<int, int>>> ex(n);
+    for (size_t q = 0; q < s.vscale_s.size(); ++q) {
+      const double v = s.vscale_s[q];
+      int code = (int)(std::find(vals.begin(), vals.end(), v) - vals.begin());
+      if (code == (int)vals.size()) vals.push_back(v);
+      if (code >= kMaxCodes) throw ArgError("too many distinct vdW scale values");
+      const int i = s.vscale_ik[2 * q], k = s.vscale_ik[2 * q + 1];
+      ex[i].push_back({k, code});
+      ex[k].push_back({i, code});
+    }
+    for (int k = 0; k < kMaxCodes; ++k) c.vscale[k] = k < (int)vals.size() ? vals[k] : 1.0;
+    std::vector<int> off(n + 1, 0), ej, ec;
+    for (int i = 0; i < n; ++i) {
+      off[i + 1] = off[i] + (int)ex[i].size();
+      for (auto& pr : ex[i]) {
+        ej.push_back(pr.first);
+        ec.push_back(pr.second);
+      }
+    }
+    upload(c.vexcl_off, off, st);
+    upload(c.vexcl_j, ej, st);
+    upload(c.vexcl_code, ec, st);
+    // reduced sites hanging on each atom
+    std::vector<std::vector<int>> ch(n);
+    for (int i = 0; i < n; ++i)
+      if (s.vdw_parent[i] != i) ch[s.vdw_parent[i]].push_back(i);
+    std::vector<int> coff(n + 1, 0), cl;
+    for (int i = 0; i < n; ++i) {
+      coff[i + 1] = coff[i] + (int)ch[i].size();
+      cl.insert(cl.end(), ch[i].begin(), ch[i].end());
+    }
+    upload(c.vchild_off, coff, st);
+    upload(c.vchild, cl, st);
+  }
+  if (c.use_valence) {
+    upload(c.bond_ij, s.bond_ij, st);
+    upload(c.bond_kr, s.bond_kr, st);
+    upload(c.angle_abc, s.angle_abc, st);
+    upload(c.angle_kt, s.angle_kt, st);
+    upload(c.urey_ac, s.urey_ac, st);
+    upload(c.urey_kr, s.urey_kr, st);
+    c.vpoly[0] = s.ff.bond_cubic;
+    c.vpoly[1] = s.ff.bond_quartic;
+    c.vpoly[2] = s.ff.angle_cubic;
+    c.vpoly[3] = s.ff.angle_quartic;
+    c.vpoly[4] = s.ff.angle_pentic;
+    c.vpoly[5] = s.ff.angle_sextic;
+    std::vector<std::vector<int>> tl(n);
+    const int nb = (int)s.bond_kr.size() / 2, na = (int)s.angle_kt.size() / 2, nu = (int)s.urey_kr.size() / 2;
+    for (int t = 0; t < nb; ++t)
+      for (int r = 0; r < 2; ++r) tl[s.bond_ij[2 * t + r]].push_back((0 << 28) | (t << 2) | r);
+    for (int t = 0; t < na; ++t)
+      for (int r = 0; r < 3; ++r) tl[s.angle_abc[3 * t + r]].push_back((1 << 28) | (t << 2) | r);
+    for (int t = 0; t < nu; ++t)
+      for (int r = 0; r < 2; ++r) tl[s.urey_ac[2 * t + r]].push_back((2 << 28) | (t << 2) | r);
+    std::vector<int> off(n + 1, 0), tt;
+    for (int i = 0; i < n; ++i) {
+      off[i + 1] = off[i] + (int)tl[i].size();
+      tt.insert(tt.end(), tl[i].begin(), tl[i].end());
+    }
+    upload(c.vterm_off, off, st);
+    upload(c.vterm, tt, st);
+  }
+  upload(c.mass, s.mass, st);
+  c.mass_h = s.mass;
+  c.nfree = std::max(1, 3 * n - 3);
+  c.vel.alloc(sizeof(double) * 3 * n);
+  c.g_slow.alloc(sizeof(double) * 3 * n);
+  c.g_fast.alloc(sizeof(double) * 3 * n);
+  T9_HIP(hipMemsetAsync(c.vel.p, 0, sizeof(double) * 3 * n, st));
+  T9_HIP(hipMemsetAsync(c.g_fast.p, 0, sizeof(double) * 3 * n, st));
+  T9_HIP(hipStreamSynchronize(st));
+}
+
+// Maxwell-Boltzmann velocities at `kelvin` from a seeded host stream, net
+// momentum removed.
+void md_velocities(Ctx& c, double kelvin, uint64_t seed) {
+  const int n = c.n;
+  const std::vector<double>& mass = c.mass_h;
+  std::mt19937_64 gen(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  std::vector<double> v(3 * n, 0.0);
+  double p[3] = {0, 0, 0}, mt = 0;
+  for (int i = 0; i < n; ++i) {
+    if (mass[i] <= 0) continue;
+    const double sd = std::sqrt(kGasR * kelvin * kEkcalD / mass[i]);
+    for (int d = 0; d < 3; ++d) {
+      v[3 * i + d] = sd * nd(gen);
+      p[d] += mass[i] * v[3 * i + d];
+    }
+    mt += mass[i];
+  }
+  for (int i = 0; i < n; ++i)
+    if (mass[i] > 0)
+      for (int d = 0; d < 3; ++d) v[3 * i + d] -= p[d] / mt;
+  T9_HIP(hipMemcpyAsync(c.vel.p, v.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+double md_kinetic(Ctx& c) {  // kcal/mol
+  const int nb = ceil_div(c.n, 256);
+  double* rows = c.partials.as<double>();
+  hipLaunchKernelGGL(k_ekin, dim3(nb), dim3(256), 0, c.stream, c.n, c.mass.as<double>(), c.vel.as<double>(), rows);
+  hipLaunchKernelGGL(k_sum_rows1, dim3(1), dim3(64), 0, c.stream, nb, rows, c.scalars.as<double>() + 40);
+  T9_LAUNCH_CHECK();
+  T9_HIP(hipMemcpyAsync(&c.h_scalars[40], c.scalars.as<double>() + 40, sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  return 0.5 * c.h_scalars[40] / kEkcalD;
+}
+
+void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2) {
+  hipLaunchKernelGGL(k_kick, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, h1, g1, h2, g2,
+                     c.mass.as<double>(), c.vel.as<double>());
+  T9_LAUNCH_CHECK();
+}
+
+void md_drift(Ctx& c, double h) {
+  hipLaunchKernelGGL(k_drift, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, h, c.vel.as<double>(),
+                     c.xyz.as<double>());
+  T9_LAUNCH_CHECK();
+  c.coords_dirty = true;
+  c.rpole_valid = false;
+  c.bins_valid = false;
+}
+
+void md_scale_v(Ctx& c, double a) {
+  hipLaunchKernelGGL(k_scale_v, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, a, c.vel.as<double>());
+  T9_LAUNCH_CHECK();
+}
+
+void md_sorted_to_orig(Ctx& c, const double* sorted, double* orig) {
+  hipLaunchKernelGGL(k_sorted_grad_to_orig, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                     c.perm.as<int>(), sorted, orig);
+  T9_LAUNCH_CHECK();
+}
+
+}  // namespace t9

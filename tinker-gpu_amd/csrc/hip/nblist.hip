@@ -267,7 +267,7 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
 
 
 template <class real>
-void build_list(Ctx& c, NbList& L) {
+void build_list(Ctx& c, NbList& L, const void* posv, const int* eoff, const int* ej, const int* ecode) {
   const int n = c.n, nw = c.nwave;
   const int nb = ceil_div(n, 256);
   L.nlist.alloc(sizeof(int) * n);
@@ -278,11 +278,10 @@ void build_list(Ctx& c, NbList& L) {
   BoxT<real> box = make_boxT<real>(c);
   real rl2 = (real)(L.cutoff * L.cutoff);
   const int nkeys = 1 << (3 * c.morton_bits);
-  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 0,
-                     c.pos.as<PosRec<real>>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
-                     c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(), c.cell_count.as<int>(), box, rl2,
-                     c.perm.as<int>(), c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>(),
-                     L.nlist.as<int>(), (const long long*)nullptr, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 0, (const PosRec<real>*)posv,
+                     c.cellc.as<int>(), c.ncell[0], c.ncell[1], c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(),
+                     c.cell_count.as<int>(), box, rl2, c.perm.as<int>(), eoff, ej, ecode, L.nlist.as<int>(),
+                     (const long long*)nullptr, (uint32_t*)nullptr);
   T9_LAUNCH_CHECK();
   (void)nkeys;
   hipLaunchKernelGGL(k_wave_slots, dim3(ceil_div(nw, 256)), dim3(256), 0, c.stream, n, nw, L.nlist.as<int>(),
@@ -301,11 +300,10 @@ void build_list(Ctx& c, NbList& L) {
     L.capacity = cap;
   }
   L.total = total;
-  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 1,
-                     c.pos.as<PosRec<real>>(), c.cellc.as<int>(), c.ncell[0], c.ncell[1],
-                     c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(), c.cell_count.as<int>(), box, rl2,
-                     c.perm.as<int>(), c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>(),
-                     L.nlist.as<int>(), L.off.as<long long>(), L.entries.as<uint32_t>());
+  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 1, (const PosRec<real>*)posv,
+                     c.cellc.as<int>(), c.ncell[0], c.ncell[1], c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(),
+                     c.cell_count.as<int>(), box, rl2, c.perm.as<int>(), eoff, ej, ecode, L.nlist.as<int>(),
+                     L.off.as<long long>(), L.entries.as<uint32_t>());
   T9_LAUNCH_CHECK();
 }
 
@@ -349,8 +347,13 @@ void nblist_build_t(Ctx& c) {
                      c.polarity_o.as<double>(), c.pdamp_o.as<double>(), c.jpolar_o.as<int>(), c.polarity.as<real>(),
                      c.polinv.as<real>(), c.pdamp.as<real>(), c.jpolar.as<int>());
   T9_LAUNCH_CHECK();
-  build_list<real>(c, c.mlist);
-  if (c.ulist.cutoff > 0) build_list<real>(c, c.ulist);
+  build_list<real>(c, c.mlist, c.pos.p, c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>());
+  if (c.ulist.cutoff > 0)
+    build_list<real>(c, c.ulist, c.pos.p, c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>());
+  if (c.vlist.cutoff > 0) {
+    vdw_sites(c);
+    build_list<real>(c, c.vlist, c.vpos.p, c.vexcl_off.as<int>(), c.vexcl_j.as<int>(), c.vexcl_code.as<int>());
+  }
   c.builds++;
   c.list_valid = true;
   c.coords_dirty = false;
@@ -424,6 +427,7 @@ void refresh_t(Ctx& c) {
                      make_double3(c.vlen[0], c.vlen[1], c.vlen[2]), c.periodic ? 1 : 0, c.pdamp_o.as<double>(),
                      c.pos.as<PosRec<real>>());
   T9_LAUNCH_CHECK();
+  if (c.vlist.cutoff > 0) vdw_sites(c);
   c.coords_dirty = false;
   c.rpole_valid = false;
   c.bins_valid = false;

@@ -27,9 +27,10 @@ EXPORTS = [
     "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
     "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
-    "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy",
+    "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy", "t9_md_init", "t9_md_set_velocities",
+    "t9_md_get_velocities", "t9_md_run",
 ]
-K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR = range(8)
+K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL = 1, 2, 4
 
 
@@ -47,7 +48,13 @@ class InduceStats(ctypes.Structure):
 class EnergyOut(ctypes.Structure):
     _fields_ = [("esum", ctypes.c_double), ("em", ctypes.c_double), ("ep", ctypes.c_double),
                 ("em_real", ctypes.c_double), ("em_recip", ctypes.c_double), ("em_self", ctypes.c_double),
-                ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int)]
+                ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int), ("evdw", ctypes.c_double),
+                ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double)]
+
+
+class MdStats(ctypes.Structure):
+    _fields_ = [("epot", ctypes.c_double), ("ekin", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("induce_iterations", ctypes.c_int64), ("list_builds", ctypes.c_int), ("steps", ctypes.c_int)]
 
 
 _LIB = None
@@ -96,6 +103,11 @@ def lib():
             ("t9_cutoff_pairs", [_vp, ctypes.POINTER(ctypes.c_int64)]),
             ("t9_copy_probe", [ctypes.c_int64, ctypes.c_int, _d]),
             ("t9_energy", [_vp, ctypes.c_int, ctypes.POINTER(EnergyOut), _d]),
+            ("t9_md_init", [_vp, ctypes.c_double, ctypes.c_uint64]),
+            ("t9_md_set_velocities", [_vp, _d]),
+            ("t9_md_get_velocities", [_vp, _d]),
+            ("t9_md_run", [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                           ctypes.POINTER(MdStats)]),
         ]:
             if hasattr(L, name):  # device entry points are absent from a host-only build
                 getattr(L, name).argtypes = args
@@ -286,14 +298,36 @@ class Context:
         check(lib().t9_set_uind(self.h, _p(ud), _p(up)))
 
     def energy(self, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL, grad=True):
-        """Multipole + polarization energy (dict), gradient dE/dx [n,3]
-        (caller order, or None) and virial [3,3]."""
+        """Energy of the enabled terms (dict), gradient dE/dx [n,3] (caller
+        order, or None) and virial [3,3]."""
         out = EnergyOut()
         g = np.zeros((self.n, 3)) if grad and (vers & CALC_GRAD) else None
         check(lib().t9_energy(self.h, vers, ctypes.byref(out), _p(g) if g is not None else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
-                 em_self=out.em_self, iterations=out.induce_iterations)
+                 em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
+                 eangle=out.eangle, eurey=out.eurey)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
+
+    def md_init(self, kelvin, seed):
+        check(lib().t9_md_init(self.h, float(kelvin), int(seed)))
+
+    def velocities(self):
+        v = np.zeros((self.n, 3))
+        check(lib().t9_md_get_velocities(self.h, _p(v)))
+        return v
+
+    def set_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(lib().t9_md_set_velocities(self.h, _p(v)))
+
+    def md_run(self, nsteps, dt=0.002, nrespa=8, bath_kelvin=0.0, tau=0.1):
+        """nsteps RESPA steps (dt ps outer, dt/nrespa inner); bath_kelvin > 0
+        applies the Bussi thermostat.  Returns a dict of the run's stats."""
+        st = MdStats()
+        check(lib().t9_md_run(self.h, int(nsteps), float(dt), int(nrespa), float(bath_kelvin), float(tau),
+                              ctypes.byref(st)))
+        return dict(epot=st.epot, ekin=st.ekin, temperature=st.temperature, iterations=st.induce_iterations,
+                    list_builds=st.list_builds, steps=st.steps)
 
     def sync(self):
         check(lib().t9_sync(self.h))
