@@ -142,6 +142,18 @@ __device__ __forceinline__ void pair_disp(const PosRec<double>& a, const PosRec<
   z = b.z - a.z;
   if (box.periodic) min_image(x, y, z, box.l[0], box.l[1], box.l[2], box.il[0], box.il[1], box.il[2]);
 }
+// The same displacement b - a in FP64: exact for the fixed-point form (the
+// integer difference times the quantum hi + lo), the plain one otherwise.
+__device__ __forceinline__ void pair_disp64(const PosRec<float>& a, const PosRec<float>& b, const BoxT<float>& box,
+                                            double* R) {
+  R[0] = (double)(int)((unsigned)b.x - (unsigned)a.x) * ((double)box.q[0] + (double)box.qlo[0]);
+  R[1] = (double)(int)((unsigned)b.y - (unsigned)a.y) * ((double)box.q[1] + (double)box.qlo[1]);
+  R[2] = (double)(int)((unsigned)b.z - (unsigned)a.z) * ((double)box.q[2] + (double)box.qlo[2]);
+}
+__device__ __forceinline__ void pair_disp64(const PosRec<double>& a, const PosRec<double>& b, const BoxT<double>& box,
+                                            double* R) {
+  pair_disp(a, b, box, R[0], R[1], R[2]);
+}
 
 // Position record of a site from FP64 coordinates; (org, len) is the box
 // (periodic) or the virtual box (non-periodic).

@@ -19,8 +19,10 @@
 // Every ordered pair is visited from both atoms (full lists), so each lane
 // owns its atom's gradient, torque terms, half the pair energy and half the
 // pair virial, accumulated in FP64 registers: no atomics, bit-reproducible.
-// Pairs with scale factors are first taken unscaled; the same lane then adds
-// (s - 1) corrections over its scaled partners (kernel lambda_n rr_n).
+// Pairs with scale factors are skipped in the list loop; the same lane then
+// walks its scaled partners and evaluates each whole pair term (unscaled
+// Ewald part plus the (s - 1) non-Ewald correction, kernel lambda_n rr_n) in
+// FP64, whatever the working precision.
 //
 // Reciprocal space: the permanent-multipole grid gives 20 potential
 // derivatives per atom (energy 1/2 M.phi, gradient M.grad phi, torque
@@ -211,7 +213,8 @@ struct Acc {
     for (int k = 0; k < 3; ++k) gr[k] = tg[k] = 0;
     for (int k = 0; k < 6; ++k) th[k] = v[k] = 0;
   }
-  __device__ __forceinline__ void add(const PairOut<real>& o, const real* R) {
+  template <class U>
+  __device__ __forceinline__ void add(const PairOut<U>& o, const U* R) {
     e += (double)o.em;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -223,9 +226,9 @@ struct Acc {
     v[0] += (double)(o.de[0] * R[0]);
     v[1] += (double)(o.de[1] * R[1]);
     v[2] += (double)(o.de[2] * R[2]);
-    v[3] += (double)((real)0.5 * (o.de[0] * R[1] + o.de[1] * R[0]));
-    v[4] += (double)((real)0.5 * (o.de[0] * R[2] + o.de[2] * R[0]));
-    v[5] += (double)((real)0.5 * (o.de[1] * R[2] + o.de[2] * R[1]));
+    v[3] += (double)((U)0.5 * (o.de[0] * R[1] + o.de[1] * R[0]));
+    v[4] += (double)((U)0.5 * (o.de[0] * R[2] + o.de[2] * R[0]));
+    v[5] += (double)((U)0.5 * (o.de[1] * R[2] + o.de[2] * R[1]));
   }
 };
 
@@ -253,6 +256,7 @@ struct EmplarArgs {
   BoxT<real> box;
   real cut2;
   EwK6<real> ek;
+  EwK6<double> ek64;  // the same constants for the FP64 scaled-pair terms
   double* grad;  // [n][3]
   double* tq;    // [n][9]
   double* rows;  // [gridDim][kRow]
@@ -282,6 +286,7 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
     for (int kk = 0; kk < len; ++kk) {
       const uint32_t ent = nxt;
       if (kk + 1 < len) nxt = e[(long long)(kk + 1) * 64];
+      if (ent & (kMaxCodes - 1)) continue;  // scaled pairs: FP64 loop below
       const int j = (int)(ent >> kCodeBits);
       const PosRec<real> pj = A.pos[j];
       real R[3];
@@ -323,50 +328,82 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
       pair_terms<real, DM, DP>(R, bn, BT, mi, mk, ui, uk, udi, upi, udk, upk, (real)1, o);
       acc.add(o, R);
     }
-    // (s - 1) corrections of the scaled partners inside the cutoff
+    // Pairs with a scale factor (the scaled partners inside the cutoff): the
+    // whole pair term, Ewald part plus (s - 1) non-Ewald correction, in FP64
+    // from the exact displacement.  These are the large intramolecular terms,
+    // alike from one molecule to the next, so FP32 rounding in them would not
+    // average out over the system.
     const int io = A.perm[s];
+    Mp<double> mid;
+    mid.q = mi.q;
+    for (int t = 0; t < 3; ++t) mid.d[t] = mi.d[t];
+    for (int t = 0; t < 6; ++t) mid.Q[t] = mi.Q[t];
+    double udd[3], upd[3], uid[3];
+    for (int t = 0; t < 3; ++t) {
+      udd[t] = udi[t];
+      upd[t] = upi[t];
+      uid[t] = udd[t] + upd[t];
+    }
     for (int q = A.excl_off[io]; q < A.excl_off[io + 1]; ++q) {
       const int code = A.excl_code[q];
-      const real wm = (real)(A.sc.m[code] - 1), wd = (real)(A.sc.d[code] - 1), wp = (real)(A.sc.p[code] - 1),
-                 wu = (real)(A.sc.u[code] - 1);
-      if ((!DM || wm == 0) && (!DP || (wd == 0 && wp == 0 && wu == 0))) continue;
+      const double wm = A.sc.m[code] - 1, wd = A.sc.d[code] - 1, wp = A.sc.p[code] - 1, wu = A.sc.u[code] - 1;
       const int j = A.iperm[A.excl_j[q]];
       const PosRec<real> pj = A.pos[j];
       // same displacement and test as the in-cutoff compaction
       real xr, yr, zr;
       pair_disp(pi, pj, A.box, xr, yr, zr);
       if (!(xr * xr + yr * yr + zr * zr <= A.cut2)) continue;
-      real R[3] = {-xr, -yr, -zr};
-      const real r2 = dot3(R, R);
-      const real rinv = t9_rsqrt(r2);
-      const real r = r2 * rinv;
-      const real rr2 = rinv * rinv;
-      real rr[6];
+      double R[3];
+      pair_disp64(pj, pi, A.box, R);  // r_i - r_k
+      const double r2 = dot3(R, R);
+      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
+      double rr[6], bn[6], BM[6];
       rr[0] = rinv;
 #pragma unroll
-      for (int t = 1; t < 6; ++t) rr[t] = rr[t - 1] * (real)(2 * t - 1) * rr2;
-      real BM[6];
+      for (int t = 1; t < 6; ++t) rr[t] = rr[t - 1] * (double)(2 * t - 1) * rr2;
+      if (EWALD) {
+        double ex;
+        bn[0] = t9_erfc_pair(A.ek64.a * r, ex) * rinv;
 #pragma unroll
-      for (int t = 0; t < 6; ++t) BM[t] = wm * rr[t];
-      const Mp<real> mk = load_mp(A.rpole + 10 * (size_t)j);
-      real BT[5] = {0, 0, 0, 0, 0};
-      real udk[3] = {0, 0, 0}, upk[3] = {0, 0, 0}, uie[3] = {0, 0, 0}, uke[3] = {0, 0, 0};
+        for (int t = 1; t < 6; ++t) bn[t] = ((double)(2 * t - 1) * bn[t - 1] + A.ek64.e[t] * ex) * rr2;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) bn[t] = rr[t];
+      }
+#pragma unroll
+      for (int t = 0; t < 6; ++t) BM[t] = bn[t] + wm * rr[t];
+      const real* pk = A.rpole + 10 * (size_t)j;
+      Mp<double> mkd;
+      mkd.q = pk[0];
+      for (int t = 0; t < 3; ++t) mkd.d[t] = pk[1 + t];
+      for (int t = 0; t < 6; ++t) mkd.Q[t] = pk[4 + t];
+      double BT[5] = {0, 0, 0, 0, 0}, BTs[5] = {0, 0, 0, 0, 0};
+      double udk[3] = {0, 0, 0}, upk[3] = {0, 0, 0}, ukd[3] = {0, 0, 0}, uie[3] = {0, 0, 0}, uke[3] = {0, 0, 0};
       if (DP) {
-        const real pga = A.uth ? A.pga_u : A.thlval[jpi + A.jpolar[j]];
-        real lam[5];
-        thole4(r, pi.w * pj.w, pga, lam);
+        const double pga = A.uth ? (double)A.pga_u : (double)A.thlval[jpi + A.jpolar[j]];
+        double lam[5];
+        thole4(r, (double)pi.w * (double)pj.w, pga, lam);
 #pragma unroll
-        for (int t = 1; t < 5; ++t) BT[t] = lam[t] * rr[t];
+        for (int t = 1; t < 5; ++t) {
+          BT[t] = bn[t] - (1 - lam[t]) * rr[t];
+          BTs[t] = lam[t] * rr[t];
+        }
         for (int t = 0; t < 3; ++t) {
           udk[t] = A.ud[3 * j + t];
           upk[t] = A.up[3 * j + t];
-          uie[t] = wd * upi[t] + wp * udi[t];
+          ukd[t] = udk[t] + upk[t];
+          uie[t] = wd * upd[t] + wp * udd[t];
           uke[t] = wd * upk[t] + wp * udk[t];
         }
       }
-      PairOut<real> o;
-      pair_terms<real, DM, DP>(R, BM, BT, mi, mk, uie, uke, udi, upi, udk, upk, wu, o);
+      PairOut<double> o;
+      pair_terms<double, DM, DP>(R, BM, BT, mid, mkd, uid, ukd, udd, upd, udk, upk, 1.0, o);
       acc.add(o, R);
+      if (DP && (wd != 0 || wp != 0 || wu != 0)) {
+        const double Z[6] = {0, 0, 0, 0, 0, 0};
+        pair_terms<double, DM, DP>(R, Z, BTs, mid, mkd, uie, uke, udd, upd, udk, upk, wu, o);
+        acc.add(o, R);
+      }
     }
     const double f = kCoul;
     for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] = f * acc.gr[k];
@@ -864,6 +901,7 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   const double c2 = L.cut * L.cut;
   A.cut2 = (real)(c2 > 1e30 ? 1e30 : c2);
   A.ek = ewk6<real>(c);
+  A.ek64 = ewk6<double>(c);
   A.grad = c.egrad.as<double>();
   A.tq = c.etq.as<double>();
   const int nb = ceil_div(c.n, 256);
