@@ -155,6 +155,49 @@ typedef struct {
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
 
+/* ------------------------------------------------------- learned guess */
+/* A learned initial guess for the induced dipoles.  No counterpart in the
+ * reference: it plugs in where the reference's predictor runs
+ * (src/amoeba/induce.cpp:29-69, src/cu/upredict.cu:34-201; polar-predict
+ * ASPC is required and stays the fallback).  u0 = u_ASPC + R^T NN(x): R the
+ * atom's multipole local frame, x 32 local-frame features (field
+ * extrapolation errors of the atom and its frame atoms, current field, last
+ * dipole and its change, polarizability, type); NN a 32-128-128-6 BF16 MLP
+ * with a linear skip, evaluated and trained with MFMA kernels.  The network
+ * only changes the starting point: PCG still converges to polar-eps, and a
+ * guess whose initial residual exceeds the ASPC guess's (re-measured every
+ * 20 solves) is replaced by the ASPC guess.
+ * Parameters: flat FP32 [t9_guess_nparam()] in the order W1 [32][128], b1,
+ * W2 [128][128], b2, W3 [160][6] (rows 0..127 hidden, 128..159 skip), b3;
+ * norm [76]: feature mean [32], 1/std [32], target mean [6], std [6]. */
+int t9_guess_nparam(void);
+/* params NULL: seeded initialization; norm NULL: fitted by t9_guess_train. */
+int t9_guess_set_params(t9_ctx* ctx, const float* params, const float* norm, uint64_t seed);
+int t9_guess_get_params(t9_ctx* ctx, float* params, float* norm);
+int t9_guess_enable(t9_ctx* ctx, int on);
+/* Training data: from the next solve on, every stride-th atom's raw features
+ * and converged correction R (u - u_ASPC) are stored on the device, up to
+ * max_samples rows (stride 0 stops). */
+int t9_guess_capture(t9_ctx* ctx, int stride, int64_t max_samples);
+/* [0] captured rows, [1] solves that used the learned guess, [2] fallbacks
+ * to ASPC, [3] predictor of the last solve (0 direct, 1 ASPC, 2 learned). */
+int t9_guess_stats(t9_ctx* ctx, int64_t* out4);
+/* Adam over the captured rows (mean squared error of the normalized
+ * targets), seeded pseudo-random order; mean loss of the first/last epoch. */
+int t9_guess_train(t9_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss_first,
+                   double* loss_last);
+/* Kernel-level entry points (used by the parity tests): network outputs
+ * Y [m][6] for normalized features X [m][32] (rounded to BF16 on the device);
+ * the mean-squared-error gradient [nparam] against targets T [m][6]; one Adam
+ * step with a given gradient; the raw features [n][32] (caller order) that
+ * the next solve at the current coordinates would use; history entries
+ * (kind 0 E_d, 1 E_p, 2 u_d, 3 u_p; age 0 = newest) [n][3]. */
+int t9_guess_mlp_forward(t9_ctx* ctx, const float* X, int m, float* Y);
+int t9_guess_mlp_grad(t9_ctx* ctx, const float* X, const float* T, int m, float* grad, double* loss);
+int t9_guess_adam(t9_ctx* ctx, const float* grad, double lr);
+int t9_guess_features(t9_ctx* ctx, float* F);
+int t9_guess_history(t9_ctx* ctx, int kind, int age, double* out);
+
 /* ------------------------------------------------------------ dynamics */
 /* Molecular dynamics on the device, caller-order FP64 positions and
  * velocities (Å, Å/ps), masses from the system (g/mol).

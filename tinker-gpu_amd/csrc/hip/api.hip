@@ -275,7 +275,10 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.tile_atoms.alloc(sizeof(int) * n);
   }
   alloc_vectors(c);
-  if (s.opt.predictor == "ASPC") c.ualt.alloc(rs * 6 * (size_t)n * 16);
+  if (s.opt.predictor == "ASPC") {
+    c.ualt.alloc(rs * 6 * (size_t)n * 16);
+    guess_setup(c);
+  }
   else if (s.opt.predictor != "NONE") throw ArgError("polar-predict " + s.opt.predictor + " is not supported");
   energy_setup(c, s);
   md_setup(c, s);
@@ -719,6 +722,169 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
       stats->induce_iterations = iters;
       stats->list_builds = c.builds - builds0;
       stats->steps = nsteps;
+    }
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_nparam(void) { return t9::guess_nparam(); }
+
+int t9_guess_set_params(t9_ctx* ctx, const float* params, const float* norm, uint64_t seed) {
+  return guarded([&] {
+    t9::guess_set_params(ctx->c, params, norm, seed);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_get_params(t9_ctx* ctx, float* params, float* norm) {
+  return guarded([&] {
+    t9::guess_get_params(ctx->c, params, norm);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_enable(t9_ctx* ctx, int on) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (on && !c.guess_ready) throw t9::ArgError("the learned guess has no trained parameters");
+    if (on && c.opt.predictor != "ASPC") throw t9::ArgError("the learned guess needs polar-predict ASPC");
+    c.guess_on = on != 0;
+    c.guess_steps = 0;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_capture(t9_ctx* ctx, int stride, int64_t max_samples) {
+  return guarded([&] {
+    if (ctx->c.opt.predictor != "ASPC") throw t9::ArgError("capture needs polar-predict ASPC");
+    t9::guess_capture_start(ctx->c, stride, max_samples);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_stats(t9_ctx* ctx, int64_t* out4) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    out4[0] = c.cap_n;
+    out4[1] = c.guess_used;
+    out4[2] = c.guess_fallback;
+    out4[3] = c.last_predictor;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_train(t9_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss_first,
+                   double* loss_last) {
+  return guarded([&] {
+    t9::guess_train(ctx->c, epochs, batch, lr, seed, loss_first, loss_last);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_mlp_forward(t9_ctx* ctx, const float* X, int m, float* Y) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (m < 1 || !X || !Y) throw t9::ArgError("bad forward arguments");
+    if (!c.guess_params) throw t9::ArgError("the learned guess has no parameters");
+    t9::DBuf xf, xb, y;
+    xf.alloc(sizeof(float) * 32 * (size_t)m);
+    xb.alloc(sizeof(uint16_t) * 32 * (size_t)m);
+    y.alloc(sizeof(float) * 8 * (size_t)m);
+    T9_HIP(hipMemcpyAsync(xf.p, X, sizeof(float) * 32 * (size_t)m, hipMemcpyHostToDevice, c.stream));
+    t9::guess_to_bf16(c, 32LL * m, xf.as<float>(), xb.p);
+    t9::mlp_forward(c, m, xb.p, y.as<float>());
+    std::vector<float> h(8 * (size_t)m);
+    T9_HIP(hipMemcpyAsync(h.data(), y.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    for (int r = 0; r < m; ++r)
+      for (int q = 0; q < 6; ++q) Y[6 * (size_t)r + q] = h[8 * (size_t)r + q];
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_mlp_grad(t9_ctx* ctx, const float* X, const float* T, int m, float* grad, double* loss) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (m < 1 || !X || !T || !grad) throw t9::ArgError("bad gradient arguments");
+    if (!c.guess_params) throw t9::ArgError("the learned guess has no parameters");
+    const int np = t9::guess_nparam();
+    t9::DBuf xf, xb, tt, g;
+    xf.alloc(sizeof(float) * 32 * (size_t)m);
+    xb.alloc(sizeof(uint16_t) * 32 * (size_t)m);
+    tt.alloc(sizeof(float) * 6 * (size_t)m);
+    g.alloc(sizeof(float) * (np + 1));
+    T9_HIP(hipMemcpyAsync(xf.p, X, sizeof(float) * 32 * (size_t)m, hipMemcpyHostToDevice, c.stream));
+    T9_HIP(hipMemcpyAsync(tt.p, T, sizeof(float) * 6 * (size_t)m, hipMemcpyHostToDevice, c.stream));
+    t9::guess_to_bf16(c, 32LL * m, xf.as<float>(), xb.p);
+    t9::mlp_grad(c, m, xb.p, tt.as<float>(), g.as<float>());
+    std::vector<float> h(np + 1);
+    T9_HIP(hipMemcpyAsync(h.data(), g.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    std::copy(h.begin(), h.begin() + np, grad);
+    if (loss) *loss = h[np] / (6.0 * m);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_adam(t9_ctx* ctx, const float* grad, double lr) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!c.guess_params) throw t9::ArgError("the learned guess has no parameters");
+    const int np = t9::guess_nparam();
+    t9::DBuf g;
+    g.alloc(sizeof(float) * np);
+    T9_HIP(hipMemcpyAsync(g.p, grad, sizeof(float) * np, hipMemcpyHostToDevice, c.stream));
+    t9::adam_step(c, g.as<float>(), lr);
+    T9_HIP(hipStreamSynchronize(c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_features(t9_ctx* ctx, float* F) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (c.opt.predictor != "ASPC") throw t9::ArgError("features need polar-predict ASPC");
+    if (c.nualt < 16 || c.nealt < 16) throw t9::ArgError("features need 16 stored solves");
+    c.mlist.cut_valid = c.ulist.cut_valid = false;
+    c.bins_valid = false;
+    c.rpole_valid = false;
+    t9::prepare(c);
+    t9::dfield_all(c);
+    const auto B = t9::aspc_coefficients();
+    double ce[16], cu[2];
+    for (int k = 0; k < 16; ++k) ce[k] = B[((c.nealt - 1 - k) % 16 + 16) % 16];
+    cu[0] = ((c.nualt - 1) % 16 + 16) % 16;
+    cu[1] = ((c.nualt - 2) % 16 + 16) % 16;
+    t9::guess_features(c, ce, cu);
+    std::vector<float> h(32 * (size_t)c.n);
+    std::vector<int> perm(c.n);
+    T9_HIP(hipMemcpyAsync(h.data(), c.gfeat.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipMemcpyAsync(perm.data(), c.perm.p, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < c.n; ++s)
+      for (int q = 0; q < 32; ++q) F[32 * (size_t)perm[s] + q] = h[32 * (size_t)s + q];
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_guess_history(t9_ctx* ctx, int kind, int age, double* out) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (kind < 0 || kind > 3 || age < 0 || age > 15) throw t9::ArgError("bad history request");
+    const int count = kind < 2 ? c.nealt : c.nualt;
+    if (age >= std::min(count, 16)) throw t9::ArgError("history entry not stored yet");
+    const int slot = ((count - 1 - age) % 16 + 16) % 16;
+    const size_t n3 = 3 * (size_t)c.n;
+    const t9::DBuf& b = kind < 2 ? c.ealt : c.ualt;
+    const size_t off = (size_t)slot * 2 * n3 + ((kind & 1) ? n3 : 0);
+    if (c.prec == 32) {
+      std::vector<float> h(n3);
+      T9_HIP(hipMemcpyAsync(h.data(), b.as<float>() + off, sizeof(float) * n3, hipMemcpyDeviceToHost, c.stream));
+      T9_HIP(hipStreamSynchronize(c.stream));
+      for (size_t q = 0; q < n3; ++q) out[q] = h[q];
+    } else {
+      T9_HIP(hipMemcpyAsync(out, b.as<double>() + off, sizeof(double) * n3, hipMemcpyDeviceToHost, c.stream));
+      T9_HIP(hipStreamSynchronize(c.stream));
     }
     return T9_OK;
   }, T9_ERR_HIP);

@@ -169,8 +169,32 @@ struct Ctx {
   // ---- induced-dipole predictor: the last 16 solutions, caller order
   DBuf ualt;                     // real [16][2][n][3]
   int nualt = 0;                 // solutions stored so far (wraps between 16 and 32)
-  int last_predictor = 0;        // 0 direct guess, 1 ASPC
+  int last_predictor = 0;        // 0 direct guess, 1 ASPC, 2 learned
   double last_guess_rms = 0;     // RMS residual of the last initial guess (Debye)
+  DBuf frot;                     // float [n][9] sorted: rows = local x, y, z axes (written by rotpole)
+
+  // ---- learned initial guess (guess.hip)
+  DBuf ealt;                     // real [16][2][n][3] caller order: direct fields of the last 16 solves
+  int nealt = 0;
+  DBuf uaspc;                    // real [2][n][3] sorted: ASPC guess of the current solve
+  DBuf gfeat;                    // float [n][32] sorted raw features of the current solve
+  DBuf gx;                       // bf16 [n][32] normalized features
+  DBuf gy;                       // float [n][8] network outputs
+  DBuf gp, gm, gv, gpb, ggrad;   // FP32 params / Adam moments, bf16 copy, gradient
+  DBuf gpart;                    // float partial gradients [blocks][nparam + 1]
+  DBuf gnorm;                    // float [76]: feature mean, 1/std, target mean, std
+  std::vector<float> gnorm_h;
+  bool guess_params = false;     // parameters set (loaded or initialized)
+  bool guess_ready = false;      // parameters and normalization present
+  bool guess_on = false;         // use the learned guess when the history allows
+  int adam_t = 0;
+  double aspc_eps0_ref = 0;      // ASPC guess residual at the last reference check (Debye)
+  int guess_steps = 0;           // learned solves since the last reference check
+  long long guess_used = 0, guess_fallback = 0;
+  // training capture: raw features and local-frame targets of every stride-th atom
+  DBuf cap_x, cap_y;             // float [cap_max][32], [cap_max][6]
+  long long cap_n = 0, cap_max = 0;
+  int cap_stride = 0;
 
   // ---- vdW and valence (MD terms)
   bool use_vdw = false, use_valence = false;
@@ -224,6 +248,23 @@ void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp
 void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();
+// learned guess (guess.hip): nparam of the network, allocation, per-solve hooks
+int guess_nparam();
+void guess_setup(Ctx& c);
+bool guess_wants_features(const Ctx& c);        // capture on or learned guess usable
+void guess_features(Ctx& c, const double* coef_e, const double* coef_u);  // raw features -> c.gfeat
+void guess_save_fields(Ctx& c);                 // current direct field -> history
+void guess_apply(Ctx& c, void* ud, void* up);   // u += network correction (local -> global)
+void guess_capture(Ctx& c);                     // after the solve: features + targets of this step
+void guess_set_params(Ctx& c, const float* params, const float* norm, uint64_t seed);
+void guess_get_params(Ctx& c, float* params, float* norm);
+void guess_to_bf16(Ctx& c, long long count, const float* src_dev, void* dst_dev);
+void mlp_forward(Ctx& c, int m, const void* X_bf16_dev, float* Y_dev);  // Y [m][8]
+double mlp_grad(Ctx& c, int m, const void* X_bf16_dev, const float* T_dev, float* grad_dev);  // grad [nparam + 1]
+void adam_step(Ctx& c, const float* grad_dev, double lr);
+void guess_capture_start(Ctx& c, int stride, long long max_samples);
+void guess_fit_norm(Ctx& c);
+void guess_train(Ctx& c, int epochs, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp);
 // spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
 // 2: multipoles src1 (real) + dipoles src2 (imaginary)
@@ -238,7 +279,7 @@ void md_setup(Ctx& c, const System& s);   // vdW / valence tables, MD buffers
 void vdw_sites(Ctx& c);                   // vdW site positions (sorted) from the coordinates
 // vdW (sorted egrad += ..) and valence (caller-order gval) terms of one
 // evaluation; partial rows reserved from the energy rows
-void vdw_run(Ctx& c);
+void vdw_run(Ctx& c, bool virial);
 void valence_run(Ctx& c, double* gval_caller);
 // Term groups of one evaluation: the electrostatics (multipole + polarization)
 // and vdW are the slow group of the RESPA split, valence the fast group.

@@ -328,83 +328,6 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
       pair_terms<real, DM, DP>(R, bn, BT, mi, mk, ui, uk, udi, upi, udk, upk, (real)1, o);
       acc.add(o, R);
     }
-    // Pairs with a scale factor (the scaled partners inside the cutoff): the
-    // whole pair term, Ewald part plus (s - 1) non-Ewald correction, in FP64
-    // from the exact displacement.  These are the large intramolecular terms,
-    // alike from one molecule to the next, so FP32 rounding in them would not
-    // average out over the system.
-    const int io = A.perm[s];
-    Mp<double> mid;
-    mid.q = mi.q;
-    for (int t = 0; t < 3; ++t) mid.d[t] = mi.d[t];
-    for (int t = 0; t < 6; ++t) mid.Q[t] = mi.Q[t];
-    double udd[3], upd[3], uid[3];
-    for (int t = 0; t < 3; ++t) {
-      udd[t] = udi[t];
-      upd[t] = upi[t];
-      uid[t] = udd[t] + upd[t];
-    }
-    for (int q = A.excl_off[io]; q < A.excl_off[io + 1]; ++q) {
-      const int code = A.excl_code[q];
-      const double wm = A.sc.m[code] - 1, wd = A.sc.d[code] - 1, wp = A.sc.p[code] - 1, wu = A.sc.u[code] - 1;
-      const int j = A.iperm[A.excl_j[q]];
-      const PosRec<real> pj = A.pos[j];
-      // same displacement and test as the in-cutoff compaction
-      real xr, yr, zr;
-      pair_disp(pi, pj, A.box, xr, yr, zr);
-      if (!(xr * xr + yr * yr + zr * zr <= A.cut2)) continue;
-      double R[3];
-      pair_disp64(pj, pi, A.box, R);  // r_i - r_k
-      const double r2 = dot3(R, R);
-      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
-      double rr[6], bn[6], BM[6];
-      rr[0] = rinv;
-#pragma unroll
-      for (int t = 1; t < 6; ++t) rr[t] = rr[t - 1] * (double)(2 * t - 1) * rr2;
-      if (EWALD) {
-        double ex;
-        bn[0] = t9_erfc_pair(A.ek64.a * r, ex) * rinv;
-#pragma unroll
-        for (int t = 1; t < 6; ++t) bn[t] = ((double)(2 * t - 1) * bn[t - 1] + A.ek64.e[t] * ex) * rr2;
-      } else {
-#pragma unroll
-        for (int t = 0; t < 6; ++t) bn[t] = rr[t];
-      }
-#pragma unroll
-      for (int t = 0; t < 6; ++t) BM[t] = bn[t] + wm * rr[t];
-      const real* pk = A.rpole + 10 * (size_t)j;
-      Mp<double> mkd;
-      mkd.q = pk[0];
-      for (int t = 0; t < 3; ++t) mkd.d[t] = pk[1 + t];
-      for (int t = 0; t < 6; ++t) mkd.Q[t] = pk[4 + t];
-      double BT[5] = {0, 0, 0, 0, 0}, BTs[5] = {0, 0, 0, 0, 0};
-      double udk[3] = {0, 0, 0}, upk[3] = {0, 0, 0}, ukd[3] = {0, 0, 0}, uie[3] = {0, 0, 0}, uke[3] = {0, 0, 0};
-      if (DP) {
-        const double pga = A.uth ? (double)A.pga_u : (double)A.thlval[jpi + A.jpolar[j]];
-        double lam[5];
-        thole4(r, (double)pi.w * (double)pj.w, pga, lam);
-#pragma unroll
-        for (int t = 1; t < 5; ++t) {
-          BT[t] = bn[t] - (1 - lam[t]) * rr[t];
-          BTs[t] = lam[t] * rr[t];
-        }
-        for (int t = 0; t < 3; ++t) {
-          udk[t] = A.ud[3 * j + t];
-          upk[t] = A.up[3 * j + t];
-          ukd[t] = udk[t] + upk[t];
-          uie[t] = wd * upd[t] + wp * udd[t];
-          uke[t] = wd * upk[t] + wp * udk[t];
-        }
-      }
-      PairOut<double> o;
-      pair_terms<double, DM, DP>(R, BM, BT, mid, mkd, uid, ukd, udd, upd, udk, upk, 1.0, o);
-      acc.add(o, R);
-      if (DP && (wd != 0 || wp != 0 || wu != 0)) {
-        const double Z[6] = {0, 0, 0, 0, 0, 0};
-        pair_terms<double, DM, DP>(R, Z, BTs, mid, mkd, uie, uke, udd, upd, udk, upk, wu, o);
-        acc.add(o, R);
-      }
-    }
     const double f = kCoul;
     for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] = f * acc.gr[k];
     for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] = f * acc.tg[k];
@@ -856,6 +779,108 @@ __global__ void k_grad_to_orig(int n, const int* __restrict__ perm, const double
     dst[3 * (size_t)i + d] = src[3 * (size_t)s + d] + (add ? add[3 * (size_t)i + d] : 0.0);
 }
 
+// The scaled partners of each atom (inside the cutoff): each whole pair term,
+// unscaled Ewald part plus the (s - 1) non-Ewald correction, in FP64 from the
+// exact displacement, whatever the working precision.  These are the large
+// intramolecular terms, alike from one molecule to the next, so FP32 rounding
+// in them would not average out over the system.  Runs after k_emplar (a
+// kernel of its own so the FP64 registers do not lower the occupancy of the
+// list loop) and adds to the same lane-owned gradient and torque terms.
+template <class real, int EWALD, int MODE>
+__global__ __launch_bounds__(256) void k_emplar_scaled(EmplarArgs<real> A) {
+  constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  Acc<real> acc;
+  acc.zero();
+  if (s < A.n) {
+    const int io = A.perm[s];
+    const PosRec<real> pi = A.pos[s];
+    const real* pp = A.rpole + 10 * (size_t)s;
+    Mp<double> mid;
+    mid.q = pp[0];
+    for (int t = 0; t < 3; ++t) mid.d[t] = pp[1 + t];
+    for (int t = 0; t < 6; ++t) mid.Q[t] = pp[4 + t];
+    double udd[3] = {0, 0, 0}, upd[3] = {0, 0, 0}, uid[3] = {0, 0, 0};
+    if (DP)
+      for (int t = 0; t < 3; ++t) {
+        udd[t] = A.ud[3 * s + t];
+        upd[t] = A.up[3 * s + t];
+        uid[t] = udd[t] + upd[t];
+      }
+    const int jpi = A.uth ? 0 : A.jpolar[s] * A.nj;
+    for (int q = A.excl_off[io]; q < A.excl_off[io + 1]; ++q) {
+      const int code = A.excl_code[q];
+      const double wm = A.sc.m[code] - 1, wd = A.sc.d[code] - 1, wp = A.sc.p[code] - 1, wu = A.sc.u[code] - 1;
+      const int j = A.iperm[A.excl_j[q]];
+      const PosRec<real> pj = A.pos[j];
+      // same displacement and test as the in-cutoff compaction
+      real xr, yr, zr;
+      pair_disp(pi, pj, A.box, xr, yr, zr);
+      if (!(xr * xr + yr * yr + zr * zr <= A.cut2)) continue;
+      double R[3];
+      pair_disp64(pj, pi, A.box, R);  // r_i - r_k
+      const double r2 = dot3(R, R);
+      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
+      double rr[6], bn[6], BM[6];
+      rr[0] = rinv;
+#pragma unroll
+      for (int t = 1; t < 6; ++t) rr[t] = rr[t - 1] * (double)(2 * t - 1) * rr2;
+      if (EWALD) {
+        double ex;
+        bn[0] = t9_erfc_pair(A.ek64.a * r, ex) * rinv;
+#pragma unroll
+        for (int t = 1; t < 6; ++t) bn[t] = ((double)(2 * t - 1) * bn[t - 1] + A.ek64.e[t] * ex) * rr2;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) bn[t] = rr[t];
+      }
+#pragma unroll
+      for (int t = 0; t < 6; ++t) BM[t] = bn[t] + wm * rr[t];
+      const real* pk = A.rpole + 10 * (size_t)j;
+      Mp<double> mkd;
+      mkd.q = pk[0];
+      for (int t = 0; t < 3; ++t) mkd.d[t] = pk[1 + t];
+      for (int t = 0; t < 6; ++t) mkd.Q[t] = pk[4 + t];
+      double BT[5] = {0, 0, 0, 0, 0}, BTs[5] = {0, 0, 0, 0, 0};
+      double udk[3] = {0, 0, 0}, upk[3] = {0, 0, 0}, ukd[3] = {0, 0, 0}, uie[3] = {0, 0, 0}, uke[3] = {0, 0, 0};
+      if (DP) {
+        const double pga = A.uth ? (double)A.pga_u : (double)A.thlval[jpi + A.jpolar[j]];
+        double lam[5];
+        thole4(r, (double)pi.w * (double)pj.w, pga, lam);
+#pragma unroll
+        for (int t = 1; t < 5; ++t) {
+          BT[t] = bn[t] - (1 - lam[t]) * rr[t];
+          BTs[t] = lam[t] * rr[t];
+        }
+        for (int t = 0; t < 3; ++t) {
+          udk[t] = A.ud[3 * j + t];
+          upk[t] = A.up[3 * j + t];
+          ukd[t] = udk[t] + upk[t];
+          uie[t] = wd * upd[t] + wp * udd[t];
+          uke[t] = wd * upk[t] + wp * udk[t];
+        }
+      }
+      PairOut<double> o;
+      pair_terms<double, DM, DP>(R, BM, BT, mid, mkd, uid, ukd, udd, upd, udk, upk, 1.0, o);
+      acc.add(o, R);
+      if (DP && (wd != 0 || wp != 0 || wu != 0)) {
+        const double Z[6] = {0, 0, 0, 0, 0, 0};
+        pair_terms<double, DM, DP>(R, Z, BTs, mid, mkd, uie, uke, udd, upd, udk, upk, wu, o);
+        acc.add(o, R);
+      }
+    }
+    const double f = kCoul;
+    for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] += f * acc.gr[k];
+    for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] += f * acc.tg[k];
+    for (int k = 0; k < 6; ++k) A.tq[9 * (size_t)s + 3 + k] += f * acc.th[k];
+  }
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  red[EP_MREAL] = 0.5 * kCoul * acc.e;
+  for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * kCoul * acc.v[k];
+  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kRow);
+}
+
 // ------------------------------------------------------------ host side
 
 template <class real>
@@ -910,8 +935,20 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   void (*kern)(EmplarArgs<real>) = nullptr;
   if (c.ewald) kern = mode == 1 ? k_emplar<real, 1, 1> : (mode == 2 ? k_emplar<real, 1, 2> : k_emplar<real, 1, 3>);
   else kern = mode == 1 ? k_emplar<real, 0, 1> : (mode == 2 ? k_emplar<real, 0, 2> : k_emplar<real, 0, 3>);
-  KScope ks_(c, KT_EMPLAR);
-  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, A);
+  void (*kscaled)(EmplarArgs<real>) = nullptr;
+  if (c.ewald)
+    kscaled = mode == 1 ? k_emplar_scaled<real, 1, 1>
+                        : (mode == 2 ? k_emplar_scaled<real, 1, 2> : k_emplar_scaled<real, 1, 3>);
+  else
+    kscaled = mode == 1 ? k_emplar_scaled<real, 0, 1>
+                        : (mode == 2 ? k_emplar_scaled<real, 0, 2> : k_emplar_scaled<real, 0, 3>);
+  {
+    KScope ks_(c, KT_EMPLAR);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, A);
+    T9_LAUNCH_CHECK();
+  }
+  A.rows = energy_rows(c, nb);
+  hipLaunchKernelGGL(kscaled, dim3(nb), dim3(256), 0, c.stream, A);
   T9_LAUNCH_CHECK();
 }
 
@@ -993,7 +1030,7 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   } else {
     T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
   }
-  if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c);
+  if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c, want_v);
   if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>());
   if (c.erows == 0) {
     for (int q = 0; q < 16; ++q) out16[q] = 0;
@@ -1047,13 +1084,13 @@ void energy_setup(Ctx& c, const System& s) {
   c.rev_ent.alloc(sizeof(int) * ent.size());
   T9_HIP(hipMemcpyAsync(c.rev_off.p, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c.stream));
   T9_HIP(hipMemcpyAsync(c.rev_ent.p, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice, c.stream));
-  // reduction rows: per-atom kernels (real space, 2 recip, self, torque, vdW,
-  // valence) plus up to three convolution passes of one row per z-pass
+  // reduction rows: per-atom kernels (real space list and scaled pairs, 2
+  // recip, self, torque, vdW, valence) plus up to three convolution passes of one row per z-pass
   // workgroup
   const int nb = ceil_div(n, 256);
   long long lines = 0;
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
-  c.ecap = (int)(8LL * nb + lines + 16);
+  c.ecap = (int)(9LL * nb + lines + 16);
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }

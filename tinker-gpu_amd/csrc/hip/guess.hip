@@ -1,0 +1,726 @@
+// Learned initial guess for the induced dipoles (gfx950, BF16 MFMA).
+//
+// No counterpart in the reference: it replaces the predictor at the plug
+// point of src/amoeba/induce.cpp:29-69 / src/cu/upredict.cu:34-201 (ASPC),
+// which stays as the fallback.  The guess is
+//     u0 = u_ASPC + R_i^T NN(x_i)          (d and p dipoles)
+// where R_i is the atom's multipole local frame and x_i 32 features in that
+// frame: the extrapolation error of the direct field (current field minus the
+// ASPC combination of the previous 16 fields) at the atom and at its two
+// frame atoms, the current field, the last dipole and its change, the
+// polarizability and a type one-hot.  Everything rotates with the frame, so
+// the guess is rotation-equivariant.
+//
+// Network: 32 -> 128 (SiLU) -> 128 (SiLU) -> 6, the last layer also seeing
+// the inputs (linear skip).  Inputs, weights and activations are BF16, every
+// product accumulates in FP32 on v_mfma_f32_32x32x16_bf16; one wavefront
+// owns 32 rows and stages its activations in LDS.  Training: the backward
+// pass in the same arithmetic, per-workgroup partial weight gradients
+// reduced in a fixed order (bit-reproducible), Adam on FP32 master weights.
+// Training data: during MD, every stride-th atom's raw features and its
+// converged correction R_i (u - u_ASPC) are appended to a device buffer.
+#include <algorithm>
+#include <cmath>
+#include <random>
+
+#include "ctx.hpp"
+
+namespace t9 {
+namespace {
+
+constexpr int NF = 32, NH = 128, NO = 6, NSK = NH + NF;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NF * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NSK * NO, NPARAM = OFF_B3 + NO;
+constexpr int kNorm = 2 * NF + 2 * NO;  // feature mean, 1/std; target mean, std
+
+typedef unsigned short u16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// One wavefront: acc += A(32 x K) B(K x 32) with A(m, k) = A[m*ars + k*acs],
+// B(k, n) = B[k*brs + n*bcs]; entries with k >= kv or n >= nv read as zero.
+// K is a multiple of 16.  Operand layout of v_mfma_f32_32x32x16_bf16: lane l
+// holds A(l % 32, 8 (l / 32) + i) and B(8 (l / 32) + i, l % 32), i = 0..7.
+// Result: lane l, register j holds C(8 (j / 4) + 4 (l / 32) + j % 4, l % 32).
+__device__ __forceinline__ f32x16 mm32(const u16* A, int ars, int acs, const u16* B, int brs, int bcs, int K, int kv,
+                                       int nv, f32x16 acc) {
+  const int l = threadIdx.x & 63, r = l & 31, kh = (l >> 5) << 3;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    s16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = k0 + kh + i;
+      a[i] = k < kv ? (short)A[r * ars + k * acs] : (short)0;
+      b[i] = (k < kv && r < nv) ? (short)B[k * brs + r * bcs] : (short)0;
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                  0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ int mrow(int j) { return 8 * (j >> 2) + 4 * ((threadIdx.x & 63) >> 5) + (j & 3); }
+__device__ __forceinline__ int ncol() { return threadIdx.x & 31; }
+
+struct Net {
+  const u16* w1;  // [32][128] bf16
+  const float* b1;
+  const u16* w2;  // [128][128]
+  const float* b2;
+  const u16* w3;  // [160][6]
+  const float* b3;
+};
+
+__device__ __forceinline__ Net net_of(const u16* pb, const float* p) {
+  Net t;
+  t.w1 = pb + OFF_W1;
+  t.b1 = p + OFF_B1;
+  t.w2 = pb + OFF_W2;
+  t.b2 = p + OFF_B2;
+  t.w3 = pb + OFF_W3;
+  t.b3 = p + OFF_B3;
+  return t;
+}
+
+// Forward pass of the wave's 32 rows from sX (bf16 [32][32]); activations to
+// sH1 / sH2 (bf16 [32][128]), pre-activations to sZ1 / sZ2 when given.
+// Returns the output accumulator (columns 0..5 valid, bias added).
+__device__ f32x16 wave_forward(const Net& w, const u16* sX, u16* sH1, u16* sH2, float* sZ1, float* sZ2) {
+  const f32x16 zero = {0};
+  for (int t = 0; t < 4; ++t) {
+    const f32x16 acc = mm32(sX, NF, 1, w.w1 + 32 * t, NH, 1, NF, NF, 32, zero);
+    const int n = 32 * t + ncol();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float z = acc[j] + w.b1[n];
+      if (sZ1) sZ1[mrow(j) * NH + n] = z;
+      sH1[mrow(j) * NH + n] = f2bf(silu(z));
+    }
+  }
+  __syncthreads();
+  for (int t = 0; t < 4; ++t) {
+    const f32x16 acc = mm32(sH1, NH, 1, w.w2 + 32 * t, NH, 1, NH, NH, 32, zero);
+    const int n = 32 * t + ncol();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float z = acc[j] + w.b2[n];
+      if (sZ2) sZ2[mrow(j) * NH + n] = z;
+      sH2[mrow(j) * NH + n] = f2bf(silu(z));
+    }
+  }
+  __syncthreads();
+  f32x16 acc = mm32(sH2, NH, 1, w.w3, NO, 1, NH, NH, NO, zero);
+  acc = mm32(sX, NF, 1, w.w3 + NH * NO, NO, 1, NF, NF, NO, acc);
+  const float b = ncol() < NO ? w.b3[ncol()] : 0.0f;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] += b;
+  return acc;
+}
+
+// Y[m][8] (columns 0..5) = network(X[m][32]); one wavefront per 32 rows.
+__global__ __launch_bounds__(64) void k_mlp_fwd(int m, const u16* __restrict__ X, const u16* __restrict__ pb,
+                                                const float* __restrict__ p, float* __restrict__ Y) {
+  __shared__ u16 sX[32 * NF], sH1[32 * NH], sH2[32 * NH];
+  const int row0 = blockIdx.x * 32;
+  for (int e = threadIdx.x; e < 32 * NF; e += 64) {
+    const int rr = e / NF;
+    sX[e] = row0 + rr < m ? X[(size_t)(row0 + rr) * NF + (e % NF)] : (u16)0;
+  }
+  __syncthreads();
+  const f32x16 acc = wave_forward(net_of(pb, p), sX, sH1, sH2, nullptr, nullptr);
+  const int n = ncol();
+  if (n < NO)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = row0 + mrow(j);
+      if (r < m) Y[(size_t)r * 8 + n] = acc[j];
+    }
+}
+
+// Per-workgroup partial gradient of the mean squared error over `mtot`
+// rows (this workgroup: rows row0 .. row0+31 of X, T) and the partial sum of
+// squared errors, written to part[blockIdx.x][NPARAM + 1].
+__global__ __launch_bounds__(64) void k_mlp_bwd(int m, int mtot, const u16* __restrict__ X,
+                                                const float* __restrict__ T, const u16* __restrict__ pb,
+                                                const float* __restrict__ p, float* __restrict__ part) {
+  __shared__ u16 sX[32 * NF], sH1[32 * NH], sH2[32 * NH], sdY[32 * 32];
+  __shared__ float sZ1[32 * NH], sZ2[32 * NH], sred[64];
+  const int row0 = blockIdx.x * 32;
+  const int l = threadIdx.x;
+  float* gp = part + (size_t)blockIdx.x * (NPARAM + 1);
+  for (int e = l; e < 32 * NF; e += 64) {
+    const int rr = e / NF;
+    sX[e] = row0 + rr < m ? X[(size_t)(row0 + rr) * NF + (e % NF)] : (u16)0;
+  }
+  __syncthreads();
+  const Net w = net_of(pb, p);
+  const f32x16 y = wave_forward(w, sX, sH1, sH2, sZ1, sZ2);
+  const f32x16 zero = {0};
+  const int n = ncol();
+  // dY = 2 (Y - T) / (mtot * 6); loss partial
+  const float sc = 2.0f / ((float)mtot * NO);
+  float lsum = 0, bsum = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int r = row0 + mrow(j);
+    float d = 0;
+    if (n < NO && r < m) {
+      const float e = y[j] - T[(size_t)r * NO + n];
+      lsum += e * e;
+      d = sc * e;
+    }
+    bsum += d;
+    sdY[mrow(j) * 32 + n] = f2bf(d);
+  }
+  // bias 3 and loss: column sums over the two lane halves
+  sred[l] = bsum;
+  __syncthreads();
+  if (l < NO) gp[OFF_B3 + l] = sred[l] + sred[l + 32];
+  __syncthreads();
+  sred[l] = lsum;
+  __syncthreads();
+  if (l == 0) {
+    float t = 0;
+    for (int q = 0; q < 64; ++q) t += sred[q];
+    gp[NPARAM] = t;
+  }
+  // dW3 = [H2 X]^T dY  (160 x 6)
+  for (int t = 0; t < 5; ++t) {
+    const f32x16 acc = t < 4 ? mm32(sH2 + 32 * t, 1, NH, sdY, 32, 1, 32, 32, 32, zero)
+                             : mm32(sX, 1, NF, sdY, 32, 1, 32, 32, 32, zero);
+    if (n < NO)
+#pragma unroll
+      for (int j = 0; j < 16; ++j) gp[OFF_W3 + (32 * t + mrow(j)) * NO + n] = acc[j];
+  }
+  __syncthreads();
+  // dZ2 = (dY W3[:128]^T) * silu'(Z2) -> sH2 (bf16); bias 2
+  for (int t = 0; t < 4; ++t) {
+    const f32x16 acc = mm32(sdY, 32, 1, w.w3 + 32 * t * NO, 1, NO, 32, NO, 32, zero);
+    const int h = 32 * t + n;
+    float bs = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float d = acc[j] * dsilu(sZ2[mrow(j) * NH + h]);
+      bs += d;
+      sZ2[mrow(j) * NH + h] = d;
+    }
+    sred[l] = bs;
+    __syncthreads();
+    if (l < 32) gp[OFF_B2 + 32 * t + l] = sred[l] + sred[l + 32];
+    __syncthreads();
+  }
+  for (int e = l; e < 32 * NH; e += 64) sH2[e] = f2bf(sZ2[e]);
+  __syncthreads();
+  // dW2 = H1^T dZ2  (128 x 128)
+  for (int tm = 0; tm < 4; ++tm)
+    for (int tn = 0; tn < 4; ++tn) {
+      const f32x16 acc = mm32(sH1 + 32 * tm, 1, NH, sH2 + 32 * tn, NH, 1, 32, 32, 32, zero);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) gp[OFF_W2 + (32 * tm + mrow(j)) * NH + 32 * tn + n] = acc[j];
+    }
+  __syncthreads();
+  // dZ1 = (dZ2 W2^T) * silu'(Z1) -> sH1 (bf16); bias 1
+  for (int t = 0; t < 4; ++t) {
+    const f32x16 acc = mm32(sH2, NH, 1, w.w2 + 32 * t * NH, 1, NH, NH, NH, 32, zero);
+    const int h = 32 * t + n;
+    float bs = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float d = acc[j] * dsilu(sZ1[mrow(j) * NH + h]);
+      bs += d;
+      sZ1[mrow(j) * NH + h] = d;
+    }
+    sred[l] = bs;
+    __syncthreads();
+    if (l < 32) gp[OFF_B1 + 32 * t + l] = sred[l] + sred[l + 32];
+    __syncthreads();
+  }
+  for (int e = l; e < 32 * NH; e += 64) sH1[e] = f2bf(sZ1[e]);
+  __syncthreads();
+  // dW1 = X^T dZ1  (32 x 128)
+  for (int tn = 0; tn < 4; ++tn) {
+    const f32x16 acc = mm32(sX, 1, NF, sH1 + 32 * tn, NH, 1, 32, 32, 32, zero);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) gp[OFF_W1 + mrow(j) * NH + 32 * tn + n] = acc[j];
+  }
+}
+
+// g[q] = sum over workgroups of part[w][q], in workgroup order (q = NPARAM: loss)
+__global__ void k_grad_reduce(int nwg, const float* __restrict__ part, float* __restrict__ g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > NPARAM) return;
+  float s = 0;
+  for (int w = 0; w < nwg; ++w) s += part[(size_t)w * (NPARAM + 1) + q];
+  g[q] = s;
+}
+
+// Adam (bias-corrected), FP32 master parameters; refreshes the bf16 copy.
+__global__ void k_adam(const float* __restrict__ g, float* __restrict__ p, float* __restrict__ mo,
+                       float* __restrict__ ve, u16* __restrict__ pb, float lr, float c1, float c2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= NPARAM) return;
+  const float gq = g[q];
+  const float m = 0.9f * mo[q] + 0.1f * gq;
+  const float v = 0.999f * ve[q] + 0.001f * gq * gq;
+  mo[q] = m;
+  ve[q] = v;
+  const float mh = m / c1, vh = v / c2;
+  const float pq = p[q] - lr * mh / (sqrtf(vh) + 1e-8f);
+  p[q] = pq;
+  pb[q] = f2bf(pq);
+}
+
+__global__ void k_to_bf16(int np, const float* __restrict__ p, u16* __restrict__ pb) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) pb[q] = f2bf(p[q]);
+}
+
+// ------------------------------------------------------------ features
+
+struct Coef16f {
+  double c[16];
+};
+
+__device__ __forceinline__ void rot3(const float* R, const float* v, float* o) {
+  for (int k = 0; k < 3; ++k) o[k] = R[3 * k] * v[0] + R[3 * k + 1] * v[1] + R[3 * k + 2] * v[2];
+}
+
+// dE (d and p) of caller atom j: current field (sorted slot sj) minus the
+// ASPC combination of its history (slots weighted by ce)
+template <class real>
+__device__ void field_err(int n, int j, int sj, const real* fd, const real* fp, const real* ealt, const Coef16f& ce,
+                          float* dd, float* dp) {
+  const size_t slot = 6 * (size_t)n;
+  for (int d = 0; d < 3; ++d) {
+    double a = (double)fd[3 * sj + d], b = (double)fp[3 * sj + d];
+    for (int k = 0; k < 16; ++k) {
+      a -= ce.c[k] * (double)ealt[k * slot + 3 * (size_t)j + d];
+      b -= ce.c[k] * (double)ealt[k * slot + 3 * (size_t)n + 3 * (size_t)j + d];
+    }
+    dd[d] = (float)a;
+    dp[d] = (float)b;
+  }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_features(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
+                                                  const int* __restrict__ frame, const int* __restrict__ zaxis,
+                                                  const int* __restrict__ xaxis, const float* __restrict__ frot,
+                                                  const real* __restrict__ polarity, const int* __restrict__ jpolar,
+                                                  const real* __restrict__ fd, const real* __restrict__ fp,
+                                                  const real* __restrict__ ealt, Coef16f ce,
+                                                  const real* __restrict__ ualt, int u0, int u1,
+                                                  float* __restrict__ F) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  const int fr = frame[i];
+  const int zi = fr > 0 ? zaxis[i] : i;
+  const int xi = (fr > 1 && xaxis[i] >= 0) ? xaxis[i] : i;
+  float R[9];
+  for (int q = 0; q < 9; ++q) R[q] = frot[9 * (size_t)s + q];
+  float* f = F + (size_t)s * NF;
+  float a[3], b[3];
+  const int js[3] = {i, zi, xi};
+  for (int t = 0; t < 3; ++t) {
+    field_err<real>(n, js[t], iperm[js[t]], fd, fp, ealt, ce, a, b);
+    rot3(R, a, f + 6 * t);
+    rot3(R, b, f + 6 * t + 3);
+  }
+  float e[3];
+  for (int d = 0; d < 3; ++d) e[d] = (float)fd[3 * s + d];
+  rot3(R, e, f + 18);
+  const size_t slot = 6 * (size_t)n;
+  float du[3], u[3];
+  for (int d = 0; d < 3; ++d) {
+    const float x0 = (float)ualt[u0 * slot + 3 * (size_t)i + d], x1 = (float)ualt[u1 * slot + 3 * (size_t)i + d];
+    du[d] = x0 - x1;
+    u[d] = x0;
+  }
+  rot3(R, du, f + 21);
+  rot3(R, u, f + 24);
+  f[27] = (float)polarity[s];
+  for (int q = 0; q < 4; ++q) f[28 + q] = 0.0f;
+  f[28 + min(jpolar[s], 3)] = 1.0f;
+}
+
+__global__ void k_normalize(int m, const float* __restrict__ F, const float* __restrict__ norm,
+                            u16* __restrict__ X) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= (long long)m * NF) return;
+  const int c = (int)(e % NF);
+  X[e] = f2bf((F[e] - norm[c]) * norm[NF + c]);
+}
+
+// u += R^T (Y * std + mean), d and p
+template <class real>
+__global__ void k_apply(int n, const float* __restrict__ frot, const float* __restrict__ Y,
+                        const float* __restrict__ norm, real* __restrict__ ud, real* __restrict__ up) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float* R = frot + 9 * (size_t)s;
+  float y[6];
+  for (int q = 0; q < 6; ++q) y[q] = Y[(size_t)s * 8 + q] * norm[2 * NF + NO + q] + norm[2 * NF + q];
+  for (int g = 0; g < 3; ++g) {
+    ud[3 * s + g] += (real)(R[g] * y[0] + R[3 + g] * y[1] + R[6 + g] * y[2]);
+    up[3 * s + g] += (real)(R[g] * y[3] + R[3 + g] * y[4] + R[6 + g] * y[5]);
+  }
+}
+
+// history slot <- current direct field (sorted -> caller order)
+template <class real>
+__global__ void k_save_field(int n, const int* __restrict__ perm, const real* __restrict__ fd,
+                             const real* __restrict__ fp, real* __restrict__ slot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  for (int d = 0; d < 3; ++d) {
+    slot[3 * (size_t)i + d] = fd[3 * s + d];
+    slot[3 * (size_t)n + 3 * (size_t)i + d] = fp[3 * s + d];
+  }
+}
+
+// capture every stride-th caller atom: raw features and R (u - u_aspc)
+template <class real>
+__global__ void k_capture(int n, int stride, long long base, const int* __restrict__ perm,
+                          const float* __restrict__ F, const float* __restrict__ frot, const real* __restrict__ ud,
+                          const real* __restrict__ up, const real* __restrict__ ua, float* __restrict__ cx,
+                          float* __restrict__ cy) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  if (i % stride) return;
+  const long long slot = base + i / stride;
+  for (int q = 0; q < NF; ++q) cx[slot * NF + q] = F[(size_t)s * NF + q];
+  const float* R = frot + 9 * (size_t)s;
+  float a[3], b[3];
+  for (int d = 0; d < 3; ++d) {
+    a[d] = (float)((double)ud[3 * s + d] - (double)ua[3 * s + d]);
+    b[d] = (float)((double)up[3 * s + d] - (double)ua[3 * (size_t)n + 3 * s + d]);
+  }
+  rot3(R, a, cy + slot * NO);
+  rot3(R, b, cy + slot * NO + 3);
+}
+
+// column sums and sums of squares of rows [r0, r1) of a [rows][w] matrix,
+// one block per chunk: out[blk][2w]
+__global__ __launch_bounds__(256) void k_colstats(long long rows, int w, const float* __restrict__ A,
+                                                  long long chunk, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const long long r0 = blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
+  for (int c = 0; c < w; ++c) {
+    double s1 = 0, s2 = 0;
+    for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
+      const double v = A[r * w + c];
+      s1 += v;
+      s2 += v * v;
+    }
+    double t1 = block_sum<256>(s1, sh);
+    double t2 = block_sum<256>(s2, sh);
+    if (threadIdx.x == 0) {
+      out[(size_t)blockIdx.x * 2 * w + c] = t1;
+      out[(size_t)blockIdx.x * 2 * w + w + c] = t2;
+    }
+  }
+}
+
+// training batch: rows idx = (a * (start + b) + c) mod ncap
+__global__ void k_make_batch(int m, long long start, unsigned long long a, unsigned long long c, long long ncap,
+                             const float* __restrict__ cx, const float* __restrict__ cy,
+                             const float* __restrict__ norm, u16* __restrict__ X, float* __restrict__ T) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= m) return;
+  const unsigned long long idx = (unsigned long long)(((unsigned __int128)a * (unsigned long long)(start + b) + c) %
+                                                      (unsigned long long)ncap);
+  for (int q = 0; q < NF; ++q) X[(size_t)b * NF + q] = f2bf((cx[idx * NF + q] - norm[q]) * norm[NF + q]);
+  for (int q = 0; q < NO; ++q)
+    T[(size_t)b * NO + q] = (cy[idx * NO + q] - norm[2 * NF + q]) / norm[2 * NF + NO + q];
+}
+
+template <class real>
+void features_t(Ctx& c, const double* ce, const double* cu) {
+  Coef16f cf;
+  for (int k = 0; k < 16; ++k) cf.c[k] = ce[k];
+  const int u0 = (int)cu[0], u1 = (int)cu[1];
+  c.gfeat.alloc(sizeof(float) * NF * (size_t)c.n);
+  hipLaunchKernelGGL(k_features<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
+                     c.iperm.as<int>(), c.frame.as<int>(), c.zaxis.as<int>(), c.xaxis.as<int>(), c.frot.as<float>(),
+                     c.polarity.as<real>(), c.jpolar.as<int>(), c.field.as<real>(), c.fieldp.as<real>(),
+                     c.ealt.as<real>(), cf, c.ualt.as<real>(), u0, u1, c.gfeat.as<float>());
+  T9_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+int guess_nparam() { return NPARAM; }
+
+void guess_setup(Ctx& c) {
+  c.gp.alloc(sizeof(float) * (NPARAM + 1));
+  c.gm.alloc(sizeof(float) * NPARAM);
+  c.gv.alloc(sizeof(float) * NPARAM);
+  c.gpb.alloc(sizeof(u16) * NPARAM);
+  c.ggrad.alloc(sizeof(float) * (NPARAM + 1));
+  c.gnorm.alloc(sizeof(float) * kNorm);
+  c.ealt.alloc(rsize(c) * 6 * (size_t)c.n * 16);
+  c.uaspc.alloc(rsize(c) * 6 * (size_t)c.n);
+}
+
+bool guess_wants_features(const Ctx& c) { return c.cap_stride > 0 || (c.guess_on && c.guess_ready); }
+
+// coef_e[k]: weight of history slot k of the fields; coef_u[0], [1]: the
+// slots of u(t-1), u(t-2)
+void guess_features(Ctx& c, const double* coef_e, const double* coef_u) {
+  if (c.prec == 32) features_t<float>(c, coef_e, coef_u);
+  else features_t<double>(c, coef_e, coef_u);
+}
+
+void guess_save_fields(Ctx& c) {
+  const size_t slot = 6 * (size_t)c.n;
+  const int k = c.nealt % 16;
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_save_field<float>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
+                       c.field.as<float>(), c.fieldp.as<float>(), c.ealt.as<float>() + k * slot);
+  else
+    hipLaunchKernelGGL(k_save_field<double>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                       c.perm.as<int>(), c.field.as<double>(), c.fieldp.as<double>(), c.ealt.as<double>() + k * slot);
+  T9_LAUNCH_CHECK();
+  c.nealt += 1;
+  if (c.nealt > 32) c.nealt -= 16;
+}
+
+void guess_to_bf16(Ctx& c, long long count, const float* src, void* dst) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(ceil_div(count, 256)), dim3(256), 0, c.stream, (int)count, src, (u16*)dst);
+  T9_LAUNCH_CHECK();
+}
+
+void mlp_forward(Ctx& c, int m, const void* X, float* Y) {
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(ceil_div(m, 32)), dim3(64), 0, c.stream, m, (const u16*)X, c.gpb.as<u16>(),
+                     c.gp.as<float>(), Y);
+  T9_LAUNCH_CHECK();
+}
+
+void guess_apply(Ctx& c, void* ud, void* up) {
+  const int n = c.n;
+  c.gx.alloc(sizeof(u16) * NF * (size_t)n);
+  c.gy.alloc(sizeof(float) * 8 * (size_t)n);
+  hipLaunchKernelGGL(k_normalize, dim3(ceil_div((long)n * NF, 256)), dim3(256), 0, c.stream, n, c.gfeat.as<float>(),
+                     c.gnorm.as<float>(), c.gx.as<u16>());
+  T9_LAUNCH_CHECK();
+  mlp_forward(c, n, c.gx.p, c.gy.as<float>());
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_apply<float>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.frot.as<float>(),
+                       c.gy.as<float>(), c.gnorm.as<float>(), (float*)ud, (float*)up);
+  else
+    hipLaunchKernelGGL(k_apply<double>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.frot.as<float>(),
+                       c.gy.as<float>(), c.gnorm.as<float>(), (double*)ud, (double*)up);
+  T9_LAUNCH_CHECK();
+}
+
+void guess_capture(Ctx& c) {
+  if (c.cap_stride <= 0) return;
+  const int n = c.n;
+  const long long per = (n + c.cap_stride - 1) / c.cap_stride;
+  if (c.cap_n + per > c.cap_max) {  // buffer full: capture ends
+    c.cap_stride = 0;
+    return;
+  }
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_capture<float>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.cap_stride, c.cap_n,
+                       c.perm.as<int>(), c.gfeat.as<float>(), c.frot.as<float>(), c.uind.as<float>(),
+                       c.uinp.as<float>(), c.uaspc.as<float>(), c.cap_x.as<float>(), c.cap_y.as<float>());
+  else
+    hipLaunchKernelGGL(k_capture<double>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.cap_stride, c.cap_n,
+                       c.perm.as<int>(), c.gfeat.as<float>(), c.frot.as<float>(), c.uind.as<double>(),
+                       c.uinp.as<double>(), c.uaspc.as<double>(), c.cap_x.as<float>(), c.cap_y.as<float>());
+  T9_LAUNCH_CHECK();
+  c.cap_n += per;
+}
+
+// ---- host-side operations behind the C ABI
+
+void guess_set_params(Ctx& c, const float* params, const float* norm, uint64_t seed) {
+  guess_setup(c);
+  std::vector<float> p(NPARAM);
+  if (params) {
+    std::copy(params, params + NPARAM, p.begin());
+  } else {  // seeded uniform init, fan-in scaled; last layer small
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    auto fill = [&](int off, int fan_in, int count, double scale) {
+      const double lim = std::sqrt(3.0 / fan_in) * scale;
+      for (int q = 0; q < count; ++q) p[off + q] = (float)(lim * u(g));
+    };
+    fill(OFF_W1, NF, NF * NH, 1.0);
+    fill(OFF_W2, NH, NH * NH, 1.0);
+    fill(OFF_W3, NSK, NSK * NO, 0.1);
+  }
+  T9_HIP(hipMemcpyAsync(c.gp.p, p.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice, c.stream));
+  T9_HIP(hipMemsetAsync(c.gm.p, 0, sizeof(float) * NPARAM, c.stream));
+  T9_HIP(hipMemsetAsync(c.gv.p, 0, sizeof(float) * NPARAM, c.stream));
+  hipLaunchKernelGGL(k_to_bf16, dim3(ceil_div(NPARAM, 256)), dim3(256), 0, c.stream, NPARAM, c.gp.as<float>(),
+                     c.gpb.as<u16>());
+  T9_LAUNCH_CHECK();
+  c.adam_t = 0;
+  c.guess_params = true;
+  if (norm) {
+    c.gnorm_h.assign(norm, norm + kNorm);
+    T9_HIP(hipMemcpyAsync(c.gnorm.p, c.gnorm_h.data(), sizeof(float) * kNorm, hipMemcpyHostToDevice, c.stream));
+    c.guess_ready = true;
+  }
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+void guess_get_params(Ctx& c, float* params, float* norm) {
+  if (!c.guess_params) throw ArgError("the learned guess has no parameters");
+  if (params)
+    T9_HIP(hipMemcpyAsync(params, c.gp.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  if (norm) {
+    if (c.gnorm_h.size() != (size_t)kNorm) throw ArgError("the learned guess has no normalization yet");
+    std::copy(c.gnorm_h.begin(), c.gnorm_h.end(), norm);
+  }
+}
+
+// Mean squared error and its gradient on rows X (bf16 bits) / T (normalized).
+double mlp_grad(Ctx& c, int m, const void* X, const float* T, float* grad_dev) {
+  const int nwg = ceil_div(m, 32);
+  c.gpart.alloc(sizeof(float) * (size_t)nwg * (NPARAM + 1));
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(nwg), dim3(64), 0, c.stream, m, m, (const u16*)X, T, c.gpb.as<u16>(),
+                     c.gp.as<float>(), c.gpart.as<float>());
+  T9_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_grad_reduce, dim3(ceil_div(NPARAM + 1, 256)), dim3(256), 0, c.stream, nwg,
+                     c.gpart.as<float>(), grad_dev);
+  T9_LAUNCH_CHECK();
+  return 0.0;
+}
+
+void adam_step(Ctx& c, const float* grad_dev, double lr) {
+  c.adam_t += 1;
+  const float c1 = (float)(1.0 - std::pow(0.9, c.adam_t)), c2 = (float)(1.0 - std::pow(0.999, c.adam_t));
+  hipLaunchKernelGGL(k_adam, dim3(ceil_div(NPARAM, 256)), dim3(256), 0, c.stream, grad_dev, c.gp.as<float>(),
+                     c.gm.as<float>(), c.gv.as<float>(), c.gpb.as<u16>(), (float)lr, c1, c2);
+  T9_LAUNCH_CHECK();
+}
+
+void guess_capture_start(Ctx& c, int stride, long long max_samples) {
+  if (stride < 0 || max_samples < 0) throw ArgError("bad capture arguments");
+  c.cap_stride = stride;
+  if (stride == 0) return;
+  guess_setup(c);
+  if (max_samples > c.cap_max) {
+    c.cap_x.alloc(sizeof(float) * NF * (size_t)max_samples);
+    c.cap_y.alloc(sizeof(float) * NO * (size_t)max_samples);
+    c.cap_max = max_samples;
+    c.cap_n = 0;
+  }
+}
+
+// Normalization from the captured samples (mean, 1/std of the features; mean,
+// std of the targets), fixed-order reductions.
+void guess_fit_norm(Ctx& c) {
+  if (c.cap_n < 2) throw ArgError("no captured samples to normalize");
+  const int nblk = 256;
+  const long long chunk = (c.cap_n + nblk - 1) / nblk;
+  DBuf sx, sy;
+  sx.alloc(sizeof(double) * nblk * 2 * NF);
+  sy.alloc(sizeof(double) * nblk * 2 * NO);
+  hipLaunchKernelGGL(k_colstats, dim3(nblk), dim3(256), 0, c.stream, c.cap_n, NF, c.cap_x.as<float>(), chunk,
+                     sx.as<double>());
+  hipLaunchKernelGGL(k_colstats, dim3(nblk), dim3(256), 0, c.stream, c.cap_n, NO, c.cap_y.as<float>(), chunk,
+                     sy.as<double>());
+  T9_LAUNCH_CHECK();
+  std::vector<double> hx(nblk * 2 * NF), hy(nblk * 2 * NO);
+  T9_HIP(hipMemcpyAsync(hx.data(), sx.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipMemcpyAsync(hy.data(), sy.p, sizeof(double) * hy.size(), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  std::vector<float> nm(kNorm);
+  const double N = (double)c.cap_n;
+  auto stats = [&](const std::vector<double>& h, int w, int col, double& mean, double& sd) {
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < nblk; ++b) {
+      s1 += h[(size_t)b * 2 * w + col];
+      s2 += h[(size_t)b * 2 * w + w + col];
+    }
+    mean = s1 / N;
+    sd = std::sqrt(std::max(0.0, s2 / N - mean * mean));
+  };
+  for (int q = 0; q < NF; ++q) {
+    double mu, sd;
+    stats(hx, NF, q, mu, sd);
+    nm[q] = (float)mu;
+    nm[NF + q] = (float)(sd > 1e-12 ? 1.0 / sd : 1.0);
+  }
+  for (int q = 0; q < NO; ++q) {
+    double mu, sd;
+    stats(hy, NO, q, mu, sd);
+    nm[2 * NF + q] = (float)mu;
+    nm[2 * NF + NO + q] = (float)(sd > 1e-30 ? sd : 1.0);
+  }
+  c.gnorm_h = nm;
+  T9_HIP(hipMemcpyAsync(c.gnorm.p, nm.data(), sizeof(float) * kNorm, hipMemcpyHostToDevice, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  c.guess_ready = true;
+}
+
+// Adam over the captured samples: `epochs` passes in a seeded pseudo-random
+// order (i -> (a i + b) mod N with a coprime to N), batches of `batch` rows.
+// Returns the mean loss of the first and the last epoch.
+void guess_train(Ctx& c, int epochs, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+  if (epochs < 1 || batch < 32) throw ArgError("bad training arguments");
+  if (!c.guess_params) guess_set_params(c, nullptr, nullptr, seed);
+  if (!c.guess_ready) guess_fit_norm(c);
+  const long long N = c.cap_n;
+  if (N < batch) throw ArgError("fewer captured samples than one batch");
+  DBuf X, T, L;
+  X.alloc(sizeof(u16) * NF * (size_t)batch);
+  T.alloc(sizeof(float) * NO * (size_t)batch);
+  const long long nb = N / batch;
+  L.alloc(sizeof(float) * (size_t)nb);
+  std::mt19937_64 g(seed);
+  auto gcd = [](unsigned long long a, unsigned long long b) {
+    while (b) {
+      const unsigned long long t = a % b;
+      a = b;
+      b = t;
+    }
+    return a;
+  };
+  for (int ep = 0; ep < epochs; ++ep) {
+    unsigned long long a = 0;
+    do a = g() % (unsigned long long)N; while (a == 0 || gcd(a, (unsigned long long)N) != 1);
+    const unsigned long long b = g() % (unsigned long long)N;
+    for (long long q = 0; q < nb; ++q) {
+      hipLaunchKernelGGL(k_make_batch, dim3(ceil_div(batch, 256)), dim3(256), 0, c.stream, batch, q * batch, a, b, N,
+                         c.cap_x.as<float>(), c.cap_y.as<float>(), c.gnorm.as<float>(), X.as<u16>(), T.as<float>());
+      T9_LAUNCH_CHECK();
+      mlp_grad(c, batch, X.p, T.as<float>(), c.ggrad.as<float>());
+      T9_HIP(hipMemcpyAsync(L.as<float>() + q, c.ggrad.as<float>() + NPARAM, sizeof(float), hipMemcpyDeviceToDevice,
+                            c.stream));
+      adam_step(c, c.ggrad.as<float>(), lr);
+    }
+    std::vector<float> hl(nb);
+    T9_HIP(hipMemcpyAsync(hl.data(), L.p, sizeof(float) * nb, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    double s = 0;
+    for (float v : hl) s += v;
+    s /= (double)nb * batch * NO;
+    if (!std::isfinite(s)) throw ArgError("training diverged (non-finite loss)");
+    if (ep == 0 && loss_first) *loss_first = s;
+    if (loss_last) *loss_last = s;
+  }
+}
+
+}  // namespace t9

@@ -68,7 +68,9 @@ struct VdwArgs {
   double* rows;
 };
 
-template <class real>
+// VIR: accumulate the virial (only when it is requested).  The site gradient
+// is summed in FP32 over runs of 16 list entries and flushed to FP64.
+template <class real, bool VIR>
 __global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0, g[3] = {0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
@@ -78,8 +80,17 @@ __global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
     const int len = A.nlist[s];
     const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
     const real inv_tc = 1 / (A.taper - A.cut);
+    real gs[3] = {0, 0, 0};
+    uint32_t nxt = len > 0 ? en[0] : 0;
     for (int kk = 0; kk < len; ++kk) {
-      const uint32_t ent = en[(long long)kk * 64];
+      const uint32_t ent = nxt;
+      if (kk + 1 < len) nxt = en[(long long)(kk + 1) * 64];
+      if ((kk & 15) == 15) {
+        for (int d = 0; d < 3; ++d) {
+          g[d] += (double)gs[d];
+          gs[d] = 0;
+        }
+      }
       const int k = (int)(ent >> kCodeBits);
       const int code = (int)(ent & (kMaxCodes - 1));
       const real sc = code ? (real)A.vscale[code] : (real)1;
@@ -117,14 +128,17 @@ __global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
       }
       e += (double)en_;
       const real f = de * rinv;
-      for (int d = 0; d < 3; ++d) g[d] += (double)(f * R[d]);
-      v[0] += (double)(f * R[0] * R[0]);
-      v[1] += (double)(f * R[1] * R[1]);
-      v[2] += (double)(f * R[2] * R[2]);
-      v[3] += (double)(f * R[0] * R[1]);
-      v[4] += (double)(f * R[0] * R[2]);
-      v[5] += (double)(f * R[1] * R[2]);
+      for (int d = 0; d < 3; ++d) gs[d] += f * R[d];
+      if (VIR) {
+        v[0] += (double)(f * R[0] * R[0]);
+        v[1] += (double)(f * R[1] * R[1]);
+        v[2] += (double)(f * R[2] * R[2]);
+        v[3] += (double)(f * R[0] * R[1]);
+        v[4] += (double)(f * R[0] * R[2]);
+        v[5] += (double)(f * R[1] * R[2]);
+      }
     }
+    for (int d = 0; d < 3; ++d) g[d] += (double)gs[d];
     for (int d = 0; d < 3; ++d) A.gsite[3 * (size_t)s + d] = g[d];
   }
   double red[EP_N];
@@ -338,7 +352,7 @@ void vdw_sites_t(Ctx& c) {
 }
 
 template <class real>
-void vdw_run_t(Ctx& c) {
+void vdw_run_t(Ctx& c, bool vir) {
   VdwArgs<real> A;
   A.n = c.n;
   A.nlist = c.vlist.nlist.as<int>();
@@ -359,7 +373,8 @@ void vdw_run_t(Ctx& c) {
   A.rows = energy_rows(c, nb);
   {
     KScope ks_(c, KT_VDW);
-    hipLaunchKernelGGL(k_vdw<real>, dim3(nb), dim3(256), 0, c.stream, A);
+    if (vir) hipLaunchKernelGGL((k_vdw<real, true>), dim3(nb), dim3(256), 0, c.stream, A);
+    else hipLaunchKernelGGL((k_vdw<real, false>), dim3(nb), dim3(256), 0, c.stream, A);
     T9_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_vdw_to_atoms, dim3(nb), dim3(256), 0, c.stream, c.n, c.perm.as<int>(), c.iperm.as<int>(),
@@ -375,10 +390,10 @@ void vdw_sites(Ctx& c) {
   else vdw_sites_t<double>(c);
 }
 
-void vdw_run(Ctx& c) {
+void vdw_run(Ctx& c, bool vir) {
   if (!c.use_vdw) return;
-  if (c.prec == 32) vdw_run_t<float>(c);
-  else vdw_run_t<double>(c);
+  if (c.prec == 32) vdw_run_t<float>(c, vir);
+  else vdw_run_t<double>(c, vir);
 }
 
 void valence_run(Ctx& c, double* gval) {

@@ -1,6 +1,8 @@
 // Local-frame -> global multipoles (mpoleInit: chkpole + rotpole).
 // One thread per sorted atom, arithmetic in FP64 from the FP64 caller-order
-// coordinates; the result is stored in the working precision, sorted order.
+// coordinates; the result is stored in the working precision, sorted order,
+// and the frame itself (rows = local x, y, z axes) in FP32 for the learned
+// initial guess.
 // Follows include/seq/rotpole.h:8-224 (frame rules) and
 // src/cu/amoeba/rotpole.cu:7-42.
 #include "ctx.hpp"
@@ -33,7 +35,7 @@ template <class real>
 __global__ void k_rotpole(int n, const int* __restrict__ perm, const double* __restrict__ xyz, double lx, double ly,
                           double lz, int periodic, const double* __restrict__ pole, const int* __restrict__ frame,
                           const int* __restrict__ zaxis, const int* __restrict__ xaxis, const int* __restrict__ yaxis,
-                          const int* __restrict__ ysign, real* __restrict__ rpole) {
+                          const int* __restrict__ ysign, real* __restrict__ rpole, float* __restrict__ frot) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
@@ -87,6 +89,8 @@ __global__ void k_rotpole(int n, const int* __restrict__ perm, const double* __r
     a[1][1] = zz[2] * xx[0] - zz[0] * xx[2];
     a[1][2] = zz[0] * xx[1] - zz[1] * xx[0];
   }
+  for (int k = 0; k < 3; ++k)
+    for (int g = 0; g < 3; ++g) frot[9 * (size_t)s + 3 * k + g] = (float)a[k][g];
   real* out = rpole + 10 * (size_t)s;
   out[0] = (real)p[0];
   for (int g = 0; g < 3; ++g) out[1 + g] = (real)(a[0][g] * p[1] + a[1][g] * p[2] + a[2][g] * p[3]);
@@ -110,9 +114,11 @@ __global__ void k_rotpole(int n, const int* __restrict__ perm, const double* __r
 template <class real>
 void mpole_init_t(Ctx& c) {
   const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0, lz = c.periodic ? c.box[2] : 1.0;
+  c.frot.alloc(sizeof(float) * 9 * (size_t)c.n);
   hipLaunchKernelGGL(k_rotpole<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
                      c.xyz.as<double>(), lx, ly, lz, c.periodic ? 1 : 0, c.pole.as<double>(), c.frame.as<int>(),
-                     c.zaxis.as<int>(), c.xaxis.as<int>(), c.yaxis.as<int>(), c.ysign.as<int>(), c.rpole.as<real>());
+                     c.zaxis.as<int>(), c.xaxis.as<int>(), c.yaxis.as<int>(), c.ysign.as<int>(), c.rpole.as<real>(),
+                     c.frot.as<float>());
   T9_LAUNCH_CHECK();
   c.rpole_valid = true;
 }

@@ -15,7 +15,10 @@
 // sum_j B_j u(t - j) over the last 16 solutions with Kolafa's always-stable
 // predictor coefficients (k = 14); before that, and without the keyword, the
 // direct-field guess.  The history is kept in caller order so a re-sort of
-// the atoms at a list rebuild does not disturb it.
+// the atoms at a list rebuild does not disturb it.  With the learned guess on
+// (guess.hip) the ASPC guess is corrected by the network; the corrected guess
+// is kept only when its initial residual is not worse than the ASPC guess's
+// (re-measured every 20 solves), else the solve starts from ASPC.
 #include <array>
 
 #include "ctx.hpp"
@@ -254,19 +257,74 @@ int induce_t(Ctx& c, int* iters, double* rms) {
 
   dfield_all(c);
   // ASPC once 16 solutions are stored (src/cu/amoeba/pcg.cu:30-33)
-  const bool predict = c.opt.predictor == "ASPC" && c.nualt >= kMaxUalt;
+  const bool aspc = c.opt.predictor == "ASPC";
+  const bool predict = aspc && c.nualt >= kMaxUalt;
+  // learned-guess features need 16 stored fields as well
+  const bool feats = predict && c.nealt >= kMaxUalt && guess_wants_features(c);
+  const bool learned = feats && c.guess_on && c.guess_ready;
   hipLaunchKernelGGL(k_udir<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, c.udir.as<real>(),
                      c.udirp.as<real>(), ud, up, (guess || predict) ? 1 : 0);
   T9_LAUNCH_CHECK();
+  auto slot_of_age = [](int count, int age) { return ((count - 1 - age) % kMaxUalt + kMaxUalt) % kMaxUalt; };
+  // r0 of the guess in ud/up
+  auto residual = [&]() {
+    ufield_all(c, ud, up, rd, rp);
+    hipLaunchKernelGGL(k_rsd_guess<real>, G, Bk, 0, st, n, c.polarity.as<real>(), c.polinv.as<real>(),
+                       c.udir.as<real>(), c.udirp.as<real>(), ud, up, rd, rp);
+    T9_LAUNCH_CHECK();
+  };
+  // RMS of r0 in Debye, read back now
+  auto residual_rms = [&]() {
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
+    T9_LAUNCH_CHECK();
+    T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
+    T9_HIP(hipStreamSynchronize(st));
+    return c.h_scalars[S_EPS0];
+  };
+  const size_t vbytes = sizeof(real) * 3 * (size_t)n;
+  real* ua = c.uaspc.as<real>();
+  c.last_predictor = predict ? 1 : 0;
   if (predict) {
     const auto B = aspc_coefficients();
     Coef16 cf;
     for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
     hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.perm.as<int>(), cf, c.ualt.as<real>(), ud, up);
     T9_LAUNCH_CHECK();
-    ufield_all(c, ud, up, rd, rp);
-    hipLaunchKernelGGL(k_rsd_guess<real>, G, Bk, 0, st, n, c.polarity.as<real>(), c.polinv.as<real>(),
-                       c.udir.as<real>(), c.udirp.as<real>(), ud, up, rd, rp);
+    if (feats) {
+      T9_HIP(hipMemcpyAsync(ua, ud, vbytes, hipMemcpyDeviceToDevice, st));
+      T9_HIP(hipMemcpyAsync(ua + 3 * (size_t)n, up, vbytes, hipMemcpyDeviceToDevice, st));
+      double ce[kMaxUalt], cu[2];
+      for (int k = 0; k < kMaxUalt; ++k) ce[k] = B[((c.nealt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
+      cu[0] = slot_of_age(c.nualt, 0);
+      cu[1] = slot_of_age(c.nualt, 1);
+      guess_features(c, ce, cu);
+    }
+  }
+  if (aspc) guess_save_fields(c);
+  if (learned) {
+    // the ASPC residual is re-measured every 20 learned solves; the learned
+    // guess is kept only when its residual is not worse than that reference
+    if (c.guess_steps % 20 == 0) {
+      residual();
+      c.aspc_eps0_ref = residual_rms();
+    }
+    ++c.guess_steps;
+    guess_apply(c, ud, up);
+    residual();
+    const double eps_learned = residual_rms();
+    if (!(eps_learned <= c.aspc_eps0_ref)) {  // worse, or not finite
+      T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
+      T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
+      residual();
+      ++c.guess_fallback;
+    } else {
+      c.last_predictor = 2;
+      ++c.guess_used;
+    }
+  } else if (predict) {
+    residual();
   } else if (guess) {
     ufield_all(c, ud, up, rd, rp);
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), rd, rp, rd, rp);
@@ -274,7 +332,6 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, rd, rp);
   }
   T9_LAUNCH_CHECK();
-  c.last_predictor = predict ? 1 : 0;
   // residual norm of the initial guess (Debye), read back with the first iteration
   hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr, (const real*)nullptr,
                      (const real*)nullptr, (const real*)nullptr, part);
@@ -320,6 +377,8 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   }
   c.uind_valid = true;
   c.last_guess_rms = c.h_scalars[S_EPS0];
+  if (c.last_predictor == 1 && !learned) c.aspc_eps0_ref = c.last_guess_rms;
+  if (feats) guess_capture(c);
   if (c.opt.predictor == "ASPC") {  // store the solution (src/amoeba/induce.cpp:29-62)
     const size_t slot = 6 * (size_t)n;
     hipLaunchKernelGGL(k_save_hist<real>, G, Bk, 0, st, n, c.perm.as<int>(), ud, up,

@@ -28,7 +28,9 @@ EXPORTS = [
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
     "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
     "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy", "t9_md_init", "t9_md_set_velocities",
-    "t9_md_get_velocities", "t9_md_run",
+    "t9_md_get_velocities", "t9_md_run", "t9_guess_nparam", "t9_guess_set_params", "t9_guess_get_params",
+    "t9_guess_enable", "t9_guess_capture", "t9_guess_stats", "t9_guess_train", "t9_guess_mlp_forward",
+    "t9_guess_mlp_grad", "t9_guess_adam", "t9_guess_features", "t9_guess_history",
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL = 1, 2, 4
@@ -59,6 +61,7 @@ class MdStats(ctypes.Structure):
 
 _LIB = None
 _d = ctypes.POINTER(ctypes.c_double)
+_f = ctypes.POINTER(ctypes.c_float)
 _i = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
@@ -108,6 +111,18 @@ def lib():
             ("t9_md_get_velocities", [_vp, _d]),
             ("t9_md_run", [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(MdStats)]),
+            ("t9_guess_nparam", []),
+            ("t9_guess_set_params", [_vp, _f, _f, ctypes.c_uint64]),
+            ("t9_guess_get_params", [_vp, _f, _f]),
+            ("t9_guess_enable", [_vp, ctypes.c_int]),
+            ("t9_guess_capture", [_vp, ctypes.c_int, ctypes.c_int64]),
+            ("t9_guess_stats", [_vp, ctypes.POINTER(ctypes.c_int64)]),
+            ("t9_guess_train", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _d, _d]),
+            ("t9_guess_mlp_forward", [_vp, _f, ctypes.c_int, _f]),
+            ("t9_guess_mlp_grad", [_vp, _f, _f, ctypes.c_int, _f, _d]),
+            ("t9_guess_adam", [_vp, _f, ctypes.c_double]),
+            ("t9_guess_features", [_vp, _f]),
+            ("t9_guess_history", [_vp, ctypes.c_int, ctypes.c_int, _d]),
         ]:
             if hasattr(L, name):  # device entry points are absent from a host-only build
                 getattr(L, name).argtypes = args
@@ -122,6 +137,10 @@ def check(rc):
 
 def _p(a, t=_d):
     return a.ctypes.data_as(t) if a is not None else None
+
+
+def _pf(a):
+    return _p(a, _f)
 
 
 class System:
@@ -307,6 +326,64 @@ class Context:
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
                  eangle=out.eangle, eurey=out.eurey)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
+
+    # ---- learned initial guess
+    def guess_set_params(self, params=None, norm=None, seed=0):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        q = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        check(lib().t9_guess_set_params(self.h, _pf(p) if p is not None else None,
+                                        _pf(q) if q is not None else None, int(seed)))
+
+    def guess_get_params(self):
+        p = np.zeros(lib().t9_guess_nparam(), np.float32)
+        q = np.zeros(76, np.float32)
+        check(lib().t9_guess_get_params(self.h, _pf(p), _pf(q)))
+        return p, q
+
+    def guess_enable(self, on=True):
+        check(lib().t9_guess_enable(self.h, 1 if on else 0))
+
+    def guess_capture(self, stride, max_samples):
+        check(lib().t9_guess_capture(self.h, int(stride), int(max_samples)))
+
+    def guess_stats(self):
+        out = np.zeros(4, np.int64)
+        check(lib().t9_guess_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return dict(captured=int(out[0]), used=int(out[1]), fallback=int(out[2]), last_predictor=int(out[3]))
+
+    def guess_train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().t9_guess_train(self.h, int(epochs), int(batch), float(lr), int(seed), ctypes.byref(a),
+                                   ctypes.byref(b)))
+        return a.value, b.value
+
+    def guess_mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 6), np.float32)
+        check(lib().t9_guess_mlp_forward(self.h, _pf(X), X.shape[0], _pf(Y)))
+        return Y
+
+    def guess_mlp_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(lib().t9_guess_nparam(), np.float32)
+        loss = ctypes.c_double()
+        check(lib().t9_guess_mlp_grad(self.h, _pf(X), _pf(T), X.shape[0], _pf(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def guess_adam(self, grad, lr):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        check(lib().t9_guess_adam(self.h, _pf(g), float(lr)))
+
+    def guess_features(self):
+        F = np.zeros((self.n, 32), np.float32)
+        check(lib().t9_guess_features(self.h, _pf(F)))
+        return F
+
+    def guess_history(self, kind, age):
+        out = np.zeros((self.n, 3))
+        check(lib().t9_guess_history(self.h, int(kind), int(age), _p(out)))
+        return out
 
     def md_init(self, kelvin, seed):
         check(lib().t9_md_init(self.h, float(kelvin), int(seed)))
