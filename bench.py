@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "tinker-gpu_amd"))
 
@@ -64,15 +66,33 @@ def cpu_baseline(system_arrays, threads):
 FAMILIES = None
 
 
-def run_gpu(t9hip, system, prec, steps, warmup):
+def equilibrate(t9hip, system, steps):
+    """FP32 RESPA + Bussi from the lattice: the rigid construction geometry
+    relaxes and heats the box; the thermostat brings it back to 298 K."""
+    ctx = t9hip.Context(system, 32)
+    ctx.md_init(KELVIN, SEED)
+    st = ctx.md_run(steps, DT, NRESPA, KELVIN, TAU)
+    x, v = ctx.coords(), ctx.velocities()
+    ctx.close()
+    return x, v, st
+
+
+def run_gpu(t9hip, system, prec, steps, warmup, guess=None, start=None):
     global FAMILIES
     FAMILIES = (("ufield_real", t9hip.K_UFIELD), ("dfield_real", t9hip.K_DFIELD), ("emplar_real", t9hip.K_EMPLAR),
                 ("fft_pass", t9hip.K_FFT), ("pme_spread", t9hip.K_SPREAD), ("pme_gather", t9hip.K_GATHER),
                 ("precond", t9hip.K_PRECOND), ("vdw_pairs", t9hip.K_VDW), ("valence", t9hip.K_VALENCE))
     ctx = t9hip.Context(system, prec)
+    if start is not None:
+        ctx.set_coords(start[0])
     ctx.nblist_build()
     pairs = ctx.cutoff_pairs()
+    if guess is not None:  # learned guess: active once the 16-deep histories are filled
+        ctx.guess_set_params(guess["params"], guess["norm"])
+        ctx.guess_enable(True)
     ctx.md_init(KELVIN, SEED)
+    if start is not None:
+        ctx.set_velocities(start[1])
     # warmup: fills the 16-deep ASPC history, so the timed steps use ASPC
     wst = ctx.md_run(warmup, DT, NRESPA, KELVIN, TAU)
     ctx.sync()
@@ -88,8 +108,9 @@ def run_gpu(t9hip, system, prec, steps, warmup):
         ms, cnt = ctx.kernel_timing(kid, 0)
         kt[name] = {"total_ms_per_step": round(ms / steps, 4), "launches_per_step": round(cnt / steps, 2),
                     "avg_us": round(1e3 * ms / max(cnt, 1), 3)}
+    gs = ctx.guess_stats() if guess is not None else None
     return {"sec_per_step": (t1 - t0) / steps, "iters_per_step": st["iterations"] / steps, "kernels": kt,
-            "pairs": pairs, "md": st, "warmup_md": wst}
+            "pairs": pairs, "md": st, "warmup_md": wst, "guess": gs}
 
 
 def roofline_of(name, k, pairs, prec, natoms):
@@ -120,7 +141,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--equil", type=int, default=2000, help="untimed equilibration steps before the legs")
+    ap.add_argument("--no-learned", action="store_true", help="ASPC instead of the learned guess")
+    ap.add_argument("--weights", default=os.path.join(REPO, "tinker-gpu_amd", "guess_water.npz"))
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -134,7 +158,13 @@ def main():
     import t9hip
     prec = 32 if args.precision == "fp32" else 64
     system = t9hip.System.water_box(NMOL, NSIDE, BOX, SEED, key=KEY.format(K=GRID, buf=LIST_BUFFER))
-    main_run = run_gpu(t9hip, system, prec, args.steps, args.warmup)
+    guess = None
+    if not args.no_learned and prec == 32 and os.path.exists(args.weights):
+        w = np.load(args.weights)
+        guess = {"params": w["params"], "norm": w["norm"]}
+    x_eq, v_eq, eq = equilibrate(t9hip, system, args.equil)
+    start = (x_eq, v_eq)
+    main_run = run_gpu(t9hip, system, prec, args.steps, args.warmup, guess, start)
     sec = main_run["sec_per_step"]
     ns_day = 0.1728 / sec
     kt = main_run["kernels"]
@@ -159,9 +189,14 @@ def main():
     roof.update({"kernel": dom, "traffic": traffic, "avg_launch_us": kt[dom]["avg_us"],
                  "pairs_within_cutoff": pairs})
 
+    fp32_aspc = None
+    if guess is not None and not args.no_fp64:
+        ra = run_gpu(t9hip, system, 32, args.steps, args.warmup, None, start)
+        fp32_aspc = {"ms_per_step": round(1e3 * ra["sec_per_step"], 4), "ns_day": round(0.1728 / ra["sec_per_step"], 3),
+                     "cg_iters_per_step_mean": round(ra["iters_per_step"], 2)}
     fp64 = None
     if not args.no_fp64 and prec == 32:
-        r64 = run_gpu(t9hip, system, 64, max(20, args.steps // 2), args.warmup)
+        r64 = run_gpu(t9hip, system, 64, args.steps, args.warmup, None, start)
         fp64 = {"ms_per_step": round(1e3 * r64["sec_per_step"], 4),
                 "ns_day": round(0.1728 / r64["sec_per_step"], 3),
                 "cg_iters_per_step_mean": round(r64["iters_per_step"], 2),
@@ -191,22 +226,27 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == 32 else "f64",
-        "data": "synthetic (seeded 32^3-lattice water box, random orientations, Maxwell-Boltzmann 298 K)",
+        "data": "synthetic (seeded 32^3-lattice water box, random orientations, Maxwell-Boltzmann 298 K, "
+                "equilibrated untimed before the timed steps)",
         "config": {
             "workload": "AMOEBA water MD on 96,000 atoms: 2-fs RESPA steps (8 x 0.25-fs valence inner steps; "
                         "slow forces = multipole + mutual polarization (PCG to 1e-6 D) + buffered 14-7 vdW), "
                         "Bussi thermostat 298 K",
             "atoms": system.n, "pme_grid": GRID, "ewald_cutoff": 7.0, "vdw_cutoff": 9.0,
             "list_buffer": LIST_BUFFER, "polar_eps": 1e-6, "dt_fs": 1e3 * DT, "nrespa": NRESPA,
-            "predictor": "ASPC (the learned guess is not built yet)",
+            "predictor": ("learned guess (ASPC fallback)" if guess is not None else "ASPC"),
+            "learned_guess": main_run["guess"],
             "precision": args.precision,
             "cg_iters_per_step_mean": round(main_run["iters_per_step"], 2),
             "epot_kcal_mol": md["epot"], "temperature_K": round(md["temperature"], 2),
             "list_builds": md["list_builds"],
+            "equilibration": {"steps": args.equil, "temperature_K": round(eq["temperature"], 2),
+                              "cg_iters_per_step_mean": round(eq["iterations"] / max(1, args.equil), 2)},
             "parallelism": "single GPU",
         },
         "roofline": roof,
         "kernels": kt,
+        "fp32_aspc": fp32_aspc,
         "fp64_plain": fp64,
         "cpu_baseline": cpu,
     }

@@ -220,6 +220,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   c.morton_bits = 0;
   if (c.periodic) {
     const double rl = std::max(c.mlist.cutoff, c.vlist.cutoff + site_slack);
+    c.cell_reach = rl;
     for (int d = 0; d < 3; ++d)
       if (std::max(c.mlist.cutoff, c.vlist.cutoff) > 0.5 * c.box[d] + 1e-9)
         throw ArgError("cutoff plus list buffer exceeds half the box edge");

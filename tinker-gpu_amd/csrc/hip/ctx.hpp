@@ -136,6 +136,7 @@ struct Ctx {
   bool brute = true;             // all-pairs search (small or non-periodic systems)
   DBuf cell_key, cell_count, cell_start, cell_cursor, unsorted;
   NbList mlist, ulist, vlist;    // multipole/polar, preconditioner, vdW sites
+  double cell_reach = 0;         // radius the cell search must cover (A)
   int builds = 0;
   bool list_valid = false;
 

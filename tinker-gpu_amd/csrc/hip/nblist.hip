@@ -174,68 +174,310 @@ __global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double*
                           len, periodic);
 }
 
-// Count (fill == 0) or write (fill == 1) each sorted atom's neighbors.
+// The three lists share one scan over the candidate cells: the multipole
+// list (atoms within rm), the preconditioner list (atoms within ru <= rm) and
+// the vdW list (sites within rv, sites read from vpos).
 template <class real>
-__global__ void k_nb_list(int n, int fill, const PosRec<real>* __restrict__ pos,
-                          const int* __restrict__ cellc, int ncx, int ncy, int ncz, int brute,
-                          const int* __restrict__ cell_start, const int* __restrict__ cell_count, BoxT<real> box,
-                          real rl2, const int* __restrict__ perm, const int* __restrict__ excl_off,
-                          const int* __restrict__ excl_j, const int* __restrict__ excl_code, int* __restrict__ nlist,
-                          const long long* __restrict__ woff, uint32_t* __restrict__ entries) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const auto pi = pos[s];
-  const int oi = perm[s];
-  const int e0 = excl_off[oi], e1 = excl_off[oi + 1];
-  int cnt = 0;
-  uint32_t* out = nullptr;
-  int limit = 0x7fffffff;
-  if (fill) {
-    out = entries + woff[s >> 6] + (s & 63);
-    limit = nlist[s];  // the count pass sized this atom's slots
-  }
-  auto test = [&](int j) {
-    if (j == s) return;
-    const auto pj = pos[j];
-    real xr, yr, zr;
-    pair_disp(pi, pj, box, xr, yr, zr);
-    real r2 = xr * xr + yr * yr + zr * zr;
-    if (r2 < rl2 && cnt < limit) {
-      if (fill) {
-        int code = 0;
-        const int oj = perm[j];
-        for (int e = e0; e < e1; ++e)
-          if (excl_j[e] == oj) code = excl_code[e];
-        out[(long long)cnt * 64] = ((uint32_t)j << kCodeBits) | (uint32_t)code;
-      }
-      ++cnt;
+struct ListArgs {
+  int n, fill, brute;
+  const PosRec<real>* pos;
+  const PosRec<real>* vpos;  // null: no vdW list
+  const int* cellc;
+  int nc[3];
+  const int* cell_start;
+  const int* cell_count;
+  BoxT<real> box;
+  real rm2, ru2, rv2;        // ru2 <= 0: no preconditioner list
+  double reach2;             // squared reach of the cell search (A^2)
+  double csize[3];           // cell edge (A)
+  double flen[3];            // box edge (FP64 positions are wrapped coordinates)
+  const int* perm;
+  const int *eoff, *ej, *ecode;     // scaled partners of the atom lists
+  const int *veoff, *vej, *vecode;  // scaled partners of the vdW list
+  int *nm, *nu, *nv;                // totals (written by k_sum_parts)
+  int* parts;                       // [3][kParts][n] counts per part of the cell region
+  const long long *om, *ou, *ov;    // slab offsets (fill pass)
+  uint32_t *em, *eu, *ev;
+};
+
+constexpr int kParts = 16;  // waves sharing one wave of atoms (one workgroup)
+
+__device__ __forceinline__ double frac_of(const PosRec<float>& p, int d, const double*) {
+  const unsigned v = d == 0 ? (unsigned)p.x : (d == 1 ? (unsigned)p.y : (unsigned)p.z);
+  return (double)v * (1.0 / 4294967296.0);
+}
+__device__ __forceinline__ double frac_of(const PosRec<double>& p, int d, const double* flen) {
+  return (d == 0 ? p.x : (d == 1 ? p.y : p.z)) / flen[d];
+}
+
+// A lane's scaled partners: the first 8 in registers, the rest read when
+// needed (none for water and most of a protein's atoms).
+struct Partners {
+  int j[8], code[8];
+  int e0, e1;
+  __device__ __forceinline__ void load(int a, int b, const int* __restrict__ ej, const int* __restrict__ ec) {
+    e0 = a;
+    e1 = b;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const bool in = a + k < b;
+      j[k] = in ? ej[a + k] : -1;
+      code[k] = in ? ec[a + k] : 0;
     }
-  };
-  if (brute) {
-    for (int j = 0; j < n; ++j) test(j);
-  } else {
-    const int c = cellc[s];
+  }
+  __device__ __forceinline__ int find(int oj, const int* __restrict__ ej, const int* __restrict__ ec) const {
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c = j[k] == oj ? code[k] : c;
+    for (int e = e0 + 8; e < e1; ++e)
+      if (ej[e] == oj) c = ec[e];
+    return c;
+  }
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ double wave_minf(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_maxf(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ int wrapc(int x, int nc) {
+  x += x < 0 ? nc : 0;
+  return x - (x >= nc ? nc : 0);
+}
+
+// Wave-uniform broadcast of lane t's value (t uniform).
+__device__ __forceinline__ int bcast(int v, int t) { return __builtin_amdgcn_readlane(v, t); }
+__device__ __forceinline__ float bcast(float v, int t) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
+}
+__device__ __forceinline__ double bcast(double v, int t) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), t);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), t);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <class real>
+__device__ __forceinline__ PosRec<real> bcast(const PosRec<real>& p, int t) {
+  PosRec<real> q;
+  q.x = bcast(p.x, t);
+  q.y = bcast(p.y, t);
+  q.z = bcast(p.z, t);
+  q.w = bcast(p.w, t);
+  return q;
+}
+
+// Count (fill == 0) or write (fill == 1) each sorted atom's neighbors.
+// The candidates are walked wave-uniformly in chunks of up to 64 atoms of
+// one cell: each lane loads one candidate (one load instruction per chunk,
+// the next chunk prefetched while the current one is tested), then every
+// lane tests every candidate of the chunk, broadcast lane by lane.  The cells
+// are those around the wave's bounding box when its atoms span at most 4
+// cells per axis (the usual case: 64 consecutive atoms in cell Morton
+// order), else the 5^3 stencil of each home cell in turn, tested only by
+// that cell's lanes; without cells, all atoms.  The kParts waves of a
+// workgroup take the same 64 atoms and split the cells (or the atom range)
+// in kParts fixed parts (latency hiding: 16 waves per 64 atoms); each part
+// writes its own range of list slots.
+template <class real>
+__global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const int s0 = blockIdx.x * 64 + lane;
+  const bool active = s0 < A.n;
+  const int s = active ? s0 : A.n - 1;  // idle lanes mirror the last atom
+  const auto pi = A.pos[s];
+  const bool has_u = A.ru2 > 0, has_v = A.vpos != nullptr;
+  PosRec<real> vi = pi;
+  if (has_v) vi = A.vpos[s];
+  const int oi = A.perm[s];
+  Partners px, pv;
+  if (A.fill) {
+    px.load(A.eoff[oi], A.eoff[oi + 1], A.ej, A.ecode);
+    if (has_v) pv.load(A.veoff[oi], A.veoff[oi + 1], A.vej, A.vecode);
+  }
+  int cm = 0, cu = 0, cv = 0;
+  int lm = 0x7fffffff, lu = 0x7fffffff, lv = 0x7fffffff;
+  uint32_t *om = nullptr, *ou = nullptr, *ov = nullptr;
+  const size_t pn = (size_t)kParts * A.n;
+  if (A.fill && active) {
+    // this part's slots follow those of the parts before it
+    int pm = 0, pu = 0, pv = 0;
+    for (int q = 0; q < part; ++q) {
+      pm += A.parts[(size_t)q * A.n + s];
+      pu += A.parts[pn + (size_t)q * A.n + s];
+      pv += A.parts[2 * pn + (size_t)q * A.n + s];
+    }
+    lm = A.parts[(size_t)part * A.n + s];
+    lu = A.parts[pn + (size_t)part * A.n + s];
+    lv = A.parts[2 * pn + (size_t)part * A.n + s];
+    om = A.em + A.om[s >> 6] + lane + 64LL * pm;
+    if (has_u) ou = A.eu + A.ou[s >> 6] + lane + 64LL * pu;
+    if (has_v) ov = A.ev + A.ov[s >> 6] + lane + 64LL * pv;
+  }
+  // region of cells: base r0, size rs; pruned against the wave's bounding box
+  int c = 0, lo0 = 0, lo1 = 0, lo2 = 0, hi0 = 0, hi1 = 0, hi2 = 0;
+  double bl0 = 0, bl1 = 0, bl2 = 0, bh0 = 0, bh1 = 0, bh2 = 0;
+  bool compact = true;
+  if (!A.brute) {
+    c = A.cellc[s];
     const int cx = c & 1023, cy = (c >> 10) & 1023, cz = (c >> 20) & 1023;
-    for (int dz = -2; dz <= 2; ++dz) {
-      int z = cz + dz;
-      z += z < 0 ? ncz : 0;
-      z -= z >= ncz ? ncz : 0;
-      for (int dy = -2; dy <= 2; ++dy) {
-        int y = cy + dy;
-        y += y < 0 ? ncy : 0;
-        y -= y >= ncy ? ncy : 0;
-        for (int dx = -2; dx <= 2; ++dx) {
-          int x = cx + dx;
-          x += x < 0 ? ncx : 0;
-          x -= x >= ncx ? ncx : 0;
-          int k = morton3(x, y, z);
-          int j0 = cell_start[k], j1 = j0 + cell_count[k];
-          for (int j = j0; j < j1; ++j) test(j);
+    lo0 = wave_min(cx);
+    lo1 = wave_min(cy);
+    lo2 = wave_min(cz);
+    hi0 = wave_max(cx);
+    hi1 = wave_max(cy);
+    hi2 = wave_max(cz);
+    compact = hi0 - lo0 <= 3 && hi1 - lo1 <= 3 && hi2 - lo2 <= 3 && hi0 - lo0 + 5 <= A.nc[0] &&
+              hi1 - lo1 + 5 <= A.nc[1] && hi2 - lo2 + 5 <= A.nc[2];
+    if (compact) {
+      bl0 = wave_minf(frac_of(pi, 0, A.flen) * A.nc[0]);
+      bh0 = wave_maxf(frac_of(pi, 0, A.flen) * A.nc[0]);
+      bl1 = wave_minf(frac_of(pi, 1, A.flen) * A.nc[1]);
+      bh1 = wave_maxf(frac_of(pi, 1, A.flen) * A.nc[1]);
+      bl2 = wave_minf(frac_of(pi, 2, A.flen) * A.nc[2]);
+      bh2 = wave_maxf(frac_of(pi, 2, A.flen) * A.nc[2]);
+    }
+  }
+  unsigned long long todo = (A.brute || compact) ? 1ull : __ballot(1);
+  while (todo) {
+    // one walk: the whole box (brute), the wave's region (compact), or the
+    // stencil of one home cell
+    int r00, r01, r02, rs0, rs1, rs2;
+    bool on = active, prune = false;
+    if (A.brute || compact) {
+      todo = 0;
+      r00 = lo0 - 2;
+      r01 = lo1 - 2;
+      r02 = lo2 - 2;
+      rs0 = hi0 - lo0 + 5;
+      rs1 = hi1 - lo1 + 5;
+      rs2 = hi2 - lo2 + 5;
+      prune = !A.brute;
+    } else {
+      const int lead = __builtin_ctzll(todo);
+      const int h = __shfl(c, lead, 64);
+      const bool mine = c == h;
+      todo &= ~__ballot(mine);
+      on = active && mine;
+      r00 = (h & 1023) - 2;
+      r01 = ((h >> 10) & 1023) - 2;
+      r02 = ((h >> 20) & 1023) - 2;
+      rs0 = rs1 = rs2 = 5;
+    }
+    const int ncells = A.brute ? 1 : rs0 * rs1 * rs2;
+    const int qa = A.brute ? 0 : part * ncells / kParts, qb = A.brute ? 1 : (part + 1) * ncells / kParts;
+    int q = qa - 1, off = 0, total = 0, start = 0;
+    bool have = true;
+    int j0 = 0, cnt = 0;
+    PosRec<real> np = pi, nv = vi;
+    int no = 0;
+    // first chunk, then each iteration prefetches the next one
+    for (int stage = 0;; ++stage) {
+      // advance (q, off) to the next chunk (j0, cnt)
+      if (q >= qa && off + 64 < total) {
+        off += 64;
+      } else {
+        have = false;
+        while (++q < qb) {
+          if (A.brute) {  // the atom range is split instead of the cells
+            start = part * A.n / kParts;
+            total = (part + 1) * A.n / kParts - start;
+          } else {
+            const int x = r00 + q % rs0, y = r01 + (q / rs0) % rs1, z = r02 + q / (rs0 * rs1);
+            if (prune) {
+              const double gx = fmax(0.0, fmax((double)x - bh0, bl0 - (double)(x + 1))) * A.csize[0];
+              const double gy = fmax(0.0, fmax((double)y - bh1, bl1 - (double)(y + 1))) * A.csize[1];
+              const double gz = fmax(0.0, fmax((double)z - bh2, bl2 - (double)(z + 1))) * A.csize[2];
+              if (gx * gx + gy * gy + gz * gz > A.reach2) continue;
+            }
+            const int k = morton3(wrapc(x, A.nc[0]), wrapc(y, A.nc[1]), wrapc(z, A.nc[2]));
+            total = __builtin_amdgcn_readfirstlane(A.cell_count[k]);
+            start = __builtin_amdgcn_readfirstlane(A.cell_start[k]);
+          }
+          if (total > 0) {
+            have = true;
+            break;
+          }
+        }
+        off = 0;
+      }
+      const PosRec<real> cp = np, cvp = nv;
+      const int co = no, cj0 = j0, ccnt = stage == 0 ? 0 : cnt;
+      if (have) {
+        j0 = start + off;
+        cnt = min(64, total - off);
+        if (lane < cnt) {  // prefetch: lane t loads candidate j0 + t
+          np = A.pos[j0 + lane];
+          if (has_v) nv = A.vpos[j0 + lane];
+          if (A.fill) no = A.perm[j0 + lane];
         }
       }
+      for (int t = 0; t < ccnt; ++t) {
+        const int j = cj0 + t;
+        const PosRec<real> pj = bcast(cp, t);
+        if (!on || j == s) continue;
+        real xr, yr, zr;
+        pair_disp(pi, pj, A.box, xr, yr, zr);
+        const real r2 = xr * xr + yr * yr + zr * zr;
+        const int oj = A.fill ? bcast(co, t) : 0;
+        if (r2 < A.rm2 && cm < lm) {
+          const bool in_u = has_u && r2 < A.ru2 && cu < lu;
+          if (A.fill) {
+            const int code = px.find(oj, A.ej, A.ecode);
+            const uint32_t ent = ((uint32_t)j << kCodeBits) | (uint32_t)code;
+            om[(long long)cm * 64] = ent;
+            if (in_u) ou[(long long)cu * 64] = ent;
+          }
+          ++cm;
+          cu += in_u ? 1 : 0;
+        }
+        if (has_v) {
+          const PosRec<real> vj = bcast(cvp, t);
+          pair_disp(vi, vj, A.box, xr, yr, zr);
+          if (xr * xr + yr * yr + zr * zr < A.rv2 && cv < lv) {
+            if (A.fill) {
+              const int code = pv.find(oj, A.vej, A.vecode);
+              ov[(long long)cv * 64] = ((uint32_t)j << kCodeBits) | (uint32_t)code;
+            }
+            ++cv;
+          }
+        }
+      }
+      if (!have) break;
     }
   }
-  if (!fill) nlist[s] = cnt;
+  if (!A.fill && active) {
+    A.parts[(size_t)part * A.n + s] = cm;
+    A.parts[pn + (size_t)part * A.n + s] = cu;
+    A.parts[2 * pn + (size_t)part * A.n + s] = cv;
+  }
+}
+
+__global__ void k_sum_parts(int n, const int* __restrict__ parts, int* __restrict__ nm, int* __restrict__ nu,
+                            int* __restrict__ nv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const size_t pn = (size_t)kParts * n;
+  int a = 0, b = 0, c = 0;
+  for (int q = 0; q < kParts; ++q) {
+    a += parts[(size_t)q * n + s];
+    b += parts[pn + (size_t)q * n + s];
+    c += parts[2 * pn + (size_t)q * n + s];
+  }
+  nm[s] = a;
+  if (nu) nu[s] = b;
+  if (nv) nv[s] = c;
 }
 
 __global__ void k_wave_slots(int n, int nwave, const int* __restrict__ nlist, int* __restrict__ maxlen,
@@ -266,24 +508,13 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
 }
 
 
-template <class real>
-void build_list(Ctx& c, NbList& L, const void* posv, const int* eoff, const int* ej, const int* ecode) {
+// Slab offsets and capacity of one list after the count pass.
+void list_slots(Ctx& c, NbList& L) {
   const int n = c.n, nw = c.nwave;
-  const int nb = ceil_div(n, 256);
-  L.nlist.alloc(sizeof(int) * n);
   L.maxlen.alloc(sizeof(int) * nw);
   L.off.alloc(sizeof(long long) * (nw + 1));
   DBuf slots;
   slots.alloc(sizeof(long long) * nw);
-  BoxT<real> box = make_boxT<real>(c);
-  real rl2 = (real)(L.cutoff * L.cutoff);
-  const int nkeys = 1 << (3 * c.morton_bits);
-  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 0, (const PosRec<real>*)posv,
-                     c.cellc.as<int>(), c.ncell[0], c.ncell[1], c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(),
-                     c.cell_count.as<int>(), box, rl2, c.perm.as<int>(), eoff, ej, ecode, L.nlist.as<int>(),
-                     (const long long*)nullptr, (uint32_t*)nullptr);
-  T9_LAUNCH_CHECK();
-  (void)nkeys;
   hipLaunchKernelGGL(k_wave_slots, dim3(ceil_div(nw, 256)), dim3(256), 0, c.stream, n, nw, L.nlist.as<int>(),
                      L.maxlen.as<int>(), slots.as<long long>());
   T9_LAUNCH_CHECK();
@@ -300,10 +531,68 @@ void build_list(Ctx& c, NbList& L, const void* posv, const int* eoff, const int*
     L.capacity = cap;
   }
   L.total = total;
-  hipLaunchKernelGGL(k_nb_list<real>, dim3(nb), dim3(256), 0, c.stream, n, 1, (const PosRec<real>*)posv,
-                     c.cellc.as<int>(), c.ncell[0], c.ncell[1], c.ncell[2], c.brute ? 1 : 0, c.cell_start.as<int>(),
-                     c.cell_count.as<int>(), box, rl2, c.perm.as<int>(), eoff, ej, ecode, L.nlist.as<int>(),
-                     L.off.as<long long>(), L.entries.as<uint32_t>());
+}
+
+// All lists in one count pass and one fill pass over the candidate cells.
+template <class real>
+void build_lists(Ctx& c) {
+  const int n = c.n;
+  const bool has_u = c.ulist.cutoff > 0, has_v = c.vlist.cutoff > 0;
+  ListArgs<real> A;
+  A.n = n;
+  A.brute = c.brute ? 1 : 0;
+  A.pos = c.pos.as<PosRec<real>>();
+  A.vpos = has_v ? c.vpos.as<PosRec<real>>() : nullptr;
+  A.cellc = c.cellc.as<int>();
+  for (int d = 0; d < 3; ++d) {
+    A.nc[d] = c.ncell[d];
+    A.csize[d] = c.periodic ? c.box[d] / c.ncell[d] : 1.0;
+    A.flen[d] = c.periodic ? c.box[d] : 1.0;
+  }
+  A.cell_start = c.cell_start.as<int>();
+  A.cell_count = c.cell_count.as<int>();
+  A.box = make_boxT<real>(c);
+  A.rm2 = (real)(c.mlist.cutoff * c.mlist.cutoff);
+  A.ru2 = has_u ? (real)(c.ulist.cutoff * c.ulist.cutoff) : (real)0;
+  A.rv2 = has_v ? (real)(c.vlist.cutoff * c.vlist.cutoff) : (real)0;
+  A.reach2 = c.cell_reach * c.cell_reach;
+  A.perm = c.perm.as<int>();
+  A.eoff = c.excl_off.as<int>();
+  A.ej = c.excl_j.as<int>();
+  A.ecode = c.excl_code.as<int>();
+  A.veoff = has_v ? c.vexcl_off.as<int>() : nullptr;
+  A.vej = has_v ? c.vexcl_j.as<int>() : nullptr;
+  A.vecode = has_v ? c.vexcl_code.as<int>() : nullptr;
+  for (NbList* L : {&c.mlist, &c.ulist, &c.vlist})
+    if (L->cutoff > 0) L->nlist.alloc(sizeof(int) * n);
+  A.nm = c.mlist.nlist.as<int>();
+  A.nu = has_u ? c.ulist.nlist.as<int>() : nullptr;
+  A.nv = has_v ? c.vlist.nlist.as<int>() : nullptr;
+  DBuf parts;
+  parts.alloc(sizeof(int) * 3 * kParts * (size_t)n);
+  A.parts = parts.as<int>();
+  A.fill = 0;
+  A.om = A.ou = A.ov = nullptr;
+  A.em = A.eu = A.ev = nullptr;
+  const int nwg = c.nwave;
+  hipLaunchKernelGGL(k_nb_lists<real>, dim3(nwg), dim3(64 * kParts), 0, c.stream, A);
+  T9_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_parts, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, A.parts, A.nm, A.nu, A.nv);
+  T9_LAUNCH_CHECK();
+  for (NbList* L : {&c.mlist, &c.ulist, &c.vlist})
+    if (L->cutoff > 0) list_slots(c, *L);
+  A.fill = 1;
+  A.om = c.mlist.off.as<long long>();
+  A.em = c.mlist.entries.as<uint32_t>();
+  if (has_u) {
+    A.ou = c.ulist.off.as<long long>();
+    A.eu = c.ulist.entries.as<uint32_t>();
+  }
+  if (has_v) {
+    A.ov = c.vlist.off.as<long long>();
+    A.ev = c.vlist.entries.as<uint32_t>();
+  }
+  hipLaunchKernelGGL(k_nb_lists<real>, dim3(nwg), dim3(64 * kParts), 0, c.stream, A);
   T9_LAUNCH_CHECK();
 }
 
@@ -347,13 +636,8 @@ void nblist_build_t(Ctx& c) {
                      c.polarity_o.as<double>(), c.pdamp_o.as<double>(), c.jpolar_o.as<int>(), c.polarity.as<real>(),
                      c.polinv.as<real>(), c.pdamp.as<real>(), c.jpolar.as<int>());
   T9_LAUNCH_CHECK();
-  build_list<real>(c, c.mlist, c.pos.p, c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>());
-  if (c.ulist.cutoff > 0)
-    build_list<real>(c, c.ulist, c.pos.p, c.excl_off.as<int>(), c.excl_j.as<int>(), c.excl_code.as<int>());
-  if (c.vlist.cutoff > 0) {
-    vdw_sites(c);
-    build_list<real>(c, c.vlist, c.vpos.p, c.vexcl_off.as<int>(), c.vexcl_j.as<int>(), c.vexcl_code.as<int>());
-  }
+  if (c.vlist.cutoff > 0) vdw_sites(c);
+  build_lists<real>(c);
   c.builds++;
   c.list_valid = true;
   c.coords_dirty = false;
