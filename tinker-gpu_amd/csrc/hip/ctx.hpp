@@ -156,6 +156,11 @@ struct Ctx {
   DBuf pme_w, pme_base;          // real4 / int4 [n] per sorted atom
   DBuf tile_count, tile_start, tile_cursor, tile_atoms;
   bool bins_valid = false;
+  DBuf conv_table;               // real [K1*K2*K3] influence function (z-pass order)
+  DBuf qgrid_m;                  // convolved permanent-multipole grid of the last direct field
+  long long qgrid_m_stamp = -1;  // rpole_stamp it belongs to
+  long long rpole_stamp = 0;     // incremented by every rotation of the multipoles
+  bool conv_table_valid = false; // false after a box change
 
   // ---- energy / gradient / virial (sorted order, FP64)
   DBuf egrad;                    // double [n][3] dE/dx

@@ -966,13 +966,19 @@ template <class real>
 void recip_t(Ctx& c, bool use_m, bool use_p, bool want_v) {
   const int n = c.n, nb = ceil_div(n, 256);
   const double f = kCoul;
-  // permanent multipoles
-  pme_spread(c, 1, c.rpole.p, nullptr);
-  pme_fft_virial(c, 1, (want_v && use_m) ? f : 0.0);
+  // permanent multipoles: the direct field of the same multipoles left its
+  // convolved grid; spread and transform again only for the virial sums
+  const void* mgrid = c.qgrid.p;
+  if (!(want_v && use_m) && c.qgrid_m_stamp == c.rpole_stamp && c.qgrid_m.p) {
+    mgrid = c.qgrid_m.p;
+  } else {
+    pme_spread(c, 1, c.rpole.p, nullptr);
+    pme_fft_virial(c, 1, (want_v && use_m) ? f : 0.0);
+  }
   {
     KScope ks_(c, KT_GATHER);
     hipLaunchKernelGGL(k_recip_mpole<real>, dim3(nb), dim3(256), 0, c.stream, n, ggeom<real>(c),
-                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), (const void*)c.qgrid.p, c.rpole.as<real>(),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), mgrid, c.rpole.as<real>(),
                        use_m ? 1 : 0, c.egrad.as<double>(), c.etq.as<double>(),
                        use_p ? c.ecphim.as<double>() : nullptr, energy_rows(c, nb));
     T9_LAUNCH_CHECK();

@@ -10,7 +10,9 @@
 // The influence function is the reference's pmeConv
 // (src/acc/pme.cpp:220-303): exp(-pi^2 h^2 / a^2) / (pi V h^2 b1 b2 b3),
 // k = 0 set to zero, terms with exponent below -50 dropped; it is evaluated
-// in FP64 on the fly.
+// in FP64 once per box into a table (z-pass order) and on the fly when the
+// virial sums are wanted.  Transform lengths used by the PME grids of
+// interest (24 ... 160) get kernels with compile-time index arithmetic.
 #include "ctx.hpp"
 #include "fft_core.hpp"
 
@@ -19,6 +21,7 @@ namespace {
 
 struct ConvArgs {
   int enabled;
+  const void* table;  // precomputed influence function (real [K1][K2][K3] in z-pass order), or null
   double pterm;    // (pi / alpha)^2
   double volterm;  // pi * V
   double il[3];    // 1 / box edge
@@ -39,12 +42,14 @@ __device__ __forceinline__ long long elem_addr(int axis, int K1, int K2, int lb,
   return (x0 + lb) + (long long)K1 * (other + (long long)K2 * e);
 }
 
-template <class real, int B>
+// NC > 0: the transform length is the compile-time NC (fast index
+// arithmetic); NC == 0: any 2,3,5-smooth length from `plan`.
+template <class real, int B, int NC>
 __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid, int K1, int K2, int K3, int axis,
                                                   int sign, FftPlan plan, const Cplx<real>* __restrict__ tw,
                                                   ConvArgs conv) {
   extern __shared__ __align__(16) char smem[];
-  const int N = plan.n;
+  const int N = NC > 0 ? NC : plan.n;
   Cplx<real>* buf0 = reinterpret_cast<Cplx<real>*>(smem);
   const int LP = N + 1;  // padded LDS line pitch: adjacent lines start on different banks
   Cplx<real>* buf1 = buf0 + B * LP;
@@ -73,8 +78,24 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     buf0[lb * LP + e] = grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)];
   }
   __syncthreads();
-  Cplx<real>* res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
-  if (conv.enabled) {
+  Cplx<real>* res;
+  if constexpr (NC > 0) res = fft_stages_ct<real, NC>(buf0, buf1, nl, tw, sign, threadIdx.x, blockDim.x);
+  else res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
+  if (conv.enabled && conv.table && !conv.vrows) {
+    // convolution with the tabulated influence function
+    const real* tb = (const real*)conv.table + ((long long)other * K1 + x0) * N;
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+      const real f = tb[q];
+      Cplx<real> v = res[(q / N) * LP + (q % N)];
+      v.x *= f;
+      v.y *= f;
+      res[(q / N) * LP + (q % N)] = v;
+    }
+    __syncthreads();
+    Cplx<real>* spare = res == buf0 ? buf1 : buf0;
+    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, tw, -sign, threadIdx.x, blockDim.x);
+    else res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
+  } else if (conv.enabled) {
     // z pass: lines are (x0 + lb, y = other), element index is k3.  The
     // strain derivative of 1/2 sum_k G(k) |S(k)|^2 (G the influence function)
     // is accumulated from the transformed values before they are scaled.
@@ -121,7 +142,8 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     if (conv.fwd_only) return;
     __syncthreads();
     Cplx<real>* spare = res == buf0 ? buf1 : buf0;
-    res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
+    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, tw, -sign, threadIdx.x, blockDim.x);
+    else res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
   }
   for (int q = threadIdx.x; q < tot; q += blockDim.x) {
     int lb, e;
@@ -134,6 +156,30 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     }
     grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)] = res[lb * LP + e];
   }
+}
+
+// Influence function in z-pass order: table[(k2 * K1 + k1) * K3 + k3]
+// (lines at consecutive x, element index k3), the same FP64 expression as
+// the on-the-fly path.
+template <class real>
+__global__ void k_conv_table(int K1, int K2, int K3, double pterm, double volterm, double il0, double il1, double il2,
+                             const double* __restrict__ bsmod, real* __restrict__ table) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= (long long)K1 * K2 * K3) return;
+  const int k3 = (int)(e % K3);
+  const long long l = e / K3;
+  const int k1 = (int)(l % K1), k2 = (int)(l / K1);
+  double f = 0;
+  if (k1 != 0 || k2 != 0 || k3 != 0) {
+    const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
+    const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
+    const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
+    const double h1 = r1 * il0, h2 = r2 * il1, h3 = r3 * il2;
+    const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
+    const double term = -pterm * hsq;
+    if (term > -50) f = exp(term) / (hsq * volterm * bsmod[k1] * bsmod[K1 + k2] * bsmod[K1 + K2 + k3]);
+  }
+  table[e] = (real)f;
 }
 
 template <class real>
@@ -161,12 +207,35 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
   conv.vrows = vrows;
   const int xb = (K1 + B - 1) / B;
   const Cplx<real>* twa[3] = {tw, tw + K1, tw + K1 + K2};
+  if (!c.conv_table_valid) {
+    c.conv_table.alloc(sizeof(real) * (size_t)K1 * K2 * K3);
+    hipLaunchKernelGGL(k_conv_table<real>, dim3(ceil_div((long)K1 * K2 * K3, 256)), dim3(256), 0, c.stream, K1, K2, K3,
+                       conv.pterm, conv.volterm, conv.il[0], conv.il[1], conv.il[2], conv.bsmod,
+                       c.conv_table.as<real>());
+    T9_LAUNCH_CHECK();
+    c.conv_table_valid = true;
+  }
+  conv.table = c.conv_table.p;
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
     const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
     const size_t sh = 2 * (size_t)B * (p[axis].n + 1) * sizeof(Cplx<real>);
     KScope ks_(c, KT_FFT);
-    hipLaunchKernelGGL((k_fft_pass<real, B>), dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign,
-                       p[axis], twa[axis], cv);
+    void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs);
+    switch (p[axis].n) {
+      case 24: kern = k_fft_pass<real, B, 24>; break;
+      case 32: kern = k_fft_pass<real, B, 32>; break;
+      case 40: kern = k_fft_pass<real, B, 40>; break;
+      case 48: kern = k_fft_pass<real, B, 48>; break;
+      case 64: kern = k_fft_pass<real, B, 64>; break;
+      case 80: kern = k_fft_pass<real, B, 80>; break;
+      case 96: kern = k_fft_pass<real, B, 96>; break;
+      case 120: kern = k_fft_pass<real, B, 120>; break;
+      case 128: kern = k_fft_pass<real, B, 128>; break;
+      case 160: kern = k_fft_pass<real, B, 160>; break;
+      default: kern = k_fft_pass<real, B, 0>; break;
+    }
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign, p[axis], twa[axis],
+                       cv);
     T9_LAUNCH_CHECK();
   };
   launch(0, -1, none);

@@ -151,6 +151,70 @@ __host__ __device__ __forceinline__ Cplx<real>* fft_stages(Cplx<real>* buf0, Cpl
   return src;
 }
 
+// Compile-time plan: the st-th radix of N (same order as make_fft_plan).
+__host__ __device__ constexpr int ct_radix(int N, int st) {
+  int m = N, c = 0;
+  const int rs[4] = {4, 2, 3, 5};
+  for (int a = 0; a < 4; ++a)
+    while (m % rs[a] == 0) {
+      if (c == st) return rs[a];
+      m /= rs[a];
+      ++c;
+    }
+  return 0;
+}
+__host__ __device__ constexpr int ct_nstage(int N) {
+  int c = 0;
+  while (ct_radix(N, c) != 0) ++c;
+  return c;
+}
+
+// One Stockham stage with N, R and Ns known at compile time (the index
+// arithmetic reduces to multiplications and shifts); lines of pitch N + 1.
+template <int R, int N, int NS, class real>
+__device__ __forceinline__ void stockham_stage_ct(const Cplx<real>* src, Cplx<real>* dst, int nl,
+                                                  const Cplx<real>* tw, real sg, int tid, int nthr) {
+  constexpr int nb = N / R;
+  constexpr int tstep = N / (NS * R);
+  constexpr int P = N + 1;
+  const int total = nl * nb;
+  for (int q = tid; q < total; q += nthr) {
+    const int lb = q / nb;
+    const int j = q - lb * nb;
+    const int jd = j / NS;
+    const int k = j - jd * NS;
+    const Cplx<real>* s = src + lb * P + j;
+    Cplx<real> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        Cplx<real> t = tw[k * r * tstep];
+        t.y = sg > 0 ? -t.y : t.y;
+        v[r] = cmulc(v[r], t);
+      }
+    }
+    small_dft<R>(v, sg);
+    Cplx<real>* d = dst + lb * P + jd * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r * NS] = v[r];
+  }
+}
+
+template <class real, int N, int ST = 0, int NS = 1>
+__device__ __forceinline__ Cplx<real>* fft_stages_ct(Cplx<real>* src, Cplx<real>* dst, int nl, const Cplx<real>* tw,
+                                                    int sign, int tid, int nthr) {
+  if constexpr (ST == ct_nstage(N)) {
+    return src;
+  } else {
+    constexpr int R = ct_radix(N, ST);
+    stockham_stage_ct<R, N, NS, real>(src, dst, nl, tw, (real)sign, tid, nthr);
+    __syncthreads();
+    return fft_stages_ct<real, N, ST + 1, NS * R>(dst, src, nl, tw, sign, tid, nthr);
+  }
+}
+
 inline FftPlan make_fft_plan(int N) {
   FftPlan p{};
   p.n = N;

@@ -121,6 +121,7 @@ void mpole_init_t(Ctx& c) {
                      c.frot.as<float>());
   T9_LAUNCH_CHECK();
   c.rpole_valid = true;
+  ++c.rpole_stamp;
 }
 
 }  // namespace

@@ -132,7 +132,7 @@ struct TileGeom {
 // Integer addition is associative, so the tile holds the same bits whatever
 // order the items run in; it is converted and written to the grid once.
 // Every grid point has one owner: no global atomics, bit-reproducible.
-constexpr int kSpreadChunk = 256;
+constexpr int kSpreadChunk = 64;
 constexpr int kMaxTile = 16;
 
 template <class real, int MODE>
@@ -547,6 +547,12 @@ void mpole_field_t(Ctx& c, void* fd, void* fp) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 1, c.rpole.p, nullptr);
   pme_fft_roundtrip(c);
+  // keep the convolved grid: the reciprocal multipole energy of the same
+  // coordinates gathers from it instead of spreading and transforming again
+  const size_t gbytes = 2 * sizeof(real) * (size_t)c.grid[0] * c.grid[1] * c.grid[2];
+  c.qgrid_m.alloc(gbytes);
+  T9_HIP(hipMemcpyAsync(c.qgrid_m.p, c.qgrid.p, gbytes, hipMemcpyDeviceToDevice, c.stream));
+  c.qgrid_m_stamp = c.rpole_stamp;
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
