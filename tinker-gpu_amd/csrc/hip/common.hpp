@@ -236,6 +236,29 @@ __device__ __forceinline__ void ewald_bn(real* bn, real r, real rinv, real rr2, 
   }
 }
 
+// Pair kernels run kPairParts wavefronts per 64 atoms: wave p of a
+// workgroup walks list entries p, p + kPairParts, ... of each of the 64
+// atoms, and the parts are summed in part order through LDS (parts_reduce),
+// so results do not depend on scheduling.  Four waves per 64 atoms keep
+// ~6 waves per SIMD busy on 96k atoms instead of ~1.5.
+constexpr int kPairParts = 4;
+
+// v[0..NV) of wave 0 += the same values of waves 1..NP-1, in that order;
+// sh: (NP - 1) * NV * 64 doubles.  Every thread of the workgroup calls it.
+template <int NP, int NV>
+__device__ __forceinline__ void parts_reduce(double* v, double* sh) {
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  if (part > 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[((part - 1) * NV + q) * 64 + lane] = v[q];
+  __syncthreads();
+  if (part == 0)
+    for (int p = 1; p < NP; ++p)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] += sh[((p - 1) * NV + q) * 64 + lane];
+  __syncthreads();
+}
+
 // Deterministic block sum of doubles: fixed tree over a fixed block size.
 template <int BLOCK>
 __device__ __forceinline__ double block_sum(double v, double* sh) {

@@ -184,6 +184,7 @@ struct Ctx {
   int nealt = 0;
   DBuf uaspc;                    // real [2][n][3] sorted: ASPC guess of the current solve
   DBuf gfeat;                    // float [n][32] sorted raw features of the current solve
+  DBuf gde;                      // float [n][6] caller order: field extrapolation errors (d, p)
   DBuf gx;                       // bf16 [n][32] normalized features
   DBuf gy;                       // float [n][8] network outputs
   DBuf gp, gm, gv, gpb, ggrad;   // FP32 params / Adam moments, bf16 copy, gradient

@@ -208,6 +208,20 @@ __device__ __forceinline__ void thole4(real r, real rpdik, real pga, real* lam) 
 template <class real>
 struct Acc {
   double e, gr[3], tg[3], th[6], v[6];
+  __device__ __forceinline__ void pack(double* o) const {
+    o[0] = e;
+    for (int k = 0; k < 3; ++k) o[1 + k] = gr[k];
+    for (int k = 0; k < 3; ++k) o[4 + k] = tg[k];
+    for (int k = 0; k < 6; ++k) o[7 + k] = th[k];
+    for (int k = 0; k < 6; ++k) o[13 + k] = v[k];
+  }
+  __device__ __forceinline__ void unpack(const double* o) {
+    e = o[0];
+    for (int k = 0; k < 3; ++k) gr[k] = o[1 + k];
+    for (int k = 0; k < 3; ++k) tg[k] = o[4 + k];
+    for (int k = 0; k < 6; ++k) th[k] = o[7 + k];
+    for (int k = 0; k < 6; ++k) v[k] = o[13 + k];
+  }
   __device__ void zero() {
     e = 0;
     for (int k = 0; k < 3; ++k) gr[k] = tg[k] = 0;
@@ -264,9 +278,11 @@ struct EmplarArgs {
 
 // MODE bit 0: multipole terms, bit 1: polarization terms.
 template <class real, int EWALD, int MODE>
-__global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
+__global__ __launch_bounds__(64 * kPairParts) void k_emplar(EmplarArgs<real> A) {
   constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sh[(kPairParts - 1) * 19 * 64];
+  const int part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   Acc<real> acc;
   acc.zero();
   if (s < A.n) {
@@ -282,11 +298,11 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
     const int jpi = A.uth ? 0 : A.jpolar[s] * A.nj;
     const int len = A.ncut[s];
     const uint32_t* e = A.ecut + A.woff[s >> 6] + (s & 63);
-    uint32_t nxt = len > 0 ? e[0] : 0;
-    for (int kk = 0; kk < len; ++kk) {
+    uint32_t nxt = part < len ? e[(long long)part * 64] : 0;
+    for (int kk = part; kk < len; kk += kPairParts) {
       const uint32_t ent = nxt;
-      if (kk + 1 < len) nxt = e[(long long)(kk + 1) * 64];
-      if (ent & (kMaxCodes - 1)) continue;  // scaled pairs: FP64 loop below
+      if (kk + kPairParts < len) nxt = e[(long long)(kk + kPairParts) * 64];
+      if (ent & (kMaxCodes - 1)) continue;  // scaled pairs: k_emplar_scaled
       const int j = (int)(ent >> kCodeBits);
       const PosRec<real> pj = A.pos[j];
       real R[3];
@@ -328,15 +344,23 @@ __global__ __launch_bounds__(256) void k_emplar(EmplarArgs<real> A) {
       pair_terms<real, DM, DP>(R, bn, BT, mi, mk, ui, uk, udi, upi, udk, upk, (real)1, o);
       acc.add(o, R);
     }
-    const double f = kCoul;
-    for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] = f * acc.gr[k];
-    for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] = f * acc.tg[k];
-    for (int k = 0; k < 6; ++k) A.tq[9 * (size_t)s + 3 + k] = f * acc.th[k];
   }
+  double av[19];
+  acc.pack(av);
+  parts_reduce<kPairParts, 19>(av, sh);
   double red[EP_N];
   for (int q = 0; q < EP_N; ++q) red[q] = 0;
-  red[EP_MREAL] = 0.5 * kCoul * acc.e;
-  for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * kCoul * acc.v[k];
+  if (part == 0) {
+    acc.unpack(av);
+    if (s < A.n) {
+      const double f = kCoul;
+      for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] = f * acc.gr[k];
+      for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] = f * acc.tg[k];
+      for (int k = 0; k < 6; ++k) A.tq[9 * (size_t)s + 3 + k] = f * acc.th[k];
+    }
+    red[EP_MREAL] = 0.5 * kCoul * acc.e;
+    for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * kCoul * acc.v[k];
+  }
   block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kRow);
 }
 
@@ -930,7 +954,6 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   A.grad = c.egrad.as<double>();
   A.tq = c.etq.as<double>();
   const int nb = ceil_div(c.n, 256);
-  A.rows = energy_rows(c, nb);
   const int mode = (use_m ? 1 : 0) | (use_p ? 2 : 0);
   void (*kern)(EmplarArgs<real>) = nullptr;
   if (c.ewald) kern = mode == 1 ? k_emplar<real, 1, 1> : (mode == 2 ? k_emplar<real, 1, 2> : k_emplar<real, 1, 3>);
@@ -942,9 +965,10 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   else
     kscaled = mode == 1 ? k_emplar_scaled<real, 0, 1>
                         : (mode == 2 ? k_emplar_scaled<real, 0, 2> : k_emplar_scaled<real, 0, 3>);
+  A.rows = energy_rows(c, c.nwave);
   {
     KScope ks_(c, KT_EMPLAR);
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, A);
+    hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
     T9_LAUNCH_CHECK();
   }
   A.rows = energy_rows(c, nb);
@@ -1096,7 +1120,7 @@ void energy_setup(Ctx& c, const System& s) {
   const int nb = ceil_div(n, 256);
   long long lines = 0;
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
-  c.ecap = (int)(9LL * nb + lines + 16);
+  c.ecap = (int)(9LL * nb + 2LL * ceil_div(n, 64) + lines + 16);  // list loops: one row per 64 atoms
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }

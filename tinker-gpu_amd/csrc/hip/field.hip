@@ -10,9 +10,11 @@
 // Execution: one lane per sorted atom, 64 consecutive atoms per wavefront,
 // each lane walking its own in-cutoff neighbor list (compacted once per
 // coordinate update from the buffered list, see nblist.hip), so no lane
-// idles on out-of-range entries and no distance test is repeated.  A lane
-// accumulates its atom's field in FP64 registers and stores it once, so the
-// result is independent of scheduling (no atomics).
+// idles on out-of-range entries and no distance test is repeated.  The
+// kPairParts waves of a workgroup share the same 64 atoms and take every
+// kPairParts-th list entry; each accumulates in FP64 registers and the parts
+// are added in a fixed order, so the result is independent of scheduling
+// (no atomics).
 //
 // Per-pair data layout: pos[j] = (x, y, z, 1/pdamp) so one 16-byte load
 // brings coordinates and the Thole radius; when every pair shares one Thole
@@ -97,69 +99,84 @@ __device__ __forceinline__ void ufield_pair(uint32_t ent, const R4<real>& pi, co
 }
 
 template <class real, int EWALD, int UTH>
-__global__ __launch_bounds__(256) void k_ufield_real(int n, const int* __restrict__ ncut,
-                                                     const long long* __restrict__ woff,
-                                                     const uint32_t* __restrict__ ecut,
-                                                     const R4<real>* __restrict__ pos, const real* __restrict__ ud,
-                                                     const real* __restrict__ up, const int* __restrict__ jpolar,
-                                                     const real* __restrict__ thlval, int nj, real pga_u,
-                                                     ScaleTable sc, BoxT<real> box, EwaldK<real> ek,
-                                                     real* __restrict__ fd, real* __restrict__ fp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int len = ncut[s];
-  const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
-  const R4<real> pi = pos[s];
-  const int jpi = UTH ? 0 : jpolar[s] * nj;
-  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
-  int k = 0;
-  // two independent pairs per iteration: their gathers overlap in flight
-  for (; k + 1 < len; k += 2) {
-    const uint32_t e0 = e[(long long)k * 64], e1 = e[(long long)(k + 1) * 64];
-    const int j0 = (int)(e0 >> kCodeBits), j1 = (int)(e1 >> kCodeBits);
-    const R4<real> p0 = pos[j0], p1 = pos[j1];
-    const real d0x = ud[3 * j0], d0y = ud[3 * j0 + 1], d0z = ud[3 * j0 + 2];
-    const real q0x = up[3 * j0], q0y = up[3 * j0 + 1], q0z = up[3 * j0 + 2];
-    const real d1x = ud[3 * j1], d1y = ud[3 * j1 + 1], d1z = ud[3 * j1 + 2];
-    const real q1x = up[3 * j1], q1y = up[3 * j1 + 1], q1z = up[3 * j1 + 2];
-    const int t0 = UTH ? 0 : jpolar[j0], t1 = UTH ? 0 : jpolar[j1];
-    ufield_pair<real, EWALD, UTH>(e0, pi, p0, d0x, d0y, d0z, q0x, q0y, q0z, jpi, t0, thlval, pga_u, sc, box, ek, a, b);
-    ufield_pair<real, EWALD, UTH>(e1, pi, p1, d1x, d1y, d1z, q1x, q1y, q1z, jpi, t1, thlval, pga_u, sc, box, ek, a, b);
+__global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const int* __restrict__ ncut,
+                                                                 const long long* __restrict__ woff,
+                                                                 const uint32_t* __restrict__ ecut,
+                                                                 const R4<real>* __restrict__ pos,
+                                                                 const real* __restrict__ ud,
+                                                                 const real* __restrict__ up,
+                                                                 const int* __restrict__ jpolar,
+                                                                 const real* __restrict__ thlval, int nj, real pga_u,
+                                                                 ScaleTable sc, BoxT<real> box, EwaldK<real> ek,
+                                                                 real* __restrict__ fd, real* __restrict__ fp) {
+  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  const int part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  double* a = ab;
+  double* b = ab + 3;
+  if (s < n) {
+    const int len = ncut[s];
+    const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
+    const R4<real> pi = pos[s];
+    const int jpi = UTH ? 0 : jpolar[s] * nj;
+    int k = part;
+    constexpr int P = kPairParts;
+    // two independent pairs per iteration: their gathers overlap in flight
+    for (; k + P < len; k += 2 * P) {
+      const uint32_t e0 = e[(long long)k * 64], e1 = e[(long long)(k + P) * 64];
+      const int j0 = (int)(e0 >> kCodeBits), j1 = (int)(e1 >> kCodeBits);
+      const R4<real> p0 = pos[j0], p1 = pos[j1];
+      const real d0x = ud[3 * j0], d0y = ud[3 * j0 + 1], d0z = ud[3 * j0 + 2];
+      const real q0x = up[3 * j0], q0y = up[3 * j0 + 1], q0z = up[3 * j0 + 2];
+      const real d1x = ud[3 * j1], d1y = ud[3 * j1 + 1], d1z = ud[3 * j1 + 2];
+      const real q1x = up[3 * j1], q1y = up[3 * j1 + 1], q1z = up[3 * j1 + 2];
+      const int t0 = UTH ? 0 : jpolar[j0], t1 = UTH ? 0 : jpolar[j1];
+      ufield_pair<real, EWALD, UTH>(e0, pi, p0, d0x, d0y, d0z, q0x, q0y, q0z, jpi, t0, thlval, pga_u, sc, box, ek, a,
+                                    b);
+      ufield_pair<real, EWALD, UTH>(e1, pi, p1, d1x, d1y, d1z, q1x, q1y, q1z, jpi, t1, thlval, pga_u, sc, box, ek, a,
+                                    b);
+    }
+    if (k < len) {
+      const uint32_t e0 = e[(long long)k * 64];
+      const int j0 = (int)(e0 >> kCodeBits);
+      const int t0 = UTH ? 0 : jpolar[j0];
+      ufield_pair<real, EWALD, UTH>(e0, pi, pos[j0], ud[3 * j0], ud[3 * j0 + 1], ud[3 * j0 + 2], up[3 * j0],
+                                    up[3 * j0 + 1], up[3 * j0 + 2], jpi, t0, thlval, pga_u, sc, box, ek, a, b);
+    }
   }
-  if (k < len) {
-    const uint32_t e0 = e[(long long)k * 64];
-    const int j0 = (int)(e0 >> kCodeBits);
-    const int t0 = UTH ? 0 : jpolar[j0];
-    ufield_pair<real, EWALD, UTH>(e0, pi, pos[j0], ud[3 * j0], ud[3 * j0 + 1], ud[3 * j0 + 2], up[3 * j0],
-                                  up[3 * j0 + 1], up[3 * j0 + 2], jpi, t0, thlval, pga_u, sc, box, ek, a, b);
+  parts_reduce<kPairParts, 6>(ab, sh);
+  if (part == 0 && s < n) {
+    fd[3 * s] = (real)a[0];
+    fd[3 * s + 1] = (real)a[1];
+    fd[3 * s + 2] = (real)a[2];
+    fp[3 * s] = (real)b[0];
+    fp[3 * s + 1] = (real)b[1];
+    fp[3 * s + 2] = (real)b[2];
   }
-  fd[3 * s] = (real)a[0];
-  fd[3 * s + 1] = (real)a[1];
-  fd[3 * s + 2] = (real)a[2];
-  fp[3 * s] = (real)b[0];
-  fp[3 * s + 1] = (real)b[1];
-  fp[3 * s + 2] = (real)b[2];
 }
 
 template <class real, int EWALD, int UTH>
-__global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restrict__ ncut,
+__global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const int* __restrict__ ncut,
                                                      const long long* __restrict__ woff,
                                                      const uint32_t* __restrict__ ecut,
                                                      const R4<real>* __restrict__ pos, const real* __restrict__ rpole,
                                                      const int* __restrict__ jpolar, const real* __restrict__ thlval,
                                                      int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                      EwaldK<real> ek, real* __restrict__ fd, real* __restrict__ fp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int len = ncut[s];
-  const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
-  const R4<real> pi = pos[s];
-  const int jpi = UTH ? 0 : jpolar[s] * nj;
-  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-  uint32_t nxt = len > 0 ? e[0] : 0;
-  for (int k = 0; k < len; ++k) {
+  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  const int part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  double &a0 = ab[0], &a1 = ab[1], &a2 = ab[2], &b0 = ab[3], &b1 = ab[4], &b2 = ab[5];
+  const int len = s < n ? ncut[s] : 0;
+  const uint32_t* e = ecut + (s < n ? woff[s >> 6] + (s & 63) : 0);
+  const R4<real> pi = pos[s < n ? s : 0];
+  const int jpi = UTH ? 0 : jpolar[s < n ? s : 0] * nj;
+  uint32_t nxt = part < len ? e[(long long)part * 64] : 0;
+  for (int k = part; k < len; k += kPairParts) {
     const uint32_t ent = nxt;
-    if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
+    if (k + kPairParts < len) nxt = e[(long long)(k + kPairParts) * 64];
     const int j = (int)(ent >> kCodeBits);
     const R4<real> pj = pos[j];
     real xr, yr, zr;
@@ -222,18 +239,21 @@ __global__ __launch_bounds__(256) void k_dfield_real(int n, const int* __restric
     b1 += (double)fy;
     b2 += (double)fz;
   }
-  fd[3 * s] = (real)a0;
-  fd[3 * s + 1] = (real)a1;
-  fd[3 * s + 2] = (real)a2;
-  fp[3 * s] = (real)b0;
-  fp[3 * s + 1] = (real)b1;
-  fp[3 * s + 2] = (real)b2;
+  parts_reduce<kPairParts, 6>(ab, sh);
+  if (part == 0 && s < n) {
+    fd[3 * s] = (real)a0;
+    fd[3 * s + 1] = (real)a1;
+    fd[3 * s + 2] = (real)a2;
+    fp[3 * s] = (real)b0;
+    fp[3 * s + 1] = (real)b1;
+    fp[3 * s + 2] = (real)b2;
+  }
 }
 
 // z = udiag * alpha * r + sum_{r_ik <= ucut} alpha_i alpha_k T_ik r_k
 // (sparse preconditioner, src/acc/amoeba/induce.cpp:310-420).
 template <class real, int UTH>
-__global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, const int* __restrict__ ncut,
+__global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, real udiag, const int* __restrict__ ncut,
                                                  const long long* __restrict__ woff,
                                                  const uint32_t* __restrict__ ecut,
                                                  const R4<real>* __restrict__ pos, const real* __restrict__ polarity,
@@ -241,21 +261,31 @@ __global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, 
                                                  int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                  const real* __restrict__ rd, const real* __restrict__ rp,
                                                  real* __restrict__ zd, real* __restrict__ zp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const real poli = polarity[s];
+  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  const int part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int si = s < n ? s : 0;
+  const real poli = polarity[si];
   const real f = sparse ? udiag * poli : poli;
-  double a0 = f * rd[3 * s], a1 = f * rd[3 * s + 1], a2 = f * rd[3 * s + 2];
-  double b0 = f * rp[3 * s], b1 = f * rp[3 * s + 1], b2 = f * rp[3 * s + 2];
-  if (sparse) {
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  double &a0 = ab[0], &a1 = ab[1], &a2 = ab[2], &b0 = ab[3], &b1 = ab[4], &b2 = ab[5];
+  if (part == 0 && s < n) {
+    a0 = f * rd[3 * s];
+    a1 = f * rd[3 * s + 1];
+    a2 = f * rd[3 * s + 2];
+    b0 = f * rp[3 * s];
+    b1 = f * rp[3 * s + 1];
+    b2 = f * rp[3 * s + 2];
+  }
+  if (sparse && s < n) {
     const int len = ncut[s];
     const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
     const R4<real> pi = pos[s];
     const int jpi = UTH ? 0 : jpolar[s] * nj;
-    uint32_t nxt = len > 0 ? e[0] : 0;
-    for (int k = 0; k < len; ++k) {
+    uint32_t nxt = part < len ? e[(long long)part * 64] : 0;
+    for (int k = part; k < len; k += kPairParts) {
       const uint32_t ent = nxt;
-      if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
+      if (k + kPairParts < len) nxt = e[(long long)(k + kPairParts) * 64];
       const int j = (int)(ent >> kCodeBits);
       const R4<real> pj = pos[j];
       real xr, yr, zr;
@@ -284,12 +314,15 @@ __global__ __launch_bounds__(256) void k_precond(int n, int sparse, real udiag, 
       b2 += (double)(m2 * y0 + m4 * y1 + m5 * y2);
     }
   }
-  zd[3 * s] = (real)a0;
-  zd[3 * s + 1] = (real)a1;
-  zd[3 * s + 2] = (real)a2;
-  zp[3 * s] = (real)b0;
-  zp[3 * s + 1] = (real)b1;
-  zp[3 * s + 2] = (real)b2;
+  parts_reduce<kPairParts, 6>(ab, sh);
+  if (part == 0 && s < n) {
+    zd[3 * s] = (real)a0;
+    zd[3 * s + 1] = (real)a1;
+    zd[3 * s + 2] = (real)a2;
+    zp[3 * s] = (real)b0;
+    zp[3 * s + 1] = (real)b1;
+    zp[3 * s + 2] = (real)b2;
+  }
 }
 
 
@@ -315,7 +348,7 @@ void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   auto kern = c.ewald ? (c.uniform_thole ? k_ufield_real<real, 1, 1> : k_ufield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_ufield_real<real, 0, 1> : k_ufield_real<real, 0, 0>);
   KScope ks_(c, KT_UFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
                      (const real*)up, c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale,
                      make_boxT<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp);
@@ -329,7 +362,7 @@ void dfield_real_t(Ctx& c) {
   auto kern = c.ewald ? (c.uniform_thole ? k_dfield_real<real, 1, 1> : k_dfield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_dfield_real<real, 0, 1> : k_dfield_real<real, 0, 0>);
   KScope ks_(c, KT_DFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
                      c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, make_boxT<real>(c),
                      ewald_k<real>(c), c.field.as<real>(), c.fieldp.as<real>());
@@ -343,7 +376,7 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
   if (sparse) require_cut(L);
   auto kern = c.uniform_thole ? k_precond<real, 1> : k_precond<real, 0>;
   KScope ks_(c, KT_PRECOND);
-  hipLaunchKernelGGL(kern, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, sparse ? 1 : 0,
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, sparse ? 1 : 0,
                      (real)c.opt.udiag, L.ncut.as<int>(), L.off.as<long long>(), L.ecut.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
                      (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,

@@ -298,32 +298,42 @@ __device__ __forceinline__ void rot3(const float* R, const float* v, float* o) {
   for (int k = 0; k < 3; ++k) o[k] = R[3 * k] * v[0] + R[3 * k + 1] * v[1] + R[3 * k + 2] * v[2];
 }
 
-// dE (d and p) of caller atom j: current field (sorted slot sj) minus the
-// ASPC combination of its history (slots weighted by ce)
+// dE (d and p) of each caller-order atom i: the current field (sorted slot
+// iperm[i]) minus the ASPC combination of its history (slots weighted by ce);
+// lanes in caller order so the history rows are read contiguously.
 template <class real>
-__device__ void field_err(int n, int j, int sj, const real* fd, const real* fp, const real* ealt, const Coef16f& ce,
-                          float* dd, float* dp) {
+__global__ __launch_bounds__(256) void k_field_err(int n, const int* __restrict__ iperm, const real* __restrict__ fd,
+                                                   const real* __restrict__ fp, const real* __restrict__ ealt,
+                                                   Coef16f ce, float* __restrict__ de) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int si = iperm[i];
   const size_t slot = 6 * (size_t)n;
+  double a[3], b[3];
   for (int d = 0; d < 3; ++d) {
-    double a = (double)fd[3 * sj + d], b = (double)fp[3 * sj + d];
-    for (int k = 0; k < 16; ++k) {
-      a -= ce.c[k] * (double)ealt[k * slot + 3 * (size_t)j + d];
-      b -= ce.c[k] * (double)ealt[k * slot + 3 * (size_t)n + 3 * (size_t)j + d];
+    a[d] = (double)fd[3 * si + d];
+    b[d] = (double)fp[3 * si + d];
+  }
+  for (int k = 0; k < 16; ++k) {
+    const real* h = ealt + k * slot;
+    for (int d = 0; d < 3; ++d) {
+      a[d] -= ce.c[k] * (double)h[3 * (size_t)i + d];
+      b[d] -= ce.c[k] * (double)h[3 * (size_t)n + 3 * (size_t)i + d];
     }
-    dd[d] = (float)a;
-    dp[d] = (float)b;
+  }
+  for (int d = 0; d < 3; ++d) {
+    de[6 * (size_t)i + d] = (float)a[d];
+    de[6 * (size_t)i + 3 + d] = (float)b[d];
   }
 }
 
 template <class real>
-__global__ __launch_bounds__(256) void k_features(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
-                                                  const int* __restrict__ frame, const int* __restrict__ zaxis,
-                                                  const int* __restrict__ xaxis, const float* __restrict__ frot,
-                                                  const real* __restrict__ polarity, const int* __restrict__ jpolar,
-                                                  const real* __restrict__ fd, const real* __restrict__ fp,
-                                                  const real* __restrict__ ealt, Coef16f ce,
-                                                  const real* __restrict__ ualt, int u0, int u1,
-                                                  float* __restrict__ F) {
+__global__ __launch_bounds__(256) void k_features(int n, const int* __restrict__ perm, const int* __restrict__ frame,
+                                                  const int* __restrict__ zaxis, const int* __restrict__ xaxis,
+                                                  const float* __restrict__ frot, const real* __restrict__ polarity,
+                                                  const int* __restrict__ jpolar, const real* __restrict__ fd,
+                                                  const float* __restrict__ de, const real* __restrict__ ualt,
+                                                  int u0, int u1, float* __restrict__ F) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
@@ -333,12 +343,11 @@ __global__ __launch_bounds__(256) void k_features(int n, const int* __restrict__
   float R[9];
   for (int q = 0; q < 9; ++q) R[q] = frot[9 * (size_t)s + q];
   float* f = F + (size_t)s * NF;
-  float a[3], b[3];
   const int js[3] = {i, zi, xi};
   for (int t = 0; t < 3; ++t) {
-    field_err<real>(n, js[t], iperm[js[t]], fd, fp, ealt, ce, a, b);
-    rot3(R, a, f + 6 * t);
-    rot3(R, b, f + 6 * t + 3);
+    const float* e = de + 6 * (size_t)js[t];
+    rot3(R, e, f + 6 * t);
+    rot3(R, e + 3, f + 6 * t + 3);
   }
   float e[3];
   for (int d = 0; d < 3; ++d) e[d] = (float)fd[3 * s + d];
@@ -456,10 +465,14 @@ void features_t(Ctx& c, const double* ce, const double* cu) {
   for (int k = 0; k < 16; ++k) cf.c[k] = ce[k];
   const int u0 = (int)cu[0], u1 = (int)cu[1];
   c.gfeat.alloc(sizeof(float) * NF * (size_t)c.n);
+  c.gde.alloc(sizeof(float) * 6 * (size_t)c.n);
+  hipLaunchKernelGGL(k_field_err<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.iperm.as<int>(),
+                     c.field.as<real>(), c.fieldp.as<real>(), c.ealt.as<real>(), cf, c.gde.as<float>());
+  T9_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_features<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
-                     c.iperm.as<int>(), c.frame.as<int>(), c.zaxis.as<int>(), c.xaxis.as<int>(), c.frot.as<float>(),
-                     c.polarity.as<real>(), c.jpolar.as<int>(), c.field.as<real>(), c.fieldp.as<real>(),
-                     c.ealt.as<real>(), cf, c.ualt.as<real>(), u0, u1, c.gfeat.as<float>());
+                     c.frame.as<int>(), c.zaxis.as<int>(), c.xaxis.as<int>(), c.frot.as<float>(),
+                     c.polarity.as<real>(), c.jpolar.as<int>(), c.field.as<real>(), c.gde.as<float>(),
+                     c.ualt.as<real>(), u0, u1, c.gfeat.as<float>());
   T9_LAUNCH_CHECK();
 }
 

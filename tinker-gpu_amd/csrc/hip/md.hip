@@ -69,11 +69,18 @@ struct VdwArgs {
 };
 
 // VIR: accumulate the virial (only when it is requested).  The site gradient
-// is summed in FP32 over runs of 16 list entries and flushed to FP64.
+// is summed in FP32 over runs of 16 list entries and flushed to FP64.  The
+// kPairParts waves of a workgroup split each site's list (entries
+// p, p + kPairParts, ...) and their sums are added in part order.
 template <class real, bool VIR>
-__global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0, g[3] = {0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+__global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
+  __shared__ double sh[(kPairParts - 1) * 10 * 64];
+  const int part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // e, g[3], v[6]
+  double& e = acc[0];
+  double* g = acc + 1;
+  double* v = acc + 4;
   if (s < A.n) {
     const PosRec<real> pi = A.pos[s];
     const int ji = A.jvdw[s] * A.nj;
@@ -81,11 +88,11 @@ __global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
     const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
     const real inv_tc = 1 / (A.taper - A.cut);
     real gs[3] = {0, 0, 0};
-    uint32_t nxt = len > 0 ? en[0] : 0;
-    for (int kk = 0; kk < len; ++kk) {
+    uint32_t nxt = part < len ? en[(long long)part * 64] : 0;
+    for (int kk = part; kk < len; kk += kPairParts) {
       const uint32_t ent = nxt;
-      if (kk + 1 < len) nxt = en[(long long)(kk + 1) * 64];
-      if ((kk & 15) == 15) {
+      if (kk + kPairParts < len) nxt = en[(long long)(kk + kPairParts) * 64];
+      if ((kk & 63) >= 64 - kPairParts) {  // every 16 entries of this part
         for (int d = 0; d < 3; ++d) {
           g[d] += (double)gs[d];
           gs[d] = 0;
@@ -139,12 +146,16 @@ __global__ __launch_bounds__(256) void k_vdw(VdwArgs<real> A) {
       }
     }
     for (int d = 0; d < 3; ++d) g[d] += (double)gs[d];
-    for (int d = 0; d < 3; ++d) A.gsite[3 * (size_t)s + d] = g[d];
   }
+  parts_reduce<kPairParts, 10>(acc, sh);
   double red[EP_N];
   for (int q = 0; q < EP_N; ++q) red[q] = 0;
-  red[EP_EVDW] = 0.5 * e;
-  for (int q = 0; q < 6; ++q) red[EP_VXX + q] = 0.5 * v[q];
+  if (part == 0) {
+    if (s < A.n)
+      for (int d = 0; d < 3; ++d) A.gsite[3 * (size_t)s + d] = g[d];
+    red[EP_EVDW] = 0.5 * e;
+    for (int q = 0; q < 6; ++q) red[EP_VXX + q] = 0.5 * v[q];
+  }
   block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
 }
 
@@ -370,11 +381,11 @@ void vdw_run_t(Ctx& c, bool vir) {
   A.taper = (real)c.vdw_taper;
   A.gsite = c.vsite_grad.as<double>();
   const int nb = ceil_div(c.n, 256);
-  A.rows = energy_rows(c, nb);
+  A.rows = energy_rows(c, c.nwave);
   {
     KScope ks_(c, KT_VDW);
-    if (vir) hipLaunchKernelGGL((k_vdw<real, true>), dim3(nb), dim3(256), 0, c.stream, A);
-    else hipLaunchKernelGGL((k_vdw<real, false>), dim3(nb), dim3(256), 0, c.stream, A);
+    if (vir) hipLaunchKernelGGL((k_vdw<real, true>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
+    else hipLaunchKernelGGL((k_vdw<real, false>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
     T9_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_vdw_to_atoms, dim3(nb), dim3(256), 0, c.stream, c.n, c.perm.as<int>(), c.iperm.as<int>(),

@@ -303,26 +303,20 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     }
   }
   if (aspc) guess_save_fields(c);
+  bool learned_try = false;  // the learned guess is in use; judged at the first readback
   if (learned) {
-    // the ASPC residual is re-measured every 20 learned solves; the learned
-    // guess is kept only when its residual is not worse than that reference
-    if (c.guess_steps % 20 == 0) {
+    // the ASPC residual is re-measured every 50 learned solves (one extra
+    // matrix-vector product); between checks the learned guess is compared
+    // with that reference at the first iteration's readback
+    if (c.guess_steps % 50 == 0) {
       residual();
       c.aspc_eps0_ref = residual_rms();
     }
     ++c.guess_steps;
     guess_apply(c, ud, up);
     residual();
-    const double eps_learned = residual_rms();
-    if (!(eps_learned <= c.aspc_eps0_ref)) {  // worse, or not finite
-      T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
-      T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
-      residual();
-      ++c.guess_fallback;
-    } else {
-      c.last_predictor = 2;
-      ++c.guess_used;
-    }
+    learned_try = true;
+    c.last_predictor = 2;
   } else if (predict) {
     residual();
   } else if (guess) {
@@ -332,22 +326,26 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, rd, rp);
   }
   T9_LAUNCH_CHECK();
-  // residual norm of the initial guess (Debye), read back with the first iteration
-  hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr, (const real*)nullptr,
-                     (const real*)nullptr, (const real*)nullptr, part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
-  T9_LAUNCH_CHECK();
-  precond_apply(c, rd, rp, zd, zp);
-  T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
-  T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, zd, rp, zp, (const real*)nullptr, (const real*)nullptr,
-                     (const real*)nullptr, (const real*)nullptr, part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 0, n, debye, part, sc);
-  T9_LAUNCH_CHECK();
+  // residual norm of the initial guess (Debye, read back with the first
+  // iteration), first preconditioned search direction
+  auto start_cg = [&]() {
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
+    T9_LAUNCH_CHECK();
+    precond_apply(c, rd, rp, zd, zp);
+    T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
+    T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, zd, rp, zp, (const real*)nullptr,
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 0, n, debye, part, sc);
+    T9_LAUNCH_CHECK();
+  };
+  start_cg();
 
   const int maxiter = 100;
   const int miniter = n < 3 ? n : 3;
-  int iter = 0;
+  int iter = 0, iter0 = 0;  // iter0: iterations spent before a restart from ASPC
   double eps = 100;
   bool done = false;
   while (!done) {
@@ -367,8 +365,22 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     T9_HIP(hipStreamSynchronize(st));
     eps = c.h_scalars[S_EPS];
+    if (learned_try) {
+      learned_try = false;
+      if (!(c.h_scalars[S_EPS0] <= c.aspc_eps0_ref)) {  // worse than ASPC, or not finite: restart
+        T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
+        T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
+        residual();
+        start_cg();
+        ++c.guess_fallback;
+        c.last_predictor = 1;
+        iter0 = iter;
+        continue;
+      }
+      ++c.guess_used;
+    }
     if (eps < c.opt.polar_eps) done = true;
-    if (iter < miniter) done = false;
+    if (iter - iter0 < miniter) done = false;
     if (iter >= c.opt.polar_iter) done = true;
     if (done) {
       hipLaunchKernelGGL(k_peek<real>, G, Bk, 0, st, n, (real)c.opt.pcg_peek, c.polarity.as<real>(), rd, rp, ud, up);
