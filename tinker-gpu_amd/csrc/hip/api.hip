@@ -689,8 +689,7 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
       for (int k = 1; k < nrespa; ++k) {
         t9::md_drift(c, dta);
         if (c.use_valence) {
-          c.erows = 0;
-          t9::valence_run(c, gf);
+          t9::valence_run(c, gf, false);
           t9::md_kick(c, dta, gf, 0.0, nullptr);  // velR0
         }
       }

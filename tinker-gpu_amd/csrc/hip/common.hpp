@@ -239,9 +239,9 @@ __device__ __forceinline__ void ewald_bn(real* bn, real r, real rinv, real rr2, 
 // Pair kernels run kPairParts wavefronts per 64 atoms: wave p of a
 // workgroup walks list entries p, p + kPairParts, ... of each of the 64
 // atoms, and the parts are summed in part order through LDS (parts_reduce),
-// so results do not depend on scheduling.  Four waves per 64 atoms keep
-// ~6 waves per SIMD busy on 96k atoms instead of ~1.5.
-constexpr int kPairParts = 4;
+// so results do not depend on scheduling.  Eight waves per 64 atoms keep
+// ~12 waves per SIMD in flight on 96k atoms instead of ~1.5.
+constexpr int kPairParts = 8;
 
 // v[0..NV) of wave 0 += the same values of waves 1..NP-1, in that order;
 // sh: (NP - 1) * NV * 64 doubles.  Every thread of the workgroup calls it.

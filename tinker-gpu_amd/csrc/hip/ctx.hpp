@@ -216,6 +216,8 @@ struct Ctx {
   DBuf vchild_off, vchild;       // caller order: reduced sites hanging on each atom
   DBuf vsite_grad;               // double [n][3] sorted
   DBuf vterm_off, vterm;         // caller order: valence terms touching each atom
+  DBuf vterm_grad;               // double [terms][3][3] per-term gradient of its atoms
+  int nbond = 0, nangle = 0, nurey = 0;
   DBuf bond_ij, angle_abc, urey_ac;    // int
   DBuf bond_kr, angle_kt, urey_kr;     // double
   double vpoly[6] = {0, 0, 0, 0, 0, 0};  // bond cubic, quartic; angle cubic..sextic
@@ -287,7 +289,7 @@ void vdw_sites(Ctx& c);                   // vdW site positions (sorted) from th
 // vdW (sorted egrad += ..) and valence (caller-order gval) terms of one
 // evaluation; partial rows reserved from the energy rows
 void vdw_run(Ctx& c, bool virial);
-void valence_run(Ctx& c, double* gval_caller);
+void valence_run(Ctx& c, double* gval_caller, bool energy);
 // Term groups of one evaluation: the electrostatics (multipole + polarization)
 // and vdW are the slow group of the RESPA split, valence the fast group.
 enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_ALL = 7 };

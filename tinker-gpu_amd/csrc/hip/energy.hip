@@ -52,6 +52,9 @@ template <class real>
 using R4 = typename RealTraits<real>::real4;
 
 constexpr int kRow = kEnergyRow;
+// the fused energy kernel holds more registers per lane: four waves per 64
+// atoms balance latency hiding against occupancy
+constexpr int kEmplarParts = 4;
 
 template <class real>
 struct Mp {
@@ -278,9 +281,9 @@ struct EmplarArgs {
 
 // MODE bit 0: multipole terms, bit 1: polarization terms.
 template <class real, int EWALD, int MODE>
-__global__ __launch_bounds__(64 * kPairParts) void k_emplar(EmplarArgs<real> A) {
+__global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A) {
   constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
-  __shared__ double sh[(kPairParts - 1) * 19 * 64];
+  __shared__ double sh[(kEmplarParts - 1) * 19 * 64];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   Acc<real> acc;
@@ -299,9 +302,9 @@ __global__ __launch_bounds__(64 * kPairParts) void k_emplar(EmplarArgs<real> A) 
     const int len = A.ncut[s];
     const uint32_t* e = A.ecut + A.woff[s >> 6] + (s & 63);
     uint32_t nxt = part < len ? e[(long long)part * 64] : 0;
-    for (int kk = part; kk < len; kk += kPairParts) {
+    for (int kk = part; kk < len; kk += kEmplarParts) {
       const uint32_t ent = nxt;
-      if (kk + kPairParts < len) nxt = e[(long long)(kk + kPairParts) * 64];
+      if (kk + kEmplarParts < len) nxt = e[(long long)(kk + kEmplarParts) * 64];
       if (ent & (kMaxCodes - 1)) continue;  // scaled pairs: k_emplar_scaled
       const int j = (int)(ent >> kCodeBits);
       const PosRec<real> pj = A.pos[j];
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_emplar(EmplarArgs<real> A) 
   }
   double av[19];
   acc.pack(av);
-  parts_reduce<kPairParts, 19>(av, sh);
+  parts_reduce<kEmplarParts, 19>(av, sh);
   double red[EP_N];
   for (int q = 0; q < EP_N; ++q) red[q] = 0;
   if (part == 0) {
@@ -968,7 +971,7 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
   A.rows = energy_rows(c, c.nwave);
   {
     KScope ks_(c, KT_EMPLAR);
-    hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
+    hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kEmplarParts), 0, c.stream, A);
     T9_LAUNCH_CHECK();
   }
   A.rows = energy_rows(c, nb);
@@ -1061,7 +1064,7 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
     T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
   }
   if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c, want_v);
-  if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>());
+  if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>(), true);
   if (c.erows == 0) {
     for (int q = 0; q < 16; ++q) out16[q] = 0;
     return;
@@ -1120,7 +1123,9 @@ void energy_setup(Ctx& c, const System& s) {
   const int nb = ceil_div(n, 256);
   long long lines = 0;
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
-  c.ecap = (int)(9LL * nb + 2LL * ceil_div(n, 64) + lines + 16);  // list loops: one row per 64 atoms
+  const long long nterm = (long long)(s.bond_kr.size() + s.angle_kt.size() + s.urey_kr.size()) / 2;
+  // list loops: one row per 64 atoms; valence: one row per 256 terms
+  c.ecap = (int)(9LL * nb + 2LL * ceil_div(n, 64) + (nterm + 255) / 256 + lines + 16);
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }

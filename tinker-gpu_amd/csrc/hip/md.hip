@@ -260,36 +260,59 @@ __device__ void valence_term(const ValArgs& A, int kind, int idx, double* e, dou
   }
 }
 
-__global__ __launch_bounds__(256) void k_valence(ValArgs A) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// One thread per term (bonds, then angles, then Urey-Bradley): the term's
+// gradient on each of its atoms goes to tg[term][slot][3]; with ENE the
+// term's energy and virial go to the block's partial row.
+template <bool ENE>
+__global__ __launch_bounds__(256) void k_valence_terms(ValArgs A, int nb, int na, int nu, double* __restrict__ tg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double red[EP_N];
-  for (int q = 0; q < EP_N; ++q) red[q] = 0;
-  if (j < A.n) {
-    double gj[3] = {0, 0, 0};
-    for (int q = A.toff[j]; q < A.toff[j + 1]; ++q) {
-      const int t = A.term[q];
-      const int kind = t >> 28, idx = (t >> 2) & 0x3ffffff, role = t & 3;
-      double e, g[3][3], rel[3][3];
-      int at[3], nat;
-      valence_term(A, kind, idx, &e, g, rel, at, &nat);
-      for (int d = 0; d < 3; ++d) gj[d] += g[role][d];
-      if (role == 0) {
-        red[kind == 0 ? EP_EBOND : (kind == 1 ? EP_EANGLE : EP_EUREY)] += e;
-        double W[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int m = 0; m < nat; ++m)
-          for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) W[a][b] += g[m][a] * rel[m][b];
-        red[EP_VXX] += W[0][0];
-        red[EP_VYY] += W[1][1];
-        red[EP_VZZ] += W[2][2];
-        red[EP_VXY] += 0.5 * (W[0][1] + W[1][0]);
-        red[EP_VXZ] += 0.5 * (W[0][2] + W[2][0]);
-        red[EP_VYZ] += 0.5 * (W[1][2] + W[2][1]);
-      }
+  if (ENE)
+    for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  if (t < nb + na + nu) {
+    const int kind = t < nb ? 0 : (t < nb + na ? 1 : 2);
+    const int idx = kind == 0 ? t : (kind == 1 ? t - nb : t - nb - na);
+    double e, g[3][3], rel[3][3];
+    int at[3], nat;
+    valence_term(A, kind, idx, &e, g, rel, at, &nat);
+    for (int m = 0; m < nat; ++m)
+      for (int d = 0; d < 3; ++d) tg[9 * (size_t)t + 3 * m + d] = g[m][d];
+    if (ENE) {
+      red[kind == 0 ? EP_EBOND : (kind == 1 ? EP_EANGLE : EP_EUREY)] = e;
+      double W[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int m = 0; m < nat; ++m)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) W[a][b] += g[m][a] * rel[m][b];
+      red[EP_VXX] = W[0][0];
+      red[EP_VYY] = W[1][1];
+      red[EP_VZZ] = W[2][2];
+      red[EP_VXY] = 0.5 * (W[0][1] + W[1][0]);
+      red[EP_VXZ] = 0.5 * (W[0][2] + W[2][0]);
+      red[EP_VYZ] = 0.5 * (W[1][2] + W[2][1]);
     }
-    for (int d = 0; d < 3; ++d) A.grad[3 * (size_t)j + d] = gj[d];
   }
-  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
+  if (ENE) block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
+}
+
+// Each atom sums the gradients its terms left for it, in its term order.
+__global__ __launch_bounds__(256) void k_valence_gather(int n, const int* __restrict__ toff,
+                                                        const int* __restrict__ term, int nb, int na,
+                                                        const double* __restrict__ tg, double* __restrict__ grad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double g0 = 0, g1 = 0, g2 = 0;
+  for (int q = toff[j]; q < toff[j + 1]; ++q) {
+    const int tt = term[q];
+    const int kind = tt >> 28, idx = (tt >> 2) & 0x3ffffff, role = tt & 3;
+    const int t = kind == 0 ? idx : (kind == 1 ? nb + idx : nb + na + idx);
+    const double* p = tg + 9 * (size_t)t + 3 * role;
+    g0 += p[0];
+    g1 += p[1];
+    g2 += p[2];
+  }
+  grad[3 * (size_t)j] = g0;
+  grad[3 * (size_t)j + 1] = g1;
+  grad[3 * (size_t)j + 2] = g2;
 }
 
 // ------------------------------------------------------------ integrator
@@ -407,7 +430,7 @@ void vdw_run(Ctx& c, bool vir) {
   else vdw_run_t<double>(c, vir);
 }
 
-void valence_run(Ctx& c, double* gval) {
+void valence_run(Ctx& c, double* gval, bool energy) {
   if (!c.use_valence) return;
   ValArgs A;
   A.n = c.n;
@@ -429,10 +452,22 @@ void valence_run(Ctx& c, double* gval) {
   A.pa = c.vpoly[4];
   A.sa = c.vpoly[5];
   A.grad = gval;
-  const int nb = ceil_div(c.n, 256);
-  A.rows = energy_rows(c, nb);
+  const int nt = c.nbond + c.nangle + c.nurey;
+  const int nb = ceil_div(std::max(nt, 1), 256);
+  c.vterm_grad.alloc(sizeof(double) * 9 * (size_t)std::max(nt, 1));
   KScope ks_(c, KT_VALENCE);
-  hipLaunchKernelGGL(k_valence, dim3(nb), dim3(256), 0, c.stream, A);
+  if (energy) {
+    A.rows = energy_rows(c, nb);
+    hipLaunchKernelGGL(k_valence_terms<true>, dim3(nb), dim3(256), 0, c.stream, A, c.nbond, c.nangle, c.nurey,
+                       c.vterm_grad.as<double>());
+  } else {
+    A.rows = nullptr;
+    hipLaunchKernelGGL(k_valence_terms<false>, dim3(nb), dim3(256), 0, c.stream, A, c.nbond, c.nangle, c.nurey,
+                       c.vterm_grad.as<double>());
+  }
+  T9_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_valence_gather, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                     c.vterm_off.as<int>(), c.vterm.as<int>(), c.nbond, c.nangle, c.vterm_grad.as<double>(), gval);
   T9_LAUNCH_CHECK();
 }
 
@@ -528,6 +563,9 @@ void md_setup(Ctx& c, const System& s) {
     c.vpoly[5] = s.ff.angle_sextic;
     std::vector<std::vector<int>> tl(n);
     const int nb = (int)s.bond_kr.size() / 2, na = (int)s.angle_kt.size() / 2, nu = (int)s.urey_kr.size() / 2;
+    c.nbond = nb;
+    c.nangle = na;
+    c.nurey = nu;
     for (int t = 0; t < nb; ++t)
       for (int r = 0; r < 2; ++r) tl[s.bond_ij[2 * t + r]].push_back((0 << 28) | (t << 2) | r);
     for (int t = 0; t < na; ++t)
