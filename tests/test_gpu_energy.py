@@ -116,6 +116,23 @@ def test_water_ions_energy(prec):
     _check(prec, eg, gg, vg, eo, go, vo, "water1000+ions")
 
 
+def test_water96k_fp32_vs_fp64():
+    """The 96,000-atom box (PME 120^3: whole-plane FP32 FFT kernels, cell
+    lists, 8-wave pair kernels) at the same coordinates in FP32 and FP64: the
+    FP64 form is pinned to the oracle above, so agreement to the FP32 bounds
+    checks the large-system FP32 path end to end."""
+    print("case energy water96k fp32 vs fp64", flush=True)
+    key = WATER_KEY.format(K=120, buf=2.0).replace("polar-eps 1e-6", "polar-eps 1e-8")
+    s = t9hip.System.water_box(32000, 32, 98.65, 2024, 0, 0, key=key)
+    e64, g64, v64 = t9hip.Context(s, t9hip.T9_FP64).energy()
+    e32, g32, v32 = t9hip.Context(s, t9hip.T9_FP32).energy()
+    rel = abs(e32["esum"] - e64["esum"]) / abs(e64["esum"])
+    grms = np.sqrt(np.mean((g32 - g64) ** 2))
+    vrel = np.abs(v32 - v64).max() / np.abs(v64).max()
+    print(f"water96k E64={e64['esum']:.6f} relE={rel:.2e} gradRMS={grms:.2e} vir={vrel:.2e}", flush=True)
+    assert rel < 1e-6 and grms < 1e-5 and vrel < 1e-4
+
+
 def test_energy_bitwise_reproducible():
     s = water(216, 6, 18.643, 24, buf=1.5)
     ctx = t9hip.Context(s, t9hip.T9_FP32)

@@ -193,9 +193,11 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   const double buf = s.opt.list_buffer;
   c.mlist.cutoff = c.cut < 1e20 ? c.cut + buf : 1e30;
   const bool sparse = s.opt.pcg_prec && s.opt.usolve_cut > 0;
-  c.ulist.cutoff = sparse ? std::min(s.opt.usolve_cut + buf, c.mlist.cutoff) : 0;
+  // the preconditioner's pairs (usolve cutoff) are a second compaction of the
+  // multipole list, which covers them at any time between rebuilds
+  c.ulist.cutoff = 0;
   c.mlist.cut = c.cut < 1e20 ? c.cut : 1e30;
-  c.ulist.cut = s.opt.usolve_cut;
+  c.mlist.cut2nd = sparse ? std::min(s.opt.usolve_cut, c.mlist.cut) : 0;
   // vdW sites sit off their atoms by (1 - red) times the bond to the parent;
   // the cell search around the atom's cell must cover twice that offset
   double site_slack = 0;
@@ -694,23 +696,19 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
         }
       }
       t9::md_drift(c, dta);
-      iters += t9::evaluate(c, T9_CALC_ENERGY, t9::TERM_ALL, e, nullptr);
+      // energies come back to the host only for the last step's statistics
+      iters += t9::evaluate(c, step == nsteps - 1 ? T9_CALC_ENERGY : 0, t9::TERM_ALL, e, nullptr);
       t9::md_sorted_to_orig(c, c.egrad.as<double>(), gs);
       t9::md_kick(c, 0.5 * dta, gf, 0.5 * dt, gs);  // velR2
       if (bath_kelvin > 0) {
-        // Bussi stochastic velocity rescaling over the full step
-        const double ek = t9::md_kinetic(c);
-        double temp = 2.0 * ek / (c.nfree * t9::kGasConst);
-        if (temp == 0) temp = 0.1;
-        const double cc = std::exp(-dt / tau);
-        const double d = (1 - cc) * (bath_kelvin / temp) / c.nfree;
+        // Bussi stochastic velocity rescaling over the full step; the two
+        // random numbers come from the host stream, the kinetic energy and
+        // the factor stay on the device
         std::normal_distribution<double> nd(0.0, 1.0);
         const double r = nd(c.md_rng);
         std::gamma_distribution<double> gd(0.5 * (c.nfree - 1), 2.0);
         const double s2 = c.nfree > 1 ? gd(c.md_rng) : 0.0;
-        double sc = std::sqrt(cc + (s2 + r * r) * d + 2 * r * std::sqrt(cc * d));
-        if (r + std::sqrt(cc / d) < 0) sc = -sc;
-        t9::md_scale_v(c, sc);
+        t9::md_bussi(c, dt, tau, bath_kelvin, r, s2);
       }
     }
     if (stats) {

@@ -67,6 +67,10 @@ struct NbList {
   DBuf ncut;           // int [n]
   DBuf ecut;           // uint32 [capacity]
   bool cut_valid = false;
+  // optional second compaction at a shorter radius (same slab layout): the
+  // preconditioner's pairs, taken from the multipole list
+  double cut2nd = 0;
+  DBuf ncut2, ecut2;
 };
 
 // Live kernel timing with HIP events recorded on the context stream around
@@ -135,7 +139,7 @@ struct Ctx {
   int morton_bits = 0;
   bool brute = true;             // all-pairs search (small or non-periodic systems)
   DBuf cell_key, cell_count, cell_start, cell_cursor, unsorted;
-  NbList mlist, ulist, vlist;    // multipole/polar, preconditioner, vdW sites
+  NbList mlist, ulist, vlist;    // multipole/polar (+ preconditioner compaction), unused, vdW sites
   double cell_reach = 0;         // radius the cell search must cover (A)
   int builds = 0;
   bool list_valid = false;
@@ -306,6 +310,9 @@ double md_kinetic(Ctx& c);                // kcal/mol
 void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
 void md_scale_v(Ctx& c, double a);
+// Bussi rescaling with host-drawn deviates r (normal) and s2 (chi-square,
+// nfree - 1), kinetic energy and factor on the device
+void md_bussi(Ctx& c, double dt, double tau, double kelvin, double r, double s2);
 void md_sorted_to_orig(Ctx& c, const double* sorted, double* orig);
 
 // sorted <-> caller order transfers of [n][3] real vectors

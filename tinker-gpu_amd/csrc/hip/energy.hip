@@ -1065,7 +1065,8 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   }
   if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c, want_v);
   if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>(), true);
-  if (c.erows == 0) {
+  // without CALC_ENERGY / CALC_VIRIAL nothing comes back to the host (no sync)
+  if (c.erows == 0 || !(vers & 5)) {
     for (int q = 0; q < 16; ++q) out16[q] = 0;
     return;
   }

@@ -371,13 +371,13 @@ void dfield_real_t(Ctx& c) {
 
 template <class real>
 void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
-  const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.ulist.cutoff > 0;
-  const auto& L = sparse ? c.ulist : c.mlist;
+  const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.mlist.cut2nd > 0;
+  const auto& L = c.mlist;
   if (sparse) require_cut(L);
   auto kern = c.uniform_thole ? k_precond<real, 1> : k_precond<real, 0>;
   KScope ks_(c, KT_PRECOND);
   hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, sparse ? 1 : 0,
-                     (real)c.opt.udiag, L.ncut.as<int>(), L.off.as<long long>(), L.ecut.as<uint32_t>(),
+                     (real)c.opt.udiag, L.ncut2.as<int>(), L.off.as<long long>(), L.ecut2.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
                      (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
                      (real*)zp);

@@ -353,11 +353,41 @@ __global__ __launch_bounds__(256) void k_ekin(int n, const double* __restrict__ 
   block_row<1>(red, rows + blockIdx.x);
 }
 
-__global__ void k_sum_rows1(int nrows, const double* __restrict__ rows, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = 0;
-  for (int r = 0; r < nrows; ++r) s += rows[r];
-  out[0] = s;
+// out[0] = sum of rows (fixed order: strided partial sums, fixed tree)
+__global__ __launch_bounds__(256) void k_sum_rows1(int nrows, const double* __restrict__ rows,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[256];
+  double v = 0;
+  for (int r = threadIdx.x; r < nrows; r += 256) v += rows[r];
+  const double t = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+// Bussi factor from the kinetic energy rows (src/mdpt.cpp:41-71): the
+// host supplies the normal deviate r and the chi-square deviate s2 of
+// nfree - 1 degrees of freedom; the factor goes to out[0].
+__global__ __launch_bounds__(256) void k_bussi_factor(int nrows, const double* __restrict__ rows, int nfree,
+                                                      double cc, double kelvin, double r, double s2,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[256];
+  double v = 0;
+  for (int q = threadIdx.x; q < nrows; q += 256) v += rows[q];
+  const double k2 = block_sum<256>(v, sh);  // 2 K in g/mol A^2/ps^2
+  if (threadIdx.x != 0) return;
+  const double ek = 0.5 * k2 / kEkcal;
+  double temp = 2.0 * ek / (nfree * kGasConst);
+  if (temp == 0) temp = 0.1;
+  const double d = (1 - cc) * (kelvin / temp) / nfree;
+  double sc = sqrt(cc + (s2 + r * r) * d + 2 * r * sqrt(cc * d));
+  if (r + sqrt(cc / d) < 0) sc = -sc;
+  out[0] = sc;
+}
+
+__global__ void k_scale_v_dev(int n, const double* __restrict__ a, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double f = a[0];
+  for (int d = 0; d < 3; ++d) v[3 * i + d] *= f;
 }
 
 __global__ void k_sorted_grad_to_orig(int n, const int* __restrict__ perm, const double* __restrict__ src,
@@ -620,7 +650,7 @@ double md_kinetic(Ctx& c) {  // kcal/mol
   const int nb = ceil_div(c.n, 256);
   double* rows = c.partials.as<double>();
   hipLaunchKernelGGL(k_ekin, dim3(nb), dim3(256), 0, c.stream, c.n, c.mass.as<double>(), c.vel.as<double>(), rows);
-  hipLaunchKernelGGL(k_sum_rows1, dim3(1), dim3(64), 0, c.stream, nb, rows, c.scalars.as<double>() + 40);
+  hipLaunchKernelGGL(k_sum_rows1, dim3(1), dim3(256), 0, c.stream, nb, rows, c.scalars.as<double>() + 40);
   T9_LAUNCH_CHECK();
   T9_HIP(hipMemcpyAsync(&c.h_scalars[40], c.scalars.as<double>() + 40, sizeof(double), hipMemcpyDeviceToHost,
                         c.stream));
@@ -641,6 +671,17 @@ void md_drift(Ctx& c, double h) {
   c.coords_dirty = true;
   c.rpole_valid = false;
   c.bins_valid = false;
+}
+
+void md_bussi(Ctx& c, double dt, double tau, double kelvin, double r, double s2) {
+  const int nb = ceil_div(c.n, 256);
+  double* rows = c.partials.as<double>();
+  double* f = c.scalars.as<double>() + 41;
+  hipLaunchKernelGGL(k_ekin, dim3(nb), dim3(256), 0, c.stream, c.n, c.mass.as<double>(), c.vel.as<double>(), rows);
+  hipLaunchKernelGGL(k_bussi_factor, dim3(1), dim3(256), 0, c.stream, nb, rows, c.nfree, std::exp(-dt / tau), kelvin,
+                     r, s2, f);
+  hipLaunchKernelGGL(k_scale_v_dev, dim3(nb), dim3(256), 0, c.stream, c.n, f, c.vel.as<double>());
+  T9_LAUNCH_CHECK();
 }
 
 void md_scale_v(Ctx& c, double a) {

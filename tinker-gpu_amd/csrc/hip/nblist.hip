@@ -647,44 +647,62 @@ void nblist_build_t(Ctx& c) {
 }
 
 // Keep the entries within the interaction cutoff (same FP arithmetic as the
-// pair kernels, so those kernels need no distance test of their own).
+// pair kernels, so those kernels need no distance test of their own), in
+// list order (which keeps the pair kernels' gathers spatially local); with
+// cut2b > 0 also the entries within the shorter second radius into
+// ecut2 / ncut2, same layout.
 template <class real>
-__global__ void k_compact(int n, const int* __restrict__ nlist, const long long* __restrict__ woff,
-                          const uint32_t* __restrict__ entries, const PosRec<real>* __restrict__ pos,
-                          BoxT<real> box, real cut2, int* __restrict__ ncut, uint32_t* __restrict__ ecut) {
+__global__ __launch_bounds__(256) void k_compact(int n, const int* __restrict__ nlist,
+                                                 const long long* __restrict__ woff,
+                                                 const uint32_t* __restrict__ entries,
+                                                 const PosRec<real>* __restrict__ pos, BoxT<real> box, real cut2,
+                                                 real cut2b, int* __restrict__ ncut, uint32_t* __restrict__ ecut,
+                                                 int* __restrict__ ncut2, uint32_t* __restrict__ ecut2) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const bool two = ecut2 != nullptr;
   const long long base = woff[s >> 6] + (s & 63);
   const uint32_t* e = entries + base;
   uint32_t* o = ecut + base;
+  uint32_t* ob = two ? ecut2 + base : nullptr;
   const auto pi = pos[s];
-  int cnt = 0;
+  int cnt = 0, cntb = 0;
   const int len = nlist[s];
+  uint32_t nxt = len > 0 ? e[0] : 0;
   for (int k = 0; k < len; ++k) {
-    const uint32_t ent = e[(long long)k * 64];
+    const uint32_t ent = nxt;
+    if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
     const auto pj = pos[ent >> kCodeBits];
     real xr, yr, zr;
     pair_disp(pi, pj, box, xr, yr, zr);
-    if (xr * xr + yr * yr + zr * zr <= cut2) o[(long long)(cnt++) * 64] = ent;
+    const real r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 <= cut2) o[(long long)(cnt++) * 64] = ent;
+    if (two && r2 <= cut2b) ob[(long long)(cntb++) * 64] = ent;
   }
   ncut[s] = cnt;
+  if (two) ncut2[s] = cntb;
 }
 
 template <class real>
 void compact_t(Ctx& c) {
   BoxT<real> box = make_boxT<real>(c);
-  for (NbList* L : {&c.mlist, &c.ulist}) {
-    if (L->cutoff <= 0 || !L->entries.p) continue;
-    L->ncut.alloc(sizeof(int) * c.n);
-    L->ecut.alloc(sizeof(uint32_t) * L->capacity);
-    const double c2 = L->cut * L->cut;
-    hipLaunchKernelGGL(k_compact<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L->nlist.as<int>(),
-                       L->off.as<long long>(), L->entries.as<uint32_t>(),
-                       c.pos.as<PosRec<real>>(), box, (real)(c2 > 1e30 ? 1e30 : c2),
-                       L->ncut.as<int>(), L->ecut.as<uint32_t>());
-    T9_LAUNCH_CHECK();
-    L->cut_valid = true;
+  NbList* L = &c.mlist;
+  if (L->cutoff <= 0 || !L->entries.p) return;
+  L->ncut.alloc(sizeof(int) * c.n);
+  L->ecut.alloc(sizeof(uint32_t) * L->capacity);
+  const bool two = L->cut2nd > 0;
+  if (two) {
+    L->ncut2.alloc(sizeof(int) * c.n);
+    L->ecut2.alloc(sizeof(uint32_t) * L->capacity);
   }
+  const double c2 = L->cut * L->cut, c2b = L->cut2nd * L->cut2nd;
+  hipLaunchKernelGGL(k_compact<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L->nlist.as<int>(),
+                     L->off.as<long long>(), L->entries.as<uint32_t>(), c.pos.as<PosRec<real>>(), box,
+                     (real)(c2 > 1e30 ? 1e30 : c2), (real)(two ? c2b : -1.0), L->ncut.as<int>(),
+                     L->ecut.as<uint32_t>(), two ? L->ncut2.as<int>() : nullptr,
+                     two ? L->ecut2.as<uint32_t>() : nullptr);
+  T9_LAUNCH_CHECK();
+  L->cut_valid = true;
 }
 
 template <class real>

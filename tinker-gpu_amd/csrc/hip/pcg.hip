@@ -35,14 +35,15 @@ struct Coef16 {
   double c[kMaxUalt];
 };
 
-// u0 = sum_k c_k hist_k (caller-order history, slot k: [2][n][3])
+// u0 = sum_k c_k hist_k (caller-order history, slot k: [2][n][3]); lanes in
+// caller order so the 16 history rows are read contiguously
 template <class real>
-__global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict__ perm, Coef16 cf,
+__global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict__ iperm, Coef16 cf,
                                                   const real* __restrict__ hist, real* __restrict__ ud,
                                                   real* __restrict__ up) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int i = perm[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = iperm[i];
   const size_t slot = 6 * (size_t)n;
   for (int d = 0; d < 3; ++d) {
     double a = 0, b = 0;
@@ -290,7 +291,7 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     const auto B = aspc_coefficients();
     Coef16 cf;
     for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
-    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.perm.as<int>(), cf, c.ualt.as<real>(), ud, up);
+    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up);
     T9_LAUNCH_CHECK();
     if (feats) {
       T9_HIP(hipMemcpyAsync(ua, ud, vbytes, hipMemcpyDeviceToDevice, st));
