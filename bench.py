@@ -77,7 +77,7 @@ def equilibrate(t9hip, system, steps):
     return x, v, st
 
 
-def run_gpu(t9hip, system, prec, steps, warmup, guess=None, start=None):
+def run_gpu(t9hip, system, prec, steps, warmup, guess=None, start=None, timers=True):
     global FAMILIES
     FAMILIES = (("ufield_real", t9hip.K_UFIELD), ("dfield_real", t9hip.K_DFIELD), ("emplar_real", t9hip.K_EMPLAR),
                 ("fft_pass", t9hip.K_FFT), ("pme_spread", t9hip.K_SPREAD), ("pme_gather", t9hip.K_GATHER),
@@ -96,15 +96,16 @@ def run_gpu(t9hip, system, prec, steps, warmup, guess=None, start=None):
     # warmup: fills the 16-deep ASPC history, so the timed steps use ASPC
     wst = ctx.md_run(warmup, DT, NRESPA, KELVIN, TAU)
     ctx.sync()
-    for _, kid in FAMILIES:
-        ctx.kernel_timing(kid, 1)
+    if timers:
+        for _, kid in FAMILIES:
+            ctx.kernel_timing(kid, 1)
     ctx.sync()
     t0 = time.perf_counter()
     st = ctx.md_run(steps, DT, NRESPA, KELVIN, TAU)
     ctx.sync()
     t1 = time.perf_counter()
     kt = {}
-    for name, kid in FAMILIES:
+    for name, kid in (FAMILIES if timers else ()):
         ms, cnt = ctx.kernel_timing(kid, 0)
         kt[name] = {"total_ms_per_step": round(ms / steps, 4), "launches_per_step": round(cnt / steps, 2),
                     "avg_us": round(1e3 * ms / max(cnt, 1), 3)}

@@ -65,6 +65,8 @@ void alloc_vectors(Ctx& c) {
     T9_HIP(hipMemsetAsync(b->p, 0, vb, c.stream));
   }
   c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
+  c.dot_counter.alloc(sizeof(unsigned));
+  T9_HIP(hipMemsetAsync(c.dot_counter.p, 0, sizeof(unsigned), c.stream));
   c.scalars.alloc(sizeof(double) * 48);
   T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 48, c.stream));
   T9_HIP(hipHostMalloc((void**)&c.h_scalars, sizeof(double) * 48, hipHostMallocDefault));
@@ -688,14 +690,16 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
     for (int step = 0; step < nsteps; ++step) {
       // velR1: half kick of the slow forces and the first inner half kick
       t9::md_kick(c, 0.5 * dta, gf, 0.5 * dt, gs);
+      // inner steps: drift, then per step the valence gradient, the inner
+      // kick (velR0) and the next drift in one pass over the atoms
+      t9::md_drift(c, dta);
       for (int k = 1; k < nrespa; ++k) {
-        t9::md_drift(c, dta);
         if (c.use_valence) {
-          t9::valence_run(c, gf, false);
-          t9::md_kick(c, dta, gf, 0.0, nullptr);  // velR0
+          t9::valence_inner(c, gf, dta);
+        } else {
+          t9::md_drift(c, dta);
         }
       }
-      t9::md_drift(c, dta);
       // energies come back to the host only for the last step's statistics
       iters += t9::evaluate(c, step == nsteps - 1 ? T9_CALC_ENERGY : 0, t9::TERM_ALL, e, nullptr);
       t9::md_sorted_to_orig(c, c.egrad.as<double>(), gs);

@@ -147,6 +147,7 @@ struct Ctx {
   // ---- CG / field vectors, real [n][3] each, sorted order
   DBuf field, fieldp, udir, udirp, uind, uinp, rsd, rsdp, zrsd, zrsdp, conj, conjp, vec, vecp;
   DBuf partials;                 // double [nblocks * 8]
+  DBuf dot_counter;              // unsigned: workgroups done in the current dot kernel (0 between launches)
   DBuf scalars;                  // double [32] on device
   DBuf hscalars_pinned;          // host-pinned mirror
   double* h_scalars = nullptr;
@@ -309,6 +310,7 @@ double md_kinetic(Ctx& c);                // kcal/mol
 // v -= (h1 g1 + h2 g2) / m, g2 may be null
 void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
+void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h
 void md_scale_v(Ctx& c, double a);
 // Bussi rescaling with host-drawn deviates r (normal) and s2 (chi-square,
 // nfree - 1), kinetic energy and factor on the device

@@ -315,6 +315,34 @@ __global__ __launch_bounds__(256) void k_valence_gather(int n, const int* __rest
   grad[3 * (size_t)j + 2] = g2;
 }
 
+// Inner RESPA step after the term kernel: gather the valence gradient of
+// each atom (as k_valence_gather), kick v -= h g / m, then drift x += h v.
+__global__ __launch_bounds__(256) void k_valence_gather_kick_drift(int n, const int* __restrict__ toff,
+                                                                   const int* __restrict__ term, int nb, int na,
+                                                                   const double* __restrict__ tg,
+                                                                   double* __restrict__ grad,
+                                                                   const double* __restrict__ mass, double h,
+                                                                   double* __restrict__ v, double* __restrict__ x) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double g[3] = {0, 0, 0};
+  for (int q = toff[j]; q < toff[j + 1]; ++q) {
+    const int tt = term[q];
+    const int kind = tt >> 28, idx = (tt >> 2) & 0x3ffffff, role = tt & 3;
+    const int t = kind == 0 ? idx : (kind == 1 ? nb + idx : nb + na + idx);
+    const double* p = tg + 9 * (size_t)t + 3 * role;
+    for (int d = 0; d < 3; ++d) g[d] += p[d];
+  }
+  const double m = mass[j];
+  const double c = m > 0 ? h * kEkcal / m : 0.0;
+  for (int d = 0; d < 3; ++d) {
+    grad[3 * (size_t)j + d] = g[d];
+    const double vv = v[3 * (size_t)j + d] - c * g[d];
+    v[3 * (size_t)j + d] = vv;
+    x[3 * (size_t)j + d] += h * vv;
+  }
+}
+
 // ------------------------------------------------------------ integrator
 
 // v -= (h1 g1 + h2 g2) / m (h in ps, g in kcal/mol/Å, v in Å/ps); g2 may be null
@@ -499,6 +527,44 @@ void valence_run(Ctx& c, double* gval, bool energy) {
   hipLaunchKernelGGL(k_valence_gather, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
                      c.vterm_off.as<int>(), c.vterm.as<int>(), c.nbond, c.nangle, c.vterm_grad.as<double>(), gval);
   T9_LAUNCH_CHECK();
+}
+
+// One inner RESPA step of the fast (valence) group on the current
+// coordinates: gradient to gval, kick by h, then drift by h.
+void valence_inner(Ctx& c, double* gval, double h) {
+  ValArgs A;
+  A.n = c.n;
+  A.xyz = c.xyz.as<double>();
+  for (int d = 0; d < 3; ++d) A.L[d] = c.periodic ? c.box[d] : 1.0;
+  A.periodic = c.periodic ? 1 : 0;
+  A.toff = c.vterm_off.as<int>();
+  A.term = c.vterm.as<int>();
+  A.bij = c.bond_ij.as<int>();
+  A.bkr = c.bond_kr.as<double>();
+  A.aabc = c.angle_abc.as<int>();
+  A.akt = c.angle_kt.as<double>();
+  A.uac = c.urey_ac.as<int>();
+  A.ukr = c.urey_kr.as<double>();
+  A.cb = c.vpoly[0];
+  A.qb = c.vpoly[1];
+  A.ca = c.vpoly[2];
+  A.qa = c.vpoly[3];
+  A.pa = c.vpoly[4];
+  A.sa = c.vpoly[5];
+  A.grad = gval;
+  A.rows = nullptr;
+  const int nt = c.nbond + c.nangle + c.nurey;
+  c.vterm_grad.alloc(sizeof(double) * 9 * (size_t)std::max(nt, 1));
+  KScope ks_(c, KT_VALENCE);
+  hipLaunchKernelGGL(k_valence_terms<false>, dim3(ceil_div(std::max(nt, 1), 256)), dim3(256), 0, c.stream, A,
+                     c.nbond, c.nangle, c.nurey, c.vterm_grad.as<double>());
+  hipLaunchKernelGGL(k_valence_gather_kick_drift, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                     c.vterm_off.as<int>(), c.vterm.as<int>(), c.nbond, c.nangle, c.vterm_grad.as<double>(), gval,
+                     c.mass.as<double>(), h, c.vel.as<double>(), c.xyz.as<double>());
+  T9_LAUNCH_CHECK();
+  c.coords_dirty = true;
+  c.rpole_valid = false;
+  c.bins_valid = false;
 }
 
 void md_setup(Ctx& c, const System& s) {

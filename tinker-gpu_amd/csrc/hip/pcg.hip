@@ -115,34 +115,12 @@ __global__ __launch_bounds__(kB) void k_copy3_zero_nopol(int n, const real* __re
   }
 }
 
-// partial sums of two dot products a1.b1 and a2.b2 (slot0, slot1) and optionally
-// a3.b3, a4.b4 (slot2, slot3); partials[blk * 4 + slot]
-template <class real>
-__global__ __launch_bounds__(kB) void k_dots(int n, int ndots, const real* __restrict__ a1,
-                                            const real* __restrict__ b1, const real* __restrict__ a2,
-                                            const real* __restrict__ b2, const real* __restrict__ a3,
-                                            const real* __restrict__ b3, const real* __restrict__ a4,
-                                            const real* __restrict__ b4, double* __restrict__ partials) {
-  __shared__ double sh[kB];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const real* A[4] = {a1, a2, a3, a4};
-  const real* Bv[4] = {b1, b2, b3, b4};
-  for (int q = 0; q < ndots; ++q) {
-    double v = 0;
-    if (s < n)
-      for (int d = 0; d < 3; ++d) v += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
-    double t = block_sum<kB>(v, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x * 4 + q] = t;
-  }
-}
-
-// Finalize: sum partials in fixed order, then update the CG scalars.
-// stage 0: initial sum = r.z          (slots SUM, SUMP)
-// stage 1: a = sum / p.Tp             (slots A, AP)
-// stage 2: b = sum1 / sum; sum = sum1; eps from r.r
-__global__ __launch_bounds__(kB) void k_finalize(int nblk, int ndots, int stage, int n, double debye,
-                                                const double* __restrict__ partials, double* __restrict__ sc) {
-  __shared__ double sh[kB];
+// Finalize: sum the per-block partials in fixed order, then update the CG
+// scalars.  stage 0: initial sum = r.z (slots SUM, SUMP); 1: a = sum / p.Tp
+// (slots A, AP); 2: b = sum1 / sum, sum = sum1, eps from r.r; 3: eps of
+// the initial residual.  Run by the last workgroup of the dot kernel.
+__device__ void finalize_block(int nblk, int ndots, int stage, int n, double debye,
+                               const double* __restrict__ partials, double* __restrict__ sc, double* sh) {
   double tot[4] = {0, 0, 0, 0};
   for (int q = 0; q < ndots; ++q) {
     double v = 0;
@@ -167,6 +145,80 @@ __global__ __launch_bounds__(kB) void k_finalize(int nblk, int ndots, int stage,
     sc[S_EPSP] = tot[3];
     sc[S_EPS] = debye * sqrt(fmax(tot[2], tot[3]) / n);
   }
+}
+
+struct DotOut {
+  double* partials;   // [nblk][4]
+  unsigned* counter;  // workgroups finished (reset by the last one)
+  double* sc;         // CG scalars
+  int stage, n;
+  double debye;
+};
+
+// Every workgroup writes its partial sums; the last one to finish (counted
+// with a device-scope atomic after a release fence) sums all partials in
+// workgroup order and updates the scalars, so the result does not depend on
+// which workgroup finishes last.
+__device__ __forceinline__ void dots_epilogue(const double* tot, int ndots, const DotOut& o, double* sh) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < ndots; ++q) o.partials[blockIdx.x * 4 + q] = tot[q];
+    __threadfence();
+    last = atomicAdd(o.counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  finalize_block(gridDim.x, ndots, o.stage, o.n, o.debye, o.partials, o.sc, sh);
+  if (threadIdx.x == 0) atomicExch(o.counter, 0u);
+}
+
+// two dot products a1.b1 and a2.b2 and optionally a3.b3, a4.b4, finalized
+template <class real>
+__global__ __launch_bounds__(kB) void k_dots(int n, int ndots, const real* __restrict__ a1,
+                                            const real* __restrict__ b1, const real* __restrict__ a2,
+                                            const real* __restrict__ b2, const real* __restrict__ a3,
+                                            const real* __restrict__ b3, const real* __restrict__ a4,
+                                            const real* __restrict__ b4, DotOut o) {
+  __shared__ double sh[kB];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const real* A[4] = {a1, a2, a3, a4};
+  const real* Bv[4] = {b1, b2, b3, b4};
+  double tot[4] = {0, 0, 0, 0};
+  for (int q = 0; q < ndots; ++q) {
+    double v = 0;
+    if (s < n)
+      for (int d = 0; d < 3; ++d) v += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
+    tot[q] = block_sum<kB>(v, sh);
+  }
+  dots_epilogue(tot, ndots, o, sh);
+}
+
+// vec = conj / alpha - field (reference pcgP1) with the dots conj.vec,
+// finalized into a = sum / p.Tp
+template <class real>
+__global__ __launch_bounds__(kB) void k_p1_dots(int n, const real* __restrict__ polinv, const real* __restrict__ cd,
+                                               const real* __restrict__ cp, const real* __restrict__ fd,
+                                               const real* __restrict__ fp, real* __restrict__ vd,
+                                               real* __restrict__ vp, DotOut o) {
+  __shared__ double sh[kB];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double x = 0, y = 0;
+  if (s < n) {
+    const real pi = polinv[s];
+    for (int d = 0; d < 3; ++d) {
+      const real a = pi * cd[3 * s + d] - fd[3 * s + d];
+      const real b = pi * cp[3 * s + d] - fp[3 * s + d];
+      vd[3 * s + d] = a;
+      vp[3 * s + d] = b;
+      x += (double)cd[3 * s + d] * (double)a;
+      y += (double)cp[3 * s + d] * (double)b;
+    }
+  }
+  double tot[4] = {0, 0, 0, 0};
+  tot[0] = block_sum<kB>(x, sh);
+  tot[1] = block_sum<kB>(y, sh);
+  dots_epilogue(tot, 2, o, sh);
 }
 
 // vec = conj / alpha - field   (reference pcgP1)
@@ -255,6 +307,16 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   double* sc = c.scalars.as<double>();
   const double debye = kDebye;
   const bool guess = c.opt.pcg_guess;
+  auto dout = [&](int stage) {
+    DotOut o;
+    o.partials = part;
+    o.counter = c.dot_counter.as<unsigned>();
+    o.sc = sc;
+    o.stage = stage;
+    o.n = n;
+    o.debye = debye;
+    return o;
+  };
 
   dfield_all(c);
   // ASPC once 16 solutions are stored (src/cu/amoeba/pcg.cu:30-33)
@@ -277,8 +339,7 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   // RMS of r0 in Debye, read back now
   auto residual_rms = [&]() {
     hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(3));
     T9_LAUNCH_CHECK();
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     T9_HIP(hipStreamSynchronize(st));
@@ -331,15 +392,13 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   // iteration), first preconditioned search direction
   auto start_cg = [&]() {
     hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 3, n, debye, part, sc);
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(3));
     T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp);
     T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
     T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, zd, rp, zp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 0, n, debye, part, sc);
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(0));
     T9_LAUNCH_CHECK();
   };
   start_cg();
@@ -352,15 +411,12 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   while (!done) {
     ++iter;
     ufield_all(c, cd, cp, field, fieldp);
-    hipLaunchKernelGGL(k_p1<real>, G, Bk, 0, st, n, c.polinv.as<real>(), cd, cp, field, fieldp, vd, vp);
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, cd, vd, cp, vp, (const real*)nullptr, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 2, 1, n, debye, part, sc);
+    hipLaunchKernelGGL(k_p1_dots<real>, G, Bk, 0, st, n, c.polinv.as<real>(), cd, cp, field, fieldp, vd, vp,
+                       dout(1));
     hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up, rd, rp);
     T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp);
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), Bk, 0, st, nblk, 4, 2, n, debye, part, sc);
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, dout(2));
     hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, sc, zd, zp, cd, cp);
     T9_LAUNCH_CHECK();
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
