@@ -337,7 +337,7 @@ void ktimer_collect(Ctx& c, int id) {
     float ms = 0;
     T9_HIP(hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]));
     t.ms += ms;
-    t.count += 1;
+    t.count += t.nl[k / 2];
   }
   t.used = 0;
 }

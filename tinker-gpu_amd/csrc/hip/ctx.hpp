@@ -92,6 +92,7 @@ enum KernelId : int {
 struct KTimer {
   bool on = false;
   std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  std::vector<int> nl;         // launches inside each pair
   size_t used = 0;             // events in use
   double ms = 0;
   long long count = 0;
@@ -338,20 +339,25 @@ BoxT<real> make_boxT(const Ctx& c) {
   return b;
 }
 
-// Records start/stop events around a launch when the timer is on.
+// Records start/stop events around `launches` back-to-back launches of one
+// kernel family when its timer is on.  The events skip the system-scope
+// cache release (device-scope timing only), which would otherwise write L2
+// back at every record.
 struct KScope {
   KTimer* t;
   hipStream_t st;
   hipEvent_t stop = nullptr;
-  KScope(Ctx& c, int id) : t(c.timers[id].on ? &c.timers[id] : nullptr), st(c.stream) {
+  KScope(Ctx& c, int id, int launches = 1) : t(c.timers[id].on ? &c.timers[id] : nullptr), st(c.stream) {
     if (!t) return;
     if (t->used + 2 > t->ev.size()) {
       hipEvent_t a, b;
-      T9_HIP(hipEventCreate(&a));
-      T9_HIP(hipEventCreate(&b));
+      T9_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+      T9_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
       t->ev.push_back(a);
       t->ev.push_back(b);
+      t->nl.push_back(0);
     }
+    t->nl[t->used / 2] = launches;
     T9_HIP(hipEventRecord(t->ev[t->used], st));
     stop = t->ev[t->used + 1];
     t->used += 2;

@@ -219,7 +219,6 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
     const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
     const size_t sh = 2 * (size_t)B * (p[axis].n + 1) * sizeof(Cplx<real>);
-    KScope ks_(c, KT_FFT);
     void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs);
     switch (p[axis].n) {
       case 24: kern = k_fft_pass<real, B, 24>; break;
@@ -238,6 +237,7 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
                        cv);
     T9_LAUNCH_CHECK();
   };
+  KScope ks_(c, KT_FFT, fwd_only ? 3 : 5);  // the passes run back to back
   launch(0, -1, none);
   launch(1, -1, none);
   launch(2, -1, conv);
