@@ -128,6 +128,11 @@ typedef struct {
  * Replaces induce(uind, uinp) (include/ff/amoeba/induce.h:34,
  * src/amoeba/induce.cpp:108-113 -> src/cu/amoeba/pcg.cu:14-185). */
 int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats);
+/* RMS residual (Debye) of the last solve at each iteration: eps[0] is the
+ * initial guess, eps[k] iteration k; *count = iterations + 1 (copies at most
+ * cap values, at most 256 are kept).  The values the reference prints per
+ * iteration in debug mode (src/cu/amoeba/pcg.cu:150-157). */
+int t9_induce_trace(t9_ctx* ctx, double* eps, int cap, int* count);
 int t9_get_uind(t9_ctx* ctx, double* uind, double* uinp);
 int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
 

@@ -44,6 +44,12 @@ def test_aspc_trajectory(prec):
             og = None
         hist.append((od, op))
         assert st.predictor == (1 if t >= 16 else 0)
+        # per-iteration residuals: the stopping rule (first iteration >= 3
+        # below polar-eps) and the stats agree with the trace
+        tr = ctx.induce_trace()
+        assert len(tr) == st.iterations + 1
+        assert tr[0] == st.guess_rms and tr[-1] == st.rms_debye
+        assert tr[-1] < 1e-6 and all(tr[k] >= 1e-6 for k in range(3, st.iterations))
         tol = 1e-9 if prec == t9hip.T9_FP64 else 2e-5
         assert np.abs(ud - od).max() < tol * 10 and np.abs(up - op).max() < tol * 10
         if og is not None:

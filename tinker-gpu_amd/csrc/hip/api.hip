@@ -65,6 +65,7 @@ void alloc_vectors(Ctx& c) {
     T9_HIP(hipMemsetAsync(b->p, 0, vb, c.stream));
   }
   c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
+  c.pcg_trace.alloc(sizeof(double) * kTraceCap);
   c.dot_counter.alloc(sizeof(unsigned));
   T9_HIP(hipMemsetAsync(c.dot_counter.p, 0, sizeof(unsigned), c.stream));
   c.scalars.alloc(sizeof(double) * 48);
@@ -607,6 +608,20 @@ int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats) {
     if (uind) t9::gather_to_host(c, c.uind.p, uind);
     if (uinp) t9::gather_to_host(c, c.uinp.p, uinp);
     if (rc != 0) throw t9::NotConverged("INDUCE -- induced dipoles are not converged");
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_induce_trace(t9_ctx* ctx, double* eps, int cap, int* count) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (cap < 0 || (cap > 0 && !eps)) throw t9::ArgError("t9_induce_trace: bad buffer");
+    const int m = std::min(cap, c.pcg_trace_len);
+    if (m > 0) {
+      T9_HIP(hipMemcpyAsync(eps, c.pcg_trace.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+      T9_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (count) *count = c.pcg_trace_len;
     return T9_OK;
   }, T9_ERR_HIP);
 }

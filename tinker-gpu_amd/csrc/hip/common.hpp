@@ -243,6 +243,22 @@ __device__ __forceinline__ void ewald_bn(real* bn, real r, real rinv, real rr2, 
 // ~12 waves per SIMD in flight on 96k atoms instead of ~1.5.
 constexpr int kPairParts = 8;
 
+// Early exit for the kernels of one CG iteration.  The solver keeps the
+// iteration at which it converged in CG scalar slot kDoneSlot (0 while it
+// runs); kernels enqueued for a later iteration return at once, so the host
+// can enqueue several iterations without reading the residual back.
+constexpr int kDoneSlot = 15;
+constexpr int kTraceCap = 256;  // per-iteration residual trace of the last CG solve
+struct Gate {
+  const double* sc = nullptr;  // CG scalars; nullptr: no gate
+  int iter = 0;                // iteration the launch belongs to
+};
+__device__ __forceinline__ bool gate_closed(const Gate& g) {
+  if (!g.sc) return false;
+  const double d = g.sc[kDoneSlot];
+  return d != 0 && d < (double)g.iter;
+}
+
 // v[0..NV) of wave 0 += the same values of waves 1..NP-1, in that order;
 // sh: (NP - 1) * NV * 64 doubles.  Every thread of the workgroup calls it.
 template <int NP, int NV>

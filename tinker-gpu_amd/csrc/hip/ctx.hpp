@@ -149,6 +149,10 @@ struct Ctx {
   DBuf field, fieldp, udir, udirp, uind, uinp, rsd, rsdp, zrsd, zrsdp, conj, conjp, vec, vecp;
   DBuf partials;                 // double [nblocks * 8]
   DBuf dot_counter;              // unsigned: workgroups done in the current dot kernel (0 between launches)
+  Gate gate;                     // set while a CG iteration is enqueued (early exit once converged)
+  int pcg_planned = 0;           // iterations the last solve needed: enqueued before the first readback
+  DBuf pcg_trace;                // double [kTraceCap]: RMS residual (Debye) of the last solve per iteration
+  int pcg_trace_len = 0;         // entries 0..iterations of the last solve (0: initial guess)
   DBuf scalars;                  // double [32] on device
   DBuf hscalars_pinned;          // host-pinned mirror
   double* h_scalars = nullptr;
@@ -191,9 +195,9 @@ struct Ctx {
   DBuf uaspc;                    // real [2][n][3] sorted: ASPC guess of the current solve
   DBuf gfeat;                    // float [n][32] sorted raw features of the current solve
   DBuf gde;                      // float [n][6] caller order: field extrapolation errors (d, p)
-  DBuf gx;                       // bf16 [n][32] normalized features
-  DBuf gy;                       // float [n][8] network outputs
   DBuf gp, gm, gv, gpb, ggrad;   // FP32 params / Adam moments, bf16 copy, gradient
+  DBuf gpack;                    // bf16 weights in MFMA operand order (inference), rebuilt when stale
+  bool gpack_valid = false;
   DBuf gpart;                    // float partial gradients [blocks][nparam + 1]
   DBuf gnorm;                    // float [76]: feature mean, 1/std, target mean, std
   std::vector<float> gnorm_h;

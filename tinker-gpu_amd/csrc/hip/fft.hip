@@ -47,7 +47,8 @@ __device__ __forceinline__ long long elem_addr(int axis, int K1, int K2, int lb,
 template <class real, int B, int NC>
 __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid, int K1, int K2, int K3, int axis,
                                                   int sign, FftPlan plan, const Cplx<real>* __restrict__ tw,
-                                                  ConvArgs conv) {
+                                                  ConvArgs conv, Gate gate) {
+  if (gate_closed(gate)) return;
   extern __shared__ __align__(16) char smem[];
   const int N = NC > 0 ? NC : plan.n;
   Cplx<real>* buf0 = reinterpret_cast<Cplx<real>*>(smem);
@@ -219,7 +220,7 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
     const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
     const size_t sh = 2 * (size_t)B * (p[axis].n + 1) * sizeof(Cplx<real>);
-    void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs);
+    void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs, Gate);
     switch (p[axis].n) {
       case 24: kern = k_fft_pass<real, B, 24>; break;
       case 32: kern = k_fft_pass<real, B, 32>; break;
@@ -234,7 +235,7 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
       default: kern = k_fft_pass<real, B, 0>; break;
     }
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign, p[axis], twa[axis],
-                       cv);
+                       cv, c.gate);
     T9_LAUNCH_CHECK();
   };
   KScope ks_(c, KT_FFT, fwd_only ? 3 : 5);  // the passes run back to back

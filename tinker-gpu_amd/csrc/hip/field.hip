@@ -108,7 +108,9 @@ __global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const in
                                                                  const int* __restrict__ jpolar,
                                                                  const real* __restrict__ thlval, int nj, real pga_u,
                                                                  ScaleTable sc, BoxT<real> box, EwaldK<real> ek,
-                                                                 real* __restrict__ fd, real* __restrict__ fp) {
+                                                                 real* __restrict__ fd, real* __restrict__ fp,
+                                                                 Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ double sh[(kPairParts - 1) * 6 * 64];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -260,7 +262,8 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
                                                  const int* __restrict__ jpolar, const real* __restrict__ thlval,
                                                  int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                  const real* __restrict__ rd, const real* __restrict__ rp,
-                                                 real* __restrict__ zd, real* __restrict__ zp) {
+                                                 real* __restrict__ zd, real* __restrict__ zp, Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ double sh[(kPairParts - 1) * 6 * 64];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -351,7 +354,7 @@ void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
   hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
                      (const real*)up, c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale,
-                     make_boxT<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp);
+                     make_boxT<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp, c.gate);
   T9_LAUNCH_CHECK();
 }
 
@@ -380,7 +383,7 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
                      (real)c.opt.udiag, L.ncut2.as<int>(), L.off.as<long long>(), L.ecut2.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
                      (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
-                     (real*)zp);
+                     (real*)zp, c.gate);
   T9_LAUNCH_CHECK();
 }
 

@@ -130,24 +130,195 @@ __device__ f32x16 wave_forward(const Net& w, const u16* sX, u16* sH1, u16* sH2, 
   return acc;
 }
 
-// Y[m][8] (columns 0..5) = network(X[m][32]); one wavefront per 32 rows.
-__global__ __launch_bounds__(64) void k_mlp_fwd(int m, const u16* __restrict__ X, const u16* __restrict__ pb,
-                                                const float* __restrict__ p, float* __restrict__ Y) {
-  __shared__ u16 sX[32 * NF], sH1[32 * NH], sH2[32 * NH];
-  const int row0 = blockIdx.x * 32;
-  for (int e = threadIdx.x; e < 32 * NF; e += 64) {
-    const int rr = e / NF;
-    sX[e] = row0 + rr < m ? X[(size_t)(row0 + rr) * NF + (e % NF)] : (u16)0;
+// ---- inference: weights in MFMA operand order, layers computed transposed
+//
+// Each layer is computed as out^T (32 units x 32 samples) = W^T (32 x K)
+// act^T (K x 32 samples), so that samples run along the MFMA columns.  In
+// that form the output registers of one layer are the B operand of the next
+// once the hidden units are renumbered: output tile t, lane half h = l / 32,
+// register j holds unit p = 32 t + 8 (j / 4) + 4 h + j % 4, and is used as
+// B row k = 8 h + (j % 8) of k-step 2 t + j / 8 of the next layer.  The
+// weights are packed once per parameter update as A-operand fragments
+// (fragment f, lane l: 8 bf16), 50 fragments of 1 KB:
+//   f  0.. 7: W1^T, tile t = f / 2, k-step f % 2 (features in natural order)
+//   f  8..39: W2^T, tile (f - 8) / 8, k-step (f - 8) % 8 (renumbered units)
+//   f 40..47: W3^T over the hidden units (renumbered), rows >= 6 zero
+//   f 48..49: W3^T over the skip features, rows >= 6 zero
+constexpr int kFrags = 50;
+
+__host__ __device__ __forceinline__ int hidden_of(int ks, int h, int i) {  // k-step, lane half, element -> unit
+  const int t = ks >> 1, j = 8 * (ks & 1) + i;
+  return 32 * t + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+
+__global__ void k_pack_weights(const u16* __restrict__ pb, u16* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kFrags * 64 * 8) return;
+  const int f = e >> 9, l = (e >> 3) & 63, i = e & 7, h = l >> 5, r = l & 31;
+  u16 v = 0;
+  if (f < 8) {
+    const int t = f >> 1, k = 16 * (f & 1) + 8 * h + i;
+    v = pb[OFF_W1 + k * NH + 32 * t + r];
+  } else if (f < 40) {
+    const int t2 = (f - 8) >> 3, ks = (f - 8) & 7;
+    v = pb[OFF_W2 + hidden_of(ks, h, i) * NH + 32 * t2 + r];
+  } else if (f < 48) {
+    if (r < NO) v = pb[OFF_W3 + hidden_of(f - 40, h, i) * NO + r];
+  } else {
+    if (r < NO) v = pb[OFF_W3 + (NH + 16 * (f - 48) + 8 * h + i) * NO + r];
   }
-  __syncthreads();
-  const f32x16 acc = wave_forward(net_of(pb, p), sX, sH1, sH2, nullptr, nullptr);
-  const int n = ncol();
-  if (n < NO)
+  out[e] = v;
+}
+
+__device__ __forceinline__ f32x16 mfma16(const uint4 a, const bf16x8 b, f32x16 acc) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), b, acc, 0, 0, 0);
+}
+
+// One wavefront, 32 samples: bX = the two B fragments of the normalized
+// features (k-steps 0, 1).  Returns the output tile: lanes 0..31 hold
+// outputs 0..3 of sample l in registers 0..3, lanes 32..63 outputs 4, 5 of
+// sample l - 32 in registers 0, 1 (bias added).
+__device__ __forceinline__ f32x16 tile_forward(const uint4* __restrict__ sW, const float* __restrict__ sb,
+                                               const bf16x8 bX[2]) {
+  const int l = threadIdx.x & 63, h = l >> 5;
+  const f32x16 zero = {0};
+  bf16x8 hB[8], h2B[8];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x16 acc = mfma16(sW[(2 * t) * 64 + l], bX[0], zero);
+    acc = mfma16(sW[(2 * t + 1) * 64 + l], bX[1], acc);
+    s16x8 lo, hi;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int r = row0 + mrow(j);
-      if (r < m) Y[(size_t)r * 8 + n] = acc[j];
+      const float z = acc[j] + sb[32 * t + 8 * (j >> 2) + 4 * h + (j & 3)];
+      const short v = (short)f2bf(silu(z));
+      if (j < 8) lo[j] = v;
+      else hi[j - 8] = v;
     }
+    hB[2 * t] = __builtin_bit_cast(bf16x8, lo);
+    hB[2 * t + 1] = __builtin_bit_cast(bf16x8, hi);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x16 acc = zero;
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) acc = mfma16(sW[(8 + 8 * t + ks) * 64 + l], hB[ks], acc);
+    s16x8 lo, hi;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float z = acc[j] + sb[NH + 32 * t + 8 * (j >> 2) + 4 * h + (j & 3)];
+      const short v = (short)f2bf(silu(z));
+      if (j < 8) lo[j] = v;
+      else hi[j - 8] = v;
+    }
+    h2B[2 * t] = __builtin_bit_cast(bf16x8, lo);
+    h2B[2 * t + 1] = __builtin_bit_cast(bf16x8, hi);
+  }
+  f32x16 acc = zero;
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks) acc = mfma16(sW[(40 + ks) * 64 + l], h2B[ks], acc);
+  acc = mfma16(sW[48 * 64 + l], bX[0], acc);
+  acc = mfma16(sW[49 * 64 + l], bX[1], acc);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] += sb[2 * NH + 4 * h + j];
+  return acc;
+}
+
+// Stage the packed weights, the biases (and the normalization) in LDS.
+__device__ __forceinline__ void stage_net(const uint4* __restrict__ packed, const float* __restrict__ p,
+                                          const float* __restrict__ norm, uint4* sW, float* sb, float* sn) {
+  for (int e = threadIdx.x; e < kFrags * 64; e += blockDim.x) sW[e] = packed[e];
+  for (int e = threadIdx.x; e < 2 * NH + 8; e += blockDim.x)
+    sb[e] = e < NH ? p[OFF_B1 + e] : (e < 2 * NH ? p[OFF_B2 + e - NH] : (e - 2 * NH < NO ? p[OFF_B3 + e - 2 * NH] : 0.f));
+  if (sn)
+    for (int e = threadIdx.x; e < kNorm; e += blockDim.x) sn[e] = norm[e];
+  __syncthreads();
+}
+
+// Y[m][8] (columns 0..5) = network(X[m][32]), X normalized bf16.  Four
+// wavefronts per workgroup, each looping over 32-row tiles.
+__global__ __launch_bounds__(256) void k_mlp_infer_y(int m, const u16* __restrict__ X, const uint4* __restrict__ packed,
+                                                     const float* __restrict__ p, float* __restrict__ Y) {
+  __shared__ uint4 sW[kFrags * 64];
+  __shared__ float sb[2 * NH + 8];
+  stage_net(packed, p, nullptr, sW, sb, nullptr);
+  const int l = threadIdx.x & 63, h = l >> 5;
+  const int ntile = (m + 31) / 32;
+  for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntile; tile += gridDim.x * 4) {
+    // weights are re-read from LDS per tile rather than held in registers
+    asm volatile("" ::: "memory");
+    const int s = 32 * tile + (l & 31);
+    bf16x8 bX[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      uint4 v = {0, 0, 0, 0};
+      if (s < m) v = *reinterpret_cast<const uint4*>(X + (size_t)s * NF + 16 * ks + 8 * h);
+      bX[ks] = __builtin_bit_cast(bf16x8, v);
+    }
+    const f32x16 acc = tile_forward(sW, sb, bX);
+    if (s < m) {
+      if (h == 0)
+        for (int j = 0; j < 4; ++j) Y[(size_t)s * 8 + j] = acc[j];
+      else
+        for (int j = 0; j < 2; ++j) Y[(size_t)s * 8 + 4 + j] = acc[j];
+    }
+  }
+}
+
+// The learned correction applied in place: normalize the raw features F
+// [n][32], run the network, u += R^T (y * std + mean) for d (outputs 0..2)
+// and p (3..5) with R the atom's frame rotation.
+template <class real>
+__global__ __launch_bounds__(256) void k_mlp_apply(int n, const float* __restrict__ F, const uint4* __restrict__ packed,
+                                                   const float* __restrict__ p, const float* __restrict__ norm,
+                                                   const float* __restrict__ frot, real* __restrict__ ud,
+                                                   real* __restrict__ up) {
+  __shared__ uint4 sW[kFrags * 64];
+  __shared__ float sb[2 * NH + 8], sn[kNorm];
+  stage_net(packed, p, norm, sW, sb, sn);
+  const int l = threadIdx.x & 63, h = l >> 5;
+  const int ntile = (n + 31) / 32;
+  for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntile; tile += gridDim.x * 4) {
+    // weights are re-read from LDS per tile rather than held in registers
+    asm volatile("" ::: "memory");
+    const int s = 32 * tile + (l & 31);
+    bf16x8 bX[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      s16x8 v;
+      const int k0 = 16 * ks + 8 * h;
+      if (s < n) {
+        const float4 f0 = *reinterpret_cast<const float4*>(F + (size_t)s * NF + k0);
+        const float4 f1 = *reinterpret_cast<const float4*>(F + (size_t)s * NF + k0 + 4);
+        const float f[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (short)f2bf((f[i] - sn[k0 + i]) * sn[NF + k0 + i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = 0;
+      }
+      bX[ks] = __builtin_bit_cast(bf16x8, v);
+    }
+    const f32x16 acc = tile_forward(sW, sb, bX);
+    // lanes 0..31: outputs 0..3; lanes 32..63: outputs 4, 5 (same sample)
+    const float x3 = __shfl_xor(acc[3], 32);
+    if (s < n) {
+      float y0, y1, y2;
+      if (h == 0) {
+        y0 = acc[0] * sn[2 * NF + NO + 0] + sn[2 * NF + 0];
+        y1 = acc[1] * sn[2 * NF + NO + 1] + sn[2 * NF + 1];
+        y2 = acc[2] * sn[2 * NF + NO + 2] + sn[2 * NF + 2];
+      } else {
+        y0 = x3 * sn[2 * NF + NO + 3] + sn[2 * NF + 3];
+        y1 = acc[0] * sn[2 * NF + NO + 4] + sn[2 * NF + 4];
+        y2 = acc[1] * sn[2 * NF + NO + 5] + sn[2 * NF + 5];
+      }
+      const float* R = frot + 9 * (size_t)s;
+      real* u = h == 0 ? ud : up;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) u[3 * s + g] += (real)(R[g] * y0 + R[3 + g] * y1 + R[6 + g] * y2);
+    }
+  }
 }
 
 // Per-workgroup partial gradient of the mean squared error over `mtot`
@@ -366,29 +537,6 @@ __global__ __launch_bounds__(256) void k_features(int n, const int* __restrict__
   f[28 + min(jpolar[s], 3)] = 1.0f;
 }
 
-__global__ void k_normalize(int m, const float* __restrict__ F, const float* __restrict__ norm,
-                            u16* __restrict__ X) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= (long long)m * NF) return;
-  const int c = (int)(e % NF);
-  X[e] = f2bf((F[e] - norm[c]) * norm[NF + c]);
-}
-
-// u += R^T (Y * std + mean), d and p
-template <class real>
-__global__ void k_apply(int n, const float* __restrict__ frot, const float* __restrict__ Y,
-                        const float* __restrict__ norm, real* __restrict__ ud, real* __restrict__ up) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const float* R = frot + 9 * (size_t)s;
-  float y[6];
-  for (int q = 0; q < 6; ++q) y[q] = Y[(size_t)s * 8 + q] * norm[2 * NF + NO + q] + norm[2 * NF + q];
-  for (int g = 0; g < 3; ++g) {
-    ud[3 * s + g] += (real)(R[g] * y[0] + R[3 + g] * y[1] + R[6 + g] * y[2]);
-    up[3 * s + g] += (real)(R[g] * y[3] + R[3 + g] * y[4] + R[6 + g] * y[5]);
-  }
-}
-
 // history slot <- current direct field (sorted -> caller order)
 template <class real>
 __global__ void k_save_field(int n, const int* __restrict__ perm, const real* __restrict__ fd,
@@ -519,26 +667,38 @@ void guess_to_bf16(Ctx& c, long long count, const float* src, void* dst) {
   T9_LAUNCH_CHECK();
 }
 
+// Inference weights in MFMA operand order, rebuilt after a parameter change.
+void pack_weights(Ctx& c) {
+  if (c.gpack_valid) return;
+  c.gpack.alloc(sizeof(u16) * kFrags * 64 * 8);
+  hipLaunchKernelGGL(k_pack_weights, dim3(ceil_div(kFrags * 64 * 8, 256)), dim3(256), 0, c.stream, c.gpb.as<u16>(),
+                     c.gpack.as<u16>());
+  T9_LAUNCH_CHECK();
+  c.gpack_valid = true;
+}
+
+// Workgroups of the inference kernels: four 32-row tiles per pass, at most
+// two workgroups per CU (each stages the 52 KB of weights in LDS).
+int infer_blocks(int m) { return std::max(1, std::min(ceil_div(ceil_div(m, 32), 4), 512)); }
+
 void mlp_forward(Ctx& c, int m, const void* X, float* Y) {
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(ceil_div(m, 32)), dim3(64), 0, c.stream, m, (const u16*)X, c.gpb.as<u16>(),
-                     c.gp.as<float>(), Y);
+  pack_weights(c);
+  hipLaunchKernelGGL(k_mlp_infer_y, dim3(infer_blocks(m)), dim3(256), 0, c.stream, m, (const u16*)X,
+                     c.gpack.as<uint4>(), c.gp.as<float>(), Y);
   T9_LAUNCH_CHECK();
 }
 
 void guess_apply(Ctx& c, void* ud, void* up) {
   const int n = c.n;
-  c.gx.alloc(sizeof(u16) * NF * (size_t)n);
-  c.gy.alloc(sizeof(float) * 8 * (size_t)n);
-  hipLaunchKernelGGL(k_normalize, dim3(ceil_div((long)n * NF, 256)), dim3(256), 0, c.stream, n, c.gfeat.as<float>(),
-                     c.gnorm.as<float>(), c.gx.as<u16>());
-  T9_LAUNCH_CHECK();
-  mlp_forward(c, n, c.gx.p, c.gy.as<float>());
+  pack_weights(c);
   if (c.prec == 32)
-    hipLaunchKernelGGL(k_apply<float>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.frot.as<float>(),
-                       c.gy.as<float>(), c.gnorm.as<float>(), (float*)ud, (float*)up);
+    hipLaunchKernelGGL(k_mlp_apply<float>, dim3(infer_blocks(n)), dim3(256), 0, c.stream, n, c.gfeat.as<float>(),
+                       c.gpack.as<uint4>(), c.gp.as<float>(), c.gnorm.as<float>(), c.frot.as<float>(), (float*)ud,
+                       (float*)up);
   else
-    hipLaunchKernelGGL(k_apply<double>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.frot.as<float>(),
-                       c.gy.as<float>(), c.gnorm.as<float>(), (double*)ud, (double*)up);
+    hipLaunchKernelGGL(k_mlp_apply<double>, dim3(infer_blocks(n)), dim3(256), 0, c.stream, n, c.gfeat.as<float>(),
+                       c.gpack.as<uint4>(), c.gp.as<float>(), c.gnorm.as<float>(), c.frot.as<float>(), (double*)ud,
+                       (double*)up);
   T9_LAUNCH_CHECK();
 }
 
@@ -566,6 +726,7 @@ void guess_capture(Ctx& c) {
 
 void guess_set_params(Ctx& c, const float* params, const float* norm, uint64_t seed) {
   guess_setup(c);
+  c.gpack_valid = false;
   std::vector<float> p(NPARAM);
   if (params) {
     std::copy(params, params + NPARAM, p.begin());
@@ -622,6 +783,7 @@ double mlp_grad(Ctx& c, int m, const void* X, const float* T, float* grad_dev) {
 
 void adam_step(Ctx& c, const float* grad_dev, double lr) {
   c.adam_t += 1;
+  c.gpack_valid = false;
   const float c1 = (float)(1.0 - std::pow(0.9, c.adam_t)), c2 = (float)(1.0 - std::pow(0.999, c.adam_t));
   hipLaunchKernelGGL(k_adam, dim3(ceil_div(NPARAM, 256)), dim3(256), 0, c.stream, grad_dev, c.gp.as<float>(),
                      c.gm.as<float>(), c.gv.as<float>(), c.gpb.as<u16>(), (float)lr, c1, c2);

@@ -21,6 +21,8 @@
 // (re-measured every 20 solves), else the solve starts from ASPC.
 #include <array>
 
+#include <algorithm>
+
 #include "ctx.hpp"
 
 namespace t9 {
@@ -28,7 +30,7 @@ namespace {
 
 constexpr int kB = 256;
 // device scalar slots
-enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_EPS0, S_NSLOT = 16 };
+enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_EPS0, S_DONE = kDoneSlot, S_NSLOT = 16 };
 constexpr int kMaxUalt = 16;
 
 struct Coef16 {
@@ -115,12 +117,29 @@ __global__ __launch_bounds__(kB) void k_copy3_zero_nopol(int n, const real* __re
   }
 }
 
+struct DotOut {
+  double* partials;   // [nblk][4]
+  unsigned* counter;  // workgroups finished (reset by the last one)
+  double* sc;         // CG scalars
+  int stage, n;
+  double debye;
+  // stage 2: convergence test of iteration `iter` (src/cu/amoeba/pcg.cu:147-170):
+  // eps < tol once iter >= iter_min, or iter >= iter_max; the converged
+  // iteration is stored in slot S_DONE
+  double tol;
+  int iter, iter_min, iter_max;
+  double* trace;  // RMS residual per iteration (entry 0: initial guess)
+};
+
 // Finalize: sum the per-block partials in fixed order, then update the CG
 // scalars.  stage 0: initial sum = r.z (slots SUM, SUMP); 1: a = sum / p.Tp
-// (slots A, AP); 2: b = sum1 / sum, sum = sum1, eps from r.r; 3: eps of
-// the initial residual.  Run by the last workgroup of the dot kernel.
-__device__ void finalize_block(int nblk, int ndots, int stage, int n, double debye,
-                               const double* __restrict__ partials, double* __restrict__ sc, double* sh) {
+// (slots A, AP); 2: b = sum1 / sum, sum = sum1, eps from r.r, convergence
+// test; 3: eps of the initial residual.  Run by the last workgroup of the
+// dot kernel.
+__device__ void finalize_block(int nblk, int ndots, const DotOut& o, const double* __restrict__ partials,
+                               double* __restrict__ sc, double* sh) {
+  const int stage = o.stage, n = o.n;
+  const double debye = o.debye;
   double tot[4] = {0, 0, 0, 0};
   for (int q = 0; q < ndots; ++q) {
     double v = 0;
@@ -133,6 +152,7 @@ __device__ void finalize_block(int nblk, int ndots, int stage, int n, double deb
     sc[S_SUMP] = tot[1];
   } else if (stage == 3) {
     sc[S_EPS0] = debye * sqrt(fmax(tot[0], tot[1]) / n);
+    o.trace[0] = sc[S_EPS0];
   } else if (stage == 1) {
     sc[S_A] = tot[0] != 0 ? sc[S_SUM] / tot[0] : 0;
     sc[S_AP] = tot[1] != 0 ? sc[S_SUMP] / tot[1] : 0;
@@ -143,17 +163,15 @@ __device__ void finalize_block(int nblk, int ndots, int stage, int n, double deb
     sc[S_SUMP] = tot[1];
     sc[S_EPSD] = tot[2];
     sc[S_EPSP] = tot[3];
-    sc[S_EPS] = debye * sqrt(fmax(tot[2], tot[3]) / n);
+    const double eps = debye * sqrt(fmax(tot[2], tot[3]) / n);
+    sc[S_EPS] = eps;
+    if (o.iter < kTraceCap) o.trace[o.iter] = eps;
+    bool done = eps < o.tol;
+    if (o.iter < o.iter_min) done = false;
+    if (o.iter >= o.iter_max) done = true;
+    if (done && sc[S_DONE] == 0) sc[S_DONE] = o.iter;
   }
 }
-
-struct DotOut {
-  double* partials;   // [nblk][4]
-  unsigned* counter;  // workgroups finished (reset by the last one)
-  double* sc;         // CG scalars
-  int stage, n;
-  double debye;
-};
 
 // Every workgroup writes its partial sums; the last one to finish (counted
 // with a device-scope atomic after a release fence) sums all partials in
@@ -169,7 +187,7 @@ __device__ __forceinline__ void dots_epilogue(const double* tot, int ndots, cons
   __syncthreads();
   if (!last) return;
   __threadfence();
-  finalize_block(gridDim.x, ndots, o.stage, o.n, o.debye, o.partials, o.sc, sh);
+  finalize_block(gridDim.x, ndots, o, o.partials, o.sc, sh);
   if (threadIdx.x == 0) atomicExch(o.counter, 0u);
 }
 
@@ -179,7 +197,8 @@ __global__ __launch_bounds__(kB) void k_dots(int n, int ndots, const real* __res
                                             const real* __restrict__ b1, const real* __restrict__ a2,
                                             const real* __restrict__ b2, const real* __restrict__ a3,
                                             const real* __restrict__ b3, const real* __restrict__ a4,
-                                            const real* __restrict__ b4, DotOut o) {
+                                            const real* __restrict__ b4, DotOut o, Gate gate = Gate()) {
+  if (gate_closed(gate)) return;
   __shared__ double sh[kB];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const real* A[4] = {a1, a2, a3, a4};
@@ -200,7 +219,8 @@ template <class real>
 __global__ __launch_bounds__(kB) void k_p1_dots(int n, const real* __restrict__ polinv, const real* __restrict__ cd,
                                                const real* __restrict__ cp, const real* __restrict__ fd,
                                                const real* __restrict__ fp, real* __restrict__ vd,
-                                               real* __restrict__ vp, DotOut o) {
+                                               real* __restrict__ vp, DotOut o, Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ double sh[kB];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double x = 0, y = 0;
@@ -221,28 +241,14 @@ __global__ __launch_bounds__(kB) void k_p1_dots(int n, const real* __restrict__ 
   dots_epilogue(tot, 2, o, sh);
 }
 
-// vec = conj / alpha - field   (reference pcgP1)
-template <class real>
-__global__ __launch_bounds__(kB) void k_p1(int n, const real* __restrict__ polinv, const real* __restrict__ cd,
-                                          const real* __restrict__ cp, const real* __restrict__ fd,
-                                          const real* __restrict__ fp, real* __restrict__ vd,
-                                          real* __restrict__ vp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const real pi = polinv[s];
-  for (int d = 0; d < 3; ++d) {
-    vd[3 * s + d] = pi * cd[3 * s + d] - fd[3 * s + d];
-    vp[3 * s + d] = pi * cp[3 * s + d] - fp[3 * s + d];
-  }
-}
-
 // u += a p; r -= a vec; r = 0 on unpolarizable sites   (reference pcgP2)
 template <class real>
 __global__ __launch_bounds__(kB) void k_p2(int n, const double* __restrict__ sc, const real* __restrict__ pol,
                                           const real* __restrict__ cd, const real* __restrict__ cp,
                                           const real* __restrict__ vd, const real* __restrict__ vp,
                                           real* __restrict__ ud, real* __restrict__ up, real* __restrict__ rd,
-                                          real* __restrict__ rp) {
+                                          real* __restrict__ rp, Gate gate) {
+  if (gate_closed(gate)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const real a = (real)sc[S_A], ap = (real)sc[S_AP];
@@ -256,11 +262,15 @@ __global__ __launch_bounds__(kB) void k_p2(int n, const double* __restrict__ sc,
   }
 }
 
-// p = z + b p   (reference pcgP3)
+// p = z + b p   (reference pcgP3); in the iteration that converged, also
+// the peek step u += peek * alpha * r (reference pcgPeek)
 template <class real>
 __global__ __launch_bounds__(kB) void k_p3(int n, const double* __restrict__ sc, const real* __restrict__ zd,
                                           const real* __restrict__ zp, real* __restrict__ cd,
-                                          real* __restrict__ cp) {
+                                          real* __restrict__ cp, real peek, const real* __restrict__ pol,
+                                          const real* __restrict__ rd, const real* __restrict__ rp,
+                                          real* __restrict__ ud, real* __restrict__ up, Gate gate) {
+  if (gate_closed(gate)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const real b = (real)sc[S_B], bp = (real)sc[S_BP];
@@ -269,20 +279,17 @@ __global__ __launch_bounds__(kB) void k_p3(int n, const double* __restrict__ sc,
     cd[k] = zd[k] + b * cd[k];
     cp[k] = zp[k] + bp * cp[k];
   }
+  if (sc[S_DONE] == (double)gate.iter) {
+    const real t = peek * pol[s];
+    for (int d = 0; d < 3; ++d) {
+      ud[3 * s + d] += t * rd[3 * s + d];
+      up[3 * s + d] += t * rp[3 * s + d];
+    }
+  }
 }
 
-// u += peek * alpha * r   (reference pcgPeek)
-template <class real>
-__global__ __launch_bounds__(kB) void k_peek(int n, real peek, const real* __restrict__ pol,
-                                            const real* __restrict__ rd, const real* __restrict__ rp,
-                                            real* __restrict__ ud, real* __restrict__ up) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const real t = peek * pol[s];
-  for (int d = 0; d < 3; ++d) {
-    ud[3 * s + d] += t * rd[3 * s + d];
-    up[3 * s + d] += t * rp[3 * s + d];
-  }
+__global__ void k_set_scalar(double* sc, int slot, double v) {
+  if (threadIdx.x == 0) sc[slot] = v;
 }
 
 template <class real>
@@ -307,7 +314,10 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   double* sc = c.scalars.as<double>();
   const double debye = kDebye;
   const bool guess = c.opt.pcg_guess;
-  auto dout = [&](int stage) {
+  const int maxiter = 100;
+  const int miniter = n < 3 ? n : 3;
+  int iter = 0, iter0 = 0;  // iter0: iterations spent before a restart from ASPC
+  auto dout = [&](int stage, int it = 0) {
     DotOut o;
     o.partials = part;
     o.counter = c.dot_counter.as<unsigned>();
@@ -315,7 +325,17 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     o.stage = stage;
     o.n = n;
     o.debye = debye;
+    o.tol = c.opt.polar_eps;
+    o.iter = it;
+    o.iter_min = iter0 + miniter;
+    o.iter_max = c.opt.polar_iter;
+    o.trace = c.pcg_trace.as<double>();
     return o;
+  };
+  // slot S_DONE = 0: no iteration has converged yet
+  auto reset_done = [&]() {
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, sc, (int)S_DONE, 0.0);
+    T9_LAUNCH_CHECK();
   };
 
   dfield_all(c);
@@ -401,49 +421,62 @@ int induce_t(Ctx& c, int* iters, double* rms) {
                        (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(0));
     T9_LAUNCH_CHECK();
   };
+  reset_done();
   start_cg();
 
-  const int maxiter = 100;
-  const int miniter = n < 3 ? n : 3;
-  int iter = 0, iter0 = 0;  // iter0: iterations spent before a restart from ASPC
-  double eps = 100;
-  bool done = false;
-  while (!done) {
-    ++iter;
+  // One CG iteration, gated: its kernels return at once if an earlier
+  // iteration converged (the test runs in the last workgroup of the r.r dot
+  // product; the peek step is applied by that iteration's p3).
+  auto enqueue = [&](int it) {
+    c.gate.sc = sc;
+    c.gate.iter = it;
+    const Gate g = c.gate;
     ufield_all(c, cd, cp, field, fieldp);
     hipLaunchKernelGGL(k_p1_dots<real>, G, Bk, 0, st, n, c.polinv.as<real>(), cd, cp, field, fieldp, vd, vp,
-                       dout(1));
-    hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up, rd, rp);
+                       dout(1, it), g);
+    hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up, rd, rp, g);
     T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp);
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, dout(2));
-    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, sc, zd, zp, cd, cp);
+    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, dout(2, it), g);
+    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, sc, zd, zp, cd, cp, (real)c.opt.pcg_peek,
+                       c.polarity.as<real>(), rd, rp, ud, up, g);
     T9_LAUNCH_CHECK();
+    c.gate = Gate();
+  };
+  // The host enqueues as many iterations as the previous solve needed, then
+  // reads the scalars back once; after that one iteration per readback.  A
+  // learned guess is judged after its first iteration.
+  double eps = 100;
+  int enq = 0;
+  for (;;) {
+    const int upto = learned_try ? enq + 1 : std::max(enq + 1, iter0 + c.pcg_planned);
+    while (enq < upto) enqueue(++enq);
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     T9_HIP(hipStreamSynchronize(st));
-    eps = c.h_scalars[S_EPS];
     if (learned_try) {
       learned_try = false;
       if (!(c.h_scalars[S_EPS0] <= c.aspc_eps0_ref)) {  // worse than ASPC, or not finite: restart
         T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
         T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
         residual();
+        reset_done();
         start_cg();
         ++c.guess_fallback;
         c.last_predictor = 1;
-        iter0 = iter;
+        iter0 = enq;
         continue;
       }
       ++c.guess_used;
     }
-    if (eps < c.opt.polar_eps) done = true;
-    if (iter - iter0 < miniter) done = false;
-    if (iter >= c.opt.polar_iter) done = true;
-    if (done) {
-      hipLaunchKernelGGL(k_peek<real>, G, Bk, 0, st, n, (real)c.opt.pcg_peek, c.polarity.as<real>(), rd, rp, ud, up);
-      T9_LAUNCH_CHECK();
+    const double done_at = c.h_scalars[S_DONE];
+    if (done_at != 0) {
+      iter = (int)done_at;
+      eps = c.h_scalars[S_EPS];
+      break;
     }
   }
+  c.pcg_planned = iter - iter0;
+  c.pcg_trace_len = std::min(iter + 1, kTraceCap);
   c.uind_valid = true;
   c.last_guess_rms = c.h_scalars[S_EPS0];
   if (c.last_predictor == 1 && !learned) c.aspc_eps0_ref = c.last_guess_rms;

@@ -187,7 +187,8 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
                                                 const int* __restrict__ tile_atoms, const R4<real>* __restrict__ pw,
                                                 const int4* __restrict__ pbase, const real* __restrict__ src1,
                                                 const real* __restrict__ src2, real a1, real a2, real a3, int chunk,
-                                                cplx_t<real>* __restrict__ grid) {
+                                                cplx_t<real>* __restrict__ grid, Gate gate) {
+  if (gate_closed(gate)) return;
   using SA = SpreadAtom<real, MODE>;
   constexpr int LV = SA::LV;
   constexpr int NACC = MODE == 1 ? 1 : 2;
@@ -383,7 +384,8 @@ __global__ __launch_bounds__(256) void k_gather_dip(int n, TileGeom g, const R4<
                                                     const cplx_t<real>* __restrict__ grid, real a1, real a2, real a3,
                                                     real term, const real* __restrict__ ud,
                                                     const real* __restrict__ up, real* __restrict__ fd,
-                                                    real* __restrict__ fp) {
+                                                    real* __restrict__ fp, Gate gate) {
+  if (gate_closed(gate)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const R4<real> w = pw[s];
@@ -524,7 +526,7 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
                      c.tile_count.as<int>(), c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
                      (const real*)s1, (const real*)s2, a1, a2, a3, chunk,
-                     c.qgrid.as<cplx_t<real>>());
+                     c.qgrid.as<cplx_t<real>>(), c.gate);
   T9_LAUNCH_CHECK();
 }
 
@@ -538,7 +540,7 @@ void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) 
   KScope ks_(c, KT_GATHER);
   hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
                      c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
-                     (const real*)ud, (const real*)up, (real*)fd, (real*)fp);
+                     (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
   T9_LAUNCH_CHECK();
 }
 

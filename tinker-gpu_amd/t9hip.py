@@ -26,7 +26,7 @@ EXPORTS = [
     "t9_system_get_f64", "t9_system_get_i32", "t9_system_get_scalar", "t9_system_get_scales",
     "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
-    "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_get_uind", "t9_set_uind", "t9_sync",
+    "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_induce_trace", "t9_get_uind", "t9_set_uind", "t9_sync",
     "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy", "t9_md_init", "t9_md_set_velocities",
     "t9_md_get_velocities", "t9_md_run", "t9_guess_nparam", "t9_guess_set_params", "t9_guess_get_params",
     "t9_guess_enable", "t9_guess_capture", "t9_guess_stats", "t9_guess_train", "t9_guess_mlp_forward",
@@ -99,6 +99,7 @@ def lib():
             ("t9_dfield", [_vp, _d, _d]),
             ("t9_ufield", [_vp, _d, _d, _d, _d]),
             ("t9_induce", [_vp, _d, _d, ctypes.POINTER(InduceStats)]),
+            ("t9_induce_trace", [_vp, _d, ctypes.c_int, _i]),
             ("t9_get_uind", [_vp, _d, _d]),
             ("t9_set_uind", [_vp, _d, _d]),
             ("t9_sync", [_vp]),
@@ -305,6 +306,13 @@ class Context:
             return ud, up, st
         check(lib().t9_induce(self.h, None, None, ctypes.byref(st)))
         return None, None, st
+
+    def induce_trace(self):
+        """RMS residual (Debye) of the last solve: [initial guess, iteration 1, ...]."""
+        eps = np.zeros(256)
+        cnt = ctypes.c_int(0)
+        check(lib().t9_induce_trace(self.h, _p(eps), 256, ctypes.byref(cnt)))
+        return eps[:min(cnt.value, 256)].copy()
 
     def uind(self):
         ud, up = np.zeros((self.n, 3)), np.zeros((self.n, 3))
