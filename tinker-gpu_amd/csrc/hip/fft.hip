@@ -42,6 +42,23 @@ __device__ __forceinline__ long long elem_addr(int axis, int K1, int K2, int lb,
   return (x0 + lb) + (long long)K1 * (other + (long long)K2 * e);
 }
 
+// FP32 passes with a compile-time length issue all of a thread's loads
+// before the first LDS write and keep a copy of the twiddles in LDS; the
+// FP64 passes keep the plain load loop (their prefetch arrays would go to
+// scratch memory) and read the twiddles from global memory.
+template <class real>
+__host__ __device__ constexpr bool twiddles_in_lds() {
+  return sizeof(real) == 4;
+}
+
+// transform lengths with a compile-time kernel (the switch in roundtrip_t)
+inline bool ct_len(int n) {
+  switch (n) {
+    case 24: case 32: case 40: case 48: case 64: case 80: case 96: case 120: case 128: case 160: return true;
+    default: return false;
+  }
+}
+
 // NC > 0: the transform length is the compile-time NC (fast index
 // arithmetic); NC == 0: any 2,3,5-smooth length from `plan`.
 template <class real, int B, int NC>
@@ -67,34 +84,92 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     nl = min(B, K1 - x0);
   }
   const int tot = nl * N;
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-    int lb, e;
+  // element q of the block's lines -> (line, index along the transform)
+  auto split = [&](int q, int& lb, int& e) {
     if (axis == 0) {
       lb = q / N;
       e = q - lb * N;
+    } else if (nl == B) {
+      e = q / B;
+      lb = q - e * B;
     } else {
       e = q / nl;
       lb = q - e * nl;
     }
-    buf0[lb * LP + e] = grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)];
+  };
+  const bool tabconv = conv.enabled && conv.table && !conv.vrows;
+  const real* tb = (const real*)conv.table + ((long long)other * K1 + x0) * N;
+  const Cplx<real>* twk = tw;
+  constexpr bool kPipe = NC > 0 && twiddles_in_lds<real>();
+  constexpr int PER = kPipe ? (B * NC + 255) / 256 : 1;  // elements per thread (256 threads)
+  real ftab[PER];
+  if constexpr (kPipe) {
+    // all of a thread's loads are issued before the first LDS write; the
+    // twiddles (and the z pass's influence-table values) come in the same phase
+    Cplx<real> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < tot) {
+        int lb, e;
+        split(q, lb, e);
+        v[k] = grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)];
+      }
+    }
+    Cplx<real>* twl = buf1 + B * LP;
+    if (threadIdx.x < NC) twl[threadIdx.x] = tw[threadIdx.x];
+    twk = twl;
+    if (tabconv)
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int q = threadIdx.x + 256 * k;
+        ftab[k] = q < tot ? tb[q] : (real)0;
+      }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < tot) {
+        int lb, e;
+        split(q, lb, e);
+        buf0[lb * LP + e] = v[k];
+      }
+    }
+  } else {
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+      int lb, e;
+      split(q, lb, e);
+      buf0[lb * LP + e] = grid[elem_addr(axis, K1, K2, lb, e, l0, x0, other)];
+    }
   }
   __syncthreads();
   Cplx<real>* res;
-  if constexpr (NC > 0) res = fft_stages_ct<real, NC>(buf0, buf1, nl, tw, sign, threadIdx.x, blockDim.x);
+  if constexpr (NC > 0) res = fft_stages_ct<real, NC>(buf0, buf1, nl, twk, sign, threadIdx.x, blockDim.x);
   else res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
-  if (conv.enabled && conv.table && !conv.vrows) {
+  if (tabconv) {
     // convolution with the tabulated influence function
-    const real* tb = (const real*)conv.table + ((long long)other * K1 + x0) * N;
-    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-      const real f = tb[q];
-      Cplx<real> v = res[(q / N) * LP + (q % N)];
-      v.x *= f;
-      v.y *= f;
-      res[(q / N) * LP + (q % N)] = v;
+    if constexpr (kPipe) {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int q = threadIdx.x + 256 * k;
+        if (q < tot) {
+          Cplx<real> v = res[(q / N) * LP + (q % N)];
+          v.x *= ftab[k];
+          v.y *= ftab[k];
+          res[(q / N) * LP + (q % N)] = v;
+        }
+      }
+    } else {
+      for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+        const real f = tb[q];
+        Cplx<real> v = res[(q / N) * LP + (q % N)];
+        v.x *= f;
+        v.y *= f;
+        res[(q / N) * LP + (q % N)] = v;
+      }
     }
     __syncthreads();
     Cplx<real>* spare = res == buf0 ? buf1 : buf0;
-    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, tw, -sign, threadIdx.x, blockDim.x);
+    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, twk, -sign, threadIdx.x, blockDim.x);
     else res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
   } else if (conv.enabled) {
     // z pass: lines are (x0 + lb, y = other), element index is k3.  The
@@ -143,7 +218,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     if (conv.fwd_only) return;
     __syncthreads();
     Cplx<real>* spare = res == buf0 ? buf1 : buf0;
-    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, tw, -sign, threadIdx.x, blockDim.x);
+    if constexpr (NC > 0) res = fft_stages_ct<real, NC>(res, spare, nl, twk, -sign, threadIdx.x, blockDim.x);
     else res = fft_stages<real>(res, spare, nl, LP, plan, tw, -sign, threadIdx.x, blockDim.x, DevBarrier());
   }
   for (int q = threadIdx.x; q < tot; q += blockDim.x) {
@@ -219,7 +294,9 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
   conv.table = c.conv_table.p;
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
     const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
-    const size_t sh = 2 * (size_t)B * (p[axis].n + 1) * sizeof(Cplx<real>);
+    // two line buffers + the twiddles (compile-time lengths stage them in LDS)
+    const size_t sh = (2 * (size_t)B * (p[axis].n + 1) + (twiddles_in_lds<real>() && ct_len(p[axis].n) ? p[axis].n : 0)) *
+                      sizeof(Cplx<real>);
     void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs, Gate);
     switch (p[axis].n) {
       case 24: kern = k_fft_pass<real, B, 24>; break;
