@@ -393,6 +393,8 @@ __device__ __forceinline__ void gather20(const GridGeom& g, const R4<real>& w, c
     int z = b.z + m3;
     if (z >= g.K[2]) z -= g.K[2];
     const real v0 = th3[0][m3], v1 = th3[1][m3], v2 = th3[2][m3], v3 = th3[3][m3];
+    // the FP64 variant keeps this loop rolled (unrolled, it spills to scratch)
+#pragma unroll(sizeof(real) == 4 ? 5 : 1)
     for (int m2 = 0; m2 < 5; ++m2) {
       int y = b.y + m2;
       if (y >= g.K[1]) y -= g.K[1];

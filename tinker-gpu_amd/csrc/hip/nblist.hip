@@ -650,37 +650,93 @@ void nblist_build_t(Ctx& c) {
 // pair kernels, so those kernels need no distance test of their own), in
 // list order (which keeps the pair kernels' gathers spatially local); with
 // cut2b > 0 also the entries within the shorter second radius into
-// ecut2 / ncut2, same layout.
+// ecut2 / ncut2, same layout.  kPairParts wavefronts per 64 atoms: in each
+// round the wavefronts take consecutive, wave-uniform chunks of C <= 64 list
+// positions (so a load instruction reads one contiguous 256-byte row) and
+// record which entries fall inside each radius in 64-bit masks; the per-part
+// counts are summed in part order through LDS, and each part then places its
+// kept entries from its offset on.  The output slab of the 64 atoms is
+// assembled in LDS (the first kCompactRowsA / kCompactRowsB rows; later rows,
+// rare, go straight to memory) and written as whole 256-byte rows: written
+// entry by entry, every store of a wavefront would touch 64 different rows.
+constexpr int kCompactRowsA = 176, kCompactRowsB = 64;
+
 template <class real>
-__global__ __launch_bounds__(256) void k_compact(int n, const int* __restrict__ nlist,
-                                                 const long long* __restrict__ woff,
-                                                 const uint32_t* __restrict__ entries,
-                                                 const PosRec<real>* __restrict__ pos, BoxT<real> box, real cut2,
-                                                 real cut2b, int* __restrict__ ncut, uint32_t* __restrict__ ecut,
-                                                 int* __restrict__ ncut2, uint32_t* __restrict__ ecut2) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+__global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
+    int n, const int* __restrict__ nlist, const long long* __restrict__ woff, const uint32_t* __restrict__ entries,
+    const PosRec<real>* __restrict__ pos, BoxT<real> box, real cut2, real cut2b, int* __restrict__ ncut,
+    uint32_t* __restrict__ ecut, int* __restrict__ ncut2, uint32_t* __restrict__ ecut2) {
+  constexpr int P = kPairParts;
+  __shared__ unsigned short cnt_a[P][64], cnt_b[P][64];
+  __shared__ uint32_t rows_a[kCompactRowsA][64], rows_b[kCompactRowsB][64];
+  const int part = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 64 + lane;
   const bool two = ecut2 != nullptr;
-  const long long base = woff[s >> 6] + (s & 63);
+  const bool valid = s < n;
+  const int len = valid ? nlist[s] : 0;
+  const int maxlen = wave_max(len);
+  const long long base = woff[blockIdx.x] + lane;
   const uint32_t* e = entries + base;
-  uint32_t* o = ecut + base;
-  uint32_t* ob = two ? ecut2 + base : nullptr;
-  const auto pi = pos[s];
-  int cnt = 0, cntb = 0;
-  const int len = nlist[s];
-  uint32_t nxt = len > 0 ? e[0] : 0;
-  for (int k = 0; k < len; ++k) {
-    const uint32_t ent = nxt;
-    if (k + 1 < len) nxt = e[(long long)(k + 1) * 64];
-    const auto pj = pos[ent >> kCodeBits];
-    real xr, yr, zr;
-    pair_disp(pi, pj, box, xr, yr, zr);
-    const real r2 = xr * xr + yr * yr + zr * zr;
-    if (r2 <= cut2) o[(long long)(cnt++) * 64] = ent;
-    if (two && r2 <= cut2b) ob[(long long)(cntb++) * 64] = ent;
+  PosRec<real> pi;
+  if (valid) pi = pos[s];
+  int tot_a = 0, tot_b = 0;  // kept entries of this atom in earlier rounds
+  for (int k0 = 0; k0 < maxlen;) {
+    const int C = min(64, (maxlen - k0 + P - 1) / P);
+    const int kb = k0 + part * C;
+    const int ke = min(kb + C, len);
+    unsigned long long ma = 0, mb = 0;
+    for (int k = kb; k < ke; ++k) {
+      const uint32_t ent = e[(long long)k * 64];
+      const auto pj = pos[ent >> kCodeBits];
+      real xr, yr, zr;
+      pair_disp(pi, pj, box, xr, yr, zr);
+      const real r2 = xr * xr + yr * yr + zr * zr;
+      if (r2 <= cut2) ma |= 1ull << (k - kb);
+      if (two && r2 <= cut2b) mb |= 1ull << (k - kb);
+    }
+    cnt_a[part][lane] = (unsigned short)__popcll(ma);
+    cnt_b[part][lane] = (unsigned short)__popcll(mb);
+    __syncthreads();
+    int oa = tot_a, ob = tot_b;
+    for (int q = 0; q < P; ++q) {
+      if (q < part) {
+        oa += cnt_a[q][lane];
+        ob += cnt_b[q][lane];
+      }
+      tot_a += cnt_a[q][lane];
+      tot_b += cnt_b[q][lane];
+    }
+    __syncthreads();
+    while (ma | mb) {
+      const int ja = ma ? __ffsll((long long)ma) - 1 : 64, jb = mb ? __ffsll((long long)mb) - 1 : 64;
+      const int j = min(ja, jb);
+      const uint32_t ent = e[(long long)(kb + j) * 64];
+      if (j == ja) {
+        if (oa < kCompactRowsA) rows_a[oa][lane] = ent;
+        else ecut[base + (long long)oa * 64] = ent;
+        ++oa;
+        ma &= ma - 1;
+      }
+      if (j == jb) {
+        if (ob < kCompactRowsB) rows_b[ob][lane] = ent;
+        else ecut2[base + (long long)ob * 64] = ent;
+        ++ob;
+        mb &= mb - 1;
+      }
+    }
+    k0 += P * C;
   }
-  ncut[s] = cnt;
-  if (two) ncut2[s] = cntb;
+  if (valid && part == 0) {
+    ncut[s] = tot_a;
+    if (two) ncut2[s] = tot_b;
+  }
+  // whole rows out; slots past an atom's count hold stale values that no
+  // kernel reads (it reads positions below ncut)
+  const int ra = min(wave_max(tot_a), kCompactRowsA), rb = two ? min(wave_max(tot_b), kCompactRowsB) : 0;
+  __syncthreads();
+  const long long wb = woff[blockIdx.x];
+  for (int q = threadIdx.x; q < ra * 64; q += 64 * P) ecut[wb + q] = rows_a[q >> 6][q & 63];
+  for (int q = threadIdx.x; q < rb * 64; q += 64 * P) ecut2[wb + q] = rows_b[q >> 6][q & 63];
 }
 
 template <class real>
@@ -696,9 +752,9 @@ void compact_t(Ctx& c) {
     L->ecut2.alloc(sizeof(uint32_t) * L->capacity);
   }
   const double c2 = L->cut * L->cut, c2b = L->cut2nd * L->cut2nd;
-  hipLaunchKernelGGL(k_compact<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L->nlist.as<int>(),
-                     L->off.as<long long>(), L->entries.as<uint32_t>(), c.pos.as<PosRec<real>>(), box,
-                     (real)(c2 > 1e30 ? 1e30 : c2), (real)(two ? c2b : -1.0), L->ncut.as<int>(),
+  hipLaunchKernelGGL(k_compact_parts<real>, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n,
+                     L->nlist.as<int>(), L->off.as<long long>(), L->entries.as<uint32_t>(), c.pos.as<PosRec<real>>(),
+                     box, (real)(c2 > 1e30 ? 1e30 : c2), (real)(two ? c2b : -1.0), L->ncut.as<int>(),
                      L->ecut.as<uint32_t>(), two ? L->ncut2.as<int>() : nullptr,
                      two ? L->ecut2.as<uint32_t>() : nullptr);
   T9_LAUNCH_CHECK();
