@@ -66,8 +66,6 @@ void alloc_vectors(Ctx& c) {
   }
   c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
   c.pcg_trace.alloc(sizeof(double) * kTraceCap);
-  c.dot_counter.alloc(sizeof(unsigned));
-  T9_HIP(hipMemsetAsync(c.dot_counter.p, 0, sizeof(unsigned), c.stream));
   c.scalars.alloc(sizeof(double) * 48);
   T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 48, c.stream));
   T9_HIP(hipHostMalloc((void**)&c.h_scalars, sizeof(double) * 48, hipHostMallocDefault));

@@ -148,7 +148,6 @@ struct Ctx {
   // ---- CG / field vectors, real [n][3] each, sorted order
   DBuf field, fieldp, udir, udirp, uind, uinp, rsd, rsdp, zrsd, zrsdp, conj, conjp, vec, vecp;
   DBuf partials;                 // double [nblocks * 8]
-  DBuf dot_counter;              // unsigned: workgroups done in the current dot kernel (0 between launches)
   Gate gate;                     // set while a CG iteration is enqueued (early exit once converged)
   int pcg_planned = 0;           // iterations the last solve needed: enqueued before the first readback
   DBuf pcg_trace;                // double [kTraceCap]: RMS residual (Debye) of the last solve per iteration

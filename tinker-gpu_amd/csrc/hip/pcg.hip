@@ -30,7 +30,8 @@ namespace {
 
 constexpr int kB = 256;
 // device scalar slots
-enum { S_SUM = 0, S_SUMP, S_A, S_AP, S_B, S_BP, S_EPSD, S_EPSP, S_EPS, S_EPS0, S_DONE = kDoneSlot, S_NSLOT = 16 };
+// slots 0..3: r.z sums SUM[0] (d, p), SUM[1] (d, p) by iteration parity
+enum { S_EPS = 8, S_EPS0 = 9, S_DONE = kDoneSlot, S_NSLOT = 16 };
 constexpr int kMaxUalt = 16;
 
 struct Coef16 {
@@ -117,113 +118,107 @@ __global__ __launch_bounds__(kB) void k_copy3_zero_nopol(int n, const real* __re
   }
 }
 
-struct DotOut {
-  double* partials;   // [nblk][4]
-  unsigned* counter;  // workgroups finished (reset by the last one)
-  double* sc;         // CG scalars
-  int stage, n;
-  double debye;
-  // stage 2: convergence test of iteration `iter` (src/cu/amoeba/pcg.cu:147-170):
-  // eps < tol once iter >= iter_min, or iter >= iter_max; the converged
-  // iteration is stored in slot S_DONE
-  double tol;
-  int iter, iter_min, iter_max;
-  double* trace;  // RMS residual per iteration (entry 0: initial guess)
-};
+// ---- CG scalars
+//
+// Dot products are summed per workgroup (partials[block][4]); the kernel that
+// needs a scalar sums all partials itself, in every workgroup, in the same
+// fixed order (thread t takes blocks t, t + 256, ..., then a fixed-order
+// block reduction), so every workgroup holds the same bits without a fence,
+// an atomic or an extra launch.  The running r.z sum of iteration `it` lives
+// in slots SUM[it & 1] (d, p); iteration it writes SUM[(it + 1) & 1], so no
+// workgroup reads a slot that another workgroup of the same kernel writes.
 
-// Finalize: sum the per-block partials in fixed order, then update the CG
-// scalars.  stage 0: initial sum = r.z (slots SUM, SUMP); 1: a = sum / p.Tp
-// (slots A, AP); 2: b = sum1 / sum, sum = sum1, eps from r.r, convergence
-// test; 3: eps of the initial residual.  Run by the last workgroup of the
-// dot kernel.
-__device__ void finalize_block(int nblk, int ndots, const DotOut& o, const double* __restrict__ partials,
-                               double* __restrict__ sc, double* sh) {
-  const int stage = o.stage, n = o.n;
-  const double debye = o.debye;
-  double tot[4] = {0, 0, 0, 0};
-  for (int q = 0; q < ndots; ++q) {
-    double v = 0;
-    for (int b = threadIdx.x; b < nblk; b += kB) v += partials[b * 4 + q];
-    tot[q] = block_sum<kB>(v, sh);
-  }
-  if (threadIdx.x != 0) return;
-  if (stage == 0) {
-    sc[S_SUM] = tot[0];
-    sc[S_SUMP] = tot[1];
-  } else if (stage == 3) {
-    sc[S_EPS0] = debye * sqrt(fmax(tot[0], tot[1]) / n);
-    o.trace[0] = sc[S_EPS0];
-  } else if (stage == 1) {
-    sc[S_A] = tot[0] != 0 ? sc[S_SUM] / tot[0] : 0;
-    sc[S_AP] = tot[1] != 0 ? sc[S_SUMP] / tot[1] : 0;
-  } else {
-    sc[S_B] = sc[S_SUM] != 0 ? tot[0] / sc[S_SUM] : 0;
-    sc[S_BP] = sc[S_SUMP] != 0 ? tot[1] / sc[S_SUMP] : 0;
-    sc[S_SUM] = tot[0];
-    sc[S_SUMP] = tot[1];
-    sc[S_EPSD] = tot[2];
-    sc[S_EPSP] = tot[3];
-    const double eps = debye * sqrt(fmax(tot[2], tot[3]) / n);
-    sc[S_EPS] = eps;
-    if (o.iter < kTraceCap) o.trace[o.iter] = eps;
-    bool done = eps < o.tol;
-    if (o.iter < o.iter_min) done = false;
-    if (o.iter >= o.iter_max) done = true;
-    if (done && sc[S_DONE] == 0) sc[S_DONE] = o.iter;
-  }
-}
-
-// Every workgroup writes its partial sums; the last one to finish (counted
-// with a device-scope atomic after a release fence) sums all partials in
-// workgroup order and updates the scalars, so the result does not depend on
-// which workgroup finishes last.
-__device__ __forceinline__ void dots_epilogue(const double* tot, int ndots, const DotOut& o, double* sh) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < ndots; ++q) o.partials[blockIdx.x * 4 + q] = tot[q];
-    __threadfence();
-    last = atomicAdd(o.counter, 1u) == gridDim.x - 1;
+// Fixed-order sum over a 256-thread block of NV values per thread: wave
+// butterflies, lane 0 of each wave to LDS, the four wave sums in order.
+// Every thread gets the same result.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV]) {
+  __shared__ double sh[4][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) sh[w][q] = x;
   }
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  finalize_block(gridDim.x, ndots, o, o.partials, o.sc, sh);
-  if (threadIdx.x == 0) atomicExch(o.counter, 0u);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+  __syncthreads();
 }
 
-// two dot products a1.b1 and a2.b2 and optionally a3.b3, a4.b4, finalized
-template <class real>
-__global__ __launch_bounds__(kB) void k_dots(int n, int ndots, const real* __restrict__ a1,
-                                            const real* __restrict__ b1, const real* __restrict__ a2,
-                                            const real* __restrict__ b2, const real* __restrict__ a3,
-                                            const real* __restrict__ b3, const real* __restrict__ a4,
-                                            const real* __restrict__ b4, DotOut o, Gate gate = Gate()) {
+// this workgroup's sums of NV per-thread values -> partials[blockIdx.x][q]
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ part) {
+  block_sums<NV>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[blockIdx.x * 4 + q] = v[q];
+}
+
+// totals of the NV partial columns over nblk workgroups
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nblk, double (&t)[NV]) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) t[q] = 0;
+  for (int b = threadIdx.x; b < nblk; b += kB)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] += part[b * 4 + q];
+  block_sums<NV>(t);
+}
+
+__device__ __forceinline__ double sum_slot(int it, int dp) { return 2 * (it & 1) + dp; }
+
+// NV (2 or 4) dot products a_q . b_q -> partials
+template <class real, int NV>
+__global__ __launch_bounds__(kB) void k_dots(int n, const real* __restrict__ a1, const real* __restrict__ b1,
+                                            const real* __restrict__ a2, const real* __restrict__ b2,
+                                            const real* __restrict__ a3, const real* __restrict__ b3,
+                                            const real* __restrict__ a4, const real* __restrict__ b4,
+                                            double* __restrict__ part, Gate gate = Gate()) {
   if (gate_closed(gate)) return;
-  __shared__ double sh[kB];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const real* A[4] = {a1, a2, a3, a4};
   const real* Bv[4] = {b1, b2, b3, b4};
-  double tot[4] = {0, 0, 0, 0};
-  for (int q = 0; q < ndots; ++q) {
-    double v = 0;
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    v[q] = 0;
     if (s < n)
-      for (int d = 0; d < 3; ++d) v += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
-    tot[q] = block_sum<kB>(v, sh);
+      for (int d = 0; d < 3; ++d) v[q] += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
   }
-  dots_epilogue(tot, ndots, o, sh);
+  write_partials<NV>(v, part);
 }
 
-// vec = conj / alpha - field (reference pcgP1) with the dots conj.vec,
-// finalized into a = sum / p.Tp
+// Start of a solve, one workgroup.  mode 1: partials hold r.z (d, p) and r.r
+// (d, p): SUM[parity] = r.z, initial RMS residual from r.r; mode 0: partials
+// hold r.r only (the residual of a trial guess).
+__global__ __launch_bounds__(kB) void k_cg_start(int nblk, int mode, int parity, int n, double debye,
+                                                const double* __restrict__ part, double* __restrict__ sc,
+                                                double* __restrict__ trace) {
+  double t[4];
+  reduce_partials<4>(part, nblk, t);
+  if (threadIdx.x != 0) return;
+  const double rr_d = mode ? t[2] : t[0], rr_p = mode ? t[3] : t[1];
+  const double eps0 = debye * sqrt(fmax(rr_d, rr_p) / n);
+  sc[S_EPS0] = eps0;
+  trace[0] = eps0;
+  if (mode) {
+    sc[2 * parity] = t[0];
+    sc[2 * parity + 1] = t[1];
+  }
+}
+
+// vec = conj / alpha - field (reference pcgP1), partials of conj.vec
 template <class real>
 __global__ __launch_bounds__(kB) void k_p1_dots(int n, const real* __restrict__ polinv, const real* __restrict__ cd,
                                                const real* __restrict__ cp, const real* __restrict__ fd,
                                                const real* __restrict__ fp, real* __restrict__ vd,
-                                               real* __restrict__ vp, DotOut o, Gate gate) {
+                                               real* __restrict__ vp, double* __restrict__ part, Gate gate) {
   if (gate_closed(gate)) return;
-  __shared__ double sh[kB];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double x = 0, y = 0;
+  double v[2] = {0, 0};
   if (s < n) {
     const real pi = polinv[s];
     for (int d = 0; d < 3; ++d) {
@@ -231,27 +226,29 @@ __global__ __launch_bounds__(kB) void k_p1_dots(int n, const real* __restrict__ 
       const real b = pi * cp[3 * s + d] - fp[3 * s + d];
       vd[3 * s + d] = a;
       vp[3 * s + d] = b;
-      x += (double)cd[3 * s + d] * (double)a;
-      y += (double)cp[3 * s + d] * (double)b;
+      v[0] += (double)cd[3 * s + d] * (double)a;
+      v[1] += (double)cp[3 * s + d] * (double)b;
     }
   }
-  double tot[4] = {0, 0, 0, 0};
-  tot[0] = block_sum<kB>(x, sh);
-  tot[1] = block_sum<kB>(y, sh);
-  dots_epilogue(tot, 2, o, sh);
+  write_partials<2>(v, part);
 }
 
-// u += a p; r -= a vec; r = 0 on unpolarizable sites   (reference pcgP2)
+// a = sum / p.vec from the p1 partials; u += a p; r -= a vec; r = 0 on
+// unpolarizable sites   (reference pcgP2)
 template <class real>
-__global__ __launch_bounds__(kB) void k_p2(int n, const double* __restrict__ sc, const real* __restrict__ pol,
+__global__ __launch_bounds__(kB) void k_p2(int n, int nblk, const double* __restrict__ part,
+                                          const double* __restrict__ sc, const real* __restrict__ pol,
                                           const real* __restrict__ cd, const real* __restrict__ cp,
                                           const real* __restrict__ vd, const real* __restrict__ vp,
                                           real* __restrict__ ud, real* __restrict__ up, real* __restrict__ rd,
                                           real* __restrict__ rp, Gate gate) {
   if (gate_closed(gate)) return;
+  double t[2];
+  reduce_partials<2>(part, nblk, t);
+  const double sd = sc[(int)sum_slot(gate.iter, 0)], sp = sc[(int)sum_slot(gate.iter, 1)];
+  const real a = (real)(t[0] != 0 ? sd / t[0] : 0), ap = (real)(t[1] != 0 ? sp / t[1] : 0);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const real a = (real)sc[S_A], ap = (real)sc[S_AP];
   const bool zero = pol[s] == 0;
   for (int d = 0; d < 3; ++d) {
     const int k = 3 * s + d;
@@ -262,28 +259,51 @@ __global__ __launch_bounds__(kB) void k_p2(int n, const double* __restrict__ sc,
   }
 }
 
-// p = z + b p   (reference pcgP3); in the iteration that converged, also
-// the peek step u += peek * alpha * r (reference pcgPeek)
+// From the partials of r.z (d, p) and r.r (d, p): b = sum1 / sum, the RMS
+// residual and the convergence test (src/cu/amoeba/pcg.cu:147-170: eps < tol
+// once iter >= iter_min, or iter >= iter_max); p = z + b p (reference
+// pcgP3); in the iteration that converged also the peek step u += peek *
+// alpha * r (reference pcgPeek).  Workgroup 0 stores sum1 for the next
+// iteration, eps, the trace entry and the converged iteration (slot S_DONE).
 template <class real>
-__global__ __launch_bounds__(kB) void k_p3(int n, const double* __restrict__ sc, const real* __restrict__ zd,
+__global__ __launch_bounds__(kB) void k_p3(int n, int nblk, const double* __restrict__ part, double* __restrict__ sc,
+                                          double tol, int iter_min, int iter_max, double debye,
+                                          double* __restrict__ trace, const real* __restrict__ zd,
                                           const real* __restrict__ zp, real* __restrict__ cd,
                                           real* __restrict__ cp, real peek, const real* __restrict__ pol,
                                           const real* __restrict__ rd, const real* __restrict__ rp,
                                           real* __restrict__ ud, real* __restrict__ up, Gate gate) {
   if (gate_closed(gate)) return;
+  const int it = gate.iter;
+  double t[4];
+  reduce_partials<4>(part, nblk, t);
+  const double sd = sc[(int)sum_slot(it, 0)], sp = sc[(int)sum_slot(it, 1)];
+  const real b = (real)(sd != 0 ? t[0] / sd : 0), bp = (real)(sp != 0 ? t[1] / sp : 0);
+  const double eps = debye * sqrt(fmax(t[2], t[3]) / n);
+  bool done = eps < tol;
+  if (it < iter_min) done = false;
+  if (it >= iter_max) done = true;
+  // slot S_DONE reads 0, or `it` once workgroup 0 has stored it below
+  const bool converged_here = done && (sc[S_DONE] == 0 || sc[S_DONE] == (double)it);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[(int)sum_slot(it + 1, 0)] = t[0];
+    sc[(int)sum_slot(it + 1, 1)] = t[1];
+    sc[S_EPS] = eps;
+    if (it < kTraceCap) trace[it] = eps;
+    if (converged_here) sc[S_DONE] = it;
+  }
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const real b = (real)sc[S_B], bp = (real)sc[S_BP];
   for (int d = 0; d < 3; ++d) {
     const int k = 3 * s + d;
     cd[k] = zd[k] + b * cd[k];
     cp[k] = zp[k] + bp * cp[k];
   }
-  if (sc[S_DONE] == (double)gate.iter) {
-    const real t = peek * pol[s];
+  if (converged_here) {
+    const real tp = peek * pol[s];
     for (int d = 0; d < 3; ++d) {
-      ud[3 * s + d] += t * rd[3 * s + d];
-      up[3 * s + d] += t * rp[3 * s + d];
+      ud[3 * s + d] += tp * rd[3 * s + d];
+      up[3 * s + d] += tp * rp[3 * s + d];
     }
   }
 }
@@ -317,21 +337,6 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   const int maxiter = 100;
   const int miniter = n < 3 ? n : 3;
   int iter = 0, iter0 = 0;  // iter0: iterations spent before a restart from ASPC
-  auto dout = [&](int stage, int it = 0) {
-    DotOut o;
-    o.partials = part;
-    o.counter = c.dot_counter.as<unsigned>();
-    o.sc = sc;
-    o.stage = stage;
-    o.n = n;
-    o.debye = debye;
-    o.tol = c.opt.polar_eps;
-    o.iter = it;
-    o.iter_min = iter0 + miniter;
-    o.iter_max = c.opt.polar_iter;
-    o.trace = c.pcg_trace.as<double>();
-    return o;
-  };
   // slot S_DONE = 0: no iteration has converged yet
   auto reset_done = [&]() {
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, sc, (int)S_DONE, 0.0);
@@ -357,9 +362,11 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     T9_LAUNCH_CHECK();
   };
   // RMS of r0 in Debye, read back now
+  double* trace = c.pcg_trace.as<double>();
   auto residual_rms = [&]() {
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(3));
+    hipLaunchKernelGGL((k_dots<real, 2>), G, Bk, 0, st, n, rd, rd, rp, rp, (const real*)nullptr,
+                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part, Gate());
+    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk, 0, 0, n, debye, part, sc, trace);
     T9_LAUNCH_CHECK();
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     T9_HIP(hipStreamSynchronize(st));
@@ -408,17 +415,15 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     hipLaunchKernelGGL(k_copy3_zero_nopol<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, rd, rp);
   }
   T9_LAUNCH_CHECK();
-  // residual norm of the initial guess (Debye, read back with the first
-  // iteration), first preconditioned search direction
+  // first preconditioned search direction; r.z of the first iteration and
+  // the residual norm of the initial guess (Debye, read back with the first
+  // iteration)
   auto start_cg = [&]() {
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, rd, rp, rp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(3));
-    T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp);
     T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
     T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 2, rd, zd, rp, zp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, dout(0));
+    hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, zd, rp, zp, rd, rd, rp, rp, part, Gate());
+    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk, 1, (iter0 + 1) & 1, n, debye, part, sc, trace);
     T9_LAUNCH_CHECK();
   };
   reset_done();
@@ -433,13 +438,15 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     const Gate g = c.gate;
     ufield_all(c, cd, cp, field, fieldp);
     hipLaunchKernelGGL(k_p1_dots<real>, G, Bk, 0, st, n, c.polinv.as<real>(), cd, cp, field, fieldp, vd, vp,
-                       dout(1, it), g);
-    hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up, rd, rp, g);
+                       part, g);
+    hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, nblk, part, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up,
+                       rd, rp, g);
     T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp);
-    hipLaunchKernelGGL(k_dots<real>, G, Bk, 0, st, n, 4, rd, zd, rp, zp, rd, rd, rp, rp, dout(2, it), g);
-    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, sc, zd, zp, cd, cp, (real)c.opt.pcg_peek,
-                       c.polarity.as<real>(), rd, rp, ud, up, g);
+    hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, zd, rp, zp, rd, rd, rp, rp, part, g);
+    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, nblk, part, sc, c.opt.polar_eps, iter0 + miniter,
+                       c.opt.polar_iter, debye, trace, zd, zp, cd, cp, (real)c.opt.pcg_peek, c.polarity.as<real>(),
+                       rd, rp, ud, up, g);
     T9_LAUNCH_CHECK();
     c.gate = Gate();
   };
@@ -458,12 +465,12 @@ int induce_t(Ctx& c, int* iters, double* rms) {
       if (!(c.h_scalars[S_EPS0] <= c.aspc_eps0_ref)) {  // worse than ASPC, or not finite: restart
         T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
         T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
+        iter0 = enq;  // before start_cg: the next iteration's r.z slot depends on it
         residual();
         reset_done();
         start_cg();
         ++c.guess_fallback;
         c.last_predictor = 1;
-        iter0 = enq;
         continue;
       }
       ++c.guess_used;
