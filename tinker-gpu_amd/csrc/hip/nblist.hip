@@ -378,6 +378,31 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
     const int ncells = A.brute ? 1 : rs0 * rs1 * rs2;
     const int qa = A.brute ? 0 : part * ncells / kParts, qb = A.brute ? 1 : (part + 1) * ncells / kParts;
     int q = qa - 1, off = 0, total = 0, start = 0;
+    // cell table of this walk: lane t holds the atom count and start of cell
+    // qa + t (count 0 when pruned), loaded at once; the walk then takes the
+    // nonempty cells in order from a ballot, with no memory wait per cell
+    const bool table = !A.brute && qb - qa <= 64;
+    int tcnt = 0, tstart = 0;
+    unsigned long long cmask = 0;
+    if (table) {
+      const int qq = qa + lane;
+      if (qq < qb) {
+        const int x = r00 + qq % rs0, y = r01 + (qq / rs0) % rs1, z = r02 + qq / (rs0 * rs1);
+        bool keep = true;
+        if (prune) {
+          const double gx = fmax(0.0, fmax((double)x - bh0, bl0 - (double)(x + 1))) * A.csize[0];
+          const double gy = fmax(0.0, fmax((double)y - bh1, bl1 - (double)(y + 1))) * A.csize[1];
+          const double gz = fmax(0.0, fmax((double)z - bh2, bl2 - (double)(z + 1))) * A.csize[2];
+          keep = gx * gx + gy * gy + gz * gz <= A.reach2;
+        }
+        if (keep) {
+          const int k = morton3(wrapc(x, A.nc[0]), wrapc(y, A.nc[1]), wrapc(z, A.nc[2]));
+          tcnt = A.cell_count[k];
+          tstart = A.cell_start[k];
+        }
+      }
+      cmask = __ballot(tcnt > 0);
+    }
     bool have = true;
     int j0 = 0, cnt = 0;
     PosRec<real> np = pi, nv = vi;
@@ -389,7 +414,18 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
         off += 64;
       } else {
         have = false;
-        while (++q < qb) {
+        if (table) {
+          if (cmask) {
+            const int t = __builtin_ctzll(cmask);
+            cmask &= cmask - 1;
+            q = qa + t;
+            total = __builtin_amdgcn_readlane(tcnt, t);
+            start = __builtin_amdgcn_readlane(tstart, t);
+            have = true;
+          } else {
+            q = qb;
+          }
+        } else while (++q < qb) {
           if (A.brute) {  // the atom range is split instead of the cells
             start = part * A.n / kParts;
             total = (part + 1) * A.n / kParts - start;
