@@ -148,7 +148,7 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp);
  * torque(vers, gx, gy, gz), src/cu/amoeba/torque.cu:9-380; evdw(vers) for
  * AMOEBA (src/evdw.cpp, src/cu/evdw.cu ehal path); ebond / eangle / eurey
  * (src/cu/evalence.cu). */
-enum { T9_CALC_ENERGY = 1, T9_CALC_GRAD = 2, T9_CALC_VIRIAL = 4 };
+enum { T9_CALC_ENERGY = 1, T9_CALC_GRAD = 2, T9_CALC_VIRIAL = 4, T9_CALC_ANALYZE = 8 };
 typedef struct {
   double esum;          /* sum of the terms below, kcal/mol */
   double em, ep;        /* permanent multipole, polarization */
@@ -157,6 +157,11 @@ typedef struct {
   int induce_iterations;
   double evdw;          /* buffered 14-7 vdW */
   double ebond, eangle, eurey;  /* valence */
+  /* with T9_CALC_ANALYZE (the reference's calc::analyz, energy(calc::v3)):
+   * multipole and polarization interaction counts nem / nep, the pairs
+   * within the cutoff less the excluded ones, plus one self term per atom
+   * with Ewald (src/acc/amoeba/empoleewald.cpp, epolarewald.cpp); else 0 */
+  int64_t nem, nep;
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
 

@@ -83,6 +83,39 @@ def aspc_coefficients(k=14):
     return np.array([(-1) ** (j + 1) * j * comb(2 * k + 4, k + 2 - j) / den for j in range(1, k + 3)])
 
 
+def interaction_counts(a):
+    """Analyze-mode interaction counts (nem, nep) of energy(calc::v3):
+    src/acc/amoeba/empoleewald.cpp:78-79,132-133,177-180 and
+    epolarewald.cpp:128-130,280-282,499-500 (Ewald), empolenonewald.cpp:74-75,
+    159-161 and epolarnonewald.cpp:164-165,313-314 (no Ewald).  Pairs within
+    the cutoff, less the excluded pairs (m scale 0 for nem, p scale 0 for
+    nep); a polarization pair counts only when its energy is nonzero, i.e.
+    when one of the two atoms is polarizable; plus one self term per atom
+    with Ewald.  Brute force over all pairs (minimum image when periodic);
+    a: t9hip.System.arrays()."""
+    x = np.asarray(a["xyz"])
+    n = len(x)
+    ewald = bool(a["ewald"])
+    cut = a["ewald_cut"] if ewald else a["mpole_cut"]
+    d = x[None, :, :] - x[:, None, :]
+    if bool(a["periodic"]):
+        box = np.asarray(a["box"])
+        d -= box * np.round(d / box)
+    iu = np.triu_indices(n, 1)
+    within = (d ** 2).sum(-1)[iu] <= cut * cut
+    pol = np.asarray(a["polarity"]) != 0
+    polpair = (pol[:, None] | pol[None, :])[iu]
+    m0 = np.zeros((n, n), bool)
+    p0 = np.zeros((n, n), bool)
+    for (i, k), sc in zip(a["scale_ik"], a["scale_mdpu"]):
+        m0[i, k] = m0[k, i] = sc[0] == 0
+        p0[i, k] = p0[k, i] = sc[2] == 0
+    self_terms = n if ewald else 0
+    nem = int(np.sum(within & ~m0[iu])) + self_terms if a["use_mpole"] else 0
+    nep = int(np.sum(within & polpair & ~p0[iu])) + self_terms if a["use_polar"] else 0
+    return nem, nep
+
+
 class Oracle:
     """FP64 reference computation for one system (see module docstring)."""
 

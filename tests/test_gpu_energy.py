@@ -160,3 +160,30 @@ def test_energy_gradient_finite_difference():
             fd = (es[0] - es[1]) / (2 * h)
             assert abs(fd - g[atom, d]) < 1e-4 * max(1.0, abs(g[atom, d])), (atom, d, fd, g[atom, d])
     ctx.set_coords(x0)
+
+
+COUNT_GOLDEN = {"lf1": (201, 0), "lf2": (222, 0), "lf3": (0, 201), "lf4": (0, 222), "lf32": (138, 138)}
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+@pytest.mark.parametrize("case", sorted(COUNT_GOLDEN))
+def test_interaction_counts(golden_dir, case, prec):
+    """energy(calc::v3): the reference's interaction counts nem / nep
+    (test/localframe.cpp, test/localframe3.cpp), energy unchanged by the
+    analyze bit."""
+    xyz, key, mpole, polar = CASES[case]
+    s = t9hip.System.from_files(os.path.join(golden_dir, xyz), key=key)
+    ctx = t9hip.Context(s, prec)
+    e, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    print(f"counts {case} prec={prec}: nem {e['nem']} nep {e['nep']}", flush=True)
+    assert (e["nem"], e["nep"]) == COUNT_GOLDEN[case]
+    e0, _, _ = ctx.energy(t9hip.CALC_ENERGY, grad=False)
+    assert e0["esum"] == e["esum"]
+
+
+def test_interaction_counts_water_box():
+    # FP64: an FP32 distance within rounding of the 7 A cutoff could flip a pair
+    s = t9hip.System.water_box(216, 6, 18.643, 2024, key=WATER_KEY.format(K=24, buf=1.0))
+    ctx = t9hip.Context(s, t9hip.T9_FP64)
+    e, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert (e["nem"], e["nep"]) == oracle.interaction_counts(s.arrays())

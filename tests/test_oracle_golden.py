@@ -149,3 +149,24 @@ def test_induce_solves_linear_system(golden_dir):
     ufd, ufp = o.ufield(ud, up)
     res = ud / d["polarity"][:, None] - ufd - fd
     assert np.sqrt(np.mean(np.sum((res * d["polarity"][:, None]) ** 2, axis=1))) * DEBYE < 1e-8
+
+
+# analyze-mode interaction counts (test/localframe.cpp:47,90,131,157,342,519;
+# test/localframe3.cpp:114,141-142)
+COUNT_CASES = [
+    ("Local-Frame-1", "empole -- gas", "local_frame.xyz", "multipoleterm only\n", (201, 0)),
+    ("Local-Frame-2", "empole -- pme", "local_frame.xyz",
+     "multipoleterm only\newald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\n", (222, 0)),
+    ("Local-Frame-3", "various epolar", "local_frame.xyz", LF_KEY3, (0, 201)),
+    ("Local-Frame-4", "various epolar", "local_frame.xyz", LF_KEY4, (0, 222)),
+    ("Local-Frame3-2", "- emplar", "local_frame2.xyz",
+     "cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\newald\n", (138, 138)),
+]
+
+
+@pytest.mark.parametrize("case,section,xyz,key,expect", COUNT_CASES)
+def test_interaction_counts_golden(golden_dir, case, section, xyz, key, expect):
+    ref = int(_get(_ref(golden_dir), case, section, "ref_count"))
+    assert ref in expect
+    s = t9hip.System.from_files(os.path.join(golden_dir, xyz), key=key)
+    assert oracle.interaction_counts(s.arrays()) == expect

@@ -646,10 +646,17 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp) {
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad) {
   return guarded([&] {
     auto& c = ctx->c;
-    if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL)) throw t9::ArgError("unknown vers bits");
+    if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL | T9_CALC_ANALYZE))
+      throw t9::ArgError("unknown vers bits");
     double e[16];
     const int iters = t9::evaluate(c, vers, t9::TERM_ALL, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
-    if (out) t9::fill_energy_out(e, vers, iters, out);
+    long long nem = 0, nep = 0;
+    if (vers & T9_CALC_ANALYZE) t9::count_interactions(c, &nem, &nep);
+    if (out) {
+      t9::fill_energy_out(e, vers, iters, out);
+      out->nem = nem;
+      out->nep = nep;
+    }
     return T9_OK;
   }, T9_ERR_HIP);
 }

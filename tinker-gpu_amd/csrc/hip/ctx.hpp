@@ -314,6 +314,7 @@ double md_kinetic(Ctx& c);                // kcal/mol
 // v -= (h1 g1 + h2 g2) / m, g2 may be null
 void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
+void count_interactions(Ctx& c, long long* nem, long long* nep);
 void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h
 void md_scale_v(Ctx& c, double a);
 // Bussi rescaling with host-drawn deviates r (normal) and s2 (chi-square,

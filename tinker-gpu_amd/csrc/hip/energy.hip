@@ -1133,6 +1133,73 @@ void energy_setup(Ctx& c, const System& s) {
   T9_HIP(hipStreamSynchronize(c.stream));
 }
 
+namespace {
+
+// Interaction counts of the analyze mode over the in-cutoff list, each pair
+// once (j > i): multipole pairs unless their m scale is 0; polarization
+// pairs unless their p scale is 0 or neither atom is polarizable (the pair
+// energy is then exactly zero, which the reference does not count).
+template <class real>
+__global__ __launch_bounds__(256) void k_counts(int n, const int* __restrict__ ncut,
+                                                const long long* __restrict__ woff,
+                                                const uint32_t* __restrict__ ecut,
+                                                const real* __restrict__ polarity, ScaleTable sc,
+                                                unsigned long long* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cm = 0, cp = 0;
+  if (s < n) {
+    const int len = ncut[s];
+    const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
+    const bool pol_i = polarity[s] != 0;
+    for (int k = 0; k < len; ++k) {
+      const uint32_t ent = e[(long long)k * 64];
+      const int j = (int)(ent >> kCodeBits);
+      if (j <= s) continue;
+      const int code = (int)(ent & (kMaxCodes - 1));
+      if (code == 0 || sc.m[code] != 0) ++cm;
+      if ((code == 0 || sc.p[code] != 0) && (pol_i || polarity[j] != 0)) ++cp;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cm += __shfl_xor(cm, o, 64);
+    cp += __shfl_xor(cp, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], cm);
+    atomicAdd(&out[1], cp);
+  }
+}
+
+}  // namespace
+
+// Analyze-mode interaction counts of the multipole and polarization terms
+// (the reference's nem / nep of energy(calc::v3): src/acc/amoeba/
+// empoleewald.cpp:78-79,132-133,177-180 and epolarewald.cpp:128-130,
+// 280-282,499-500): pairs within the cutoff, less excluded (scale 0) pairs,
+// plus one self term per atom with Ewald.
+void count_interactions(Ctx& c, long long* nem, long long* nep) {
+  const auto& L = c.mlist;
+  if (!L.cut_valid) throw ArgError("in-cutoff neighbor list is stale (internal ordering error)");
+  DBuf out;
+  out.alloc(sizeof(unsigned long long) * 2);
+  T9_HIP(hipMemsetAsync(out.p, 0, sizeof(unsigned long long) * 2, c.stream));
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_counts<float>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+                       L.off.as<long long>(), L.ecut.as<uint32_t>(), c.polarity.as<float>(), c.scale,
+                       out.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(k_counts<double>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, L.ncut.as<int>(),
+                       L.off.as<long long>(), L.ecut.as<uint32_t>(), c.polarity.as<double>(), c.scale,
+                       out.as<unsigned long long>());
+  T9_LAUNCH_CHECK();
+  unsigned long long h[2] = {0, 0};
+  T9_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  const long long self = c.ewald ? c.n : 0;
+  *nem = c.opt.use_mpole ? (long long)h[0] + self : 0;
+  *nep = c.opt.use_polar ? (long long)h[1] + self : 0;
+}
+
 void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig) {
   if (c.prec == 32) energy_t<float>(c, vers, terms, out16);
   else energy_t<double>(c, vers, terms, out16);

@@ -33,7 +33,7 @@ EXPORTS = [
     "t9_guess_mlp_grad", "t9_guess_adam", "t9_guess_features", "t9_guess_history",
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
-CALC_ENERGY, CALC_GRAD, CALC_VIRIAL = 1, 2, 4
+CALC_ENERGY, CALC_GRAD, CALC_VIRIAL, CALC_ANALYZE = 1, 2, 4, 8
 
 
 class T9Error(RuntimeError):
@@ -51,7 +51,8 @@ class EnergyOut(ctypes.Structure):
     _fields_ = [("esum", ctypes.c_double), ("em", ctypes.c_double), ("ep", ctypes.c_double),
                 ("em_real", ctypes.c_double), ("em_recip", ctypes.c_double), ("em_self", ctypes.c_double),
                 ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int), ("evdw", ctypes.c_double),
-                ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double)]
+                ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double),
+                ("nem", ctypes.c_int64), ("nep", ctypes.c_int64)]
 
 
 class MdStats(ctypes.Structure):
@@ -333,6 +334,8 @@ class Context:
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
                  eangle=out.eangle, eurey=out.eurey)
+        if vers & CALC_ANALYZE:
+            e.update(nem=out.nem, nep=out.nep)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     # ---- learned initial guess
