@@ -91,6 +91,9 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     throw ArgError("non-Ewald periodic multipole cutoffs (switching taper) are not supported");
   for (int d = 0; d < 3; ++d) c.grid[d] = s.opt.pme_grid[d];
   T9_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  T9_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
   hipStream_t st = c.stream;
   const int n = s.n;
 
@@ -412,12 +415,28 @@ void rpole_to_host(Ctx& c, double* out) {
 // then the requested term groups.  Returns the CG iteration count.
 int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad_host) {
   int iters = 0;
+  if (c.vdw_pending) {  // left over from a call that threw: drain and drop it
+    T9_HIP(hipStreamSynchronize(c.side));
+    c.vdw_pending = false;
+  }
   if (terms & (TERM_ELEC | TERM_VDW)) {
     c.mlist.cut_valid = c.ulist.cut_valid = false;
     c.bins_valid = false;
     c.rpole_valid = false;
     prepare(c);
-    if ((terms & TERM_ELEC) && c.opt.use_polar) {
+    const bool solve = (terms & TERM_ELEC) && c.opt.use_polar;
+    if (solve && (terms & TERM_VDW) && c.use_vdw) {
+      // the vdW pair kernel does not depend on the dipoles: it runs on the
+      // second stream during the solve, into the first energy rows
+      c.erows = 0;
+      double* rows = energy_rows(c, c.nwave);
+      T9_HIP(hipEventRecord(c.ev_fork, c.stream));
+      T9_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+      vdw_launch(c, (vers & 4) != 0, c.side, rows);
+      T9_HIP(hipEventRecord(c.ev_join, c.side));
+      c.vdw_pending = true;
+    }
+    if (solve) {
       double rms = 0;
       if (induce_pcg(c, &iters, &rms) != 0) throw NotConverged("INDUCE -- induced dipoles are not converged");
     }
@@ -457,6 +476,9 @@ int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out) {
       t9::ctx_init(h->c, t9_system_ref(sys), precision);
     } catch (...) {
       if (h->c.stream) (void)hipStreamDestroy(h->c.stream);
+      if (h->c.side) (void)hipStreamDestroy(h->c.side);
+      if (h->c.ev_fork) (void)hipEventDestroy(h->c.ev_fork);
+      if (h->c.ev_join) (void)hipEventDestroy(h->c.ev_join);
       if (h->c.h_scalars) (void)hipHostFree(h->c.h_scalars);
       delete h;
       throw;
@@ -470,12 +492,17 @@ int t9_ctx_destroy(t9_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return T9_OK;
     (void)hipStreamSynchronize(ctx->c.stream);
-    hipStream_t st = ctx->c.stream;
+    (void)hipStreamSynchronize(ctx->c.side);
+    hipStream_t st = ctx->c.stream, side = ctx->c.side;
+    hipEvent_t e1 = ctx->c.ev_fork, e2 = ctx->c.ev_join;
     double* hs = ctx->c.h_scalars;
     t9::ktimer_release(ctx->c);
     delete ctx;
     if (hs) (void)hipHostFree(hs);
     if (st) (void)hipStreamDestroy(st);
+    if (side) (void)hipStreamDestroy(side);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
     return T9_OK;
   }, T9_ERR_HIP);
 }

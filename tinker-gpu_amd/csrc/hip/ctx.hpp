@@ -103,6 +103,11 @@ struct Ctx {
   KTimer timers[KT_COUNT];
   int prec = 32;
   hipStream_t stream = nullptr;
+  // second stream (default priority): the vdW pair kernel runs there while
+  // the induced dipoles are solved on `stream`; joined by events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool vdw_pending = false;  // vdW pair kernel enqueued on `side`, not yet joined
   Options opt;
   double box[3] = {0, 0, 0};
   bool periodic = false;
@@ -298,6 +303,8 @@ void vdw_sites(Ctx& c);                   // vdW site positions (sorted) from th
 // vdW (sorted egrad += ..) and valence (caller-order gval) terms of one
 // evaluation; partial rows reserved from the energy rows
 void vdw_run(Ctx& c, bool virial);
+void vdw_launch(Ctx& c, bool virial, hipStream_t st, double* rows);  // pair kernel only
+void vdw_finish(Ctx& c);                                             // site gradients onto atoms
 void valence_run(Ctx& c, double* gval_caller, bool energy);
 // Term groups of one evaluation: the electrostatics (multipole + polarization)
 // and vdW are the slow group of the RESPA split, valence the fast group.
@@ -351,7 +358,8 @@ struct KScope {
   KTimer* t;
   hipStream_t st;
   hipEvent_t stop = nullptr;
-  KScope(Ctx& c, int id, int launches = 1) : t(c.timers[id].on ? &c.timers[id] : nullptr), st(c.stream) {
+  KScope(Ctx& c, int id, int launches = 1, hipStream_t on = nullptr)
+      : t(c.timers[id].on ? &c.timers[id] : nullptr), st(on ? on : c.stream) {
     if (!t) return;
     if (t->used + 2 > t->ev.size()) {
       hipEvent_t a, b;

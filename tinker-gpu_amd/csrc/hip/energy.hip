@@ -1044,7 +1044,7 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   const bool use_m = c.opt.use_mpole && (terms & TERM_ELEC), use_p = c.opt.use_polar && (terms & TERM_ELEC);
   const bool want_v = (vers & 4) != 0;
   const int n = c.n, nb = ceil_div(n, 256);
-  c.erows = 0;
+  if (!c.vdw_pending) c.erows = 0;  // else the rows after the pending vdW kernel's
   if (use_m || use_p) {
     real_space_t<real>(c, use_m, use_p);
     if (c.ewald) recip_t<real>(c, use_m, use_p, want_v);
@@ -1065,7 +1065,13 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   } else {
     T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
   }
-  if ((terms & TERM_VDW) && c.use_vdw) vdw_run(c, want_v);
+  if (c.vdw_pending) {  // join the vdW pair kernel of the second stream
+    T9_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    vdw_finish(c);
+    c.vdw_pending = false;
+  } else if ((terms & TERM_VDW) && c.use_vdw) {
+    vdw_run(c, want_v);
+  }
   if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>(), true);
   // without CALC_ENERGY / CALC_VIRIAL nothing comes back to the host (no sync)
   if (c.erows == 0 || !(vers & 5)) {

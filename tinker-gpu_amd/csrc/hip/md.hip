@@ -444,7 +444,7 @@ void vdw_sites_t(Ctx& c) {
 }
 
 template <class real>
-void vdw_run_t(Ctx& c, bool vir) {
+void vdw_launch_t(Ctx& c, bool vir, hipStream_t st, double* rows) {
   VdwArgs<real> A;
   A.n = c.n;
   A.nlist = c.vlist.nlist.as<int>();
@@ -461,17 +461,10 @@ void vdw_run_t(Ctx& c, bool vir) {
   A.cut2 = (real)(c.vdw_cut * c.vdw_cut);
   A.taper = (real)c.vdw_taper;
   A.gsite = c.vsite_grad.as<double>();
-  const int nb = ceil_div(c.n, 256);
-  A.rows = energy_rows(c, c.nwave);
-  {
-    KScope ks_(c, KT_VDW);
-    if (vir) hipLaunchKernelGGL((k_vdw<real, true>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
-    else hipLaunchKernelGGL((k_vdw<real, false>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A);
-    T9_LAUNCH_CHECK();
-  }
-  hipLaunchKernelGGL(k_vdw_to_atoms, dim3(nb), dim3(256), 0, c.stream, c.n, c.perm.as<int>(), c.iperm.as<int>(),
-                     c.vparent.as<int>(), c.vred.as<double>(), c.vchild_off.as<int>(), c.vchild.as<int>(),
-                     c.vsite_grad.as<double>(), c.egrad.as<double>());
+  A.rows = rows;
+  KScope ks_(c, KT_VDW, 1, st);
+  if (vir) hipLaunchKernelGGL((k_vdw<real, true>), dim3(c.nwave), dim3(64 * kPairParts), 0, st, A);
+  else hipLaunchKernelGGL((k_vdw<real, false>), dim3(c.nwave), dim3(64 * kPairParts), 0, st, A);
   T9_LAUNCH_CHECK();
 }
 
@@ -482,10 +475,24 @@ void vdw_sites(Ctx& c) {
   else vdw_sites_t<double>(c);
 }
 
+void vdw_launch(Ctx& c, bool vir, hipStream_t st, double* rows) {
+  if (c.prec == 32) vdw_launch_t<float>(c, vir, st, rows);
+  else vdw_launch_t<double>(c, vir, st, rows);
+}
+
+// per-site gradients (reduced hydrogen sites) onto the atoms, into the
+// energy-path gradient
+void vdw_finish(Ctx& c) {
+  hipLaunchKernelGGL(k_vdw_to_atoms, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
+                     c.iperm.as<int>(), c.vparent.as<int>(), c.vred.as<double>(), c.vchild_off.as<int>(),
+                     c.vchild.as<int>(), c.vsite_grad.as<double>(), c.egrad.as<double>());
+  T9_LAUNCH_CHECK();
+}
+
 void vdw_run(Ctx& c, bool vir) {
   if (!c.use_vdw) return;
-  if (c.prec == 32) vdw_run_t<float>(c, vir);
-  else vdw_run_t<double>(c, vir);
+  vdw_launch(c, vir, c.stream, energy_rows(c, c.nwave));
+  vdw_finish(c);
 }
 
 void valence_run(Ctx& c, double* gval, bool energy) {
