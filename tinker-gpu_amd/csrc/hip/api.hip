@@ -94,6 +94,9 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   T9_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
   T9_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
   T9_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  T9_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_ufork, hipEventDisableTiming));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin, hipEventDisableTiming));
   hipStream_t st = c.stream;
   const int n = s.n;
 
@@ -479,6 +482,9 @@ int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out) {
       if (h->c.side) (void)hipStreamDestroy(h->c.side);
       if (h->c.ev_fork) (void)hipEventDestroy(h->c.ev_fork);
       if (h->c.ev_join) (void)hipEventDestroy(h->c.ev_join);
+      if (h->c.side2) (void)hipStreamDestroy(h->c.side2);
+      if (h->c.ev_ufork) (void)hipEventDestroy(h->c.ev_ufork);
+      if (h->c.ev_ujoin) (void)hipEventDestroy(h->c.ev_ujoin);
       if (h->c.h_scalars) (void)hipHostFree(h->c.h_scalars);
       delete h;
       throw;
@@ -493,8 +499,9 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     if (!ctx) return T9_OK;
     (void)hipStreamSynchronize(ctx->c.stream);
     (void)hipStreamSynchronize(ctx->c.side);
-    hipStream_t st = ctx->c.stream, side = ctx->c.side;
-    hipEvent_t e1 = ctx->c.ev_fork, e2 = ctx->c.ev_join;
+    (void)hipStreamSynchronize(ctx->c.side2);
+    hipStream_t st = ctx->c.stream, side = ctx->c.side, side2 = ctx->c.side2;
+    hipEvent_t e1 = ctx->c.ev_fork, e2 = ctx->c.ev_join, e3 = ctx->c.ev_ufork, e4 = ctx->c.ev_ujoin;
     double* hs = ctx->c.h_scalars;
     t9::ktimer_release(ctx->c);
     delete ctx;
@@ -503,6 +510,9 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     if (side) (void)hipStreamDestroy(side);
     if (e1) (void)hipEventDestroy(e1);
     if (e2) (void)hipEventDestroy(e2);
+    if (side2) (void)hipStreamDestroy(side2);
+    if (e3) (void)hipEventDestroy(e3);
+    if (e4) (void)hipEventDestroy(e4);
     return T9_OK;
   }, T9_ERR_HIP);
 }

@@ -107,6 +107,10 @@ struct Ctx {
   // the induced dipoles are solved on `stream`; joined by events
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // third stream (default priority): the real-space mutual field runs there
+  // while the PME part of the same field spreads and transforms
+  hipStream_t side2 = nullptr;
+  hipEvent_t ev_ufork = nullptr, ev_ujoin = nullptr;
   bool vdw_pending = false;  // vdW pair kernel enqueued on `side`, not yet joined
   Options opt;
   double box[3] = {0, 0, 0};
@@ -267,7 +271,7 @@ void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp);  // sorted real
 void pme_bin(Ctx& c);
 void pme_fft_roundtrip(Ctx& c);           // forward FFT, influence function, inverse FFT of c.qgrid
 void pme_check_fft_sizes(const Ctx& c);
-void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp);  // adds recip+self
+void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();

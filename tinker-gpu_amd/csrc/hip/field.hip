@@ -345,13 +345,13 @@ void require_cut(const NbList& L) {
 }
 
 template <class real>
-void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
+void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipStream_t st) {
   const auto& L = c.mlist;
   require_cut(L);
   auto kern = c.ewald ? (c.uniform_thole ? k_ufield_real<real, 1, 1> : k_ufield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_ufield_real<real, 0, 1> : k_ufield_real<real, 0, 0>);
-  KScope ks_(c, KT_UFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, L.ncut.as<int>(),
+  KScope ks_(c, KT_UFIELD_REAL, 1, st);
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
                      (const real*)up, c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale,
                      make_boxT<real>(c), ewald_k<real>(c), (real*)fd, (real*)fp, c.gate);
@@ -396,9 +396,19 @@ void dfield_all(Ctx& c) {
 }
 
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {
-  if (c.prec == 32) ufield_real_t<float>(c, ud, up, fd, fp);
-  else ufield_real_t<double>(c, ud, up, fd, fp);
-  if (c.ewald) pme_dipole_field(c, ud, up, fd, fp);
+  if (!c.ewald) {
+    if (c.prec == 32) ufield_real_t<float>(c, ud, up, fd, fp, c.stream);
+    else ufield_real_t<double>(c, ud, up, fd, fp, c.stream);
+    return;
+  }
+  // real space on the third stream (it writes fd, fp) while the PME part
+  // spreads and transforms; the gather (fd, fp +=) waits for it
+  T9_HIP(hipEventRecord(c.ev_ufork, c.stream));
+  T9_HIP(hipStreamWaitEvent(c.side2, c.ev_ufork, 0));
+  if (c.prec == 32) ufield_real_t<float>(c, ud, up, fd, fp, c.side2);
+  else ufield_real_t<double>(c, ud, up, fd, fp, c.side2);
+  T9_HIP(hipEventRecord(c.ev_ujoin, c.side2));
+  pme_dipole_field(c, ud, up, fd, fp, c.ev_ujoin);
 }
 
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {

@@ -531,10 +531,11 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
 }
 
 template <class real>
-void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
+void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 0, ud, up);
   pme_fft_roundtrip(c);
+  if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
@@ -581,9 +582,9 @@ void pme_spread(Ctx& c, int mode, const void* s1, const void* s2) {
   }
 }
 
-void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp) {
-  if (c.prec == 32) dipole_field_t<float>(c, ud, up, fd, fp);
-  else dipole_field_t<double>(c, ud, up, fd, fp);
+void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather) {
+  if (c.prec == 32) dipole_field_t<float>(c, ud, up, fd, fp, before_gather);
+  else dipole_field_t<double>(c, ud, up, fd, fp, before_gather);
 }
 
 void pme_mpole_field(Ctx& c, void* fd, void* fp) {
