@@ -272,7 +272,7 @@ void pme_bin(Ctx& c);
 void pme_fft_roundtrip(Ctx& c);           // forward FFT, influence function, inverse FFT of c.qgrid
 void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self
-void pme_mpole_field(Ctx& c, void* fd, void* fp);  // adds recip+self of the permanent multipoles
+void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self of the permanent multipoles
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();
 // learned guess (guess.hip): nparam of the network, allocation, per-solve hooks

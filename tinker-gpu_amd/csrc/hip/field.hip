@@ -359,13 +359,13 @@ void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, h
 }
 
 template <class real>
-void dfield_real_t(Ctx& c) {
+void dfield_real_t(Ctx& c, hipStream_t st) {
   const auto& L = c.mlist;
   require_cut(L);
   auto kern = c.ewald ? (c.uniform_thole ? k_dfield_real<real, 1, 1> : k_dfield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_dfield_real<real, 0, 1> : k_dfield_real<real, 0, 0>);
-  KScope ks_(c, KT_DFIELD_REAL);
-  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n, L.ncut.as<int>(),
+  KScope ks_(c, KT_DFIELD_REAL, 1, st);
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n, L.ncut.as<int>(),
                      L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
                      c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, make_boxT<real>(c),
                      ewald_k<real>(c), c.field.as<real>(), c.fieldp.as<real>());
@@ -390,9 +390,19 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
 }  // namespace
 
 void dfield_all(Ctx& c) {
-  if (c.prec == 32) dfield_real_t<float>(c);
-  else dfield_real_t<double>(c);
-  if (c.ewald) pme_mpole_field(c, c.field.p, c.fieldp.p);
+  if (!c.ewald) {
+    if (c.prec == 32) dfield_real_t<float>(c, c.stream);
+    else dfield_real_t<double>(c, c.stream);
+    return;
+  }
+  // real space on the third stream beside the PME spreading and FFT; the
+  // PME gather (+=) waits for it
+  T9_HIP(hipEventRecord(c.ev_ufork, c.stream));
+  T9_HIP(hipStreamWaitEvent(c.side2, c.ev_ufork, 0));
+  if (c.prec == 32) dfield_real_t<float>(c, c.side2);
+  else dfield_real_t<double>(c, c.side2);
+  T9_HIP(hipEventRecord(c.ev_ujoin, c.side2));
+  pme_mpole_field(c, c.field.p, c.fieldp.p, c.ev_ujoin);
 }
 
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {

@@ -546,7 +546,7 @@ void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, 
 }
 
 template <class real>
-void mpole_field_t(Ctx& c, void* fd, void* fp) {
+void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 1, c.rpole.p, nullptr);
   pme_fft_roundtrip(c);
@@ -558,6 +558,7 @@ void mpole_field_t(Ctx& c, void* fd, void* fp) {
   c.qgrid_m_stamp = c.rpole_stamp;
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
+  if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   KScope ks_(c, KT_GATHER);
   hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
                      c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
@@ -587,9 +588,9 @@ void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp
   else dipole_field_t<double>(c, ud, up, fd, fp, before_gather);
 }
 
-void pme_mpole_field(Ctx& c, void* fd, void* fp) {
-  if (c.prec == 32) mpole_field_t<float>(c, fd, fp);
-  else mpole_field_t<double>(c, fd, fp);
+void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
+  if (c.prec == 32) mpole_field_t<float>(c, fd, fp, before_gather);
+  else mpole_field_t<double>(c, fd, fp, before_gather);
 }
 
 // B-spline moduli of one axis (the dftmod rule, with the zeta correction),
