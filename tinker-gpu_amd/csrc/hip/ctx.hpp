@@ -46,6 +46,10 @@ struct DBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
 };
 
 // Neighbor list in "wave slabs": atoms are grouped 64 to a wavefront in
@@ -292,7 +296,8 @@ void adam_step(Ctx& c, const float* grad_dev, double lr);
 void guess_capture_start(Ctx& c, int stride, long long max_samples);
 void guess_fit_norm(Ctx& c);
 void guess_train(Ctx& c, int epochs, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
-void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp);
+void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d = nullptr,
+                   void* z2p = nullptr);  // z (and a second copy of z when given)
 // spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
 // 2: multipoles src1 (real) + dipoles src2 (imaginary)
 void pme_spread(Ctx& c, int mode, const void* s1, const void* s2);

@@ -262,7 +262,8 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
                                                  const int* __restrict__ jpolar, const real* __restrict__ thlval,
                                                  int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                  const real* __restrict__ rd, const real* __restrict__ rp,
-                                                 real* __restrict__ zd, real* __restrict__ zp, Gate gate) {
+                                                 real* __restrict__ zd, real* __restrict__ zp,
+                                                 real* __restrict__ z2d, real* __restrict__ z2p, Gate gate) {
   if (gate_closed(gate)) return;
   __shared__ double sh[(kPairParts - 1) * 6 * 64];
   const int part = threadIdx.x >> 6;
@@ -325,6 +326,14 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
     zp[3 * s] = (real)b0;
     zp[3 * s + 1] = (real)b1;
     zp[3 * s + 2] = (real)b2;
+    if (z2d) {  // second copy (the first CG search direction)
+      z2d[3 * s] = (real)a0;
+      z2d[3 * s + 1] = (real)a1;
+      z2d[3 * s + 2] = (real)a2;
+      z2p[3 * s] = (real)b0;
+      z2p[3 * s + 1] = (real)b1;
+      z2p[3 * s + 2] = (real)b2;
+    }
   }
 }
 
@@ -373,7 +382,7 @@ void dfield_real_t(Ctx& c, hipStream_t st) {
 }
 
 template <class real>
-void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
+void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p) {
   const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.mlist.cut2nd > 0;
   const auto& L = c.mlist;
   if (sparse) require_cut(L);
@@ -383,7 +392,7 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
                      (real)c.opt.udiag, L.ncut2.as<int>(), L.off.as<long long>(), L.ecut2.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
                      (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
-                     (real*)zp, c.gate);
+                     (real*)zp, (real*)z2d, (real*)z2p, c.gate);
   T9_LAUNCH_CHECK();
 }
 
@@ -421,9 +430,9 @@ void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {
   pme_dipole_field(c, ud, up, fd, fp, c.ev_ujoin);
 }
 
-void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp) {
-  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp);
-  else precond_t<double>(c, rd, rp, zd, zp);
+void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p) {
+  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp, z2d, z2p);
+  else precond_t<double>(c, rd, rp, zd, zp, z2d, z2p);
 }
 
 }  // namespace t9

@@ -43,7 +43,7 @@ struct Coef16 {
 template <class real>
 __global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict__ iperm, Coef16 cf,
                                                   const real* __restrict__ hist, real* __restrict__ ud,
-                                                  real* __restrict__ up) {
+                                                  real* __restrict__ up, real* __restrict__ keep) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int s = iperm[i];
@@ -56,6 +56,10 @@ __global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict_
     }
     ud[3 * s + d] = (real)a;
     up[3 * s + d] = (real)b;
+    if (keep) {  // a copy of the ASPC guess (d, then p) for the learned correction's fallback
+      keep[3 * s + d] = (real)a;
+      keep[3 * (size_t)n + 3 * s + d] = (real)b;
+    }
   }
 }
 
@@ -379,11 +383,10 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     const auto B = aspc_coefficients();
     Coef16 cf;
     for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
-    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up);
+    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up,
+                       feats ? ua : nullptr);
     T9_LAUNCH_CHECK();
     if (feats) {
-      T9_HIP(hipMemcpyAsync(ua, ud, vbytes, hipMemcpyDeviceToDevice, st));
-      T9_HIP(hipMemcpyAsync(ua + 3 * (size_t)n, up, vbytes, hipMemcpyDeviceToDevice, st));
       double ce[kMaxUalt], cu[2];
       for (int k = 0; k < kMaxUalt; ++k) ce[k] = B[((c.nealt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
       cu[0] = slot_of_age(c.nualt, 0);
@@ -419,9 +422,7 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   // the residual norm of the initial guess (Debye, read back with the first
   // iteration)
   auto start_cg = [&]() {
-    precond_apply(c, rd, rp, zd, zp);
-    T9_HIP(hipMemcpyAsync(cd, zd, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
-    T9_HIP(hipMemcpyAsync(cp, zp, sizeof(real) * 3 * n, hipMemcpyDeviceToDevice, st));
+    precond_apply(c, rd, rp, zd, zp, cd, cp);  // z, and the first search direction p = z
     hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, zd, rp, zp, rd, rd, rp, rp, part, Gate());
     hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk, 1, (iter0 + 1) & 1, n, debye, part, sc, trace);
     T9_LAUNCH_CHECK();

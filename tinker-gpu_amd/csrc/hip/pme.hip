@@ -551,17 +551,18 @@ void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   spread_t<real>(c, 1, c.rpole.p, nullptr);
   pme_fft_roundtrip(c);
   // keep the convolved grid: the reciprocal multipole energy of the same
-  // coordinates gathers from it instead of spreading and transforming again
-  const size_t gbytes = 2 * sizeof(real) * (size_t)c.grid[0] * c.grid[1] * c.grid[2];
-  c.qgrid_m.alloc(gbytes);
-  T9_HIP(hipMemcpyAsync(c.qgrid_m.p, c.qgrid.p, gbytes, hipMemcpyDeviceToDevice, c.stream));
+  // coordinates gathers from it instead of spreading and transforming again.
+  // The two grid buffers trade places (no copy); the dipole round trips use
+  // the other one.
+  c.qgrid_m.alloc(c.qgrid.bytes);
+  c.qgrid.swap(c.qgrid_m);
   c.qgrid_m_stamp = c.rpole_stamp;
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   KScope ks_(c, KT_GATHER);
   hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
-                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
+                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
                      c.rpole.as<real>(), (real*)fd, (real*)fp);
   T9_LAUNCH_CHECK();
 }
