@@ -70,6 +70,15 @@ int t9_system_get_scales(const t9_system* sys, int* ik, double* mdpu);
 int t9_system_set_xyz(t9_system* sys, const double* xyz);
 /* Override a keyword after setup (same names as t9_system_get_scalar). */
 int t9_system_set_scalar(t9_system* sys, const char* name, double value);
+/* One frame in Tinker's .xyz / dump layout (test/ref/tinkernist.arc, .uind,
+ * .udir, .def, .tef; the reference writes them through Tinker's Fortran
+ * dump routine, src/mdsave.cpp:41-215, not in the reference tree): header
+ * "%6d  title", the box line when periodic, then per atom index, name (3
+ * columns), three values in f12.6, type and the 1-2 connected atoms.
+ * values [n][3] in caller order: coordinates (.arc, NULL = the system's own),
+ * or induced dipoles in Debye (.uind), fields (.def / .tef), ...
+ * append != 0 adds the frame to an existing file (an archive). */
+int t9_system_write_frame(const t9_system* sys, const char* path, const double* values, int append);
 
 /* ---------------------------------------------------------------- context */
 typedef struct t9_ctx t9_ctx;

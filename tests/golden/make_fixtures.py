@@ -11,6 +11,10 @@ What it writes (numbers and data files only, no reference source text):
   * tests/golden/tinkernist.npz — test/ref/tinkernist.{arc,uind,udir,def,tef}:
     2 frames x 2,684 atoms (coordinates Å, dipoles Debye, fields as dumped),
     plus types and 1-2 connectivity of test/file/tinkernist/water30.xyz
+  * tests/golden/tinkernist_text.json — the dump layout of the same files: the
+    title and atom names of water30.xyz and, of the first frame of
+    tinkernist.arc and tinkernist.uind, the header, box and first two atom
+    lines and the last atom line (text as written by Tinker's dump routine)
   * tests/golden/aspc_reference.json — the ASPC predictor coefficients of
     src/cu/upredict.cu:50-65 (16 numbers)
   * tests/golden/nacl_vdw_ref.json — test/nacl.cpp NaCl-1 vdW energies,
@@ -169,6 +173,19 @@ def write_tinkernist():
     np.savez_compressed(os.path.join(HERE, "tinkernist.npz"), types=types, i12=i12, **d)
 
 
+def write_tinkernist_text():
+    lines = open(os.path.join(REF, "test/file/tinkernist/water30.xyz")).read().splitlines()
+    n = int(lines[0].split()[0])
+    title = lines[0].split(None, 1)[1].strip()
+    names = [ln.split()[1] for ln in lines[2:2 + n]]
+    out = {"title": title, "names": names}
+    for k in ("arc", "uind"):
+        f = open(os.path.join(REF, "test/ref/tinkernist." + k)).read().splitlines()
+        out[k] = f[:4] + [f[1 + n]]
+    json.dump(out, open(os.path.join(HERE, "tinkernist_text.json"), "w"), indent=0)
+    return out[k][:2]
+
+
 def write_aspc():
     """The reference's ASPC predictor table (src/cu/upredict.cu:50-65) as
     numbers: {"aspc": [16 floats]}, most recent solution first."""
@@ -208,6 +225,7 @@ def main():
     json.dump(ref, open(os.path.join(HERE, "localframe_ref.json"), "w"), indent=0, sort_keys=True)
     print("golden values:", len(ref))
     write_tinkernist()
+    print("tinkernist text:", write_tinkernist_text())
     print("aspc table:", write_aspc())
     print("nacl:", write_nacl())
 

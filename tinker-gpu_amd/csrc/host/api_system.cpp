@@ -1,5 +1,6 @@
 // C ABI for the host-side system (t9_system_*): setup from Tinker files or
 // the seeded water-box generator, and read access to the derived arrays.
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -218,6 +219,30 @@ int t9_system_get_scales(const t9_system* sys, int* ik, double* mdpu) {
 int t9_system_set_xyz(t9_system* sys, const double* xyz) {
   return t9::guarded([&] {
     std::memcpy(sys->s.xyz.data(), xyz, sys->s.xyz.size() * sizeof(double));
+    return T9_OK;
+  }, T9_ERR_ARG);
+}
+
+int t9_system_write_frame(const t9_system* sys, const char* path, const double* values, int append) {
+  return t9::guarded([&] {
+    if (!sys || !path) throw t9::ArgError("null argument");
+    const t9::System& s = sys->s;
+    const double* v = values ? values : s.xyz.data();
+    std::FILE* f = std::fopen(path, append ? "a" : "w");
+    if (!f) throw t9::ArgError(std::string("cannot open ") + path);
+    std::fprintf(f, "%6d  %s\n", s.n, s.title.c_str());
+    if (s.periodic)
+      std::fprintf(f, " %12.6f%12.6f%12.6f%12.6f%12.6f%12.6f\n", s.box[0], s.box[1], s.box[2], 90.0, 90.0, 90.0);
+    for (int i = 0; i < s.n; ++i) {
+      const std::string name = s.name[i].substr(0, 3);
+      std::fprintf(f, "%6d  %-3s%12.6f%12.6f%12.6f%6d", i + 1, name.c_str(), v[3 * i], v[3 * i + 1], v[3 * i + 2],
+                   s.type[i]);
+      for (int j : s.i12[i]) std::fprintf(f, "%6d", j + 1);
+      std::fprintf(f, "\n");
+    }
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    if (!ok) throw t9::ArgError(std::string("write failed: ") + path);
     return T9_OK;
   }, T9_ERR_ARG);
 }

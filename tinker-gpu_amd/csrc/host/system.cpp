@@ -224,7 +224,13 @@ void read_xyz(const std::string& path, System& s) {
   std::getline(in, line);
   auto tok = split(line);
   s.n = std::stoi(tok.at(0));
-  s.title = line;
+  {  // the title: the header after the atom count
+    const size_t a = line.find_first_not_of(" \t");
+    const size_t b = a == std::string::npos ? a : line.find_first_of(" \t", a);
+    const size_t c = b == std::string::npos ? b : line.find_first_not_of(" \t", b);
+    s.title = c == std::string::npos ? std::string() : line.substr(c);
+    while (!s.title.empty() && (s.title.back() == ' ' || s.title.back() == '\r')) s.title.pop_back();
+  }
   s.xyz.assign(3 * s.n, 0.0);
   s.type.assign(s.n, 0);
   s.name.assign(s.n, "");

@@ -24,7 +24,7 @@ T9_FP32, T9_FP64 = 32, 64
 EXPORTS = [
     "t9_last_error", "t9_system_from_files", "t9_system_water_box", "t9_system_destroy", "t9_system_natoms",
     "t9_system_get_f64", "t9_system_get_i32", "t9_system_get_scalar", "t9_system_get_scales",
-    "t9_system_set_xyz", "t9_system_set_scalar", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
+    "t9_system_set_xyz", "t9_system_set_scalar", "t9_system_write_frame", "t9_ctx_create", "t9_ctx_destroy", "t9_set_coords",
     "t9_get_coords", "t9_nblist_refresh", "t9_nblist_build", "t9_nblist_stats", "t9_mpole_init",
     "t9_get_rpole", "t9_dfield", "t9_ufield", "t9_induce", "t9_induce_trace", "t9_get_uind", "t9_set_uind", "t9_sync",
     "t9_kernel_timing", "t9_cutoff_pairs", "t9_copy_probe", "t9_energy", "t9_md_init", "t9_md_set_velocities",
@@ -87,6 +87,7 @@ def lib():
         L.t9_system_get_scales.argtypes = [_vp, _i, _d]
         L.t9_system_set_xyz.argtypes = [_vp, _d]
         L.t9_system_set_scalar.argtypes = [_vp, ctypes.c_char_p, ctypes.c_double]
+        L.t9_system_write_frame.argtypes = [_vp, ctypes.c_char_p, _d, ctypes.c_int]
         for name, args in [
             ("t9_ctx_create", [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
             ("t9_ctx_destroy", [_vp]),
@@ -187,6 +188,13 @@ class System:
 
     def set_scalar(self, name, value):
         check(lib().t9_system_set_scalar(self.h, name.encode(), float(value)))
+
+    def write_frame(self, path, values=None, append=False):
+        """One frame in Tinker's .xyz / dump layout (values [n,3] in caller
+        order: coordinates by default, or dipoles in Debye, fields, ...)."""
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().t9_system_write_frame(self.h, str(path).encode(), _p(v) if v is not None else None,
+                                          1 if append else 0))
 
     def set_xyz(self, xyz):
         a = np.ascontiguousarray(xyz, dtype=np.float64)
