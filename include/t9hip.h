@@ -169,8 +169,9 @@ typedef struct {
   /* with T9_CALC_ANALYZE (the reference's calc::analyz, energy(calc::v3)):
    * multipole and polarization interaction counts nem / nep, the pairs
    * within the cutoff less the excluded ones, plus one self term per atom
-   * with Ewald (src/acc/amoeba/empoleewald.cpp, epolarewald.cpp); else 0 */
-  int64_t nem, nep;
+   * with Ewald (src/acc/amoeba/empoleewald.cpp, epolarewald.cpp), and the
+   * vdW count nev (src/acc/ehal.cpp); else 0 */
+  int64_t nem, nep, nev;
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
 

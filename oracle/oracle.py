@@ -116,6 +116,31 @@ def interaction_counts(a):
     return nem, nep
 
 
+def vdw_count(a):
+    """Analyze-mode vdW count nev (src/acc/ehal.cpp:127,215-216): site pairs
+    within the vdW cutoff less the pairs with vdW scale 0.  Sites: reduced
+    hydrogens sit at x_p + red (x_i - x_p).  Brute force over all pairs."""
+    if not a["use_vdw"]:
+        return 0
+    x = np.asarray(a["xyz"])
+    n = len(x)
+    par = np.asarray(a["vdw_parent"])
+    red = np.asarray(a["vdw_red"])
+    site = x.copy()
+    m = par != np.arange(n)
+    site[m] = x[par[m]] + red[m, None] * (x[m] - x[par[m]])
+    d = site[None, :, :] - site[:, None, :]
+    if bool(a["periodic"]):
+        box = np.asarray(a["box"])
+        d -= box * np.round(d / box)
+    iu = np.triu_indices(n, 1)
+    within = (d ** 2).sum(-1)[iu] <= a["vdw_cut"] ** 2
+    z = np.zeros((n, n), bool)
+    for (i, k), sv in zip(a["vscale_ik"], a["vscale_s"]):
+        z[i, k] = z[k, i] = sv == 0
+    return int(np.sum(within & ~z[iu]))
+
+
 class Oracle:
     """FP64 reference computation for one system (see module docstring)."""
 

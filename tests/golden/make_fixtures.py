@@ -207,7 +207,8 @@ def write_nacl():
         lines = open(os.path.join(REF, "test/file/nacl", f)).read().splitlines()
         r = float(lines[2].split()[2])
         pre = f"NaCl-1/- ehal -- {sec}/"
-        out[sec] = dict(r=r, eng=d[pre + "ref_eng"], grad=d[pre + "ref_grad"], vir=d[pre + "ref_v"])
+        out[sec] = dict(r=r, eng=d[pre + "ref_eng"], grad=d[pre + "ref_grad"], vir=d[pre + "ref_v"],
+                        count=d[pre + "ref_count"])
     json.dump(out, open(os.path.join(HERE, "nacl_vdw_ref.json"), "w"), indent=1)
     return out
 

@@ -141,3 +141,31 @@ def test_bussi_thermostat_heats_cold_box():
     print(f"Bussi: T after 40 steps = {st['temperature']:.1f} K (bath 298)", flush=True)
     assert 200.0 < st["temperature"] < 400.0
     assert st["iterations"] > 0
+
+
+NACL_KEY = "a-axis 18.643\nneighbor-list\nlist-buffer 0.1\nvdw-cutoff 2.6\nmpole-cutoff 5.0\nvdwterm only\n"
+
+
+@pytest.mark.parametrize("section", ["no switch", "switch, near cut", "switch, near off"])
+def test_vdw_count_nacl_golden(golden_dir, tmp_path, section):
+    """test/nacl.cpp NaCl-1 through the C ABI: vdW energy and gradient at the
+    reference's margin 1e-3, and the energy(calc::v3) count nev."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(golden_dir, "nacl_vdw_ref.json")))[section]
+    f = tmp_path / "nacl.xyz"
+    f.write_text(f"2 NaCl\n1 Na+ 0.0 0.0 0.0 7\n2 Cl- {ref['r']:.6f} 0.0 0.0 15\n")
+    s = t9hip.System.from_files(str(f), key=NACL_KEY)
+    for prec in (t9hip.T9_FP64, t9hip.T9_FP32):
+        ctx = t9hip.Context(s, prec)
+        e, g, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_GRAD | t9hip.CALC_ANALYZE)
+        assert e["nev"] == int(ref["count"])
+        assert abs(e["evdw"] - ref["eng"]) < 1e-3
+        np.testing.assert_allclose(g, np.array(ref["grad"]), atol=1e-3)
+
+
+def test_vdw_count_water_box():
+    s = _box(extra="vdwterm\n")
+    ctx = t9hip.Context(s, t9hip.T9_FP64)  # FP64: no pair within rounding of the cutoff
+    e, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert e["nev"] == oracle.vdw_count(s.arrays())

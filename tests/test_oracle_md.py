@@ -29,6 +29,7 @@ def test_nacl_vdw_golden(golden_dir, tmp_path, section):
     assert abs(e["evdw"] - ref["eng"]) < eps
     np.testing.assert_allclose(g, np.array(ref["grad"]), atol=eps)
     np.testing.assert_allclose(v, np.array(ref["vir"]), atol=eps)
+    assert oracle.vdw_count(s.arrays()) == int(ref["count"])  # test/nacl.cpp ref_count
 
 
 def _water():

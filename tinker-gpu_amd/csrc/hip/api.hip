@@ -84,10 +84,11 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   c.ewald = s.opt.ewald;
   c.aewald = s.opt.ewald_alpha;
   c.cut = c.ewald ? s.opt.ewald_cut : s.opt.mpole_cut;
-  if (c.periodic && !c.ewald && c.cut > 1e20)
+  const bool elec = s.opt.use_mpole || s.opt.use_polar;
+  if (elec && c.periodic && !c.ewald && c.cut > 1e20)
     throw ArgError("periodic systems need ewald or a finite mpole-cutoff");
   if (!c.periodic && c.ewald) throw ArgError("ewald requires a periodic box");
-  if (c.periodic && c.cut < 1e20 && !c.ewald)
+  if (elec && c.periodic && c.cut < 1e20 && !c.ewald)
     throw ArgError("non-Ewald periodic multipole cutoffs (switching taper) are not supported");
   for (int d = 0; d < 3; ++d) c.grid[d] = s.opt.pme_grid[d];
   T9_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -687,12 +688,16 @@ int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad) {
       throw t9::ArgError("unknown vers bits");
     double e[16];
     const int iters = t9::evaluate(c, vers, t9::TERM_ALL, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
-    long long nem = 0, nep = 0;
-    if (vers & T9_CALC_ANALYZE) t9::count_interactions(c, &nem, &nep);
+    long long nem = 0, nep = 0, nev = 0;
+    if (vers & T9_CALC_ANALYZE) {
+      t9::count_interactions(c, &nem, &nep);
+      nev = t9::vdw_count(c);
+    }
     if (out) {
       t9::fill_energy_out(e, vers, iters, out);
       out->nem = nem;
       out->nep = nep;
+      out->nev = nev;
     }
     return T9_OK;
   }, T9_ERR_HIP);

@@ -331,6 +331,7 @@ double md_kinetic(Ctx& c);                // kcal/mol
 void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
 void count_interactions(Ctx& c, long long* nem, long long* nep);
+long long vdw_count(Ctx& c);  // analyze-mode vdW count (0 without the vdW term)
 void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h
 void md_scale_v(Ctx& c, double a);
 // Bussi rescaling with host-drawn deviates r (normal) and s2 (chi-square,

@@ -50,6 +50,10 @@ __global__ void k_vsite_pos(int n, const int* __restrict__ perm, const double* _
   jvdw[s] = jvdw_o[i];
 }
 
+struct VScale {
+  double s[kMaxCodes];
+};
+
 template <class real>
 struct VdwArgs {
   int n;
@@ -473,6 +477,63 @@ void vdw_launch_t(Ctx& c, bool vir, hipStream_t st, double* rows) {
 void vdw_sites(Ctx& c) {
   if (c.prec == 32) vdw_sites_t<float>(c);
   else vdw_sites_t<double>(c);
+}
+
+namespace {
+
+// Analyze-mode count of the vdW term (the reference's nev, src/acc/ehal.cpp:
+// 127,215-216): site pairs within the cutoff, each once, less the pairs with
+// vdW scale 0; the same distance test as the pair kernel.
+template <class real>
+__global__ __launch_bounds__(256) void k_vdw_count(int n, const int* __restrict__ nlist,
+                                                   const long long* __restrict__ woff,
+                                                   const uint32_t* __restrict__ entries,
+                                                   const PosRec<real>* __restrict__ pos, BoxT<real> box, real cut2,
+                                                   VScale vs, unsigned long long* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cnt = 0;
+  if (s < n) {
+    const PosRec<real> pi = pos[s];
+    const int len = nlist[s];
+    const uint32_t* en = entries + woff[s >> 6] + (s & 63);
+    for (int kk = 0; kk < len; ++kk) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      if (k <= s) continue;
+      const int code = (int)(ent & (kMaxCodes - 1));
+      if (code && vs.s[code] == 0) continue;
+      real R[3];
+      pair_disp(pos[k], pi, box, R[0], R[1], R[2]);
+      if (R[0] * R[0] + R[1] * R[1] + R[2] * R[2] <= cut2) ++cnt;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, cnt);
+}
+
+template <class real>
+long long vdw_count_t(Ctx& c) {
+  DBuf out;
+  out.alloc(sizeof(unsigned long long));
+  T9_HIP(hipMemsetAsync(out.p, 0, sizeof(unsigned long long), c.stream));
+  VScale vs;
+  for (int k = 0; k < kMaxCodes; ++k) vs.s[k] = c.vscale[k];
+  hipLaunchKernelGGL(k_vdw_count<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n,
+                     c.vlist.nlist.as<int>(), c.vlist.off.as<long long>(), c.vlist.entries.as<uint32_t>(),
+                     c.vpos.as<PosRec<real>>(), make_boxT<real>(c), (real)(c.vdw_cut * c.vdw_cut), vs,
+                     out.as<unsigned long long>());
+  T9_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  T9_HIP(hipMemcpyAsync(&h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  return (long long)h;
+}
+
+}  // namespace
+
+long long vdw_count(Ctx& c) {
+  if (!c.use_vdw) return 0;
+  return c.prec == 32 ? vdw_count_t<float>(c) : vdw_count_t<double>(c);
 }
 
 void vdw_launch(Ctx& c, bool vir, hipStream_t st, double* rows) {

@@ -52,7 +52,7 @@ class EnergyOut(ctypes.Structure):
                 ("em_real", ctypes.c_double), ("em_recip", ctypes.c_double), ("em_self", ctypes.c_double),
                 ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int), ("evdw", ctypes.c_double),
                 ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double),
-                ("nem", ctypes.c_int64), ("nep", ctypes.c_int64)]
+                ("nem", ctypes.c_int64), ("nep", ctypes.c_int64), ("nev", ctypes.c_int64)]
 
 
 class MdStats(ctypes.Structure):
@@ -343,7 +343,7 @@ class Context:
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
                  eangle=out.eangle, eurey=out.eurey)
         if vers & CALC_ANALYZE:
-            e.update(nem=out.nem, nep=out.nep)
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     # ---- learned initial guess
