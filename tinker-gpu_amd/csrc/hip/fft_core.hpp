@@ -1,4 +1,4 @@
-// Mixed-radix Stockham FFT building blocks (radices 2, 3, 4, 5), usable from
+// Mixed-radix Stockham FFT building blocks (radices 2, 3, 4, 5, 8), usable from
 // device kernels and from host code (the host path exists only so the same
 // arithmetic can be exercised on a CPU).
 //
@@ -61,6 +61,23 @@ __host__ __device__ __forceinline__ void small_dft(Cplx<real>* v, real sg) {
     v[2] = csub(a, c);
     v[1] = cadd(b, d);
     v[3] = csub(b, d);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs (even, odd inputs) and the eighth roots of unity
+    Cplx<real> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    small_dft<4, real>(e, sg);
+    small_dft<4, real>(o, sg);
+    const real h = (real)0.70710678118654752440;
+    const Cplx<real> w1 = {h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};   // (1 + sg i)/sqrt2
+    const Cplx<real> w2 = ctimes_i(o[2], sg);                                           // sg i
+    const Cplx<real> w3 = {h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};  // (-1 + sg i)/sqrt2
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
   } else {
     static_assert(R == 5, "radix");
     const real c1 = (real)0.30901699437494742410, c2 = (real)-0.80901699437494742410;
@@ -140,6 +157,7 @@ __host__ __device__ __forceinline__ Cplx<real>* fft_stages(Cplx<real>* buf0, Cpl
       case 2: stockham_stage<2, real>(src, dst, N, pitch, Ns, nl, tw, sg, tid, nthr); break;
       case 3: stockham_stage<3, real>(src, dst, N, pitch, Ns, nl, tw, sg, tid, nthr); break;
       case 4: stockham_stage<4, real>(src, dst, N, pitch, Ns, nl, tw, sg, tid, nthr); break;
+      case 8: stockham_stage<8, real>(src, dst, N, pitch, Ns, nl, tw, sg, tid, nthr); break;
       default: stockham_stage<5, real>(src, dst, N, pitch, Ns, nl, tw, sg, tid, nthr); break;
     }
     barrier();
@@ -151,11 +169,12 @@ __host__ __device__ __forceinline__ Cplx<real>* fft_stages(Cplx<real>* buf0, Cpl
   return src;
 }
 
-// Compile-time plan: the st-th radix of N (same order as make_fft_plan).
+// Compile-time plan: the st-th radix of N, radix 8 first (fewer stages and
+// barriers), then 4, 2, 3, 5.
 __host__ __device__ constexpr int ct_radix(int N, int st) {
   int m = N, c = 0;
-  const int rs[4] = {4, 2, 3, 5};
-  for (int a = 0; a < 4; ++a)
+  const int rs[5] = {8, 4, 2, 3, 5};
+  for (int a = 0; a < 5; ++a)
     while (m % rs[a] == 0) {
       if (c == st) return rs[a];
       m /= rs[a];
