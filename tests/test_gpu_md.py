@@ -8,6 +8,9 @@
 * The RESPA step (t9_md_run, NVE) against the same scheme written out over
   oracle forces (velR1 / inner velR0 / velR2 of the reference integrator):
   positions after three outer steps to 1e-8 Å in FP64.
+* The three RESPA inner loops (a kernel per inner step, one thread per
+  molecule, 8-lane groups per molecule; T9HIP_RESPA) give the same bits on
+  the same run: each atom sums its terms in the same order on every path.
 * Energy conservation of NVE RESPA, and the Bussi thermostat pulling the
   temperature of a cold box to the bath."""
 import numpy as np
@@ -114,6 +117,22 @@ def test_respa_matches_oracle_scheme():
     dx, dv = np.abs(xg - xo).max(), np.abs(vg - vo).max()
     print(f"respa 3 steps: max|dx|={dx:.2e} A max|dv|={dv:.2e} A/ps", flush=True)
     assert dx < 1e-8 and dv < 1e-5
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_respa_inner_paths_agree(prec, monkeypatch):
+    out = {}
+    for path in ("split", "thread", "group"):
+        monkeypatch.setenv("T9HIP_RESPA", path)
+        ctx = t9hip.Context(_box("vdwterm\nbondterm\n", eps=1e-8), prec)
+        ctx.md_init(298.0, 6)
+        ctx.md_run(4, dt=0.002, nrespa=8)
+        out[path] = (ctx.coords(), ctx.velocities())
+    for path in ("thread", "group"):
+        dx = np.abs(out[path][0] - out["split"][0]).max()
+        dv = np.abs(out[path][1] - out["split"][1]).max()
+        print(f"respa inner {path} vs split prec={prec}: max|dx|={dx:.2e} A max|dv|={dv:.2e} A/ps", flush=True)
+        assert dx == 0 and dv == 0
 
 
 def test_nve_energy_conservation():

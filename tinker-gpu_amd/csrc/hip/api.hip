@@ -754,12 +754,16 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
       t9::md_kick(c, 0.5 * dta, gf, 0.5 * dt, gs);
       // inner steps: drift, then per step the valence gradient, the inner
       // kick (velR0) and the next drift in one pass over the atoms
-      t9::md_drift(c, dta);
-      for (int k = 1; k < nrespa; ++k) {
-        if (c.use_valence) {
-          t9::valence_inner(c, gf, dta);
-        } else {
-          t9::md_drift(c, dta);
+      if (t9::respa_fused(c)) {
+        t9::respa_inner(c, gf, dta, nrespa - 1);  // one launch, one thread per molecule
+      } else {
+        t9::md_drift(c, dta);
+        for (int k = 1; k < nrespa; ++k) {
+          if (c.use_valence) {
+            t9::valence_inner(c, gf, dta);
+          } else {
+            t9::md_drift(c, dta);
+          }
         }
       }
       // energies come back to the host only for the last step's statistics

@@ -240,6 +240,12 @@ struct Ctx {
   DBuf vterm_off, vterm;         // caller order: valence terms touching each atom
   DBuf vterm_grad;               // double [terms][3][3] per-term gradient of its atoms
   int nbond = 0, nangle = 0, nurey = 0;
+  // molecules (connected by valence terms) for the fused RESPA inner loop:
+  // atoms and term indices of each molecule (nmol = 0: per-step kernels)
+  int nmol = 0;
+  bool mol_small = false;  // every molecule <= 4 atoms and <= 6 terms: 8 lanes per molecule
+  int respa_path = 2;      // inner loop: 0 a kernel per step, 1 a thread per molecule, 2 lane groups
+  DBuf mol_off, mol_atom, mol_toff, mol_term;
   DBuf bond_ij, angle_abc, urey_ac;    // int
   DBuf bond_kr, angle_kt, urey_kr;     // double
   double vpoly[6] = {0, 0, 0, 0, 0, 0};  // bond cubic, quartic; angle cubic..sextic
@@ -332,6 +338,8 @@ void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
 void count_interactions(Ctx& c, long long* nem, long long* nep);
 long long vdw_count(Ctx& c);  // analyze-mode vdW count (0 without the vdW term)
+void respa_inner(Ctx& c, double* gfast, double h, int nrep);  // drift + nrep x (fast forces, kick, drift)
+bool respa_fused(const Ctx& c);
 void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h
 void md_scale_v(Ctx& c, double a);
 // Bussi rescaling with host-drawn deviates r (normal) and s2 (chi-square,

@@ -17,6 +17,8 @@
 // the energy and virial.  No atomics anywhere, bit-reproducible.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <random>
 
 #include "ctx.hpp"
@@ -184,9 +186,25 @@ __global__ void k_vdw_to_atoms(int n, const int* __restrict__ perm, const int* _
   for (int d = 0; d < 3; ++d) grad[3 * (size_t)t + d] += g[d];
 }
 
-__device__ __forceinline__ void mdiff(const double* xyz, int a, int b, const double* L, int periodic, double* v) {
+// positions by atom index: the coordinate array, or a molecule staged in LDS
+struct GPos {
+  const double* xyz;
+  __device__ __forceinline__ double operator()(int a, int d) const { return xyz[3 * a + d]; }
+};
+struct LdsPos {  // a molecule's positions in LDS, slot k holding atom id[k]
+  const double* s;
+  int id0, id1, id2;
+  __device__ __forceinline__ double operator()(int a, int d) const {
+    const int k = a == id0 ? 0 : (a == id1 ? 1 : (a == id2 ? 2 : 3));
+    return s[3 * k + d];
+  }
+};
+
+template <class Pos>
+__device__ __forceinline__ void mdiff(const Pos& P, int a, int b, const double* L, int periodic, double* v) {
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double x = xyz[3 * a + d] - xyz[3 * b + d];
+    double x = P(a, d) - P(b, d);
     if (periodic) x -= L[d] * rint(x / L[d]);
     v[d] = x;
   }
@@ -212,8 +230,9 @@ struct ValArgs {
 
 // Gradient of one valence term for its atoms (up to 3), energy, and the
 // relative positions used by the virial.
-__device__ void valence_term(const ValArgs& A, int kind, int idx, double* e, double g[3][3], double rel[3][3],
-                             int* at, int* nat) {
+template <class Pos>
+__device__ __forceinline__ void valence_term(const ValArgs& A, const Pos& P, int kind, int idx, double* e,
+                                             double g[3][3], double rel[3][3], int* at, int* nat) {
   if (kind == 0 || kind == 2) {  // bond or Urey-Bradley: stretch polynomial
     const int* ij = kind == 0 ? A.bij + 2 * idx : A.uac + 2 * idx;
     const double* kr = kind == 0 ? A.bkr + 2 * idx : A.ukr + 2 * idx;
@@ -222,7 +241,7 @@ __device__ void valence_term(const ValArgs& A, int kind, int idx, double* e, dou
     at[1] = ij[1];
     *nat = 2;
     double R[3];
-    mdiff(A.xyz, ij[0], ij[1], A.L, A.periodic, R);
+    mdiff(P, ij[0], ij[1], A.L, A.periodic, R);
     const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
     const double d = r - kr[1];
     *e = kr[0] * d * d * (1 + c * d + q * d * d);
@@ -240,8 +259,8 @@ __device__ void valence_term(const ValArgs& A, int kind, int idx, double* e, dou
     at[2] = abc[2];
     *nat = 3;
     double u[3], v[3];
-    mdiff(A.xyz, abc[0], abc[1], A.L, A.periodic, u);
-    mdiff(A.xyz, abc[2], abc[1], A.L, A.periodic, v);
+    mdiff(P, abc[0], abc[1], A.L, A.periodic, u);
+    mdiff(P, abc[2], abc[1], A.L, A.periodic, v);
     const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
@@ -278,7 +297,7 @@ __global__ __launch_bounds__(256) void k_valence_terms(ValArgs A, int nb, int na
     const int idx = kind == 0 ? t : (kind == 1 ? t - nb : t - nb - na);
     double e, g[3][3], rel[3][3];
     int at[3], nat;
-    valence_term(A, kind, idx, &e, g, rel, at, &nat);
+    valence_term(A, GPos{A.xyz}, kind, idx, &e, g, rel, at, &nat);
     for (int m = 0; m < nat; ++m)
       for (int d = 0; d < 3; ++d) tg[9 * (size_t)t + 3 * m + d] = g[m][d];
     if (ENE) {
@@ -344,6 +363,169 @@ __global__ __launch_bounds__(256) void k_valence_gather_kick_drift(int n, const 
     const double vv = v[3 * (size_t)j + d] - c * g[d];
     v[3 * (size_t)j + d] = vv;
     x[3 * (size_t)j + d] += h * vv;
+  }
+}
+
+// The whole RESPA inner loop of one outer step, one thread per molecule
+// (the fast forces couple only atoms of one molecule): drift, then nrep times
+// the molecule's valence terms, the gather of each atom's term shares, the
+// inner kick and the next drift.  The same arithmetic, in the same order, as
+// k_valence_terms<false> + k_valence_gather_kick_drift launched per inner
+// step; the thread reads the coordinates it has just written, so x and v
+// carry no restrict.
+__global__ __launch_bounds__(256) void k_respa_inner(ValArgs A, int nmol, const int* __restrict__ mol_off,
+                                                     const int* __restrict__ mol_atom,
+                                                     const int* __restrict__ mol_toff,
+                                                     const int* __restrict__ mol_term, int nb, int na, double* tg,
+                                                     double* grad, const double* __restrict__ mass, double h, int nrep,
+                                                     double* v, double* x) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  const int a0 = mol_off[m], a1 = mol_off[m + 1];
+  for (int q = a0; q < a1; ++q) {
+    const int i = mol_atom[q];
+    for (int d = 0; d < 3; ++d) x[3 * i + d] += h * v[3 * i + d];
+  }
+  const int t0 = mol_toff[m], t1 = mol_toff[m + 1];
+  for (int rep = 0; rep < nrep; ++rep) {
+    for (int q = t0; q < t1; ++q) {
+      const int t = mol_term[q];
+      const int kind = t < nb ? 0 : (t < nb + na ? 1 : 2);
+      const int idx = kind == 0 ? t : (kind == 1 ? t - nb : t - nb - na);
+      double e, g[3][3], rel[3][3];
+      int at[3], nat;
+      valence_term(A, GPos{A.xyz}, kind, idx, &e, g, rel, at, &nat);
+      for (int r = 0; r < nat; ++r)
+        for (int d = 0; d < 3; ++d) tg[9 * (size_t)t + 3 * r + d] = g[r][d];
+    }
+    for (int q = a0; q < a1; ++q) {
+      const int j = mol_atom[q];
+      double gj[3] = {0, 0, 0};
+      for (int p = A.toff[j]; p < A.toff[j + 1]; ++p) {
+        const int tt = A.term[p];
+        const int kind = tt >> 28, idx = (tt >> 2) & 0x3ffffff, role = tt & 3;
+        const int t = kind == 0 ? idx : (kind == 1 ? nb + idx : nb + na + idx);
+        const double* pp = tg + 9 * (size_t)t + 3 * role;
+        for (int d = 0; d < 3; ++d) gj[d] += pp[d];
+      }
+      const double mj = mass[j];
+      const double c = mj > 0 ? h * kEkcal / mj : 0.0;
+      for (int d = 0; d < 3; ++d) {
+        grad[3 * (size_t)j + d] = gj[d];
+        const double vv = v[3 * (size_t)j + d] - c * gj[d];
+        v[3 * (size_t)j + d] = vv;
+        x[3 * (size_t)j + d] += h * vv;
+      }
+    }
+  }
+}
+
+// The same loop for molecules of at most 4 atoms and 6 terms (water, ions):
+// a group of 8 lanes per molecule.  Lanes 0..3 hold the atoms' positions and
+// velocities in registers for all inner steps; lanes 0..5 each evaluate one
+// term.  Per step the atom lanes stage positions in LDS, the term lanes write
+// their per-atom gradients to LDS, and each atom lane sums its own terms in
+// the order of its term list, as the gather does, then kicks and drifts.
+// Positions and velocities are written once at the end.
+constexpr int kRespaGrp = 8, kRespaMols = 256 / kRespaGrp;
+__global__ __launch_bounds__(256) void k_respa_inner_grp(ValArgs A, int nmol, const int* __restrict__ mol_off,
+                                                         const int* __restrict__ mol_atom,
+                                                         const int* __restrict__ mol_toff,
+                                                         const int* __restrict__ mol_term, int nb, int na,
+                                                         double* __restrict__ grad, const double* __restrict__ mass,
+                                                         double h, int nrep, double* __restrict__ v,
+                                                         double* __restrict__ x) {
+  __shared__ double spos[kRespaMols][4][3];
+  __shared__ double sg[kRespaMols][6][3][3];
+  const int lm = threadIdx.x / kRespaGrp, lane = threadIdx.x % kRespaGrp;
+  const int m = blockIdx.x * kRespaMols + lm;
+  int a0 = 0, nat_m = 0, t0 = 0, nt_m = 0;
+  if (m < nmol) {
+    a0 = mol_off[m];
+    nat_m = mol_off[m + 1] - a0;
+    t0 = mol_toff[m];
+    nt_m = mol_toff[m + 1] - t0;
+  }
+  // atom lane: position, velocity, and the LDS offsets of its terms' rows
+  const bool is_atom = lane < nat_m;
+  const int i = is_atom ? mol_atom[a0 + lane] : 0;
+  double xl[3] = {0, 0, 0}, vl[3] = {0, 0, 0}, cm = 0;
+  int ncsr = 0, soff[6];
+  if (is_atom) {
+    const double mi = mass[i];
+    cm = mi > 0 ? h * kEkcal / mi : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      vl[d] = v[3 * (size_t)i + d];
+      xl[d] = x[3 * (size_t)i + d] + h * vl[d];  // the first drift
+    }
+    const int p0 = A.toff[i];
+    ncsr = A.toff[i + 1] - p0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      soff[p] = 0;
+      if (p < ncsr) {
+        const int tt = A.term[p0 + p];
+        const int kind = tt >> 28, idx = (tt >> 2) & 0x3ffffff, role = tt & 3;
+        const int t = kind == 0 ? idx : (kind == 1 ? nb + idx : nb + na + idx);
+        int q = 0;
+        for (int u = 0; u < nt_m; ++u)
+          if (mol_term[t0 + u] == t) q = u;
+        soff[p] = 9 * q + 3 * role;
+      }
+    }
+  }
+  // term lane: which term, and the atom ids of the molecule's slots
+  const bool is_term = lane < nt_m;
+  int kind = 0, idx = 0;
+  LdsPos P{&spos[lm][0][0], -1, -1, -1};
+  if (is_term) {
+    const int t = mol_term[t0 + lane];
+    kind = t < nb ? 0 : (t < nb + na ? 1 : 2);
+    idx = kind == 0 ? t : (kind == 1 ? t - nb : t - nb - na);
+    P.id0 = mol_atom[a0];
+    P.id1 = nat_m > 1 ? mol_atom[a0 + 1] : -1;
+    P.id2 = nat_m > 2 ? mol_atom[a0 + 2] : -1;
+  }
+  const double* gmol = &sg[lm][0][0][0];
+  double gl[3] = {0, 0, 0};
+  for (int rep = 0; rep < nrep; ++rep) {
+    if (is_atom)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) spos[lm][lane][d] = xl[d];
+    __syncthreads();
+    if (is_term) {
+      double e, g[3][3], rel[3][3];
+      int at[3], nat;
+      valence_term(A, P, kind, idx, &e, g, rel, at, &nat);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        if (r < nat)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) sg[lm][lane][r][d] = g[r][d];
+    }
+    __syncthreads();
+    if (is_atom) {
+      gl[0] = gl[1] = gl[2] = 0;
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+        if (p < ncsr)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) gl[d] += gmol[soff[p] + d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        vl[d] -= cm * gl[d];
+        xl[d] += h * vl[d];
+      }
+    }
+  }
+  if (is_atom) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[3 * (size_t)i + d] = xl[d];
+      v[3 * (size_t)i + d] = vl[d];
+      if (nrep > 0) grad[3 * (size_t)i + d] = gl[d];
+    }
   }
 }
 
@@ -635,6 +817,48 @@ void valence_inner(Ctx& c, double* gval, double h) {
   c.bins_valid = false;
 }
 
+bool respa_fused(const Ctx& c) { return c.use_valence && c.nmol > 0 && c.respa_path != 0; }
+
+void respa_inner(Ctx& c, double* gfast, double h, int nrep) {
+  ValArgs A;
+  A.n = c.n;
+  A.xyz = c.xyz.as<double>();
+  for (int d = 0; d < 3; ++d) A.L[d] = c.periodic ? c.box[d] : 1.0;
+  A.periodic = c.periodic ? 1 : 0;
+  A.toff = c.vterm_off.as<int>();
+  A.term = c.vterm.as<int>();
+  A.bij = c.bond_ij.as<int>();
+  A.bkr = c.bond_kr.as<double>();
+  A.aabc = c.angle_abc.as<int>();
+  A.akt = c.angle_kt.as<double>();
+  A.uac = c.urey_ac.as<int>();
+  A.ukr = c.urey_kr.as<double>();
+  A.cb = c.vpoly[0];
+  A.qb = c.vpoly[1];
+  A.ca = c.vpoly[2];
+  A.qa = c.vpoly[3];
+  A.pa = c.vpoly[4];
+  A.sa = c.vpoly[5];
+  A.grad = gfast;
+  A.rows = nullptr;
+  const int nt = c.nbond + c.nangle + c.nurey;
+  c.vterm_grad.alloc(sizeof(double) * 9 * (size_t)std::max(nt, 1));
+  KScope ks_(c, KT_VALENCE);
+  if (c.mol_small && c.respa_path == 2)
+    hipLaunchKernelGGL(k_respa_inner_grp, dim3(ceil_div(c.nmol, kRespaMols)), dim3(256), 0, c.stream, A, c.nmol,
+                       c.mol_off.as<int>(), c.mol_atom.as<int>(), c.mol_toff.as<int>(), c.mol_term.as<int>(), c.nbond,
+                       c.nangle, gfast, c.mass.as<double>(), h, nrep, c.vel.as<double>(), c.xyz.as<double>());
+  else
+    hipLaunchKernelGGL(k_respa_inner, dim3(ceil_div(c.nmol, 256)), dim3(256), 0, c.stream, A, c.nmol,
+                       c.mol_off.as<int>(), c.mol_atom.as<int>(), c.mol_toff.as<int>(), c.mol_term.as<int>(), c.nbond,
+                       c.nangle, c.vterm_grad.as<double>(), gfast, c.mass.as<double>(), h, nrep, c.vel.as<double>(),
+                       c.xyz.as<double>());
+  T9_LAUNCH_CHECK();
+  c.coords_dirty = true;
+  c.rpole_valid = false;
+  c.bins_valid = false;
+}
+
 void md_setup(Ctx& c, const System& s) {
   const int n = c.n;
   hipStream_t st = c.stream;
@@ -743,6 +967,64 @@ void md_setup(Ctx& c, const System& s) {
     }
     upload(c.vterm_off, off, st);
     upload(c.vterm, tt, st);
+    // molecules: connected components of the valence terms (union-find);
+    // the fused inner loop runs one thread per molecule, so it is used only
+    // when no molecule is large
+    std::vector<int> par(n);
+    for (int i = 0; i < n; ++i) par[i] = i;
+    auto root = [&](int a) {
+      while (par[a] != a) a = par[a] = par[par[a]];
+      return a;
+    };
+    auto join = [&](int a, int b) {
+      a = root(a);
+      b = root(b);
+      if (a != b) par[std::max(a, b)] = std::min(a, b);
+    };
+    for (int t = 0; t < nb; ++t) join(s.bond_ij[2 * t], s.bond_ij[2 * t + 1]);
+    for (int t = 0; t < na; ++t) {
+      join(s.angle_abc[3 * t], s.angle_abc[3 * t + 1]);
+      join(s.angle_abc[3 * t + 1], s.angle_abc[3 * t + 2]);
+    }
+    for (int t = 0; t < nu; ++t) join(s.urey_ac[2 * t], s.urey_ac[2 * t + 1]);
+    std::vector<int> mid(n, -1), moff(1, 0), matom, mtoff(1, 0), mterm;
+    std::vector<std::vector<int>> atoms_of, terms_of;
+    for (int i = 0; i < n; ++i) {  // molecules in the order of their first atom
+      const int r = root(i);
+      if (mid[r] < 0) {
+        mid[r] = (int)atoms_of.size();
+        atoms_of.emplace_back();
+        terms_of.emplace_back();
+      }
+      atoms_of[mid[r]].push_back(i);
+    }
+    for (int t = 0; t < nb + na + nu; ++t) {
+      const int a = t < nb ? s.bond_ij[2 * t] : (t < nb + na ? s.angle_abc[3 * (t - nb)] : s.urey_ac[2 * (t - nb - na)]);
+      terms_of[mid[root(a)]].push_back(t);
+    }
+    size_t big = 0, bigt = 0;
+    for (size_t m = 0; m < atoms_of.size(); ++m) {
+      big = std::max(big, atoms_of[m].size());
+      bigt = std::max(bigt, terms_of[m].size());
+      matom.insert(matom.end(), atoms_of[m].begin(), atoms_of[m].end());
+      moff.push_back((int)matom.size());
+      mterm.insert(mterm.end(), terms_of[m].begin(), terms_of[m].end());
+      mtoff.push_back((int)mterm.size());
+    }
+    c.nmol = big <= 64 ? (int)atoms_of.size() : 0;
+    c.mol_small = big <= 4 && bigt <= 6;
+    // T9HIP_RESPA selects the inner loop: "split" (a kernel per inner step),
+    // "thread" (one thread per molecule), otherwise the 8-lane groups where
+    // the molecules are small enough
+    const char* rp = std::getenv("T9HIP_RESPA");
+    c.respa_path = !rp ? 2 : (!std::strcmp(rp, "split") ? 0 : (!std::strcmp(rp, "thread") ? 1 : 2));
+    if (c.nmol > 0) {
+      upload(c.mol_off, moff, st);
+      upload(c.mol_atom, matom, st);
+      upload(c.mol_toff, mtoff, st);
+      if (mterm.empty()) mterm.push_back(0);
+      upload(c.mol_term, mterm, st);
+    }
   }
   upload(c.mass, s.mass, st);
   c.mass_h = s.mass;
