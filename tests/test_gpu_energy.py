@@ -133,6 +133,29 @@ def test_water96k_fp32_vs_fp64():
     assert rel < 1e-6 and grms < 1e-5 and vrel < 1e-4
 
 
+@pytest.mark.parametrize("case", ["water216", "water1000+ions", "water96k"])
+def test_fft_plane_matches_lines(case, monkeypatch):
+    """FP32 x and y transforms per z-plane in LDS against the line passes
+    (T9HIP_FFT=lines): every butterfly is the same arithmetic, so energy,
+    gradient and virial are the same bits (PME 24^3: radices 8, 3; 40^3: 8, 5;
+    120^3: 8, 3, 5)."""
+    print(f"case fft plane vs lines {case}", flush=True)
+    if case == "water216":
+        s = water(216, 6, 18.643, 24, buf=1.5)
+    elif case == "water1000+ions":
+        s = water(1000, 11, 32.6, 40, seed=7, buf=1.0, n_na=20, n_cl=20)
+    else:
+        s = t9hip.System.water_box(32000, 32, 98.65, 2024, 0, 0, key=WATER_KEY.format(K=120, buf=2.0))
+    monkeypatch.delenv("T9HIP_FFT", raising=False)
+    ep, gp, vp = t9hip.Context(s, t9hip.T9_FP32).energy()
+    monkeypatch.setenv("T9HIP_FFT", "lines")
+    el, gl, vl = t9hip.Context(s, t9hip.T9_FP32).energy()
+    print(f"{case}: plane E={ep['esum']:.10f} lines E={el['esum']:.10f} max|dg|={np.abs(gp - gl).max():.2e}",
+          flush=True)
+    assert ep["esum"] == el["esum"] and ep["iterations"] == el["iterations"]
+    assert np.array_equal(gp, gl) and np.array_equal(vp, vl)
+
+
 def test_energy_bitwise_reproducible():
     s = water(216, 6, 18.643, 24, buf=1.5)
     ctx = t9hip.Context(s, t9hip.T9_FP32)

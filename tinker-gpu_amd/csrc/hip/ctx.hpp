@@ -183,6 +183,7 @@ struct Ctx {
   long long qgrid_m_stamp = -1;  // rpole_stamp it belongs to
   long long rpole_stamp = 0;     // incremented by every rotation of the multipoles
   bool conv_table_valid = false; // false after a box change
+  int fft_plane = -1;            // 1: x and y transforms per z-plane in LDS (FP32, square planes); -1 undecided
 
   // ---- energy / gradient / virial (sorted order, FP64)
   DBuf egrad;                    // double [n][3] dE/dx

@@ -13,6 +13,9 @@
 // in FP64 once per box into a table (z-pass order) and on the fly when the
 // virial sums are wanted.  Transform lengths used by the PME grids of
 // interest (24 ... 160) get kernels with compile-time index arithmetic.
+#include <cstdlib>
+#include <cstring>
+
 #include "ctx.hpp"
 #include "fft_core.hpp"
 
@@ -258,6 +261,119 @@ __global__ void k_conv_table(int K1, int K2, int K3, double pterm, double volter
   table[e] = (real)f;
 }
 
+// ---------------------------------------------------------------- planes
+// FP32 x and y transforms of one z-plane (N x N, contiguous in the grid) in
+// LDS: one 1024-thread workgroup per plane loads it with 16-byte loads,
+// runs the Stockham stages of every row, then of every column, in place, and
+// stores it back.  A stage in place: each thread reads its butterflies'
+// inputs into registers and computes them, a barrier, then it writes the
+// outputs.  Each butterfly is the arithmetic of the line passes (same
+// indices, twiddles and radix order), so the result is the same bits; the
+// round trip takes 3 grid passes (plane, z with the influence function,
+// plane) instead of 5.
+constexpr int kPlaneThreads = 1024;
+
+// Element (line lb, index e) of the plane at lb * LS + e * ES: rows have
+// LS = N + 1, ES = 1; columns LS = 1, ES = N + 1.  Threads take butterflies
+// line-fastest, so a wavefront's column accesses are contiguous.
+template <int N, int LS, int ES, int ST = 0, int NS = 1>
+__device__ __forceinline__ void plane_stages(Cplx<float>* buf, const Cplx<float>* tw, float sg) {
+  if constexpr (ST < ct_nstage(N)) {
+    constexpr int R = ct_radix(N, ST);
+    constexpr int nb = N / R, total = N * nb, PER = (total + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int tstep = N / (NS * R);
+    Cplx<float> v[PER][R];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = threadIdx.x + kPlaneThreads * u;
+      if (q < total) {
+        const int j = q / N, lb = q - j * N;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u][r] = buf[lb * LS + (j + r * nb) * ES];
+        if constexpr (NS > 1) {
+          const int k = j - (j / NS) * NS;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            Cplx<float> t = tw[k * r * tstep];
+            t.y = sg > 0 ? -t.y : t.y;
+            v[u][r] = cmulc(v[u][r], t);
+          }
+        }
+        small_dft<R>(v[u], sg);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = threadIdx.x + kPlaneThreads * u;
+      if (q < total) {
+        const int j = q / N, lb = q - j * N;
+        const int jd = j / NS, k = j - jd * NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[lb * LS + (jd * NS * R + k + r * NS) * ES] = v[u][r];
+      }
+    }
+    __syncthreads();
+    plane_stages<N, LS, ES, ST + 1, NS * R>(buf, tw, sg);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(Cplx<float>* __restrict__ grid, int sign,
+                                                             const Cplx<float>* __restrict__ twx,
+                                                             const Cplx<float>* __restrict__ twy, Gate gate) {
+  if (gate_closed(gate)) return;
+  constexpr int P = N + 1;
+  __shared__ Cplx<float> pl[N * P];
+  __shared__ Cplx<float> tws[2 * N];
+  float4* g = reinterpret_cast<float4*>(grid + (size_t)blockIdx.x * N * N);
+  constexpr int NE = N * N / 2;  // two complex values per 16-byte load
+  constexpr int PER = (NE + kPlaneThreads - 1) / kPlaneThreads;
+  float4 v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = threadIdx.x + kPlaneThreads * u;
+    if (q < NE) v[u] = g[q];
+  }
+  if (threadIdx.x < N) tws[threadIdx.x] = twx[threadIdx.x];
+  else if (threadIdx.x < 2 * N) tws[threadIdx.x] = twy[threadIdx.x - N];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = threadIdx.x + kPlaneThreads * u;
+    if (q < NE) {
+      const int y = (2 * q) / N, x = 2 * q - y * N;
+      pl[y * P + x] = Cplx<float>{v[u].x, v[u].y};
+      pl[y * P + x + 1] = Cplx<float>{v[u].z, v[u].w};
+    }
+  }
+  __syncthreads();
+  const float sg = (float)sign;
+  if (sign < 0) {  // forward: x, then y (the order of the line passes)
+    plane_stages<N, P, 1>(pl, tws, sg);
+    plane_stages<N, 1, P>(pl, tws + N, sg);
+  } else {         // inverse: y, then x
+    plane_stages<N, 1, P>(pl, tws + N, sg);
+    plane_stages<N, P, 1>(pl, tws, sg);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = threadIdx.x + kPlaneThreads * u;
+    if (q < NE) {
+      const int y = (2 * q) / N, x = 2 * q - y * N;
+      const Cplx<float> a = pl[y * P + x], b = pl[y * P + x + 1];
+      g[q] = float4{a.x, a.y, b.x, b.y};
+    }
+  }
+}
+
+// plane lengths with a kernel (LDS: N (N + 1) + 2 N complex values)
+inline bool plane_len(int n) {
+  switch (n) {
+    case 24: case 32: case 40: case 48: case 64: case 80: case 96: case 120: case 128: return true;
+    default: return false;
+  }
+}
+
 template <class real>
 constexpr int lines_per_block() {
   return sizeof(real) == 4 ? 16 : 8;
@@ -315,6 +431,34 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
                        cv, c.gate);
     T9_LAUNCH_CHECK();
   };
+  if (c.fft_plane < 0) {
+    const char* fp = std::getenv("T9HIP_FFT");
+    c.fft_plane = sizeof(real) == 4 && K1 == K2 && plane_len(K1) && !(fp && !std::strcmp(fp, "lines")) ? 1 : 0;
+  }
+  if constexpr (sizeof(real) == 4) {
+    if (c.fft_plane == 1) {
+      void (*pk)(Cplx<float>*, int, const Cplx<float>*, const Cplx<float>*, Gate);
+      switch (K1) {
+        case 24: pk = k_fft_plane<24>; break;
+        case 32: pk = k_fft_plane<32>; break;
+        case 40: pk = k_fft_plane<40>; break;
+        case 48: pk = k_fft_plane<48>; break;
+        case 64: pk = k_fft_plane<64>; break;
+        case 80: pk = k_fft_plane<80>; break;
+        case 96: pk = k_fft_plane<96>; break;
+        case 120: pk = k_fft_plane<120>; break;
+        default: pk = k_fft_plane<128>; break;
+      }
+      KScope ks_(c, KT_FFT, fwd_only ? 2 : 3);
+      hipLaunchKernelGGL(pk, dim3(K3), dim3(kPlaneThreads), 0, c.stream, grid, -1, twa[0], twa[1], c.gate);
+      T9_LAUNCH_CHECK();
+      launch(2, -1, conv);
+      if (fwd_only) return;
+      hipLaunchKernelGGL(pk, dim3(K3), dim3(kPlaneThreads), 0, c.stream, grid, +1, twa[0], twa[1], c.gate);
+      T9_LAUNCH_CHECK();
+      return;
+    }
+  }
   KScope ks_(c, KT_FFT, fwd_only ? 3 : 5);  // the passes run back to back
   launch(0, -1, none);
   launch(1, -1, none);
