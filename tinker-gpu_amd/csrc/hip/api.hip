@@ -281,7 +281,8 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.pme_base.alloc(sizeof(int4) * n);
     const int nt = c.ntile[0] * c.ntile[1] * c.ntile[2];
     c.tile_count.alloc(sizeof(int) * nt);
-    c.tile_start.alloc(sizeof(int) * nt);
+    T9_HIP(hipMemsetAsync(c.tile_count.p, 0, sizeof(int) * nt, st));  // each binning's scan clears it again
+    c.tile_start.alloc(sizeof(int) * (nt + 1));
     c.tile_cursor.alloc(sizeof(int) * nt);
     c.tile_atoms.alloc(sizeof(int) * n);
   }

@@ -79,38 +79,48 @@ __global__ void k_pme_bin(int n, const int* __restrict__ perm, const double* __r
   atomicAdd(&tile_count[tile], 1);
 }
 
-__global__ void k_tile_scatter(int n, const int4* __restrict__ pbase, const int* __restrict__ start,
-                               int* __restrict__ cursor, int* __restrict__ atoms) {
+__global__ void k_tile_scatter(int n, const int4* __restrict__ pbase, int* __restrict__ cursor,
+                               int* __restrict__ atoms) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int t = pbase[s].w;
-  atoms[start[t] + atomicAdd(&cursor[t], 1)] = s;
+  atoms[atomicAdd(&cursor[t], 1)] = s;
 }
 
 // Order each bin's atoms by sorted index (insertion sort per bin), which
 // fixes the summation order of the spreading gather.
 
-__global__ __launch_bounds__(1024) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int m) {
-  __shared__ int sh[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < m; base += 1024) {
-    int idx = base + threadIdx.x;
-    int v = idx < m ? in[idx] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < 1024; s <<= 1) {
-      int t = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (idx < m) out[idx] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += sh[1023];
-    __syncthreads();
+// Exclusive prefix sums of the per-tile atom counts into start[0..m]
+// (start[m] the total), each tile's scatter cursor set to its start, and the
+// counts cleared for the next binning.  One workgroup: each thread sums a
+// contiguous segment, the segment sums are scanned across the wavefront by
+// shuffles and across the four wavefronts through LDS (one barrier).
+__global__ __launch_bounds__(256) void k_scan_tiles(int* __restrict__ count, int* __restrict__ start,
+                                                    int* __restrict__ cursor, int m) {
+  __shared__ int wsum[4];
+  const int per = (m + 255) / 256;
+  const int b = min(m, (int)threadIdx.x * per), e = min(m, b + per);
+  int s = 0;
+  for (int t = b; t < e; ++t) s += count[t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
   }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int run = incl - s;
+  for (int u = 0; u < w; ++u) run += wsum[u];
+  for (int t = b; t < e; ++t) {
+    const int c = count[t];
+    start[t] = run;
+    cursor[t] = run;
+    count[t] = 0;
+    run += c;
+  }
+  if (threadIdx.x == 255) start[m] = run;
 }
 
 // ------------------------------------------------------------ spreading
@@ -183,7 +193,6 @@ __device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
 
 template <class real, int MODE>
 __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restrict__ tile_start,
-                                                const int* __restrict__ tile_count,
                                                 const int* __restrict__ tile_atoms, const R4<real>* __restrict__ pw,
                                                 const int4* __restrict__ pbase, const real* __restrict__ src1,
                                                 const real* __restrict__ src2, real a1, real a2, real a3, int chunk,
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
       if (b3 < 0) b3 += g.nt[2];
       const int bin = b1 + g.nt[0] * (b2 + g.nt[1] * b3);
       st = tile_start[bin];
-      ct = tile_count[bin];
+      ct = tile_start[bin + 1] - st;
     }
     bin_start[c] = st;
     bin_count[c] = ct;
@@ -490,18 +499,17 @@ template <class real>
 void pme_bin_t(Ctx& c) {
   const int n = c.n;
   const int ntot = c.ntile[0] * c.ntile[1] * c.ntile[2];
-  T9_HIP(hipMemsetAsync(c.tile_count.p, 0, sizeof(int) * ntot, c.stream));
-  T9_HIP(hipMemsetAsync(c.tile_cursor.p, 0, sizeof(int) * ntot, c.stream));
+  // tile_count is zero here: cleared at allocation and by every scan
   hipLaunchKernelGGL(k_pme_bin<real>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.perm.as<int>(),
                      c.xyz.as<double>(), c.box[0], c.box[1], c.box[2], c.grid[0], c.grid[1], c.grid[2], c.tsz[0],
                      c.tsz[1], c.tsz[2], c.ntile[0], c.ntile[1], c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
                      c.tile_count.as<int>());
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c.stream, c.tile_count.as<int>(), c.tile_start.as<int>(),
-                     ntot);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, c.stream, c.tile_count.as<int>(), c.tile_start.as<int>(),
+                     c.tile_cursor.as<int>(), ntot);
   T9_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_tile_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.pme_base.as<int4>(),
-                     c.tile_start.as<int>(), c.tile_cursor.as<int>(), c.tile_atoms.as<int>());
+                     c.tile_cursor.as<int>(), c.tile_atoms.as<int>());
   T9_LAUNCH_CHECK();
   // bins stay in arrival order: the fixed-point tile sums do not depend on it
   c.bins_valid = true;
@@ -524,7 +532,7 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   auto kern = mode == 0 ? k_spread<real, 0> : (mode == 1 ? k_spread<real, 1> : k_spread<real, 2>);
   KScope ks_(c, KT_SPREAD);
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
-                     c.tile_count.as<int>(), c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
+                     c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
                      (const real*)s1, (const real*)s2, a1, a2, a3, chunk,
                      c.qgrid.as<cplx_t<real>>(), c.gate);
   T9_LAUNCH_CHECK();
