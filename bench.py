@@ -173,8 +173,13 @@ def main():
     # (the events add ~0.2 ms per step, so this leg is not the measured one)
     brk = run_gpu(t9hip, system, prec, min(args.steps, 30), args.warmup, guess, start, timers=True)
     kt = brk["kernels"]
-    # dominant kernel family by time per step; the measured leg times only it
-    dom = max(kt, key=lambda n: kt[n]["total_ms_per_step"])
+    # dominant kernel family by time per step; the measured leg times only it.
+    # Families within 5 % of the largest count as tied; a tie goes to the PME
+    # FFT, whose launches are timed as whole round trips (one event pair per
+    # 3 or 5 launches, so the per-launch average carries no event overhead).
+    top = max(kt[n]["total_ms_per_step"] for n in kt)
+    tied = [n for n in kt if kt[n]["total_ms_per_step"] >= 0.95 * top]
+    dom = "fft_pass" if "fft_pass" in tied else max(tied, key=lambda n: kt[n]["total_ms_per_step"])
     main_run = run_gpu(t9hip, system, prec, args.steps, args.warmup, guess, start, timers=(dom,))
     sec = main_run["sec_per_step"]
     ns_day = 0.1728 / sec

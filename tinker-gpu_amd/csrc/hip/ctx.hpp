@@ -280,7 +280,9 @@ void mpole_init(Ctx& c);
 void dfield_all(Ctx& c);                  // c.field/fieldp = full direct field
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp);  // sorted real vectors
 void pme_bin(Ctx& c);
-void pme_fft_roundtrip(Ctx& c);           // forward FFT, influence function, inverse FFT of c.qgrid
+// forward FFT, influence function, inverse FFT of c.qgrid; `lines`: x and y
+// as line passes even where the plane kernel applies (same bits)
+void pme_fft_roundtrip(Ctx& c, bool lines = false);
 void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self of the permanent multipoles

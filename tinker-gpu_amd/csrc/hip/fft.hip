@@ -380,7 +380,7 @@ constexpr int lines_per_block() {
 }
 
 template <class real>
-void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
+void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale, bool lines = false) {
   constexpr int B = lines_per_block<real>();
   const int K1 = c.grid[0], K2 = c.grid[1], K3 = c.grid[2];
   auto* grid = c.qgrid.as<Cplx<real>>();
@@ -436,7 +436,7 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale) {
     c.fft_plane = sizeof(real) == 4 && K1 == K2 && plane_len(K1) && !(fp && !std::strcmp(fp, "lines")) ? 1 : 0;
   }
   if constexpr (sizeof(real) == 4) {
-    if (c.fft_plane == 1) {
+    if (c.fft_plane == 1 && !lines) {
       void (*pk)(Cplx<float>*, int, const Cplx<float>*, const Cplx<float>*, Gate);
       switch (K1) {
         case 24: pk = k_fft_plane<24>; break;
@@ -476,9 +476,9 @@ int zpass_blocks(const Ctx& c) {
 
 }  // namespace
 
-void pme_fft_roundtrip(Ctx& c) {
-  if (c.prec == 32) roundtrip_t<float>(c, false, nullptr, 0.0);
-  else roundtrip_t<double>(c, false, nullptr, 0.0);
+void pme_fft_roundtrip(Ctx& c, bool lines) {
+  if (c.prec == 32) roundtrip_t<float>(c, false, nullptr, 0.0, lines);
+  else roundtrip_t<double>(c, false, nullptr, 0.0, lines);
 }
 
 void pme_fft_virial(Ctx& c, int roundtrip, double vscale) {

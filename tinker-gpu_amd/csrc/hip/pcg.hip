@@ -453,11 +453,14 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   };
   // The host enqueues as many iterations as the previous solve needed, then
   // reads the scalars back once; after that one iteration per readback.  A
-  // learned guess is judged after its first iteration.
+  // learned guess is judged at the first readback, from the residual norm of
+  // the initial guess (written before the first iteration): a rejected guess
+  // restarts from the ASPC guess, and the iterations already run on it are
+  // discarded (counted in the iteration total, not in the next solve's plan).
   double eps = 100;
   int enq = 0;
   for (;;) {
-    const int upto = learned_try ? enq + 1 : std::max(enq + 1, iter0 + c.pcg_planned);
+    const int upto = std::max(enq + 1, iter0 + c.pcg_planned);
     while (enq < upto) enqueue(++enq);
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     T9_HIP(hipStreamSynchronize(st));

@@ -557,7 +557,11 @@ template <class real>
 void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   if (!c.bins_valid) pme_bin_t<real>(c);
   spread_t<real>(c, 1, c.rpole.p, nullptr);
-  pme_fft_roundtrip(c);
+  // the direct field's round trip runs beside the real-space direct field and
+  // the vdW pair kernel of the other streams, which hold every CU: a plane
+  // workgroup (a whole CU's LDS) would wait for them to drain, the line
+  // passes' small workgroups fit in between
+  pme_fft_roundtrip(c, before_gather != nullptr);
   // keep the convolved grid: the reciprocal multipole energy of the same
   // coordinates gathers from it instead of spreading and transforming again.
   // The two grid buffers trade places (no copy); the dipole round trips use
