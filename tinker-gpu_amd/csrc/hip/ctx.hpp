@@ -167,7 +167,9 @@ struct Ctx {
   int pcg_trace_len = 0;         // entries 0..iterations of the last solve (0: initial guess)
   DBuf scalars;                  // double [32] on device
   DBuf hscalars_pinned;          // host-pinned mirror
-  double* h_scalars = nullptr;
+  double* h_scalars = nullptr;   // pinned, 48 doubles: CG scalars, and the list check's flag at kDispSlot
+  DBuf disp_flag;                // int: the last displacement check that found an atom beyond half the buffer
+  int disp_stamp = 0;
 
   // ---- PME
   int tsz[3] = {8, 8, 8};       // spreading tile edge per axis (divides the grid size)

@@ -14,6 +14,8 @@
 // reference's exclusion bit masks, spatial.h si1..si4).  Pair kernels walk
 // their own atom's list and write only their own atom's sums: no atomics,
 // fixed summation order, bit-reproducible results.
+#include <climits>
+
 #include "ctx.hpp"
 
 namespace t9 {
@@ -529,9 +531,12 @@ __global__ void k_wave_slots(int n, int nwave, const int* __restrict__ nlist, in
   slots[w] = 64LL * m;
 }
 
+// Displacement check: any atom that moved more than half the list buffer
+// since the last build writes `stamp` (a new value per check) to the flag, so
+// the flag needs no clearing between checks.
 template <class real>
 __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* __restrict__ last, double lx,
-                           double ly, double lz, int periodic, double lim2, int* __restrict__ flag) {
+                           double ly, double lz, int periodic, double lim2, int stamp, int* __restrict__ flag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double dx = xyz[3 * i] - last[3 * i], dy = xyz[3 * i + 1] - last[3 * i + 1], dz = xyz[3 * i + 2] - last[3 * i + 2];
@@ -540,7 +545,7 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
     dy -= ly * rint(dy / ly);
     dz -= lz * rint(dz / lz);
   }
-  if (dx * dx + dy * dy + dz * dz > lim2) atomicOr(flag, 1);
+  if (dx * dx + dy * dy + dz * dz > lim2) *flag = stamp;
 }
 
 
@@ -800,18 +805,21 @@ void compact_t(Ctx& c) {
 template <class real>
 bool needs_rebuild_t(Ctx& c) {
   if (!c.list_valid) return true;
-  DBuf flag;
-  flag.alloc(sizeof(int));
-  T9_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), c.stream));
+  if (!c.disp_flag.p) {  // allocated and cleared once; stamps start at 1
+    c.disp_flag.alloc(sizeof(int));
+    T9_HIP(hipMemsetAsync(c.disp_flag.p, 0, sizeof(int), c.stream));
+  }
+  const int stamp = c.disp_stamp = c.disp_stamp == INT_MAX ? 1 : c.disp_stamp + 1;
   double half = 0.5 * c.opt.list_buffer;
   hipLaunchKernelGGL(k_max_disp<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.xyz.as<double>(),
-                     c.xyz_last.as<double>(), c.box[0], c.box[1], c.box[2], c.periodic ? 1 : 0, half * half,
-                     flag.as<int>());
+                     c.xyz_last.as<double>(), c.box[0], c.box[1], c.box[2], c.periodic ? 1 : 0, half * half, stamp,
+                     c.disp_flag.as<int>());
   T9_LAUNCH_CHECK();
-  int h = 0;
-  T9_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  // read into pinned memory (a spare slot of the scalar readback buffer)
+  int* h = reinterpret_cast<int*>(c.h_scalars + kDispSlot);
+  T9_HIP(hipMemcpyAsync(h, c.disp_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   T9_HIP(hipStreamSynchronize(c.stream));
-  return h != 0;
+  return *h == stamp;
 }
 
 template <class real>
