@@ -98,6 +98,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   T9_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
   T9_HIP(hipEventCreateWithFlags(&c.ev_ufork, hipEventDisableTiming));
   T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin, hipEventDisableTiming));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_read, hipEventDisableTiming));
   hipStream_t st = c.stream;
   const int n = s.n;
 
@@ -325,13 +326,23 @@ void scatter_from_host(Ctx& c, const double* host_orig, void* dev_sorted) {
   T9_HIP(hipStreamSynchronize(c.stream));
 }
 
+void mark_readback(Ctx& c) { T9_HIP(hipEventRecord(c.ev_read, c.stream)); }
+
+void wait_marked(Ctx& c) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(c.ev_read);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) T9_HIP(e);
+  }
+}
+
 // Make the list, sorted coordinates and global multipoles current.
 void prepare(Ctx& c) {
   if (!c.list_valid) {
     nblist_build(c);
   } else if (c.coords_dirty) {
-    if (nblist_needs_rebuild(c)) nblist_build(c);
-    else refresh_sorted_coords(c);
+    if (nblist_needs_rebuild(c, true)) nblist_build(c);
+    else nblist_mark_refreshed(c);
   }
   if (!c.mlist.cut_valid) nblist_compact(c);
   if (!c.rpole_valid) mpole_init(c);
@@ -487,6 +498,7 @@ int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out) {
       if (h->c.side2) (void)hipStreamDestroy(h->c.side2);
       if (h->c.ev_ufork) (void)hipEventDestroy(h->c.ev_ufork);
       if (h->c.ev_ujoin) (void)hipEventDestroy(h->c.ev_ujoin);
+      if (h->c.ev_read) (void)hipEventDestroy(h->c.ev_read);
       if (h->c.h_scalars) (void)hipHostFree(h->c.h_scalars);
       delete h;
       throw;
@@ -504,6 +516,7 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->c.side2);
     hipStream_t st = ctx->c.stream, side = ctx->c.side, side2 = ctx->c.side2;
     hipEvent_t e1 = ctx->c.ev_fork, e2 = ctx->c.ev_join, e3 = ctx->c.ev_ufork, e4 = ctx->c.ev_ujoin;
+    hipEvent_t e5 = ctx->c.ev_read;
     double* hs = ctx->c.h_scalars;
     t9::ktimer_release(ctx->c);
     delete ctx;
@@ -515,6 +528,7 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     if (side2) (void)hipStreamDestroy(side2);
     if (e3) (void)hipEventDestroy(e3);
     if (e4) (void)hipEventDestroy(e4);
+    if (e5) (void)hipEventDestroy(e5);
     return T9_OK;
   }, T9_ERR_HIP);
 }

@@ -115,6 +115,7 @@ struct Ctx {
   // while the PME part of the same field spreads and transforms
   hipStream_t side2 = nullptr;
   hipEvent_t ev_ufork = nullptr, ev_ujoin = nullptr;
+  hipEvent_t ev_read = nullptr;  // marks a readback the host waits for by polling
   bool vdw_pending = false;  // vdW pair kernel enqueued on `side`, not yet joined
   Options opt;
   double box[3] = {0, 0, 0};
@@ -275,7 +276,10 @@ constexpr int kEnergyRow = 16;
 // ----- entry points implemented across the .hip files
 void ctx_init(Ctx& c, const System& s, int precision);
 void nblist_build(Ctx& c);
-bool nblist_needs_rebuild(Ctx& c);
+// refresh: enqueue the sorted-coordinate refresh while the host waits for the
+// check (a rebuild overwrites it); then call nblist_mark_refreshed if no rebuild
+bool nblist_needs_rebuild(Ctx& c, bool refresh = false);
+void nblist_mark_refreshed(Ctx& c);
 void nblist_compact(Ctx& c);              // refresh the in-cutoff lists for the current coordinates
 void refresh_sorted_coords(Ctx& c);
 void mpole_init(Ctx& c);
@@ -333,6 +337,10 @@ enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_ALL = 7 };
 // the electrostatics and vdW in c.egrad, caller-order valence dE/dx in
 // c.g_fast; grad_orig (host, caller order, or null) receives their sum
 void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig);
+// readbacks the host waits for: mark after the copy, then wait by polling the
+// marker event (a blocking synchronize wakes the thread tens of microseconds late)
+void mark_readback(Ctx& c);
+void wait_marked(Ctx& c);
 void prepare(Ctx& c);                     // list, sorted coordinates, global multipoles current
 
 // molecular dynamics (caller order, FP64)

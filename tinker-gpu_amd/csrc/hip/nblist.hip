@@ -803,7 +803,10 @@ void compact_t(Ctx& c) {
 }
 
 template <class real>
-bool needs_rebuild_t(Ctx& c) {
+void refresh_kernels_t(Ctx& c);
+
+template <class real>
+bool needs_rebuild_t(Ctx& c, bool refresh) {
   if (!c.list_valid) return true;
   if (!c.disp_flag.p) {  // allocated and cleared once; stamps start at 1
     c.disp_flag.alloc(sizeof(int));
@@ -818,32 +821,45 @@ bool needs_rebuild_t(Ctx& c) {
   // read into pinned memory (a spare slot of the scalar readback buffer)
   int* h = reinterpret_cast<int*>(c.h_scalars + kDispSlot);
   T9_HIP(hipMemcpyAsync(h, c.disp_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  T9_HIP(hipStreamSynchronize(c.stream));
+  mark_readback(c);
+  if (refresh) refresh_kernels_t<real>(c);  // runs while the host waits; a rebuild overwrites it
+  wait_marked(c);
   return *h == stamp;
 }
 
 template <class real>
-void refresh_t(Ctx& c) {
+void refresh_kernels_t(Ctx& c) {
   hipLaunchKernelGGL(k_refresh_pos<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
                      c.xyz.as<double>(), make_double3(c.vorg[0], c.vorg[1], c.vorg[2]),
                      make_double3(c.vlen[0], c.vlen[1], c.vlen[2]), c.periodic ? 1 : 0, c.pdamp_o.as<double>(),
                      c.pos.as<PosRec<real>>());
   T9_LAUNCH_CHECK();
   if (c.vlist.cutoff > 0) vdw_sites(c);
+}
+
+template <class real>
+void refresh_t(Ctx& c) {
+  refresh_kernels_t<real>(c);
+  nblist_mark_refreshed(c);
+}
+
+}  // namespace
+
+void nblist_mark_refreshed(Ctx& c) {
   c.coords_dirty = false;
   c.rpole_valid = false;
   c.bins_valid = false;
   c.mlist.cut_valid = c.ulist.cut_valid = false;
 }
 
-}  // namespace
-
 void nblist_build(Ctx& c) {
   if (c.prec == 32) nblist_build_t<float>(c);
   else nblist_build_t<double>(c);
 }
 
-bool nblist_needs_rebuild(Ctx& c) { return c.prec == 32 ? needs_rebuild_t<float>(c) : needs_rebuild_t<double>(c); }
+bool nblist_needs_rebuild(Ctx& c, bool refresh) {
+  return c.prec == 32 ? needs_rebuild_t<float>(c, refresh) : needs_rebuild_t<double>(c, refresh);
+}
 
 void nblist_compact(Ctx& c) {
   if (c.prec == 32) compact_t<float>(c);

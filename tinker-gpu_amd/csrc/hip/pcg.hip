@@ -463,7 +463,8 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     const int upto = std::max(enq + 1, iter0 + c.pcg_planned);
     while (enq < upto) enqueue(++enq);
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
-    T9_HIP(hipStreamSynchronize(st));
+    mark_readback(c);
+    wait_marked(c);
     if (learned_try) {
       learned_try = false;
       if (!(c.h_scalars[S_EPS0] <= c.aspc_eps0_ref)) {  // worse than ASPC, or not finite: restart
