@@ -249,7 +249,7 @@ constexpr int kPairParts = 8;
 // can enqueue several iterations without reading the residual back.
 constexpr int kDoneSlot = 15;
 constexpr int kTraceCap = 256;  // per-iteration residual trace of the last CG solve
-constexpr int kDispSlot = 44;   // slot of the pinned readback buffer holding the list check's flag (40: kinetic energy)
+constexpr int kDispSlot = 44;   // pinned readback slot of the list check's flag (40: kinetic energy; 45-47: list totals)
 struct Gate {
   const double* sc = nullptr;  // CG scalars; nullptr: no gate
   int iter = 0;                // iteration the launch belongs to

@@ -61,6 +61,7 @@ struct NbList {
   DBuf nlist;          // int [n]     neighbors of each sorted atom
   DBuf maxlen;         // int [nwave]
   DBuf off;            // int64 [nwave + 1]
+  DBuf slots;          // int64 [nwave] slab sizes (build scratch)
   DBuf entries;        // uint32 [capacity]
   size_t capacity = 0;
   long long total = 0;  // slots used (including padding)

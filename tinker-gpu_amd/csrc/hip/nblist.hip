@@ -57,54 +57,35 @@ __global__ void k_cell_assign(int n, const double* __restrict__ xyz, double lx, 
   atomicAdd(&count[k], 1);
 }
 
-// Single-block exclusive scan of m ints (chunks of 1024, carried offset).
-__global__ __launch_bounds__(kScanBlock) void k_scan_int(const int* __restrict__ in, int* __restrict__ out, int m) {
-  __shared__ int sh[kScanBlock];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < m; base += kScanBlock) {
-    int idx = base + threadIdx.x;
-    int v = idx < m ? in[idx] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < kScanBlock; s <<= 1) {
-      int t = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (idx < m) out[idx] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == kScanBlock - 1) carry += sh[kScanBlock - 1];
-    __syncthreads();
+// Single-workgroup exclusive scan of m values into out[0..m-1], and the sum
+// into out[m] when `with_total`: each thread sums a contiguous segment, the
+// segment sums are scanned across the wavefront by shuffles and across the
+// wavefronts through LDS (one barrier).
+template <class T>
+__global__ __launch_bounds__(kScanBlock) void k_scan(const T* __restrict__ in, T* __restrict__ out, int m,
+                                                     int with_total) {
+  __shared__ T wsum[kScanBlock / 64];
+  const int per = (m + kScanBlock - 1) / kScanBlock;
+  const int b = min(m, (int)threadIdx.x * per), e = min(m, b + per);
+  T sum = 0;
+  for (int t = b; t < e; ++t) sum += in[t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
   }
-}
-
-// Single-block exclusive scan of m int64 values; out has m + 1 entries.
-__global__ __launch_bounds__(kScanBlock) void k_scan_i64(const long long* __restrict__ in, long long* __restrict__ out,
-                                                         int m) {
-  __shared__ long long sh[kScanBlock];
-  __shared__ long long carry;
-  if (threadIdx.x == 0) carry = 0;
+  if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  for (int base = 0; base < m; base += kScanBlock) {
-    int idx = base + threadIdx.x;
-    long long v = idx < m ? in[idx] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < kScanBlock; s <<= 1) {
-      long long t = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (idx < m) out[idx] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == kScanBlock - 1) carry += sh[kScanBlock - 1];
-    __syncthreads();
+  T run = incl - sum;
+  for (int u = 0; u < w; ++u) run += wsum[u];
+  for (int t = b; t < e; ++t) {
+    const T c = in[t];
+    out[t] = run;
+    run += c;
   }
-  if (threadIdx.x == 0) out[m] = carry;
+  if (with_total && threadIdx.x == kScanBlock - 1) out[m] = run;
 }
 
 __global__ void k_cell_scatter(int n, const int* __restrict__ key, const int* __restrict__ start,
@@ -549,22 +530,24 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
 }
 
 
-// Slab offsets and capacity of one list after the count pass.
-void list_slots(Ctx& c, NbList& L) {
+// Slab offsets and total of one list after the count pass; the total is
+// copied to pinned memory at `h_total` (the host waits once for all lists,
+// then list_capacity sizes the slabs).
+void list_slots(Ctx& c, NbList& L, long long* h_total) {
   const int n = c.n, nw = c.nwave;
   L.maxlen.alloc(sizeof(int) * nw);
   L.off.alloc(sizeof(long long) * (nw + 1));
-  DBuf slots;
-  slots.alloc(sizeof(long long) * nw);
+  L.slots.alloc(sizeof(long long) * nw);
   hipLaunchKernelGGL(k_wave_slots, dim3(ceil_div(nw, 256)), dim3(256), 0, c.stream, n, nw, L.nlist.as<int>(),
-                     L.maxlen.as<int>(), slots.as<long long>());
+                     L.maxlen.as<int>(), L.slots.as<long long>());
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(kScanBlock), 0, c.stream, slots.as<long long>(),
-                     L.off.as<long long>(), nw);
+  hipLaunchKernelGGL(k_scan<long long>, dim3(1), dim3(kScanBlock), 0, c.stream, L.slots.as<long long>(),
+                     L.off.as<long long>(), nw, 1);
   T9_LAUNCH_CHECK();
-  long long total = 0;
-  T9_HIP(hipMemcpyAsync(&total, L.off.as<long long>() + nw, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
-  T9_HIP(hipStreamSynchronize(c.stream));
+  T9_HIP(hipMemcpyAsync(h_total, L.off.as<long long>() + nw, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
+}
+
+void list_capacity(NbList& L, long long total) {
   if (total < 0 || total > (1LL << 31)) throw ListOverflow("neighbor list exceeds 2^31 slots");
   if ((size_t)total > L.capacity || !L.entries.p) {
     size_t cap = (size_t)(total * 1.2) + 64 * 64;
@@ -620,8 +603,15 @@ void build_lists(Ctx& c) {
   T9_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_parts, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, A.parts, A.nm, A.nu, A.nv);
   T9_LAUNCH_CHECK();
-  for (NbList* L : {&c.mlist, &c.ulist, &c.vlist})
-    if (L->cutoff > 0) list_slots(c, *L);
+  // the three totals come back in one readback (pinned slots 45..47)
+  long long* h_tot = reinterpret_cast<long long*>(c.h_scalars + 45);
+  NbList* lists[3] = {&c.mlist, &c.ulist, &c.vlist};
+  for (int q = 0; q < 3; ++q)
+    if (lists[q]->cutoff > 0) list_slots(c, *lists[q], h_tot + q);
+  mark_readback(c);
+  wait_marked(c);
+  for (int q = 0; q < 3; ++q)
+    if (lists[q]->cutoff > 0) list_capacity(*lists[q], h_tot[q]);
   A.fill = 1;
   A.om = c.mlist.off.as<long long>();
   A.em = c.mlist.entries.as<uint32_t>();
@@ -656,8 +646,8 @@ void nblist_build_t(Ctx& c) {
                      c.ncell[1], c.ncell[2], c.brute ? 1 : 0, c.cell_key.as<int>(), c.cell_count.as<int>(),
                      cellc_orig.as<int>());
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_int, dim3(1), dim3(kScanBlock), 0, c.stream, c.cell_count.as<int>(),
-                     c.cell_start.as<int>(), nkeys);
+  hipLaunchKernelGGL(k_scan<int>, dim3(1), dim3(kScanBlock), 0, c.stream, c.cell_count.as<int>(),
+                     c.cell_start.as<int>(), nkeys, 0);
   T9_LAUNCH_CHECK();
   if (c.brute) {
     hipLaunchKernelGGL(k_identity, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>());
