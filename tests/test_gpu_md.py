@@ -11,6 +11,8 @@
 * The three RESPA inner loops (a kernel per inner step, one thread per
   molecule, 8-lane groups per molecule; T9HIP_RESPA) give the same bits on
   the same run: each atom sums its terms in the same order on every path.
+* The three neighbor-list builds (staged single scan, count-then-write, and
+  the overflow fallback) give the same bits.
 * Energy conservation of NVE RESPA, and the Bussi thermostat pulling the
   temperature of a cold box to the bath."""
 import numpy as np
@@ -133,6 +135,34 @@ def test_respa_inner_paths_agree(prec, monkeypatch):
         dv = np.abs(out[path][1] - out["split"][1]).max()
         print(f"respa inner {path} vs split prec={prec}: max|dx|={dx:.2e} A max|dv|={dv:.2e} A/ps", flush=True)
         assert dx == 0 and dv == 0
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_list_build_modes_agree(prec, monkeypatch):
+    """Neighbor-list builds: the staged single scan (every build after a
+    context's first), the count-then-write build (T9HIP_LISTS=twopass) and the
+    staged scan whose part windows overflow and fall back (T9HIP_LISTS=tiny)
+    give the same lists: the same bits in the energy after a second explicit
+    build and in positions after MD steps with list rebuilds (0.3 A buffer)."""
+    out = {}
+    for mode in ("staged", "twopass", "tiny"):
+        if mode == "staged":
+            monkeypatch.delenv("T9HIP_LISTS", raising=False)
+        else:
+            monkeypatch.setenv("T9HIP_LISTS", mode)
+        ctx = t9hip.Context(_box("vdwterm\nbondterm\n", eps=1e-8), prec)
+        ctx.nblist_build()
+        ctx.nblist_build()
+        e, g, v = ctx.energy()
+        ctx.md_init(298.0, 3)
+        ctx.md_run(6, dt=0.002, nrespa=8)
+        out[mode] = (e["esum"], g, v, ctx.coords())
+    for mode in ("twopass", "tiny"):
+        e0, g0, v0, x0 = out["staged"]
+        e1, g1, v1, x1 = out[mode]
+        print(f"lists staged vs {mode} prec={prec}: E {e0:.10f} / {e1:.10f}, max|dx| {np.abs(x1 - x0).max():.2e}",
+              flush=True)
+        assert e0 == e1 and np.array_equal(g0, g1) and np.array_equal(v0, v1) and np.array_equal(x0, x1)
 
 
 def test_nve_energy_conservation():

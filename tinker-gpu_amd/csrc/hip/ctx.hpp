@@ -171,6 +171,10 @@ struct Ctx {
   DBuf hscalars_pinned;          // host-pinned mirror
   double* h_scalars = nullptr;   // pinned, 48 doubles: CG scalars, and the list check's flag at kDispSlot
   DBuf disp_flag;                // int: the last displacement check that found an atom beyond half the buffer
+  // list build: per-part counts, the largest part count per list, and the
+  // part windows of the staged build (widths from the previous build; 0: none yet)
+  DBuf list_parts, list_pmax, stage_m, stage_u, stage_v;
+  int list_w[3] = {0, 0, 0};
   int disp_stamp = 0;
 
   // ---- PME

@@ -15,6 +15,8 @@
 // their own atom's list and write only their own atom's sums: no atomics,
 // fixed summation order, bit-reproducible results.
 #include <climits>
+#include <cstdlib>
+#include <cstring>
 
 #include "ctx.hpp"
 
@@ -162,7 +164,7 @@ __global__ void k_refresh_pos(int n, const int* __restrict__ perm, const double*
 // the vdW list (sites within rv, sites read from vpos).
 template <class real>
 struct ListArgs {
-  int n, fill, brute;
+  int n, fill, brute;  // fill: 0 count, 1 write at the counted offsets, 2 count and stage in part windows
   const PosRec<real>* pos;
   const PosRec<real>* vpos;  // null: no vdW list
   const int* cellc;
@@ -180,7 +182,8 @@ struct ListArgs {
   int *nm, *nu, *nv;                // totals (written by k_sum_parts)
   int* parts;                       // [3][kParts][n] counts per part of the cell region
   const long long *om, *ou, *ov;    // slab offsets (fill pass)
-  uint32_t *em, *eu, *ev;
+  uint32_t *em, *eu, *ev;           // list slabs (fill 1) or part windows (fill 2)
+  int wm, wu, wv;                   // part window widths (fill 2)
 };
 
 constexpr int kParts = 16;  // waves sharing one wave of atoms (one workgroup)
@@ -293,7 +296,15 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
   int lm = 0x7fffffff, lu = 0x7fffffff, lv = 0x7fffffff;
   uint32_t *om = nullptr, *ou = nullptr, *ov = nullptr;
   const size_t pn = (size_t)kParts * A.n;
-  if (A.fill && active) {
+  if (A.fill == 2 && active) {
+    // a window of wm (wu, wv) slots per atom and part; entries beyond it are
+    // counted, not written (the host then writes the lists at the counted offsets)
+    const size_t wb = (size_t)blockIdx.x * kParts + part;
+    om = A.em + wb * A.wm * 64 + lane;
+    if (has_u) ou = A.eu + wb * A.wu * 64 + lane;
+    if (has_v) ov = A.ev + wb * A.wv * 64 + lane;
+  }
+  if (A.fill == 1 && active) {
     // this part's slots follow those of the parts before it
     int pm = 0, pu = 0, pv = 0;
     for (int q = 0; q < part; ++q) {
@@ -455,8 +466,8 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
           if (A.fill) {
             const int code = px.find(oj, A.ej, A.ecode);
             const uint32_t ent = ((uint32_t)j << kCodeBits) | (uint32_t)code;
-            om[(long long)cm * 64] = ent;
-            if (in_u) ou[(long long)cu * 64] = ent;
+            if (cm < A.wm) om[(long long)cm * 64] = ent;
+            if (in_u && cu < A.wu) ou[(long long)cu * 64] = ent;
           }
           ++cm;
           cu += in_u ? 1 : 0;
@@ -467,7 +478,7 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
           if (xr * xr + yr * yr + zr * zr < A.rv2 && cv < lv) {
             if (A.fill) {
               const int code = pv.find(oj, A.vej, A.vecode);
-              ov[(long long)cv * 64] = ((uint32_t)j << kCodeBits) | (uint32_t)code;
+              if (cv < A.wv) ov[(long long)cv * 64] = ((uint32_t)j << kCodeBits) | (uint32_t)code;
             }
             ++cv;
           }
@@ -476,27 +487,106 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
       if (!have) break;
     }
   }
-  if (!A.fill && active) {
+  if (A.fill != 1 && active) {
     A.parts[(size_t)part * A.n + s] = cm;
     A.parts[pn + (size_t)part * A.n + s] = cu;
     A.parts[2 * pn + (size_t)part * A.n + s] = cv;
   }
 }
 
+// Per-atom totals of the part counts, and per wavefront the largest part
+// count of each list (wmax[3 * wave + list]; k_max3 reduces them: they size
+// the next build's part windows).
 __global__ void k_sum_parts(int n, const int* __restrict__ parts, int* __restrict__ nm, int* __restrict__ nu,
-                            int* __restrict__ nv) {
+                            int* __restrict__ nv, int* __restrict__ wmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  const bool on = s < n;  // every lane takes part in the wavefront maxima
   const size_t pn = (size_t)kParts * n;
-  int a = 0, b = 0, c = 0;
-  for (int q = 0; q < kParts; ++q) {
-    a += parts[(size_t)q * n + s];
-    b += parts[pn + (size_t)q * n + s];
-    c += parts[2 * pn + (size_t)q * n + s];
+  int a = 0, b = 0, c = 0, ma = 0, mb = 0, mc = 0;
+  if (on)
+    for (int q = 0; q < kParts; ++q) {
+      const int x = parts[(size_t)q * n + s], y = parts[pn + (size_t)q * n + s],
+                z = parts[2 * pn + (size_t)q * n + s];
+      a += x;
+      b += y;
+      c += z;
+      ma = max(ma, x);
+      mb = max(mb, y);
+      mc = max(mc, z);
+    }
+  ma = wave_max(ma);
+  mb = wave_max(mb);
+  mc = wave_max(mc);
+  const int wv = s >> 6;
+  if ((threadIdx.x & 63) == 0 && wv * 64 < n) {
+    wmax[3 * wv] = ma;
+    wmax[3 * wv + 1] = mb;
+    wmax[3 * wv + 2] = mc;
   }
+  if (!on) return;
   nm[s] = a;
   if (nu) nu[s] = b;
   if (nv) nv[s] = c;
+}
+
+// Copy the staged part windows of the workgroup's 64 atoms into their list
+// slab, parts in order: the entries land where the fill pass writes them.
+// Rows are assembled kPackRows at a time in LDS (each part drops its entries
+// at their final row) and written as whole 256-byte rows; written entry by
+// entry, one store of a wavefront would touch 64 different rows.
+constexpr int kPackRows = 128;
+__global__ __launch_bounds__(64 * kParts) void k_pack_list(int n, const int* __restrict__ parts, int w,
+                                                           const uint32_t* __restrict__ stage,
+                                                           const long long* __restrict__ off,
+                                                           uint32_t* __restrict__ out) {
+  __shared__ int cnt[kParts][64];
+  __shared__ uint32_t rows[kPackRows][64];
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const int s = blockIdx.x * 64 + lane;
+  const int mine = s < n ? parts[(size_t)part * n + s] : 0;
+  cnt[part][lane] = mine;
+  __syncthreads();
+  int before = 0, len = 0;
+  for (int q = 0; q < kParts; ++q) {
+    const int v = cnt[q][lane];
+    before += q < part ? v : 0;
+    len += v;
+  }
+  const int maxlen = wave_max(len);  // the same in every wavefront of the workgroup
+  const uint32_t* src = stage + ((size_t)blockIdx.x * kParts + part) * w * 64 + lane;
+  uint32_t* dst = out + off[blockIdx.x];
+  for (int r0 = 0; r0 < maxlen; r0 += kPackRows) {
+    const int k0 = max(0, r0 - before), k1 = min(mine, r0 + kPackRows - before);
+    for (int k = k0; k < k1; k += 8) {  // 8 loads in flight
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k + u < k1) v[u] = src[(long long)(k + u) * 64];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k + u < k1) rows[before + k + u - r0][lane] = v[u];
+    }
+    __syncthreads();
+    const int nr = min(kPackRows, maxlen - r0);
+    for (int q = threadIdx.x; q < nr * 64; q += 64 * kParts) dst[(long long)(r0 + (q >> 6)) * 64 + (q & 63)] = rows[q >> 6][q & 63];
+    __syncthreads();
+  }
+}
+
+// Largest of the per-wavefront maxima of each list (one workgroup).
+__global__ __launch_bounds__(256) void k_max3(int nw, const int* __restrict__ wmax, int* __restrict__ pmax) {
+  __shared__ int sm[3][4];
+  int m[3] = {0, 0, 0};
+  for (int w = threadIdx.x; w < nw; w += 256)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) m[q] = max(m[q], wmax[3 * w + q]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    m[q] = wave_max(m[q]);
+    if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = m[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) pmax[threadIdx.x] = max(max(sm[threadIdx.x][0], sm[threadIdx.x][1]), max(sm[threadIdx.x][2], sm[threadIdx.x][3]));
 }
 
 __global__ void k_wave_slots(int n, int nwave, const int* __restrict__ nlist, int* __restrict__ maxlen,
@@ -592,27 +682,85 @@ void build_lists(Ctx& c) {
   A.nm = c.mlist.nlist.as<int>();
   A.nu = has_u ? c.ulist.nlist.as<int>() : nullptr;
   A.nv = has_v ? c.vlist.nlist.as<int>() : nullptr;
-  DBuf parts;
-  parts.alloc(sizeof(int) * 3 * kParts * (size_t)n);
-  A.parts = parts.as<int>();
+  // One candidate scan per build: the staged pass counts every list and
+  // writes each part's entries into a window sized from the previous build;
+  // the lists are then packed in part order.  The first build, a window that
+  // overflowed, or T9HIP_LISTS=twopass count first and write at the counted
+  // offsets in a second scan (the same lists; T9HIP_LISTS=tiny forces
+  // one-entry windows, so the fallback runs).
+  const size_t pn = (size_t)kParts * n;
+  c.list_parts.alloc(sizeof(int) * 3 * pn);
+  A.parts = c.list_parts.as<int>();
+  c.list_pmax.alloc(sizeof(int) * (3 * (size_t)c.nwave + 4));  // per-wave maxima, then the 3 list maxima
+  const char* mode = std::getenv("T9HIP_LISTS");
+  const bool twopass = mode && !std::strcmp(mode, "twopass"), tiny = mode && !std::strcmp(mode, "tiny");
+  NbList* lists[3] = {&c.mlist, &c.ulist, &c.vlist};
+  DBuf* stage[3] = {&c.stage_m, &c.stage_u, &c.stage_v};
+  int w[3];
+  bool staged = !twopass;
+  for (int q = 0; q < 3; ++q) {
+    w[q] = tiny ? 1 : c.list_w[q];
+    if (lists[q]->cutoff > 0 && w[q] <= 0) staged = false;
+  }
+  const int nwg = c.nwave;
   A.fill = 0;
   A.om = A.ou = A.ov = nullptr;
   A.em = A.eu = A.ev = nullptr;
-  const int nwg = c.nwave;
+  A.wm = A.wu = A.wv = INT_MAX;
+  if (staged) {
+    A.fill = 2;
+    for (int q = 0; q < 3; ++q)
+      if (lists[q]->cutoff > 0) {
+        const size_t need = sizeof(uint32_t) * (size_t)w[q] * kParts * 64 * nwg;
+        if (need > stage[q]->bytes) stage[q]->alloc(need + need / 2);  // headroom: windows drift up and down
+      }
+    A.em = c.stage_m.as<uint32_t>();
+    A.wm = w[0];
+    if (has_u) {
+      A.eu = c.stage_u.as<uint32_t>();
+      A.wu = w[1];
+    }
+    if (has_v) {
+      A.ev = c.stage_v.as<uint32_t>();
+      A.wv = w[2];
+    }
+  }
   hipLaunchKernelGGL(k_nb_lists<real>, dim3(nwg), dim3(64 * kParts), 0, c.stream, A);
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_parts, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, A.parts, A.nm, A.nu, A.nv);
+  int* wmax = c.list_pmax.as<int>();
+  int* pmax = wmax + 3 * (size_t)nwg;
+  hipLaunchKernelGGL(k_sum_parts, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, A.parts, A.nm, A.nu, A.nv,
+                     wmax);
   T9_LAUNCH_CHECK();
-  // the three totals come back in one readback (pinned slots 45..47)
+  hipLaunchKernelGGL(k_max3, dim3(1), dim3(256), 0, c.stream, nwg, wmax, pmax);
+  T9_LAUNCH_CHECK();
+  // the totals and the largest part counts come back in one readback
+  // (pinned slots 45..47 and 42..43)
   long long* h_tot = reinterpret_cast<long long*>(c.h_scalars + 45);
-  NbList* lists[3] = {&c.mlist, &c.ulist, &c.vlist};
+  int* h_pmax = reinterpret_cast<int*>(c.h_scalars + 42);
   for (int q = 0; q < 3; ++q)
     if (lists[q]->cutoff > 0) list_slots(c, *lists[q], h_tot + q);
+  T9_HIP(hipMemcpyAsync(h_pmax, pmax, sizeof(int) * 3, hipMemcpyDeviceToHost, c.stream));
   mark_readback(c);
   wait_marked(c);
+  bool overflow = false;
   for (int q = 0; q < 3; ++q)
-    if (lists[q]->cutoff > 0) list_capacity(*lists[q], h_tot[q]);
+    if (lists[q]->cutoff > 0) {
+      list_capacity(*lists[q], h_tot[q]);
+      if (h_pmax[q] > w[q]) overflow = true;
+      c.list_w[q] = h_pmax[q] + h_pmax[q] / 4 + 8;  // the next build's window
+    }
+  if (staged && !overflow) {
+    for (int q = 0; q < 3; ++q)
+      if (lists[q]->cutoff > 0) {
+        hipLaunchKernelGGL(k_pack_list, dim3(nwg), dim3(64 * kParts), 0, c.stream, n, A.parts + q * pn, w[q],
+                           stage[q]->as<uint32_t>(), lists[q]->off.as<long long>(), lists[q]->entries.as<uint32_t>());
+        T9_LAUNCH_CHECK();
+      }
+    return;
+  }
   A.fill = 1;
+  A.wm = A.wu = A.wv = INT_MAX;
   A.om = c.mlist.off.as<long long>();
   A.em = c.mlist.entries.as<uint32_t>();
   if (has_u) {
