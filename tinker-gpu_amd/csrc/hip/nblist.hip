@@ -70,7 +70,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const T* __restrict__ in, T
   const int per = (m + kScanBlock - 1) / kScanBlock;
   const int b = min(m, (int)threadIdx.x * per), e = min(m, b + per);
   T sum = 0;
-  for (int t = b; t < e; ++t) sum += in[t];
+  for (int t0 = b; t0 < e; t0 += 8) {  // 8 loads in flight
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = t0 + u < e ? in[t0 + u] : (T)0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sum += v[u];
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   T incl = sum;
 #pragma unroll
@@ -82,10 +88,16 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const T* __restrict__ in, T
   __syncthreads();
   T run = incl - sum;
   for (int u = 0; u < w; ++u) run += wsum[u];
-  for (int t = b; t < e; ++t) {
-    const T c = in[t];
-    out[t] = run;
-    run += c;
+  for (int t0 = b; t0 < e; t0 += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = t0 + u < e ? in[t0 + u] : (T)0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u < e) {
+        out[t0 + u] = run;
+        run += v[u];
+      }
   }
   if (with_total && threadIdx.x == kScanBlock - 1) out[m] = run;
 }
@@ -872,14 +884,23 @@ __global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
     const int kb = k0 + part * C;
     const int ke = min(kb + C, len);
     unsigned long long ma = 0, mb = 0;
-    for (int k = kb; k < ke; ++k) {
-      const uint32_t ent = e[(long long)k * 64];
-      const auto pj = pos[ent >> kCodeBits];
-      real xr, yr, zr;
-      pair_disp(pi, pj, box, xr, yr, zr);
-      const real r2 = xr * xr + yr * yr + zr * zr;
-      if (r2 <= cut2) ma |= 1ull << (k - kb);
-      if (two && r2 <= cut2b) mb |= 1ull << (k - kb);
+    for (int k0 = kb; k0 < ke; k0 += 8) {  // 8 entries, then their 8 partners, in flight
+      uint32_t ent[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ent[u] = k0 + u < ke ? e[(long long)(k0 + u) * 64] : 0u;
+      PosRec<real> pj[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < ke) pj[u] = pos[ent[u] >> kCodeBits];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < ke) {
+          real xr, yr, zr;
+          pair_disp(pi, pj[u], box, xr, yr, zr);
+          const real r2 = xr * xr + yr * yr + zr * zr;
+          if (r2 <= cut2) ma |= 1ull << (k0 + u - kb);
+          if (two && r2 <= cut2b) mb |= 1ull << (k0 + u - kb);
+        }
     }
     cnt_a[part][lane] = (unsigned short)__popcll(ma);
     cnt_b[part][lane] = (unsigned short)__popcll(mb);
