@@ -93,11 +93,15 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   for (int d = 0; d < 3; ++d) c.grid[d] = s.opt.pme_grid[d];
   T9_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   T9_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
-  T9_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
-  T9_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  // fork/join events order kernels of one device: the producer kernel's own
+  // release already makes its writes visible to the consumer, so they skip
+  // the host-scope fence (the readback event keeps it)
+  const unsigned evf = hipEventDisableTiming | hipEventDisableSystemFence;
+  T9_HIP(hipEventCreateWithFlags(&c.ev_fork, evf));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_join, evf));
   T9_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
-  T9_HIP(hipEventCreateWithFlags(&c.ev_ufork, hipEventDisableTiming));
-  T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin, hipEventDisableTiming));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_ufork, evf));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin, evf));
   T9_HIP(hipEventCreateWithFlags(&c.ev_read, hipEventDisableTiming));
   hipStream_t st = c.stream;
   const int n = s.n;
