@@ -65,6 +65,7 @@ void alloc_vectors(Ctx& c) {
     T9_HIP(hipMemsetAsync(b->p, 0, vb, c.stream));
   }
   c.partials.alloc(sizeof(double) * 4 * (ceil_div(c.n, 256) + 1));
+  c.partials64.alloc(sizeof(double) * 4 * (ceil_div(c.n, 64) + 1));
   c.pcg_trace.alloc(sizeof(double) * kTraceCap);
   c.scalars.alloc(sizeof(double) * 48);
   T9_HIP(hipMemsetAsync(c.scalars.p, 0, sizeof(double) * 48, c.stream));

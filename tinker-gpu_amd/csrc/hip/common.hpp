@@ -249,6 +249,19 @@ constexpr int kPairParts = 8;
 // can enqueue several iterations without reading the residual back.
 constexpr int kDoneSlot = 15;
 constexpr int kTraceCap = 256;  // per-iteration residual trace of the last CG solve
+// a . b over the 3 components of element s, in FP64 (CG dot products)
+template <class real>
+__device__ __forceinline__ double dot3(const real* __restrict__ a, const real* __restrict__ b, int s) {
+  double v = 0;
+  for (int d = 0; d < 3; ++d) v += (double)a[3 * s + d] * (double)b[3 * s + d];
+  return v;
+}
+// sum over the 64 lanes of a wavefront by a fixed butterfly (every lane gets it)
+__device__ __forceinline__ double wave_bfly_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
 constexpr int kDispSlot = 44;   // pinned readback slot of the list check's flag (40: kinetic energy; 45-47: list totals)
 struct Gate {
   const double* sc = nullptr;  // CG scalars; nullptr: no gate

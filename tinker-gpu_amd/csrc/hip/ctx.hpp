@@ -170,6 +170,7 @@ struct Ctx {
   DBuf scalars;                  // double [32] on device
   DBuf hscalars_pinned;          // host-pinned mirror
   double* h_scalars = nullptr;   // pinned, 48 doubles: CG scalars, and the list check's flag at kDispSlot
+  DBuf partials64;               // double [4 * ceil(n / 64)]: CG dots per 64 elements (k_precond)
   DBuf disp_flag;                // int: the last displacement check that found an atom beyond half the buffer
   // list build: per-part counts, the largest part count per list, and the
   // part windows of the staged build (widths from the previous build; 0: none yet)
@@ -316,8 +317,9 @@ void adam_step(Ctx& c, const float* grad_dev, double lr);
 void guess_capture_start(Ctx& c, int stride, long long max_samples);
 void guess_fit_norm(Ctx& c);
 void guess_train(Ctx& c, int epochs, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+// dots64: also r.z (d, p) and r.r (d, p) per 64 elements (4 doubles each), or null
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d = nullptr,
-                   void* z2p = nullptr);  // z (and a second copy of z when given)
+                   void* z2p = nullptr, double* dots64 = nullptr);  // z (and a second copy of z when given)
 // spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
 // 2: multipoles src1 (real) + dipoles src2 (imaginary)
 void pme_spread(Ctx& c, int mode, const void* s1, const void* s2);

@@ -263,7 +263,8 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
                                                  int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                  const real* __restrict__ rd, const real* __restrict__ rp,
                                                  real* __restrict__ zd, real* __restrict__ zp,
-                                                 real* __restrict__ z2d, real* __restrict__ z2p, Gate gate) {
+                                                 real* __restrict__ z2d, real* __restrict__ z2p,
+                                                 double* __restrict__ dots64, Gate gate) {
   if (gate_closed(gate)) return;
   __shared__ double sh[(kPairParts - 1) * 6 * 64];
   const int part = threadIdx.x >> 6;
@@ -335,6 +336,22 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
       z2p[3 * s + 2] = (real)b2;
     }
   }
+  if (dots64 && part == 0) {
+    // r.z (d, p) and r.r (d, p) of this workgroup's 64 elements: the same
+    // per-element products and wavefront butterfly as the CG dot kernel
+    double v[4] = {0, 0, 0, 0};
+    if (s < n) {
+      v[0] = dot3(rd, (const real*)zd, s);
+      v[1] = dot3(rp, (const real*)zp, s);
+      v[2] = dot3(rd, rd, s);
+      v[3] = dot3(rp, rp, s);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double x = wave_bfly_sum(v[q]);
+      if ((threadIdx.x & 63) == 0) dots64[blockIdx.x * 4 + q] = x;
+    }
+  }
 }
 
 
@@ -382,7 +399,7 @@ void dfield_real_t(Ctx& c, hipStream_t st) {
 }
 
 template <class real>
-void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p) {
+void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p, double* dots64) {
   const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.mlist.cut2nd > 0;
   const auto& L = c.mlist;
   if (sparse) require_cut(L);
@@ -392,7 +409,7 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void*
                      (real)c.opt.udiag, L.ncut2.as<int>(), L.off.as<long long>(), L.ecut2.as<uint32_t>(),
                      c.pos.as<R4<real>>(), c.polarity.as<real>(), c.jpolar.as<int>(), c.thlval.as<real>(), c.nj,
                      (real)c.thole_value, c.scale, make_boxT<real>(c), (const real*)rd, (const real*)rp, (real*)zd,
-                     (real*)zp, (real*)z2d, (real*)z2p, c.gate);
+                     (real*)zp, (real*)z2d, (real*)z2p, dots64, c.gate);
   T9_LAUNCH_CHECK();
 }
 
@@ -430,9 +447,10 @@ void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {
   pme_dipole_field(c, ud, up, fd, fp, c.ev_ujoin);
 }
 
-void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p) {
-  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp, z2d, z2p);
-  else precond_t<double>(c, rd, rp, zd, zp, z2d, z2p);
+void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p,
+                   double* dots64) {
+  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp, z2d, z2p, dots64);
+  else precond_t<double>(c, rd, rp, zd, zp, z2d, z2p, dots64);
 }
 
 }  // namespace t9

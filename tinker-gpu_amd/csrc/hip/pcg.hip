@@ -141,9 +141,7 @@ __device__ __forceinline__ void block_sums(double (&v)[NV]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double x = v[q];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    const double x = wave_bfly_sum(v[q]);
     if (lane == 0) sh[w][q] = x;
   }
   __syncthreads();
@@ -172,6 +170,26 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
   block_sums<NV>(t);
 }
 
+// The same totals from partials written per 64 elements (one wavefront's
+// butterfly sum each, by k_precond): each group of 4 is combined as
+// (w0 + w1) + (w2 + w3), the sum block_sums forms in a 256-thread workgroup,
+// so the totals are those of k_dots' partials, bit for bit.
+template <int NV>
+__device__ __forceinline__ void reduce_partials64(const double* __restrict__ part, int nblk64, double (&t)[NV]) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) t[q] = 0;
+  const int ng = (nblk64 + 3) / 4;
+  for (int b = threadIdx.x; b < ng; b += kB)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = 4 * b + u < nblk64 ? part[(4 * b + u) * 4 + q] : 0.0;
+      t[q] += (w[0] + w[1]) + (w[2] + w[3]);
+    }
+  block_sums<NV>(t);
+}
+
 __device__ __forceinline__ double sum_slot(int it, int dp) { return 2 * (it & 1) + dp; }
 
 // NV (2 or 4) dot products a_q . b_q -> partials
@@ -187,22 +205,20 @@ __global__ __launch_bounds__(kB) void k_dots(int n, const real* __restrict__ a1,
   const real* Bv[4] = {b1, b2, b3, b4};
   double v[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    v[q] = 0;
-    if (s < n)
-      for (int d = 0; d < 3; ++d) v[q] += (double)A[q][3 * s + d] * (double)Bv[q][3 * s + d];
-  }
+  for (int q = 0; q < NV; ++q) v[q] = s < n ? dot3(A[q], Bv[q], s) : 0.0;
   write_partials<NV>(v, part);
 }
 
 // Start of a solve, one workgroup.  mode 1: partials hold r.z (d, p) and r.r
-// (d, p): SUM[parity] = r.z, initial RMS residual from r.r; mode 0: partials
-// hold r.r only (the residual of a trial guess).
+// (d, p), per 64 elements (k_precond): SUM[parity] = r.z, initial RMS residual
+// from r.r; mode 0: partials hold r.r only (k_dots, the residual of a trial
+// guess).
 __global__ __launch_bounds__(kB) void k_cg_start(int nblk, int mode, int parity, int n, double debye,
                                                 const double* __restrict__ part, double* __restrict__ sc,
                                                 double* __restrict__ trace) {
   double t[4];
-  reduce_partials<4>(part, nblk, t);
+  if (mode) reduce_partials64<4>(part, nblk, t);
+  else reduce_partials<4>(part, nblk, t);
   if (threadIdx.x != 0) return;
   const double rr_d = mode ? t[2] : t[0], rr_p = mode ? t[3] : t[1];
   const double eps0 = debye * sqrt(fmax(rr_d, rr_p) / n);
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(kB) void k_p3(int n, int nblk, const double* __rest
   if (gate_closed(gate)) return;
   const int it = gate.iter;
   double t[4];
-  reduce_partials<4>(part, nblk, t);
+  reduce_partials64<4>(part, nblk, t);  // r.z and r.r, per 64 elements from k_precond
   const double sd = sc[(int)sum_slot(it, 0)], sp = sc[(int)sum_slot(it, 1)];
   const real b = (real)(sd != 0 ? t[0] / sd : 0), bp = (real)(sp != 0 ? t[1] / sp : 0);
   const double eps = debye * sqrt(fmax(t[2], t[3]) / n);
@@ -319,7 +335,8 @@ __global__ void k_set_scalar(double* sc, int slot, double v) {
 template <class real>
 int induce_t(Ctx& c, int* iters, double* rms) {
   const int n = c.n;
-  const int nblk = ceil_div(n, kB);
+  const int nblk = ceil_div(n, kB), nblk64 = ceil_div(n, 64);
+  double* part64 = c.partials64.as<double>();
   const dim3 G(nblk), Bk(kB);
   hipStream_t st = c.stream;
   real* field = c.field.as<real>();
@@ -422,9 +439,9 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   // the residual norm of the initial guess (Debye, read back with the first
   // iteration)
   auto start_cg = [&]() {
-    precond_apply(c, rd, rp, zd, zp, cd, cp);  // z, and the first search direction p = z
-    hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, zd, rp, zp, rd, rd, rp, rp, part, Gate());
-    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk, 1, (iter0 + 1) & 1, n, debye, part, sc, trace);
+    // z, the first search direction p = z, and r.z, r.r per 64 elements
+    precond_apply(c, rd, rp, zd, zp, cd, cp, part64);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk64, 1, (iter0 + 1) & 1, n, debye, part64, sc, trace);
     T9_LAUNCH_CHECK();
   };
   reset_done();
@@ -443,9 +460,8 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     hipLaunchKernelGGL(k_p2<real>, G, Bk, 0, st, n, nblk, part, sc, c.polarity.as<real>(), cd, cp, vd, vp, ud, up,
                        rd, rp, g);
     T9_LAUNCH_CHECK();
-    precond_apply(c, rd, rp, zd, zp);
-    hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, zd, rp, zp, rd, rd, rp, rp, part, g);
-    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, nblk, part, sc, c.opt.polar_eps, iter0 + miniter,
+    precond_apply(c, rd, rp, zd, zp, nullptr, nullptr, part64);  // z, and r.z, r.r per 64 elements
+    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, nblk64, part64, sc, c.opt.polar_eps, iter0 + miniter,
                        c.opt.polar_iter, debye, trace, zd, zp, cd, cp, (real)c.opt.pcg_peek, c.polarity.as<real>(),
                        rd, rp, ud, up, g);
     T9_LAUNCH_CHECK();
