@@ -568,23 +568,18 @@ __global__ __launch_bounds__(64 * kParts) void k_pack_list(int n, const int* __r
   const uint32_t* src = stage + ((size_t)blockIdx.x * kParts + part) * w * 64 + lane;
   uint32_t* dst = out + off[blockIdx.x];
   for (int r0 = 0; r0 < maxlen; r0 += kPackRows) {
-    // this lane's window rows [k0, k1) land in the chunk; the wavefront (one
-    // part) walks the union of its lanes' ranges together, so each load reads
-    // one whole 256-byte row of the window
+    // this lane's window rows [k0, k1) land in the chunk (walking the union
+    // of the wavefront's ranges row by row was measured slower: the ranges
+    // are far apart)
     const int k0 = max(0, r0 - before), k1 = min(mine, r0 + kPackRows - before);
-    int klo = k1 > k0 ? k0 : 0x7fffffff, khi = k1 > k0 ? k1 : 0;
-    for (int o = 32; o > 0; o >>= 1) {
-      klo = min(klo, __shfl_xor(klo, o, 64));
-      khi = max(khi, __shfl_xor(khi, o, 64));
-    }
-    for (int k = klo; k < khi; k += 8) {  // 8 rows in flight
+    for (int k = k0; k < k1; k += 8) {  // 8 loads in flight
       uint32_t v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (k + u < khi) v[u] = src[(long long)(k + u) * 64];
+        if (k + u < k1) v[u] = src[(long long)(k + u) * 64];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (k + u >= k0 && k + u < k1) rows[before + k + u - r0][lane] = v[u];
+        if (k + u < k1) rows[before + k + u - r0][lane] = v[u];
     }
     __syncthreads();
     const int nr = min(kPackRows, maxlen - r0);
