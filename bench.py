@@ -156,8 +156,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the sharded multi-GPU form is described, not built", file=sys.stderr)
         sys.exit(2)
 
     import t9hip
