@@ -11,7 +11,12 @@ network trained on the solver's own MD data.
   GPU's own coordinates, fields and histories, 1e-5 of each column's scale.
 * Training: 216 waters, 150 MD steps of captured data, then 40 MD steps with
   the learned guess against 40 steps of ASPC from the same state: fewer PCG
-  iterations per step, every solve still converged to polar-eps."""
+  iterations per step, every solve still converged to polar-eps.
+* Fallback: a network whose guess is far worse than ASPC is rejected at the
+  solve's readback; the solve restarts from the ASPC guess, so the trajectory
+  is the ASPC-only trajectory, bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -140,3 +145,21 @@ def test_learned_guess_cuts_iterations():
     assert l1 < 0.8 * l0
     assert st["used"] > 0
     assert ia.mean() < ib.mean()
+
+
+def test_bad_learned_guess_falls_back_to_aspc():
+    A = t9hip.Context(_box(), t9hip.T9_FP32)
+    B = t9hip.Context(_box(), t9hip.T9_FP32)
+    for c in (A, B):
+        c.md_init(298.0, 12)
+        c.md_run(20, dt=0.002, nrespa=8, bath_kelvin=298.0, tau=0.1)  # fills the 16-deep histories
+    w = np.load(os.path.join(os.path.dirname(t9hip.__file__), "guess_water.npz"))
+    A.guess_set_params(w["params"] * 1000.0, w["norm"])  # the shipped network, scaled out of range
+    A.guess_enable(True)
+    for c in (A, B):
+        c.md_run(4, dt=0.002, nrespa=8, bath_kelvin=298.0, tau=0.1)
+    st = A.guess_stats()
+    dx = np.abs(A.coords() - B.coords()).max()
+    print(f"bad guess: learned solves {st['used']}, fallbacks {st['fallback']}, max|dx| vs ASPC {dx:.2e}", flush=True)
+    assert st["fallback"] >= 4 and st["used"] == 0
+    assert np.array_equal(A.coords(), B.coords()) and np.array_equal(A.velocities(), B.velocities())
