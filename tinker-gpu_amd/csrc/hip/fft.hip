@@ -1,9 +1,10 @@
 // 3D FFT of the PME grid for gfx950 (replaces cuFFT C2C in place,
 // src/cudart/fft.cpp:16-99).
 //
-// Five passes per reciprocal-space round trip: x forward, y forward,
+// Line passes, five per reciprocal-space round trip: x forward, y forward,
 // z forward + influence function + z inverse (one kernel, data stays in
-// LDS), y inverse, x inverse.  Each workgroup loads 16 (FP32) or 8 (FP64)
+// LDS), y inverse, x inverse; FP32 square planes take the x and y passes in
+// one plane kernel each way (three grid passes).  Each workgroup loads 16 (FP32) or 8 (FP64)
 // lines — contiguous lines for x, lines at consecutive x for y and z so every
 // global row access is a contiguous 128-byte segment — transforms them in
 // LDS with the Stockham stages of fft_core.hpp and writes them back.

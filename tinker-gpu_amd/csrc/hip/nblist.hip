@@ -193,7 +193,7 @@ struct ListArgs {
   const int *veoff, *vej, *vecode;  // scaled partners of the vdW list
   int *nm, *nu, *nv;                // totals (written by k_sum_parts)
   int* parts;                       // [3][kParts][n] counts per part of the cell region
-  const long long *om, *ou, *ov;    // slab offsets (fill pass)
+  const long long *om, *ou, *ov;    // slab offsets (fill 1)
   uint32_t *em, *eu, *ev;           // list slabs (fill 1) or part windows (fill 2)
   int wm, wu, wv;                   // part window widths (fill 2)
 };
@@ -542,7 +542,7 @@ __global__ void k_sum_parts(int n, const int* __restrict__ parts, int* __restric
 }
 
 // Copy the staged part windows of the workgroup's 64 atoms into their list
-// slab, parts in order: the entries land where the fill pass writes them.
+// slab, parts in order: the entries land where a count-then-write build puts them.
 // Rows are assembled kPackRows at a time in LDS (each part drops its entries
 // at their final row) and written as whole 256-byte rows; written entry by
 // entry, one store of a wavefront would touch 64 different rows.
@@ -635,7 +635,7 @@ __global__ void k_max_disp(int n, const double* __restrict__ xyz, const double* 
 }
 
 
-// Slab offsets and total of one list after the count pass; the total is
+// Slab offsets and total of one list once its counts are known; the total is
 // copied to pinned memory at `h_total` (the host waits once for all lists,
 // then list_capacity sizes the slabs).
 void list_slots(Ctx& c, NbList& L, long long* h_total) {
@@ -662,7 +662,7 @@ void list_capacity(NbList& L, long long total) {
   L.total = total;
 }
 
-// All lists in one count pass and one fill pass over the candidate cells.
+// All three lists from the candidate cells (staged single scan, or count then write).
 template <class real>
 void build_lists(Ctx& c) {
   const int n = c.n;
