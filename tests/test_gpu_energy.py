@@ -133,17 +133,19 @@ def test_water96k_fp32_vs_fp64():
     assert rel < 1e-6 and grms < 1e-5 and vrel < 1e-4
 
 
-@pytest.mark.parametrize("case", ["water216", "water216-k32", "water1000+ions", "water96k"])
+@pytest.mark.parametrize("case", ["water216", "water216-k32", "water216-k48", "water216-k64", "water216-k80",
+                                  "water216-k96", "water216-k128", "water1000+ions", "water96k"])
 def test_fft_plane_matches_lines(case, monkeypatch):
     """FP32 x and y transforms per z-plane in LDS against the line passes
     (T9HIP_FFT=lines): every butterfly is the same arithmetic, so energy,
-    gradient and virial are the same bits (PME 24^3: radices 8, 3; 32^3: 8, 4;
-    40^3: 8, 5; 120^3: 8, 3, 5)."""
+    gradient and virial are the same bits.  Every plane length with a kernel:
+    24 (radices 8, 3), 32 (8, 4), 40 (8, 5), 48 (8, 2, 3), 64 (8, 8), 80
+    (8, 2, 5), 96 (8, 4, 3), 120 (8, 3, 5), 128 (8, 8, 2)."""
     print(f"case fft plane vs lines {case}", flush=True)
     if case == "water216":
         s = water(216, 6, 18.643, 24, buf=1.5)
-    elif case == "water216-k32":
-        s = water(216, 6, 18.643, 32, buf=1.5)
+    elif case.startswith("water216-k"):
+        s = water(216, 6, 18.643, int(case.split("-k")[1]), buf=1.5)
     elif case == "water1000+ions":
         s = water(1000, 11, 32.6, 40, seed=7, buf=1.0, n_na=20, n_cl=20)
     else:
