@@ -122,18 +122,25 @@ def test_respa_matches_oracle_scheme():
 
 
 @pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
-def test_respa_inner_paths_agree(prec, monkeypatch):
+@pytest.mark.parametrize("ions", [0, 4])
+def test_respa_inner_paths_agree(prec, ions, monkeypatch):
     out = {}
     for path in ("split", "thread", "group"):
         monkeypatch.setenv("T9HIP_RESPA", path)
-        ctx = t9hip.Context(_box("vdwterm\nbondterm\n", eps=1e-8), prec)
+        if ions:  # one-atom molecules without valence terms beside the waters
+            s = t9hip.System.water_box(216 - 2 * ions, 6, 18.643, 11, ions, ions,
+                                       key=BASE.format(eps=1e-8) + "vdwterm\nbondterm\n")
+        else:
+            s = _box("vdwterm\nbondterm\n", eps=1e-8)
+        ctx = t9hip.Context(s, prec)
         ctx.md_init(298.0, 6)
         ctx.md_run(4, dt=0.002, nrespa=8)
         out[path] = (ctx.coords(), ctx.velocities())
     for path in ("thread", "group"):
         dx = np.abs(out[path][0] - out["split"][0]).max()
         dv = np.abs(out[path][1] - out["split"][1]).max()
-        print(f"respa inner {path} vs split prec={prec}: max|dx|={dx:.2e} A max|dv|={dv:.2e} A/ps", flush=True)
+        print(f"respa inner {path} vs split prec={prec} ions={ions}: max|dx|={dx:.2e} A max|dv|={dv:.2e} A/ps",
+              flush=True)
         assert dx == 0 and dv == 0
 
 
