@@ -145,7 +145,8 @@ def test_respa_inner_paths_agree(prec, ions, monkeypatch):
 
 
 @pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
-def test_list_build_modes_agree(prec, monkeypatch):
+@pytest.mark.parametrize("ions", [0, 4])
+def test_list_build_modes_agree(prec, ions, monkeypatch):
     """Neighbor-list builds: the staged single scan (every build after a
     context's first), the count-then-write build (T9HIP_LISTS=twopass) and the
     staged scan whose part windows overflow and fall back (T9HIP_LISTS=tiny)
@@ -157,7 +158,12 @@ def test_list_build_modes_agree(prec, monkeypatch):
             monkeypatch.delenv("T9HIP_LISTS", raising=False)
         else:
             monkeypatch.setenv("T9HIP_LISTS", mode)
-        ctx = t9hip.Context(_box("vdwterm\nbondterm\n", eps=1e-8), prec)
+        if ions:
+            s = t9hip.System.water_box(216 - 2 * ions, 6, 18.643, 11, ions, ions,
+                                       key=BASE.format(eps=1e-8) + "vdwterm\nbondterm\n")
+        else:
+            s = _box("vdwterm\nbondterm\n", eps=1e-8)
+        ctx = t9hip.Context(s, prec)
         ctx.nblist_build()
         ctx.nblist_build()
         e, g, v = ctx.energy()
@@ -167,8 +173,8 @@ def test_list_build_modes_agree(prec, monkeypatch):
     for mode in ("twopass", "tiny"):
         e0, g0, v0, x0 = out["staged"]
         e1, g1, v1, x1 = out[mode]
-        print(f"lists staged vs {mode} prec={prec}: E {e0:.10f} / {e1:.10f}, max|dx| {np.abs(x1 - x0).max():.2e}",
-              flush=True)
+        print(f"lists staged vs {mode} prec={prec} ions={ions}: E {e0:.10f} / {e1:.10f}, "
+              f"max|dx| {np.abs(x1 - x0).max():.2e}", flush=True)
         assert e0 == e1 and np.array_equal(g0, g1) and np.array_equal(v0, v1) and np.array_equal(x0, x1)
 
 
