@@ -110,6 +110,12 @@ int t9_nblist_build(t9_ctx* ctx);
 /* List statistics: [0] list pairs (directed), [1] atoms, [2] waves,
  * [3] padded slots, [4] builds so far. */
 int t9_nblist_stats(t9_ctx* ctx, int64_t* out5);
+/* The pairs stored in a neighbor list (which 0: multipole/polarization list
+ * at cutoff + buffer, 1: vdW-site list), each once as caller-order indices
+ * (i, k), i < k, into pairs[cap][2]; *count = number of pairs (may exceed
+ * cap: then only the first cap are written).  The readback the reference's
+ * neighbor-list test takes of nlst / lst (test/nblist.cpp:160-185). */
+int t9_nblist_pairs(t9_ctx* ctx, int which, int64_t cap, int* pairs, int64_t* count);
 
 /* Local frame -> global multipoles.  Replaces mpoleInit's chkpole/rotpole
  * (src/amoeba/mpole.cpp:28-58, src/cu/amoeba/rotpole.cu:7-42). */
@@ -174,6 +180,69 @@ typedef struct {
   int64_t nem, nep, nev;
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
+/* Term selection of one energy call, as energy_core selects its terms by the
+ * calc flags and the RESPA time-scale group (src/energy.cpp:176-291):
+ * T9_TERM_ELEC the multipole + polarization pair (emplar, with the induced
+ * dipole solve), T9_TERM_VDW the buffered 14-7 vdW (evdw), T9_TERM_VALENCE
+ * bond / angle / Urey-Bradley (ebond, eangle, eurey).  A term the system's
+ * keywords switch off stays off.  t9_energy is all three.  A host that
+ * keeps its own vdW and valence terms asks for T9_TERM_ELEC only. */
+enum { T9_TERM_ELEC = 1, T9_TERM_VDW = 2, T9_TERM_VALENCE = 4, T9_TERM_ALL = 7 };
+int t9_energy_terms(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* grad);
+
+/* Device-resident coordinates and gradient: d_xyz and d_grad are device
+ * pointers to [n][3] FP64 in caller order (the reference keeps x/y/z and the
+ * gradient on the device: copyPosToXyz, src/atom.cpp:153-166, and the
+ * gx/gy/gz accumulators read by sumGradient, src/energy.cpp:319-448).  No
+ * PCIe transfer of per-atom data; d_grad receives dE/dx of the selected
+ * terms when vers has T9_CALC_GRAD. */
+int t9_set_coords_device(t9_ctx* ctx, const double* d_xyz);
+int t9_get_coords_device(t9_ctx* ctx, double* d_xyz);
+int t9_energy_device(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* d_grad);
+
+/* Preconditioners of the PCG solve on caller-order residuals [n][3]
+ * (d and p): z = alpha r (replaces diagPrecond, include/ff/amoeba/induce.h:23,
+ * src/cu/diagprecond.cu:5-42), and the sparse form
+ * z = udiag alpha r + sum_{r_ik < usolve-cutoff} alpha_i alpha_k T_ik r_k
+ * with Thole damping (replaces sparsePrecondApply,
+ * include/ff/amoeba/induce.h:25-29, src/cu/amoeba/precond.cu:12-43;
+ * T9_ERR_ARG when pcg-precond or usolve-cutoff leave it off). */
+int t9_diag_precond(t9_ctx* ctx, const double* rsd, const double* rsdp, double* zrsd, double* zrsdp);
+int t9_sparse_precond_apply(t9_ctx* ctx, const double* rsd, const double* rsdp, double* zrsd, double* zrsdp);
+
+/* PME sub-steps on the context's complex grid (Ewald systems only; the
+ * reference's PMEUnit calls, include/ff/pme.h:64-91).  grid, when non-NULL,
+ * is a host array [K3][K2][K1][2] FP64 (real, imaginary; x fastest): the
+ * spreading calls copy the grid out, the transforms and the convolution
+ * read it in and write the result back, the gathers read it in; NULL keeps
+ * the grid on the device between calls.
+ *  - t9_pme_grid_mpole: B-spline spreading of the global multipoles in
+ *    fractional form, real part (gridMpole, src/cu/pme.cu:185-255);
+ *  - t9_pme_grid_uind: fractional dipoles, d real and p imaginary (gridUind);
+ *  - t9_pme_fft_front / t9_pme_fft_back: unnormalised forward (exponent -1)
+ *    and inverse (+1) 3D transforms in place (fftfront / fftback,
+ *    src/cudart/fft.cpp:16-99);
+ *  - t9_pme_conv: the influence function exp(-pi^2 h^2/a^2) /
+ *    (pi V h^2 bsmod1 bsmod2 bsmod3), k = 0 and exponents below -50 zeroed
+ *    (pmeConv, src/cu/pme.cu:908-1007);
+ *  - t9_pme_fphi_mpole / t9_pme_fphi_uind: the Cartesian field -grad phi of
+ *    the grid at each atom [n][3], caller order, without the self term (the
+ *    field part of fphiMpole / fphiUind + fphiToCphi, src/cu/pme.cu:334-380). */
+int t9_pme_grid_mpole(t9_ctx* ctx, double* grid);
+int t9_pme_grid_uind(t9_ctx* ctx, const double* uind, const double* uinp, double* grid);
+int t9_pme_fft_front(t9_ctx* ctx, double* grid);
+int t9_pme_fft_back(t9_ctx* ctx, double* grid);
+int t9_pme_conv(t9_ctx* ctx, double* grid);
+int t9_pme_fphi_mpole(t9_ctx* ctx, const double* grid, double* field);
+int t9_pme_fphi_uind(t9_ctx* ctx, const double* grid, double* fd, double* fp);
+
+/* Context-free unnormalised 3D complex transform in place of a device grid
+ * [k3][k2][k1] of interleaved complex T9_FP32 or T9_FP64 values, sign -1
+ * forward, +1 inverse, on the current device's null stream (returns when
+ * done).  Lengths: products of 2, 3 and 5 up to 255.
+ * Replaces the cuFFT plan + cufftExecC2C/Z2Z pair of the reference's FFT
+ * unit (src/cudart/fft.cpp:16-99); allocates its twiddle table per call. */
+int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign);
 
 /* ------------------------------------------------------- learned guess */
 /* A learned initial guess for the induced dipoles.  No counterpart in the
@@ -246,6 +315,13 @@ typedef struct {
  * BussiThermostat::control2 -> bussiThermostat (src/md/thermostat.cpp:77-82,
  * src/mdpt.cpp:41-71). */
 int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin, double tau, t9_md_stats* stats);
+
+/* One Bussi velocity rescaling with given deviates: r standard normal, s2
+ * chi-square with nfree - 1 degrees of freedom (t9_md_run draws them itself).
+ * The kernel-level step of bussiThermostat (src/mdpt.cpp:41-71): scale =
+ * sqrt(c + (s2 + r^2) d + 2 r sqrt(c d)), c = exp(-dt/tau),
+ * d = (1 - c) (kelvin / T) / nfree, negative when r + sqrt(c/d) < 0. */
+int t9_md_bussi(t9_ctx* ctx, double dt, double tau, double kelvin, double r, double s2);
 
 /* Wait for all work queued on the context's stream. */
 int t9_sync(t9_ctx* ctx);

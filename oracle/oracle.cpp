@@ -843,6 +843,67 @@ double orc_recip_mpole(void* h, double* field, double* fphi) {
   return e;
 }
 
+// PME sub-steps on interleaved complex grids [K3][K2][K1][2]: spreading of
+// the fractional multipoles (mode 0, real part) or fractional dipoles (mode
+// 1, d real and p imaginary); the influence function in place; the Cartesian
+// field -grad phi of a grid's real (or imaginary) part at each atom.
+void orc_pme_grid(void* h, int mode, const double* ud, const double* up, double* grid) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  auto sp = make_splines(o);
+  std::vector<cplx> g;
+  if (mode == 0) {
+    std::vector<double> fmp(10 * o.n);
+    cart_to_frac_mpole(o, o.rpole.data(), fmp.data());
+    spread(o, sp, fmp.data(), nullptr, nullptr, g);
+  } else {
+    double a[3];
+    frac_scale(o, a);
+    std::vector<double> fud(3 * o.n), fup(3 * o.n);
+    for (int i = 0; i < o.n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        fud[3 * i + d] = ud[3 * i + d] * a[d];
+        fup[3 * i + d] = up[3 * i + d] * a[d];
+      }
+    spread(o, sp, nullptr, fud.data(), fup.data(), g);
+  }
+  for (size_t i = 0; i < g.size(); ++i) {
+    grid[2 * i] = g[i].real();
+    grid[2 * i + 1] = g[i].imag();
+  }
+}
+
+void orc_pme_conv(void* h, double* grid) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  size_t tot = (size_t)o.grid[0] * o.grid[1] * o.grid[2];
+  std::vector<cplx> g(tot);
+  for (size_t i = 0; i < tot; ++i) g[i] = cplx(grid[2 * i], grid[2 * i + 1]);
+  pme_conv(o, g, nullptr);
+  for (size_t i = 0; i < tot; ++i) {
+    grid[2 * i] = g[i].real();
+    grid[2 * i + 1] = g[i].imag();
+  }
+}
+
+void orc_pme_field(void* h, const double* grid, int imag, double* field) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  size_t tot = (size_t)o.grid[0] * o.grid[1] * o.grid[2];
+  std::vector<cplx> g(tot);
+  for (size_t i = 0; i < tot; ++i) g[i] = cplx(grid[2 * i], grid[2 * i + 1]);
+  auto sp = make_splines(o);
+  std::vector<double> fphi(20 * o.n);
+  gather(o, sp, g, imag != 0, fphi.data());
+  double a[3];
+  frac_scale(o, a);
+  for (int i = 0; i < o.n; ++i)
+    for (int d = 0; d < 3; ++d) field[3 * i + d] = -a[d] * fphi[20 * i + 1 + d];
+}
+
+// Preconditioner z = M r (sparse != 0: the usolve-cutoff pair sum).
+void orc_precond(void* h, int sparse, double ucut, double udiag, const double* rd, const double* rp, double* zd,
+                 double* zp) {
+  precond(*static_cast<Oracle*>(h), sparse != 0, ucut, udiag, rd, rp, zd, zp);
+}
+
 // 3D DFT of an interleaved complex grid (re, im) in place.
 void orc_dft3(void* h, double* grid, int sign) {
   Oracle* o = static_cast<Oracle*>(h);

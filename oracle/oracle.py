@@ -64,6 +64,10 @@ def _lib():
                                  ctypes.c_int, _i, _d, _d]
         L.orc_md_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d]
         L.orc_md_energy.restype = ctypes.c_int
+        L.orc_pme_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d]
+        L.orc_pme_conv.argtypes = [ctypes.c_void_p, _d]
+        L.orc_pme_field.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, _d]
+        L.orc_precond.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d, _d, _d]
         if hasattr(L, "orc_energy"):
             L.orc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d, _d, _d, _d]
             L.orc_energy.restype = ctypes.c_int
@@ -241,6 +245,45 @@ class Oracle:
         g = np.ascontiguousarray(np.stack([grid_c.real, grid_c.imag], axis=-1), dtype=np.float64)
         _lib().orc_dft3(self.h, _p(g), sign)
         return g[..., 0] + 1j * g[..., 1]
+
+    def _kgrid(self):
+        k1, k2, k3 = (int(v) for v in self.sysd["grid"])
+        return (k3, k2, k1, 2)
+
+    def pme_grid_mpole(self):
+        """Spread fractional multipoles: grid [K3, K2, K1, 2] (real, imaginary)."""
+        g = np.zeros(self._kgrid())
+        _lib().orc_pme_grid(self.h, 0, None, None, _p(g))
+        return g
+
+    def pme_grid_uind(self, ud, up):
+        a = np.ascontiguousarray(ud, dtype=np.float64)
+        b = np.ascontiguousarray(up, dtype=np.float64)
+        g = np.zeros(self._kgrid())
+        _lib().orc_pme_grid(self.h, 1, _p(a), _p(b), _p(g))
+        return g
+
+    def pme_conv(self, grid):
+        g = np.ascontiguousarray(grid, dtype=np.float64).copy()
+        _lib().orc_pme_conv(self.h, _p(g))
+        return g
+
+    def pme_field(self, grid, imag=False):
+        g = np.ascontiguousarray(grid, dtype=np.float64)
+        f = np.zeros((self.n, 3))
+        _lib().orc_pme_field(self.h, _p(g), 1 if imag else 0, _p(f))
+        return f
+
+    def precond(self, rd, rp, sparse=True, ucut=None, udiag=None):
+        """z = M r for both residual sets (sparse: usolve pair sum)."""
+        s = self.sysd
+        ucut = s["usolve_cut"] if ucut is None else ucut
+        udiag = s["udiag"] if udiag is None else udiag
+        a = np.ascontiguousarray(rd, dtype=np.float64)
+        b = np.ascontiguousarray(rp, dtype=np.float64)
+        zd, zp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _lib().orc_precond(self.h, 1 if sparse else 0, float(ucut), float(udiag), _p(a), _p(b), _p(zd), _p(zp))
+        return zd, zp
 
     def md_energy(self, vdw=True, valence=True):
         """vdW and valence energies (dict), gradient [n,3] and virial [3,3]."""

@@ -35,7 +35,7 @@ REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 
-TYPES = {7, 15, 36, 37, 61, 62, 210, 211, 212, 213}
+TYPES = {3, 7, 15, 36, 37, 61, 62, 210, 211, 212, 213}
 HEADER_KEYS = {"forcefield", "vdwtype", "radiusrule", "radiustype", "radiussize", "epsilonrule", "dielectric",
                "polarization", "opbendtype", "torsionunit", "angle-trigonal"}
 
@@ -92,9 +92,11 @@ def write_prm_subset():
     return dst
 
 
-def parse_test_arrays(path):
+def parse_test_arrays(path, strip_comments=False):
     """Extract `ref_*` numeric arrays/scalars, tracking TEST_CASE / SECTION."""
     text = open(path).read()
+    if strip_comments:  # commented-out alternatives must not replace the live values
+        text = re.sub(r"//[^\n]*", "", text)
     out = {}
     case, section = "", ""
     pos = 0
@@ -213,6 +215,103 @@ def write_nacl():
     return out
 
 
+def parse_ref_txt(path):
+    """A test/ref/*.txt analysis printout: total (or the one component)
+    energy, interaction count when printed, virial [3][3], gradient [n][3]."""
+    lines = open(path).read().splitlines()
+    out = {"grad": []}
+    for i, ln in enumerate(lines):
+        t = ln.split()
+        if "Total Potential Energy" in ln:
+            out["energy"] = float(t[-2])
+        elif "Energy Component Breakdown" in ln:
+            c = lines[i + 1].split()
+            out["energy"] = float(c[-2])
+            out["count"] = int(c[-1])
+        elif "Internal Virial Tensor" in ln:
+            out["virial"] = [[float(v) for v in t[-3:]]] + [[float(v) for v in lines[i + k].split()] for k in (1, 2)]
+        elif t and t[0] == "Anlyt":
+            out["grad"].append([float(v) for v in t[2:5]])
+    return out
+
+
+def copy_file(src_rel, dst_name):
+    src = open(os.path.join(REF, src_rel)).read()
+    open(os.path.join(HERE, dst_name), "w").write(src)
+
+
+def write_prm_records(src_rel, dst_name, keep_keys, title):
+    """A test-only parameter subset: header scalars plus the named records."""
+    lines = open(os.path.join(REF, src_rel)).read().splitlines()
+    out = ["# " + title, ""]
+    i = 0
+    while i < len(lines):
+        t = lines[i].split()
+        if t and t[0] in ("forcefield", "bond-cubic", "bond-quartic", "angle-cubic", "angle-quartic",
+                          "angle-pentic", "angle-sextic", "ureybrad-cubic", "ureybrad-quartic") and len(t) == 2:
+            out.append(lines[i])
+        elif t and t[0] in keep_keys:
+            out.append(lines[i])
+            if t[0] == "multipole":
+                out.extend(lines[i + 1:i + 5])
+                i += 5
+                continue
+        i += 1
+    open(os.path.join(HERE, dst_name), "w").write("\n".join(out) + "\n")
+
+
+def write_more():
+    """Further reference fixtures: polpair (test/polpair.cpp), NaCl-2/NaCl-3
+    multipoles (test/nacl.cpp:207-304), the Urey-Bradley golden of ten waters
+    (test/urey.cpp) and the bond golden of trp-cage (test/bond.cpp), and the
+    argon-box neighbor sets (test/nblist.cpp, test/nblist.hh)."""
+    out = {}
+    copy_file("test/file/polpair/nacl.xyz", "polpair_nacl.xyz")
+    out["polpair_ewald"] = parse_ref_txt(os.path.join(REF, "test/ref/polpair.1.txt"))
+    out["polpair_nonewald"] = parse_ref_txt(os.path.join(REF, "test/ref/polpair.2.txt"))
+    copy_file("test/file/nacl/nacl1.xyz", "nacl1.xyz")
+    copy_file("test/file/nacl/nacl4.xyz", "nacl4.xyz")
+    # the multipole records of nacl4.key (its parameter lines, without keywords)
+    kl = open(os.path.join(REF, "test/file/nacl/nacl4.key")).read().splitlines()
+    recs = []
+    for i, ln in enumerate(kl):
+        if ln.split()[:1] == ["multipole"]:
+            recs.extend(kl[i:i + 5])
+    open(os.path.join(HERE, "nacl4_multipoles.key"), "w").write("\n".join(recs) + "\n")
+    d = parse_test_arrays(os.path.join(REF, "test/nacl.cpp"), strip_comments=True)
+    for case, sec in (("NaCl-2", "empole -- non-ewald, pbc"), ("NaCl-3", "empole -- pme")):
+        pre = f"{case}/{sec}/"
+        out[case] = dict(energy=d[pre + "ref_eng"], grad=d[pre + "ref_grad"], virial=d[pre + "ref_v"],
+                         count=d[pre + "ref_count"])
+    copy_file("test/file/water10/h2o10.xyz", "h2o10.xyz")
+    write_prm_records("test/file/commit_6fe8e913/water03.prm", "water03_subset.prm",
+                      ("atom", "vdw", "bond", "angle", "ureybrad", "multipole", "polarize"),
+                      "Subset of the AMOEBA-WATER-2003 parameters (test/file/commit_6fe8e913/water03.prm)")
+    out["urey_h2o10"] = parse_ref_txt(os.path.join(REF, "test/ref/urey.txt"))
+    copy_file("test/file/trpcage/trpcage.xyz", "trpcage.xyz")
+    write_prm_records("test/file/commit_6fe8e913/amoebapro13.prm", "amoebapro13_bond_subset.prm",
+                      ("atom", "bond"),
+                      "Atom and bond records of AMOEBA-PRO-2013 (test/file/commit_6fe8e913/amoebapro13.prm)")
+    out["bond_trpcage"] = parse_ref_txt(os.path.join(REF, "test/ref/bond.txt"))
+    json.dump(out, open(os.path.join(HERE, "more_ref.json"), "w"), indent=0)
+    # argon box: five frames of coordinates and the reference's neighbor sets
+    arc = open(os.path.join(REF, "test/file/arbox/arbox.arc")).read().splitlines()
+    n = int(arc[0].split()[0])
+    frames, boxes, i = [], [], 0
+    while i < len(arc) and len(frames) < 5:
+        boxes.append([float(v) for v in arc[i + 1].split()[:3]])
+        frames.append([[float(v) for v in arc[i + 2 + a].split()[2:5]] for a in range(n)])
+        i += n + 2
+    hh = open(os.path.join(REF, "test/nblist.hh")).read()
+    body = hh[hh.index("ans[nframe][216][70] = {"):]
+    nums = [int(x) for x in re.findall(r"-?\d+", body[body.index("{"):])]
+    ans = np.array(nums[:5 * 216 * 70], dtype=np.int16).reshape(5, 216, 70)
+    np.savez_compressed(os.path.join(HERE, "arbox_nblist.npz"), xyz=np.array(frames), box=np.array(boxes), ans=ans)
+    head = arc[:n + 2]
+    open(os.path.join(HERE, "arbox_frame1.xyz"), "w").write("\n".join(head) + "\n")
+    return sorted(out)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not mounted")
@@ -229,6 +328,7 @@ def main():
     print("tinkernist text:", write_tinkernist_text())
     print("aspc table:", write_aspc())
     print("nacl:", write_nacl())
+    print("more:", write_more())
 
 
 if __name__ == "__main__":

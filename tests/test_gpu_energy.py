@@ -21,9 +21,13 @@ K_LF1 = "multipoleterm only\n"
 K_LF2 = K_LF1 + "ewald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\n"
 K_LF3 = "usolve-cutoff 0.01\npolarizeterm only\n"
 K_LF4 = K_LF3 + "ewald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\n"
-K_LF32 = "cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\newald\n"
+# the reference's key for Local-Frame3 (test/localframe3.cpp:9-23): every term but the electrostatics off
+K_LF3X = ("cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\nbondterm none\nangleterm none\n"
+          "strbndterm none\nureybradterm none\ntorsionterm none\nvdwterm none\n")
+K_LF32 = K_LF3X + "ewald\n"
 WATER_KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid {K} {K} {K}\npme-order 5\npolar-eps 1e-6\n"
-             "polarization mutual\nneighbor-list\nlist-buffer {buf}\nusolve-cutoff 4.5\nusolve-diag 2.0\n")
+             "polarization mutual\nneighbor-list\nlist-buffer {buf}\nusolve-cutoff 4.5\nusolve-diag 2.0\n"
+             "vdwterm none\nbondterm none\n")
 
 CASES = {
     "lf1": ("local_frame.xyz", K_LF1, True, False),

@@ -62,7 +62,7 @@ def _compare(prec, eg, gg, vg, eo, go, vo, label):
 @pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
 @pytest.mark.parametrize("terms", ["vdw", "valence", "all"])
 def test_md_terms_vs_oracle(prec, terms):
-    extra = {"vdw": "vdwterm only\n", "valence": "bondterm only\n", "all": "vdwterm\nbondterm\n"}[terms]
+    extra = {"vdw": "vdwterm only\n", "valence": "multipoleterm none\npolarizeterm none\nvdwterm none\n", "all": "vdwterm\nbondterm\n"}[terms]
     # with the electrostatics, both solves go to 1e-8 D so the comparison
     # measures the kernels' precision rather than the solver tolerance
     s = _box(extra, eps=1e-8 if terms == "all" else 1e-6)

@@ -17,7 +17,8 @@ K_LF2 = K_LF1 + "ewald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis
 K_LF3 = "usolve-cutoff 0.01\npolarizeterm only\n"
 K_LF4 = K_LF3 + "ewald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\n"
 # test/localframe3.cpp:9-22 (valence and vdW terms are not part of this path)
-K_LF32 = "cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\newald\n"
+K_LF32 = ("cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\nbondterm none\nangleterm none\n"
+     "strbndterm none\nureybradterm none\ntorsionterm none\nvdwterm none\newald\n")
 
 
 def _ref(golden_dir):

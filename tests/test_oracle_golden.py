@@ -160,7 +160,8 @@ COUNT_CASES = [
     ("Local-Frame-3", "various epolar", "local_frame.xyz", LF_KEY3, (0, 201)),
     ("Local-Frame-4", "various epolar", "local_frame.xyz", LF_KEY4, (0, 222)),
     ("Local-Frame3-2", "- emplar", "local_frame2.xyz",
-     "cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\newald\n", (138, 138)),
+     ("cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\nbondterm none\nangleterm none\n"
+     "strbndterm none\nureybradterm none\ntorsionterm none\nvdwterm none\newald\n"), (138, 138)),
 ]
 
 

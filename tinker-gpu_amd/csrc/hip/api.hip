@@ -89,8 +89,6 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   if (elec && c.periodic && !c.ewald && c.cut > 1e20)
     throw ArgError("periodic systems need ewald or a finite mpole-cutoff");
   if (!c.periodic && c.ewald) throw ArgError("ewald requires a periodic box");
-  if (elec && c.periodic && c.cut < 1e20 && !c.ewald)
-    throw ArgError("non-Ewald periodic multipole cutoffs (switching taper) are not supported");
   for (int d = 0; d < 3; ++d) c.grid[d] = s.opt.pme_grid[d];
   T9_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   T9_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
@@ -434,7 +432,7 @@ void rpole_to_host(Ctx& c, double* out) {
 
 // One evaluation on the current coordinates: list refresh, induced dipoles,
 // then the requested term groups.  Returns the CG iteration count.
-int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad_host) {
+int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad, bool grad_on_device = false) {
   int iters = 0;
   if (c.vdw_pending) {  // left over from a call that threw: drain and drop it
     T9_HIP(hipStreamSynchronize(c.side));
@@ -461,8 +459,10 @@ int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad_host) {
       double rms = 0;
       if (induce_pcg(c, &iters, &rms) != 0) throw NotConverged("INDUCE -- induced dipoles are not converged");
     }
+  } else if (!c.list_valid) {
+    prepare(c);  // the sorted order (perm) the gradient is mapped back through
   }
-  energy_run(c, vers, terms, e16, grad_host);
+  energy_run(c, vers, terms, e16, grad, grad_on_device);
   return iters;
 }
 
@@ -600,6 +600,48 @@ int t9_nblist_stats(t9_ctx* ctx, int64_t* out5) {
   }, T9_ERR_HIP);
 }
 
+int t9_nblist_pairs(t9_ctx* ctx, int which, int64_t cap, int* pairs, int64_t* count) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (which != 0 && which != 1) throw t9::ArgError("which must be 0 (multipole list) or 1 (vdW list)");
+    if (cap < 0 || (cap > 0 && !pairs)) throw t9::ArgError("bad pair buffer");
+    if (!c.list_valid) t9::nblist_build(c);
+    t9::NbList& L = which == 0 ? c.mlist : c.vlist;
+    if (L.cutoff <= 0) throw t9::ArgError("that list is not built for this system");
+    const int n = c.n, nw = c.nwave;
+    std::vector<int> nl(n), perm(n);
+    std::vector<long long> off(nw + 1);
+    std::vector<uint32_t> ent((size_t)std::max<long long>(L.total, 1));
+    T9_HIP(hipMemcpyAsync(nl.data(), L.nlist.p, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipMemcpyAsync(perm.data(), c.perm.p, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipMemcpyAsync(off.data(), L.off.p, sizeof(long long) * (nw + 1), hipMemcpyDeviceToHost, c.stream));
+    if (L.total > 0)
+      T9_HIP(hipMemcpyAsync(ent.data(), L.entries.p, sizeof(uint32_t) * L.total, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    // each pair appears in both atoms' lists: report it once, caller order i < k
+    long long m = 0;
+    for (int s = 0; s < n; ++s) {
+      const int i = perm[s];
+      for (int k = 0; k < nl[s]; ++k) {
+        const long long at = off[s >> 6] + 64LL * k + (s & 63);
+        if (at < 0 || at >= L.total) throw t9::ArgError("neighbor list entry out of range (internal error)");
+        const int j = (int)(ent[at] >> t9::kCodeBits);
+        if (j < 0 || j >= n) throw t9::ArgError("neighbor index out of range (internal error)");
+        const int kk = perm[j];
+        if (i < kk) {
+          if (m < cap) {
+            pairs[2 * m] = i;
+            pairs[2 * m + 1] = kk;
+          }
+          ++m;
+        }
+      }
+    }
+    if (count) *count = m;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
 int t9_mpole_init(t9_ctx* ctx) {
   return guarded([&] {
     auto& c = ctx->c;
@@ -701,24 +743,216 @@ int t9_set_uind(t9_ctx* ctx, const double* uind, const double* uinp) {
   }, T9_ERR_HIP);
 }
 
+namespace {
+int energy_common(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* grad, bool grad_on_device) {
+  auto& c = ctx->c;
+  if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL | T9_CALC_ANALYZE))
+    throw t9::ArgError("unknown vers bits");
+  if (terms & ~t9::TERM_ALL) throw t9::ArgError("unknown term bits");
+  double e[16];
+  const int iters = t9::evaluate(c, vers, terms, e, (vers & T9_CALC_GRAD) ? grad : nullptr, grad_on_device);
+  long long nem = 0, nep = 0, nev = 0;
+  if (vers & T9_CALC_ANALYZE) {
+    if (terms & t9::TERM_ELEC) t9::count_interactions(c, &nem, &nep);
+    if (terms & t9::TERM_VDW) nev = t9::vdw_count(c);
+  }
+  if (out) {
+    t9::fill_energy_out(e, vers, iters, out);
+    out->nem = nem;
+    out->nep = nep;
+    out->nev = nev;
+  }
+  return T9_OK;
+}
+
+// Host grid [K3][K2][K1][2] FP64 <-> the context's complex grid.
+void grid_from_host(t9::Ctx& c, const double* g) {
+  const size_t ng = (size_t)c.grid[0] * c.grid[1] * c.grid[2];
+  if (c.prec == 64) {
+    T9_HIP(hipMemcpyAsync(c.qgrid.p, g, sizeof(double) * 2 * ng, hipMemcpyHostToDevice, c.stream));
+  } else {
+    std::vector<float> f(g, g + 2 * ng);
+    T9_HIP(hipMemcpyAsync(c.qgrid.p, f.data(), sizeof(float) * 2 * ng, hipMemcpyHostToDevice, c.stream));
+  }
+  T9_HIP(hipStreamSynchronize(c.stream));
+}
+
+void grid_to_host(t9::Ctx& c, double* g) {
+  const size_t ng = (size_t)c.grid[0] * c.grid[1] * c.grid[2];
+  if (c.prec == 64) {
+    T9_HIP(hipMemcpyAsync(g, c.qgrid.p, sizeof(double) * 2 * ng, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> f(2 * ng);
+    T9_HIP(hipMemcpyAsync(f.data(), c.qgrid.p, sizeof(float) * 2 * ng, hipMemcpyDeviceToHost, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    std::copy(f.begin(), f.end(), g);
+  }
+}
+
+void require_pme(const t9::Ctx& c) {
+  if (!c.ewald) throw t9::ArgError("PME sub-steps need ewald");
+}
+
+// A PME sub-step on a new evaluation's spline bins: the coordinates may have
+// changed since the last call.
+void pme_prepare(t9::Ctx& c) {
+  require_pme(c);
+  c.mlist.cut_valid = c.ulist.cut_valid = false;
+  c.bins_valid = false;
+  c.rpole_valid = false;
+  t9::prepare(c);
+}
+}  // namespace
+
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad) {
+  return guarded([&] { return energy_common(ctx, vers, t9::TERM_ALL, out, grad, false); }, T9_ERR_HIP);
+}
+
+int t9_energy_terms(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* grad) {
+  return guarded([&] { return energy_common(ctx, vers, terms, out, grad, false); }, T9_ERR_HIP);
+}
+
+int t9_energy_device(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* d_grad) {
+  return guarded([&] { return energy_common(ctx, vers, terms, out, d_grad, true); }, T9_ERR_HIP);
+}
+
+int t9_set_coords_device(t9_ctx* ctx, const double* d_xyz) {
   return guarded([&] {
     auto& c = ctx->c;
-    if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL | T9_CALC_ANALYZE))
-      throw t9::ArgError("unknown vers bits");
-    double e[16];
-    const int iters = t9::evaluate(c, vers, t9::TERM_ALL, e, (vers & T9_CALC_GRAD) ? grad : nullptr);
-    long long nem = 0, nep = 0, nev = 0;
-    if (vers & T9_CALC_ANALYZE) {
-      t9::count_interactions(c, &nem, &nep);
-      nev = t9::vdw_count(c);
+    if (!d_xyz) throw t9::ArgError("null coordinates");
+    T9_HIP(hipMemcpyAsync(c.xyz.p, d_xyz, sizeof(double) * 3 * c.n, hipMemcpyDeviceToDevice, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    c.coords_dirty = true;
+    c.rpole_valid = false;
+    c.bins_valid = false;
+    c.md_ready = false;
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_get_coords_device(t9_ctx* ctx, double* d_xyz) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!d_xyz) throw t9::ArgError("null coordinates");
+    T9_HIP(hipMemcpyAsync(d_xyz, c.xyz.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToDevice, c.stream));
+    T9_HIP(hipStreamSynchronize(c.stream));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+namespace {
+int precond_host(t9_ctx* ctx, int mode, const double* rsd, const double* rsdp, double* zrsd, double* zrsdp) {
+  auto& c = ctx->c;
+  if (!rsd || !rsdp || !zrsd || !zrsdp) throw t9::ArgError("null vector");
+  t9::prepare(c);
+  // the CG scratch vectors hold the inputs (rsd, rsdp) and outputs (vec, vecp)
+  t9::scatter_from_host(c, rsd, c.rsd.p);
+  t9::scatter_from_host(c, rsdp, c.rsdp.p);
+  t9::precond_apply(c, c.rsd.p, c.rsdp.p, c.vec.p, c.vecp.p, nullptr, nullptr, nullptr, mode);
+  t9::gather_to_host(c, c.vec.p, zrsd);
+  t9::gather_to_host(c, c.vecp.p, zrsdp);
+  return T9_OK;
+}
+}  // namespace
+
+int t9_diag_precond(t9_ctx* ctx, const double* rsd, const double* rsdp, double* zrsd, double* zrsdp) {
+  return guarded([&] { return precond_host(ctx, 0, rsd, rsdp, zrsd, zrsdp); }, T9_ERR_HIP);
+}
+
+int t9_sparse_precond_apply(t9_ctx* ctx, const double* rsd, const double* rsdp, double* zrsd, double* zrsdp) {
+  return guarded([&] { return precond_host(ctx, 1, rsd, rsdp, zrsd, zrsdp); }, T9_ERR_HIP);
+}
+
+int t9_pme_grid_mpole(t9_ctx* ctx, double* grid) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    pme_prepare(c);
+    t9::pme_grid_mpole(c);
+    if (grid) grid_to_host(c, grid);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_pme_grid_uind(t9_ctx* ctx, const double* uind, const double* uinp, double* grid) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
+    pme_prepare(c);
+    t9::scatter_from_host(c, uind, c.zrsd.p);
+    t9::scatter_from_host(c, uinp, c.zrsdp.p);
+    t9::pme_grid_uind(c, c.zrsd.p, c.zrsdp.p);
+    if (grid) grid_to_host(c, grid);
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+namespace {
+int pme_grid_op(t9_ctx* ctx, double* grid, int op) {
+  auto& c = ctx->c;
+  require_pme(c);
+  if (grid) grid_from_host(c, grid);
+  if (op == 0) t9::pme_conv_grid(c);
+  else t9::pme_fft_dir(c, op);
+  if (grid) grid_to_host(c, grid);
+  else T9_HIP(hipStreamSynchronize(c.stream));
+  return T9_OK;
+}
+}  // namespace
+
+int t9_pme_fft_front(t9_ctx* ctx, double* grid) {
+  return guarded([&] { return pme_grid_op(ctx, grid, -1); }, T9_ERR_HIP);
+}
+
+int t9_pme_fft_back(t9_ctx* ctx, double* grid) {
+  return guarded([&] { return pme_grid_op(ctx, grid, +1); }, T9_ERR_HIP);
+}
+
+int t9_pme_conv(t9_ctx* ctx, double* grid) {
+  return guarded([&] { return pme_grid_op(ctx, grid, 0); }, T9_ERR_HIP);
+}
+
+namespace {
+int pme_fphi_host(t9_ctx* ctx, bool mpole, const double* grid, double* fd, double* fp) {
+  auto& c = ctx->c;
+  require_pme(c);
+  t9::prepare(c);
+  if (grid) grid_from_host(c, grid);
+  t9::pme_fphi(c, mpole, c.vec.p, c.vecp.p);
+  if (fd) t9::gather_to_host(c, c.vec.p, fd);
+  if (fp) t9::gather_to_host(c, c.vecp.p, fp);
+  if (!fd && !fp) T9_HIP(hipStreamSynchronize(c.stream));
+  return T9_OK;
+}
+}  // namespace
+
+int t9_pme_fphi_mpole(t9_ctx* ctx, const double* grid, double* field) {
+  return guarded([&] { return pme_fphi_host(ctx, true, grid, field, nullptr); }, T9_ERR_HIP);
+}
+
+int t9_pme_fphi_uind(t9_ctx* ctx, const double* grid, double* fd, double* fp) {
+  return guarded([&] { return pme_fphi_host(ctx, false, grid, fd, fp); }, T9_ERR_HIP);
+}
+
+int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign) {
+  return guarded([&] {
+    if (!grid) throw t9::ArgError("null grid");
+    if (precision != T9_FP32 && precision != T9_FP64) throw t9::ArgError("precision must be T9_FP32 or T9_FP64");
+    if (sign != -1 && sign != 1) throw t9::ArgError("sign must be -1 (forward) or +1 (inverse)");
+    const int K[3] = {k1, k2, k3};
+    for (int d = 0; d < 3; ++d) t9::fft_check_size(K[d], precision);
+    const auto tw = t9::fft_twiddles(K);
+    t9::DBuf dtw;
+    if (precision == T9_FP64) {
+      dtw.alloc(sizeof(double) * tw.size());
+      T9_HIP(hipMemcpy(dtw.p, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(tw.begin(), tw.end());
+      dtw.alloc(sizeof(float) * f.size());
+      T9_HIP(hipMemcpy(dtw.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
     }
-    if (out) {
-      t9::fill_energy_out(e, vers, iters, out);
-      out->nem = nem;
-      out->nep = nep;
-      out->nev = nev;
-    }
+    t9::fft3d_device(grid, precision, K, sign, dtw.p, nullptr);
+    T9_HIP(hipStreamSynchronize(nullptr));
     return T9_OK;
   }, T9_ERR_HIP);
 }
@@ -811,6 +1045,17 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
       stats->list_builds = c.builds - builds0;
       stats->steps = nsteps;
     }
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
+int t9_md_bussi(t9_ctx* ctx, double dt, double tau, double kelvin, double r, double s2) {
+  return guarded([&] {
+    auto& c = ctx->c;
+    if (!(dt > 0) || !(tau > 0) || !(kelvin > 0)) throw t9::ArgError("bad Bussi arguments");
+    if (c.nfree < 2) throw t9::ArgError("the thermostat needs md_init (degrees of freedom)");
+    t9::md_bussi(c, dt, tau, kelvin, r, s2);
+    T9_HIP(hipStreamSynchronize(c.stream));
     return T9_OK;
   }, T9_ERR_HIP);
 }

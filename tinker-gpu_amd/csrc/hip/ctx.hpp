@@ -298,6 +298,19 @@ void pme_fft_roundtrip(Ctx& c, bool lines = false);
 void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self of the permanent multipoles
+// PME sub-steps on c.qgrid: spreading, one-direction transforms (sign -1
+// forward, +1 inverse), the influence function alone, and the field of the
+// grid at the atoms (fd, fp overwritten, no self term)
+void pme_grid_mpole(Ctx& c);
+void pme_grid_uind(Ctx& c, const void* ud, const void* up);
+void pme_fft_dir(Ctx& c, int sign);
+void pme_conv_grid(Ctx& c);
+void pme_fphi(Ctx& c, bool mpole, void* fd, void* fp);
+// context-free 3D transform of a device grid [K3][K2][K1] (complex FP32 or
+// FP64), line passes in place; tw = forward twiddles of the three axes
+void fft_check_size(int K, int prec);
+std::vector<double> fft_twiddles(const int* K);
+void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st);
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();
 // learned guess (guess.hip): nparam of the network, allocation, per-solve hooks
@@ -318,8 +331,10 @@ void guess_capture_start(Ctx& c, int stride, long long max_samples);
 void guess_fit_norm(Ctx& c);
 void guess_train(Ctx& c, int epochs, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
 // dots64: also r.z (d, p) and r.r (d, p) per 64 elements (4 doubles each), or null
+// mode -1: the keywords' choice (sparse when pcg-precond and usolve-cutoff
+// allow, else diagonal); 0: diagonal z = alpha r; 1: sparse (error if off)
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d = nullptr,
-                   void* z2p = nullptr, double* dots64 = nullptr);  // z (and a second copy of z when given)
+                   void* z2p = nullptr, double* dots64 = nullptr, int mode = -1);  // z (and a second copy of z when given)
 // spread mode 0: dipoles (src1 real, src2 imaginary); 1: multipoles (real);
 // 2: multipoles src1 (real) + dipoles src2 (imaginary)
 void pme_spread(Ctx& c, int mode, const void* s1, const void* s2);
@@ -343,7 +358,7 @@ enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_ALL = 7 };
 // vers bit 2: virial; out16[EP_*] energies / virial; sorted-order dE/dx of
 // the electrostatics and vdW in c.egrad, caller-order valence dE/dx in
 // c.g_fast; grad_orig (host, caller order, or null) receives their sum
-void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig);
+void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig, bool grad_on_device = false);
 // readbacks the host waits for: mark after the copy, then wait by polling the
 // marker event (a blocking synchronize wakes the thread tens of microseconds late)
 void mark_readback(Ctx& c);

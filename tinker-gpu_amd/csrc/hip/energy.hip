@@ -1206,17 +1206,19 @@ void count_interactions(Ctx& c, long long* nem, long long* nep) {
   *nep = c.opt.use_polar ? (long long)h[1] + self : 0;
 }
 
-void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig) {
+void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig, bool grad_on_device) {
   if (c.prec == 32) energy_t<float>(c, vers, terms, out16);
   else energy_t<double>(c, vers, terms, out16);
   if (grad_orig) {
     const bool val = (terms & TERM_VALENCE) && c.use_valence;
     DBuf tmp;
-    tmp.alloc(sizeof(double) * 3 * c.n);
+    if (!grad_on_device) tmp.alloc(sizeof(double) * 3 * c.n);
+    double* dst = grad_on_device ? grad_orig : tmp.as<double>();
     hipLaunchKernelGGL(k_grad_to_orig, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, c.perm.as<int>(),
-                       c.egrad.as<double>(), val ? c.g_fast.as<double>() : nullptr, tmp.as<double>());
+                       c.egrad.as<double>(), val ? c.g_fast.as<double>() : nullptr, dst);
     T9_LAUNCH_CHECK();
-    T9_HIP(hipMemcpyAsync(grad_orig, tmp.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+    if (!grad_on_device)
+      T9_HIP(hipMemcpyAsync(grad_orig, tmp.p, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
     T9_HIP(hipStreamSynchronize(c.stream));
   }
 }

@@ -14,6 +14,7 @@
 // in FP64 once per box into a table (z-pass order) and on the fly when the
 // virial sums are wanted.  Transform lengths used by the PME grids of
 // interest (24 ... 160) get kernels with compile-time index arithmetic.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -380,6 +381,76 @@ constexpr int lines_per_block() {
   return sizeof(real) == 4 ? 16 : 8;
 }
 
+// One line pass of the 3D transform along `axis` (sign -1 forward, +1
+// inverse), with the influence function when cv.enabled (z pass only).
+template <class real>
+void launch_pass(Cplx<real>* grid, int K1, int K2, int K3, int axis, int sign, const FftPlan& plan,
+                 const Cplx<real>* tw, const ConvArgs& cv, Gate gate, hipStream_t st) {
+  constexpr int B = lines_per_block<real>();
+  const int xb = (K1 + B - 1) / B;
+  const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
+  // two line buffers + the twiddles (compile-time lengths stage them in LDS)
+  const size_t sh =
+      (2 * (size_t)B * (plan.n + 1) + (twiddles_in_lds<real>() && ct_len(plan.n) ? plan.n : 0)) * sizeof(Cplx<real>);
+  void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs, Gate);
+  switch (plan.n) {
+    case 24: kern = k_fft_pass<real, B, 24>; break;
+    case 32: kern = k_fft_pass<real, B, 32>; break;
+    case 40: kern = k_fft_pass<real, B, 40>; break;
+    case 48: kern = k_fft_pass<real, B, 48>; break;
+    case 64: kern = k_fft_pass<real, B, 64>; break;
+    case 80: kern = k_fft_pass<real, B, 80>; break;
+    case 96: kern = k_fft_pass<real, B, 96>; break;
+    case 120: kern = k_fft_pass<real, B, 120>; break;
+    case 128: kern = k_fft_pass<real, B, 128>; break;
+    case 160: kern = k_fft_pass<real, B, 160>; break;
+    default: kern = k_fft_pass<real, B, 0>; break;
+  }
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), sh, st, grid, K1, K2, K3, axis, sign, plan, tw, cv, gate);
+  T9_LAUNCH_CHECK();
+}
+
+// Unnormalised forward (sign -1: x, y, z) or inverse (+1: z, y, x) 3D
+// transform in place, line passes only; tw holds the forward twiddles of
+// each axis back to back (K1 + K2 + K3 values).
+template <class real>
+void fft3d_lines(Cplx<real>* grid, const int* K, int sign, const Cplx<real>* tw, hipStream_t st) {
+  const FftPlan p[3] = {make_fft_plan(K[0]), make_fft_plan(K[1]), make_fft_plan(K[2])};
+  const Cplx<real>* twa[3] = {tw, tw + K[0], tw + K[0] + K[1]};
+  ConvArgs none{};
+  for (int q = 0; q < 3; ++q) {
+    const int axis = sign < 0 ? q : 2 - q;
+    launch_pass<real>(grid, K[0], K[1], K[2], axis, sign, p[axis], twa[axis], none, Gate{}, st);
+  }
+}
+
+// Reciprocal-space influence function applied point by point (the
+// reference's pmeConv, src/acc/pme.cpp:220-303): grid index
+// k1 + K1 (k2 + K2 k3), FP64 factor, k = 0 and exponents below -50 zeroed.
+template <class real>
+__global__ void k_pme_conv(int K1, int K2, int K3, double pterm, double volterm, double il0, double il1, double il2,
+                           const double* __restrict__ bsmod, Cplx<real>* __restrict__ grid) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= (long long)K1 * K2 * K3) return;
+  const int k1 = (int)(e % K1);
+  const long long l = e / K1;
+  const int k2 = (int)(l % K2), k3 = (int)(l / K2);
+  double f = 0;
+  if (k1 != 0 || k2 != 0 || k3 != 0) {
+    const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
+    const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
+    const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
+    const double h1 = r1 * il0, h2 = r2 * il1, h3 = r3 * il2;
+    const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
+    const double term = -pterm * hsq;
+    if (term > -50) f = exp(term) / (hsq * volterm * bsmod[k1] * bsmod[K1 + k2] * bsmod[K1 + K2 + k3]);
+  }
+  Cplx<real> v = grid[e];
+  v.x = (real)(v.x * f);
+  v.y = (real)(v.y * f);
+  grid[e] = v;
+}
+
 template <class real>
 void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale, bool lines = false) {
   constexpr int B = lines_per_block<real>();
@@ -398,7 +469,6 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale, bool lines
   conv.fwd_only = fwd_only ? 1 : 0;
   conv.vscale = vscale;
   conv.vrows = vrows;
-  const int xb = (K1 + B - 1) / B;
   const Cplx<real>* twa[3] = {tw, tw + K1, tw + K1 + K2};
   if (!c.conv_table_valid) {
     c.conv_table.alloc(sizeof(real) * (size_t)K1 * K2 * K3);
@@ -410,27 +480,7 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale, bool lines
   }
   conv.table = c.conv_table.p;
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
-    const int nblk = axis == 0 ? ceil_div((long)K2 * K3, B) : (axis == 1 ? xb * K3 : xb * K2);
-    // two line buffers + the twiddles (compile-time lengths stage them in LDS)
-    const size_t sh = (2 * (size_t)B * (p[axis].n + 1) + (twiddles_in_lds<real>() && ct_len(p[axis].n) ? p[axis].n : 0)) *
-                      sizeof(Cplx<real>);
-    void (*kern)(Cplx<real>*, int, int, int, int, int, FftPlan, const Cplx<real>*, ConvArgs, Gate);
-    switch (p[axis].n) {
-      case 24: kern = k_fft_pass<real, B, 24>; break;
-      case 32: kern = k_fft_pass<real, B, 32>; break;
-      case 40: kern = k_fft_pass<real, B, 40>; break;
-      case 48: kern = k_fft_pass<real, B, 48>; break;
-      case 64: kern = k_fft_pass<real, B, 64>; break;
-      case 80: kern = k_fft_pass<real, B, 80>; break;
-      case 96: kern = k_fft_pass<real, B, 96>; break;
-      case 120: kern = k_fft_pass<real, B, 120>; break;
-      case 128: kern = k_fft_pass<real, B, 128>; break;
-      case 160: kern = k_fft_pass<real, B, 160>; break;
-      default: kern = k_fft_pass<real, B, 0>; break;
-    }
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), sh, c.stream, grid, K1, K2, K3, axis, sign, p[axis], twa[axis],
-                       cv, c.gate);
-    T9_LAUNCH_CHECK();
+    launch_pass<real>(grid, K1, K2, K3, axis, sign, p[axis], twa[axis], cv, c.gate, c.stream);
   };
   if (c.fft_plane < 0) {
     const char* fp = std::getenv("T9HIP_FFT");
@@ -499,6 +549,56 @@ void pme_check_fft_sizes(const Ctx& c) {
     if (2 * (size_t)B * (c.grid[d] + 1) * esize > 65536)
       throw ArgError("PME grid dimension too large for the LDS FFT");
   }
+}
+
+}  // namespace t9
+
+namespace t9 {
+
+void pme_fft_dir(Ctx& c, int sign) {
+  if (c.prec == 32)
+    fft3d_lines<float>(c.qgrid.as<Cplx<float>>(), c.grid, sign, c.twiddle.as<Cplx<float>>(), c.stream);
+  else
+    fft3d_lines<double>(c.qgrid.as<Cplx<double>>(), c.grid, sign, c.twiddle.as<Cplx<double>>(), c.stream);
+}
+
+void pme_conv_grid(Ctx& c) {
+  const int K1 = c.grid[0], K2 = c.grid[1], K3 = c.grid[2];
+  const double pi = 3.14159265358979323846;
+  const double pterm = (pi / c.aewald) * (pi / c.aewald), volterm = pi * c.box[0] * c.box[1] * c.box[2];
+  const long long ng = (long long)K1 * K2 * K3;
+  if (c.prec == 32)
+    hipLaunchKernelGGL(k_pme_conv<float>, dim3(ceil_div(ng, 256)), dim3(256), 0, c.stream, K1, K2, K3, pterm, volterm,
+                       1.0 / c.box[0], 1.0 / c.box[1], 1.0 / c.box[2], c.bsmod.as<double>(),
+                       c.qgrid.as<Cplx<float>>());
+  else
+    hipLaunchKernelGGL(k_pme_conv<double>, dim3(ceil_div(ng, 256)), dim3(256), 0, c.stream, K1, K2, K3, pterm,
+                       volterm, 1.0 / c.box[0], 1.0 / c.box[1], 1.0 / c.box[2], c.bsmod.as<double>(),
+                       c.qgrid.as<Cplx<double>>());
+  T9_LAUNCH_CHECK();
+}
+
+void fft_check_size(int K, int prec) {
+  const size_t esize = prec == 32 ? 8 : 16;
+  const int B = prec == 32 ? 16 : 8;
+  if (K < 1 || make_fft_plan(K).nstage < 0)
+    throw ArgError("FFT length " + std::to_string(K) + " is not a product of 2, 3 and 5");
+  if (2 * (size_t)B * (K + 1) * esize > 65536) throw ArgError("FFT length too large for the LDS line pass");
+}
+
+std::vector<double> fft_twiddles(const int* K) {
+  std::vector<double> tw;
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < K[d]; ++k) {
+      tw.push_back(std::cos(2 * M_PI * k / K[d]));
+      tw.push_back(-std::sin(2 * M_PI * k / K[d]));
+    }
+  return tw;
+}
+
+void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st) {
+  if (prec == 32) fft3d_lines<float>((Cplx<float>*)grid, K, sign, (const Cplx<float>*)tw, st);
+  else fft3d_lines<double>((Cplx<double>*)grid, K, sign, (const Cplx<double>*)tw, st);
 }
 
 }  // namespace t9

@@ -399,8 +399,10 @@ void dfield_real_t(Ctx& c, hipStream_t st) {
 }
 
 template <class real>
-void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p, double* dots64) {
-  const bool sparse = c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.mlist.cut2nd > 0;
+void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p, double* dots64,
+               int mode) {
+  const bool sparse = mode == 0 ? false : (c.opt.pcg_prec && c.opt.usolve_cut > 0 && c.mlist.cut2nd > 0);
+  if (mode == 1 && !sparse) throw ArgError("the sparse preconditioner needs pcg-precond and usolve-cutoff > 0");
   const auto& L = c.mlist;
   if (sparse) require_cut(L);
   auto kern = c.uniform_thole ? k_precond<real, 1> : k_precond<real, 0>;
@@ -448,9 +450,9 @@ void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {
 }
 
 void precond_apply(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void* z2d, void* z2p,
-                   double* dots64) {
-  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp, z2d, z2p, dots64);
-  else precond_t<double>(c, rd, rp, zd, zp, z2d, z2p, dots64);
+                   double* dots64, int mode) {
+  if (c.prec == 32) precond_t<float>(c, rd, rp, zd, zp, z2d, z2p, dots64, mode);
+  else precond_t<double>(c, rd, rp, zd, zp, z2d, z2p, dots64, mode);
 }
 
 }  // namespace t9

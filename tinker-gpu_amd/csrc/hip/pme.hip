@@ -606,6 +606,43 @@ void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   else mpole_field_t<double>(c, fd, fp, before_gather);
 }
 
+// PME sub-steps on the context grid (the boundary's gridMpole / gridUind /
+// fphiMpole / fphiUind, include/ff/pme.h:64-91).  The gathers return the
+// Cartesian field -grad phi of the grid at each sorted atom (real part into
+// fd, imaginary part into fp; the multipole grid's real part into both),
+// without the self term.
+void pme_grid_mpole(Ctx& c) { pme_spread(c, 1, c.rpole.p, nullptr); }
+
+void pme_grid_uind(Ctx& c, const void* ud, const void* up) { pme_spread(c, 0, ud, up); }
+
+namespace {
+template <class real>
+void fphi_t(Ctx& c, bool mpole, void* fd, void* fp) {
+  if (!c.bins_valid) pme_bin_t<real>(c);
+  const size_t vb = sizeof(real) * 3 * (size_t)c.n;
+  T9_HIP(hipMemsetAsync(fd, 0, vb, c.stream));
+  T9_HIP(hipMemsetAsync(fp, 0, vb, c.stream));
+  const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
+  if (mpole) {
+    hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3,
+                       (real)0, c.rpole.as<real>(), (real*)fd, (real*)fp);
+  } else {
+    // the self term is off (term 0); its dipole inputs read the zeroed outputs
+    T9_HIP(hipMemsetAsync(c.zrsd.p, 0, vb, c.stream));
+    hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3,
+                       (real)0, c.zrsd.as<real>(), c.zrsd.as<real>(), (real*)fd, (real*)fp, Gate{});
+  }
+  T9_LAUNCH_CHECK();
+}
+}  // namespace
+
+void pme_fphi(Ctx& c, bool mpole, void* fd, void* fp) {
+  if (c.prec == 32) fphi_t<float>(c, mpole, fd, fp);
+  else fphi_t<double>(c, mpole, fd, fp);
+}
+
 // B-spline moduli of one axis (the dftmod rule, with the zeta correction),
 // FP64 on the host.
 std::vector<double> pme_bsmod(int K, int order) {

@@ -27,6 +27,7 @@ int finish_setup(t9_system* out, const char* prm_path, const char* key_text) {
   if (key_text) t9::read_key_text(key_text, out->s, &prm_name);
   if (!prm_path || !*prm_path) throw std::runtime_error("parameter file path required");
   t9::read_prm(prm_path, out->s.ff);
+  if (key_text) t9::read_prm_text(key_text, out->s.ff);
   t9::setup_system(out->s);
   return T9_OK;
 }

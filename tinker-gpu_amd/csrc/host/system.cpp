@@ -95,12 +95,40 @@ struct Rng {
 
 // ---------------------------------------------------------------- parsing
 
+namespace {
+void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff);
+}
+
 void read_prm(const std::string& path, ForceField& ff) {
   std::ifstream in(path);
   if (!in) throw std::runtime_error("cannot open parameter file " + path);
   std::vector<std::string> lines;
   std::string line;
   while (std::getline(in, line)) lines.push_back(line);
+  parse_prm_lines(lines, ff);
+}
+
+// Parameter records in the keyword text (Tinker reads the .key file after
+// the parameter file, so its records replace the file's: a multipole record
+// of a type replaces every earlier multipole record of that type).
+void read_prm_text(const std::string& text, ForceField& ff) {
+  std::istringstream in(text);
+  std::vector<std::string> lines;
+  std::string line;
+  while (std::getline(in, line)) lines.push_back(line);
+  const size_t first = ff.mpoles.size();
+  parse_prm_lines(lines, ff);
+  if (ff.mpoles.size() == first) return;
+  std::set<int> types;
+  for (size_t q = first; q < ff.mpoles.size(); ++q) types.insert(ff.mpoles[q].type);
+  std::vector<MultipoleParam> kept;
+  for (size_t q = 0; q < ff.mpoles.size(); ++q)
+    if (q >= first || !types.count(ff.mpoles[q].type)) kept.push_back(ff.mpoles[q]);
+  ff.mpoles.swap(kept);
+}
+
+namespace {
+void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
   for (size_t li = 0; li < lines.size(); ++li) {
     auto tok = split(lines[li]);
     if (tok.empty()) continue;
@@ -216,6 +244,7 @@ void read_prm(const std::string& path, ForceField& ff) {
     }
   }
 }
+}  // namespace
 
 void read_xyz(const std::string& path, System& s) {
   std::ifstream in(path);
@@ -262,6 +291,7 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
   std::string line;
   Options& o = s.opt;
   bool mpole_only = false, polar_only = false, vdw_only = false, valence_only = false;
+  int other_off = 0, only_mask = 0;
   while (std::getline(in, line)) {
     auto tok = split(line);
     if (tok.empty()) continue;
@@ -330,17 +360,38 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
       if (v1 == "only") vdw_only = true;
       else o.use_vdw = (v1 != "none");
     } else if (k == "bondterm" || k == "angleterm" || k == "ureybradterm") {
-      if (v1 == "only") valence_only = true;
-      else o.use_valence = (v1 != "none");
+      bool& flag = k == "bondterm" ? o.use_bond : (k == "angleterm" ? o.use_angle : o.use_urey);
+      if (v1 == "only") {
+        valence_only = true;
+        only_mask |= k == "bondterm" ? 1 : (k == "angleterm" ? 2 : 4);
+      } else {
+        flag = (v1 != "none");
+      }
+    } else if (k == "strbndterm" || k == "opbendterm" || k == "opdistterm" || k == "torsionterm" ||
+               k == "pitorsterm" || k == "tortorterm" || k == "angtorterm" || k == "strtorterm" ||
+               k == "improperterm" || k == "imptorsterm") {
+      if (v1 == "none") ++other_off;
     } else if (k == "vdw-taper") {
       const double v = to_d(tok[1]);
       o.vdw_taper = v <= 1 ? v * o.vdw_cut : v;
     }
   }
-  if (vdw_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = true; o.use_valence = false; }
-  if (valence_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = false; o.use_valence = true; }
-  if (mpole_only) { o.use_mpole = true; o.use_polar = false; o.use_vdw = false; o.use_valence = false; }
-  if (polar_only) { o.use_mpole = false; o.use_polar = true; o.use_vdw = false; o.use_valence = false; }
+  // stretch-bend, out-of-plane bending and the torsion family all named off
+  if (other_off >= 7) o.other_valence_off = true;
+  if (vdw_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = true; }
+  if (valence_only) {
+    o.other_valence_off = true;  // "only" switches every other term off
+    o.use_mpole = false;
+    o.use_polar = false;
+    o.use_vdw = false;
+    o.use_bond = (only_mask & 1) != 0;
+    o.use_angle = (only_mask & 2) != 0;
+    o.use_urey = (only_mask & 4) != 0;
+  }
+  if (mpole_only) { o.use_mpole = true; o.use_polar = false; o.use_vdw = false; o.use_bond = o.use_angle = o.use_urey = false; }
+  if (polar_only) { o.use_mpole = false; o.use_polar = true; o.use_vdw = false; o.use_bond = o.use_angle = o.use_urey = false; }
+  if (vdw_only) o.use_bond = o.use_angle = o.use_urey = false;
+  o.use_valence = o.use_bond || o.use_angle || o.use_urey;
 }
 
 // ---------------------------------------------------------------- setup
@@ -507,8 +558,11 @@ void assign_multipoles(System& s) {
       }
     }
   }
-  for (int i = 0; i < n; ++i)
-    if (!done[i]) throw std::runtime_error("no multipole parameters for atom " + std::to_string(i + 1));
+  // with the electrostatics switched off an atom may lack multipole records
+  // (it then carries none)
+  if (s.opt.use_mpole || s.opt.use_polar)
+    for (int i = 0; i < n; ++i)
+      if (!done[i]) throw std::runtime_error("no multipole parameters for atom " + std::to_string(i + 1));
 }
 
 void assign_polarize(System& s) {
@@ -660,11 +714,9 @@ void build_valence(System& s) {
   for (int i = 0; i < n; ++i)
     for (int j : s.i12[i]) {
       if (j <= i) continue;
+      if (!s.opt.use_bond) continue;
       auto it = ff.bond.find({std::min(s.cls[i], s.cls[j]), std::max(s.cls[i], s.cls[j])});
-      if (it == ff.bond.end()) {
-        if (s.opt.use_valence) throw std::runtime_error("no bond parameters for atoms " + std::to_string(i + 1));
-        continue;
-      }
+      if (it == ff.bond.end()) throw std::runtime_error("no bond parameters for atoms " + std::to_string(i + 1));
       s.bond_ij.insert(s.bond_ij.end(), {i, j});
       s.bond_kr.insert(s.bond_kr.end(), {it->second.k, it->second.r0});
     }
@@ -675,14 +727,13 @@ void build_valence(System& s) {
         const int a = nb[p], c = nb[q];
         const auto key = std::make_tuple(std::min(s.cls[a], s.cls[c]), s.cls[b], std::max(s.cls[a], s.cls[c]));
         auto it = ff.angle.find(key);
-        if (it != ff.angle.end()) {
+        if (s.opt.use_angle) {
+          if (it == ff.angle.end()) throw std::runtime_error("no angle parameters at atom " + std::to_string(b + 1));
           s.angle_abc.insert(s.angle_abc.end(), {a, b, c});
           s.angle_kt.insert(s.angle_kt.end(), {it->second.k, it->second.t0});
-        } else if (s.opt.use_valence) {
-          throw std::runtime_error("no angle parameters at atom " + std::to_string(b + 1));
         }
         auto iu = ff.urey.find(key);
-        if (iu != ff.urey.end()) {
+        if (s.opt.use_urey && iu != ff.urey.end()) {
           s.urey_ac.insert(s.urey_ac.end(), {a, c});
           s.urey_kr.insert(s.urey_kr.end(), {iu->second.k, iu->second.r0});
         }
@@ -714,6 +765,17 @@ void setup_system(System& s) {
   build_vdw_tables(s);
   build_valence(s);
   Options& o = s.opt;
+  if (o.use_valence && !o.other_valence_off) {
+    // bond, angle and Urey-Bradley are the valence terms evaluated here; an
+    // atom with three bonds (out-of-plane, stretch-bend) or a 1-4 pair
+    // (torsions) would need terms that are not
+    for (int i = 0; i < n; ++i)
+      if (s.i12[i].size() >= 3 || !s.i14[i].empty())
+        throw std::runtime_error(
+            "valence terms beyond bond / angle / Urey-Bradley are not evaluated: switch them off "
+            "(strbndterm, opbendterm, torsionterm, pitorsterm, tortorterm, angtorterm, strtorterm none) "
+            "or switch the valence terms off (bondterm none)");
+  }
   if (o.vdw_taper <= 0) o.vdw_taper = 0.9 * o.vdw_cut;
   if (o.ewald) {
     if (!s.periodic) throw std::runtime_error("ewald requires a periodic box");

@@ -89,7 +89,12 @@ struct ForceField {
 // Fortran that is not in its tree); callers are expected to state every value
 // that matters for a run.
 struct Options {
-  bool use_mpole = true, use_polar = true, use_vdw = false, use_valence = false;
+  // every term on unless the keywords switch it off, as in Tinker
+  bool use_mpole = true, use_polar = true, use_vdw = true, use_valence = true;
+  bool use_bond = true, use_angle = true, use_urey = true;  // use_valence: any of the three
+  // valence terms this path does not evaluate (stretch-bend, out-of-plane,
+  // torsions, ...): a system that would need them must switch them off
+  bool other_valence_off = false;
   bool ewald = false;
   double ewald_cut = 9.0;
   double ewald_alpha = 0;      // 0: derive from the cutoff (ewaldcof rule)
@@ -161,6 +166,7 @@ struct System {
 
 // Parsing and setup (system.cpp).  All throw std::runtime_error on bad input.
 void read_prm(const std::string& path, ForceField& ff);
+void read_prm_text(const std::string& text, ForceField& ff);  // parameter records of a .key
 void read_xyz(const std::string& path, System& s);
 void read_key_text(const std::string& text, System& s, std::string* prm_name);
 void setup_system(System& s);  // connectivity, groups, frames, scales, vdW

@@ -31,9 +31,13 @@ EXPORTS = [
     "t9_md_get_velocities", "t9_md_run", "t9_guess_nparam", "t9_guess_set_params", "t9_guess_get_params",
     "t9_guess_enable", "t9_guess_capture", "t9_guess_stats", "t9_guess_train", "t9_guess_mlp_forward",
     "t9_guess_mlp_grad", "t9_guess_adam", "t9_guess_features", "t9_guess_history",
+    "t9_energy_terms", "t9_set_coords_device", "t9_get_coords_device", "t9_energy_device", "t9_diag_precond",
+    "t9_sparse_precond_apply", "t9_pme_grid_mpole", "t9_pme_grid_uind", "t9_pme_fft_front", "t9_pme_fft_back",
+    "t9_pme_conv", "t9_pme_fphi_mpole", "t9_pme_fphi_uind", "t9_fft3d", "t9_nblist_pairs", "t9_md_bussi",
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL, CALC_ANALYZE = 1, 2, 4, 8
+TERM_ELEC, TERM_VDW, TERM_VALENCE, TERM_ALL = 1, 2, 4, 7
 
 
 class T9Error(RuntimeError):
@@ -126,6 +130,23 @@ def lib():
             ("t9_guess_adam", [_vp, _f, ctypes.c_double]),
             ("t9_guess_features", [_vp, _f]),
             ("t9_guess_history", [_vp, ctypes.c_int, ctypes.c_int, _d]),
+            ("t9_energy_terms", [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(EnergyOut), _d]),
+            ("t9_set_coords_device", [_vp, _vp]),
+            ("t9_get_coords_device", [_vp, _vp]),
+            ("t9_energy_device", [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(EnergyOut), _vp]),
+            ("t9_diag_precond", [_vp, _d, _d, _d, _d]),
+            ("t9_sparse_precond_apply", [_vp, _d, _d, _d, _d]),
+            ("t9_pme_grid_mpole", [_vp, _d]),
+            ("t9_pme_grid_uind", [_vp, _d, _d, _d]),
+            ("t9_pme_fft_front", [_vp, _d]),
+            ("t9_pme_fft_back", [_vp, _d]),
+            ("t9_pme_conv", [_vp, _d]),
+            ("t9_pme_fphi_mpole", [_vp, _d, _d]),
+            ("t9_pme_fphi_uind", [_vp, _d, _d, _d]),
+            ("t9_fft3d", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            ("t9_nblist_pairs", [_vp, ctypes.c_int, ctypes.c_int64, _i, ctypes.POINTER(ctypes.c_int64)]),
+            ("t9_md_bussi", [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double]),
         ]:
             if hasattr(L, name):  # device entry points are absent from a host-only build
                 getattr(L, name).argtypes = args
@@ -287,6 +308,14 @@ class Context:
         check(lib().t9_nblist_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
         return out
 
+    def nblist_pairs(self, which=0):
+        """Pairs of a stored list (0 multipole, 1 vdW), caller order, i < k: int array [m, 2]."""
+        cnt = ctypes.c_int64(0)
+        check(lib().t9_nblist_pairs(self.h, which, 0, None, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), np.int32)
+        check(lib().t9_nblist_pairs(self.h, which, cnt.value, _p(out, _i), ctypes.byref(cnt)))
+        return out[:cnt.value]
+
     def mpole_init(self):
         check(lib().t9_mpole_init(self.h))
 
@@ -345,6 +374,91 @@ class Context:
         if vers & CALC_ANALYZE:
             e.update(nem=out.nem, nep=out.nep, nev=out.nev)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
+
+    def energy_terms(self, terms, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL, grad=True):
+        """As energy(), for the selected term groups (TERM_ELEC | TERM_VDW | TERM_VALENCE)."""
+        out = EnergyOut()
+        g = np.zeros((self.n, 3)) if grad and (vers & CALC_GRAD) else None
+        check(lib().t9_energy_terms(self.h, vers, terms, ctypes.byref(out), _p(g) if g is not None else None))
+        e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
+                 em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
+                 eangle=out.eangle, eurey=out.eurey)
+        if vers & CALC_ANALYZE:
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev)
+        return e, g, np.array(out.vir[:]).reshape(3, 3)
+
+    def set_coords_device(self, d_xyz_ptr):
+        """Coordinates from a device pointer ([n][3] FP64, caller order)."""
+        check(lib().t9_set_coords_device(self.h, ctypes.c_void_p(int(d_xyz_ptr))))
+
+    def get_coords_device(self, d_xyz_ptr):
+        check(lib().t9_get_coords_device(self.h, ctypes.c_void_p(int(d_xyz_ptr))))
+
+    def energy_device(self, terms, d_grad_ptr, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL):
+        """Energy of the selected terms; the gradient goes to a device pointer
+        ([n][3] FP64, caller order).  Returns the energy dict and the virial."""
+        out = EnergyOut()
+        check(lib().t9_energy_device(self.h, vers, terms, ctypes.byref(out),
+                                     ctypes.c_void_p(int(d_grad_ptr)) if d_grad_ptr else None))
+        e = dict(esum=out.esum, em=out.em, ep=out.ep, evdw=out.evdw, ebond=out.ebond, eangle=out.eangle,
+                 eurey=out.eurey, iterations=out.induce_iterations)
+        return e, np.array(out.vir[:]).reshape(3, 3)
+
+    def _precond(self, fn, rd, rp):
+        a = np.ascontiguousarray(rd, dtype=np.float64)
+        b = np.ascontiguousarray(rp, dtype=np.float64)
+        zd, zp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(fn(self.h, _p(a), _p(b), _p(zd), _p(zp)))
+        return zd, zp
+
+    def diag_precond(self, rd, rp):
+        return self._precond(lib().t9_diag_precond, rd, rp)
+
+    def sparse_precond(self, rd, rp):
+        return self._precond(lib().t9_sparse_precond_apply, rd, rp)
+
+    # ---- PME sub-steps; grids are [K3, K2, K1, 2] FP64 (real, imaginary)
+    def _grid_shape(self):
+        k = [int(self.system.scalar("grid%d" % j)) for j in (1, 2, 3)]
+        return (k[2], k[1], k[0], 2)
+
+    def pme_grid_mpole(self):
+        g = np.zeros(self._grid_shape())
+        check(lib().t9_pme_grid_mpole(self.h, _p(g)))
+        return g
+
+    def pme_grid_uind(self, ud, up):
+        a = np.ascontiguousarray(ud, dtype=np.float64)
+        b = np.ascontiguousarray(up, dtype=np.float64)
+        g = np.zeros(self._grid_shape())
+        check(lib().t9_pme_grid_uind(self.h, _p(a), _p(b), _p(g)))
+        return g
+
+    def _grid_op(self, fn, grid):
+        g = np.ascontiguousarray(grid, dtype=np.float64).copy()
+        check(fn(self.h, _p(g)))
+        return g
+
+    def pme_fft_front(self, grid):
+        return self._grid_op(lib().t9_pme_fft_front, grid)
+
+    def pme_fft_back(self, grid):
+        return self._grid_op(lib().t9_pme_fft_back, grid)
+
+    def pme_conv(self, grid):
+        return self._grid_op(lib().t9_pme_conv, grid)
+
+    def pme_fphi_mpole(self, grid):
+        g = np.ascontiguousarray(grid, dtype=np.float64)
+        f = np.zeros((self.n, 3))
+        check(lib().t9_pme_fphi_mpole(self.h, _p(g), _p(f)))
+        return f
+
+    def pme_fphi_uind(self, grid):
+        g = np.ascontiguousarray(grid, dtype=np.float64)
+        fd, fp = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(lib().t9_pme_fphi_uind(self.h, _p(g), _p(fd), _p(fp)))
+        return fd, fp
 
     # ---- learned initial guess
     def guess_set_params(self, params=None, norm=None, seed=0):
@@ -425,6 +539,10 @@ class Context:
         return dict(epot=st.epot, ekin=st.ekin, temperature=st.temperature, iterations=st.induce_iterations,
                     list_builds=st.list_builds, steps=st.steps)
 
+    def md_bussi(self, dt, tau, kelvin, r, s2):
+        """One Bussi rescaling of the velocities with the given deviates."""
+        check(lib().t9_md_bussi(self.h, float(dt), float(tau), float(kelvin), float(r), float(s2)))
+
     def sync(self):
         check(lib().t9_sync(self.h))
 
@@ -439,6 +557,11 @@ class Context:
         v = ctypes.c_int64()
         check(lib().t9_cutoff_pairs(self.h, ctypes.byref(v)))
         return v.value
+
+
+def fft3d(d_grid_ptr, k1, k2, k3, precision, sign):
+    """In-place 3D transform of a device grid [k3][k2][k1] complex (sign -1 forward, +1 inverse)."""
+    check(lib().t9_fft3d(ctypes.c_void_p(int(d_grid_ptr)), int(k1), int(k2), int(k3), int(precision), int(sign)))
 
 
 def copy_probe(nbytes, reps=20):
