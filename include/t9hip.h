@@ -254,8 +254,10 @@ int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign);
  * dipole and its change, polarizability, type); NN a 32-128-128-6 BF16 MLP
  * with a linear skip, evaluated and trained with MFMA kernels.  The network
  * only changes the starting point: PCG still converges to polar-eps, and a
- * guess whose initial residual exceeds the ASPC guess's (re-measured every
- * 20 solves) is replaced by the ASPC guess.
+ * guess whose initial residual exceeds the ASPC guess's is replaced by the
+ * ASPC guess.  The ASPC residual (one extra matrix-vector product) is
+ * measured on every learned solve with the keyword guess-check 1 (the
+ * default), on every N-th with guess-check N.
  * Parameters: flat FP32 [t9_guess_nparam()] in the order W1 [32][128], b1,
  * W2 [128][128], b2, W3 [160][6] (rows 0..127 hidden, 128..159 skip), b3;
  * norm [76]: feature mean [32], 1/std [32], target mean [6], std [6]. */

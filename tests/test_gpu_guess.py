@@ -163,3 +163,58 @@ def test_bad_learned_guess_falls_back_to_aspc():
     print(f"bad guess: learned solves {st['used']}, fallbacks {st['fallback']}, max|dx| vs ASPC {dx:.2e}", flush=True)
     assert st["fallback"] >= 4 and st["used"] == 0
     assert np.array_equal(A.coords(), B.coords()) and np.array_equal(A.velocities(), B.velocities())
+
+
+def test_fallback_judged_against_this_solve():
+    """The ASPC residual falls once the coordinates move smoothly (straight
+    lines at the MD velocities: the 16 stored solutions become a smooth
+    history); a learned guess with a fixed correction of a chosen size is
+    then worse than this solve's ASPC guess but better than the ASPC residual
+    of the first learned solve.  guess-check 1 compares with this solve's
+    ASPC residual and falls back; guess-check 50 compares with the first
+    solve's and keeps the worse guess."""
+    names = (("aspc", 1), ("exact", 1), ("stale", 50))
+    ctxs = {}
+    for k, chk in names:
+        s = _box(seed=21)
+        s.set_scalar("guess_check", chk)
+        c = t9hip.Context(s, t9hip.T9_FP32)
+        c.md_init(298.0, 13)
+        c.md_run(20, dt=0.002, nrespa=8, bath_kelvin=298.0, tau=0.1)  # identical ASPC histories
+        ctxs[k] = c
+    x0, v0 = ctxs["aspc"].coords(), ctxs["aspc"].velocities()
+
+    def solve_path(c, steps=24):
+        out = []
+        for k in range(steps):
+            c.set_coords(x0 + (k + 1) * 0.002 * v0)
+            st = c.induce(copy_out=False)[2]
+            out.append((st.predictor, st.guess_rms))
+        return out
+
+    ref = solve_path(ctxs["aspc"])
+    eps_first, eps_last = ref[0][1], ref[-1][1]
+    print(f"ASPC residual along the path: first {eps_first:.3e} D, last {eps_last:.3e} D", flush=True)
+    assert eps_first > 100 * eps_last  # the scenario: ASPC improves by orders of magnitude
+    # a network that outputs a fixed local-frame correction of size m on every
+    # atom (all weights zero, the target mean = the correction); its residual
+    # is about 8 m Debye, placed between the two ASPC residuals
+    target = np.sqrt(eps_first * eps_last)
+    m = target / 8.0
+    norm = np.zeros(76, np.float32)
+    norm[32:64] = 1.0
+    norm[64:70] = [m, 0, 0, m, 0, 0]
+    norm[70:76] = 1.0
+    zeros = np.zeros(t9hip.lib().t9_guess_nparam(), np.float32)
+    for k in ("exact", "stale"):
+        ctxs[k].guess_set_params(zeros, norm)
+        ctxs[k].guess_enable(True)
+    ex, st = solve_path(ctxs["exact"]), solve_path(ctxs["stale"])
+    for q in (0, len(ref) - 1):
+        print(f"solve {q}: ASPC {ref[q][1]:.3e}; exact predictor {ex[q][0]} guess {ex[q][1]:.3e}; "
+              f"stale predictor {st[q][0]} guess {st[q][1]:.3e}", flush=True)
+    # the stale reference keeps the learned guess although this solve's ASPC guess is better
+    assert st[-1][0] == 2 and st[-1][1] > 3 * ref[-1][1]
+    # checked against this solve's ASPC residual, it falls back to ASPC
+    assert ex[-1][0] == 1
+    assert t9hip.Context is not None

@@ -17,8 +17,11 @@
 // direct-field guess.  The history is kept in caller order so a re-sort of
 // the atoms at a list rebuild does not disturb it.  With the learned guess on
 // (guess.hip) the ASPC guess is corrected by the network; the corrected guess
-// is kept only when its initial residual is not worse than the ASPC guess's
-// (re-measured every 20 solves), else the solve starts from ASPC.
+// is kept only when its initial residual is not worse than the ASPC guess's,
+// else the solve starts from ASPC.  The ASPC residual costs one extra
+// matrix-vector product: it is measured on every learned solve
+// (guess-check 1, the default) or on every N-th (guess-check N, the
+// reference value then being up to N - 1 solves old).
 #include <array>
 
 #include <algorithm>
@@ -414,10 +417,10 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   if (aspc) guess_save_fields(c);
   bool learned_try = false;  // the learned guess is in use; judged at the first readback
   if (learned) {
-    // the ASPC residual is re-measured every 50 learned solves (one extra
-    // matrix-vector product); between checks the learned guess is compared
-    // with that reference at the first iteration's readback
-    if (c.guess_steps % 50 == 0) {
+    // the ASPC residual of this solve (guess-check 1), or of every N-th
+    // learned solve; the learned guess is compared with it at the first
+    // iteration's readback
+    if (c.guess_steps % std::max(1, c.opt.guess_check) == 0) {
       residual();
       c.aspc_eps0_ref = residual_rms();
     }
