@@ -38,7 +38,7 @@ sys.path.insert(0, os.path.join(REPO, "tinker-gpu_amd"))
 NMOL, NSIDE, BOX, GRID, SEED = 32000, 32, 98.65, 120, 2024
 # the 648-atom box of the same construction (216 waters, K = 24)
 KEY0 = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid 24 24 24\npme-order 5\npolar-eps 1e-6\n"
-        "polarization mutual\nneighbor-list\nlist-buffer 1.5\npcg-precond\nusolve-cutoff 4.5\nusolve-diag 2.0\n"
+        "polarization mutual\nneighbor-list\nlist-buffer 0.3\npcg-precond\nusolve-cutoff 4.5\nusolve-diag 2.0\n"
         "vdwterm\nbondterm\n")
 KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid {K} {K} {K}\npme-order 5\npolar-eps 1e-6\n"
        "polar-iter 100\npolarization mutual\nneighbor-list\nlist-buffer {buf}\npcg-precond\npcg-guess\n"

@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid 24 24 24\npme-order 5\npolar-eps 1e-6\n"
        "polarization mutual\nneighbor-list\nlist-buffer 1.5\nusolve-cutoff 4.5\nusolve-diag 2.0\n")
+FIELDS = "vdwterm none\n"
 TOL = {t9hip.T9_FP64: 1e-10, t9hip.T9_FP32: 2e-5}
 
 
@@ -24,7 +25,7 @@ def rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
 
 
-def water216(extra=""):
+def water216(extra=FIELDS):
     return t9hip.System.water_box(216, 6, 18.643, 2024, key=KEY + extra)
 
 
@@ -92,7 +93,7 @@ def test_preconditioners(prec):
     zd, zp = ctx.diag_precond(rd, rp)
     od, op = o.precond(rd, rp, sparse=False)
     assert rel(zd, od) < TOL[prec] and rel(zp, op) < TOL[prec]
-    s2 = water216("pcg-noprecond\n")
+    s2 = water216(FIELDS + "pcg-noprecond\n")
     with pytest.raises(t9hip.T9Error) as ei:
         t9hip.Context(s2, prec).sparse_precond(rd, rp)
     assert ei.value.code == t9hip.T9_ERR_ARG
@@ -103,7 +104,7 @@ def test_energy_terms_no_double_count():
     and the electrostatics gradient, the other groups theirs; the parts add
     up to the all-terms call."""
     print("case energy term selection", flush=True)
-    s = water216("vdwterm\nbondterm\n")
+    s = water216("vdw-cutoff 7.0\n")
     o = oracle.Oracle(s.arrays())
     ctx = t9hip.Context(s, t9hip.T9_FP64)
     ea, ga, va = ctx.energy()
@@ -129,7 +130,7 @@ def test_device_coordinates_and_gradient():
     the host path's bits."""
     import torch
     print("case device pointers", flush=True)
-    s = water216("vdwterm\nbondterm\n")
+    s = water216("vdw-cutoff 7.0\n")
     xyz = s.f64("xyz", (s.n, 3))
     xyz[5] += 0.01
     h = t9hip.Context(s, t9hip.T9_FP32)

@@ -43,6 +43,10 @@ def test_config_30k_ions_vs_oracle():
         ud, up, st = ctx.induce()
         e, g, v = ctx.energy()
         rel = abs(e["esum"] - e_ref) / abs(e_ref)
+        parts = {"em": (e["em"], eo["em"]), "ep": (e["ep"], eo["ep"]), "evdw": (e["evdw"], mo["evdw"]),
+                 "valence": (e["ebond"] + e["eangle"] + e["eurey"], mo["ebond"] + mo["eangle"] + mo["eurey"])}
+        print("  " + ", ".join(f"{k} {a:.6f} (oracle {b:.6f}, d {a - b:.2e})" for k, (a, b) in parts.items()),
+              flush=True)
         grms = float(np.sqrt(np.mean((g - g_ref) ** 2)))
         drms = float(np.sqrt(np.mean((ud - oud) ** 2)) * DEBYE)
         vrel = float(np.abs(v - v_ref).max() / np.abs(v_ref).max())
@@ -66,6 +70,8 @@ def test_config_288k():
     c32 = t9hip.Context(s, t9hip.T9_FP32)
     e32, g32, v32 = c32.energy()
     rel = abs(e32["esum"] - e64["esum"]) / abs(e64["esum"])
+    print("  " + ", ".join(f"{k} {e32[k]:.6f} (FP64 {e64[k]:.6f}, d {e32[k] - e64[k]:.2e})"
+                           for k in ("em", "ep", "evdw", "ebond", "eangle", "eurey")), flush=True)
     grms = float(np.sqrt(np.mean((g32 - g64) ** 2)))
     vrel = float(np.abs(v32 - v64).max() / np.abs(v64).max())
     print(f"  FP32 vs FP64: energy {e64['esum']:.6f} rel err {rel:.2e}, force RMS err {grms:.2e}, "

@@ -27,7 +27,7 @@ K_LF3X = ("cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\nbondterm non
 K_LF32 = K_LF3X + "ewald\n"
 WATER_KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid {K} {K} {K}\npme-order 5\npolar-eps 1e-6\n"
              "polarization mutual\nneighbor-list\nlist-buffer {buf}\nusolve-cutoff 4.5\nusolve-diag 2.0\n"
-             "vdwterm none\nbondterm none\n")
+             "vdwterm none\nbondterm none\nangleterm none\nureybradterm none\n")
 
 CASES = {
     "lf1": ("local_frame.xyz", K_LF1, True, False),

@@ -187,7 +187,7 @@ def test_fallback_judged_against_this_solve():
     def solve_path(c, steps=24):
         out = []
         for k in range(steps):
-            c.set_coords(x0 + (k + 1) * 0.002 * v0)
+            c.set_coords(x0 + (k + 1) * 0.00005 * v0)
             st = c.induce(copy_out=False)[2]
             out.append((st.predictor, st.guess_rms))
         return out

@@ -16,7 +16,7 @@ DEBYE = 4.803206802
 LF_KEY3 = "usolve-cutoff 0.01\npolarizeterm only\n"
 LF_KEY4 = LF_KEY3 + "ewald\newald-cutoff 7.0\nneighbor-list\nlist-buffer 0.1\na-axis 20.0\n"
 WATER_KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid {K} {K} {K}\npme-order 5\npolar-eps 1e-6\n"
-             "polarization mutual\nneighbor-list\nlist-buffer {buf}\nusolve-cutoff 4.5\nusolve-diag 2.0\n")
+             "polarization mutual\nneighbor-list\nlist-buffer {buf}\nusolve-cutoff 4.5\nusolve-diag 2.0\nvdwterm none\n")
 
 
 def rel_err(a, b):
@@ -156,22 +156,12 @@ def test_bitwise_reproducible(prec):
 
 
 def test_nblist_complete_and_refresh():
-    """Every pair within the cutoff is listed, and a rebuild is triggered only
-    after an atom moves more than half the buffer."""
-    print("case nblist", flush=True)
+    """A rebuild is triggered only after an atom moves more than half the
+    buffer (the pair sets themselves: test_gpu_goldens.py)."""
+    print("case nblist refresh", flush=True)
     s = water(4000, 16, 49.32, 48, buf=1.0)
     ctx = t9hip.Context(s, t9hip.T9_FP32)
-    st = ctx.nblist_stats()
     xyz = s.f64("xyz", (s.n, 3))
-    # brute-force count of directed pairs within 7 + 1 Å (minimum image)
-    L = 49.32
-    cnt = 0
-    for i in range(0, s.n, 1000):
-        d = xyz[i:i + 1000, None, :] - xyz[None, :, :]
-        d -= L * np.round(d / L)
-        r2 = np.sum(d * d, axis=-1)
-        cnt += int(np.sum(r2 < 64.0)) - min(1000, s.n - i)
-    assert abs(int(st[0]) - cnt) <= 2, (st[0], cnt)
     assert not ctx.nblist_refresh()
     xyz2 = xyz.copy()
     xyz2[7, 0] += 0.3  # 0.3 Å < half the 1 Å buffer: keep the list

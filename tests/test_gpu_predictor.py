@@ -11,7 +11,7 @@ import t9hip
 pytestmark = pytest.mark.gpu
 KEY = ("ewald\newald-cutoff 7.0\nvdw-cutoff 9.0\npme-grid 24 24 24\npme-order 5\npolar-eps 1e-6\n"
        "polarization mutual\nneighbor-list\nlist-buffer 1.5\nusolve-cutoff 4.5\nusolve-diag 2.0\n"
-       "polar-predict aspc\n")
+       "polar-predict aspc\nvdwterm none\n")
 
 
 def frames(x0, nframe, seed=11):
