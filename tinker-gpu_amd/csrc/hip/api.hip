@@ -941,16 +941,9 @@ int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign) {
     if (sign != -1 && sign != 1) throw t9::ArgError("sign must be -1 (forward) or +1 (inverse)");
     const int K[3] = {k1, k2, k3};
     for (int d = 0; d < 3; ++d) t9::fft_check_size(K[d], precision);
-    const auto tw = t9::fft_twiddles(K);
-    t9::DBuf dtw;
-    if (precision == T9_FP64) {
-      dtw.alloc(sizeof(double) * tw.size());
-      T9_HIP(hipMemcpy(dtw.p, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> f(tw.begin(), tw.end());
-      dtw.alloc(sizeof(float) * f.size());
-      T9_HIP(hipMemcpy(dtw.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
-    }
+    t9::DBuf dtw;  // the twiddle table of this call
+    dtw.alloc((precision == T9_FP64 ? 16 : 8) * (size_t)(k1 + k2 + k3));
+    t9::fft_twiddles_device(dtw.p, precision, K, nullptr);
     t9::fft3d_device(grid, precision, K, sign, dtw.p, nullptr);
     T9_HIP(hipStreamSynchronize(nullptr));
     return T9_OK;

@@ -309,7 +309,7 @@ void pme_fphi(Ctx& c, bool mpole, void* fd, void* fp);
 // context-free 3D transform of a device grid [K3][K2][K1] (complex FP32 or
 // FP64), line passes in place; tw = forward twiddles of the three axes
 void fft_check_size(int K, int prec);
-std::vector<double> fft_twiddles(const int* K);
+void fft_twiddles_device(void* tw, int prec, const int* K, hipStream_t st);  // (K1 + K2 + K3) complex
 void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st);
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();

@@ -586,14 +586,36 @@ void fft_check_size(int K, int prec) {
   if (2 * (size_t)B * (K + 1) * esize > 65536) throw ArgError("FFT length too large for the LDS line pass");
 }
 
-std::vector<double> fft_twiddles(const int* K) {
-  std::vector<double> tw;
-  for (int d = 0; d < 3; ++d)
-    for (int k = 0; k < K[d]; ++k) {
-      tw.push_back(std::cos(2 * M_PI * k / K[d]));
-      tw.push_back(-std::sin(2 * M_PI * k / K[d]));
-    }
-  return tw;
+namespace {
+// Forward twiddles of the three axes back to back: (cos, -sin)(2 pi k / K),
+// evaluated in FP64 (sincospi: the angle's argument reduction is exact).
+template <class real>
+__global__ void k_fft_twiddles(int K1, int K2, int K3, Cplx<real>* __restrict__ tw) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= K1 + K2 + K3) return;
+  int K = K1, k = t;
+  if (t >= K1 + K2) {
+    K = K3;
+    k = t - K1 - K2;
+  } else if (t >= K1) {
+    K = K2;
+    k = t - K1;
+  }
+  double sn, cs;
+  sincospi(2.0 * k / K, &sn, &cs);
+  tw[t] = Cplx<real>{(real)cs, (real)-sn};
+}
+}  // namespace
+
+void fft_twiddles_device(void* tw, int prec, const int* K, hipStream_t st) {
+  const int tot = K[0] + K[1] + K[2];
+  if (prec == 32)
+    hipLaunchKernelGGL(k_fft_twiddles<float>, dim3(ceil_div(tot, 256)), dim3(256), 0, st, K[0], K[1], K[2],
+                       (Cplx<float>*)tw);
+  else
+    hipLaunchKernelGGL(k_fft_twiddles<double>, dim3(ceil_div(tot, 256)), dim3(256), 0, st, K[0], K[1], K[2],
+                       (Cplx<double>*)tw);
+  T9_LAUNCH_CHECK();
 }
 
 void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st) {
