@@ -243,6 +243,15 @@ int t9_pme_fphi_uind(t9_ctx* ctx, const double* grid, double* fd, double* fp);
  * Replaces the cuFFT plan + cufftExecC2C/Z2Z pair of the reference's FFT
  * unit (src/cudart/fft.cpp:16-99); allocates its twiddle table per call. */
 int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign);
+/* The same transform without any device allocation: the twiddle table lives
+ * in a static per-GPU scratch array of 12,288 bytes (enough for three axes of
+ * up to 255 points in T9_FP64), filled by a kernel on every call.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Results are bit-identical to t9_fft3d.  Every call on
+ * one GPU shares that GPU's table: calls that overlap on one GPU race.
+ * T9_ERR_ARG for lengths beyond the table (or not products of 2, 3, 5). */
+int t9_fft3d_noalloc(void* grid, int k1, int k2, int k3, int precision, int sign);
 
 /* ------------------------------------------------------- learned guess */
 /* A learned initial guess for the induced dipoles.  No counterpart in the

@@ -950,6 +950,19 @@ int t9_fft3d(void* grid, int k1, int k2, int k3, int precision, int sign) {
   }, T9_ERR_HIP);
 }
 
+int t9_fft3d_noalloc(void* grid, int k1, int k2, int k3, int precision, int sign) {
+  return guarded([&] {
+    if (!grid) throw t9::ArgError("null grid");
+    if (precision != T9_FP32 && precision != T9_FP64) throw t9::ArgError("precision must be T9_FP32 or T9_FP64");
+    if (sign != -1 && sign != 1) throw t9::ArgError("sign must be -1 (forward) or +1 (inverse)");
+    const int K[3] = {k1, k2, k3};
+    for (int d = 0; d < 3; ++d) t9::fft_check_size(K[d], precision);
+    t9::fft3d_noalloc(grid, precision, K, sign);
+    T9_HIP(hipStreamSynchronize(nullptr));
+    return T9_OK;
+  }, T9_ERR_HIP);
+}
+
 int t9_md_init(t9_ctx* ctx, double kelvin, uint64_t seed) {
   return guarded([&] {
     auto& c = ctx->c;

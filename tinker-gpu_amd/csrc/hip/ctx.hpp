@@ -311,6 +311,8 @@ void pme_fphi(Ctx& c, bool mpole, void* fd, void* fp);
 void fft_check_size(int K, int prec);
 void fft_twiddles_device(void* tw, int prec, const int* K, hipStream_t st);  // (K1 + K2 + K3) complex
 void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st);
+// the same transform with the twiddles in a static per-device table: no allocation
+void fft3d_noalloc(void* grid, int prec, const int* K, int sign);
 int induce_pcg(Ctx& c, int* iters, double* rms);
 std::array<double, 16> aspc_coefficients();
 // learned guess (guess.hip): nparam of the network, allocation, per-solve hooks

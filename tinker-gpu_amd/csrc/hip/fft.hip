@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "ctx.hpp"
 #include "fft_core.hpp"
@@ -621,6 +623,45 @@ void fft_twiddles_device(void* tw, int prec, const int* K, hipStream_t st) {
 void fft3d_device(void* grid, int prec, const int* K, int sign, const void* tw, hipStream_t st) {
   if (prec == 32) fft3d_lines<float>((Cplx<float>*)grid, K, sign, (const Cplx<float>*)tw, st);
   else fft3d_lines<double>((Cplx<double>*)grid, K, sign, (const Cplx<double>*)tw, st);
+}
+
+// Scratch of the allocation-free transform: the twiddle table of the three
+// axes, (K1 + K2 + K3) complex values of at most 16 bytes each, K <= 255 per
+// axis.  One copy per GPU, held until the process ends; every call on a GPU
+// rewrites it, so calls that overlap on one GPU race.
+constexpr size_t kFftScratchBytes = 3 * 256 * 16;
+static __device__ __align__(256) unsigned char scratch[kFftScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per device
+// with that device current.
+void* fft_scratch_ptr() {
+  int dev = -1;
+  T9_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) throw ArgError("device index out of range for the FFT scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    T9_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void fft3d_noalloc(void* grid, int prec, const int* K, int sign) {
+  const size_t need = (prec == 32 ? 8 : 16) * ((size_t)K[0] + K[1] + K[2]);
+  if (need > kFftScratchBytes)
+    throw ArgError("FFT lengths " + std::to_string(K[0]) + "+" + std::to_string(K[1]) + "+" + std::to_string(K[2]) +
+                   " need " + std::to_string(need) + " twiddle bytes, above the " +
+                   std::to_string(kFftScratchBytes) + "-byte scratch table of the allocation-free transform");
+  void* tw = fft_scratch_ptr();
+  fft_twiddles_device(tw, prec, K, nullptr);
+  fft3d_device(grid, prec, K, sign, tw, nullptr);
 }
 
 }  // namespace t9

@@ -33,7 +33,7 @@ EXPORTS = [
     "t9_guess_mlp_grad", "t9_guess_adam", "t9_guess_features", "t9_guess_history",
     "t9_energy_terms", "t9_set_coords_device", "t9_get_coords_device", "t9_energy_device", "t9_diag_precond",
     "t9_sparse_precond_apply", "t9_pme_grid_mpole", "t9_pme_grid_uind", "t9_pme_fft_front", "t9_pme_fft_back",
-    "t9_pme_conv", "t9_pme_fphi_mpole", "t9_pme_fphi_uind", "t9_fft3d", "t9_nblist_pairs", "t9_md_bussi",
+    "t9_pme_conv", "t9_pme_fphi_mpole", "t9_pme_fphi_uind", "t9_fft3d", "t9_fft3d_noalloc", "t9_nblist_pairs", "t9_md_bussi",
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL, CALC_ANALYZE = 1, 2, 4, 8
@@ -144,6 +144,7 @@ def lib():
             ("t9_pme_fphi_mpole", [_vp, _d, _d]),
             ("t9_pme_fphi_uind", [_vp, _d, _d, _d]),
             ("t9_fft3d", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            ("t9_fft3d_noalloc", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             ("t9_nblist_pairs", [_vp, ctypes.c_int, ctypes.c_int64, _i, ctypes.POINTER(ctypes.c_int64)]),
             ("t9_md_bussi", [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double]),
@@ -562,6 +563,12 @@ class Context:
 def fft3d(d_grid_ptr, k1, k2, k3, precision, sign):
     """In-place 3D transform of a device grid [k3][k2][k1] complex (sign -1 forward, +1 inverse)."""
     check(lib().t9_fft3d(ctypes.c_void_p(int(d_grid_ptr)), int(k1), int(k2), int(k3), int(precision), int(sign)))
+
+
+def fft3d_noalloc(d_grid_ptr, k1, k2, k3, precision, sign):
+    """The same transform through the allocation-free entry point (static per-GPU twiddle table)."""
+    check(lib().t9_fft3d_noalloc(ctypes.c_void_p(int(d_grid_ptr)), int(k1), int(k2), int(k3), int(precision),
+                                 int(sign)))
 
 
 def copy_probe(nbytes, reps=20):
