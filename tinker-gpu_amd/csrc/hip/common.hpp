@@ -47,7 +47,10 @@ __device__ __forceinline__ float t9_exp(float x) { return expf(x); }
 __device__ __forceinline__ double t9_exp(double x) { return exp(x); }
 __device__ __forceinline__ float t9_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double t9_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float t9_rsqrt(float x) { return rsqrtf(x); }
+// 1/sqrt(x) of a pair distance squared: the hardware reciprocal square root
+// (1 ulp) without the library's denormal-input scaling, which a squared
+// distance of two atoms never needs.
+__device__ __forceinline__ float t9_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double t9_rsqrt(double x) { return rsqrt(x); }
 
 // erfc: the device library's erfcf / erfc.  The reference's FP32 build uses
@@ -57,28 +60,48 @@ __device__ __forceinline__ double t9_rsqrt(double x) { return rsqrt(x); }
 __device__ __forceinline__ float t9_erfc_ex(float x, float /*expmx2*/) { return erfcf(x); }
 __device__ __forceinline__ double t9_erfc_ex(double x, double /*expmx2*/) { return erfc(x); }
 
+// exp(x) for the Thole damping factors, x <= 0: FP32 uses the hardware
+// base-2 exponential of x log2(e) directly.  Its relative error grows as
+// |x| 6e-8, so the damping terms 1 - e, 1 - e (1 - x), ... carry an absolute
+// error below max |x| e^x 6e-8 = 2.2e-8; results below 2^-126 flush to 0.
+__device__ __forceinline__ float t9_exp_damp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ double t9_exp_damp(double x) { return exp(x); }
+
 // erfc(z) together with exp(-z^2) for z >= 0, the pair needed by the Ewald
-// real-space recursion.  FP32: exp(-z^2) with z^2 split exactly into hi + lo,
-// and erfc = t exp(-z^2) exp(P(t)), t = 1/(1 + z/2), P the 10-term Chebyshev
-// fit of Numerical Recipes' erfcc (W. H. Press et al., Numerical Recipes,
-// 2nd ed., sec. 6.2; fractional error < 1.2e-7 in exact arithmetic, 1.3e-6
-// worst / 1.7e-7 mean in FP32 over 0 <= z <= 4).  FP64: library routines.
+// real-space recursion.  FP32 (one hardware exponential):
+//  * exp(-z^2): z^2 = hi + lo exactly (FMA), hi log2(e) = y + ylo in two
+//    floats, exp(-z^2) = 2^-y (1 - ylo ln 2 - lo); relative error 1.2e-7;
+//  * erfc(z) = exp(-z^2) t Q(s), t = 1 / (1 + z/2), s = (8t - 5) / 3 in
+//    [-1, 1] for 0 <= z <= 6, Q a degree-9 polynomial for erfc(z) exp(z^2)/t
+//    (smooth in t down to t = 0): least-squares fit on Chebyshev nodes
+//    reweighted toward the largest relative error, then its FP32
+//    coefficients moved by a few ulps to cancel the mean error of the FP32
+//    evaluation over the pair distances of a 7 Å Ewald cutoff.  Error of the
+//    FP32 evaluation: 3.6e-7 at most, 9.3e-8 RMS over 0 <= z <= 6, mean
+//    (weighted by erfc(z) z, the energy of a shell of pairs) below 1e-8 over
+//    0.4 <= z <= 3.9; beyond z = 6 (erfc < 2e-17) the relative error grows.
+// FP64: library routines.
 __device__ __forceinline__ float t9_erfc_pair(float z, float& expmz2) {
   const float hi = z * z;
   const float lo = fmaf(z, z, -hi);
-  expmz2 = expf(-hi) * (1.0f - lo);
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
-  float p = 0.17087277f;
-  p = fmaf(t, p, -0.82215223f);
-  p = fmaf(t, p, 1.48851587f);
-  p = fmaf(t, p, -1.13520398f);
-  p = fmaf(t, p, 0.27886807f);
-  p = fmaf(t, p, -0.18628806f);
-  p = fmaf(t, p, 0.09678418f);
-  p = fmaf(t, p, 0.37409196f);
-  p = fmaf(t, p, 1.00002368f);
-  p = fmaf(t, p, -1.26551223f);
-  return t * expmz2 * __expf(p);
+  const float y = hi * 1.44269504088896341f;
+  float ylo = fmaf(hi, 1.44269504088896341f, -y);
+  ylo = fmaf(hi, 1.92596299e-8f, ylo);  // log2(e) - (float)log2(e)
+  const float e0 = __builtin_amdgcn_exp2f(-y);
+  expmz2 = fmaf(e0, fmaf(-ylo, 0.693147181f, -lo), e0);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
+  const float q = fmaf(2.666666746e+00f, t, -1.666666627e+00f);
+  float p = 4.076925961e-06f;
+  p = fmaf(p, q, -2.033701276e-05f);
+  p = fmaf(p, q, -2.042249071e-05f);
+  p = fmaf(p, q, 3.052748507e-04f);
+  p = fmaf(p, q, -1.570393651e-04f);
+  p = fmaf(p, q, -4.181238357e-03f);
+  p = fmaf(p, q, 2.516048728e-03f);
+  p = fmaf(p, q, 8.378951252e-02f);
+  p = fmaf(p, q, 3.121041656e-01f);
+  p = fmaf(p, q, 6.056598425e-01f);
+  return t * expmz2 * p;
 }
 __device__ __forceinline__ double t9_erfc_pair(double z, double& expmz2) {
   expmz2 = exp(-z * z);

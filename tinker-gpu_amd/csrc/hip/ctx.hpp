@@ -193,6 +193,7 @@ struct Ctx {
   long long rpole_stamp = 0;     // incremented by every rotation of the multipoles
   bool conv_table_valid = false; // false after a box change
   int fft_plane = -1;            // 1: x and y transforms per z-plane in LDS (FP32, square planes); -1 undecided
+  int gather_tiled = -1;        // 1: tile-staged PME gathers, 0: one thread per atom; -1 undecided
 
   // ---- energy / gradient / virial (sorted order, FP64)
   DBuf egrad;                    // double [n][3] dE/dx

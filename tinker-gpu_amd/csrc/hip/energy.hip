@@ -201,7 +201,7 @@ template <class real>
 __device__ __forceinline__ void thole4(real r, real rpdik, real pga, real* lam) {
   const real ratio = r * rpdik;
   const real a = rpdik == 0 ? (real)1e16 : pga * ratio * ratio * ratio;
-  const real e = t9_exp(-a);
+  const real e = t9_exp_damp(-a);
   lam[1] = 1 - e;
   lam[2] = 1 - e * (1 + a);
   lam[3] = 1 - e * (1 + a + (real)0.6 * a * a);

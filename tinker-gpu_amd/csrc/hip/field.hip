@@ -43,7 +43,7 @@ template <class real>
 __device__ __forceinline__ void thole_r(real r, real rpdik, real pga, real& s3, real& s5, real& s7) {
   const real ratio = r * rpdik;
   const real damp = rpdik == 0 ? (real)-1e16 : -pga * ratio * ratio * ratio;
-  const real e = t9_exp(damp);
+  const real e = t9_exp_damp(damp);
   s3 = 1 - e;
   s5 = 1 - e * (1 - damp);
   s7 = 1 - e * (1 - damp + (real)0.6 * damp * damp);
@@ -58,13 +58,13 @@ __device__ __forceinline__ void ewald_b(real* bn, real r, real rinv, real rr2, c
   if (NB > 3) bn[3 < NB ? 3 : 0] = (5 * bn[2 < NB ? 2 : 0] + ek.e3 * ex) * rr2;
 }
 
-// Field of induced dipole pair (ud_k, up_k) at site i, accumulated into
-// a (d) and b (p).  (xr, yr, zr) = x_k - x_i.
+// Field of induced dipole pair (ud_k, up_k) at site i, added to o[0..2] (d)
+// and o[3..5] (p) in the working precision.  (xr, yr, zr) = x_k - x_i.
 template <class real, int EWALD, int UTH>
 __device__ __forceinline__ void ufield_pair(uint32_t ent, const R4<real>& pi, const R4<real>& pj, real dx, real dy,
                                             real dz, real px, real py, real pz, int jpi, int jpk,
                                             const real* __restrict__ thlval, real pga_u, const ScaleTable& sc,
-                                            const BoxT<real>& box, const EwaldK<real>& ek, double* a, double* b) {
+                                            const BoxT<real>& box, const EwaldK<real>& ek, real* o) {
   real xr, yr, zr;
   pair_disp(pi, pj, box, xr, yr, zr);
   const real r2 = xr * xr + yr * yr + zr * zr;
@@ -90,12 +90,12 @@ __device__ __forceinline__ void ufield_pair(uint32_t ent, const R4<real>& pi, co
   }
   const real cd = c2 * (dx * xr + dy * yr + dz * zr);
   const real cp = c2 * (px * xr + py * yr + pz * zr);
-  a[0] += (double)(cd * xr - c1 * dx);
-  a[1] += (double)(cd * yr - c1 * dy);
-  a[2] += (double)(cd * zr - c1 * dz);
-  b[0] += (double)(cp * xr - c1 * px);
-  b[1] += (double)(cp * yr - c1 * py);
-  b[2] += (double)(cp * zr - c1 * pz);
+  o[0] += cd * xr - c1 * dx;
+  o[1] += cd * yr - c1 * dy;
+  o[2] += cd * zr - c1 * dz;
+  o[3] += cp * xr - c1 * px;
+  o[4] += cp * yr - c1 * py;
+  o[5] += cp * zr - c1 * pz;
 }
 
 template <class real, int EWALD, int UTH>
@@ -115,8 +115,6 @@ __global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const in
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   double ab[6] = {0, 0, 0, 0, 0, 0};
-  double* a = ab;
-  double* b = ab + 3;
   if (s < n) {
     const int len = ncut[s];
     const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
@@ -134,27 +132,33 @@ __global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const in
       const real d1x = ud[3 * j1], d1y = ud[3 * j1 + 1], d1z = ud[3 * j1 + 2];
       const real q1x = up[3 * j1], q1y = up[3 * j1 + 1], q1z = up[3 * j1 + 2];
       const int t0 = UTH ? 0 : jpolar[j0], t1 = UTH ? 0 : jpolar[j1];
-      ufield_pair<real, EWALD, UTH>(e0, pi, p0, d0x, d0y, d0z, q0x, q0y, q0z, jpi, t0, thlval, pga_u, sc, box, ek, a,
-                                    b);
-      ufield_pair<real, EWALD, UTH>(e1, pi, p1, d1x, d1y, d1z, q1x, q1y, q1z, jpi, t1, thlval, pga_u, sc, box, ek, a,
-                                    b);
+      // the two pairs are summed in the working precision, then added to the
+      // FP64 accumulators (one conversion per component for both)
+      real o[6] = {0, 0, 0, 0, 0, 0};
+      ufield_pair<real, EWALD, UTH>(e0, pi, p0, d0x, d0y, d0z, q0x, q0y, q0z, jpi, t0, thlval, pga_u, sc, box, ek, o);
+      ufield_pair<real, EWALD, UTH>(e1, pi, p1, d1x, d1y, d1z, q1x, q1y, q1z, jpi, t1, thlval, pga_u, sc, box, ek, o);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ab[q] += (double)o[q];
     }
     if (k < len) {
       const uint32_t e0 = e[(long long)k * 64];
       const int j0 = (int)(e0 >> kCodeBits);
       const int t0 = UTH ? 0 : jpolar[j0];
+      real o[6] = {0, 0, 0, 0, 0, 0};
       ufield_pair<real, EWALD, UTH>(e0, pi, pos[j0], ud[3 * j0], ud[3 * j0 + 1], ud[3 * j0 + 2], up[3 * j0],
-                                    up[3 * j0 + 1], up[3 * j0 + 2], jpi, t0, thlval, pga_u, sc, box, ek, a, b);
+                                    up[3 * j0 + 1], up[3 * j0 + 2], jpi, t0, thlval, pga_u, sc, box, ek, o);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ab[q] += (double)o[q];
     }
   }
   parts_reduce<kPairParts, 6>(ab, sh);
   if (part == 0 && s < n) {
-    fd[3 * s] = (real)a[0];
-    fd[3 * s + 1] = (real)a[1];
-    fd[3 * s + 2] = (real)a[2];
-    fp[3 * s] = (real)b[0];
-    fp[3 * s + 1] = (real)b[1];
-    fp[3 * s + 2] = (real)b[2];
+    fd[3 * s] = (real)ab[0];
+    fd[3 * s + 1] = (real)ab[1];
+    fd[3 * s + 2] = (real)ab[2];
+    fp[3 * s] = (real)ab[3];
+    fp[3 * s + 1] = (real)ab[4];
+    fp[3 * s + 2] = (real)ab[5];
   }
 }
 
@@ -252,6 +256,42 @@ __global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const in
   }
 }
 
+// One sparse-preconditioner pair: alpha_i alpha_k T_ik r_k added to o[0..2]
+// (d) and o[3..5] (p).
+template <class real, int UTH>
+__device__ __forceinline__ void precond_pair(uint32_t ent, const R4<real>& pi, real poli,
+                                             const R4<real>* __restrict__ pos, const real* __restrict__ polarity,
+                                             const int* __restrict__ jpolar, const real* __restrict__ thlval, int jpi,
+                                             real pga_u, const ScaleTable& sc, const BoxT<real>& box,
+                                             const real* __restrict__ rd, const real* __restrict__ rp, real* o) {
+  const int j = (int)(ent >> kCodeBits);
+  const R4<real> pj = pos[j];
+  real xr, yr, zr;
+  pair_disp(pi, pj, box, xr, yr, zr);
+  const real r2 = xr * xr + yr * yr + zr * zr;
+  const real rinv = t9_rsqrt(r2);
+  const real r = r2 * rinv;
+  const real pga = UTH ? pga_u : thlval[jpi + jpolar[j]];
+  real s3, s5, s7;
+  thole_r(r, pi.w * pj.w, pga, s3, s5, s7);
+  const int code = (int)(ent & (kMaxCodes - 1));
+  const real us = code ? (real)sc.u[code] : (real)1;
+  const real polik = poli * polarity[j];
+  const real ri3 = rinv * rinv * rinv;
+  const real rr3 = us * s3 * polik * ri3;
+  const real rr5 = 3 * us * s5 * polik * ri3 * rinv * rinv;
+  const real m0 = rr5 * xr * xr - rr3, m1 = rr5 * xr * yr, m2 = rr5 * xr * zr;
+  const real m3 = rr5 * yr * yr - rr3, m4 = rr5 * yr * zr, m5 = rr5 * zr * zr - rr3;
+  const real x0 = rd[3 * j], x1 = rd[3 * j + 1], x2 = rd[3 * j + 2];
+  const real y0 = rp[3 * j], y1 = rp[3 * j + 1], y2 = rp[3 * j + 2];
+  o[0] += m0 * x0 + m1 * x1 + m2 * x2;
+  o[1] += m1 * x0 + m3 * x1 + m4 * x2;
+  o[2] += m2 * x0 + m4 * x1 + m5 * x2;
+  o[3] += m0 * y0 + m1 * y1 + m2 * y2;
+  o[4] += m1 * y0 + m3 * y1 + m4 * y2;
+  o[5] += m2 * y0 + m4 * y1 + m5 * y2;
+}
+
 // z = udiag * alpha * r + sum_{r_ik <= ucut} alpha_i alpha_k T_ik r_k
 // (sparse preconditioner, src/acc/amoeba/induce.cpp:310-420).
 template <class real, int UTH>
@@ -287,36 +327,24 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
     const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
     const R4<real> pi = pos[s];
     const int jpi = UTH ? 0 : jpolar[s] * nj;
-    uint32_t nxt = part < len ? e[(long long)part * 64] : 0;
-    for (int k = part; k < len; k += kPairParts) {
-      const uint32_t ent = nxt;
-      if (k + kPairParts < len) nxt = e[(long long)(k + kPairParts) * 64];
-      const int j = (int)(ent >> kCodeBits);
-      const R4<real> pj = pos[j];
-      real xr, yr, zr;
-      pair_disp(pi, pj, box, xr, yr, zr);
-      const real r2 = xr * xr + yr * yr + zr * zr;
-      const real rinv = t9_rsqrt(r2);
-      const real r = r2 * rinv;
-      const real pga = UTH ? pga_u : thlval[jpi + jpolar[j]];
-      real s3, s5, s7;
-      thole_r(r, pi.w * pj.w, pga, s3, s5, s7);
-      const int code = (int)(ent & (kMaxCodes - 1));
-      const real us = code ? (real)sc.u[code] : (real)1;
-      const real polik = poli * polarity[j];
-      const real ri3 = rinv * rinv * rinv;
-      const real rr3 = us * s3 * polik * ri3;
-      const real rr5 = 3 * us * s5 * polik * ri3 * rinv * rinv;
-      const real m0 = rr5 * xr * xr - rr3, m1 = rr5 * xr * yr, m2 = rr5 * xr * zr;
-      const real m3 = rr5 * yr * yr - rr3, m4 = rr5 * yr * zr, m5 = rr5 * zr * zr - rr3;
-      const real x0 = rd[3 * j], x1 = rd[3 * j + 1], x2 = rd[3 * j + 2];
-      const real y0 = rp[3 * j], y1 = rp[3 * j + 1], y2 = rp[3 * j + 2];
-      a0 += (double)(m0 * x0 + m1 * x1 + m2 * x2);
-      a1 += (double)(m1 * x0 + m3 * x1 + m4 * x2);
-      a2 += (double)(m2 * x0 + m4 * x1 + m5 * x2);
-      b0 += (double)(m0 * y0 + m1 * y1 + m2 * y2);
-      b1 += (double)(m1 * y0 + m3 * y1 + m4 * y2);
-      b2 += (double)(m2 * y0 + m4 * y1 + m5 * y2);
+    // two entries per iteration, summed in the working precision before the
+    // FP64 accumulators
+    int k = part;
+    constexpr int P = kPairParts;
+    for (; k + P < len; k += 2 * P) {
+      const uint32_t e0 = e[(long long)k * 64], e1 = e[(long long)(k + P) * 64];
+      real o[6] = {0, 0, 0, 0, 0, 0};
+      precond_pair<real, UTH>(e0, pi, poli, pos, polarity, jpolar, thlval, jpi, pga_u, sc, box, rd, rp, o);
+      precond_pair<real, UTH>(e1, pi, poli, pos, polarity, jpolar, thlval, jpi, pga_u, sc, box, rd, rp, o);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ab[q] += (double)o[q];
+    }
+    if (k < len) {
+      real o[6] = {0, 0, 0, 0, 0, 0};
+      precond_pair<real, UTH>(e[(long long)k * 64], pi, poli, pos, polarity, jpolar, thlval, jpi, pga_u, sc, box, rd,
+                              rp, o);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ab[q] += (double)o[q];
     }
   }
   parts_reduce<kPairParts, 6>(ab, sh);

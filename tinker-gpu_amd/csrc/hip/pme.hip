@@ -20,6 +20,9 @@
 // FFT: one workgroup transforms 16 (FP32) or 8 (FP64) grid lines in LDS with
 // a Stockham auto-sort over radices 4, 2, 3, 5; the influence-function
 // multiply is fused between the forward and inverse transforms of the z pass.
+#include <cstdlib>
+#include <cstring>
+
 #include "bspline.hpp"
 #include "ctx.hpp"
 
@@ -436,6 +439,103 @@ __global__ __launch_bounds__(256) void k_gather_dip(int n, TileGeom g, const R4<
   fp[3 * s + 2] += term * up[3 * s + 2] - a3 * p3;
 }
 
+// Tile-staged gathers: one 64-thread workgroup per spreading tile takes the
+// atoms whose support base lies in that tile (their support spans the tile
+// and the next 4 points per axis), stages that (T+4)^3 region of the grid in
+// LDS with coalesced loads (x fastest, wrapped by K), and each lane gathers
+// one atom's 125 points from LDS.  Each grid point is read from memory by at
+// most 8 workgroups (instead of by every atom whose support covers it), and
+// the per-atom sums run in the same order with the same arithmetic as
+// k_gather_dip / k_gather_mpole_field, so the results are the same bits.
+// MODE 0: dipole grid (d real, p imaginary) -> fd, fp += term u - a grad phi;
+// MODE 1: multipole grid (real part) -> fd and fp += term p - a grad phi.
+constexpr int kGatherThreads = 64;
+
+template <class real, int MODE>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_tiled(TileGeom g, const int* __restrict__ tile_start,
+                                                                 const int* __restrict__ tile_atoms,
+                                                                 const R4<real>* __restrict__ pw,
+                                                                 const int4* __restrict__ pbase,
+                                                                 const cplx_t<real>* __restrict__ grid, real a1,
+                                                                 real a2, real a3, real term,
+                                                                 const real* __restrict__ ud,
+                                                                 const real* __restrict__ up, real* __restrict__ fd,
+                                                                 real* __restrict__ fp, Gate gate) {
+  if (gate_closed(gate)) return;
+  extern __shared__ __align__(16) char smem[];
+  cplx_t<real>* reg = reinterpret_cast<cplx_t<real>*>(smem);
+  const int tt = blockIdx.x;
+  const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
+  const int a0 = tile_start[tt], na = tile_start[tt + 1] - a0;
+  if (na == 0) return;
+  const int lo[3] = {t1 * g.T[0], t2 * g.T[1], t3 * g.T[2]};
+  const int E1 = g.T[0] + 4, E2 = g.T[1] + 4, E3 = g.T[2] + 4;
+  const int npt = E1 * E2 * E3;
+  for (int q = threadIdx.x; q < npt; q += kGatherThreads) {
+    const int qx = q % E1, r = q / E1;
+    const int qy = r % E2, qz = r / E2;
+    int x = lo[0] + qx, y = lo[1] + qy, z = lo[2] + qz;
+    while (x >= g.K[0]) x -= g.K[0];
+    while (y >= g.K[1]) y -= g.K[1];
+    while (z >= g.K[2]) z -= g.K[2];
+    reg[q] = grid[x + (long long)g.K[0] * (y + (long long)g.K[1] * z)];
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < na; a += kGatherThreads) {
+    const int s = tile_atoms[a0 + a];
+    const R4<real> w = pw[s];
+    const int4 b = pbase[s];
+    real th1[2][5], th2[2][5], th3[2][5];
+    bspline5<2>(w.x, th1);
+    bspline5<2>(w.y, th2);
+    bspline5<2>(w.z, th3);
+    const int ox = b.x - lo[0], oy = b.y - lo[1], oz = b.z - lo[2];
+    real d1 = 0, d2 = 0, d3 = 0, p1 = 0, p2 = 0, p3 = 0;
+    for (int m3 = 0; m3 < 5; ++m3) {
+      const real v0 = th3[0][m3], v1 = th3[1][m3];
+      for (int m2 = 0; m2 < 5; ++m2) {
+        const real u0 = th2[0][m2], u1 = th2[1][m2];
+        const cplx_t<real>* row = reg + ox + E1 * ((oy + m2) + E2 * (oz + m3));
+#pragma unroll
+        for (int m1 = 0; m1 < 5; ++m1) {
+          const cplx_t<real> q = row[m1];
+          const real t0 = th1[0][m1], tv = th1[1][m1];
+          if (MODE == 0) {
+            const real w1 = tv * u0 * v0, w2 = t0 * u1 * v0, w3 = t0 * u0 * v1;
+            d1 += q.x * w1;
+            d2 += q.x * w2;
+            d3 += q.x * w3;
+            p1 += q.y * w1;
+            p2 += q.y * w2;
+            p3 += q.y * w3;
+          } else {
+            d1 += q.x * tv * u0 * v0;
+            d2 += q.x * t0 * u1 * v0;
+            d3 += q.x * t0 * u0 * v1;
+          }
+        }
+      }
+    }
+    if (MODE == 0) {
+      fd[3 * s] += term * ud[3 * s] - a1 * d1;
+      fd[3 * s + 1] += term * ud[3 * s + 1] - a2 * d2;
+      fd[3 * s + 2] += term * ud[3 * s + 2] - a3 * d3;
+      fp[3 * s] += term * up[3 * s] - a1 * p1;
+      fp[3 * s + 1] += term * up[3 * s + 1] - a2 * p2;
+      fp[3 * s + 2] += term * up[3 * s + 2] - a3 * p3;
+    } else {
+      const real* p = ud + 10 * (size_t)s;  // the global multipoles
+      const real f1 = term * p[1] - a1 * d1, f2 = term * p[2] - a2 * d2, f3 = term * p[3] - a3 * d3;
+      fd[3 * s] += f1;
+      fd[3 * s + 1] += f2;
+      fd[3 * s + 2] += f3;
+      fp[3 * s] += f1;
+      fp[3 * s + 1] += f2;
+      fp[3 * s + 2] += f3;
+    }
+  }
+}
+
 // Field of the permanent-multipole grid (real part) plus the self term,
 // added to both the d and the p field.
 template <class real>
@@ -495,6 +595,24 @@ TileGeom geom(const Ctx& c) {
   return g;
 }
 
+// LDS bytes of the tile-staged gather: the (T+4)^3 grid region
+template <class real>
+size_t gather_lds(const Ctx& c) {
+  return sizeof(cplx_t<real>) * (size_t)(c.tsz[0] + 4) * (c.tsz[1] + 4) * (c.tsz[2] + 4);
+}
+
+// the tile-staged gathers (T9HIP_GATHER=tiles; measured slower than one
+// thread per atom on the 96,000-atom step) need the bins and a region that
+// fits the LDS budget of one workgroup
+template <class real>
+bool gather_tiled_ok(Ctx& c) {
+  if (c.gather_tiled < 0) {
+    const char* e = std::getenv("T9HIP_GATHER");
+    c.gather_tiled = (e && !std::strcmp(e, "tiles")) ? 1 : 0;
+  }
+  return c.gather_tiled == 1 && c.bins_valid && gather_lds<real>(c) <= 60 * 1024;
+}
+
 template <class real>
 void pme_bin_t(Ctx& c) {
   const int n = c.n;
@@ -547,9 +665,16 @@ void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, 
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
-  hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
-                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
-                     (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
+  if (gather_tiled_ok<real>(c)) {
+    hipLaunchKernelGGL((k_gather_tiled<real, 0>), dim3(c.ntile[0] * c.ntile[1] * c.ntile[2]), dim3(kGatherThreads),
+                       gather_lds<real>(c), c.stream, geom(c), c.tile_start.as<int>(), c.tile_atoms.as<int>(),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
+                       (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
+  } else {
+    hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
+                       (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
+  }
   T9_LAUNCH_CHECK();
 }
 
@@ -573,9 +698,16 @@ void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   KScope ks_(c, KT_GATHER);
-  hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
-                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
-                     c.rpole.as<real>(), (real*)fd, (real*)fp);
+  if (gather_tiled_ok<real>(c)) {
+    hipLaunchKernelGGL((k_gather_tiled<real, 1>), dim3(c.ntile[0] * c.ntile[1] * c.ntile[2]), dim3(kGatherThreads),
+                       gather_lds<real>(c), c.stream, geom(c), c.tile_start.as<int>(), c.tile_atoms.as<int>(),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
+                       c.rpole.as<real>(), (const real*)nullptr, (real*)fd, (real*)fp, Gate{});
+  } else {
+    hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
+                       c.rpole.as<real>(), (real*)fd, (real*)fp);
+  }
   T9_LAUNCH_CHECK();
 }
 
