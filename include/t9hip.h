@@ -262,11 +262,10 @@ int t9_fft3d_noalloc(void* grid, int k1, int k2, int k3, int precision, int sign
  * extrapolation errors of the atom and its frame atoms, current field, last
  * dipole and its change, polarizability, type); NN a 32-128-128-6 BF16 MLP
  * with a linear skip, evaluated and trained with MFMA kernels.  The network
- * only changes the starting point: PCG still converges to polar-eps, and a
- * guess whose initial residual exceeds the ASPC guess's is replaced by the
- * ASPC guess.  The ASPC residual (one extra matrix-vector product) is
- * measured on every learned solve with the keyword guess-check 1 (the
- * default), on every N-th with guess-check N.
+ * only changes the starting point: PCG still converges to polar-eps, and
+ * every solve starts from the better of the learned and the ASPC guess by
+ * RMS residual (the ASPC guess's residual is formed in the same solve: one
+ * extra matrix-vector product, computed beside the direct field).
  * Parameters: flat FP32 [t9_guess_nparam()] in the order W1 [32][128], b1,
  * W2 [128][128], b2, W3 [160][6] (rows 0..127 hidden, 128..159 skip), b3;
  * norm [76]: feature mean [32], 1/std [32], target mean [6], std [6]. */

@@ -76,6 +76,27 @@ def test_pme_substeps(prec):
 
 
 @pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_spread_kernels(prec, monkeypatch):
+    """Both spreading kernels against the oracle: the row-owner kernel (grid
+    sizes divisible by 8: 1,000 waters + 40 ions at K = 40) and the LDS
+    fixed-point kernel (the same box with T9HIP_SPREAD=fixed, and at K = 30,
+    which the row-owner kernel does not take); multipole and dipole grids."""
+    key = KEY.replace("pme-grid 24 24 24", "pme-grid {K} {K} {K}") + FIELDS
+    for K, mode in ((40, "rows"), (40, "fixed"), (30, "rows")):
+        print(f"case spread K={K} {mode} prec={prec}", flush=True)
+        monkeypatch.setenv("T9HIP_SPREAD", mode)
+        s = t9hip.System.water_box(1000, 11, 32.6, 7, 20, 20, key=key.format(K=K))
+        o = oracle.Oracle(s.arrays())
+        ctx = t9hip.Context(s, prec)
+        rng = np.random.default_rng(K)
+        ud, up = rng.standard_normal((2, s.n, 3)) * 0.05
+        em, eu = rel(ctx.pme_grid_mpole(), o.pme_grid_mpole()), rel(ctx.pme_grid_uind(ud, up), o.pme_grid_uind(ud, up))
+        print(f"  multipole grid rel err {em:.2e}, dipole grid {eu:.2e}", flush=True)
+        assert em < TOL[prec] and eu < TOL[prec]
+        ctx.close()
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
 def test_preconditioners(prec):
     """z = M r of the sparse preconditioner (usolve 4.5 Å, udiag 2) and of the
     diagonal one against the oracle (the oracle's M is pinned to the

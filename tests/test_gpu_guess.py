@@ -168,17 +168,13 @@ def test_bad_learned_guess_falls_back_to_aspc():
 def test_fallback_judged_against_this_solve():
     """The ASPC residual falls once the coordinates move smoothly (straight
     lines at the MD velocities: the 16 stored solutions become a smooth
-    history); a learned guess with a fixed correction of a chosen size is
-    then worse than this solve's ASPC guess but better than the ASPC residual
-    of the first learned solve.  guess-check 1 compares with this solve's
-    ASPC residual and falls back; guess-check 50 compares with the first
-    solve's and keeps the worse guess."""
-    names = (("aspc", 1), ("exact", 1), ("stale", 50))
+    history); a learned guess with a fixed correction of a chosen size
+    (about 8 m Debye of residual, between the first and the last ASPC
+    residual) is worse than the last solve's ASPC guess, so that solve
+    starts from the ASPC guess of the same solve."""
     ctxs = {}
-    for k, chk in names:
-        s = _box(seed=21)
-        s.set_scalar("guess_check", chk)
-        c = t9hip.Context(s, t9hip.T9_FP32)
+    for k in ("aspc", "learned"):
+        c = t9hip.Context(_box(seed=21), t9hip.T9_FP32)
         c.md_init(298.0, 13)
         c.md_run(20, dt=0.002, nrespa=8, bath_kelvin=298.0, tau=0.1)  # identical ASPC histories
         ctxs[k] = c
@@ -188,8 +184,8 @@ def test_fallback_judged_against_this_solve():
         out = []
         for k in range(steps):
             c.set_coords(x0 + (k + 1) * 0.00005 * v0)
-            st = c.induce(copy_out=False)[2]
-            out.append((st.predictor, st.guess_rms))
+            ud, up, st = c.induce()
+            out.append((st.predictor, st.guess_rms, st.iterations, ud))
         return out
 
     ref = solve_path(ctxs["aspc"])
@@ -206,15 +202,13 @@ def test_fallback_judged_against_this_solve():
     norm[64:70] = [m, 0, 0, m, 0, 0]
     norm[70:76] = 1.0
     zeros = np.zeros(t9hip.lib().t9_guess_nparam(), np.float32)
-    for k in ("exact", "stale"):
-        ctxs[k].guess_set_params(zeros, norm)
-        ctxs[k].guess_enable(True)
-    ex, st = solve_path(ctxs["exact"]), solve_path(ctxs["stale"])
+    ctxs["learned"].guess_set_params(zeros, norm)
+    ctxs["learned"].guess_enable(True)
+    ex = solve_path(ctxs["learned"])
     for q in (0, len(ref) - 1):
-        print(f"solve {q}: ASPC {ref[q][1]:.3e}; exact predictor {ex[q][0]} guess {ex[q][1]:.3e}; "
-              f"stale predictor {st[q][0]} guess {st[q][1]:.3e}", flush=True)
-    # the stale reference keeps the learned guess although this solve's ASPC guess is better
-    assert st[-1][0] == 2 and st[-1][1] > 3 * ref[-1][1]
-    # checked against this solve's ASPC residual, it falls back to ASPC
-    assert ex[-1][0] == 1
-    assert t9hip.Context is not None
+        print(f"solve {q}: ASPC {ref[q][1]:.3e}; learned predictor {ex[q][0]} guess {ex[q][1]:.3e}", flush=True)
+    # last solve: worse than this solve's ASPC guess, so the solve starts from
+    # the ASPC guess, whose residual is far below the learned guess's
+    assert ex[-1][0] == 1 and ex[-1][1] < 0.3 * target
+    # the answer does not depend on the starting point beyond the tolerance
+    assert np.sqrt(np.mean((ex[-1][3] - ref[-1][3]) ** 2)) * 4.803206802 < 5e-6

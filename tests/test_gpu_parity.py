@@ -172,26 +172,3 @@ def test_nblist_complete_and_refresh():
     ctx.set_coords(xyz2)
     assert ctx.nblist_refresh()
 
-
-@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
-def test_tiled_gather_same_bits(prec, monkeypatch):
-    """The tile-staged PME gathers (grid region in LDS, one lane per atom of
-    the tile's bin) give the same bits as the one-thread-per-atom gathers, for
-    the multipole field (direct field) and the dipole field (mutual field),
-    with ions in the box (3,040 atoms, K = 40), and on the 648-atom box with
-    a 10^3 grid, where one tile spans each axis and the staged region wraps."""
-    for args in ((1000, 11, 32.6, 40, 7, 1.0, 20, 20), (216, 6, 18.643, 10, 2024, 1.0)):
-        print(f"case tiled gather {args[:4]} prec={prec}", flush=True)
-        s = water(*args)
-        rng = np.random.default_rng(5)
-        ud, up = rng.normal(0, 0.02, (s.n, 3)), rng.normal(0, 0.02, (s.n, 3))
-        out = []
-        for mode in ("atoms", "tiles"):
-            monkeypatch.setenv("T9HIP_GATHER", mode)
-            ctx = t9hip.Context(s, prec)
-            fd, fp = ctx.dfield()
-            gd, gp = ctx.ufield(ud, up)
-            out.append((fd, fp, gd, gp))
-            ctx.close()
-        for a, b in zip(*out):
-            assert np.array_equal(a, b)

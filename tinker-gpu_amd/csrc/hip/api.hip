@@ -21,6 +21,7 @@ namespace t9 {
 
 std::vector<double> pme_bsmod(int K, int order);
 int pme_tile_edge(int K);
+void pme_setup_bins(Ctx& c);
 
 namespace {
 
@@ -101,6 +102,10 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   T9_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
   T9_HIP(hipEventCreateWithFlags(&c.ev_ufork, evf));
   T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin, evf));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_ujoin2, evf));
+  T9_HIP(hipStreamCreateWithFlags(&c.side3, hipStreamNonBlocking));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_bfork, evf));
+  T9_HIP(hipEventCreateWithFlags(&c.ev_bins, evf));
   T9_HIP(hipEventCreateWithFlags(&c.ev_read, hipEventDisableTiming));
   hipStream_t st = c.stream;
   const int n = s.n;
@@ -271,6 +276,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
       c.tsz[d] = pme_tile_edge(K);
       c.ntile[d] = K / c.tsz[d];
     }
+    pme_setup_bins(c);
     pme_check_fft_sizes(c);
     upload(c.bsmod, bs, st);
     if (c.prec == 32) {
@@ -283,12 +289,13 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.qgrid.alloc(2 * rs * ng);
     c.pme_w.alloc(rs * 4 * n);
     c.pme_base.alloc(sizeof(int4) * n);
-    const int nt = c.ntile[0] * c.ntile[1] * c.ntile[2];
+    const int nt = c.nbin[0] * c.nbin[1] * c.nbin[2];
     c.tile_count.alloc(sizeof(int) * nt);
     T9_HIP(hipMemsetAsync(c.tile_count.p, 0, sizeof(int) * nt, st));  // each binning's scan clears it again
     c.tile_start.alloc(sizeof(int) * (nt + 1));
     c.tile_cursor.alloc(sizeof(int) * nt);
     c.tile_atoms.alloc(sizeof(int) * n);
+    c.tile_csum.alloc(sizeof(int) * ((nt + 1023) / 1024 + 1));
   }
   alloc_vectors(c);
   if (s.opt.predictor == "ASPC") {
@@ -347,6 +354,7 @@ void prepare(Ctx& c) {
     if (nblist_needs_rebuild(c, true)) nblist_build(c);
     else nblist_mark_refreshed(c);
   }
+  pme_bin_async(c);  // beside the compaction and the rotation
   if (!c.mlist.cut_valid) nblist_compact(c);
   if (!c.rpole_valid) mpole_init(c);
 }
@@ -503,6 +511,10 @@ int t9_ctx_create(const t9_system* sys, int precision, t9_ctx** out) {
       if (h->c.side2) (void)hipStreamDestroy(h->c.side2);
       if (h->c.ev_ufork) (void)hipEventDestroy(h->c.ev_ufork);
       if (h->c.ev_ujoin) (void)hipEventDestroy(h->c.ev_ujoin);
+      if (h->c.ev_ujoin2) (void)hipEventDestroy(h->c.ev_ujoin2);
+      if (h->c.side3) (void)hipStreamDestroy(h->c.side3);
+      if (h->c.ev_bfork) (void)hipEventDestroy(h->c.ev_bfork);
+      if (h->c.ev_bins) (void)hipEventDestroy(h->c.ev_bins);
       if (h->c.ev_read) (void)hipEventDestroy(h->c.ev_read);
       if (h->c.h_scalars) (void)hipHostFree(h->c.h_scalars);
       delete h;
@@ -519,9 +531,11 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->c.stream);
     (void)hipStreamSynchronize(ctx->c.side);
     (void)hipStreamSynchronize(ctx->c.side2);
-    hipStream_t st = ctx->c.stream, side = ctx->c.side, side2 = ctx->c.side2;
+    (void)hipStreamSynchronize(ctx->c.side3);
+    hipStream_t st = ctx->c.stream, side = ctx->c.side, side2 = ctx->c.side2, side3 = ctx->c.side3;
+    hipEvent_t e7 = ctx->c.ev_bfork, e8 = ctx->c.ev_bins;
     hipEvent_t e1 = ctx->c.ev_fork, e2 = ctx->c.ev_join, e3 = ctx->c.ev_ufork, e4 = ctx->c.ev_ujoin;
-    hipEvent_t e5 = ctx->c.ev_read;
+    hipEvent_t e5 = ctx->c.ev_read, e6 = ctx->c.ev_ujoin2;
     double* hs = ctx->c.h_scalars;
     t9::ktimer_release(ctx->c);
     delete ctx;
@@ -534,6 +548,10 @@ int t9_ctx_destroy(t9_ctx* ctx) {
     if (e3) (void)hipEventDestroy(e3);
     if (e4) (void)hipEventDestroy(e4);
     if (e5) (void)hipEventDestroy(e5);
+    if (e6) (void)hipEventDestroy(e6);
+    if (side3) (void)hipStreamDestroy(side3);
+    if (e7) (void)hipEventDestroy(e7);
+    if (e8) (void)hipEventDestroy(e8);
     return T9_OK;
   }, T9_ERR_HIP);
 }
@@ -1088,7 +1106,6 @@ int t9_guess_enable(t9_ctx* ctx, int on) {
     if (on && !c.guess_ready) throw t9::ArgError("the learned guess has no trained parameters");
     if (on && c.opt.predictor != "ASPC") throw t9::ArgError("the learned guess needs polar-predict ASPC");
     c.guess_on = on != 0;
-    c.guess_steps = 0;
     return T9_OK;
   }, T9_ERR_HIP);
 }

@@ -115,7 +115,9 @@ struct Ctx {
   // third stream (default priority): the real-space mutual field runs there
   // while the PME part of the same field spreads and transforms
   hipStream_t side2 = nullptr;
-  hipEvent_t ev_ufork = nullptr, ev_ujoin = nullptr;
+  hipStream_t side3 = nullptr;   // PME binning beside the list compaction
+  hipEvent_t ev_bfork = nullptr, ev_bins = nullptr;
+  hipEvent_t ev_ufork = nullptr, ev_ujoin = nullptr, ev_ujoin2 = nullptr;
   hipEvent_t ev_read = nullptr;  // marks a readback the host waits for by polling
   bool vdw_pending = false;  // vdW pair kernel enqueued on `side`, not yet joined
   Options opt;
@@ -181,19 +183,22 @@ struct Ctx {
   // ---- PME
   int tsz[3] = {8, 8, 8};       // spreading tile edge per axis (divides the grid size)
   int ntile[3] = {1, 1, 1};
+  bool spread_rows = false;      // row-owner spreading (8^3 tiles over 4^3 bins) instead of LDS fixed point
+  int bsz[3] = {8, 8, 8};       // bin edge per axis (4 with row-owner spreading, else the tile edge)
+  int nbin[3] = {1, 1, 1};
   DBuf qgrid;                    // real2 [K1*K2*K3]
   DBuf bsmod;                    // real [K1 + K2 + K3]
   DBuf twiddle;                  // real2 [K1 + K2 + K3] forward tables per axis
   DBuf pme_w, pme_base;          // real4 / int4 [n] per sorted atom
-  DBuf tile_count, tile_start, tile_cursor, tile_atoms;
+  DBuf tile_count, tile_start, tile_cursor, tile_atoms, tile_csum;  // atoms per bin (support base), bin order
   bool bins_valid = false;
+  bool bins_pending = false;     // binning enqueued on side3; the main stream has not waited for it yet
   DBuf conv_table;               // real [K1*K2*K3] influence function (z-pass order)
   DBuf qgrid_m;                  // convolved permanent-multipole grid of the last direct field
   long long qgrid_m_stamp = -1;  // rpole_stamp it belongs to
   long long rpole_stamp = 0;     // incremented by every rotation of the multipoles
   bool conv_table_valid = false; // false after a box change
   int fft_plane = -1;            // 1: x and y transforms per z-plane in LDS (FP32, square planes); -1 undecided
-  int gather_tiled = -1;        // 1: tile-staged PME gathers, 0: one thread per atom; -1 undecided
 
   // ---- energy / gradient / virial (sorted order, FP64)
   DBuf egrad;                    // double [n][3] dE/dx
@@ -216,6 +221,7 @@ struct Ctx {
   DBuf ealt;                     // real [16][2][n][3] caller order: direct fields of the last 16 solves
   int nealt = 0;
   DBuf uaspc;                    // real [2][n][3] sorted: ASPC guess of the current solve
+  DBuf raspc;                    // real [2][n][3] sorted: field, then residual, of the ASPC guess
   DBuf gfeat;                    // float [n][32] sorted raw features of the current solve
   DBuf gde;                      // float [n][6] caller order: field extrapolation errors (d, p)
   DBuf gp, gm, gv, gpb, ggrad;   // FP32 params / Adam moments, bf16 copy, gradient
@@ -228,8 +234,6 @@ struct Ctx {
   bool guess_ready = false;      // parameters and normalization present
   bool guess_on = false;         // use the learned guess when the history allows
   int adam_t = 0;
-  double aspc_eps0_ref = 0;      // ASPC guess residual at the last reference check (Debye)
-  int guess_steps = 0;           // learned solves since the last reference check
   long long guess_used = 0, guess_fallback = 0;
   // training capture: raw features and local-frame targets of every stride-th atom
   DBuf cap_x, cap_y;             // float [cap_max][32], [cap_max][6]
@@ -290,15 +294,22 @@ void nblist_mark_refreshed(Ctx& c);
 void nblist_compact(Ctx& c);              // refresh the in-cutoff lists for the current coordinates
 void refresh_sorted_coords(Ctx& c);
 void mpole_init(Ctx& c);
-void dfield_all(Ctx& c);                  // c.field/fieldp = full direct field
+// c.field/fieldp = full direct field; with ua, also fa = the mutual field of
+// the dipoles ua ([2][n][3] sorted: d, then p), computed beside it
+void dfield_all(Ctx& c, const void* ua = nullptr, void* fa = nullptr);
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp);  // sorted real vectors
 void pme_bin(Ctx& c);
+void pme_bin_async(Ctx& c);  // binning of the current coordinates on side3
 // forward FFT, influence function, inverse FFT of c.qgrid; `lines`: x and y
 // as line passes even where the plane kernel applies (same bits)
 void pme_fft_roundtrip(Ctx& c, bool lines = false);
 void pme_check_fft_sizes(const Ctx& c);
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self
 void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather = nullptr);  // adds recip+self of the permanent multipoles
+// the same, then recip+self of the dipoles (ud, up) added to (gd, gp); each
+// gather waits for its event
+void pme_mpole_dipole_fields(Ctx& c, void* fd, void* fp, hipEvent_t e1, const void* ud, const void* up, void* gd,
+                             void* gp, hipEvent_t e2);
 // PME sub-steps on c.qgrid: spreading, one-direction transforms (sign -1
 // forward, +1 inverse), the influence function alone, and the field of the
 // grid at the atoms (fd, fp overwritten, no self term)

@@ -995,6 +995,7 @@ template <class real>
 void recip_t(Ctx& c, bool use_m, bool use_p, bool want_v) {
   const int n = c.n, nb = ceil_div(n, 256);
   const double f = kCoul;
+  pme_bin(c);  // B-spline bases of the current coordinates (valid, and waited for)
   // permanent multipoles: the direct field of the same multipoles left its
   // convolved grid; spread and transform again only for the virial sums
   const void* mgrid = c.qgrid.p;

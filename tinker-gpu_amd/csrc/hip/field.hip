@@ -445,20 +445,34 @@ void precond_t(Ctx& c, const void* rd, const void* rp, void* zd, void* zp, void*
 
 }  // namespace
 
-void dfield_all(Ctx& c) {
+void dfield_all(Ctx& c, const void* ua, void* fa) {
+  const size_t v3 = (c.prec == 32 ? 4 : 8) * 3 * (size_t)c.n;
+  const char* ua_d = (const char*)ua;
+  char* fa_d = (char*)fa;
   if (!c.ewald) {
     if (c.prec == 32) dfield_real_t<float>(c, c.stream);
     else dfield_real_t<double>(c, c.stream);
+    if (ua) ufield_all(c, (void*)ua_d, (void*)(ua_d + v3), fa_d, fa_d + v3);
     return;
   }
   // real space on the third stream beside the PME spreading and FFT; the
-  // PME gather (+=) waits for it
+  // PME gather (+=) waits for it.  The mutual field of ua (the ASPC guess
+  // of the coming solve) takes the same two routes behind the direct field:
+  // real space after it on the third stream, its PME round trip on the main
+  // stream while the direct field's real space still runs
   T9_HIP(hipEventRecord(c.ev_ufork, c.stream));
   T9_HIP(hipStreamWaitEvent(c.side2, c.ev_ufork, 0));
   if (c.prec == 32) dfield_real_t<float>(c, c.side2);
   else dfield_real_t<double>(c, c.side2);
   T9_HIP(hipEventRecord(c.ev_ujoin, c.side2));
-  pme_mpole_field(c, c.field.p, c.fieldp.p, c.ev_ujoin);
+  if (!ua) {
+    pme_mpole_field(c, c.field.p, c.fieldp.p, c.ev_ujoin);
+    return;
+  }
+  if (c.prec == 32) ufield_real_t<float>(c, ua_d, ua_d + v3, fa_d, fa_d + v3, c.side2);
+  else ufield_real_t<double>(c, ua_d, ua_d + v3, fa_d, fa_d + v3, c.side2);
+  T9_HIP(hipEventRecord(c.ev_ujoin2, c.side2));
+  pme_mpole_dipole_fields(c, c.field.p, c.fieldp.p, c.ev_ujoin, ua_d, ua_d + v3, fa_d, fa_d + v3, c.ev_ujoin2);
 }
 
 void ufield_all(Ctx& c, void* ud, void* up, void* fd, void* fp) {

@@ -637,6 +637,7 @@ void guess_setup(Ctx& c) {
   c.gnorm.alloc(sizeof(float) * kNorm);
   c.ealt.alloc(rsize(c) * 6 * (size_t)c.n * 16);
   c.uaspc.alloc(rsize(c) * 6 * (size_t)c.n);
+  c.raspc.alloc(rsize(c) * 6 * (size_t)c.n);
 }
 
 bool guess_wants_features(const Ctx& c) { return c.cap_stride > 0 || (c.guess_on && c.guess_ready); }

@@ -16,12 +16,12 @@
 // predictor coefficients (k = 14); before that, and without the keyword, the
 // direct-field guess.  The history is kept in caller order so a re-sort of
 // the atoms at a list rebuild does not disturb it.  With the learned guess on
-// (guess.hip) the ASPC guess is corrected by the network; the corrected guess
-// is kept only when its initial residual is not worse than the ASPC guess's,
-// else the solve starts from ASPC.  The ASPC residual costs one extra
-// matrix-vector product: it is measured on every learned solve
-// (guess-check 1, the default) or on every N-th (guess-check N, the
-// reference value then being up to N - 1 solves old).
+// (guess.hip) the ASPC guess is corrected by the network, and every solve
+// starts from the better of the two by RMS residual: the ASPC guess's mutual
+// field is computed beside the direct field (it needs only the stored
+// solutions), both residuals are formed, and a device kernel keeps the
+// learned guess only when its residual is not larger (else the ASPC guess and
+// its residual are copied in) before the first CG iteration.
 #include <array>
 
 #include <algorithm>
@@ -34,7 +34,7 @@ namespace {
 constexpr int kB = 256;
 // device scalar slots
 // slots 0..3: r.z sums SUM[0] (d, p), SUM[1] (d, p) by iteration parity
-enum { S_EPS = 8, S_EPS0 = 9, S_DONE = kDoneSlot, S_NSLOT = 16 };
+enum { S_EPS = 8, S_EPS0 = 9, S_PICK = 10, S_EPSA = 11, S_DONE = kDoneSlot, S_NSLOT = 16 };
 constexpr int kMaxUalt = 16;
 
 struct Coef16 {
@@ -335,6 +335,39 @@ __global__ void k_set_scalar(double* sc, int slot, double v) {
   if (threadIdx.x == 0) sc[slot] = v;
 }
 
+// The starting point of a learned solve: the learned guess (ud, up; residual
+// rd, rp) unless its RMS residual exceeds the ASPC guess's (ua; residual ra,
+// each [2][n][3]: d, then p) or is not finite; then the ASPC guess and its
+// residual are copied in.  The partials hold r.r of rd, rp, ra_d, ra_p (k_dots);
+// every workgroup sums them in the same order, so all make the same choice.
+// Workgroup 0 records it (slot S_PICK: 2 learned, 1 ASPC) and the ASPC
+// residual (slot S_EPSA, Debye).
+template <class real>
+__global__ __launch_bounds__(kB) void k_guess_select(int n, int nblk, double debye, const double* __restrict__ part,
+                                                    const real* __restrict__ ua, const real* __restrict__ ra,
+                                                    real* __restrict__ ud, real* __restrict__ up,
+                                                    real* __restrict__ rd, real* __restrict__ rp,
+                                                    double* __restrict__ sc) {
+  double t[4];
+  reduce_partials<4>(part, nblk, t);
+  const double el = fmax(t[0], t[1]), ea = fmax(t[2], t[3]);
+  const bool use_aspc = !(el <= ea);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_PICK] = use_aspc ? 1.0 : 2.0;
+    sc[S_EPSA] = debye * sqrt(ea / n);
+  }
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!use_aspc || s >= n) return;
+  const size_t h = 3 * (size_t)n;
+  for (int d = 0; d < 3; ++d) {
+    const int k = 3 * s + d;
+    ud[k] = ua[k];
+    up[k] = ua[h + k];
+    rd[k] = ra[k];
+    rp[k] = ra[h + k];
+  }
+}
+
 template <class real>
 int induce_t(Ctx& c, int* iters, double* rms) {
   const int n = c.n;
@@ -360,20 +393,36 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   const bool guess = c.opt.pcg_guess;
   const int maxiter = 100;
   const int miniter = n < 3 ? n : 3;
-  int iter = 0, iter0 = 0;  // iter0: iterations spent before a restart from ASPC
+  int iter = 0;
   // slot S_DONE = 0: no iteration has converged yet
   auto reset_done = [&]() {
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, sc, (int)S_DONE, 0.0);
     T9_LAUNCH_CHECK();
   };
 
-  dfield_all(c);
   // ASPC once 16 solutions are stored (src/cu/amoeba/pcg.cu:30-33)
   const bool aspc = c.opt.predictor == "ASPC";
   const bool predict = aspc && c.nualt >= kMaxUalt;
   // learned-guess features need 16 stored fields as well
   const bool feats = predict && c.nealt >= kMaxUalt && guess_wants_features(c);
   const bool learned = feats && c.guess_on && c.guess_ready;
+  const auto B = aspc_coefficients();
+  Coef16 cf;
+  if (predict)
+    for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
+  const size_t vbytes = sizeof(real) * 3 * (size_t)n;
+  real* ua = c.uaspc.as<real>();
+  real* ra = learned ? c.raspc.as<real>() : nullptr;
+  if (learned) {
+    // the ASPC guess first: its mutual field (into ra) is computed beside
+    // the direct field
+    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ua,
+                       ua + 3 * (size_t)n, (real*)nullptr);
+    T9_LAUNCH_CHECK();
+    dfield_all(c, ua, ra);
+  } else {
+    dfield_all(c);
+  }
   hipLaunchKernelGGL(k_udir<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, c.udir.as<real>(),
                      c.udirp.as<real>(), ud, up, (guess || predict) ? 1 : 0);
   T9_LAUNCH_CHECK();
@@ -385,27 +434,17 @@ int induce_t(Ctx& c, int* iters, double* rms) {
                        c.udir.as<real>(), c.udirp.as<real>(), ud, up, rd, rp);
     T9_LAUNCH_CHECK();
   };
-  // RMS of r0 in Debye, read back now
   double* trace = c.pcg_trace.as<double>();
-  auto residual_rms = [&]() {
-    hipLaunchKernelGGL((k_dots<real, 2>), G, Bk, 0, st, n, rd, rd, rp, rp, (const real*)nullptr,
-                       (const real*)nullptr, (const real*)nullptr, (const real*)nullptr, part, Gate());
-    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk, 0, 0, n, debye, part, sc, trace);
-    T9_LAUNCH_CHECK();
-    T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
-    T9_HIP(hipStreamSynchronize(st));
-    return c.h_scalars[S_EPS0];
-  };
-  const size_t vbytes = sizeof(real) * 3 * (size_t)n;
-  real* ua = c.uaspc.as<real>();
   c.last_predictor = predict ? 1 : 0;
   if (predict) {
-    const auto B = aspc_coefficients();
-    Coef16 cf;
-    for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
-    hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up,
-                       feats ? ua : nullptr);
-    T9_LAUNCH_CHECK();
+    if (learned) {
+      T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
+      T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
+    } else {
+      hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up,
+                         feats ? ua : nullptr);
+      T9_LAUNCH_CHECK();
+    }
     if (feats) {
       double ce[kMaxUalt], cu[2];
       for (int k = 0; k < kMaxUalt; ++k) ce[k] = B[((c.nealt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
@@ -415,19 +454,18 @@ int induce_t(Ctx& c, int* iters, double* rms) {
     }
   }
   if (aspc) guess_save_fields(c);
-  bool learned_try = false;  // the learned guess is in use; judged at the first readback
   if (learned) {
-    // the ASPC residual of this solve (guess-check 1), or of every N-th
-    // learned solve; the learned guess is compared with it at the first
-    // iteration's readback
-    if (c.guess_steps % std::max(1, c.opt.guess_check) == 0) {
-      residual();
-      c.aspc_eps0_ref = residual_rms();
-    }
-    ++c.guess_steps;
+    // the ASPC residual alpha^-1 (udir - ua) + T ua, in place of T ua; the
+    // learned guess and its residual; the better of the two
+    hipLaunchKernelGGL(k_rsd_guess<real>, G, Bk, 0, st, n, c.polarity.as<real>(), c.polinv.as<real>(),
+                       c.udir.as<real>(), c.udirp.as<real>(), ua, ua + 3 * (size_t)n, ra, ra + 3 * (size_t)n);
+    T9_LAUNCH_CHECK();
     guess_apply(c, ud, up);
     residual();
-    learned_try = true;
+    hipLaunchKernelGGL((k_dots<real, 4>), G, Bk, 0, st, n, rd, rd, rp, rp, ra, ra, ra + 3 * (size_t)n,
+                       ra + 3 * (size_t)n, part, Gate());
+    hipLaunchKernelGGL(k_guess_select<real>, G, Bk, 0, st, n, nblk, debye, part, ua, ra, ud, up, rd, rp, sc);
+    T9_LAUNCH_CHECK();
     c.last_predictor = 2;
   } else if (predict) {
     residual();
@@ -444,7 +482,7 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   auto start_cg = [&]() {
     // z, the first search direction p = z, and r.z, r.r per 64 elements
     precond_apply(c, rd, rp, zd, zp, cd, cp, part64);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk64, 1, (iter0 + 1) & 1, n, debye, part64, sc, trace);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), Bk, 0, st, nblk64, 1, 1, n, debye, part64, sc, trace);
     T9_LAUNCH_CHECK();
   };
   reset_done();
@@ -464,41 +502,22 @@ int induce_t(Ctx& c, int* iters, double* rms) {
                        rd, rp, g);
     T9_LAUNCH_CHECK();
     precond_apply(c, rd, rp, zd, zp, nullptr, nullptr, part64);  // z, and r.z, r.r per 64 elements
-    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, nblk64, part64, sc, c.opt.polar_eps, iter0 + miniter,
+    hipLaunchKernelGGL(k_p3<real>, G, Bk, 0, st, n, nblk64, part64, sc, c.opt.polar_eps, miniter,
                        c.opt.polar_iter, debye, trace, zd, zp, cd, cp, (real)c.opt.pcg_peek, c.polarity.as<real>(),
                        rd, rp, ud, up, g);
     T9_LAUNCH_CHECK();
     c.gate = Gate();
   };
   // The host enqueues as many iterations as the previous solve needed, then
-  // reads the scalars back once; after that one iteration per readback.  A
-  // learned guess is judged at the first readback, from the residual norm of
-  // the initial guess (written before the first iteration): a rejected guess
-  // restarts from the ASPC guess, and the iterations already run on it are
-  // discarded (counted in the iteration total, not in the next solve's plan).
+  // reads the scalars back once; after that one iteration per readback.
   double eps = 100;
   int enq = 0;
   for (;;) {
-    const int upto = std::max(enq + 1, iter0 + c.pcg_planned);
+    const int upto = std::max(enq + 1, c.pcg_planned);
     while (enq < upto) enqueue(++enq);
     T9_HIP(hipMemcpyAsync(c.h_scalars, sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st));
     mark_readback(c);
     wait_marked(c);
-    if (learned_try) {
-      learned_try = false;
-      if (!(c.h_scalars[S_EPS0] <= c.aspc_eps0_ref)) {  // worse than ASPC, or not finite: restart
-        T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
-        T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
-        iter0 = enq;  // before start_cg: the next iteration's r.z slot depends on it
-        residual();
-        reset_done();
-        start_cg();
-        ++c.guess_fallback;
-        c.last_predictor = 1;
-        continue;
-      }
-      ++c.guess_used;
-    }
     const double done_at = c.h_scalars[S_DONE];
     if (done_at != 0) {
       iter = (int)done_at;
@@ -506,11 +525,15 @@ int induce_t(Ctx& c, int* iters, double* rms) {
       break;
     }
   }
-  c.pcg_planned = iter - iter0;
+  if (learned) {  // the choice of k_guess_select
+    c.last_predictor = (int)c.h_scalars[S_PICK];
+    if (c.last_predictor == 2) ++c.guess_used;
+    else ++c.guess_fallback;
+  }
+  c.pcg_planned = iter;
   c.pcg_trace_len = std::min(iter + 1, kTraceCap);
   c.uind_valid = true;
   c.last_guess_rms = c.h_scalars[S_EPS0];
-  if (c.last_predictor == 1 && !learned) c.aspc_eps0_ref = c.last_guess_rms;
   if (feats) guess_capture(c);
   if (c.opt.predictor == "ASPC") {  // store the solution (src/amoeba/induce.cpp:29-62)
     const size_t slot = 6 * (size_t)n;

@@ -49,9 +49,11 @@ using R4 = typename RealTraits<real>::real4;
 
 // ------------------------------------------------------------ binning
 
+// Fractional offsets and support bases of every sorted atom, and the count of
+// atoms per bin (bin edge B per axis, nb bins per axis) of the support base.
 template <class real>
 __global__ void k_pme_bin(int n, const int* __restrict__ perm, const double* __restrict__ xyz, double lx, double ly,
-                          double lz, int K1, int K2, int K3, int T1, int T2, int T3, int nt1, int nt2,
+                          double lz, int K1, int K2, int K3, int B1, int B2, int B3, int nb1, int nb2,
                           R4<real>* __restrict__ pw, int4* __restrict__ pbase, int* __restrict__ tile_count) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -71,7 +73,7 @@ __global__ void k_pme_bin(int n, const int* __restrict__ perm, const double* __r
     while (bb >= K[d]) bb -= K[d];
     b[d] = bb;
   }
-  const int tile = b[0] / T1 + nt1 * (b[1] / T2 + nt2 * (b[2] / T3));
+  const int tile = b[0] / B1 + nb1 * (b[1] / B2 + nb2 * (b[2] / B3));
   R4<real> ww;
   ww.x = (real)w[0];
   ww.y = (real)w[1];
@@ -90,23 +92,37 @@ __global__ void k_tile_scatter(int n, const int4* __restrict__ pbase, int* __res
   atoms[atomicAdd(&cursor[t], 1)] = s;
 }
 
-// Order each bin's atoms by sorted index (insertion sort per bin), which
-// fixes the summation order of the spreading gather.
+// Each bin's atoms in ascending sorted index (insertion sort, one thread per
+// bin; the scatter above fills a bin in arrival order).  This fixes the order
+// in which the row-owner spreading adds the atoms of a tile.
+__global__ void k_bin_sort(int nbins, const int* __restrict__ start, int* __restrict__ atoms) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const int lo = start[b], hi = start[b + 1];
+  for (int i = lo + 1; i < hi; ++i) {
+    const int v = atoms[i];
+    int j = i - 1;
+    while (j >= lo && atoms[j] > v) {
+      atoms[j + 1] = atoms[j];
+      --j;
+    }
+    atoms[j + 1] = v;
+  }
+}
 
-// Exclusive prefix sums of the per-tile atom counts into start[0..m]
-// (start[m] the total), each tile's scatter cursor set to its start, and the
-// counts cleared for the next binning.  One workgroup: each thread sums a
-// contiguous segment, the segment sums are scanned across the wavefront by
-// shuffles and across the four wavefronts through LDS (one barrier).
-__global__ __launch_bounds__(256) void k_scan_tiles(int* __restrict__ count, int* __restrict__ start,
-                                                    int* __restrict__ cursor, int m) {
-  __shared__ int wsum[4];
-  const int per = (m + 255) / 256;
-  const int b = min(m, (int)threadIdx.x * per), e = min(m, b + per);
-  int s = 0;
-  for (int t = b; t < e; ++t) s += count[t];
+// Exclusive prefix sums of the per-bin atom counts into start[0..m]
+// (start[m] the total), each bin's scatter cursor set to its start, and the
+// counts cleared for the next binning.  Two launches over chunks of 1024
+// bins, one 1024-thread workgroup per chunk: k_scan_sums writes each chunk's
+// total; k_scan_apply adds the totals of the chunks before its own (in chunk
+// order), scans its chunk (wavefront shuffles, then the 16 wavefront totals
+// through LDS) and writes the starts with coalesced stores.
+constexpr int kScanChunk = 1024;
+
+__device__ __forceinline__ int block_scan_1024(int c, int& total) {
+  __shared__ int wsum[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int incl = s;
+  int incl = c;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(incl, o, 64);
@@ -114,16 +130,38 @@ __global__ __launch_bounds__(256) void k_scan_tiles(int* __restrict__ count, int
   }
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  int run = incl - s;
-  for (int u = 0; u < w; ++u) run += wsum[u];
-  for (int t = b; t < e; ++t) {
-    const int c = count[t];
-    start[t] = run;
-    cursor[t] = run;
-    count[t] = 0;
-    run += c;
+  int before = 0;
+  total = 0;
+  for (int u = 0; u < 16; ++u) {
+    if (u < w) before += wsum[u];
+    total += wsum[u];
   }
-  if (threadIdx.x == 255) start[m] = run;
+  return before + incl - c;  // exclusive
+}
+
+__global__ __launch_bounds__(kScanChunk) void k_scan_sums(const int* __restrict__ count, int* __restrict__ csum,
+                                                          int m) {
+  const int t = blockIdx.x * kScanChunk + (int)threadIdx.x;
+  int total;
+  (void)block_scan_1024(t < m ? count[t] : 0, total);
+  if (threadIdx.x == 0) csum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanChunk) void k_scan_apply(int* __restrict__ count, int* __restrict__ start,
+                                                           int* __restrict__ cursor, const int* __restrict__ csum,
+                                                           int m) {
+  const int t = blockIdx.x * kScanChunk + (int)threadIdx.x;
+  int carry = 0;
+  for (int b = 0; b < (int)blockIdx.x; ++b) carry += csum[b];
+  const int c = t < m ? count[t] : 0;
+  int total;
+  const int ex = carry + block_scan_1024(c, total);
+  if (t < m) {
+    start[t] = ex;
+    cursor[t] = ex;
+    count[t] = 0;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[m] = carry + total;
 }
 
 // ------------------------------------------------------------ spreading
@@ -386,6 +424,219 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
   }
 }
 
+// Row-owner spreading (grid sizes divisible by 8, at least 16): the grid is
+// cut into 8^3 tiles and the support bases into 4^3 bins; every atom whose
+// support reaches a tile has its base in one of the 27 bins around the tile's
+// own 8 (bin offsets -1, 0, +1 per axis).  One 256-thread workgroup per tile
+// stages those atoms in LDS (B-spline weights, scaled sources, base offset
+// from the tile origin), in bin order and, within a bin, in ascending sorted
+// index.  Lane (y, z) of each wavefront owns the tile's grid row (y, z): it
+// keeps the row's 8 points in registers and adds the atoms whose support
+// covers the row, one after another; wave w takes atoms w, w + 4, ... and
+// the four waves' rows are added in wave order at the end.  Every grid point
+// is written once, by one lane, and its sum runs in an order fixed by the
+// atoms' positions alone: bit-reproducible, with no atomics and no fixed
+// point.  Work per atom: the row factors of its (y, z) support (one set per
+// lane) and two fused multiply-adds per point and part.
+constexpr int kRowTile = 8, kRowBin = 4, kRowChunk = 128;
+
+template <class real, int MODE>
+struct RowAtom {
+  static constexpr int LV = MODE == 0 ? 2 : 3;
+  static constexpr int NF = MODE == 0 ? 6 : (MODE == 1 ? 10 : 13);
+  real th[3][LV][5];  // [axis][derivative][support point]
+  real f[NF];
+  int off[4];         // support base relative to the tile origin, x y z (-4..7)
+};
+
+template <class real, int MODE>
+__global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __restrict__ bin_start,
+                                                     const int* __restrict__ bin_atoms,
+                                                     const R4<real>* __restrict__ pw, const int4* __restrict__ pbase,
+                                                     const real* __restrict__ src1, const real* __restrict__ src2,
+                                                     real a1, real a2, real a3, cplx_t<real>* __restrict__ grid,
+                                                     Gate gate) {
+  if (gate_closed(gate)) return;
+  using RA = RowAtom<real, MODE>;
+  constexpr int LV = RA::LV;
+  constexpr int NACC = MODE == 1 ? 1 : 2;
+  constexpr int CH = (sizeof(real) == 8 && MODE != 0) ? kRowChunk / 2 : kRowChunk;
+  constexpr size_t kStage = sizeof(RA) * CH, kPart = sizeof(real) * 3 * 64 * NACC * kRowTile;
+  __shared__ __align__(16) char smem[kStage > kPart ? kStage : kPart];
+  RA* atoms = reinterpret_cast<RA*>(smem);
+  __shared__ int bstart[27], bpref[28];
+  const int tt = blockIdx.x;
+  const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
+  const int lo[3] = {t1 * kRowTile, t2 * kRowTile, t3 * kRowTile};
+  const int nb[3] = {g.K[0] / kRowBin, g.K[1] / kRowBin, g.K[2] / kRowBin};
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid < 64) {  // the 27 bins in a fixed order, and their exclusive prefix
+    int cnt = 0, st = 0;
+    if (tid < 27) {
+      int b1 = 2 * t1 + tid % 3 - 1, b2 = 2 * t2 + (tid / 3) % 3 - 1, b3 = 2 * t3 + tid / 9 - 1;
+      b1 += b1 < 0 ? nb[0] : 0;
+      b2 += b2 < 0 ? nb[1] : 0;
+      b3 += b3 < 0 ? nb[2] : 0;
+      const int bin = b1 + nb[0] * (b2 + nb[1] * b3);
+      st = bin_start[bin];
+      cnt = bin_start[bin + 1] - st;
+      bstart[tid] = st;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (tid < 27) bpref[tid + 1] = incl;
+    if (tid == 0) bpref[0] = 0;
+  }
+  __syncthreads();
+  const int total = bpref[27];
+  const int ly = lane & 7, lz = lane >> 3;
+  real acc[NACC][kRowTile];
+#pragma unroll
+  for (int q = 0; q < NACC; ++q)
+#pragma unroll
+    for (int x = 0; x < kRowTile; ++x) acc[q][x] = 0;
+  for (int base = 0; base < total; base += CH) {
+    const int nchunk = min(CH, total - base);
+    // ---- stage: one thread per atom
+    if (tid < nchunk) {
+      const int idx = base + tid;
+      int c = 0;  // bin of idx: last c with bpref[c] <= idx
+#pragma unroll
+      for (int step = 16; step >= 1; step >>= 1)
+        if (c + step < 27 && bpref[c + step] <= idx) c += step;
+      const int s = bin_atoms[bstart[c] + idx - bpref[c]];
+      RA& A = atoms[tid];
+      const int4 b = pbase[s];
+      const int bb[3] = {b.x, b.y, b.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        int o = bb[d] - lo[d];
+        if (o >= kRowTile) o -= g.K[d];  // lower neighbour bin across the periodic boundary
+        A.off[d] = o;
+      }
+      const R4<real> w = pw[s];
+      real th[LV][5];
+      bspline5<LV>(w.x, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[0][lv][m] = th[lv][m];
+      bspline5<LV>(w.y, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[1][lv][m] = th[lv][m];
+      bspline5<LV>(w.z, th);
+#pragma unroll
+      for (int lv = 0; lv < LV; ++lv)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.th[2][lv][m] = th[lv][m];
+      if (MODE == 0) {
+        A.f[0] = src1[3 * s] * a1;
+        A.f[1] = src1[3 * s + 1] * a2;
+        A.f[2] = src1[3 * s + 2] * a3;
+        A.f[3] = src2[3 * s] * a1;
+        A.f[4] = src2[3 * s + 1] * a2;
+        A.f[5] = src2[3 * s + 2] * a3;
+      } else {
+        const real* p = src1 + 10 * (size_t)s;
+        A.f[0] = p[0];
+        A.f[1] = p[1] * a1;
+        A.f[2] = p[2] * a2;
+        A.f[3] = p[3] * a3;
+        A.f[4] = p[4] * a1 * a1;
+        A.f[5] = p[5] * a2 * a2;
+        A.f[6] = p[6] * a3 * a3;
+        A.f[7] = 2 * p[7] * a1 * a2;
+        A.f[8] = 2 * p[8] * a1 * a3;
+        A.f[9] = 2 * p[9] * a2 * a3;
+        if (MODE == 2) {
+          A.f[MODE == 2 ? 10 : 0] = src2[3 * s] * a1;
+          A.f[MODE == 2 ? 11 : 0] = src2[3 * s + 1] * a2;
+          A.f[MODE == 2 ? 12 : 0] = src2[3 * s + 2] * a3;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- accumulate: each wave takes every 4th staged atom
+    for (int a = wv; a < nchunk; a += 4) {
+      const RA& A = atoms[a];
+      const int ox = __builtin_amdgcn_readfirstlane(A.off[0]);
+      const int dy = ly - A.off[1], dz = lz - A.off[2];
+      const bool act = (unsigned)dy <= 4u && (unsigned)dz <= 4u;
+      const int my = act ? dy : 0, mz = act ? dz : 0;
+      const real u0 = A.th[1][0][my], u1 = A.th[1][1][my];
+      const real v0 = A.th[2][0][mz], v1 = A.th[2][1][mz];
+      const real* f = A.f;
+      if (MODE == 0) {
+        real Ad = f[0] * u0 * v0, Bd = f[1] * u1 * v0 + f[2] * u0 * v1;
+        real Ap = f[3] * u0 * v0, Bp = f[4] * u1 * v0 + f[5] * u0 * v1;
+        if (!act) Ad = Bd = Ap = Bp = 0;
+#pragma unroll
+        for (int x = 0; x < kRowTile; ++x) {
+          const int m1 = x - ox;
+          if ((unsigned)m1 <= 4u) {
+            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1];
+            acc[0][x] += Ad * t1v + Bd * t0;
+            acc[NACC - 1][x] += Ap * t1v + Bp * t0;
+          }
+        }
+      } else {
+        const real u2 = A.th[1][LV > 2 ? 2 : 0][my], v2 = A.th[2][LV > 2 ? 2 : 0][mz];
+        real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
+                  f[9] * u1 * v1;
+        real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+        real R2 = f[4] * u0 * v0;
+        real Ap = 0, Bp = 0;
+        if (MODE == 2) {
+          Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
+          Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
+        }
+        if (!act) R0 = R1 = R2 = Ap = Bp = 0;
+#pragma unroll
+        for (int x = 0; x < kRowTile; ++x) {
+          const int m1 = x - ox;
+          if ((unsigned)m1 <= 4u) {
+            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1], t2v = A.th[0][LV > 2 ? 2 : 0][m1];
+            acc[0][x] += R0 * t0 + R1 * t1v + R2 * t2v;
+            if (MODE == 2) acc[NACC - 1][x] += Ap * t1v + Bp * t0;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // ---- the four waves' rows, added in wave order by wave 0, written once
+  real* part = reinterpret_cast<real*>(smem);  // [3][64][NACC * 8] (the staging area is free)
+  if (wv > 0) {
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+      for (int x = 0; x < kRowTile; ++x) part[((wv - 1) * 64 + lane) * (NACC * kRowTile) + q * kRowTile + x] = acc[q][x];
+  }
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+      for (int q = 0; q < NACC; ++q)
+#pragma unroll
+        for (int x = 0; x < kRowTile; ++x) acc[q][x] += part[(w * 64 + lane) * (NACC * kRowTile) + q * kRowTile + x];
+    const long long row = lo[0] + (long long)g.K[0] * ((lo[1] + ly) + (long long)g.K[1] * (lo[2] + lz));
+#pragma unroll
+    for (int x = 0; x < kRowTile; ++x) {
+      cplx_t<real> v;
+      v.x = acc[0][x];
+      v.y = NACC == 2 ? acc[NACC - 1][x] : (real)0;
+      grid[row + x] = v;
+    }
+  }
+}
+
 // ------------------------------------------------------------ gathers
 
 // Field of the dipole grid at each atom: fd -= a * grad phi (real part),
@@ -437,103 +688,6 @@ __global__ __launch_bounds__(256) void k_gather_dip(int n, TileGeom g, const R4<
   fp[3 * s] += term * up[3 * s] - a1 * p1;
   fp[3 * s + 1] += term * up[3 * s + 1] - a2 * p2;
   fp[3 * s + 2] += term * up[3 * s + 2] - a3 * p3;
-}
-
-// Tile-staged gathers: one 64-thread workgroup per spreading tile takes the
-// atoms whose support base lies in that tile (their support spans the tile
-// and the next 4 points per axis), stages that (T+4)^3 region of the grid in
-// LDS with coalesced loads (x fastest, wrapped by K), and each lane gathers
-// one atom's 125 points from LDS.  Each grid point is read from memory by at
-// most 8 workgroups (instead of by every atom whose support covers it), and
-// the per-atom sums run in the same order with the same arithmetic as
-// k_gather_dip / k_gather_mpole_field, so the results are the same bits.
-// MODE 0: dipole grid (d real, p imaginary) -> fd, fp += term u - a grad phi;
-// MODE 1: multipole grid (real part) -> fd and fp += term p - a grad phi.
-constexpr int kGatherThreads = 64;
-
-template <class real, int MODE>
-__global__ __launch_bounds__(kGatherThreads) void k_gather_tiled(TileGeom g, const int* __restrict__ tile_start,
-                                                                 const int* __restrict__ tile_atoms,
-                                                                 const R4<real>* __restrict__ pw,
-                                                                 const int4* __restrict__ pbase,
-                                                                 const cplx_t<real>* __restrict__ grid, real a1,
-                                                                 real a2, real a3, real term,
-                                                                 const real* __restrict__ ud,
-                                                                 const real* __restrict__ up, real* __restrict__ fd,
-                                                                 real* __restrict__ fp, Gate gate) {
-  if (gate_closed(gate)) return;
-  extern __shared__ __align__(16) char smem[];
-  cplx_t<real>* reg = reinterpret_cast<cplx_t<real>*>(smem);
-  const int tt = blockIdx.x;
-  const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
-  const int a0 = tile_start[tt], na = tile_start[tt + 1] - a0;
-  if (na == 0) return;
-  const int lo[3] = {t1 * g.T[0], t2 * g.T[1], t3 * g.T[2]};
-  const int E1 = g.T[0] + 4, E2 = g.T[1] + 4, E3 = g.T[2] + 4;
-  const int npt = E1 * E2 * E3;
-  for (int q = threadIdx.x; q < npt; q += kGatherThreads) {
-    const int qx = q % E1, r = q / E1;
-    const int qy = r % E2, qz = r / E2;
-    int x = lo[0] + qx, y = lo[1] + qy, z = lo[2] + qz;
-    while (x >= g.K[0]) x -= g.K[0];
-    while (y >= g.K[1]) y -= g.K[1];
-    while (z >= g.K[2]) z -= g.K[2];
-    reg[q] = grid[x + (long long)g.K[0] * (y + (long long)g.K[1] * z)];
-  }
-  __syncthreads();
-  for (int a = threadIdx.x; a < na; a += kGatherThreads) {
-    const int s = tile_atoms[a0 + a];
-    const R4<real> w = pw[s];
-    const int4 b = pbase[s];
-    real th1[2][5], th2[2][5], th3[2][5];
-    bspline5<2>(w.x, th1);
-    bspline5<2>(w.y, th2);
-    bspline5<2>(w.z, th3);
-    const int ox = b.x - lo[0], oy = b.y - lo[1], oz = b.z - lo[2];
-    real d1 = 0, d2 = 0, d3 = 0, p1 = 0, p2 = 0, p3 = 0;
-    for (int m3 = 0; m3 < 5; ++m3) {
-      const real v0 = th3[0][m3], v1 = th3[1][m3];
-      for (int m2 = 0; m2 < 5; ++m2) {
-        const real u0 = th2[0][m2], u1 = th2[1][m2];
-        const cplx_t<real>* row = reg + ox + E1 * ((oy + m2) + E2 * (oz + m3));
-#pragma unroll
-        for (int m1 = 0; m1 < 5; ++m1) {
-          const cplx_t<real> q = row[m1];
-          const real t0 = th1[0][m1], tv = th1[1][m1];
-          if (MODE == 0) {
-            const real w1 = tv * u0 * v0, w2 = t0 * u1 * v0, w3 = t0 * u0 * v1;
-            d1 += q.x * w1;
-            d2 += q.x * w2;
-            d3 += q.x * w3;
-            p1 += q.y * w1;
-            p2 += q.y * w2;
-            p3 += q.y * w3;
-          } else {
-            d1 += q.x * tv * u0 * v0;
-            d2 += q.x * t0 * u1 * v0;
-            d3 += q.x * t0 * u0 * v1;
-          }
-        }
-      }
-    }
-    if (MODE == 0) {
-      fd[3 * s] += term * ud[3 * s] - a1 * d1;
-      fd[3 * s + 1] += term * ud[3 * s + 1] - a2 * d2;
-      fd[3 * s + 2] += term * ud[3 * s + 2] - a3 * d3;
-      fp[3 * s] += term * up[3 * s] - a1 * p1;
-      fp[3 * s + 1] += term * up[3 * s + 1] - a2 * p2;
-      fp[3 * s + 2] += term * up[3 * s + 2] - a3 * p3;
-    } else {
-      const real* p = ud + 10 * (size_t)s;  // the global multipoles
-      const real f1 = term * p[1] - a1 * d1, f2 = term * p[2] - a2 * d2, f3 = term * p[3] - a3 * d3;
-      fd[3 * s] += f1;
-      fd[3 * s + 1] += f2;
-      fd[3 * s + 2] += f3;
-      fp[3 * s] += f1;
-      fp[3 * s + 1] += f2;
-      fp[3 * s + 2] += f3;
-    }
-  }
 }
 
 // Field of the permanent-multipole grid (real part) plus the self term,
@@ -595,47 +749,59 @@ TileGeom geom(const Ctx& c) {
   return g;
 }
 
-// LDS bytes of the tile-staged gather: the (T+4)^3 grid region
 template <class real>
-size_t gather_lds(const Ctx& c) {
-  return sizeof(cplx_t<real>) * (size_t)(c.tsz[0] + 4) * (c.tsz[1] + 4) * (c.tsz[2] + 4);
-}
-
-// the tile-staged gathers (T9HIP_GATHER=tiles; measured slower than one
-// thread per atom on the 96,000-atom step) need the bins and a region that
-// fits the LDS budget of one workgroup
-template <class real>
-bool gather_tiled_ok(Ctx& c) {
-  if (c.gather_tiled < 0) {
-    const char* e = std::getenv("T9HIP_GATHER");
-    c.gather_tiled = (e && !std::strcmp(e, "tiles")) ? 1 : 0;
-  }
-  return c.gather_tiled == 1 && c.bins_valid && gather_lds<real>(c) <= 60 * 1024;
-}
-
-template <class real>
-void pme_bin_t(Ctx& c) {
+void pme_bin_t(Ctx& c, hipStream_t st) {
   const int n = c.n;
-  const int ntot = c.ntile[0] * c.ntile[1] * c.ntile[2];
+  const int ntot = c.nbin[0] * c.nbin[1] * c.nbin[2];
+  const int nchunk = ceil_div(ntot, kScanChunk);
   // tile_count is zero here: cleared at allocation and by every scan
-  hipLaunchKernelGGL(k_pme_bin<real>, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.perm.as<int>(),
-                     c.xyz.as<double>(), c.box[0], c.box[1], c.box[2], c.grid[0], c.grid[1], c.grid[2], c.tsz[0],
-                     c.tsz[1], c.tsz[2], c.ntile[0], c.ntile[1], c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
+  hipLaunchKernelGGL(k_pme_bin<real>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, c.perm.as<int>(),
+                     c.xyz.as<double>(), c.box[0], c.box[1], c.box[2], c.grid[0], c.grid[1], c.grid[2], c.bsz[0],
+                     c.bsz[1], c.bsz[2], c.nbin[0], c.nbin[1], c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(),
                      c.tile_count.as<int>());
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, c.stream, c.tile_count.as<int>(), c.tile_start.as<int>(),
-                     c.tile_cursor.as<int>(), ntot);
+  hipLaunchKernelGGL(k_scan_sums, dim3(nchunk), dim3(kScanChunk), 0, st, c.tile_count.as<int>(),
+                     c.tile_csum.as<int>(), ntot);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nchunk), dim3(kScanChunk), 0, st, c.tile_count.as<int>(),
+                     c.tile_start.as<int>(), c.tile_cursor.as<int>(), c.tile_csum.as<int>(), ntot);
   T9_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_tile_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, c.stream, n, c.pme_base.as<int4>(),
+  hipLaunchKernelGGL(k_tile_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, c.pme_base.as<int4>(),
                      c.tile_cursor.as<int>(), c.tile_atoms.as<int>());
   T9_LAUNCH_CHECK();
-  // bins stay in arrival order: the fixed-point tile sums do not depend on it
+  // the fixed-point tile sums do not depend on the order within a bin; the
+  // row-owner spreading adds atoms in bin order
+  if (c.spread_rows) {
+    hipLaunchKernelGGL(k_bin_sort, dim3(ceil_div(ntot, 256)), dim3(256), 0, st, ntot, c.tile_start.as<int>(),
+                       c.tile_atoms.as<int>());
+    T9_LAUNCH_CHECK();
+  }
   c.bins_valid = true;
+}
+
+// bins of the current coordinates, enqueued on the main stream if no
+// binning is valid, and the main stream's wait for an asynchronous one
+template <class real>
+void ensure_bins(Ctx& c) {
+  if (!c.bins_valid) pme_bin_t<real>(c, c.stream);
+  if (c.bins_pending) {
+    T9_HIP(hipStreamWaitEvent(c.stream, c.ev_bins, 0));
+    c.bins_pending = false;
+  }
 }
 
 template <class real>
 void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   const int ntot = c.ntile[0] * c.ntile[1] * c.ntile[2];
+  const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
+  if (c.spread_rows) {
+    auto kern = mode == 0 ? k_spread_rows<real, 0> : (mode == 1 ? k_spread_rows<real, 1> : k_spread_rows<real, 2>);
+    KScope ks_(c, KT_SPREAD);
+    hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), 0, c.stream, geom(c), c.tile_start.as<int>(),
+                       c.tile_atoms.as<int>(), c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), (const real*)s1,
+                       (const real*)s2, a1, a2, a3, c.qgrid.as<cplx_t<real>>(), c.gate);
+    T9_LAUNCH_CHECK();
+    return;
+  }
   const int rows = c.tsz[1] * c.tsz[2];
   if (rows > 256 || c.tsz[0] > kMaxTile) throw ArgError("spreading tile too large");
   const size_t per = mode == 0 ? sizeof(SpreadAtom<real, 0>)
@@ -646,7 +812,6 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
   if (accb + 32 * (per + sizeof(int)) > budget) throw ArgError("spreading tile too large for LDS");
   const int chunk = (int)std::min<size_t>(kSpreadChunk, (budget - accb - 2 * sizeof(int)) / (per + sizeof(int)));
   const size_t sh = accb + sizeof(int) * ((chunk + 2) & ~1) + per * chunk;
-  const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   auto kern = mode == 0 ? k_spread<real, 0> : (mode == 1 ? k_spread<real, 1> : k_spread<real, 2>);
   KScope ks_(c, KT_SPREAD);
   hipLaunchKernelGGL(kern, dim3(ntot), dim3(256), sh, c.stream, geom(c), c.tile_start.as<int>(),
@@ -658,29 +823,55 @@ void spread_t(Ctx& c, int mode, const void* s1, const void* s2) {
 
 template <class real>
 void dipole_field_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather) {
-  if (!c.bins_valid) pme_bin_t<real>(c);
+  ensure_bins<real>(c);
   spread_t<real>(c, 0, ud, up);
   pme_fft_roundtrip(c);
   if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   KScope ks_(c, KT_GATHER);
-  if (gather_tiled_ok<real>(c)) {
-    hipLaunchKernelGGL((k_gather_tiled<real, 0>), dim3(c.ntile[0] * c.ntile[1] * c.ntile[2]), dim3(kGatherThreads),
-                       gather_lds<real>(c), c.stream, geom(c), c.tile_start.as<int>(), c.tile_atoms.as<int>(),
-                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
-                       (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
-  } else {
-    hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
-                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
-                       (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
+  hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
+                     (const real*)ud, (const real*)up, (real*)fd, (real*)fp, c.gate);
+  T9_LAUNCH_CHECK();
+}
+
+// direct-field round trip of the multipoles and a dipole round trip on the
+// other grid buffer, both transformed before either gather waits for the
+// real-space kernels of the third stream (line passes: a plane workgroup
+// needs a whole CU's LDS while the other streams' kernels hold every CU)
+template <class real>
+void mpole_dipole_fields_t(Ctx& c, void* fd, void* fp, hipEvent_t e1, const void* ud, const void* up, void* gd,
+                           void* gp, hipEvent_t e2) {
+  ensure_bins<real>(c);
+  spread_t<real>(c, 1, c.rpole.p, nullptr);
+  pme_fft_roundtrip(c, true);
+  c.qgrid_m.alloc(c.qgrid.bytes);
+  c.qgrid.swap(c.qgrid_m);
+  c.qgrid_m_stamp = c.rpole_stamp;
+  spread_t<real>(c, 0, ud, up);
+  pme_fft_roundtrip(c, true);
+  const real a1 = (real)(c.grid[0] / c.box[0]), a2 = (real)(c.grid[1] / c.box[1]), a3 = (real)(c.grid[2] / c.box[2]);
+  const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
+  T9_HIP(hipStreamWaitEvent(c.stream, e1, 0));
+  {
+    KScope ks_(c, KT_GATHER);
+    hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
+                       c.rpole.as<real>(), (real*)fd, (real*)fp);
+    T9_LAUNCH_CHECK();
   }
+  T9_HIP(hipStreamWaitEvent(c.stream, e2, 0));
+  KScope ks_(c, KT_GATHER);
+  hipLaunchKernelGGL(k_gather_dip<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid.as<cplx_t<real>>(), a1, a2, a3, term,
+                     (const real*)ud, (const real*)up, (real*)gd, (real*)gp, Gate{});
   T9_LAUNCH_CHECK();
 }
 
 template <class real>
 void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
-  if (!c.bins_valid) pme_bin_t<real>(c);
+  ensure_bins<real>(c);
   spread_t<real>(c, 1, c.rpole.p, nullptr);
   // the direct field's round trip runs beside the real-space direct field and
   // the vdW pair kernel of the other streams, which hold every CU: a plane
@@ -698,32 +889,38 @@ void mpole_field_t(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
   const real term = (real)(4.0 / 3.0 * c.aewald * c.aewald * c.aewald / 1.77245385090551602730);
   if (before_gather) T9_HIP(hipStreamWaitEvent(c.stream, before_gather, 0));
   KScope ks_(c, KT_GATHER);
-  if (gather_tiled_ok<real>(c)) {
-    hipLaunchKernelGGL((k_gather_tiled<real, 1>), dim3(c.ntile[0] * c.ntile[1] * c.ntile[2]), dim3(kGatherThreads),
-                       gather_lds<real>(c), c.stream, geom(c), c.tile_start.as<int>(), c.tile_atoms.as<int>(),
-                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
-                       c.rpole.as<real>(), (const real*)nullptr, (real*)fd, (real*)fp, Gate{});
-  } else {
-    hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
-                       c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
-                       c.rpole.as<real>(), (real*)fd, (real*)fp);
-  }
+  hipLaunchKernelGGL(k_gather_mpole_field<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, c.n, geom(c),
+                     c.pme_w.as<R4<real>>(), c.pme_base.as<int4>(), c.qgrid_m.as<cplx_t<real>>(), a1, a2, a3, term,
+                     c.rpole.as<real>(), (real*)fd, (real*)fp);
   T9_LAUNCH_CHECK();
 }
 
 }  // namespace
 
 void pme_bin(Ctx& c) {
-  if (c.prec == 32) pme_bin_t<float>(c);
-  else pme_bin_t<double>(c);
+  if (c.prec == 32) ensure_bins<float>(c);
+  else ensure_bins<double>(c);
+}
+
+// Binning of the current coordinates on side3, beside the list compaction
+// and the multipole rotation of the main stream; the first PME kernel of the
+// main stream waits for it (ensure_bins).
+void pme_bin_async(Ctx& c) {
+  if (!c.ewald || c.bins_valid) return;
+  T9_HIP(hipEventRecord(c.ev_bfork, c.stream));
+  T9_HIP(hipStreamWaitEvent(c.side3, c.ev_bfork, 0));
+  if (c.prec == 32) pme_bin_t<float>(c, c.side3);
+  else pme_bin_t<double>(c, c.side3);
+  T9_HIP(hipEventRecord(c.ev_bins, c.side3));
+  c.bins_pending = true;
 }
 
 void pme_spread(Ctx& c, int mode, const void* s1, const void* s2) {
   if (c.prec == 32) {
-    if (!c.bins_valid) pme_bin_t<float>(c);
+    ensure_bins<float>(c);
     spread_t<float>(c, mode, s1, s2);
   } else {
-    if (!c.bins_valid) pme_bin_t<double>(c);
+    ensure_bins<double>(c);
     spread_t<double>(c, mode, s1, s2);
   }
 }
@@ -731,6 +928,12 @@ void pme_spread(Ctx& c, int mode, const void* s1, const void* s2) {
 void pme_dipole_field(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipEvent_t before_gather) {
   if (c.prec == 32) dipole_field_t<float>(c, ud, up, fd, fp, before_gather);
   else dipole_field_t<double>(c, ud, up, fd, fp, before_gather);
+}
+
+void pme_mpole_dipole_fields(Ctx& c, void* fd, void* fp, hipEvent_t e1, const void* ud, const void* up, void* gd,
+                             void* gp, hipEvent_t e2) {
+  if (c.prec == 32) mpole_dipole_fields_t<float>(c, fd, fp, e1, ud, up, gd, gp, e2);
+  else mpole_dipole_fields_t<double>(c, fd, fp, e1, ud, up, gd, gp, e2);
 }
 
 void pme_mpole_field(Ctx& c, void* fd, void* fp, hipEvent_t before_gather) {
@@ -750,7 +953,7 @@ void pme_grid_uind(Ctx& c, const void* ud, const void* up) { pme_spread(c, 0, ud
 namespace {
 template <class real>
 void fphi_t(Ctx& c, bool mpole, void* fd, void* fp) {
-  if (!c.bins_valid) pme_bin_t<real>(c);
+  ensure_bins<real>(c);
   const size_t vb = sizeof(real) * 3 * (size_t)c.n;
   T9_HIP(hipMemsetAsync(fd, 0, vb, c.stream));
   T9_HIP(hipMemsetAsync(fp, 0, vb, c.stream));
@@ -822,6 +1025,25 @@ std::vector<double> pme_bsmod(int K, int order) {
     mod[m] *= zeta * zeta;
   }
   return mod;
+}
+
+// Spreading geometry: row-owner spreading (8^3 tiles over 4^3 bins) when every
+// grid size is a multiple of 8 and at least 16 (T9HIP_SPREAD=fixed selects the
+// LDS fixed-point kernel), else bins equal to the tiles of the fixed-point
+// kernel.
+void pme_setup_bins(Ctx& c) {
+  const char* e = std::getenv("T9HIP_SPREAD");
+  bool rows = !(e && !std::strcmp(e, "fixed"));
+  for (int d = 0; d < 3; ++d) rows = rows && c.grid[d] % kRowTile == 0 && c.grid[d] >= 2 * kRowTile;
+  c.spread_rows = rows;
+  for (int d = 0; d < 3; ++d) {
+    if (rows) {
+      c.tsz[d] = kRowTile;
+      c.ntile[d] = c.grid[d] / kRowTile;
+    }
+    c.bsz[d] = rows ? kRowBin : c.tsz[d];
+    c.nbin[d] = c.grid[d] / c.bsz[d];
+  }
 }
 
 // Tile edge dividing K, closest to 8, between 4 and 12 (K <= 12: one tile);

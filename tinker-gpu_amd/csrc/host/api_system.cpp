@@ -150,7 +150,6 @@ int t9_system_get_scalar(const t9_system* sys, const char* name, double* out) {
     else if (k == "nscale") v = (double)s.scales.size();
     else if (k == "polar_eps") v = o.polar_eps;
     else if (k == "polar_iter") v = o.polar_iter;
-    else if (k == "guess_check") v = o.guess_check;
     else if (k == "pcg_prec") v = o.pcg_prec;
     else if (k == "pcg_guess") v = o.pcg_guess;
     else if (k == "pcg_peek") v = o.pcg_peek;
@@ -190,7 +189,6 @@ int t9_system_set_scalar(t9_system* sys, const char* name, double value) {
     else if (k == "ewald_cut") o.ewald_cut = value;
     else if (k == "polar_eps") o.polar_eps = value;
     else if (k == "polar_iter") o.polar_iter = (int)value;
-    else if (k == "guess_check") o.guess_check = std::max(1, (int)value);
     else if (k == "pcg_prec") o.pcg_prec = value != 0;
     else if (k == "pcg_guess") o.pcg_guess = value != 0;
     else if (k == "pcg_peek") o.pcg_peek = value;

@@ -344,8 +344,6 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
       o.pcg_peek = to_d(tok[1]);
     } else if (k == "usolve-cutoff") {
       o.usolve_cut = to_d(tok[1]);
-    } else if (k == "guess-check") {
-      o.guess_check = std::max(1, std::stoi(tok[1]));
     } else if (k == "usolve-diag") {
       o.udiag = to_d(tok[1]);
     } else if (k == "list-buffer") {

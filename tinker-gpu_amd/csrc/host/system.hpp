@@ -112,7 +112,6 @@ struct Options {
   double vdw_cut = 9.0;
   double vdw_taper = 0;        // 0: 0.9 * vdw_cut (the reference's vdwtaper default)
   bool mutual = true;
-  int guess_check = 1;         // learned guess: ASPC residual every N-th learned solve
 };
 
 // One pair with at least one scale factor different from 1.
