@@ -193,7 +193,7 @@ struct Ctx {
   DBuf tile_count, tile_start, tile_cursor, tile_atoms, tile_csum;  // atoms per bin (support base), bin order
   bool bins_valid = false;
   bool bins_pending = false;     // binning enqueued on side3; the main stream has not waited for it yet
-  DBuf conv_table;               // real [K1*K2*K3] influence function (z-pass order)
+  DBuf conv_table;               // double [K1 + K2 + K3] separable factors of the influence function
   DBuf qgrid_m;                  // convolved permanent-multipole grid of the last direct field
   long long qgrid_m_stamp = -1;  // rpole_stamp it belongs to
   long long rpole_stamp = 0;     // incremented by every rotation of the multipoles

@@ -28,7 +28,7 @@ namespace {
 
 struct ConvArgs {
   int enabled;
-  const void* table;  // precomputed influence function (real [K1][K2][K3] in z-pass order), or null
+  const double* etab;  // separable factors of the influence function ([K1 + K2 + K3]), or null
   double pterm;    // (pi / alpha)^2
   double volterm;  // pi * V
   double il[3];    // 1 / box edge
@@ -37,6 +37,22 @@ struct ConvArgs {
   double vscale;   // weight of the convolution virial sums
   double* vrows;   // per-block partial rows, or null
 };
+
+// Influence function of reciprocal point (k1, k2, k3) (the reference's
+// pmeConv, src/acc/pme.cpp:220-303) from its separable factors:
+// exp(-pterm h^2) / (pi V bsmod1 bsmod2 bsmod3 h^2) = e1[k1] e2[k2] e3[k3] / h^2,
+// e_d[k] = exp(-pterm h_d^2) / bsmod_d[k] (and 1 / (pi V) in e1); k = 0 and
+// exponents below -50 give 0.  FP64.
+__device__ __forceinline__ double conv_factor(const ConvArgs& cv, int K1, int K2, int K3, int k1, int k2, int k3) {
+  if (k1 == 0 && k2 == 0 && k3 == 0) return 0.0;
+  const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
+  const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
+  const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
+  const double h1 = r1 * cv.il[0], h2 = r2 * cv.il[1], h3 = r3 * cv.il[2];
+  const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
+  if (!(-cv.pterm * hsq > -50)) return 0.0;
+  return cv.etab[k1] * cv.etab[K1 + k2] * cv.etab[K1 + K2 + k3] / hsq;
+}
 
 struct DevBarrier {
   __device__ void operator()() const { __syncthreads(); }
@@ -104,12 +120,11 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
       lb = q - e * nl;
     }
   };
-  const bool tabconv = conv.enabled && conv.table && !conv.vrows;
-  const real* tb = (const real*)conv.table + ((long long)other * K1 + x0) * N;
+  // z pass with the influence function computed from its separable factors
+  const bool tabconv = conv.enabled && conv.etab && !conv.vrows;
   const Cplx<real>* twk = tw;
   constexpr bool kPipe = NC > 0 && twiddles_in_lds<real>();
   constexpr int PER = kPipe ? (B * NC + 255) / 256 : 1;  // elements per thread (256 threads)
-  real ftab[PER];
   if constexpr (kPipe) {
     // all of a thread's loads are issued before the first LDS write; the
     // twiddles (and the z pass's influence-table values) come in the same phase
@@ -126,12 +141,6 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
     Cplx<real>* twl = buf1 + B * LP;
     if (threadIdx.x < NC) twl[threadIdx.x] = tw[threadIdx.x];
     twk = twl;
-    if (tabconv)
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int q = threadIdx.x + 256 * k;
-        ftab[k] = q < tot ? tb[q] : (real)0;
-      }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int q = threadIdx.x + 256 * k;
@@ -153,26 +162,15 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
   if constexpr (NC > 0) res = fft_stages_ct<real, NC>(buf0, buf1, nl, twk, sign, threadIdx.x, blockDim.x);
   else res = fft_stages<real>(buf0, buf1, nl, LP, plan, tw, sign, threadIdx.x, blockDim.x, DevBarrier());
   if (tabconv) {
-    // convolution with the tabulated influence function
-    if constexpr (kPipe) {
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int q = threadIdx.x + 256 * k;
-        if (q < tot) {
-          Cplx<real> v = res[(q / N) * LP + (q % N)];
-          v.x *= ftab[k];
-          v.y *= ftab[k];
-          res[(q / N) * LP + (q % N)] = v;
-        }
-      }
-    } else {
-      for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-        const real f = tb[q];
-        Cplx<real> v = res[(q / N) * LP + (q % N)];
-        v.x *= f;
-        v.y *= f;
-        res[(q / N) * LP + (q % N)] = v;
-      }
+    // convolution with the influence function: lines are (x0 + lb, y =
+    // other), element index k3
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+      const int lb = q / N, k3 = q - lb * N;
+      const real f = (real)conv_factor(conv, K1, K2, K3, x0 + lb, other, k3);
+      Cplx<real> v = res[lb * LP + k3];
+      v.x *= f;
+      v.y *= f;
+      res[lb * LP + k3] = v;
     }
     __syncthreads();
     Cplx<real>* spare = res == buf0 ? buf1 : buf0;
@@ -241,28 +239,28 @@ __global__ __launch_bounds__(256) void k_fft_pass(Cplx<real>* __restrict__ grid,
   }
 }
 
-// Influence function in z-pass order: table[(k2 * K1 + k1) * K3 + k3]
-// (lines at consecutive x, element index k3), the same FP64 expression as
-// the on-the-fly path.
-template <class real>
-__global__ void k_conv_table(int K1, int K2, int K3, double pterm, double volterm, double il0, double il1, double il2,
-                             const double* __restrict__ bsmod, real* __restrict__ table) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= (long long)K1 * K2 * K3) return;
-  const int k3 = (int)(e % K3);
-  const long long l = e / K3;
-  const int k1 = (int)(l % K1), k2 = (int)(l / K1);
-  double f = 0;
-  if (k1 != 0 || k2 != 0 || k3 != 0) {
-    const int r1 = k1 < (K1 + 1) / 2 ? k1 : k1 - K1;
-    const int r2 = k2 < (K2 + 1) / 2 ? k2 : k2 - K2;
-    const int r3 = k3 < (K3 + 1) / 2 ? k3 : k3 - K3;
-    const double h1 = r1 * il0, h2 = r2 * il1, h3 = r3 * il2;
-    const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
-    const double term = -pterm * hsq;
-    if (term > -50) f = exp(term) / (hsq * volterm * bsmod[k1] * bsmod[K1 + k2] * bsmod[K1 + K2 + k3]);
+// Separable factors of the influence function, FP64 (see conv_factor):
+// e_d[k] = exp(-pterm h_d^2) / bsmod_d[k], h_d the signed frequency over the
+// box edge, with 1 / (pi V) folded into axis 0.
+__global__ void k_conv_etab(int K1, int K2, int K3, double pterm, double volterm, double il0, double il1, double il2,
+                            const double* __restrict__ bsmod, double* __restrict__ etab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= K1 + K2 + K3) return;
+  int K = K1, k = t, d = 0;
+  if (t >= K1 + K2) {
+    K = K3;
+    k = t - K1 - K2;
+    d = 2;
+  } else if (t >= K1) {
+    K = K2;
+    k = t - K1;
+    d = 1;
   }
-  table[e] = (real)f;
+  const int r = k < (K + 1) / 2 ? k : k - K;
+  const double h = r * (d == 0 ? il0 : (d == 1 ? il1 : il2));
+  double e = exp(-pterm * h * h) / bsmod[t];
+  if (d == 0) e /= volterm;
+  etab[t] = e;
 }
 
 // ---------------------------------------------------------------- planes
@@ -473,14 +471,14 @@ void roundtrip_t(Ctx& c, bool fwd_only, double* vrows, double vscale, bool lines
   conv.vrows = vrows;
   const Cplx<real>* twa[3] = {tw, tw + K1, tw + K1 + K2};
   if (!c.conv_table_valid) {
-    c.conv_table.alloc(sizeof(real) * (size_t)K1 * K2 * K3);
-    hipLaunchKernelGGL(k_conv_table<real>, dim3(ceil_div((long)K1 * K2 * K3, 256)), dim3(256), 0, c.stream, K1, K2, K3,
+    c.conv_table.alloc(sizeof(double) * (size_t)(K1 + K2 + K3));
+    hipLaunchKernelGGL(k_conv_etab, dim3(ceil_div(K1 + K2 + K3, 256)), dim3(256), 0, c.stream, K1, K2, K3,
                        conv.pterm, conv.volterm, conv.il[0], conv.il[1], conv.il[2], conv.bsmod,
-                       c.conv_table.as<real>());
+                       c.conv_table.as<double>());
     T9_LAUNCH_CHECK();
     c.conv_table_valid = true;
   }
-  conv.table = c.conv_table.p;
+  conv.etab = c.conv_table.as<double>();
   auto launch = [&](int axis, int sign, const ConvArgs& cv) {
     launch_pass<real>(grid, K1, K2, K3, axis, sign, p[axis], twa[axis], cv, c.gate, c.stream);
   };
