@@ -347,7 +347,7 @@ void wait_marked(Ctx& c) {
 }
 
 // Make the list, sorted coordinates and global multipoles current.
-void prepare(Ctx& c) {
+void prepare(Ctx& c, bool defer_compact) {
   if (!c.list_valid) {
     nblist_build(c);
   } else if (c.coords_dirty) {
@@ -355,7 +355,7 @@ void prepare(Ctx& c) {
     else nblist_mark_refreshed(c);
   }
   pme_bin_async(c);  // beside the compaction and the rotation
-  if (!c.mlist.cut_valid) nblist_compact(c);
+  if (!c.mlist.cut_valid && !(defer_compact && c.ewald)) nblist_compact(c);
   if (!c.rpole_valid) mpole_init(c);
 }
 
@@ -450,8 +450,8 @@ int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad, bool grad_o
     c.mlist.cut_valid = c.ulist.cut_valid = false;
     c.bins_valid = false;
     c.rpole_valid = false;
-    prepare(c);
     const bool solve = (terms & TERM_ELEC) && c.opt.use_polar;
+    prepare(c, solve);
     if (solve && (terms & TERM_VDW) && c.use_vdw) {
       // the vdW pair kernel does not depend on the dipoles: it runs on the
       // second stream during the solve, into the first energy rows

@@ -108,6 +108,29 @@ __device__ __forceinline__ double t9_erfc_pair(double z, double& expmz2) {
   return erfc(z);
 }
 
+// sqrt(x), 1/x and a/d corrected by one residual step on the hardware
+// estimate (FP32; Markstein's forms): the correction leaves an error within
+// about half an ulp either way, so the rounding is unbiased, at a quarter of
+// the cost of the IEEE sequences.  FP64: the exact operations.
+__device__ __forceinline__ float t9_sqrt_nr(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float s = x * y;
+  return fmaf(0.5f * y, fmaf(-s, s, x), s);
+}
+__device__ __forceinline__ double t9_sqrt_nr(double x) { return sqrt(x); }
+__device__ __forceinline__ float t9_rcp_nr(float d) {
+  const float y = __builtin_amdgcn_rcpf(d);
+  return fmaf(y, fmaf(-d, y, 1.0f), y);
+}
+__device__ __forceinline__ double t9_rcp_nr(double d) { return 1.0 / d; }
+// a / d from the refined reciprocal y of d: q = a y, corrected by the
+// residual a - d q (Markstein)
+__device__ __forceinline__ float t9_div_nr(float a, float d, float y) {
+  const float q = a * y;
+  return fmaf(fmaf(-d, q, a), y, q);
+}
+__device__ __forceinline__ double t9_div_nr(double a, double d, double) { return a / d; }
+
 // Minimum image in an orthorhombic box.
 template <class real>
 __device__ __forceinline__ void min_image(real& x, real& y, real& z, real lx, real ly, real lz, real ilx, real ily,
@@ -296,20 +319,30 @@ __device__ __forceinline__ bool gate_closed(const Gate& g) {
   return d != 0 && d < (double)g.iter;
 }
 
-// v[0..NV) of wave 0 += the same values of waves 1..NP-1, in that order;
-// sh: (NP - 1) * NV * 64 doubles.  Every thread of the workgroup calls it.
+// v[0..NV) of wave 0 = the sum of the NP waves' values (NP a power of 2), by
+// a fixed tree: wave p += wave p + h for h = NP/2, NP/4, ..., 1.  sh:
+// parts_lds<NP, NV>() doubles (half the waves' values at a time, so a
+// workgroup of pair kernels keeps its LDS small enough to share a CU with an
+// FFT plane workgroup).  Every thread of the workgroup calls it.
+template <int NP, int NV>
+constexpr int parts_lds() {
+  return NP / 2 * NV * 64;
+}
 template <int NP, int NV>
 __device__ __forceinline__ void parts_reduce(double* v, double* sh) {
+  static_assert((NP & (NP - 1)) == 0, "parts_reduce: NP must be a power of 2");
   const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
-  if (part > 0)
 #pragma unroll
-    for (int q = 0; q < NV; ++q) sh[((part - 1) * NV + q) * 64 + lane] = v[q];
-  __syncthreads();
-  if (part == 0)
-    for (int p = 1; p < NP; ++p)
+  for (int h = NP / 2; h >= 1; h >>= 1) {
+    if (part >= h && part < 2 * h)
 #pragma unroll
-      for (int q = 0; q < NV; ++q) v[q] += sh[((p - 1) * NV + q) * 64 + lane];
-  __syncthreads();
+      for (int q = 0; q < NV; ++q) sh[((part - h) * NV + q) * 64 + lane] = v[q];
+    __syncthreads();
+    if (part < h)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] += sh[(part * NV + q) * 64 + lane];
+    __syncthreads();
+  }
 }
 
 // Deterministic block sum of doubles: fixed tree over a fixed block size.

@@ -291,7 +291,7 @@ void nblist_build(Ctx& c);
 // check (a rebuild overwrites it); then call nblist_mark_refreshed if no rebuild
 bool nblist_needs_rebuild(Ctx& c, bool refresh = false);
 void nblist_mark_refreshed(Ctx& c);
-void nblist_compact(Ctx& c);              // refresh the in-cutoff lists for the current coordinates
+void nblist_compact(Ctx& c, hipStream_t st = nullptr);  // refresh the in-cutoff lists (main stream unless st)
 void refresh_sorted_coords(Ctx& c);
 void mpole_init(Ctx& c);
 // c.field/fieldp = full direct field; with ua, also fa = the mutual field of
@@ -377,7 +377,10 @@ void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig, b
 // marker event (a blocking synchronize wakes the thread tens of microseconds late)
 void mark_readback(Ctx& c);
 void wait_marked(Ctx& c);
-void prepare(Ctx& c);                     // list, sorted coordinates, global multipoles current
+// list, sorted coordinates, global multipoles current; defer_compact: the
+// in-cutoff lists are left for the induced-dipole solve (dfield_all
+// compacts them on the third stream)
+void prepare(Ctx& c, bool defer_compact = false);
 
 // molecular dynamics (caller order, FP64)
 void md_velocities(Ctx& c, double kelvin, uint64_t seed);  // Maxwell-Boltzmann, net momentum removed

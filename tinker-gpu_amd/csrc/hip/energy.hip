@@ -283,7 +283,7 @@ struct EmplarArgs {
 template <class real, int EWALD, int MODE>
 __global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A) {
   constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
-  __shared__ double sh[(kEmplarParts - 1) * 19 * 64];
+  __shared__ double sh[parts_lds<kEmplarParts, 19>()];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   Acc<real> acc;

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const in
                                                                  real* __restrict__ fd, real* __restrict__ fp,
                                                                  Gate gate) {
   if (gate_closed(gate)) return;
-  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   double ab[6] = {0, 0, 0, 0, 0, 0};
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const in
                                                      const int* __restrict__ jpolar, const real* __restrict__ thlval,
                                                      int nj, real pga_u, ScaleTable sc, BoxT<real> box,
                                                      EwaldK<real> ek, real* __restrict__ fd, real* __restrict__ fp) {
-  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   double ab[6] = {0, 0, 0, 0, 0, 0};
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
                                                  real* __restrict__ z2d, real* __restrict__ z2p,
                                                  double* __restrict__ dots64, Gate gate) {
   if (gate_closed(gate)) return;
-  __shared__ double sh[(kPairParts - 1) * 6 * 64];
+  __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   const int si = s < n ? s : 0;
@@ -450,6 +450,7 @@ void dfield_all(Ctx& c, const void* ua, void* fa) {
   const char* ua_d = (const char*)ua;
   char* fa_d = (char*)fa;
   if (!c.ewald) {
+    if (!c.mlist.cut_valid) nblist_compact(c);
     if (c.prec == 32) dfield_real_t<float>(c, c.stream);
     else dfield_real_t<double>(c, c.stream);
     if (ua) ufield_all(c, (void*)ua_d, (void*)(ua_d + v3), fa_d, fa_d + v3);
@@ -462,6 +463,10 @@ void dfield_all(Ctx& c, const void* ua, void* fa) {
   // stream while the direct field's real space still runs
   T9_HIP(hipEventRecord(c.ev_ufork, c.stream));
   T9_HIP(hipStreamWaitEvent(c.side2, c.ev_ufork, 0));
+  // the in-cutoff lists of the current coordinates, if the caller left them:
+  // compacted on the third stream ahead of the real-space kernels, beside
+  // the PME work of the main stream (whose gathers wait for this stream)
+  if (!c.mlist.cut_valid) nblist_compact(c, c.side2);
   if (c.prec == 32) dfield_real_t<float>(c, c.side2);
   else dfield_real_t<double>(c, c.side2);
   T9_HIP(hipEventRecord(c.ev_ujoin, c.side2));

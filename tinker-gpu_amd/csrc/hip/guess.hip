@@ -44,9 +44,15 @@ __device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
   return (u16)(u >> 16);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative from the hardware base-2 exponential and
+// reciprocal (relative error near 1e-7, far below the BF16 rounding that
+// follows); z below -87 gives exp2 = inf, sigmoid 0.
+__device__ __forceinline__ float sigmoid_hw(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.44269504088896341f));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_hw(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = sigmoid_hw(z);
   return s * (1.0f + z * (1.0f - s));
 }
 

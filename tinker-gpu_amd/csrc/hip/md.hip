@@ -80,7 +80,7 @@ struct VdwArgs {
 // p, p + kPairParts, ...) and their sums are added in part order.
 template <class real, bool VIR>
 __global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
-  __shared__ double sh[(kPairParts - 1) * 10 * 64];
+  __shared__ double sh[parts_lds<kPairParts, 10>()];
   const int part = threadIdx.x >> 6;
   const int s = blockIdx.x * 64 + (threadIdx.x & 63);
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // e, g[3], v[6]
@@ -113,10 +113,11 @@ __global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
       pair_disp(pk, pi, A.box, R[0], R[1], R[2]);  // site i - site k
       const real r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
       if (!(r2 <= A.cut2)) continue;
-      // correctly rounded square root and division keep the per-pair
-      // rounding unbiased (the hardware reciprocal square root is not)
-      const real r = sqrt(r2);
-      const real rinv = 1 / r;
+      // refined hardware reciprocal square root and reciprocals: errors of
+      // about half an ulp either way, so the per-pair rounding stays unbiased
+      // (the raw hardware estimates are not)
+      const real r = t9_sqrt_nr(r2);
+      const real rinv = t9_rcp_nr(r);
       const int t = ji + A.jvdw[k];
       // the pair parameters are shared by every pair of two types, so their
       // rounding would bias the whole sum: hi + lo split, as for coordinates
@@ -124,14 +125,16 @@ __global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
       const real eh = A.eps[2 * t] * sc, el = A.eps[2 * t + 1] * sc;
       const real rho = fma(r, ivh, r * ivl);
       const real rho6 = rho * rho * rho * rho * rho * rho, rho7 = rho6 * rho;
-      const real b = (real)1.07 / (rho + (real)0.07);
+      const real d7 = rho + (real)0.07;
+      const real y7 = t9_rcp_nr(d7);
+      const real b = t9_div_nr((real)1.07, d7, y7);
       const real b2 = b * b, b3 = b2 * b;
       const real t1 = b3 * b3 * b;
-      const real s2 = 1 / (rho7 + (real)0.12);
+      const real s2 = t9_rcp_nr(rho7 + (real)0.12);
       const real t2 = (real)1.12 * s2;
       const real u = t1 * (t2 - 2);
       real en_ = fma(u, eh, u * el);
-      real de = eh * ivh * (-7 * t1 / (rho + (real)0.07) * (t2 - 2) - 7 * t1 * rho6 * t2 * s2);
+      real de = eh * ivh * (-7 * t1 * y7 * (t2 - 2) - 7 * t1 * rho6 * t2 * s2);
       if (r > A.taper) {
         const real x = (r - A.cut) * inv_tc;
         const real S = x * x * x * (x * (6 * x - 15) + 10);

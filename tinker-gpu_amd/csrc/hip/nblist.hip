@@ -943,7 +943,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
 }
 
 template <class real>
-void compact_t(Ctx& c) {
+void compact_t(Ctx& c, hipStream_t st) {
   BoxT<real> box = make_boxT<real>(c);
   NbList* L = &c.mlist;
   if (L->cutoff <= 0 || !L->entries.p) return;
@@ -955,7 +955,7 @@ void compact_t(Ctx& c) {
     L->ecut2.alloc(sizeof(uint32_t) * L->capacity);
   }
   const double c2 = L->cut * L->cut, c2b = L->cut2nd * L->cut2nd;
-  hipLaunchKernelGGL(k_compact_parts<real>, dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, c.n,
+  hipLaunchKernelGGL(k_compact_parts<real>, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n,
                      L->nlist.as<int>(), L->off.as<long long>(), L->entries.as<uint32_t>(), c.pos.as<PosRec<real>>(),
                      box, (real)(c2 > 1e30 ? 1e30 : c2), (real)(two ? c2b : -1.0), L->ncut.as<int>(),
                      L->ecut.as<uint32_t>(), two ? L->ncut2.as<int>() : nullptr,
@@ -1023,9 +1023,10 @@ bool nblist_needs_rebuild(Ctx& c, bool refresh) {
   return c.prec == 32 ? needs_rebuild_t<float>(c, refresh) : needs_rebuild_t<double>(c, refresh);
 }
 
-void nblist_compact(Ctx& c) {
-  if (c.prec == 32) compact_t<float>(c);
-  else compact_t<double>(c);
+void nblist_compact(Ctx& c, hipStream_t st) {
+  if (!st) st = c.stream;
+  if (c.prec == 32) compact_t<float>(c, st);
+  else compact_t<double>(c, st);
 }
 
 void refresh_sorted_coords(Ctx& c) {

@@ -95,11 +95,13 @@ __global__ __launch_bounds__(kB) void k_save_hist(int n, const int* __restrict__
   }
 }
 
+// udir = alpha E (d, p) and the initial guess: a copy of `from` ([2][n][3]:
+// d, then p) when given, else udir (guess != 0) or zero
 template <class real>
 __global__ __launch_bounds__(kB) void k_udir(int n, const real* __restrict__ pol, const real* __restrict__ fd,
                                             const real* __restrict__ fp, real* __restrict__ udir,
                                             real* __restrict__ udirp, real* __restrict__ ud, real* __restrict__ up,
-                                            int guess) {
+                                            int guess, const real* __restrict__ from) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const real a = pol[s];
@@ -107,8 +109,8 @@ __global__ __launch_bounds__(kB) void k_udir(int n, const real* __restrict__ pol
     const real x = a * fd[3 * s + d], y = a * fp[3 * s + d];
     udir[3 * s + d] = x;
     udirp[3 * s + d] = y;
-    ud[3 * s + d] = guess ? x : 0;
-    up[3 * s + d] = guess ? y : 0;
+    ud[3 * s + d] = from ? from[3 * s + d] : (guess ? x : 0);
+    up[3 * s + d] = from ? from[3 * (size_t)n + 3 * s + d] : (guess ? y : 0);
   }
 }
 
@@ -410,7 +412,6 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   Coef16 cf;
   if (predict)
     for (int k = 0; k < kMaxUalt; ++k) cf.c[k] = B[((c.nualt - 1 - k) % kMaxUalt + kMaxUalt) % kMaxUalt];
-  const size_t vbytes = sizeof(real) * 3 * (size_t)n;
   real* ua = c.uaspc.as<real>();
   real* ra = learned ? c.raspc.as<real>() : nullptr;
   if (learned) {
@@ -423,8 +424,10 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   } else {
     dfield_all(c);
   }
+  // the learned solve starts from the ASPC guess (computed above), the
+  // corrections are added below
   hipLaunchKernelGGL(k_udir<real>, G, Bk, 0, st, n, c.polarity.as<real>(), field, fieldp, c.udir.as<real>(),
-                     c.udirp.as<real>(), ud, up, (guess || predict) ? 1 : 0);
+                     c.udirp.as<real>(), ud, up, (guess || predict) ? 1 : 0, learned ? (const real*)ua : nullptr);
   T9_LAUNCH_CHECK();
   auto slot_of_age = [](int count, int age) { return ((count - 1 - age) % kMaxUalt + kMaxUalt) % kMaxUalt; };
   // r0 of the guess in ud/up
@@ -437,10 +440,7 @@ int induce_t(Ctx& c, int* iters, double* rms) {
   double* trace = c.pcg_trace.as<double>();
   c.last_predictor = predict ? 1 : 0;
   if (predict) {
-    if (learned) {
-      T9_HIP(hipMemcpyAsync(ud, ua, vbytes, hipMemcpyDeviceToDevice, st));
-      T9_HIP(hipMemcpyAsync(up, ua + 3 * (size_t)n, vbytes, hipMemcpyDeviceToDevice, st));
-    } else {
+    if (!learned) {
       hipLaunchKernelGGL(k_aspc_guess<real>, G, Bk, 0, st, n, c.iperm.as<int>(), cf, c.ualt.as<real>(), ud, up,
                          feats ? ua : nullptr);
       T9_LAUNCH_CHECK();
