@@ -41,6 +41,19 @@ struct RealTraits<double> {
   static constexpr double fixed_scale = 17592186044416.0;  // 2^44
 };
 
+// The dispatcher hands workgroup w to XCD w mod 8, and each of the 8 XCDs has
+// its own L2.  xcd_block() renumbers the workgroups of a launch so that XCD x
+// runs one contiguous range of logical blocks: neighbouring atoms (Morton
+// order) or grid tiles then share an L2 instead of being spread over all
+// eight.  It is a bijection on [0, gridDim.x); a kernel that uses it for every
+// block-indexed access computes the same values as with blockIdx.x.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ float t9_rint(float x) { return rintf(x); }
 __device__ __forceinline__ double t9_rint(double x) { return rint(x); }
 __device__ __forceinline__ float t9_exp(float x) { return expf(x); }

@@ -285,7 +285,8 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A
   constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
   __shared__ double sh[parts_lds<kEmplarParts, 19>()];
   const int part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int blk = xcd_block();
+  const int s = blk * 64 + (threadIdx.x & 63);
   Acc<real> acc;
   acc.zero();
   if (s < A.n) {
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A
     red[EP_MREAL] = 0.5 * kCoul * acc.e;
     for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * kCoul * acc.v[k];
   }
-  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kRow);
+  block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
 }
 
 // ------------------------------------------------------------ PME
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(256) void k_recip_mpole(int n, GridGeom g, const R4
                                                      const real* __restrict__ rpole, int use_m, double* __restrict__ grad,
                                                      double* __restrict__ tq, double* __restrict__ cphim,
                                                      double* __restrict__ rows) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block();
+  const int s = blk * blockDim.x + threadIdx.x;
   double red[EP_N];
   for (int q = 0; q < EP_N; ++q) red[q] = 0;
   if (s < n) {
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(256) void k_recip_mpole(int n, GridGeom g, const R4
       vir_moments(m + 1, m + 4, c + 1, c + 4, -f, red + EP_VXX);
     }
   }
-  block_row<EP_N>(red, rows + (size_t)blockIdx.x * kRow);
+  block_row<EP_N>(red, rows + (size_t)blk * kRow);
 }
 
 template <class real>
@@ -536,7 +538,8 @@ __global__ __launch_bounds__(256) void k_recip_polar(int n, GridGeom g, const R4
                                                      const real* __restrict__ up, const double* __restrict__ cphim,
                                                      double self3, double* __restrict__ grad,
                                                      double* __restrict__ tq, double* __restrict__ rows) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block();
+  const int s = blk * blockDim.x + threadIdx.x;
   double red[EP_N];
   for (int q = 0; q < EP_N; ++q) red[q] = 0;
   if (s < n) {
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(256) void k_recip_polar(int n, GridGeom g, const R4
     vir_moments(u, zq, cp + 1, cp + 4, -0.5 * f, red + EP_VXX);
     vir_moments(v, zq, cd + 1, cd + 4, -0.5 * f, red + EP_VXX);
   }
-  block_row<EP_N>(red, rows + (size_t)blockIdx.x * kRow);
+  block_row<EP_N>(red, rows + (size_t)blk * kRow);
 }
 
 // Ewald self energy of the multipoles and the dot-product polarization energy

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_ufield_real(int n, const in
   if (gate_closed(gate)) return;
   __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int s = xcd_block() * 64 + (threadIdx.x & 63);
   double ab[6] = {0, 0, 0, 0, 0, 0};
   if (s < n) {
     const int len = ncut[s];
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const in
                                                      EwaldK<real> ek, real* __restrict__ fd, real* __restrict__ fp) {
   __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int s = xcd_block() * 64 + (threadIdx.x & 63);
   double ab[6] = {0, 0, 0, 0, 0, 0};
   double &a0 = ab[0], &a1 = ab[1], &a2 = ab[2], &b0 = ab[3], &b1 = ab[4], &b2 = ab[5];
   const int len = s < n ? ncut[s] : 0;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
   if (gate_closed(gate)) return;
   __shared__ double sh[parts_lds<kPairParts, 6>()];
   const int part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int s = xcd_block() * 64 + (threadIdx.x & 63);
   const int si = s < n ? s : 0;
   const real poli = polarity[si];
   const real f = sparse ? udiag * poli : poli;
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_precond(int n, int sparse, 
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double x = wave_bfly_sum(v[q]);
-      if ((threadIdx.x & 63) == 0) dots64[blockIdx.x * 4 + q] = x;
+      if ((threadIdx.x & 63) == 0) dots64[(s >> 6) * 4 + q] = x;
     }
   }
 }

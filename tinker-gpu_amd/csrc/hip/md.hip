@@ -82,7 +82,8 @@ template <class real, bool VIR>
 __global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
   __shared__ double sh[parts_lds<kPairParts, 10>()];
   const int part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int blk = xcd_block();
+  const int s = blk * 64 + (threadIdx.x & 63);
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // e, g[3], v[6]
   double& e = acc[0];
   double* g = acc + 1;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_vdw(VdwArgs<real> A) {
     red[EP_EVDW] = 0.5 * e;
     for (int q = 0; q < 6; ++q) red[EP_VXX + q] = 0.5 * v[q];
   }
-  block_row<EP_N>(red, A.rows + (size_t)blockIdx.x * kEnergyRow);
+  block_row<EP_N>(red, A.rows + (size_t)blk * kEnergyRow);
 }
 
 // atom gradient (sorted egrad) += own site share + shares of the reduced sites

@@ -291,7 +291,8 @@ __device__ __forceinline__ PosRec<real> bcast(const PosRec<real>& p, int t) {
 template <class real>
 __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
   const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
-  const int s0 = blockIdx.x * 64 + lane;
+  const int blk = xcd_block();
+  const int s0 = blk * 64 + lane;
   const bool active = s0 < A.n;
   const int s = active ? s0 : A.n - 1;  // idle lanes mirror the last atom
   const auto pi = A.pos[s];
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(64 * kParts) void k_nb_lists(ListArgs<real> A) {
   if (A.fill == 2 && active) {
     // a window of wm (wu, wv) slots per atom and part; entries beyond it are
     // counted, not written (the host then writes the lists at the counted offsets)
-    const size_t wb = (size_t)blockIdx.x * kParts + part;
+    const size_t wb = (size_t)blk * kParts + part;
     om = A.em + wb * A.wm * 64 + lane;
     if (has_u) ou = A.eu + wb * A.wu * 64 + lane;
     if (has_v) ov = A.ev + wb * A.wv * 64 + lane;
@@ -554,7 +555,8 @@ __global__ __launch_bounds__(64 * kParts) void k_pack_list(int n, const int* __r
   __shared__ int cnt[kParts][64];
   __shared__ uint32_t rows[kPackRows][64];
   const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
-  const int s = blockIdx.x * 64 + lane;
+  const int blk = xcd_block();
+  const int s = blk * 64 + lane;
   const int mine = s < n ? parts[(size_t)part * n + s] : 0;
   cnt[part][lane] = mine;
   __syncthreads();
@@ -565,8 +567,8 @@ __global__ __launch_bounds__(64 * kParts) void k_pack_list(int n, const int* __r
     len += v;
   }
   const int maxlen = wave_max(len);  // the same in every wavefront of the workgroup
-  const uint32_t* src = stage + ((size_t)blockIdx.x * kParts + part) * w * 64 + lane;
-  uint32_t* dst = out + off[blockIdx.x];
+  const uint32_t* src = stage + ((size_t)blk * kParts + part) * w * 64 + lane;
+  uint32_t* dst = out + off[blk];
   for (int r0 = 0; r0 < maxlen; r0 += kPackRows) {
     // this lane's window rows [k0, k1) land in the chunk (walking the union
     // of the wavefront's ranges row by row was measured slower: the ranges
@@ -864,12 +866,13 @@ __global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
   __shared__ unsigned short cnt_a[P][64], cnt_b[P][64];
   __shared__ uint32_t rows_a[kCompactRowsA][64], rows_b[kCompactRowsB][64];
   const int part = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = blockIdx.x * 64 + lane;
+  const int blk = xcd_block();
+  const int s = blk * 64 + lane;
   const bool two = ecut2 != nullptr;
   const bool valid = s < n;
   const int len = valid ? nlist[s] : 0;
   const int maxlen = wave_max(len);
-  const long long base = woff[blockIdx.x] + lane;
+  const long long base = woff[blk] + lane;
   const uint32_t* e = entries + base;
   PosRec<real> pi;
   if (valid) pi = pos[s];
@@ -937,7 +940,7 @@ __global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
   // kernel reads (it reads positions below ncut)
   const int ra = min(wave_max(tot_a), kCompactRowsA), rb = two ? min(wave_max(tot_b), kCompactRowsB) : 0;
   __syncthreads();
-  const long long wb = woff[blockIdx.x];
+  const long long wb = woff[blk];
   for (int q = threadIdx.x; q < ra * 64; q += 64 * P) ecut[wb + q] = rows_a[q >> 6][q & 63];
   for (int q = threadIdx.x; q < rb * 64; q += 64 * P) ecut2[wb + q] = rows_b[q >> 6][q & 63];
 }

@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
   __shared__ __align__(16) char smem[kStage > kPart ? kStage : kPart];
   RA* atoms = reinterpret_cast<RA*>(smem);
   __shared__ int bstart[27], bpref[28];
-  const int tt = blockIdx.x;
+  const int tt = blockIdx.x;  // tiles in launch order (an XCD-contiguous order measured slower)
   const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
   const int lo[3] = {t1 * kRowTile, t2 * kRowTile, t3 * kRowTile};
   const int nb[3] = {g.K[0] / kRowBin, g.K[1] / kRowBin, g.K[2] / kRowBin};
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_gather_dip(int n, TileGeom g, const R4<
                                                     const real* __restrict__ up, real* __restrict__ fd,
                                                     real* __restrict__ fp, Gate gate) {
   if (gate_closed(gate)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const R4<real> w = pw[s];
   const int4 b = pbase[s];
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_gather_mpole_field(int n, TileGeom g, c
                                                             const cplx_t<real>* __restrict__ grid, real a1, real a2,
                                                             real a3, real term, const real* __restrict__ rpole,
                                                             real* __restrict__ fd, real* __restrict__ fp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const R4<real> w = pw[s];
   const int4 b = pbase[s];
