@@ -72,6 +72,9 @@ struct Oracle {
   std::vector<int> bond_ij, angle_abc, urey_ac;
   std::vector<double> bond_kr, angle_kt, urey_kr;
   double cbnd = 0, qbnd = 0, cang = 0, qang = 0, pang = 0, sang = 0;
+  // HIPPO charge penetration (core charge, valence charge, valence alpha)
+  bool chgpen = false;
+  std::vector<double> pcore, pval, palpha;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -808,6 +811,25 @@ void* orc_create(int n, const double* xyz, const double* pole, const int* frame,
 }
 
 void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+// HIPPO damping factors lambda_1 .. lambda_11 at distance r: core-valence
+// with exponent ai (out[0..5]), with exponent ak (out[6..11]), valence-valence
+// (out[12..17]).
+void orc_damp_chgpen(double r, double ai, double ak, double* out) {
+  damp_core_valence(r, ai, out);
+  damp_core_valence(r, ak, out + 6);
+  damp_valence_valence(r, ai, ak, out + 12);
+}
+
+// HIPPO charge penetration: per-atom core charge, valence charge and valence
+// exponent; the multipole energy then uses the damped pair kernels.
+void orc_set_chgpen(void* h, const double* pcore, const double* pval, const double* palpha) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->chgpen = true;
+  o->pcore.assign(pcore, pcore + o->n);
+  o->pval.assign(pval, pval + o->n);
+  o->palpha.assign(palpha, palpha + o->n);
+}
 
 void orc_set_xyz(void* h, const double* xyz) {
   Oracle* o = static_cast<Oracle*>(h);

@@ -43,6 +43,8 @@ def _lib():
                                  ctypes.c_int, _i, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i,
                                  ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_chgpen.argtypes = [ctypes.c_void_p, _d, _d, _d]
+        L.orc_damp_chgpen.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]
         L.orc_set_xyz.argtypes = [ctypes.c_void_p, _d]
         L.orc_get_rpole.argtypes = [ctypes.c_void_p, _d]
         L.orc_get_bsmod.argtypes = [ctypes.c_void_p, ctypes.c_int, _d]
@@ -120,6 +122,15 @@ def interaction_counts(a):
     return nem, nep
 
 
+def damp_chgpen(r, ai, ak):
+    """HIPPO damping factors lambda_1..lambda_11 at distance r (Å) for valence
+    exponents ai, ak (1/Å): (core-valence with ai, core-valence with ak,
+    valence-valence), each an array of 6."""
+    out = np.zeros(18)
+    _lib().orc_damp_chgpen(float(r), float(ai), float(ak), _p(out))
+    return out[:6], out[6:12], out[12:]
+
+
 def vdw_count(a):
     """Analyze-mode vdW count nev (src/acc/ehal.cpp:127,215-216): site pairs
     within the vdW cutoff less the pairs with vdW scale 0.  Sites: reduced
@@ -178,6 +189,8 @@ class Oracle:
             _p(f64(sysd["polarity"])), _p(f64(sysd["pdamp"])), _p(i32(sysd["jpolar"]), _i), int(sysd["njpolar"]),
             _p(f64(sysd["thlval"])), int(sysd["nscale"]), _p(ik, _i), _p(mdpu), _p(box) if box is not None else None,
             int(sysd["ewald"]), float(sysd["aewald"]), float(cut), _p(grid, _i), int(sysd["pme_order"]))
+        if sysd.get("chgpen"):
+            L.orc_set_chgpen(self.h, _p(f64(sysd["pcore"])), _p(f64(sysd["pval"])), _p(f64(sysd["palpha"])))
         if "jvdw" in sysd:
             nj = int(sysd["njvdw"])
             vs_ik = i32(sysd["vscale_ik"]) if len(sysd["vscale_s"]) else i32(np.zeros((1, 2)))

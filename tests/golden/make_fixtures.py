@@ -19,6 +19,8 @@ What it writes (numbers and data files only, no reference source text):
     src/cu/upredict.cu:50-65 (16 numbers)
   * tests/golden/nacl_vdw_ref.json — test/nacl.cpp NaCl-1 vdW energies,
     gradients, virials and the ion separations of test/file/nacl/
+  * tests/golden/c5h12acnh2.xyz, hippo19_subset.prm, hippo_ref.json — the
+    HIPPO multipole test (test/emhippo.cpp, test/ref/emhippo.{1,2}.txt)
   * tinker-gpu_amd/params/amoeba09.prm — the subset of
     test/file/commit_6fe8e913/amoeba09.prm this path uses: the scale-factor
     header and the atom/vdw/bond/angle/ureybrad/multipole/polarize records of
@@ -312,6 +314,42 @@ def write_more():
     return sorted(out)
 
 
+def write_hippo():
+    """HIPPO charge-penetration multipoles (test/emhippo.cpp): the coordinates
+    of test/file/c5h12acnh2/c5h12acnh2.xyz, the parameter header and the atom,
+    multipole and chgpen records of its types from test/file/hippo/hippo19.prm,
+    and the energy, count, virial and gradient of test/ref/emhippo.{1,2}.txt
+    (Ewald at 7 Å and non-Ewald at 9 Å in a 35 x 30 x 25 Å box)."""
+    copy_file("test/file/c5h12acnh2/c5h12acnh2.xyz", "c5h12acnh2.xyz")
+    xyz = open(os.path.join(REF, "test/file/c5h12acnh2/c5h12acnh2.xyz")).read().splitlines()
+    types = {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5}
+    lines = open(os.path.join(REF, "test/file/hippo/hippo19.prm")).read().splitlines()
+    classes = {int(ln.split()[2]) for ln in lines if ln.split()[:1] == ["atom"] and int(ln.split()[1]) in types}
+    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole and chgpen records of the c5h12acnh2 test system", ""]
+    i, header = 0, True
+    while i < len(lines):
+        t = lines[i].split()
+        if t and t[0] == "atom":
+            header = False
+        if header and len(t) >= 1 and not lines[i].lstrip().startswith("#") and re.match(r"^[a-z][a-z0-9-]*$", t[0]) \
+                and len(t) <= 2:
+            out.append(lines[i])
+        elif t and t[0] == "atom" and int(t[1]) in types:
+            out.append(lines[i])
+        elif t and t[0] == "multipole" and int(t[1]) in types:
+            out.extend(lines[i:i + 5])
+            i += 5
+            continue
+        elif t and t[0] == "chgpen" and int(t[1]) in classes:
+            out.append(lines[i])
+        i += 1
+    open(os.path.join(HERE, "hippo19_subset.prm"), "w").write("\n".join(out) + "\n")
+    ref = {"emhippo_ewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.1.txt")),
+           "emhippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.2.txt"))}
+    json.dump(ref, open(os.path.join(HERE, "hippo_ref.json"), "w"), indent=0)
+    return sorted(ref)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not mounted")
@@ -328,6 +366,7 @@ def main():
     print("tinkernist text:", write_tinkernist_text())
     print("aspc table:", write_aspc())
     print("nacl:", write_nacl())
+    print("hippo:", write_hippo())
     print("more:", write_more())
 
 

@@ -81,6 +81,9 @@ int t9_system_get_f64(const t9_system* sys, const char* name, double* out) {
     else if (k == "polarity") v = &s.polarity;
     else if (k == "pdamp") v = &s.pdamp;
     else if (k == "thole") v = &s.thole;
+    else if (k == "pcore") v = &s.pcore;
+    else if (k == "pval") v = &s.pval;
+    else if (k == "palpha") v = &s.palpha;
     else if (k == "mass") v = &s.mass;
     else if (k == "thlval") v = &s.thlval;
     else if (k == "vdw_rad") v = &s.vdw_rad;
@@ -159,6 +162,7 @@ int t9_system_get_scalar(const t9_system* sys, const char* name, double* out) {
     else if (k == "periodic") v = s.periodic;
     else if (k == "use_mpole") v = o.use_mpole;
     else if (k == "use_polar") v = o.use_polar;
+    else if (k == "chgpen") v = s.chgpen;
     else if (k == "mpole_cut") v = o.mpole_cut;
     else if (k == "vdw_cut") v = o.vdw_cut;
     else if (k == "vdw_taper") v = o.vdw_taper;

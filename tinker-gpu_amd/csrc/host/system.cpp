@@ -230,14 +230,21 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
       mp.pole[kQyz] = v[8] * b2;
       mp.pole[kQzz] = v[9] * b2;
       ff.mpoles.push_back(mp);
-    } else if (key == "polarize" && tok.size() >= 4) {
+    } else if (key == "polarize" && tok.size() >= 3) {
+      // AMOEBA: polarize type alpha thole group-types...; HIPPO files have no
+      // Thole column (their polarization damping is the charge penetration)
       PolarizeParam p;
       int t = std::stoi(tok[1]);
       p.alpha = to_d(tok[2]);
-      p.thole = to_d(tok[3]);
-      for (size_t j = 4; j < tok.size(); ++j)
+      const bool hippo = lower(ff.name).rfind("hippo", 0) == 0;
+      size_t j = 3;
+      if (!hippo && tok.size() >= 4) p.thole = to_d(tok[j++]);
+      for (; j < tok.size(); ++j)
         if (is_int(tok[j])) p.group_types.push_back(std::stoi(tok[j]));
       ff.polarize[t] = p;
+    } else if (key == "chgpen" && tok.size() >= 4) {
+      // chgpen  class  core-charge  alpha
+      ff.chgpen[std::stoi(tok[1])] = {to_d(tok[2]), to_d(tok[3])};
     } else if (key == "polpair" && tok.size() >= 4) {
       int a = std::stoi(tok[1]), b = std::stoi(tok[2]);
       ff.polpair[{std::min(a, b), std::max(a, b)}] = to_d(tok[3]);
@@ -604,6 +611,33 @@ void assign_polarize(System& s) {
     }
 }
 
+// HIPPO charge penetration: core charge and alpha of the atom's class, the
+// valence charge is the rest of the monopole.  The damped multipole kernel is
+// built for the permanent multipoles only: HIPPO polarization (its damped
+// direct and mutual fields) is not, and is rejected.
+void assign_chgpen(System& s) {
+  s.chgpen = !s.ff.chgpen.empty() && s.opt.use_mpole;
+  s.pcore.clear();
+  s.pval.clear();
+  s.palpha.clear();
+  if (!s.chgpen) return;
+  if (s.opt.use_polar)
+    throw std::runtime_error("HIPPO polarization is not built: switch it off (polarizeterm none) to use the "
+                             "charge-penetration multipoles");
+  const int n = s.n;
+  s.pcore.assign(n, 0.0);
+  s.pval.assign(n, 0.0);
+  s.palpha.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    s.pval[i] = s.pole[kMplTotal * i + kQ];
+    auto it = s.ff.chgpen.find(s.cls[i]);
+    if (it == s.ff.chgpen.end()) continue;
+    s.pcore[i] = it->second.first;
+    s.pval[i] -= it->second.first;
+    s.palpha[i] = it->second.second;
+  }
+}
+
 void build_scales(System& s) {
   const int n = s.n;
   const ForceField& ff = s.ff;
@@ -760,6 +794,7 @@ void setup_system(System& s) {
   build_polgroups(s);
   assign_multipoles(s);
   assign_polarize(s);
+  assign_chgpen(s);
   build_scales(s);
   assign_vdw(s);
   build_vdw_tables(s);

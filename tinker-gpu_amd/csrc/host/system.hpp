@@ -77,6 +77,9 @@ struct ForceField {
   std::vector<MultipoleParam> mpoles;           // file order
   std::map<int, PolarizeParam> polarize;        // by atom type
   std::map<std::pair<int, int>, double> polpair;  // pair Thole override
+  // HIPPO charge penetration by class: core charge, valence damping alpha
+  // (the chgpen records of a HIPPO parameter file)
+  std::map<int, std::pair<double, double>> chgpen;
   std::map<int, VdwParam> vdw;                  // by class
   std::map<std::pair<int, int>, BondParam> bond;  // by class pair (sorted)
   std::map<std::tuple<int, int, int>, AngleParam> angle;
@@ -138,6 +141,11 @@ struct System {
   std::vector<int> frame, zaxis, xaxis, yaxis, ysign;  // -1 = unused axis atom
   std::vector<double> polarity, thole, pdamp;
   std::vector<int> jpolar;      // index of the polarize type of each atom
+  // HIPPO charge penetration, per atom (empty unless the parameters have
+  // chgpen records): the monopole is a core charge pcore plus a damped
+  // valence charge pval = q - pcore with damping exponent palpha
+  bool chgpen = false;
+  std::vector<double> pcore, pval, palpha;
   std::vector<double> thlval;   // [njpolar][njpolar] pair Thole values
   int njpolar = 0;
 

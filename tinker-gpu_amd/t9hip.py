@@ -246,6 +246,9 @@ class System:
                   "use_polar"):
             d[k] = self.scalar(k)
         d["grid"] = np.array([int(self.scalar("grid%d" % j)) for j in (1, 2, 3)], dtype=np.int32)
+        d["chgpen"] = bool(self.scalar("chgpen"))
+        if d["chgpen"]:  # HIPPO charge penetration: core charge, valence charge, valence alpha
+            d.update(pcore=self.f64("pcore", n), pval=self.f64("pval", n), palpha=self.f64("palpha", n))
         # MD terms: vdW tables and valence lists
         nj = int(self.scalar("njvdw"))
         nvs, nb, na, nu = (int(self.scalar(k)) for k in ("nvscale", "nbond", "nangle", "nurey"))
