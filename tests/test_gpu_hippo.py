@@ -1,0 +1,55 @@
+"""HIPPO charge-penetration multipoles on the GPU through the C ABI.
+
+test/emhippo.cpp: pentane + acetamide (104 atoms, HIPPO-2019) in a
+35 x 30 x 25 Å box, Ewald at 7 Å (PME) and non-Ewald at 9 Å.  Against the
+reference goldens (test/ref/emhippo.{1,2}.txt) with the reference test's
+tolerances — FP64: energy 1e-4, gradient 1e-4 (+5e-5, the rounding of the
+printed values), virial 1e-3; FP32: energy 1.6e-3 (Ewald) / 5e-3 (non-Ewald),
+gradient 3e-4, virial 2e-3 / 2.5e-3 — and against the FP64 oracle (FP64:
+1e-10 relative energy, 1e-8 gradient and virial), plus the analyze-mode
+interaction count and the rejection of HIPPO polarization calls."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import t9hip
+from test_oracle_hippo import hippo_system
+
+pytestmark = pytest.mark.gpu
+
+TOL = {(True, t9hip.T9_FP64): (1e-4, 1.5e-4, 1e-3), (False, t9hip.T9_FP64): (1e-4, 1.5e-4, 1e-3),
+       (True, t9hip.T9_FP32): (1.6e-3, 3e-4, 2e-3), (False, t9hip.T9_FP32): (5e-3, 3e-4, 2.5e-3)}
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+@pytest.mark.parametrize("ewald", [True, False])
+def test_emhippo_gpu(golden_dir, ewald, prec):
+    print(f"case emhippo ewald={ewald} prec={prec}", flush=True)
+    ref = json.load(open(os.path.join(golden_dir, "hippo_ref.json")))["emhippo_ewald" if ewald else "emhippo_nonewald"]
+    s = hippo_system(golden_dir, ewald)
+    ctx = t9hip.Context(s, prec)
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    de, dg, dv = abs(e["esum"] - ref["energy"]), np.abs(g - rg).max(), np.abs(v - rv).max()
+    print(f"  E={e['esum']:.6f} ref={ref['energy']} |dE|={de:.2e} max|dg|={dg:.2e} max|dv|={dv:.2e}")
+    te, tg, tv = TOL[(ewald, prec)]
+    assert de < te and dg < tg and dv < tv
+    ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert ea["nem"] == ref["count"]
+    if prec == t9hip.T9_FP64:
+        eo, go, vo = oracle.Oracle(s.arrays()).energy(mpole=True, polar=False)
+        print(f"  vs oracle: {abs(e['esum'] - eo['em']):.2e} {np.abs(g - go).max():.2e} {np.abs(v - vo).max():.2e}")
+        assert abs(e["esum"] - eo["em"]) < 1e-10 * abs(eo["em"])
+        assert np.abs(g - go).max() < 1e-8
+        assert np.abs(v - vo).max() < 1e-8
+
+
+def test_hippo_polarization_rejected(golden_dir):
+    print("case hippo polarization rejected", flush=True)
+    ctx = t9hip.Context(hippo_system(golden_dir, True), t9hip.T9_FP64)
+    with pytest.raises(t9hip.T9Error) as ei:
+        ctx.induce()
+    assert ei.value.code == t9hip.T9_ERR_ARG

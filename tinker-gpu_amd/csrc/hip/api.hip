@@ -122,6 +122,17 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   upload(c.polarity_o, s.polarity, st);
   upload(c.pdamp_o, s.pdamp, st);
   upload(c.jpolar_o, s.jpolar, st);
+  c.chgpen = s.chgpen;
+  if (c.chgpen) {
+    std::vector<double> cp(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      cp[3 * i] = s.pcore[i];
+      cp[3 * i + 1] = s.pval[i];
+      cp[3 * i + 2] = s.palpha[i];
+    }
+    upload(c.chgpen_o, cp, st);
+    T9_HIP(hipStreamSynchronize(st));  // cp is a local host buffer
+  }
   c.nj = s.njpolar;
   c.uniform_thole = !s.thlval.empty();
   c.thole_value = s.thlval.empty() ? 0.0 : s.thlval[0];
@@ -682,9 +693,18 @@ int t9_get_rpole(t9_ctx* ctx, double* rpole) {
   }, T9_ERR_HIP);
 }
 
+namespace {
+// HIPPO contexts carry the charge-penetration multipoles only: their
+// polarization (damped direct and mutual fields) is not built
+void reject_chgpen_polar(const t9::Ctx& c) {
+  if (c.chgpen) throw t9::ArgError("HIPPO polarization is not built (charge-penetration multipoles only)");
+}
+}  // namespace
+
 int t9_dfield(t9_ctx* ctx, double* field, double* fieldp) {
   return guarded([&] {
     auto& c = ctx->c;
+    reject_chgpen_polar(c);
     t9::prepare(c);
     t9::dfield_all(c);
     if (field) t9::gather_to_host(c, c.field.p, field);
@@ -696,6 +716,7 @@ int t9_dfield(t9_ctx* ctx, double* field, double* fieldp) {
 int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field, double* fieldp) {
   return guarded([&] {
     auto& c = ctx->c;
+    reject_chgpen_polar(c);
     if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
     t9::prepare(c);
     // use the CG scratch vectors: zrsd/zrsdp hold the inputs, vec/vecp the fields
@@ -711,6 +732,7 @@ int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field
 int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats) {
   return guarded([&] {
     auto& c = ctx->c;
+    reject_chgpen_polar(c);
     t9::prepare(c);
     int iters = 0;
     double rms = 0;
@@ -895,6 +917,7 @@ int t9_pme_grid_mpole(t9_ctx* ctx, double* grid) {
 int t9_pme_grid_uind(t9_ctx* ctx, const double* uind, const double* uinp, double* grid) {
   return guarded([&] {
     auto& c = ctx->c;
+    reject_chgpen_polar(c);
     if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
     pme_prepare(c);
     t9::scatter_from_host(c, uind, c.zrsd.p);

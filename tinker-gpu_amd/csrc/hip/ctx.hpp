@@ -140,6 +140,11 @@ struct Ctx {
   DBuf pole;                     // [n][10] local frame
   DBuf frame, zaxis, xaxis, yaxis, ysign;  // int [n]
   DBuf polarity_o, pdamp_o;      // double [n]
+  // HIPPO charge penetration (caller order, double [n][3]: core charge,
+  // valence charge, valence exponent); chgpen selects the damped multipole
+  // pair kernel
+  bool chgpen = false;
+  DBuf chgpen_o;
   DBuf jpolar_o;                 // int [n]
   DBuf excl_off, excl_j, excl_code;  // CSR of scaled partners (both directions)
   DBuf thlval;                   // real [nj*nj]

@@ -368,6 +368,170 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A
   block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
 }
 
+// ------------------------------------------------------------ HIPPO
+
+// Charge penetration (HIPPO; replaces the multipole pair of
+// src/cu/hippo/empoleChgpen_cu1.cc with include/seq/pair_mpole_chgpen.h and
+// include/seq/damp_hippo.h).  A monopole is a point core charge Z plus a
+// valence charge q - Z spread as a Slater density of exponent alpha, which
+// also carries the dipole and quadrupole.  A pair is four charge-distribution
+// pairs, each the plain multipole contraction with its own radial series:
+// core-core undamped, core of i in the valence of k (k's damping), valence of
+// i at the core of k (i's damping), valence-valence; scale factor s and Ewald
+// enter as B_n = b_n - (1 - s lambda_n) rr_n (s lambda_n rr_n without Ewald).
+// Only the valence parts have orientation, so only they give torque terms.
+//
+// Damping factors: lambda_(2n+1) = 1 - sum_j e^-x_j P_nj(x_j) / (2n-1)!!,
+// x_j = alpha_j r, where P_0 is the leading polynomial of the kernel
+// (core-valence 1 + x/2; valence-valence with equal exponents 1 + 11x/16 +
+// 3x^2/16 + x^3/48; with exponents ai != ak the two parts
+// ti^2 (1 + 2 tk + x/2) and tk^2 (1 + 2 ti + x/2), ti = ak^2 / (ak^2 - ai^2),
+// tk = 1 - ti) and P_(n+1) = (2n+1) P_n - x P_n' + x P_n is the derivative
+// series B_(n+1) = -(1/r) dB_n/dr written on the coefficients.  Exponents
+// closer than 1e-3 use their mean (the two-part form cancels there).
+// Evaluated in FP64 whatever the working precision.
+template <int D0>
+__device__ __forceinline__ void chgpen_terms(double x, const double* d0, double* t) {
+  double d[D0 + 7];
+#pragma unroll
+  for (int k = 0; k < D0 + 7; ++k) d[k] = k <= D0 ? d0[k] : 0.0;
+  const double ex = exp(-x);
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    double p = 0;
+#pragma unroll
+    for (int k = D0 + n; k >= 0; --k) p = fma(p, x, d[k]);
+    t[n] = ex * p;
+    const double m = 2 * n + 1;
+#pragma unroll
+    for (int k = D0 + n + 1; k >= 0; --k) d[k] = (k <= D0 + n ? (m - k) * d[k] : 0.0) + (k > 0 ? d[k - 1] : 0.0);
+  }
+}
+
+__device__ __forceinline__ void chgpen_lambda(double r, double ai, double ak, double* lci, double* lck,
+                                              double* lik) {
+  const double dfac[6] = {1, 1, 3, 15, 105, 945};
+  const double cv[2] = {1.0, 0.5};
+  double ta[6], tb[6];
+  chgpen_terms<1>(ai * r, cv, ta);
+  chgpen_terms<1>(ak * r, cv, tb);
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    lci[n] = 1 - ta[n] / dfac[n];
+    lck[n] = 1 - tb[n] / dfac[n];
+  }
+  if (fabs(ai - ak) < 1e-3) {
+    const double vv[4] = {1.0, 11.0 / 16.0, 3.0 / 16.0, 1.0 / 48.0};
+    chgpen_terms<3>(0.5 * (ai + ak) * r, vv, ta);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) lik[n] = 1 - ta[n] / dfac[n];
+  } else {
+    const double ti = ak * ak / ((ak - ai) * (ak + ai)), tk = 1 - ti;
+    const double pa[2] = {ti * ti * (1 + 2 * tk), 0.5 * ti * ti}, pb[2] = {tk * tk * (1 + 2 * ti), 0.5 * tk * tk};
+    chgpen_terms<1>(ai * r, pa, ta);
+    chgpen_terms<1>(ak * r, pb, tb);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) lik[n] = 1 - (ta[n] + tb[n]) / dfac[n];
+  }
+}
+
+// kEmplarParts waves per 64 atoms over the in-cutoff list including the
+// scaled pairs (the entry's class gives the multipole scale), both sides of
+// every pair, FP64 arithmetic; the outputs of k_emplar (lane-owned gradient
+// and torque terms, half pair energy and virial).
+template <class real, int EWALD>
+__global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<real> A,
+                                                                     const double* __restrict__ cp) {
+  __shared__ double sh[parts_lds<kEmplarParts, 19>()];
+  const int part = threadIdx.x >> 6;
+  const int blk = xcd_block();
+  const int s = blk * 64 + (threadIdx.x & 63);
+  Acc<double> acc;
+  acc.zero();
+  if (s < A.n) {
+    const PosRec<real> pi = A.pos[s];
+    const int io = A.perm[s];
+    const double alpha_i = cp[3 * io + 2];
+    Mp<double> ci = {}, vi;
+    ci.q = cp[3 * io];
+    vi.q = cp[3 * io + 1];
+    for (int t = 0; t < 3; ++t) vi.d[t] = A.rpole[10 * (size_t)s + 1 + t];
+    for (int t = 0; t < 6; ++t) vi.Q[t] = A.rpole[10 * (size_t)s + 4 + t];
+    const int len = A.ncut[s];
+    const uint32_t* e = A.ecut + A.woff[s >> 6] + (s & 63);
+    const double z3[3] = {0, 0, 0};
+    for (int kk = part; kk < len; kk += kEmplarParts) {
+      const uint32_t ent = e[(long long)kk * 64];
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const int j = (int)(ent >> kCodeBits);
+      const double sm = code ? (double)A.sc.m[code] : 1.0;
+      double R[3];
+      pair_disp64(A.pos[j], pi, A.box, R);  // r_i - r_k
+      const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
+      double rr[6], bn[6] = {0, 0, 0, 0, 0, 0};
+      rr[0] = rinv;
+#pragma unroll
+      for (int q = 1; q < 6; ++q) rr[q] = rr[q - 1] * (double)(2 * q - 1) * rr2;
+      if (EWALD) {
+        double ex;
+        bn[0] = t9_erfc_pair(A.ek64.a * r, ex) * rinv;
+#pragma unroll
+        for (int q = 1; q < 6; ++q) bn[q] = ((double)(2 * q - 1) * bn[q - 1] + A.ek64.e[q] * ex) * rr2;
+      }
+      const int jo = A.perm[j];
+      double lci[6], lck[6], lik[6];
+      chgpen_lambda(r, alpha_i, cp[3 * jo + 2], lci, lck, lik);
+      double Bcc[6], Bi[6], Bk[6], Bik[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const double b = EWALD ? bn[q] - rr[q] : 0.0;  // Ewald part less the bare kernel
+        Bcc[q] = b + sm * rr[q];
+        Bi[q] = b + sm * lci[q] * rr[q];
+        Bk[q] = b + sm * lck[q] * rr[q];
+        Bik[q] = b + sm * lik[q] * rr[q];
+      }
+      Mp<double> ck = {}, vk;
+      ck.q = cp[3 * jo];
+      vk.q = cp[3 * jo + 1];
+      for (int t = 0; t < 3; ++t) vk.d[t] = A.rpole[10 * (size_t)j + 1 + t];
+      for (int t = 0; t < 6; ++t) vk.Q[t] = A.rpole[10 * (size_t)j + 4 + t];
+      PairOut<double> o;
+      // core of i: energy and gradient only (a point charge has no torque)
+      pair_terms<double, true, false>(R, Bcc, nullptr, ci, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
+      for (int t = 0; t < 3; ++t) o.g[t] = 0;
+      for (int t = 0; t < 6; ++t) o.h[t] = 0;
+      acc.add(o, R);
+      pair_terms<double, true, false>(R, Bk, nullptr, ci, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
+      for (int t = 0; t < 3; ++t) o.g[t] = 0;
+      for (int t = 0; t < 6; ++t) o.h[t] = 0;
+      acc.add(o, R);
+      // valence of i: also the potential derivatives that turn into its torque
+      pair_terms<double, true, false>(R, Bi, nullptr, vi, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
+      acc.add(o, R);
+      pair_terms<double, true, false>(R, Bik, nullptr, vi, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
+      acc.add(o, R);
+    }
+  }
+  double av[19];
+  acc.pack(av);
+  parts_reduce<kEmplarParts, 19>(av, sh);
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  if (part == 0) {
+    acc.unpack(av);
+    if (s < A.n) {
+      const double f = kCoul;
+      for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] = f * acc.gr[k];
+      for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] = f * acc.tg[k];
+      for (int k = 0; k < 6; ++k) A.tq[9 * (size_t)s + 3 + k] = f * acc.th[k];
+    }
+    red[EP_MREAL] = 0.5 * kCoul * acc.e;
+    for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * kCoul * acc.v[k];
+  }
+  block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
+}
+
 // ------------------------------------------------------------ PME
 
 struct GridGeom {
@@ -974,6 +1138,14 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
     kscaled = mode == 1 ? k_emplar_scaled<real, 0, 1>
                         : (mode == 2 ? k_emplar_scaled<real, 0, 2> : k_emplar_scaled<real, 0, 3>);
   A.rows = energy_rows(c, c.nwave);
+  if (c.chgpen) {  // HIPPO multipoles: every pair, scaled ones included, in one kernel
+    if (use_p) throw ArgError("HIPPO polarization is not built");
+    KScope ks_(c, KT_EMPLAR);
+    void (*kc)(EmplarArgs<real>, const double*) = c.ewald ? k_empole_chgpen<real, 1> : k_empole_chgpen<real, 0>;
+    hipLaunchKernelGGL(kc, dim3(c.nwave), dim3(64 * kEmplarParts), 0, c.stream, A, c.chgpen_o.as<double>());
+    T9_LAUNCH_CHECK();
+    return;
+  }
   {
     KScope ks_(c, KT_EMPLAR);
     hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kEmplarParts), 0, c.stream, A);
