@@ -612,6 +612,77 @@ void recip_dipole(const Oracle& o, const double* ud, const double* up, double* f
     }
 }
 
+// ------------------------------------------------------------ HIPPO damping
+
+// Charge penetration (HIPPO; reference: include/seq/pair_mpole_chgpen.h,
+// include/seq/damp_hippo.h).  A monopole is a point core charge Z plus a
+// valence charge q - Z spread as a Slater density (alpha^3 / 8 pi)
+// exp(-alpha r) that also carries the dipole and quadrupole.  The damped
+// Coulomb kernels are, with a = alpha_i r, b = alpha_k r,
+//   core-valence      f0 = [1 - (1 + a/2) e^-a] / r,
+//   valence-valence   f0 = [1 - (1 + 11a/16 + 3a^2/16 + a^3/48) e^-a] / r
+//                          (alpha_i = alpha_k),
+//                     f0 = [1 - ti^2 (1 + 2 tk + a/2) e^-a
+//                             - tk^2 (1 + 2 ti + b/2) e^-b] / r,
+//                     ti = alpha_k^2 / (alpha_k^2 - alpha_i^2), tk = 1 - ti,
+// (the electrostatic energy of a point charge, resp. a second density, in a
+// Slater density).  Every higher kernel follows from the derivative series
+// f_(n+1) = -(1/r) df_n/dr, computed here exactly on the polynomial
+// coefficients: f_n = [(2n-1)!! - sum_j P_nj(r) exp(-alpha_j r)] / r^(2n+1)
+// with P_(n+1),j = m P - r P' + alpha_j r P (m = 2n + 1), so
+// lambda_(2n+1) = 1 - sum_j P_nj(r) exp(-alpha_j r) / (2n-1)!!.  Exponents
+// closer than 1e-3 are replaced by their mean (the rule of Tinker's damppole:
+// the two-exponent form cancels there).
+struct ExpPart {
+  double alpha;
+  double c[16];
+  int deg;
+};
+
+void damp_series(double r, int nj, ExpPart* parts, int nmax, double* lam) {
+  double A = 1;
+  for (int n = 0; n <= nmax; ++n) {
+    double s = 0;
+    for (int j = 0; j < nj; ++j) {
+      double p = 0, rk = 1;
+      for (int k = 0; k <= parts[j].deg; ++k) {
+        p += parts[j].c[k] * rk;
+        rk *= r;
+      }
+      s += p * std::exp(-parts[j].alpha * r);
+    }
+    lam[n] = 1 - s / A;
+    const int m = 2 * n + 1;
+    for (int j = 0; j < nj; ++j) {
+      ExpPart& q = parts[j];
+      for (int k = q.deg + 1; k >= 0; --k)
+        q.c[k] = (k <= q.deg ? (m - k) * q.c[k] : 0.0) + (k > 0 ? q.alpha * q.c[k - 1] : 0.0);
+      q.deg += 1;
+    }
+    A *= m;
+  }
+}
+
+// lam[0..5] = lambda_1 .. lambda_11 of the core-valence kernel (exponent a)
+void damp_core_valence(double r, double a, double* lam) {
+  ExpPart p{a, {1.0, 0.5 * a}, 1};
+  damp_series(r, 1, &p, 5, lam);
+}
+
+// lam[0..5] of the valence-valence kernel (exponents ai, ak)
+void damp_valence_valence(double r, double ai, double ak, double* lam) {
+  if (std::fabs(ai - ak) < 1e-3) {
+    const double a = 0.5 * (ai + ak);
+    ExpPart p{a, {1.0, 11.0 / 16.0 * a, 3.0 / 16.0 * a * a, a * a * a / 48.0}, 3};
+    damp_series(r, 1, &p, 5, lam);
+    return;
+  }
+  const double ti = ak * ak / (ak * ak - ai * ai), tk = 1 - ti;
+  ExpPart p[2] = {{ai, {ti * ti * (1 + 2 * tk), ti * ti * 0.5 * ai}, 1},
+                  {ak, {tk * tk * (1 + 2 * ti), tk * tk * 0.5 * ak}, 1}};
+  damp_series(r, 2, p, 5, lam);
+}
+
 // ------------------------------------------------------------------ fields
 
 double pair_thole(const Oracle& o, int i, int k) { return o.thlval[o.jpolar[i] * o.njpolar + o.jpolar[k]]; }
@@ -637,24 +708,40 @@ void dfield(const Oracle& o, double* fd, double* fp) {
     double qky = pk[7] * xr + pk[5] * yr + pk[9] * zr;
     double qkz = pk[8] * xr + pk[9] * yr + pk[6] * zr;
     double qkr = qkx * xr + qky * yr + qkz * zr;
-    auto add = [&](double sc_, double* out) {
+    // field of charge q, dipole and quadrupole (m = 1) or of q alone (m = 0)
+    // with damping factors l3, l5, l7
+    auto add = [&](double sc_, double* out, double q, double m, double l3, double l5, double l7) {
       double b1, b2, b3;
       if (o.ewald) {
-        b1 = bn[1] - (1 - sc_ * s3) * rr3;
-        b2 = bn[2] - (1 - sc_ * s5) * rr5;
-        b3 = bn[3] - (1 - sc_ * s7) * rr7;
+        b1 = bn[1] - (1 - sc_ * l3) * rr3;
+        b2 = bn[2] - (1 - sc_ * l5) * rr5;
+        b3 = bn[3] - (1 - sc_ * l7) * rr7;
       } else {
-        b1 = sc_ * s3 * rr3;
-        b2 = sc_ * s5 * rr5;
-        b3 = sc_ * s7 * rr7;
+        b1 = sc_ * l3 * rr3;
+        b2 = sc_ * l5 * rr5;
+        b3 = sc_ * l7 * rr7;
       }
-      double c1 = -(b1 * pk[0] - b2 * dkr + b3 * qkr);
-      out[3 * i + 0] += c1 * xr - b1 * pk[1] + 2 * b2 * qkx;
-      out[3 * i + 1] += c1 * yr - b1 * pk[2] + 2 * b2 * qky;
-      out[3 * i + 2] += c1 * zr - b1 * pk[3] + 2 * b2 * qkz;
+      double c1 = -(b1 * q - m * (b2 * dkr - b3 * qkr));
+      out[3 * i + 0] += c1 * xr - m * (b1 * pk[1] - 2 * b2 * qkx);
+      out[3 * i + 1] += c1 * yr - m * (b1 * pk[2] - 2 * b2 * qky);
+      out[3 * i + 2] += c1 * zr - m * (b1 * pk[3] - 2 * b2 * qkz);
     };
-    add(dsc, fd);
-    add(psc, fp);
+    if (o.chgpen) {
+      // HIPPO: the induced dipole is a point probe; k's core charge acts
+      // undamped, k's valence density (charge, dipole, quadrupole) with its
+      // core-valence damping
+      double lck[6];
+      damp_core_valence(r, o.palpha[k], lck);
+      for (int t = 0; t < 2; ++t) {
+        const double sc_ = t == 0 ? dsc : psc;
+        double* out = t == 0 ? fd : fp;
+        add(sc_, out, o.pcore[k], 0.0, 1.0, 1.0, 1.0);
+        add(sc_, out, o.pval[k], 1.0, lck[1], lck[2], lck[3]);
+      }
+      return;
+    }
+    add(dsc, fd, pk[0], 1.0, s3, s5, s7);
+    add(psc, fp, pk[0], 1.0, s3, s5, s7);
   });
   if (o.ewald) {
     std::vector<double> rf(3 * n);
@@ -683,6 +770,12 @@ void ufield(const Oracle& o, const double* ud, const double* up, double* fd, dou
     (void)s7;
     const Scale* sc = find_scale(o, i, k);
     double usc = sc ? sc->u : 1.0;
+    if (o.chgpen) {  // HIPPO: valence-valence damping of the dipole-dipole tensor
+      double lik[6];
+      damp_valence_valence(r, o.palpha[i], o.palpha[k], lik);
+      s3 = lik[1];
+      s5 = lik[2];
+    }
     double b1, b2;
     if (o.ewald) {
       double bn[3];

@@ -19,8 +19,9 @@ What it writes (numbers and data files only, no reference source text):
     src/cu/upredict.cu:50-65 (16 numbers)
   * tests/golden/nacl_vdw_ref.json — test/nacl.cpp NaCl-1 vdW energies,
     gradients, virials and the ion separations of test/file/nacl/
-  * tests/golden/c5h12acnh2.xyz, hippo19_subset.prm, hippo_ref.json — the
-    HIPPO multipole test (test/emhippo.cpp, test/ref/emhippo.{1,2}.txt)
+  * tests/golden/c5h12acnh2.xyz, dmso.xyz, hippo19_subset.prm, hippo_ref.json —
+    the HIPPO multipole and polarization tests (test/emhippo.cpp,
+    test/ephippo.cpp, test/ref/emhippo.{1,2}.txt, ephippo.{1,2}.txt)
   * tinker-gpu_amd/params/amoeba09.prm — the subset of
     test/file/commit_6fe8e913/amoeba09.prm this path uses: the scale-factor
     header and the atom/vdw/bond/angle/ureybrad/multipole/polarize records of
@@ -315,17 +316,21 @@ def write_more():
 
 
 def write_hippo():
-    """HIPPO charge-penetration multipoles (test/emhippo.cpp): the coordinates
-    of test/file/c5h12acnh2/c5h12acnh2.xyz, the parameter header and the atom,
-    multipole and chgpen records of its types from test/file/hippo/hippo19.prm,
-    and the energy, count, virial and gradient of test/ref/emhippo.{1,2}.txt
-    (Ewald at 7 Å and non-Ewald at 9 Å in a 35 x 30 x 25 Å box)."""
-    copy_file("test/file/c5h12acnh2/c5h12acnh2.xyz", "c5h12acnh2.xyz")
-    xyz = open(os.path.join(REF, "test/file/c5h12acnh2/c5h12acnh2.xyz")).read().splitlines()
-    types = {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5}
+    """HIPPO charge-penetration multipoles and polarization (test/emhippo.cpp,
+    test/ephippo.cpp): the coordinates of test/file/c5h12acnh2/c5h12acnh2.xyz
+    and test/file/dmso/dmso.xyz, the parameter header and the atom, multipole,
+    chgpen and polarize records of their types from
+    test/file/hippo/hippo19.prm, and the energy, count, virial and gradient of
+    test/ref/emhippo.{1,2}.txt and ephippo.{1,2}.txt (Ewald at 7 Å and
+    non-Ewald at 9 Å)."""
+    types = set()
+    for rel, dst in (("test/file/c5h12acnh2/c5h12acnh2.xyz", "c5h12acnh2.xyz"), ("test/file/dmso/dmso.xyz", "dmso.xyz")):
+        copy_file(rel, dst)
+        xyz = open(os.path.join(REF, rel)).read().splitlines()
+        types |= {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5 and ln.split()[0].isdigit()}
     lines = open(os.path.join(REF, "test/file/hippo/hippo19.prm")).read().splitlines()
     classes = {int(ln.split()[2]) for ln in lines if ln.split()[:1] == ["atom"] and int(ln.split()[1]) in types}
-    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole and chgpen records of the c5h12acnh2 test system", ""]
+    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole, chgpen and polarize records of the c5h12acnh2 and dmso test systems", ""]
     i, header = 0, True
     while i < len(lines):
         t = lines[i].split()
@@ -342,10 +347,14 @@ def write_hippo():
             continue
         elif t and t[0] == "chgpen" and int(t[1]) in classes:
             out.append(lines[i])
+        elif t and t[0] == "polarize" and int(t[1]) in types:
+            out.append(lines[i])
         i += 1
     open(os.path.join(HERE, "hippo19_subset.prm"), "w").write("\n".join(out) + "\n")
     ref = {"emhippo_ewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.1.txt")),
-           "emhippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.2.txt"))}
+           "emhippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.2.txt")),
+           "ephippo_ewald": parse_ref_txt(os.path.join(REF, "test/ref/ephippo.1.txt")),
+           "ephippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/ephippo.2.txt"))}
     json.dump(ref, open(os.path.join(HERE, "hippo_ref.json"), "w"), indent=0)
     return sorted(ref)
 

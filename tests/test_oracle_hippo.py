@@ -99,3 +99,35 @@ def test_two_exponent_form_meets_equal_form():
         _, _, two = oracle.damp_chgpen(r, a - 1.5e-3, a + 1.5e-3)
         _, _, one = oracle.damp_chgpen(r, a - 4e-4, a + 4e-4)
         assert np.abs(two - one).max() < 1e-6
+
+
+HIPPO_POLAR_EWALD_KEY = "neighbor-list\na-axis 25.0\nb-axis 25.0\nc-axis 30.0\npolarizeterm only\newald-cutoff 7.0\newald\n"
+HIPPO_POLAR_NONEWALD_KEY = "neighbor-list\na-axis 25.0\nb-axis 25.0\nc-axis 30.0\npolarizeterm only\nmpole-cutoff 9.0\n"
+
+
+def hippo_polar_system(golden_dir, ewald):
+    return t9hip.System.from_files(os.path.join(golden_dir, "dmso.xyz"),
+                                   prm=os.path.join(golden_dir, "hippo19_subset.prm"),
+                                   key=HIPPO_POLAR_EWALD_KEY if ewald else HIPPO_POLAR_NONEWALD_KEY)
+
+
+@pytest.mark.parametrize("ewald", [True, False])
+def test_ephippo_goldens(golden_dir, ewald):
+    """test/ephippo.cpp: 50 dimethyl sulfoxide molecules (150 atoms,
+    25 x 25 x 30 Å): HIPPO mutual polarization (direct field with core /
+    valence damping and the polar-1x scales, mutual field with the
+    valence-valence damping and the induce-1x scales, one dipole set)."""
+    ref = json.load(open(os.path.join(golden_dir, "hippo_ref.json")))["ephippo_ewald" if ewald else "ephippo_nonewald"]
+    a = hippo_polar_system(golden_dir, ewald).arrays()
+    o = oracle.Oracle(a)
+    ud, up, it, rms, rc = o.induce()
+    assert rc == 0 and rms < 1e-6
+    assert np.abs(ud - up).max() < 1e-14  # the two dipole sets of the solver coincide
+    e, g, v = o.energy(ud, up, mpole=False, polar=True)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    print(f"ewald={ewald}: E={e['ep']:.6f} ref={ref['energy']} it={it} max|dg|={np.abs(g - rg).max():.2e} "
+          f"max|dv|={np.abs(v - rv).max():.2e}")
+    assert abs(e["ep"] - ref["energy"]) < 1e-4
+    assert np.abs(g - rg).max() < 1e-4
+    assert np.abs(v - rv).max() < 1e-3
+    assert oracle.interaction_counts(a)[1] == ref["count"]

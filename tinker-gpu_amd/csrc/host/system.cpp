@@ -133,6 +133,12 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
     auto tok = split(lines[li]);
     if (tok.empty()) continue;
     std::string key = lower(tok[0]);
+    // a parameter record starts with its keyword and an integer; other lines
+    // that start with a record keyword (the text of a file's header) are not
+    // records, and Tinker skips them too
+    static const std::set<std::string> kRecords = {"atom", "vdw", "bond", "angle", "ureybrad", "multipole",
+                                                    "polarize", "polpair", "chgpen"};
+    if (kRecords.count(key) && (tok.size() < 2 || !is_int(tok[1]))) continue;
     auto scale_kw = [&](const char* prefix, double* arr, int lo, int hi, const char* suffix) -> bool {
       for (int j = lo; j <= hi; ++j) {
         std::string kw = std::string(prefix) + std::to_string(j == 1 ? 11 : 10 + j) + suffix;
@@ -148,7 +154,8 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
     } else if (scale_kw("mpole-", ff.m_scale, 2, 5, "-scale") ||
                scale_kw("polar-", ff.p_scale, 2, 5, "-scale") ||
                scale_kw("polar-", ff.p_iscale, 2, 5, "-intra") ||
-               scale_kw("vdw-", ff.v_scale, 2, 5, "-scale")) {
+               scale_kw("vdw-", ff.v_scale, 2, 5, "-scale") ||
+               scale_kw("induce-", ff.w_scale, 2, 5, "-scale")) {
     } else if (key.rfind("direct-1", 0) == 0 && key.size() == 15 && tok.size() >= 2) {
       int j = key[8] - '0';  // direct-1j-scale
       if (j >= 1 && j <= 4) ff.d_scale[j] = to_d(tok[1]);
@@ -612,18 +619,13 @@ void assign_polarize(System& s) {
 }
 
 // HIPPO charge penetration: core charge and alpha of the atom's class, the
-// valence charge is the rest of the monopole.  The damped multipole kernel is
-// built for the permanent multipoles only: HIPPO polarization (its damped
-// direct and mutual fields) is not, and is rejected.
+// valence charge is the rest of the monopole.
 void assign_chgpen(System& s) {
-  s.chgpen = !s.ff.chgpen.empty() && s.opt.use_mpole;
+  s.chgpen = !s.ff.chgpen.empty() && (s.opt.use_mpole || s.opt.use_polar);
   s.pcore.clear();
   s.pval.clear();
   s.palpha.clear();
   if (!s.chgpen) return;
-  if (s.opt.use_polar)
-    throw std::runtime_error("HIPPO polarization is not built: switch it off (polarizeterm none) to use the "
-                             "charge-penetration multipoles");
   const int n = s.n;
   s.pcore.assign(n, 0.0);
   s.pval.assign(n, 0.0);
@@ -656,7 +658,18 @@ void build_scales(System& s) {
         if (ff.m_scale[b] != 1) get(k).m = ff.m_scale[b];
         double pv = contains(s.ip11[i], k) ? ff.p_iscale[b] : ff.p_scale[b];
         if (ff.p_scale[b] != 1 || ff.p_iscale[b] != 1) get(k).p = pv;
+        if (s.chgpen) {  // HIPPO: d = p (bond-based, intra-group variant), u = induce-1x
+          if (ff.p_scale[b] != 1 || ff.p_iscale[b] != 1) get(k).d = pv;
+          if (ff.w_scale[b] != 1) get(k).u = ff.w_scale[b];
+        }
       }
+    if (s.chgpen) {
+      for (auto& kv : row) {
+        const ScalePair& p = kv.second;
+        if (p.m != 1 || p.d != 1 || p.p != 1 || p.u != 1) s.scales.push_back(p);
+      }
+      continue;
+    }
     const std::vector<int>* grp[5] = {nullptr, &s.ip11[i], &s.ip12[i], &s.ip13[i], &s.ip14[i]};
     for (int b = 1; b <= 4; ++b)
       for (int k : *grp[b]) {

@@ -71,6 +71,7 @@ struct ForceField {
   double d_scale[5] = {0, 0, 1, 1, 1};          // index 1..4 = 1-1..1-4 groups
   double u_scale[5] = {0, 1, 1, 1, 1};
   double v_scale[6] = {0, 0, 0, 0, 1, 1};
+  double w_scale[6] = {1, 1, 1, 1, 1, 1};       // induce-1x-scale (HIPPO mutual field)
   std::map<int, int> type_class;
   std::map<int, std::string> type_name;
   std::map<int, double> type_mass;
@@ -117,7 +118,11 @@ struct Options {
   bool mutual = true;
 };
 
-// One pair with at least one scale factor different from 1.
+// One pair with at least one scale factor different from 1.  With HIPPO
+// charge penetration the four slots hold the multipole scale (m), the
+// polar-1x scale with its intra-group variant for the direct field and the
+// induced-dipole / permanent-multipole terms (d = p), and the induce-1x scale
+// of the mutual field (u), as the reference's mdw exclusion lists.
 struct ScalePair {
   int i, k;  // i < k, 0-based original indices
   double m, d, p, u;
