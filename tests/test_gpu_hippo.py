@@ -1,4 +1,4 @@
-"""HIPPO charge-penetration multipoles on the GPU through the C ABI.
+"""HIPPO charge-penetration multipoles and polarization on the GPU through the C ABI.
 
 test/emhippo.cpp: pentane + acetamide (104 atoms, HIPPO-2019) in a
 35 x 30 x 25 Å box, Ewald at 7 Å (PME) and non-Ewald at 9 Å.  Against the
@@ -7,7 +7,9 @@ tolerances — FP64: energy 1e-4, gradient 1e-4 (+5e-5, the rounding of the
 printed values), virial 1e-3; FP32: energy 1.6e-3 (Ewald) / 5e-3 (non-Ewald),
 gradient 3e-4, virial 2e-3 / 2.5e-3 — and against the FP64 oracle (FP64:
 1e-10 relative energy, 1e-8 gradient and virial), plus the analyze-mode
-interaction count and the rejection of HIPPO polarization calls."""
+interaction count; and test/ephippo.cpp (HIPPO mutual polarization of 50
+DMSO molecules, Ewald and non-Ewald) against its goldens and the oracle's
+induced dipoles."""
 import json
 import os
 
@@ -47,9 +49,32 @@ def test_emhippo_gpu(golden_dir, ewald, prec):
         assert np.abs(v - vo).max() < 1e-8
 
 
-def test_hippo_polarization_rejected(golden_dir):
-    print("case hippo polarization rejected", flush=True)
-    ctx = t9hip.Context(hippo_system(golden_dir, True), t9hip.T9_FP64)
-    with pytest.raises(t9hip.T9Error) as ei:
-        ctx.induce()
-    assert ei.value.code == t9hip.T9_ERR_ARG
+TOLP = {t9hip.T9_FP64: (1e-4, 1.5e-4, 1e-3), t9hip.T9_FP32: (1e-4, 1.5e-4, 1e-3)}
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+@pytest.mark.parametrize("ewald", [True, False])
+def test_ephippo_gpu(golden_dir, ewald, prec):
+    """test/ephippo.cpp (50 DMSO molecules, HIPPO mutual polarization): the
+    reference's tolerances (1e-4 energy and gradient, 1e-3 virial, both
+    precisions), the interaction count, and the induced dipoles against the
+    oracle's (RMS below 1e-5 D: both solves stop at 1e-6 D)."""
+    print(f"case ephippo ewald={ewald} prec={prec}", flush=True)
+    from test_oracle_hippo import hippo_polar_system
+    ref = json.load(open(os.path.join(golden_dir, "hippo_ref.json")))["ephippo_ewald" if ewald else "ephippo_nonewald"]
+    s = hippo_polar_system(golden_dir, ewald)
+    ctx = t9hip.Context(s, prec)
+    ud, up, st = ctx.induce()
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    de, dg, dv = abs(e["esum"] - ref["energy"]), np.abs(g - rg).max(), np.abs(v - rv).max()
+    o = oracle.Oracle(s.arrays())
+    oud, oup, it, rms, rc = o.induce()
+    du = float(np.sqrt(np.mean((ud - oud) ** 2)) * 4.803206802)
+    print(f"  E={e['esum']:.6f} ref={ref['energy']} |dE|={de:.2e} max|dg|={dg:.2e} max|dv|={dv:.2e} "
+          f"iters={st.iterations} (oracle {it}) dipole RMS vs oracle {du:.2e} D")
+    te, tg, tv = TOLP[prec]
+    assert de < te and dg < tg and dv < tv
+    assert du < 1e-5
+    ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert ea["nep"] == ref["count"]

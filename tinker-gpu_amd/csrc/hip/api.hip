@@ -693,18 +693,9 @@ int t9_get_rpole(t9_ctx* ctx, double* rpole) {
   }, T9_ERR_HIP);
 }
 
-namespace {
-// HIPPO contexts carry the charge-penetration multipoles only: their
-// polarization (damped direct and mutual fields) is not built
-void reject_chgpen_polar(const t9::Ctx& c) {
-  if (c.chgpen) throw t9::ArgError("HIPPO polarization is not built (charge-penetration multipoles only)");
-}
-}  // namespace
-
 int t9_dfield(t9_ctx* ctx, double* field, double* fieldp) {
   return guarded([&] {
     auto& c = ctx->c;
-    reject_chgpen_polar(c);
     t9::prepare(c);
     t9::dfield_all(c);
     if (field) t9::gather_to_host(c, c.field.p, field);
@@ -716,7 +707,6 @@ int t9_dfield(t9_ctx* ctx, double* field, double* fieldp) {
 int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field, double* fieldp) {
   return guarded([&] {
     auto& c = ctx->c;
-    reject_chgpen_polar(c);
     if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
     t9::prepare(c);
     // use the CG scratch vectors: zrsd/zrsdp hold the inputs, vec/vecp the fields
@@ -732,7 +722,6 @@ int t9_ufield(t9_ctx* ctx, const double* uind, const double* uinp, double* field
 int t9_induce(t9_ctx* ctx, double* uind, double* uinp, t9_induce_stats* stats) {
   return guarded([&] {
     auto& c = ctx->c;
-    reject_chgpen_polar(c);
     t9::prepare(c);
     int iters = 0;
     double rms = 0;
@@ -917,7 +906,6 @@ int t9_pme_grid_mpole(t9_ctx* ctx, double* grid) {
 int t9_pme_grid_uind(t9_ctx* ctx, const double* uind, const double* uinp, double* grid) {
   return guarded([&] {
     auto& c = ctx->c;
-    reject_chgpen_polar(c);
     if (!uind || !uinp) throw t9::ArgError("uind and uinp are required");
     pme_prepare(c);
     t9::scatter_from_host(c, uind, c.zrsd.p);

@@ -39,6 +39,7 @@
 #include <algorithm>
 
 #include "bspline.hpp"
+#include "chgpen.hpp"
 #include "ctx.hpp"
 
 namespace t9 {
@@ -380,68 +381,25 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_emplar(EmplarArgs<real> A
 // i at the core of k (i's damping), valence-valence; scale factor s and Ewald
 // enter as B_n = b_n - (1 - s lambda_n) rr_n (s lambda_n rr_n without Ewald).
 // Only the valence parts have orientation, so only they give torque terms.
+// Polarization (src/cu/hippo/epolarChgpen_cu1.cc, pair_polar_chgpen.h): an
+// induced dipole is a point probe — u_i sees the core of k undamped and the
+// valence multipole of k with k's damping, the multipole of i is seen by u_k
+// with i's damping (p scale, which HIPPO also uses as the d scale), and the
+// dipole-dipole term has the valence-valence damping (u scale); the terms
+// are those of k_emplar's polarization gradient with these kernels.
 //
-// Damping factors: lambda_(2n+1) = 1 - sum_j e^-x_j P_nj(x_j) / (2n-1)!!,
-// x_j = alpha_j r, where P_0 is the leading polynomial of the kernel
-// (core-valence 1 + x/2; valence-valence with equal exponents 1 + 11x/16 +
-// 3x^2/16 + x^3/48; with exponents ai != ak the two parts
-// ti^2 (1 + 2 tk + x/2) and tk^2 (1 + 2 ti + x/2), ti = ak^2 / (ak^2 - ai^2),
-// tk = 1 - ti) and P_(n+1) = (2n+1) P_n - x P_n' + x P_n is the derivative
-// series B_(n+1) = -(1/r) dB_n/dr written on the coefficients.  Exponents
-// closer than 1e-3 use their mean (the two-part form cancels there).
-// Evaluated in FP64 whatever the working precision.
-template <int D0>
-__device__ __forceinline__ void chgpen_terms(double x, const double* d0, double* t) {
-  double d[D0 + 7];
-#pragma unroll
-  for (int k = 0; k < D0 + 7; ++k) d[k] = k <= D0 ? d0[k] : 0.0;
-  const double ex = exp(-x);
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    double p = 0;
-#pragma unroll
-    for (int k = D0 + n; k >= 0; --k) p = fma(p, x, d[k]);
-    t[n] = ex * p;
-    const double m = 2 * n + 1;
-#pragma unroll
-    for (int k = D0 + n + 1; k >= 0; --k) d[k] = (k <= D0 + n ? (m - k) * d[k] : 0.0) + (k > 0 ? d[k - 1] : 0.0);
-  }
-}
-
-__device__ __forceinline__ void chgpen_lambda(double r, double ai, double ak, double* lci, double* lck,
-                                              double* lik) {
-  const double dfac[6] = {1, 1, 3, 15, 105, 945};
-  const double cv[2] = {1.0, 0.5};
-  double ta[6], tb[6];
-  chgpen_terms<1>(ai * r, cv, ta);
-  chgpen_terms<1>(ak * r, cv, tb);
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    lci[n] = 1 - ta[n] / dfac[n];
-    lck[n] = 1 - tb[n] / dfac[n];
-  }
-  if (fabs(ai - ak) < 1e-3) {
-    const double vv[4] = {1.0, 11.0 / 16.0, 3.0 / 16.0, 1.0 / 48.0};
-    chgpen_terms<3>(0.5 * (ai + ak) * r, vv, ta);
-#pragma unroll
-    for (int n = 0; n < 6; ++n) lik[n] = 1 - ta[n] / dfac[n];
-  } else {
-    const double ti = ak * ak / ((ak - ai) * (ak + ai)), tk = 1 - ti;
-    const double pa[2] = {ti * ti * (1 + 2 * tk), 0.5 * ti * ti}, pb[2] = {tk * tk * (1 + 2 * ti), 0.5 * tk * tk};
-    chgpen_terms<1>(ai * r, pa, ta);
-    chgpen_terms<1>(ak * r, pb, tb);
-#pragma unroll
-    for (int n = 0; n < 6; ++n) lik[n] = 1 - (ta[n] + tb[n]) / dfac[n];
-  }
-}
+// Damping factors: chgpen_lambda (chgpen.hpp), in FP64 whatever the working
+// precision.
 
 // kEmplarParts waves per 64 atoms over the in-cutoff list including the
-// scaled pairs (the entry's class gives the multipole scale), both sides of
-// every pair, FP64 arithmetic; the outputs of k_emplar (lane-owned gradient
-// and torque terms, half pair energy and virial).
-template <class real, int EWALD>
+// scaled pairs (the entry's class gives the scales), both sides of every
+// pair, FP64 arithmetic; the outputs of k_emplar (lane-owned gradient and
+// torque terms, half pair energy and virial).  MODE bit 0: multipole terms,
+// bit 1: polarization gradient.
+template <class real, int EWALD, int MODE>
 __global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<real> A,
                                                                      const double* __restrict__ cp) {
+  constexpr bool DM = MODE & 1, DP = (MODE & 2) != 0;
   __shared__ double sh[parts_lds<kEmplarParts, 19>()];
   const int part = threadIdx.x >> 6;
   const int blk = xcd_block();
@@ -460,11 +418,19 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<
     const int len = A.ncut[s];
     const uint32_t* e = A.ecut + A.woff[s >> 6] + (s & 63);
     const double z3[3] = {0, 0, 0};
+    double udi[3] = {0, 0, 0}, upi[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+    if (DP)
+      for (int t = 0; t < 3; ++t) {
+        udi[t] = A.ud[3 * s + t];
+        upi[t] = A.up[3 * s + t];
+        ui[t] = udi[t] + upi[t];
+      }
     for (int kk = part; kk < len; kk += kEmplarParts) {
       const uint32_t ent = e[(long long)kk * 64];
       const int code = (int)(ent & (kMaxCodes - 1));
       const int j = (int)(ent >> kCodeBits);
       const double sm = code ? (double)A.sc.m[code] : 1.0;
+      const double sp = code ? (double)A.sc.p[code] : 1.0, su = code ? (double)A.sc.u[code] : 1.0;
       double R[3];
       pair_disp64(A.pos[j], pi, A.box, R);  // r_i - r_k
       const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
@@ -497,20 +463,54 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<
       for (int t = 0; t < 3; ++t) vk.d[t] = A.rpole[10 * (size_t)j + 1 + t];
       for (int t = 0; t < 6; ++t) vk.Q[t] = A.rpole[10 * (size_t)j + 4 + t];
       PairOut<double> o;
-      // core of i: energy and gradient only (a point charge has no torque)
-      pair_terms<double, true, false>(R, Bcc, nullptr, ci, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
-      for (int t = 0; t < 3; ++t) o.g[t] = 0;
-      for (int t = 0; t < 6; ++t) o.h[t] = 0;
-      acc.add(o, R);
-      pair_terms<double, true, false>(R, Bk, nullptr, ci, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
-      for (int t = 0; t < 3; ++t) o.g[t] = 0;
-      for (int t = 0; t < 6; ++t) o.h[t] = 0;
-      acc.add(o, R);
-      // valence of i: also the potential derivatives that turn into its torque
-      pair_terms<double, true, false>(R, Bi, nullptr, vi, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
-      acc.add(o, R);
-      pair_terms<double, true, false>(R, Bik, nullptr, vi, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
-      acc.add(o, R);
+      if (DM) {
+        // core of i: energy and gradient only (a point charge has no torque)
+        pair_terms<double, true, false>(R, Bcc, nullptr, ci, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
+        for (int t = 0; t < 3; ++t) o.g[t] = 0;
+        for (int t = 0; t < 6; ++t) o.h[t] = 0;
+        acc.add(o, R);
+        pair_terms<double, true, false>(R, Bk, nullptr, ci, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
+        for (int t = 0; t < 3; ++t) o.g[t] = 0;
+        for (int t = 0; t < 6; ++t) o.h[t] = 0;
+        acc.add(o, R);
+        // valence of i: also the potential derivatives that turn into its torque
+        pair_terms<double, true, false>(R, Bi, nullptr, vi, ck, z3, z3, z3, z3, z3, z3, 1.0, o);
+        acc.add(o, R);
+        pair_terms<double, true, false>(R, Bik, nullptr, vi, vk, z3, z3, z3, z3, z3, z3, 1.0, o);
+        acc.add(o, R);
+      }
+      if (DP) {
+        double udk[3], upk[3], uk[3];
+        for (int t = 0; t < 3; ++t) {
+          udk[t] = A.ud[3 * j + t];
+          upk[t] = A.up[3 * j + t];
+          uk[t] = udk[t] + upk[t];
+        }
+        double Pc[6], Pk[6], Pi[6], Pu[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          const double b = EWALD ? bn[q] - rr[q] : 0.0;
+          Pc[q] = b + sp * rr[q];
+          Pk[q] = b + sp * lck[q] * rr[q];
+          Pi[q] = b + sp * lci[q] * rr[q];
+          Pu[q] = b + su * lik[q] * rr[q];
+        }
+        // u_i at the core of k and the core of i in the field of u_k
+        // (undamped; the core has no torque)
+        pair_terms<double, false, true>(R, nullptr, Pc, ci, ck, ui, uk, z3, z3, z3, z3, 0.0, o);
+        for (int t = 0; t < 3; ++t) o.g[t] = 0;
+        for (int t = 0; t < 6; ++t) o.h[t] = 0;
+        acc.add(o, R);
+        // u_i in the valence multipole of k (k's damping)
+        pair_terms<double, false, true>(R, nullptr, Pk, ci, vk, ui, z3, z3, z3, z3, z3, 0.0, o);
+        acc.add(o, R);
+        // the valence multipole of i in the field of u_k (i's damping; torque)
+        pair_terms<double, false, true>(R, nullptr, Pi, vi, ck, z3, uk, z3, z3, z3, z3, 0.0, o);
+        acc.add(o, R);
+        // dipole-dipole (valence-valence damping)
+        pair_terms<double, false, true>(R, nullptr, Pu, ci, ck, z3, z3, udi, upi, udk, upk, 1.0, o);
+        acc.add(o, R);
+      }
     }
   }
   double av[19];
@@ -1138,10 +1138,13 @@ void real_space_t(Ctx& c, bool use_m, bool use_p) {
     kscaled = mode == 1 ? k_emplar_scaled<real, 0, 1>
                         : (mode == 2 ? k_emplar_scaled<real, 0, 2> : k_emplar_scaled<real, 0, 3>);
   A.rows = energy_rows(c, c.nwave);
-  if (c.chgpen) {  // HIPPO multipoles: every pair, scaled ones included, in one kernel
-    if (use_p) throw ArgError("HIPPO polarization is not built");
+  if (c.chgpen) {  // HIPPO: every pair, scaled ones included, in one kernel
     KScope ks_(c, KT_EMPLAR);
-    void (*kc)(EmplarArgs<real>, const double*) = c.ewald ? k_empole_chgpen<real, 1> : k_empole_chgpen<real, 0>;
+    void (*kc)(EmplarArgs<real>, const double*) = nullptr;
+    if (c.ewald)
+      kc = mode == 1 ? k_empole_chgpen<real, 1, 1> : (mode == 2 ? k_empole_chgpen<real, 1, 2> : k_empole_chgpen<real, 1, 3>);
+    else
+      kc = mode == 1 ? k_empole_chgpen<real, 0, 1> : (mode == 2 ? k_empole_chgpen<real, 0, 2> : k_empole_chgpen<real, 0, 3>);
     hipLaunchKernelGGL(kc, dim3(c.nwave), dim3(64 * kEmplarParts), 0, c.stream, A, c.chgpen_o.as<double>());
     T9_LAUNCH_CHECK();
     return;

@@ -22,6 +22,7 @@
 // roots and reciprocals use the hardware rsqrt; the Ewald constants
 // 2^n a^(2n-1)/sqrt(pi) are precomputed on the host; erfc shares exp(-x^2)
 // with the Ewald recursion (t9_erfc_pair).
+#include "chgpen.hpp"
 #include "ctx.hpp"
 
 namespace t9 {
@@ -256,6 +257,137 @@ __global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const in
   }
 }
 
+// HIPPO fields (src/cu/hippo/dfieldChgpen_cu1.cc, ufieldChgpen_cu1.cc,
+// include/seq/pair_field_chgpen.h): an induced dipole is a point probe, so
+// the direct field at i is the field of k's core charge (undamped) plus that
+// of k's valence multipole with k's core-valence damping, scaled by the p
+// class of the pair (HIPPO's d scale); the mutual field has the
+// valence-valence damping and the u class (HIPPO's induce scale).  FP64
+// arithmetic whatever the working precision; same layout, parts and order
+// as k_dfield_real / k_ufield_real.  cp: caller-order [n][3] core charge,
+// valence charge, exponent.
+template <class real, int EWALD>
+__global__ __launch_bounds__(64 * kPairParts) void k_dfield_chgpen(int n, const int* __restrict__ ncut,
+                                                       const long long* __restrict__ woff,
+                                                       const uint32_t* __restrict__ ecut,
+                                                       const R4<real>* __restrict__ pos, const real* __restrict__ rpole,
+                                                       const int* __restrict__ perm, const double* __restrict__ cp,
+                                                       ScaleTable sc, BoxT<real> box, EwaldK<double> ek,
+                                                       real* __restrict__ fd, real* __restrict__ fp) {
+  __shared__ double sh[parts_lds<kPairParts, 6>()];
+  const int part = threadIdx.x >> 6;
+  const int s = xcd_block() * 64 + (threadIdx.x & 63);
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  if (s < n) {
+    const int len = ncut[s];
+    const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
+    const R4<real> pi = pos[s];
+    for (int k = part; k < len; k += kPairParts) {
+      const uint32_t ent = e[(long long)k * 64];
+      const int j = (int)(ent >> kCodeBits);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const double ps = code ? sc.p[code] : 1.0;
+      double d[3];
+      pair_disp64(pi, pos[j], box, d);  // x_k - x_i
+      const double xr = d[0], yr = d[1], zr = d[2];
+      const double r2 = xr * xr + yr * yr + zr * zr;
+      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
+      const double rr3 = rinv * rr2, rr5 = 3 * rr3 * rr2, rr7 = 5 * rr5 * rr2;
+      double bn[4] = {0, 0, 0, 0};
+      if (EWALD) ewald_b<double, 4>(bn, r, rinv, rr2, ek);
+      const int jo = perm[j];
+      const double corek = cp[3 * jo], valk = cp[3 * jo + 1];
+      double lci[6], lck[6], lik[6];
+      chgpen_lambda(r, cp[3 * jo + 2], cp[3 * jo + 2], lci, lck, lik);
+      auto kern = [&](double lam, double rr, double b) { return EWALD ? b - (1 - ps * lam) * rr : ps * lam * rr; };
+      const double c1 = kern(1.0, rr3, bn[1]);
+      const double k1 = kern(lck[1], rr3, bn[1]), k2 = kern(lck[2], rr5, bn[2]), k3 = kern(lck[3], rr7, bn[3]);
+      const real* pk = rpole + 10 * (size_t)j;
+      const double dkx = pk[1], dky = pk[2], dkz = pk[3];
+      const double qxx = pk[4], qyy = pk[5], qzz = pk[6], qxy = pk[7], qxz = pk[8], qyz = pk[9];
+      const double dkr = dkx * xr + dky * yr + dkz * zr;
+      const double qkx = qxx * xr + qxy * yr + qxz * zr;
+      const double qky = qxy * xr + qyy * yr + qyz * zr;
+      const double qkz = qxz * xr + qyz * yr + qzz * zr;
+      const double qkr = qkx * xr + qky * yr + qkz * zr;
+      const double t = -(c1 * corek + k1 * valk - k2 * dkr + k3 * qkr);
+      const double fx = t * xr - k1 * dkx + 2 * k2 * qkx;
+      const double fy = t * yr - k1 * dky + 2 * k2 * qky;
+      const double fz = t * zr - k1 * dkz + 2 * k2 * qkz;
+      ab[0] += fx;
+      ab[1] += fy;
+      ab[2] += fz;
+    }
+  }
+  parts_reduce<kPairParts, 6>(ab, sh);
+  if (part == 0 && s < n)
+    for (int q = 0; q < 3; ++q) {
+      fd[3 * s + q] = (real)ab[q];
+      fp[3 * s + q] = (real)ab[q];
+    }
+}
+
+template <class real, int EWALD>
+__global__ __launch_bounds__(64 * kPairParts) void k_ufield_chgpen(int n, const int* __restrict__ ncut,
+                                                       const long long* __restrict__ woff,
+                                                       const uint32_t* __restrict__ ecut,
+                                                       const R4<real>* __restrict__ pos, const real* __restrict__ ud,
+                                                       const real* __restrict__ up, const int* __restrict__ perm,
+                                                       const double* __restrict__ cp, ScaleTable sc, BoxT<real> box,
+                                                       EwaldK<double> ek, real* __restrict__ fd,
+                                                       real* __restrict__ fp, Gate gate) {
+  if (gate_closed(gate)) return;
+  __shared__ double sh[parts_lds<kPairParts, 6>()];
+  const int part = threadIdx.x >> 6;
+  const int s = xcd_block() * 64 + (threadIdx.x & 63);
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  if (s < n) {
+    const int len = ncut[s];
+    const uint32_t* e = ecut + woff[s >> 6] + (s & 63);
+    const R4<real> pi = pos[s];
+    const double ai = cp[3 * perm[s] + 2];
+    for (int k = part; k < len; k += kPairParts) {
+      const uint32_t ent = e[(long long)k * 64];
+      const int j = (int)(ent >> kCodeBits);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const double us = code ? sc.u[code] : 1.0;
+      double d[3];
+      pair_disp64(pi, pos[j], box, d);  // x_k - x_i
+      const double xr = d[0], yr = d[1], zr = d[2];
+      const double r2 = xr * xr + yr * yr + zr * zr;
+      const double r = sqrt(r2), rinv = 1 / r, rr2 = rinv * rinv;
+      const double rr3 = rinv * rr2, rr5 = 3 * rr3 * rr2;
+      double lci[6], lck[6], lik[6];
+      chgpen_lambda(r, ai, cp[3 * perm[j] + 2], lci, lck, lik);
+      double c1, c2;
+      if (EWALD) {
+        double bn[3];
+        ewald_b<double, 3>(bn, r, rinv, rr2, ek);
+        c1 = bn[1] - (1 - us * lik[1]) * rr3;
+        c2 = bn[2] - (1 - us * lik[2]) * rr5;
+      } else {
+        c1 = us * lik[1] * rr3;
+        c2 = us * lik[2] * rr5;
+      }
+      const double dx = ud[3 * j], dy = ud[3 * j + 1], dz = ud[3 * j + 2];
+      const double px = up[3 * j], py = up[3 * j + 1], pz = up[3 * j + 2];
+      const double cd = c2 * (dx * xr + dy * yr + dz * zr), cq = c2 * (px * xr + py * yr + pz * zr);
+      ab[0] += cd * xr - c1 * dx;
+      ab[1] += cd * yr - c1 * dy;
+      ab[2] += cd * zr - c1 * dz;
+      ab[3] += cq * xr - c1 * px;
+      ab[4] += cq * yr - c1 * py;
+      ab[5] += cq * zr - c1 * pz;
+    }
+  }
+  parts_reduce<kPairParts, 6>(ab, sh);
+  if (part == 0 && s < n)
+    for (int q = 0; q < 3; ++q) {
+      fd[3 * s + q] = (real)ab[q];
+      fp[3 * s + q] = (real)ab[3 + q];
+    }
+}
+
 // One sparse-preconditioner pair: alpha_i alpha_k T_ik r_k added to o[0..2]
 // (d) and o[3..5] (p).
 template <class real, int UTH>
@@ -402,6 +534,16 @@ template <class real>
 void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, hipStream_t st) {
   const auto& L = c.mlist;
   require_cut(L);
+  if (c.chgpen) {
+    auto kc = c.ewald ? k_ufield_chgpen<real, 1> : k_ufield_chgpen<real, 0>;
+    KScope ks_(c, KT_UFIELD_REAL, 1, st);
+    hipLaunchKernelGGL(kc, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n, L.ncut.as<int>(),
+                       L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), (const real*)ud,
+                       (const real*)up, c.perm.as<int>(), c.chgpen_o.as<double>(), c.scale, make_boxT<real>(c),
+                       ewald_k<double>(c), (real*)fd, (real*)fp, c.gate);
+    T9_LAUNCH_CHECK();
+    return;
+  }
   auto kern = c.ewald ? (c.uniform_thole ? k_ufield_real<real, 1, 1> : k_ufield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_ufield_real<real, 0, 1> : k_ufield_real<real, 0, 0>);
   KScope ks_(c, KT_UFIELD_REAL, 1, st);
@@ -416,6 +558,16 @@ template <class real>
 void dfield_real_t(Ctx& c, hipStream_t st) {
   const auto& L = c.mlist;
   require_cut(L);
+  if (c.chgpen) {
+    auto kc = c.ewald ? k_dfield_chgpen<real, 1> : k_dfield_chgpen<real, 0>;
+    KScope ks_(c, KT_DFIELD_REAL, 1, st);
+    hipLaunchKernelGGL(kc, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n, L.ncut.as<int>(),
+                       L.off.as<long long>(), L.ecut.as<uint32_t>(), c.pos.as<R4<real>>(), c.rpole.as<real>(),
+                       c.perm.as<int>(), c.chgpen_o.as<double>(), c.scale, make_boxT<real>(c), ewald_k<double>(c),
+                       c.field.as<real>(), c.fieldp.as<real>());
+    T9_LAUNCH_CHECK();
+    return;
+  }
   auto kern = c.ewald ? (c.uniform_thole ? k_dfield_real<real, 1, 1> : k_dfield_real<real, 1, 0>)
                       : (c.uniform_thole ? k_dfield_real<real, 0, 1> : k_dfield_real<real, 0, 0>);
   KScope ks_(c, KT_DFIELD_REAL, 1, st);
