@@ -43,7 +43,13 @@ const char* t9_last_error(void);
  * in the reference tree. */
 typedef struct t9_system t9_system;
 
-/* Tinker .xyz + .prm + keyword text (the contents of a .key file). */
+/* Tinker .xyz + .prm + keyword text (the contents of a .key file).  A
+ * parameter file with chgpen records (HIPPO, GORDON1 charge penetration)
+ * gives a system whose multipole and polarization terms use the damped
+ * core/valence kernels (the reference's empoleChgpen / epolarChgpen /
+ * dfieldChgpen / ufieldChgpen, src/cu/hippo/empole.cu, epolar.cu, field.cu);
+ * every entry point below then computes the HIPPO terms.  HIPPO's other
+ * terms (repulsion, dispersion, charge transfer) and AMOEBA+ are not built. */
 int t9_system_from_files(const char* xyz_path, const char* prm_path, const char* key_text,
                          t9_system** out);
 /* Seeded AMOEBA water box: nmol rigid waters (O-H 0.9572 Å, 104.52°) on an
