@@ -428,25 +428,25 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
 // cut into 8^3 tiles and the support bases into 4^3 bins; every atom whose
 // support reaches a tile has its base in one of the 27 bins around the tile's
 // own 8 (bin offsets -1, 0, +1 per axis).  One 256-thread workgroup per tile
-// stages those atoms in LDS (B-spline weights, scaled sources, base offset
-// from the tile origin), in bin order and, within a bin, in ascending sorted
-// index.  Lane (y, z) of each wavefront owns the tile's grid row (y, z): it
-// keeps the row's 8 points in registers and adds the atoms whose support
-// covers the row, one after another; wave w takes atoms w, w + 4, ... and
-// the four waves' rows are added in wave order at the end.  Every grid point
-// is written once, by one lane, and its sum runs in an order fixed by the
-// atoms' positions alone: bit-reproducible, with no atomics and no fixed
-// point.  Work per atom: the row factors of its (y, z) support (one set per
-// lane) and two fused multiply-adds per point and part.
+// stages those atoms in LDS, in bin order and, within a bin, in ascending
+// sorted index: their scaled sources and their B-spline weights laid out on
+// the tile's 8 points per axis (zero where the support does not reach), so
+// the accumulation has no branch and reads each weight row with broadcast
+// loads.  Lane (y, z) of each wavefront owns the tile's grid row (y, z): it
+// keeps the row's 8 points in registers and adds the atoms one after another;
+// wave w takes atoms w, w + 4, ... and the four waves' rows are added in wave
+// order at the end.  Every grid point is written once, by one lane, and its
+// sum runs in an order fixed by the atoms' positions alone: bit-reproducible,
+// with no atomics and no fixed point.
 constexpr int kRowTile = 8, kRowBin = 4, kRowChunk = 128;
 
 template <class real, int MODE>
 struct RowAtom {
   static constexpr int LV = MODE == 0 ? 2 : 3;
   static constexpr int NF = MODE == 0 ? 6 : (MODE == 1 ? 10 : 13);
-  real th[3][LV][5];  // [axis][derivative][support point]
+  real w[3][LV][kRowTile];  // [axis][derivative][tile point], zero outside the support
   real f[NF];
-  int off[4];         // support base relative to the tile origin, x y z (-4..7)
+  int off[3];  // support base relative to the tile origin per axis (-4..7)
 };
 
 template <class real, int MODE>
@@ -461,10 +461,13 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
   constexpr int LV = RA::LV;
   constexpr int NACC = MODE == 1 ? 1 : 2;
   constexpr int CH = (sizeof(real) == 8 && MODE != 0) ? kRowChunk / 2 : kRowChunk;
-  constexpr size_t kStage = sizeof(RA) * CH, kPart = sizeof(real) * 3 * 64 * NACC * kRowTile;
+  static_assert(CH % 64 == 0, "the sub-tile lists are built by whole wavefronts");
+  constexpr size_t kStage = sizeof(RA) * CH, kPart = sizeof(real) * 2 * 64 * NACC * 4;
   __shared__ __align__(16) char smem[kStage > kPart ? kStage : kPart];
   RA* atoms = reinterpret_cast<RA*>(smem);
   __shared__ int bstart[27], bpref[28];
+  __shared__ unsigned char lists[8][CH];
+  __shared__ int wcnt[CH / 64][8];
   const int tt = blockIdx.x;  // tiles in launch order (an XCD-contiguous order measured slower)
   const int t1 = tt % g.nt[0], t2 = (tt / g.nt[0]) % g.nt[1], t3 = tt / (g.nt[0] * g.nt[1]);
   const int lo[3] = {t1 * kRowTile, t2 * kRowTile, t3 * kRowTile};
@@ -493,12 +496,11 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
   }
   __syncthreads();
   const int total = bpref[27];
-  const int ly = lane & 7, lz = lane >> 3;
-  real acc[NACC][kRowTile];
+  real acc[NACC][4];
 #pragma unroll
   for (int q = 0; q < NACC; ++q)
 #pragma unroll
-    for (int x = 0; x < kRowTile; ++x) acc[q][x] = 0;
+    for (int x = 0; x < 4; ++x) acc[q][x] = 0;
   for (int base = 0; base < total; base += CH) {
     const int nchunk = min(CH, total - base);
     // ---- stage: one thread per atom
@@ -512,29 +514,26 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
       RA& A = atoms[tid];
       const int4 b = pbase[s];
       const int bb[3] = {b.x, b.y, b.z};
+      const R4<real> w = pw[s];
+      const real wf[3] = {w.x, w.y, w.z};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         int o = bb[d] - lo[d];
         if (o >= kRowTile) o -= g.K[d];  // lower neighbour bin across the periodic boundary
         A.off[d] = o;
+        real th[LV][5];
+        bspline5<LV>(wf[d], th);
+        // the row: zeros, then the support points that fall on the tile
+        // (stores at a computed offset instead of selects per point)
+#pragma unroll
+        for (int lv = 0; lv < LV; ++lv) {
+#pragma unroll
+          for (int x = 0; x < kRowTile; ++x) A.w[d][lv][x] = 0;
+#pragma unroll
+          for (int q = 0; q < 5; ++q)
+            if ((unsigned)(o + q) < (unsigned)kRowTile) A.w[d][lv][o + q] = th[lv][q];
+        }
       }
-      const R4<real> w = pw[s];
-      real th[LV][5];
-      bspline5<LV>(w.x, th);
-#pragma unroll
-      for (int lv = 0; lv < LV; ++lv)
-#pragma unroll
-        for (int m = 0; m < 5; ++m) A.th[0][lv][m] = th[lv][m];
-      bspline5<LV>(w.y, th);
-#pragma unroll
-      for (int lv = 0; lv < LV; ++lv)
-#pragma unroll
-        for (int m = 0; m < 5; ++m) A.th[1][lv][m] = th[lv][m];
-      bspline5<LV>(w.z, th);
-#pragma unroll
-      for (int lv = 0; lv < LV; ++lv)
-#pragma unroll
-        for (int m = 0; m < 5; ++m) A.th[2][lv][m] = th[lv][m];
       if (MODE == 0) {
         A.f[0] = src1[3 * s] * a1;
         A.f[1] = src1[3 * s + 1] * a2;
@@ -562,46 +561,82 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
       }
     }
     __syncthreads();
-    // ---- accumulate: each wave takes every 4th staged atom
-    for (int a = wv; a < nchunk; a += 4) {
-      const RA& A = atoms[a];
-      const int ox = __builtin_amdgcn_readfirstlane(A.off[0]);
-      const int dy = ly - A.off[1], dz = lz - A.off[2];
-      const bool act = (unsigned)dy <= 4u && (unsigned)dz <= 4u;
-      const int my = act ? dy : 0, mz = act ? dz : 0;
-      const real u0 = A.th[1][0][my], u1 = A.th[1][1][my];
-      const real v0 = A.th[2][0][mz], v1 = A.th[2][1][mz];
-      const real* f = A.f;
-      if (MODE == 0) {
-        real Ad = f[0] * u0 * v0, Bd = f[1] * u1 * v0 + f[2] * u0 * v1;
-        real Ap = f[3] * u0 * v0, Bp = f[4] * u1 * v0 + f[5] * u0 * v1;
-        if (!act) Ad = Bd = Ap = Bp = 0;
+    // ---- the staged atoms whose support reaches each 4^3 sub-tile, in
+    // staging order (wave ballots, offsets of the earlier staging waves)
+    unsigned long long bal[8];
+    if (tid < CH) {
+      unsigned m = 0;
+      if (tid < nchunk) {
+        const RA& A = atoms[tid];
 #pragma unroll
-        for (int x = 0; x < kRowTile; ++x) {
-          const int m1 = x - ox;
-          if ((unsigned)m1 <= 4u) {
-            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1];
+        for (int st = 0; st < 8; ++st) {
+          const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
+          const bool in = A.off[0] <= sx + 3 && A.off[0] + 4 >= sx && A.off[1] <= sy + 3 && A.off[1] + 4 >= sy &&
+                          A.off[2] <= sz + 3 && A.off[2] + 4 >= sz;
+          m |= in ? 1u << st : 0u;
+        }
+      }
+#pragma unroll
+      for (int st = 0; st < 8; ++st) {
+        bal[st] = __ballot((m >> st) & 1);
+        if (lane == 0) wcnt[wv][st] = __popcll(bal[st]);
+      }
+    }
+    __syncthreads();
+    if (tid < CH) {
+#pragma unroll
+      for (int st = 0; st < 8; ++st)
+        if ((bal[st] >> lane) & 1) {
+          int o = __popcll(bal[st] & ((1ull << lane) - 1));
+          for (int w = 0; w < wv; ++w) o += wcnt[w][st];
+          lists[st][o] = (unsigned char)tid;
+        }
+    }
+    __syncthreads();
+    // ---- accumulate: 16-lane group gq of wave wv owns the 16 rows of
+    // sub-tile (wv & 1) * 4 + gq; waves wv and wv + 2 split its list (even
+    // and odd entries)
+    {
+      const int gq = lane >> 4, r = lane & 15;
+      const int st = (wv & 1) * 4 + gq;
+      const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4 + (r & 3), sz = (st >> 2) * 4 + (r >> 2);
+      int cnt = 0, cmax = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int c = 0;
+        for (int w = 0; w < CH / 64; ++w) c += wcnt[w][(wv & 1) * 4 + q];
+        cnt = q == gq ? c : cnt;
+        cmax = c > cmax ? c : cmax;
+      }
+      for (int k = wv >> 1; k < cmax; k += 2) {
+        if (k >= cnt) continue;
+        const RA& A = atoms[lists[st][k]];
+        const real u0 = A.w[1][0][sy], u1 = A.w[1][1][sy];
+        const real v0 = A.w[2][0][sz], v1 = A.w[2][1][sz];
+        const real* f = A.f;
+        if (MODE == 0) {
+          const real Ad = f[0] * u0 * v0, Bd = f[1] * u1 * v0 + f[2] * u0 * v1;
+          const real Ap = f[3] * u0 * v0, Bp = f[4] * u1 * v0 + f[5] * u0 * v1;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const real t0 = A.w[0][0][sx + x], t1v = A.w[0][1][sx + x];
             acc[0][x] += Ad * t1v + Bd * t0;
             acc[NACC - 1][x] += Ap * t1v + Bp * t0;
           }
-        }
-      } else {
-        const real u2 = A.th[1][LV > 2 ? 2 : 0][my], v2 = A.th[2][LV > 2 ? 2 : 0][mz];
-        real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
-                  f[9] * u1 * v1;
-        real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
-        real R2 = f[4] * u0 * v0;
-        real Ap = 0, Bp = 0;
-        if (MODE == 2) {
-          Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
-          Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
-        }
-        if (!act) R0 = R1 = R2 = Ap = Bp = 0;
+        } else {
+          const real u2 = A.w[1][LV > 2 ? 2 : 0][sy], v2 = A.w[2][LV > 2 ? 2 : 0][sz];
+          const real R0 = f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 +
+                          f[9] * u1 * v1;
+          const real R1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+          const real R2 = f[4] * u0 * v0;
+          real Ap = 0, Bp = 0;
+          if (MODE == 2) {
+            Ap = f[MODE == 2 ? 10 : 0] * u0 * v0;
+            Bp = f[MODE == 2 ? 11 : 0] * u1 * v0 + f[MODE == 2 ? 12 : 0] * u0 * v1;
+          }
 #pragma unroll
-        for (int x = 0; x < kRowTile; ++x) {
-          const int m1 = x - ox;
-          if ((unsigned)m1 <= 4u) {
-            const real t0 = A.th[0][0][m1], t1v = A.th[0][1][m1], t2v = A.th[0][LV > 2 ? 2 : 0][m1];
+          for (int x = 0; x < 4; ++x) {
+            const real t0 = A.w[0][0][sx + x], t1v = A.w[0][1][sx + x], t2v = A.w[0][LV > 2 ? 2 : 0][sx + x];
             acc[0][x] += R0 * t0 + R1 * t1v + R2 * t2v;
             if (MODE == 2) acc[NACC - 1][x] += Ap * t1v + Bp * t0;
           }
@@ -610,25 +645,27 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
     }
     __syncthreads();
   }
-  // ---- the four waves' rows, added in wave order by wave 0, written once
-  real* part = reinterpret_cast<real*>(smem);  // [3][64][NACC * 8] (the staging area is free)
-  if (wv > 0) {
+  // ---- waves 2 and 3 hand their rows to waves 0 and 1, which add them in
+  // that order and write every point of their sub-tiles once
+  real* part = reinterpret_cast<real*>(smem);  // [2][64][NACC * 4] (the staging area is free)
+  if (wv >= 2) {
 #pragma unroll
     for (int q = 0; q < NACC; ++q)
 #pragma unroll
-      for (int x = 0; x < kRowTile; ++x) part[((wv - 1) * 64 + lane) * (NACC * kRowTile) + q * kRowTile + x] = acc[q][x];
+      for (int x = 0; x < 4; ++x) part[((wv - 2) * 64 + lane) * (NACC * 4) + q * 4 + x] = acc[q][x];
   }
   __syncthreads();
-  if (wv == 0) {
+  if (wv < 2) {
 #pragma unroll
-    for (int w = 0; w < 3; ++w)
+    for (int q = 0; q < NACC; ++q)
 #pragma unroll
-      for (int q = 0; q < NACC; ++q)
+      for (int x = 0; x < 4; ++x) acc[q][x] += part[(wv * 64 + lane) * (NACC * 4) + q * 4 + x];
+    const int gq = lane >> 4, r = lane & 15;
+    const int st = (wv & 1) * 4 + gq;
+    const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4 + (r & 3), sz = (st >> 2) * 4 + (r >> 2);
+    const long long row = lo[0] + sx + (long long)g.K[0] * ((lo[1] + sy) + (long long)g.K[1] * (lo[2] + sz));
 #pragma unroll
-        for (int x = 0; x < kRowTile; ++x) acc[q][x] += part[(w * 64 + lane) * (NACC * kRowTile) + q * kRowTile + x];
-    const long long row = lo[0] + (long long)g.K[0] * ((lo[1] + ly) + (long long)g.K[1] * (lo[2] + lz));
-#pragma unroll
-    for (int x = 0; x < kRowTile; ++x) {
+    for (int x = 0; x < 4; ++x) {
       cplx_t<real> v;
       v.x = acc[0][x];
       v.y = NACC == 2 ? acc[NACC - 1][x] : (real)0;
