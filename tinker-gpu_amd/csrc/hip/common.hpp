@@ -54,6 +54,15 @@ __device__ __forceinline__ int xcd_block() {
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// In-cutoff compaction: the first rows of each wavefront's output slabs are
+// assembled in LDS (in-cutoff list, preconditioner list) and written whole.
+constexpr int kCompactRowsA = 176, kCompactRowsB = 64;
+
 __device__ __forceinline__ float t9_rint(float x) { return rintf(x); }
 __device__ __forceinline__ double t9_rint(double x) { return rint(x); }
 __device__ __forceinline__ float t9_exp(float x) { return expf(x); }

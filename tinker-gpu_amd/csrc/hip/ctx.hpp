@@ -297,6 +297,7 @@ void nblist_build(Ctx& c);
 bool nblist_needs_rebuild(Ctx& c, bool refresh = false);
 void nblist_mark_refreshed(Ctx& c);
 void nblist_compact(Ctx& c, hipStream_t st = nullptr);  // refresh the in-cutoff lists (main stream unless st)
+bool nblist_compact_buffers(Ctx& c);  // size the in-cutoff lists; false without a list
 void refresh_sorted_coords(Ctx& c);
 void mpole_init(Ctx& c);
 // c.field/fieldp = full direct field; with ua, also fa = the mutual field of

@@ -257,6 +257,176 @@ __global__ __launch_bounds__(64 * kPairParts) void k_dfield_real(int n, const in
   }
 }
 
+// The in-cutoff compaction of the buffered list fused with the real-space
+// direct field: the same partition and order as the compaction kernel (part
+// p of the workgroup takes chunk p of each round of a lane's list; the kept
+// entries of a round go out in list order through LDS rows), and every entry
+// found within the cutoff also adds its direct-field term, with the pair
+// math of k_dfield_real, to the part's FP64 sums, which are added in part
+// order at the end.  One pass over the buffered list and its partners'
+// positions serves both.
+template <class real, int EWALD, int UTH>
+__global__ __launch_bounds__(64 * kPairParts) void k_compact_dfield(
+    int n, const int* __restrict__ nlist, const long long* __restrict__ woff, const uint32_t* __restrict__ entries,
+    const R4<real>* __restrict__ pos, BoxT<real> box, real cut2, real cut2b, int* __restrict__ ncut,
+    uint32_t* __restrict__ ecut, int* __restrict__ ncut2, uint32_t* __restrict__ ecut2,
+    const real* __restrict__ rpole, const int* __restrict__ jpolar, const real* __restrict__ thlval, int nj,
+    real pga_u, ScaleTable sc, EwaldK<real> ek, real* __restrict__ fd, real* __restrict__ fp) {
+  constexpr int P = kPairParts;
+  __shared__ unsigned short cnt_a[P][64], cnt_b[P][64];
+  __shared__ uint32_t rows_a[kCompactRowsA][64], rows_b[kCompactRowsB][64];
+  const int part = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_block();
+  const int s = blk * 64 + lane;
+  const bool two = ecut2 != nullptr;
+  const bool valid = s < n;
+  const int len = valid ? nlist[s] : 0;
+  const int maxlen = wave_max(len);
+  const long long base = woff[blk] + lane;
+  const uint32_t* e = entries + base;
+  R4<real> pi;
+  if (valid) pi = pos[s];
+  const int jpi = UTH ? 0 : jpolar[valid ? s : 0] * nj;
+  double ab[6] = {0, 0, 0, 0, 0, 0};
+  int tot_a = 0, tot_b = 0;  // kept entries of this atom in earlier rounds
+  for (int r0 = 0; r0 < maxlen;) {
+    const int C = min(64, (maxlen - r0 + P - 1) / P);
+    const int kb = r0 + part * C;
+    const int ke = min(kb + C, len);
+    unsigned long long ma = 0, mb = 0;
+    for (int k0 = kb; k0 < ke; k0 += 8) {  // 8 entries, then their 8 partners, in flight
+      uint32_t ent[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ent[u] = k0 + u < ke ? e[(long long)(k0 + u) * 64] : 0u;
+      R4<real> pj[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < ke) pj[u] = pos[ent[u] >> kCodeBits];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < ke) {
+          real xr, yr, zr;
+          pair_disp(pi, pj[u], box, xr, yr, zr);
+          const real r2 = xr * xr + yr * yr + zr * zr;
+          if (two && r2 <= cut2b) mb |= 1ull << (k0 + u - kb);
+          if (r2 <= cut2) {
+            ma |= 1ull << (k0 + u - kb);
+            const int j = (int)(ent[u] >> kCodeBits);
+            const real rinv = t9_rsqrt(r2);
+            const real r = r2 * rinv;
+            const real rr2 = rinv * rinv;
+            const real rr3 = rinv * rr2;
+            const real rr5 = 3 * rr3 * rr2;
+            const real rr7 = 5 * rr5 * rr2;
+            const real pga = UTH ? pga_u : thlval[jpi + jpolar[j]];
+            real s3, s5, s7;
+            thole_r(r, pi.w * pj[u].w, pga, s3, s5, s7);
+            const real* pk = rpole + 10 * (size_t)j;
+            const real ck = pk[0], dkx = pk[1], dky = pk[2], dkz = pk[3];
+            const real qxx = pk[4], qyy = pk[5], qzz = pk[6], qxy = pk[7], qxz = pk[8], qyz = pk[9];
+            const real dkr = dkx * xr + dky * yr + dkz * zr;
+            const real qkx = qxx * xr + qxy * yr + qxz * zr;
+            const real qky = qxy * xr + qyy * yr + qyz * zr;
+            const real qkz = qxz * xr + qyz * yr + qzz * zr;
+            const real qkr = qkx * xr + qky * yr + qkz * zr;
+            real bn[4] = {0, 0, 0, 0};
+            if (EWALD) ewald_b<real, 4>(bn, r, rinv, rr2, ek);
+            const int code = (int)(ent[u] & (kMaxCodes - 1));
+            const real ds = code ? (real)sc.d[code] : (real)1, ps = code ? (real)sc.p[code] : (real)1;
+            real c1, c2, c3;
+            if (EWALD) {
+              c1 = bn[1] - (1 - ds * s3) * rr3;
+              c2 = bn[2] - (1 - ds * s5) * rr5;
+              c3 = bn[3] - (1 - ds * s7) * rr7;
+            } else {
+              c1 = ds * s3 * rr3;
+              c2 = ds * s5 * rr5;
+              c3 = ds * s7 * rr7;
+            }
+            real t = -(c1 * ck - c2 * dkr + c3 * qkr);
+            real fx = t * xr - c1 * dkx + 2 * c2 * qkx;
+            real fy = t * yr - c1 * dky + 2 * c2 * qky;
+            real fz = t * zr - c1 * dkz + 2 * c2 * qkz;
+            ab[0] += (double)fx;
+            ab[1] += (double)fy;
+            ab[2] += (double)fz;
+            if (ps != ds) {
+              if (EWALD) {
+                c1 = bn[1] - (1 - ps * s3) * rr3;
+                c2 = bn[2] - (1 - ps * s5) * rr5;
+                c3 = bn[3] - (1 - ps * s7) * rr7;
+              } else {
+                c1 = ps * s3 * rr3;
+                c2 = ps * s5 * rr5;
+                c3 = ps * s7 * rr7;
+              }
+              t = -(c1 * ck - c2 * dkr + c3 * qkr);
+              fx = t * xr - c1 * dkx + 2 * c2 * qkx;
+              fy = t * yr - c1 * dky + 2 * c2 * qky;
+              fz = t * zr - c1 * dkz + 2 * c2 * qkz;
+            }
+            ab[3] += (double)fx;
+            ab[4] += (double)fy;
+            ab[5] += (double)fz;
+          }
+        }
+    }
+    cnt_a[part][lane] = (unsigned short)__popcll(ma);
+    cnt_b[part][lane] = (unsigned short)__popcll(mb);
+    __syncthreads();
+    int oa = tot_a, ob = tot_b;
+    for (int q = 0; q < P; ++q) {
+      if (q < part) {
+        oa += cnt_a[q][lane];
+        ob += cnt_b[q][lane];
+      }
+      tot_a += cnt_a[q][lane];
+      tot_b += cnt_b[q][lane];
+    }
+    __syncthreads();
+    while (ma | mb) {
+      const int ja = ma ? __ffsll((long long)ma) - 1 : 64, jb = mb ? __ffsll((long long)mb) - 1 : 64;
+      const int j = min(ja, jb);
+      const uint32_t ent = e[(long long)(kb + j) * 64];
+      if (j == ja) {
+        if (oa < kCompactRowsA) rows_a[oa][lane] = ent;
+        else ecut[base + (long long)oa * 64] = ent;
+        ++oa;
+        ma &= ma - 1;
+      }
+      if (j == jb) {
+        if (ob < kCompactRowsB) rows_b[ob][lane] = ent;
+        else ecut2[base + (long long)ob * 64] = ent;
+        ++ob;
+        mb &= mb - 1;
+      }
+    }
+    r0 += P * C;
+  }
+  if (valid && part == 0) {
+    ncut[s] = tot_a;
+    if (two) ncut2[s] = tot_b;
+  }
+  // whole rows out; slots past an atom's count hold stale values that no
+  // kernel reads (it reads positions below ncut)
+  const int ra = min(wave_max(tot_a), kCompactRowsA), rb = two ? min(wave_max(tot_b), kCompactRowsB) : 0;
+  __syncthreads();
+  const long long wb = woff[blk];
+  for (int q = threadIdx.x; q < ra * 64; q += 64 * P) ecut[wb + q] = rows_a[q >> 6][q & 63];
+  for (int q = threadIdx.x; q < rb * 64; q += 64 * P) ecut2[wb + q] = rows_b[q >> 6][q & 63];
+  __syncthreads();
+  // the parts' field sums, in part order (the row area is free again)
+  parts_reduce<kPairParts, 6>(ab, reinterpret_cast<double*>(&rows_a[0][0]));
+  if (part == 0 && valid) {
+    fd[3 * s] = (real)ab[0];
+    fd[3 * s + 1] = (real)ab[1];
+    fd[3 * s + 2] = (real)ab[2];
+    fp[3 * s] = (real)ab[3];
+    fp[3 * s + 1] = (real)ab[4];
+    fp[3 * s + 2] = (real)ab[5];
+  }
+}
+
 // HIPPO fields (src/cu/hippo/dfieldChgpen_cu1.cc, ufieldChgpen_cu1.cc,
 // include/seq/pair_field_chgpen.h): an induced dipole is a point probe, so
 // the direct field at i is the field of k's core charge (undamped) plus that
@@ -554,9 +724,35 @@ void ufield_real_t(Ctx& c, const void* ud, const void* up, void* fd, void* fp, h
   T9_LAUNCH_CHECK();
 }
 
+// The in-cutoff lists of the current coordinates and the real-space direct
+// field in one pass (AMOEBA; the HIPPO path compacts first).
+template <class real>
+void compact_dfield_real_t(Ctx& c, hipStream_t st) {
+  NbList& L = c.mlist;
+  if (!nblist_compact_buffers(c)) throw ArgError("no neighbor list to compact (internal ordering error)");
+  const bool two = L.cut2nd > 0;
+  const double c2 = L.cut * L.cut, c2b = L.cut2nd * L.cut2nd;
+  auto kern = c.ewald ? (c.uniform_thole ? k_compact_dfield<real, 1, 1> : k_compact_dfield<real, 1, 0>)
+                      : (c.uniform_thole ? k_compact_dfield<real, 0, 1> : k_compact_dfield<real, 0, 0>);
+  KScope ks_(c, KT_DFIELD_REAL, 1, st);
+  hipLaunchKernelGGL(kern, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n, L.nlist.as<int>(),
+                     L.off.as<long long>(), L.entries.as<uint32_t>(), c.pos.as<R4<real>>(), make_boxT<real>(c),
+                     (real)(c2 > 1e30 ? 1e30 : c2), (real)(two ? c2b : -1.0), L.ncut.as<int>(), L.ecut.as<uint32_t>(),
+                     two ? L.ncut2.as<int>() : nullptr, two ? L.ecut2.as<uint32_t>() : nullptr, c.rpole.as<real>(),
+                     c.jpolar.as<int>(), c.thlval.as<real>(), c.nj, (real)c.thole_value, c.scale, ewald_k<real>(c),
+                     c.field.as<real>(), c.fieldp.as<real>());
+  T9_LAUNCH_CHECK();
+  L.cut_valid = true;
+}
+
 template <class real>
 void dfield_real_t(Ctx& c, hipStream_t st) {
   const auto& L = c.mlist;
+  if (!L.cut_valid && !c.chgpen && L.cutoff > 0 && L.entries.p) {
+    compact_dfield_real_t<real>(c, st);
+    return;
+  }
+  if (!L.cut_valid) nblist_compact(c, st);
   require_cut(L);
   if (c.chgpen) {
     auto kc = c.ewald ? k_dfield_chgpen<real, 1> : k_dfield_chgpen<real, 0>;
@@ -602,7 +798,6 @@ void dfield_all(Ctx& c, const void* ua, void* fa) {
   const char* ua_d = (const char*)ua;
   char* fa_d = (char*)fa;
   if (!c.ewald) {
-    if (!c.mlist.cut_valid) nblist_compact(c);
     if (c.prec == 32) dfield_real_t<float>(c, c.stream);
     else dfield_real_t<double>(c, c.stream);
     if (ua) ufield_all(c, (void*)ua_d, (void*)(ua_d + v3), fa_d, fa_d + v3);
@@ -616,9 +811,9 @@ void dfield_all(Ctx& c, const void* ua, void* fa) {
   T9_HIP(hipEventRecord(c.ev_ufork, c.stream));
   T9_HIP(hipStreamWaitEvent(c.side2, c.ev_ufork, 0));
   // the in-cutoff lists of the current coordinates, if the caller left them:
-  // compacted on the third stream ahead of the real-space kernels, beside
-  // the PME work of the main stream (whose gathers wait for this stream)
-  if (!c.mlist.cut_valid) nblist_compact(c, c.side2);
+  // compacted on the third stream in the same pass as the real-space direct
+  // field, beside the PME work of the main stream (whose gathers wait for
+  // this stream)
   if (c.prec == 32) dfield_real_t<float>(c, c.side2);
   else dfield_real_t<double>(c, c.side2);
   T9_HIP(hipEventRecord(c.ev_ujoin, c.side2));

@@ -237,10 +237,6 @@ __device__ __forceinline__ int wave_min(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
   return __builtin_amdgcn_readfirstlane(v);
 }
-__device__ __forceinline__ int wave_max(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return __builtin_amdgcn_readfirstlane(v);
-}
 __device__ __forceinline__ double wave_minf(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   return v;
@@ -855,7 +851,6 @@ void nblist_build_t(Ctx& c) {
 // assembled in LDS (the first kCompactRowsA / kCompactRowsB rows; later rows,
 // rare, go straight to memory) and written as whole 256-byte rows: written
 // entry by entry, every store of a wavefront would touch 64 different rows.
-constexpr int kCompactRowsA = 176, kCompactRowsB = 64;
 
 template <class real>
 __global__ __launch_bounds__(64 * kPairParts) void k_compact_parts(
@@ -949,14 +944,8 @@ template <class real>
 void compact_t(Ctx& c, hipStream_t st) {
   BoxT<real> box = make_boxT<real>(c);
   NbList* L = &c.mlist;
-  if (L->cutoff <= 0 || !L->entries.p) return;
-  L->ncut.alloc(sizeof(int) * c.n);
-  L->ecut.alloc(sizeof(uint32_t) * L->capacity);
+  if (!nblist_compact_buffers(c)) return;
   const bool two = L->cut2nd > 0;
-  if (two) {
-    L->ncut2.alloc(sizeof(int) * c.n);
-    L->ecut2.alloc(sizeof(uint32_t) * L->capacity);
-  }
   const double c2 = L->cut * L->cut, c2b = L->cut2nd * L->cut2nd;
   hipLaunchKernelGGL(k_compact_parts<real>, dim3(c.nwave), dim3(64 * kPairParts), 0, st, c.n,
                      L->nlist.as<int>(), L->off.as<long long>(), L->entries.as<uint32_t>(), c.pos.as<PosRec<real>>(),
@@ -1024,6 +1013,18 @@ void nblist_build(Ctx& c) {
 
 bool nblist_needs_rebuild(Ctx& c, bool refresh) {
   return c.prec == 32 ? needs_rebuild_t<float>(c, refresh) : needs_rebuild_t<double>(c, refresh);
+}
+
+bool nblist_compact_buffers(Ctx& c) {
+  NbList* L = &c.mlist;
+  if (L->cutoff <= 0 || !L->entries.p) return false;
+  L->ncut.alloc(sizeof(int) * c.n);
+  L->ecut.alloc(sizeof(uint32_t) * L->capacity);
+  if (L->cut2nd > 0) {
+    L->ncut2.alloc(sizeof(int) * c.n);
+    L->ecut2.alloc(sizeof(uint32_t) * L->capacity);
+  }
+  return true;
 }
 
 void nblist_compact(Ctx& c, hipStream_t st) {
