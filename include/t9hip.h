@@ -184,16 +184,30 @@ typedef struct {
    * with Ewald (src/acc/amoeba/empoleewald.cpp, epolarewald.cpp), and the
    * vdW count nev (src/acc/ehal.cpp); else 0 */
   int64_t nem, nep, nev;
+  /* HIPPO charge transfer (echgtrn, src/hippo/echgtrn.cpp:73-100,
+   * src/acc/hippo/echgtrn.cpp:13-183) and, with T9_CALC_ANALYZE, its
+   * interaction count nct: pairs within chgtrn-cutoff with a nonzero
+   * energy, less the excluded (scale 0) pairs */
+  double ect;
+  int64_t nct;
+  /* HIPPO dispersion with a cutoff (edisp, src/hippo/edisp.cpp,
+   * src/acc/hippo/edisp.cpp:113-296 in its non-Ewald form) and its count
+   * ndisp with T9_CALC_ANALYZE */
+  double edisp;
+  int64_t ndisp;
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
 /* Term selection of one energy call, as energy_core selects its terms by the
  * calc flags and the RESPA time-scale group (src/energy.cpp:176-291):
  * T9_TERM_ELEC the multipole + polarization pair (emplar, with the induced
  * dipole solve), T9_TERM_VDW the buffered 14-7 vdW (evdw), T9_TERM_VALENCE
- * bond / angle / Urey-Bradley (ebond, eangle, eurey).  A term the system's
- * keywords switch off stays off.  t9_energy is all three.  A host that
- * keeps its own vdW and valence terms asks for T9_TERM_ELEC only. */
-enum { T9_TERM_ELEC = 1, T9_TERM_VDW = 2, T9_TERM_VALENCE = 4, T9_TERM_ALL = 7 };
+ * bond / angle / Urey-Bradley (ebond, eangle, eurey), T9_TERM_CHGTRN the
+ * HIPPO charge transfer (echgtrn), T9_TERM_DISP the HIPPO dispersion
+ * (edisp).  A term the system's keywords switch off
+ * stays off.  t9_energy is all of them.  A host that keeps its own vdW and
+ * valence terms asks for T9_TERM_ELEC only. */
+enum { T9_TERM_ELEC = 1, T9_TERM_VDW = 2, T9_TERM_VALENCE = 4, T9_TERM_CHGTRN = 8, T9_TERM_DISP = 16,
+       T9_TERM_ALL = 31 };
 int t9_energy_terms(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* grad);
 
 /* Device-resident coordinates and gradient: d_xyz and d_grad are device

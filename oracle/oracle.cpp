@@ -25,6 +25,7 @@
 // every pair is visited from both atoms, so each atom's sums are written by one
 // thread only.  The 3D DFT is evaluated as three separable naive DFTs.
 #include <cmath>
+#include <map>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -860,6 +861,38 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 }  // namespace
 
 // ==================================================================== C ABI
+// HIPPO dispersion damping of two Slater densities of exponents ai, ak at
+// distance r, D and dD/dr.  With d = a r, t_i = a_k^2 / (a_k^2 - a_i^2),
+// t_k = 1 - t_i and P(d, b) = b (1 + d + d^2/2) + d^3/4:
+//   D = 1 - t_i^2 P(d_i, 1 + 2 t_k) e^(-d_i) - t_k^2 P(d_k, 1 + 2 t_i) e^(-d_k);
+// for exponents within 1e-3, the limit at their mean exponent:
+//   D = 1 - (1 + d + d^2/2 + 17 d^3/96 + 5 d^4/96 + d^5/96) e^(-d).
+// The derivatives come from d/dd [P e^(-d)] = (P' - P) e^(-d).
+void damp_disp(double r, double ai, double ak, double* D, double* dD) {
+  if (std::fabs(ai - ak) > 1e-3) {
+    const double ti = ak * ak / (ak * ak - ai * ai), tk = 1 - ti;
+    const double ex[2] = {ai, ak}, tt[2] = {ti, tk}, bb[2] = {1 + 2 * tk, 1 + 2 * ti};
+    double d0 = 1, d1 = 0;
+    for (int j = 0; j < 2; ++j) {
+      const double d = ex[j] * r, bj = bb[j];
+      const double P = bj * (1 + d + 0.5 * d * d) + 0.25 * d * d * d;
+      const double Pp = bj * (1 + d) + 0.75 * d * d;
+      d0 -= tt[j] * tt[j] * P * std::exp(-d);
+      d1 -= tt[j] * tt[j] * (Pp - P) * std::exp(-d) * ex[j];
+    }
+    *D = d0;
+    *dD = d1;
+  } else {
+    const double a = 0.5 * (ai + ak), d = a * r;
+    const double cP[6] = {1, 1, 0.5, 17.0 / 96, 5.0 / 96, 1.0 / 96};
+    double P = 0, Pp = 0;
+    for (int j = 5; j >= 0; --j) P = P * d + cP[j];
+    for (int j = 5; j >= 1; --j) Pp = Pp * d + j * cP[j];
+    *D = 1 - P * std::exp(-d);
+    *dD = -(Pp - P) * std::exp(-d) * a;
+  }
+}
+
 extern "C" {
 
 void* orc_create(int n, const double* xyz, const double* pole, const int* frame, const int* zaxis,
@@ -923,6 +956,128 @@ void orc_set_chgpen(void* h, const double* pcore, const double* pval, const doub
   o->pval.assign(pval, pval + o->n);
   o->palpha.assign(palpha, palpha + o->n);
 }
+
+// HIPPO charge transfer (src/acc/hippo/echgtrn.cpp:13-183 with
+// include/seq/pair_chgtrn.h): every pair i < k within the cutoff (minimum
+// image), E = -f s (q_i exp(-a_k r) + q_k exp(-a_i r)), s the pair's
+// multipole scale, times the fifth-order switch S(r) = sum c_j r^j between
+// taper and cutoff, the polynomial fixed by S = 1 at the taper, S = 0 at the
+// cutoff and zero first and second derivatives at both.  ct is [n][2]
+// (charge, exponent).  Energy, dE/dx [n][3], virial [3][3] and the count of
+// pairs with a nonzero unscaled energy and a nonzero scale.
+void orc_chgtrn(void* h, const double* ct, double cut, double taper, double* e_out, double* grad, double* vir,
+                long long* count) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  const int n = o.n;
+  const double a = taper, b = cut, den = std::pow(b - a, 5);
+  const double c[6] = {b * b * b * (b * b - 5 * a * b + 10 * a * a) / den, -30 * a * a * b * b / den,
+                       30 * (a * b * b + a * a * b) / den, -10 * (a * a + 4 * a * b + b * b) / den,
+                       15 * (a + b) / den, -6 / den};
+  double e = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long long cnt = 0;
+  for (int q = 0; q < 3 * n; ++q) grad[q] = 0;
+  for (int i = 0; i < n; ++i)
+    for (int k = i + 1; k < n; ++k) {
+      double xr[3] = {o.xyz[3 * k] - o.xyz[3 * i], o.xyz[3 * k + 1] - o.xyz[3 * i + 1],
+                      o.xyz[3 * k + 2] - o.xyz[3 * i + 2]};
+      image(o, xr[0], xr[1], xr[2]);
+      const double r = std::sqrt(xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]);
+      if (r > cut) continue;
+      const Scale* sp = find_scale(o, i, k);
+      const double sc = sp ? sp->m : 1.0;
+      const double qi = ct[2 * i], ai = ct[2 * i + 1], qk = ct[2 * k], ak = ct[2 * k + 1];
+      const double ei = std::exp(-ai * r), ek = std::exp(-ak * r);
+      double e1 = -kCoulombF * (qi * ek + qk * ei);
+      double d1 = kCoulombF * (qi * ak * ek + qk * ai * ei);
+      if (r > taper) {
+        double S = 0, dS = 0;
+        for (int j = 5; j >= 0; --j) S = S * r + c[j];
+        for (int j = 5; j >= 1; --j) dS = dS * r + j * c[j];
+        d1 = e1 * dS + d1 * S;
+        e1 *= S;
+      }
+      if (e1 != 0 && sc != 0) ++cnt;
+      e += sc * e1;
+      const double g = sc * d1 / r;
+      for (int d = 0; d < 3; ++d) {
+        grad[3 * i + d] -= g * xr[d];
+        grad[3 * k + d] += g * xr[d];
+      }
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) v[3 * p + q] += g * xr[p] * xr[q];
+    }
+  *e_out = e;
+  for (int q = 0; q < 9; ++q) vir[q] = v[q];
+  *count = cnt;
+}
+
+// HIPPO dispersion with a cutoff (src/acc/hippo/edisp.cpp:113-296,
+// non-Ewald form, with include/seq/pair_disp.h): every pair i < k within the
+// cutoff (minimum image), E = -s C_i C_k D^2 / r^6 times the same switch as
+// charge transfer, s the pair's dispersion scale (pairs ik[q] with ds[q], 1
+// otherwise), D the damping of the two Slater densities (damp_disp).  cd is
+// [n][2] (C6, exponent).  The gradient differentiates E(r) numerically-free:
+// D and dD/dr from the closed forms, the switch from its polynomial.
+void orc_disp(void* h, const double* cd, double cut, double taper, int nds, const int* ik, const double* ds,
+              double* e_out, double* grad, double* vir, long long* count) {
+  const Oracle& o = *static_cast<Oracle*>(h);
+  const int n = o.n;
+  std::map<std::pair<int, int>, double> sc;
+  for (int q = 0; q < nds; ++q) sc[{std::min(ik[2 * q], ik[2 * q + 1]), std::max(ik[2 * q], ik[2 * q + 1])}] = ds[q];
+  const double a = taper, b = cut, den = std::pow(b - a, 5);
+  const double c[6] = {b * b * b * (b * b - 5 * a * b + 10 * a * a) / den, -30 * a * a * b * b / den,
+                       30 * (a * b * b + a * a * b) / den, -10 * (a * a + 4 * a * b + b * b) / den,
+                       15 * (a + b) / den, -6 / den};
+  double e = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long long cnt = 0;
+  for (int q = 0; q < 3 * n; ++q) grad[q] = 0;
+  for (int i = 0; i < n; ++i)
+    for (int k = i + 1; k < n; ++k) {
+      double xr[3] = {o.xyz[3 * k] - o.xyz[3 * i], o.xyz[3 * k + 1] - o.xyz[3 * i + 1],
+                      o.xyz[3 * k + 2] - o.xyz[3 * i + 2]};
+      image(o, xr[0], xr[1], xr[2]);
+      const double r = std::sqrt(xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]);
+      if (r > cut) continue;
+      auto it = sc.find({i, k});
+      const double s = it == sc.end() ? 1.0 : it->second;
+      double D, dD;
+      damp_disp(r, cd[2 * i + 1], cd[2 * k + 1], &D, &dD);
+      const double cc = cd[2 * i] * cd[2 * k];
+      double e1 = -cc * D * D / std::pow(r, 6);
+      double d1 = 6 * cc * D * D / std::pow(r, 7) - 2 * cc * D * dD / std::pow(r, 6);
+      if (r > taper) {
+        double S = 0, dS = 0;
+        for (int j = 5; j >= 0; --j) S = S * r + c[j];
+        for (int j = 5; j >= 1; --j) dS = dS * r + j * c[j];
+        d1 = e1 * dS + d1 * S;
+        e1 *= S;
+      }
+      if (e1 != 0 && s != 0) ++cnt;
+      e += s * e1;
+      const double g = s * d1 / r;
+      for (int d = 0; d < 3; ++d) {
+        grad[3 * i + d] -= g * xr[d];
+        grad[3 * k + d] += g * xr[d];
+      }
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) v[3 * p + q] += g * xr[p] * xr[q];
+    }
+  *e_out = e;
+  for (int q = 0; q < 9; ++q) vir[q] = v[q];
+  *count = cnt;
+}
+
+// HIPPO dispersion damping D(r) and dD/dr for exponents ai, ak (out[0..1]).
+void orc_damp_disp(double r, double ai, double ak, double* out) { damp_disp(r, ai, ak, out, out + 1); }
+
+// HIPPO Pauli repulsion (see repel): rp [n][3] size, exponent, valence count.
+double orc_repel(void* h, const double* rp, double cut, double taper, int nrs, const int* ik, const double* rs,
+                 double* grad, double* vir, long long* count) {
+  return repel(*static_cast<Oracle*>(h), rp, cut, taper, nrs, ik, rs, grad, vir, count);
+}
+
+// Radial series B_0..B_5 of the repulsion kernel (before the sizes).
+void orc_repel_series(double r, double ai, double ak, double* B) { repel_series(r, ai, ak, B); }
 
 void orc_set_xyz(void* h, const double* xyz) {
   Oracle* o = static_cast<Oracle*>(h);

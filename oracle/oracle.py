@@ -45,6 +45,15 @@ def _lib():
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_set_chgpen.argtypes = [ctypes.c_void_p, _d, _d, _d]
         L.orc_damp_chgpen.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]
+        L.orc_chgtrn.argtypes = [ctypes.c_void_p, _d, ctypes.c_double, ctypes.c_double, _d, _d, _d,
+                                 ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_disp.argtypes = [ctypes.c_void_p, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, _i, _d, _d, _d,
+                               _d, ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_damp_disp.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]
+        L.orc_repel.argtypes = [ctypes.c_void_p, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, _i, _d, _d,
+                                _d, ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_repel.restype = ctypes.c_double
+        L.orc_repel_series.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]
         L.orc_set_xyz.argtypes = [ctypes.c_void_p, _d]
         L.orc_get_rpole.argtypes = [ctypes.c_void_p, _d]
         L.orc_get_bsmod.argtypes = [ctypes.c_void_p, ctypes.c_int, _d]
@@ -129,6 +138,21 @@ def damp_chgpen(r, ai, ak):
     out = np.zeros(18)
     _lib().orc_damp_chgpen(float(r), float(ai), float(ak), _p(out))
     return out[:6], out[6:12], out[12:]
+
+
+def damp_disp(r, ai, ak):
+    """HIPPO dispersion damping D(r) and dD/dr for exponents ai, ak (1/Å)."""
+    out = np.zeros(2)
+    _lib().orc_damp_disp(float(r), float(ai), float(ak), _p(out))
+    return out[0], out[1]
+
+
+def repel_series(r, ai, ak):
+    """Radial series B_0..B_5 of the HIPPO repulsion kernel at r (Å) for
+    damping exponents ai, ak (1/Å), before the sizes."""
+    out = np.zeros(6)
+    _lib().orc_repel_series(float(r), float(ai), float(ak), _p(out))
+    return out
 
 
 def vdw_count(a):
@@ -333,6 +357,53 @@ class Oracle:
         _lib().orc_energy(self.h, terms, _p(ud), _p(up), _p(e5), _p(g), _p(v))
         e = dict(em=e5[0], ep=e5[1], em_real=e5[2], em_recip=e5[3], em_self=e5[4])
         return e, g, v
+
+    def chgtrn(self):
+        """HIPPO charge transfer of the system's chgtrn parameters: energy,
+        gradient dE/dx [n,3], virial [3,3] and the interaction count."""
+        s = self.sysd
+        ct = np.ascontiguousarray(np.stack([s["chgct"], s["dmpct"]], axis=1), dtype=np.float64)
+        e, g, v = np.zeros(1), np.zeros((self.n, 3)), np.zeros((3, 3))
+        cnt = ctypes.c_longlong(0)
+        _lib().orc_chgtrn(self.h, _p(ct), float(s["ct_cut"]), float(s["ct_taper"]), _p(e), _p(g), _p(v),
+                          ctypes.byref(cnt))
+        return float(e[0]), g, v, int(cnt.value)
+
+    def disp(self):
+        """HIPPO dispersion with a cutoff of the system's dispersion
+        parameters: energy, gradient dE/dx [n,3], virial [3,3] and the
+        interaction count."""
+        s = self.sysd
+        cd = np.ascontiguousarray(np.stack([s["csix"], s["adisp"]], axis=1), dtype=np.float64)
+        ik = np.ascontiguousarray(s["scale_ik"], dtype=np.int32).reshape(-1, 2)
+        ds = np.ascontiguousarray(s["scale_ds"], dtype=np.float64)
+        if len(ds) == 0:
+            ik, ds = np.zeros((1, 2), dtype=np.int32), np.zeros(1)
+            nds = 0
+        else:
+            nds = len(ds)
+        e, g, v = np.zeros(1), np.zeros((self.n, 3)), np.zeros((3, 3))
+        cnt = ctypes.c_longlong(0)
+        _lib().orc_disp(self.h, _p(cd), float(s["disp_cut"]), float(s["disp_taper"]), nds, _p(ik, _i), _p(ds),
+                        _p(e), _p(g), _p(v), ctypes.byref(cnt))
+        return float(e[0]), g, v, int(cnt.value)
+
+    def repel(self):
+        """HIPPO Pauli repulsion of the system's repulsion parameters:
+        energy, gradient dE/dx [n,3], virial [3,3] and the interaction
+        count."""
+        s = self.sysd
+        rp = np.ascontiguousarray(np.stack([s["sizpr"], s["dmppr"], s["elepr"]], axis=1), dtype=np.float64)
+        ik = np.ascontiguousarray(s["scale_ik"], dtype=np.int32).reshape(-1, 2)
+        rs = np.ascontiguousarray(s["scale_rs"], dtype=np.float64)
+        nrs = len(rs)
+        if nrs == 0:
+            ik, rs = np.zeros((1, 2), dtype=np.int32), np.zeros(1)
+        g, v = np.zeros((self.n, 3)), np.zeros((3, 3))
+        cnt = ctypes.c_longlong(0)
+        e = _lib().orc_repel(self.h, _p(rp), float(s["rep_cut"]), float(s["rep_taper"]), nrs, _p(ik, _i), _p(rs),
+                             _p(g), _p(v), ctypes.byref(cnt))
+        return float(e), g, v, int(cnt.value)
 
     def induce(self, eps=None, maxiter=None, prec=None, guess=None, peek=None, usolve=None, udiag=None):
         s = self.sysd

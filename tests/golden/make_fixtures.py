@@ -20,8 +20,13 @@ What it writes (numbers and data files only, no reference source text):
   * tests/golden/nacl_vdw_ref.json — test/nacl.cpp NaCl-1 vdW energies,
     gradients, virials and the ion separations of test/file/nacl/
   * tests/golden/c5h12acnh2.xyz, dmso.xyz, hippo19_subset.prm, hippo_ref.json —
-    the HIPPO multipole and polarization tests (test/emhippo.cpp,
-    test/ephippo.cpp, test/ref/emhippo.{1,2}.txt, ephippo.{1,2}.txt)
+    the HIPPO multipole, polarization and charge-transfer tests
+    (test/emhippo.cpp, test/ephippo.cpp, test/chgtrn.cpp, test/rephippo.cpp,
+    test/ref/emhippo.{1,2}.txt, ephippo.{1,2}.txt, chgtrn.1.txt,
+    rephippo.1.txt)
+  * tests/golden/hippo19_disp_subset.prm, hippo_disp_ref.json — the HIPPO
+    dispersion tests (test/disp.cpp with its own test/file/disp/hippo19.prm,
+    test/ref/disp.{1,2}.txt)
   * tinker-gpu_amd/params/amoeba09.prm — the subset of
     test/file/commit_6fe8e913/amoeba09.prm this path uses: the scale-factor
     header and the atom/vdw/bond/angle/ureybrad/multipole/polarize records of
@@ -316,13 +321,14 @@ def write_more():
 
 
 def write_hippo():
-    """HIPPO charge-penetration multipoles and polarization (test/emhippo.cpp,
-    test/ephippo.cpp): the coordinates of test/file/c5h12acnh2/c5h12acnh2.xyz
-    and test/file/dmso/dmso.xyz, the parameter header and the atom, multipole,
-    chgpen and polarize records of their types from
-    test/file/hippo/hippo19.prm, and the energy, count, virial and gradient of
-    test/ref/emhippo.{1,2}.txt and ephippo.{1,2}.txt (Ewald at 7 Å and
-    non-Ewald at 9 Å)."""
+    """HIPPO charge-penetration multipoles, polarization and charge transfer
+    (test/emhippo.cpp, test/ephippo.cpp, test/chgtrn.cpp): the coordinates of
+    test/file/c5h12acnh2/c5h12acnh2.xyz and test/file/dmso/dmso.xyz, the
+    parameter header and the atom, multipole, chgpen, chgtrn, repulsion and
+    polarize records of their types from test/file/hippo/hippo19.prm, and the
+    energy, count, virial and gradient of test/ref/emhippo.{1,2}.txt and
+    ephippo.{1,2}.txt (Ewald at 7 Å and non-Ewald at 9 Å), chgtrn.1.txt and
+    rephippo.1.txt."""
     types = set()
     for rel, dst in (("test/file/c5h12acnh2/c5h12acnh2.xyz", "c5h12acnh2.xyz"), ("test/file/dmso/dmso.xyz", "dmso.xyz")):
         copy_file(rel, dst)
@@ -330,7 +336,7 @@ def write_hippo():
         types |= {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5 and ln.split()[0].isdigit()}
     lines = open(os.path.join(REF, "test/file/hippo/hippo19.prm")).read().splitlines()
     classes = {int(ln.split()[2]) for ln in lines if ln.split()[:1] == ["atom"] and int(ln.split()[1]) in types}
-    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole, chgpen and polarize records of the c5h12acnh2 and dmso test systems", ""]
+    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole, chgpen, chgtrn, repulsion and polarize records of the c5h12acnh2 and dmso test systems", ""]
     i, header = 0, True
     while i < len(lines):
         t = lines[i].split()
@@ -345,7 +351,7 @@ def write_hippo():
             out.extend(lines[i:i + 5])
             i += 5
             continue
-        elif t and t[0] == "chgpen" and int(t[1]) in classes:
+        elif t and t[0] in ("chgpen", "chgtrn", "repulsion") and int(t[1]) in classes:
             out.append(lines[i])
         elif t and t[0] == "polarize" and int(t[1]) in types:
             out.append(lines[i])
@@ -354,9 +360,49 @@ def write_hippo():
     ref = {"emhippo_ewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.1.txt")),
            "emhippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/emhippo.2.txt")),
            "ephippo_ewald": parse_ref_txt(os.path.join(REF, "test/ref/ephippo.1.txt")),
-           "ephippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/ephippo.2.txt"))}
+           "ephippo_nonewald": parse_ref_txt(os.path.join(REF, "test/ref/ephippo.2.txt")),
+           "chgtrn": parse_ref_txt(os.path.join(REF, "test/ref/chgtrn.1.txt")),
+           "rephippo": parse_ref_txt(os.path.join(REF, "test/ref/rephippo.1.txt"))}
     json.dump(ref, open(os.path.join(HERE, "hippo_ref.json"), "w"), indent=0)
     return sorted(ref)
+
+
+def write_hippo_disp():
+    """HIPPO dispersion (test/disp.cpp): the parameter header and the atom and
+    dispersion records of the c5h12acnh2 types from test/file/disp/hippo19.prm,
+    and the energy, count, virial and gradient of test/ref/disp.1.txt (5 Å
+    cutoff) and disp.2.txt (Ewald dispersion, 5 Å real space)."""
+    xyz = open(os.path.join(REF, "test/file/c5h12acnh2/c5h12acnh2.xyz")).read().splitlines()
+    types = {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5 and ln.split()[0].isdigit()}
+    lines = open(os.path.join(REF, "test/file/disp/hippo19.prm")).read().splitlines()
+    classes = {int(ln.split()[2]) for ln in lines if ln.split()[:1] == ["atom"] and int(ln.split()[1]) in types}
+    out = ["# Subset of the dispersion test's HIPPO-2019 parameter set (test/file/disp/hippo19.prm): header, and",
+           "# the atom and dispersion records of the c5h12acnh2 test system", ""]
+    header = True
+    for ln in lines:
+        t = ln.split()
+        if t and t[0] == "atom":
+            header = False
+        if header and len(t) >= 1 and not ln.lstrip().startswith("#") and re.match(r"^[a-z][a-z0-9-]*$", t[0]) \
+                and len(t) <= 2:
+            out.append(ln)
+        elif t and t[0] == "atom" and int(t[1]) in types:
+            out.append(ln)
+        elif t and t[0] == "dispersion" and is_int_tok(t[1]) and int(t[1]) in classes:
+            out.append(ln)
+    open(os.path.join(HERE, "hippo19_disp_subset.prm"), "w").write("\n".join(out) + "\n")
+    ref = {"disp_cutoff": parse_ref_txt(os.path.join(REF, "test/ref/disp.1.txt")),
+           "disp_ewald": parse_ref_txt(os.path.join(REF, "test/ref/disp.2.txt"))}
+    json.dump(ref, open(os.path.join(HERE, "hippo_disp_ref.json"), "w"), indent=0)
+    return sorted(ref)
+
+
+def is_int_tok(t):
+    try:
+        int(t)
+        return True
+    except ValueError:
+        return False
 
 
 def main():
@@ -376,6 +422,7 @@ def main():
     print("aspc table:", write_aspc())
     print("nacl:", write_nacl())
     print("hippo:", write_hippo())
+    print("hippo dispersion:", write_hippo_disp())
     print("more:", write_more())
 
 

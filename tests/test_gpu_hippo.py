@@ -1,4 +1,5 @@
-"""HIPPO charge-penetration multipoles and polarization on the GPU through the C ABI.
+"""HIPPO charge-penetration multipoles, polarization, charge transfer and
+dispersion on the GPU through the C ABI.
 
 test/emhippo.cpp: pentane + acetamide (104 atoms, HIPPO-2019) in a
 35 x 30 x 25 Å box, Ewald at 7 Å (PME) and non-Ewald at 9 Å.  Against the
@@ -9,7 +10,9 @@ gradient 3e-4, virial 2e-3 / 2.5e-3 — and against the FP64 oracle (FP64:
 1e-10 relative energy, 1e-8 gradient and virial), plus the analyze-mode
 interaction count; and test/ephippo.cpp (HIPPO mutual polarization of 50
 DMSO molecules, Ewald and non-Ewald) against its goldens and the oracle's
-induced dipoles."""
+induced dipoles; test/chgtrn.cpp (HIPPO charge transfer of the same DMSO
+box) against test/ref/chgtrn.1.txt and the oracle; test/disp.cpp (HIPPO
+dispersion with a cutoff) against test/ref/disp.1.txt and the oracle."""
 import json
 import os
 
@@ -78,3 +81,59 @@ def test_ephippo_gpu(golden_dir, ewald, prec):
     assert du < 1e-5
     ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
     assert ea["nep"] == ref["count"]
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_chgtrn_gpu(golden_dir, prec):
+    """test/chgtrn.cpp (HIPPO charge transfer, 50 DMSO molecules, 7 Å): the
+    reference's tolerances (energy and gradient 1e-4, virial 5e-4, both
+    precisions), the interaction count, and the oracle (FP64: 1e-10 relative
+    energy, 1e-9 gradient and virial; FP32 positions are the fixed-point
+    records: 1e-7 relative energy, 1e-5 gradient and virial)."""
+    print(f"case chgtrn prec={prec}", flush=True)
+    from test_oracle_hippo import hippo_chgtrn_system
+    ref = json.load(open(os.path.join(golden_dir, "hippo_ref.json")))["chgtrn"]
+    s = hippo_chgtrn_system(golden_dir)
+    ctx = t9hip.Context(s, prec)
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    de, dg, dv = abs(e["esum"] - ref["energy"]), np.abs(g - rg).max(), np.abs(v - rv).max()
+    eo, go, vo, co = oracle.Oracle(s.arrays()).chgtrn()
+    print(f"  E={e['esum']:.6f} ect={e['ect']:.6f} ref={ref['energy']} |dE|={de:.2e} max|dg|={dg:.2e} "
+          f"max|dv|={dv:.2e}; vs oracle {abs(e['ect'] - eo):.2e} {np.abs(g - go).max():.2e} {np.abs(v - vo).max():.2e}")
+    assert e["ect"] == e["esum"]
+    assert de < 1e-4 and dg < 1.5e-4 and dv < 5e-4
+    tol = (1e-10, 1e-9) if prec == t9hip.T9_FP64 else (1e-7, 1e-5)
+    assert abs(e["ect"] - eo) < tol[0] * abs(eo)
+    assert np.abs(g - go).max() < tol[1] and np.abs(v - vo).max() < tol[1] * 10
+    ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert ea["nct"] == ref["count"] == co
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_disp_gpu(golden_dir, prec):
+    """test/disp.cpp EDISP-1-NONDEWALD (HIPPO dispersion, pentane +
+    acetamide, 5 Å cutoff): the reference's tolerances (energy and gradient
+    1e-4, virial 1e-3, both precisions), the interaction count, and the
+    oracle (FP64: 1e-10 relative energy, 1e-9 gradient and virial; FP32
+    positions are the fixed-point records: 1e-7 relative energy, 1e-5
+    gradient and virial)."""
+    print(f"case disp prec={prec}", flush=True)
+    from test_oracle_hippo import hippo_disp_system
+    ref = json.load(open(os.path.join(golden_dir, "hippo_disp_ref.json")))["disp_cutoff"]
+    s = hippo_disp_system(golden_dir)
+    ctx = t9hip.Context(s, prec)
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    de, dg, dv = abs(e["esum"] - ref["energy"]), np.abs(g - rg).max(), np.abs(v - rv).max()
+    eo, go, vo, co = oracle.Oracle(s.arrays()).disp()
+    print(f"  E={e['esum']:.6f} edisp={e['edisp']:.6f} ref={ref['energy']} |dE|={de:.2e} max|dg|={dg:.2e} "
+          f"max|dv|={dv:.2e}; vs oracle {abs(e['edisp'] - eo):.2e} {np.abs(g - go).max():.2e} "
+          f"{np.abs(v - vo).max():.2e}")
+    assert e["edisp"] == e["esum"]
+    assert de < 1e-4 and dg < 1.5e-4 and dv < 1e-3
+    tol = (1e-10, 1e-9) if prec == t9hip.T9_FP64 else (1e-7, 1e-5)
+    assert abs(e["edisp"] - eo) < tol[0] * abs(eo)
+    assert np.abs(g - go).max() < tol[1] and np.abs(v - vo).max() < tol[1] * 10
+    ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert ea["ndisp"] == ref["count"] == co

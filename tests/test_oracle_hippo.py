@@ -131,3 +131,143 @@ def test_ephippo_goldens(golden_dir, ewald):
     assert np.abs(g - rg).max() < 1e-4
     assert np.abs(v - rv).max() < 1e-3
     assert oracle.interaction_counts(a)[1] == ref["count"]
+
+
+HIPPO_CHGTRN_KEY = "neighbor-list\na-axis 25.0\nb-axis 25.0\nc-axis 30.0\nchgtrnterm only\nchgtrn-cutoff 7.0\n"
+
+
+def hippo_chgtrn_system(golden_dir):
+    return t9hip.System.from_files(os.path.join(golden_dir, "dmso.xyz"),
+                                   prm=os.path.join(golden_dir, "hippo19_subset.prm"), key=HIPPO_CHGTRN_KEY)
+
+
+def test_chgtrn_goldens(golden_dir):
+    """test/chgtrn.cpp: HIPPO charge transfer of 50 DMSO molecules (7 Å
+    cutoff, taper from 6.3 Å) against test/ref/chgtrn.1.txt at the reference
+    test's tolerances (energy and gradient 1e-4, virial 5e-4), and the
+    interaction count."""
+    ref = json.load(open(os.path.join(golden_dir, "hippo_ref.json")))["chgtrn"]
+    a = hippo_chgtrn_system(golden_dir).arrays()
+    assert a["chgtrn"] and a["ct_taper"] == pytest.approx(6.3)
+    e, g, v, cnt = oracle.Oracle(a).chgtrn()
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    print(f"E={e:.6f} ref={ref['energy']} count={cnt} max|dg|={np.abs(g - rg).max():.2e} "
+          f"max|dv|={np.abs(v - rv).max():.2e}")
+    assert abs(e - ref["energy"]) < 1e-4
+    assert np.abs(g - rg).max() < 1e-4
+    assert np.abs(v - rv).max() < 5e-4
+    assert cnt == ref["count"]
+    # a zero exponent in the parameters stands for 1000 (no transfer)
+    assert a["dmpct"].max() == 1000.0
+
+
+def test_chgtrn_gradient_and_virial_are_derivatives(golden_dir):
+    """The oracle's charge-transfer gradient against central differences of
+    its energy (atoms whose pairs cross the taper region included), and the
+    virial diagonal against the strain derivative (coordinates and box
+    scaled together)."""
+    s = hippo_chgtrn_system(golden_dir)
+    a = s.arrays()
+    o = oracle.Oracle(a)
+    e0, g, v, _ = o.chgtrn()
+    x0 = a["xyz"].copy()
+    h = 1e-5
+    for i in (0, 17, 101):
+        for d in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[i, d] += h
+            xm[i, d] -= h
+            o.set_xyz(xp)
+            ep = o.chgtrn()[0]
+            o.set_xyz(xm)
+            em = o.chgtrn()[0]
+            assert abs((ep - em) / (2 * h) - g[i, d]) < 1e-6 * max(1.0, abs(g[i, d]))
+    o.set_xyz(x0)
+    for d in range(3):
+        es = []
+        for eps in (h, -h):
+            b = dict(a)
+            f = np.ones(3)
+            f[d] += eps
+            b["xyz"] = x0 * f
+            b["box"] = np.asarray(a["box"]) * f
+            es.append(oracle.Oracle(b).chgtrn()[0])
+        assert abs((es[0] - es[1]) / (2 * h) - v[d, d]) < 1e-5 * max(1.0, abs(v[d, d]))
+
+
+HIPPO_DISP_KEY = "neighbor-list\na-axis 35.0\nb-axis 30.0\nc-axis 25.0\ndispersionterm only\ndisp-cutoff 5.0\n"
+
+
+def hippo_disp_system(golden_dir):
+    return t9hip.System.from_files(os.path.join(golden_dir, "c5h12acnh2.xyz"),
+                                   prm=os.path.join(golden_dir, "hippo19_disp_subset.prm"), key=HIPPO_DISP_KEY)
+
+
+def test_disp_goldens(golden_dir):
+    """test/disp.cpp EDISP-1-NONDEWALD: HIPPO dispersion of pentane +
+    acetamide with a 5 Å cutoff (taper from 4.5 Å), against test/ref/disp.1.txt
+    at the reference test's tolerances (energy and gradient 1e-4, virial
+    1e-3), and the interaction count."""
+    ref = json.load(open(os.path.join(golden_dir, "hippo_disp_ref.json")))["disp_cutoff"]
+    a = hippo_disp_system(golden_dir).arrays()
+    assert a["disp"] and a["disp_taper"] == pytest.approx(4.5)
+    e, g, v, cnt = oracle.Oracle(a).disp()
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    print(f"E={e:.6f} ref={ref['energy']} count={cnt} max|dg|={np.abs(g - rg).max():.2e} "
+          f"max|dv|={np.abs(v - rv).max():.2e}")
+    assert abs(e - ref["energy"]) < 1e-4
+    assert np.abs(g - rg).max() < 1e-4
+    assert np.abs(v - rv).max() < 1e-3
+    assert cnt == ref["count"]
+
+
+@pytest.mark.parametrize("ai,ak", [(4.5439, 4.5439), (4.3225, 4.5439), (3.1, 4.4)])
+def test_disp_damping_limits(ai, ak):
+    """The dispersion damping goes to 1 at long range, vanishes as r^3 at
+    short range (so C6 D^2 / r^6 stays finite; for equal exponents
+    D ~ -(a r)^3 / 96, from e^-d times the polynomial, whose d^3 coefficient
+    17/96 exceeds the exponential series' 16/96), and its derivative is dD/dr
+    (central differences)."""
+    D, _ = oracle.damp_disp(40.0, ai, ak)
+    assert abs(D - 1) < 1e-12
+    r1, r2 = 2e-3, 4e-3
+    q1 = oracle.damp_disp(r1, ai, ak)[0] / r1 ** 3
+    q2 = oracle.damp_disp(r2, ai, ak)[0] / r2 ** 3
+    assert abs(q1 / q2 - 1) < 1e-2
+    if ai == ak:
+        assert abs(q1 / (-ai ** 3 / 96) - 1) < 1e-3
+    for r in (0.8, 1.9, 3.7):
+        h = 1e-5
+        d = (oracle.damp_disp(r + h, ai, ak)[0] - oracle.damp_disp(r - h, ai, ak)[0]) / (2 * h)
+        assert abs(d - oracle.damp_disp(r, ai, ak)[1]) < 1e-8
+
+
+def test_disp_two_exponent_form_meets_equal_form():
+    """Just beyond the 1e-3 switch the two-exponent damping equals the
+    equal-exponent form at the mean exponent to second order."""
+    a = 4.3
+    for r in (0.9, 2.0, 3.3):
+        two = oracle.damp_disp(r, a - 6e-4, a + 6e-4)[0]
+        one = oracle.damp_disp(r, a - 4e-4, a + 4e-4)[0]
+        assert abs(two - one) < 1e-6
+
+
+def test_disp_gradient_is_derivative(golden_dir):
+    """The oracle's dispersion gradient against central differences of its
+    energy (atoms with pairs in the switching region among them)."""
+    a = hippo_disp_system(golden_dir).arrays()
+    o = oracle.Oracle(a)
+    _, g, _, _ = o.disp()
+    x0 = a["xyz"].copy()
+    h = 1e-4  # the energy's rounding (~1e-12) limits smaller steps
+    for i in (0, 9, 60):
+        for d in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[i, d] += h
+            xm[i, d] -= h
+            o.set_xyz(xp)
+            ep = o.disp()[0]
+            o.set_xyz(xm)
+            em = o.disp()[0]
+            assert abs((ep - em) / (2 * h) - g[i, d]) < 1e-6 * max(1.0, abs(g[i, d]))
+    o.set_xyz(x0)

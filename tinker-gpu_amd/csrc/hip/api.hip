@@ -87,6 +87,21 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   c.aewald = s.opt.ewald_alpha;
   c.cut = c.ewald ? s.opt.ewald_cut : s.opt.mpole_cut;
   const bool elec = s.opt.use_mpole || s.opt.use_polar;
+  // charge transfer walks the multipole list, whose radius covers the
+  // largest cutoff of the terms on it; the in-cutoff compaction keeps the
+  // electrostatic cutoff (or the list's, without the electrostatic terms)
+  c.list_cut = elec ? c.cut : 0.0;
+  if (s.chgtrn) {
+    if (!(s.opt.ct_cut > 0) || s.opt.ct_taper > s.opt.ct_cut) throw ArgError("bad chgtrn-cutoff or chgtrn-taper");
+    c.list_cut = std::max(c.list_cut, s.opt.ct_cut);
+  }
+  if (s.repel) throw ArgError("HIPPO repulsion is not evaluated on the device yet: switch it off (repulsionterm none)");
+  if (s.disp) {
+    if (!(s.opt.disp_cut > 0) || s.opt.disp_taper > s.opt.disp_cut) throw ArgError("bad disp-cutoff or disp-taper");
+    c.list_cut = std::max(c.list_cut, s.opt.disp_cut);
+  }
+  if (!elec && c.list_cut > 0) c.cut = c.list_cut;
+  if (elec) c.list_cut = std::max(c.list_cut, c.cut);
   if (elec && c.periodic && !c.ewald && c.cut > 1e20)
     throw ArgError("periodic systems need ewald or a finite mpole-cutoff");
   if (!c.periodic && c.ewald) throw ArgError("ewald requires a periodic box");
@@ -133,6 +148,30 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     upload(c.chgpen_o, cp, st);
     T9_HIP(hipStreamSynchronize(st));  // cp is a local host buffer
   }
+  c.use_chgtrn = s.chgtrn;
+  if (c.use_chgtrn) {
+    c.ct_cut = s.opt.ct_cut;
+    c.ct_taper = s.opt.ct_taper;
+    std::vector<double> ct(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      ct[2 * i] = s.chgct[i];
+      ct[2 * i + 1] = s.dmpct[i];
+    }
+    upload(c.chgtrn_o, ct, st);
+    T9_HIP(hipStreamSynchronize(st));  // ct is a local host buffer
+  }
+  c.use_disp = s.disp;
+  if (c.use_disp) {
+    c.disp_cut = s.opt.disp_cut;
+    c.disp_taper = s.opt.disp_taper;
+    std::vector<double> cd(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      cd[2 * i] = s.csix[i];
+      cd[2 * i + 1] = s.adisp[i];
+    }
+    upload(c.disp_o, cd, st);
+    T9_HIP(hipStreamSynchronize(st));  // cd is a local host buffer
+  }
   c.nj = s.njpolar;
   c.uniform_thole = !s.thlval.empty();
   c.thole_value = s.thlval.empty() ? 0.0 : s.thlval[0];
@@ -146,11 +185,11 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   }
 
   // ---- scale classes and the exclusion CSR (both directions)
-  std::map<std::tuple<double, double, double, double>, int> classes;
-  classes[{1.0, 1.0, 1.0, 1.0}] = 0;
+  std::map<std::tuple<double, double, double, double, double>, int> classes;
+  classes[{1.0, 1.0, 1.0, 1.0, 1.0}] = 0;
   std::vector<std::vector<std::pair<int, int>>> ex(n);
   for (const auto& p : s.scales) {
-    auto key = std::make_tuple(p.m, p.d, p.p, p.u);
+    auto key = std::make_tuple(p.m, p.d, p.p, p.u, p.ds);
     auto it = classes.find(key);
     int code;
     if (it == classes.end()) {
@@ -164,13 +203,14 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     ex[p.k].push_back({p.i, code});
   }
   c.ncodes = (int)classes.size();
-  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = 1.0;
+  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = c.scale.ds[k] = 1.0;
   for (auto& kv : classes) {
     int k = kv.second;
     c.scale.m[k] = std::get<0>(kv.first);
     c.scale.d[k] = std::get<1>(kv.first);
     c.scale.p[k] = std::get<2>(kv.first);
     c.scale.u[k] = std::get<3>(kv.first);
+    c.scale.ds[k] = std::get<4>(kv.first);
   }
   std::vector<int> eoff(n + 1, 0), ej, ecode;
   for (int i = 0; i < n; ++i) {
@@ -219,7 +259,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
 
   // ---- neighbor-list geometry
   const double buf = s.opt.list_buffer;
-  c.mlist.cutoff = c.cut < 1e20 ? c.cut + buf : 1e30;
+  c.mlist.cutoff = c.list_cut > 0 && c.list_cut < 1e20 ? c.list_cut + buf : (c.cut < 1e20 ? c.cut + buf : 1e30);
   const bool sparse = s.opt.pcg_prec && s.opt.usolve_cut > 0;
   // the preconditioner's pairs (usolve cutoff) are a second compaction of the
   // multipole list, which covers them at any time between rebuilds
@@ -457,7 +497,7 @@ int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad, bool grad_o
     T9_HIP(hipStreamSynchronize(c.side));
     c.vdw_pending = false;
   }
-  if (terms & (TERM_ELEC | TERM_VDW)) {
+  if (terms & (TERM_ELEC | TERM_VDW | TERM_CHGTRN | TERM_DISP)) {
     c.mlist.cut_valid = c.ulist.cut_valid = false;
     c.bins_valid = false;
     c.rpole_valid = false;
@@ -495,7 +535,9 @@ void fill_energy_out(const double* e, int vers, int iters, t9_energy_out* out) {
   out->ebond = e[EP_EBOND];
   out->eangle = e[EP_EANGLE];
   out->eurey = e[EP_EUREY];
-  out->esum = out->em + out->ep + out->evdw + out->ebond + out->eangle + out->eurey;
+  out->ect = e[EP_ECT];
+  out->edisp = e[EP_EDISP];
+  out->esum = out->em + out->ep + out->evdw + out->ebond + out->eangle + out->eurey + out->ect + out->edisp;
   const double v6[6] = {e[EP_VXX], e[EP_VYY], e[EP_VZZ], e[EP_VXY], e[EP_VXZ], e[EP_VYZ]};
   const double v9[9] = {v6[0], v6[3], v6[4], v6[3], v6[1], v6[5], v6[4], v6[5], v6[2]};
   for (int k = 0; k < 9; ++k) out->vir[k] = (vers & T9_CALC_VIRIAL) ? v9[k] : 0.0;
@@ -780,16 +822,20 @@ int energy_common(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* 
   if (terms & ~t9::TERM_ALL) throw t9::ArgError("unknown term bits");
   double e[16];
   const int iters = t9::evaluate(c, vers, terms, e, (vers & T9_CALC_GRAD) ? grad : nullptr, grad_on_device);
-  long long nem = 0, nep = 0, nev = 0;
+  long long nem = 0, nep = 0, nev = 0, nct = 0, ndisp = 0;
   if (vers & T9_CALC_ANALYZE) {
     if (terms & t9::TERM_ELEC) t9::count_interactions(c, &nem, &nep);
     if (terms & t9::TERM_VDW) nev = t9::vdw_count(c);
+    if (terms & t9::TERM_CHGTRN) nct = t9::chgtrn_count(c);
+    if (terms & t9::TERM_DISP) ndisp = t9::disp_count(c);
   }
   if (out) {
     t9::fill_energy_out(e, vers, iters, out);
     out->nem = nem;
     out->nep = nep;
     out->nev = nev;
+    out->nct = nct;
+    out->ndisp = ndisp;
   }
   return T9_OK;
 }

@@ -145,6 +145,17 @@ struct Ctx {
   // pair kernel
   bool chgpen = false;
   DBuf chgpen_o;
+  // HIPPO charge transfer: caller order [n][2] (charge, damping exponent),
+  // evaluated over the multipole list within ct_cut, tapered from ct_taper
+  bool use_chgtrn = false;
+  double ct_cut = 0, ct_taper = 0;
+  // HIPPO dispersion: caller order [n][2] (C6, damping exponent), over the
+  // multipole list within disp_cut, tapered from disp_taper
+  bool use_disp = false;
+  double disp_cut = 0, disp_taper = 0;
+  DBuf disp_o;
+  double list_cut = 0;  // largest cutoff of the terms on the multipole list
+  DBuf chgtrn_o;
   DBuf jpolar_o;                 // int [n]
   DBuf excl_off, excl_j, excl_code;  // CSR of scaled partners (both directions)
   DBuf thlval;                   // real [nj*nj]
@@ -285,7 +296,7 @@ struct Ctx {
 // Per-block partial rows of the energy/virial reductions.
 enum {
   EP_MREAL = 0, EP_MRECIP, EP_MSELF, EP_POLAR, EP_VXX, EP_VYY, EP_VZZ, EP_VXY, EP_VXZ, EP_VYZ,
-  EP_EVDW, EP_EBOND, EP_EANGLE, EP_EUREY, EP_N
+  EP_EVDW, EP_EBOND, EP_EANGLE, EP_EUREY, EP_ECT, EP_EDISP, EP_N
 };
 constexpr int kEnergyRow = 16;
 
@@ -374,7 +385,7 @@ void vdw_finish(Ctx& c);                                             // site gra
 void valence_run(Ctx& c, double* gval_caller, bool energy);
 // Term groups of one evaluation: the electrostatics (multipole + polarization)
 // and vdW are the slow group of the RESPA split, valence the fast group.
-enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_ALL = 7 };
+enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_CHGTRN = 8, TERM_DISP = 16, TERM_ALL = 31 };
 // vers bit 2: virial; out16[EP_*] energies / virial; sorted-order dE/dx of
 // the electrostatics and vdW in c.egrad, caller-order valence dE/dx in
 // c.g_fast; grad_orig (host, caller order, or null) receives their sum
@@ -396,6 +407,13 @@ void md_kick(Ctx& c, double h1, const double* g1, double h2, const double* g2);
 void md_drift(Ctx& c, double h);          // x += h v
 void count_interactions(Ctx& c, long long* nem, long long* nep);
 long long vdw_count(Ctx& c);  // analyze-mode vdW count (0 without the vdW term)
+// HIPPO charge transfer: energy rows, virial, sorted egrad += (after the
+// electrostatic terms have set egrad), and the analyze-mode count
+void chgtrn_run(Ctx& c, bool virial);
+long long chgtrn_count(Ctx& c);
+// HIPPO dispersion (non-Ewald), the same contract
+void disp_run(Ctx& c, bool virial);
+long long disp_count(Ctx& c);
 void respa_inner(Ctx& c, double* gfast, double h, int nrep);  // drift + nrep x (fast forces, kick, drift)
 bool respa_fused(const Ctx& c);
 void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h

@@ -532,6 +532,180 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<
   block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
 }
 
+// Charge transfer and dispersion (HIPPO) share one pair-kernel skeleton:
+// owner computes over the buffered multipole list (both directions), each
+// atom sums its own gradient, the energy and virial are halved; FP64 pair
+// arithmetic on the exact FP64 displacement of the position records, in
+// both precisions; the fifth-order switch S(x) = 1 - x^3 (10 - 15 x + 6 x^2),
+// x = (r - taper) / (cut - taper) (S = 1, S' = S'' = 0 at the taper; S = 0,
+// S' = S'' = 0 at the cutoff) between taper and cutoff.
+//
+// Charge transfer (src/acc/hippo/echgtrn.cpp:13-183, src/cu/hippo/
+// echgtrn_cu1.cc with include/seq/pair_chgtrn.h): E = -f s (q_i e^(-a_k r) +
+// q_k e^(-a_i r)), f the Coulomb constant and s the pair's multipole scale.
+//
+// Dispersion with a cutoff (src/acc/hippo/edisp.cpp:113-296 in its non-Ewald
+// form, include/seq/pair_disp.h): E = -s C_i C_k D(r)^2 / r^6, s the pair's
+// dispersion scale and D the damping of two Slater densities: with
+// d_i = a_i r, t_i = a_k^2 / (a_k^2 - a_i^2), t_k = 1 - t_i,
+//   D = 1 - t_i^2 P(d_i, 2 t_k + 1) e^(-d_i) - t_k^2 P(d_k, 2 t_i + 1) e^(-d_k),
+//   P(d, b) = b (1 + d + d^2 / 2) + d^3 / 4,
+// and for exponents within 1e-3 the limit at their mean a, d = a r:
+//   D = 1 - (1 + d + d^2/2 + 17 d^3/96 + 5 d^4/96 + d^5/96) e^(-d).
+struct HippoPairArgs {
+  int n;
+  const int* nlist;
+  const long long* woff;
+  const uint32_t* entries;
+  const int* perm;
+  const double* par;  // caller order [n][2]
+  double sc[kMaxCodes];
+  double off2, taper, inv_w;
+  int slot;           // energy row slot
+  double* grad;       // sorted [n][3], +=
+  double* rows;
+};
+
+__device__ __forceinline__ void hippo_switch(double r, double taper, double inv_w, double& e, double& de) {
+  if (r > taper) {
+    const double x = (r - taper) * inv_w;
+    const double S = 1 - x * x * x * (10 + x * (6 * x - 15));
+    const double dS = -30 * x * x * (1 - x) * (1 - x) * inv_w;
+    de = e * dS + de * S;
+    e *= S;
+  }
+}
+
+__device__ __forceinline__ double chgtrn_pair(double r, double qi, double ai, double qk, double ak, double f,
+                                              double taper, double inv_w, double& de) {
+  const double ei = exp(-ai * r), ek = exp(-ak * r);
+  double e = -f * (qi * ek + qk * ei);
+  de = f * (qi * ak * ek + qk * ai * ei);
+  hippo_switch(r, taper, inv_w, e, de);
+  return e;
+}
+
+__device__ __forceinline__ double disp_pair(double r, double ci, double ai, double ck, double ak, double s,
+                                            double taper, double inv_w, double& de) {
+  double D, dD;  // damping and its r derivative
+  if (fabs(ai - ak) > 1e-3) {
+    const double di = ai * r, dk = ak * r, ei = exp(-di), ek = exp(-dk);
+    const double ti = ak * ak / ((ak + ai) * (ak - ai)), tk = 1 - ti;
+    const double bi = 2 * tk + 1, bk = 2 * ti + 1;
+    const double Pi = bi * (1 + di + 0.5 * di * di) + 0.25 * di * di * di;
+    const double Pk = bk * (1 + dk + 0.5 * dk * dk) + 0.25 * dk * dk * dk;
+    D = 1 - ti * ti * Pi * ei - tk * tk * Pk * ek;
+    dD = ti * ti * ai * di * di * (0.25 * (di - 1) + tk) * ei + tk * tk * ak * dk * dk * (0.25 * (dk - 1) + ti) * ek;
+  } else {
+    const double a = 0.5 * (ai + ak), d = a * r, ed = exp(-d);
+    const double P = 1 + d * (1 + d * (0.5 + d * (17.0 / 96 + d * (5.0 / 96 + d / 96))));
+    D = 1 - P * ed;
+    dD = a * ed * d * d * (d * d * d - 3 * d - 3) / 96;
+  }
+  const double r2 = r * r, rr6 = 1 / (r2 * r2 * r2);
+  double e = -ci * ck * rr6;
+  de = (-6 * e / r) * D * D + 2 * e * D * dD;
+  e *= D * D;
+  hippo_switch(r, taper, inv_w, e, de);
+  de *= s;
+  return e * s;
+}
+
+// KIND 0: charge transfer (scale m); 1: dispersion (scale ds)
+template <int KIND>
+__device__ __forceinline__ double hippo_pair(double r, const double* pi, const double* pk, double s, const HippoPairArgs& A,
+                                             double& de) {
+  if (KIND == 0) return chgtrn_pair(r, pi[0], pi[1], pk[0], pk[1], kCoul * s, A.taper, A.inv_w, de);
+  return disp_pair(r, pi[0], pi[1], pk[0], pk[1], s, A.taper, A.inv_w, de);
+}
+
+template <class real, bool VIR, int KIND>
+__global__ __launch_bounds__(64 * kPairParts) void k_hippo_pair(HippoPairArgs A, const PosRec<real>* __restrict__ pos,
+                                                                BoxT<real> box) {
+  __shared__ double sh[parts_lds<kPairParts, 10>()];
+  const int part = threadIdx.x >> 6;
+  const int blk = xcd_block();
+  const int s = blk * 64 + (threadIdx.x & 63);
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // e, g[3], v[6]
+  if (s < A.n) {
+    const PosRec<real> pi = pos[s];
+    const int oi = A.perm[s];
+    const double pari[2] = {A.par[2 * oi], A.par[2 * oi + 1]};
+    const int len = A.nlist[s];
+    const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
+    for (int kk = part; kk < len; kk += kPairParts) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const double sc = A.sc[code];
+      if (sc == 0) continue;
+      double R[3];
+      pair_disp64(pos[k], pi, box, R);  // x_i - x_k
+      const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      if (!(r2 <= A.off2)) continue;
+      const int ok = A.perm[k];
+      const double park[2] = {A.par[2 * ok], A.par[2 * ok + 1]};
+      const double r = sqrt(r2);
+      double de;
+      const double e = hippo_pair<KIND>(r, pari, park, sc, A, de);
+      const double g = de / r;
+      acc[0] += e;
+      for (int d = 0; d < 3; ++d) acc[1 + d] += g * R[d];
+      if (VIR) {
+        acc[4] += g * R[0] * R[0];
+        acc[5] += g * R[1] * R[1];
+        acc[6] += g * R[2] * R[2];
+        acc[7] += g * R[0] * R[1];
+        acc[8] += g * R[0] * R[2];
+        acc[9] += g * R[1] * R[2];
+      }
+    }
+  }
+  parts_reduce<kPairParts, 10>(acc, sh);
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  if (part == 0) {
+    if (s < A.n)
+      for (int d = 0; d < 3; ++d) A.grad[3 * (size_t)s + d] += acc[1 + d];
+    red[A.slot] = 0.5 * acc[0];
+    for (int q = 0; q < 6; ++q) red[EP_VXX + q] = 0.5 * acc[4 + q];
+  }
+  block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
+}
+
+// Analyze-mode count (the reference's nct / ndisp): pairs within the cutoff,
+// each once, whose unscaled energy is not zero, less the pairs of scale 0.
+template <class real, int KIND>
+__global__ __launch_bounds__(256) void k_hippo_count(HippoPairArgs A, const PosRec<real>* __restrict__ pos,
+                                                     BoxT<real> box, unsigned long long* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cnt = 0;
+  if (s < A.n) {
+    const PosRec<real> pi = pos[s];
+    const int oi = A.perm[s];
+    const double pari[2] = {A.par[2 * oi], A.par[2 * oi + 1]};
+    const int len = A.nlist[s];
+    const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
+    for (int kk = 0; kk < len; ++kk) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      if (k <= s) continue;
+      const int code = (int)(ent & (kMaxCodes - 1));
+      if (A.sc[code] == 0) continue;
+      double R[3];
+      pair_disp64(pos[k], pi, box, R);
+      const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      if (!(r2 <= A.off2)) continue;
+      const int ok = A.perm[k];
+      const double park[2] = {A.par[2 * ok], A.par[2 * ok + 1]};
+      double de;
+      if (hippo_pair<KIND>(sqrt(r2), pari, park, 1.0, A, de) != 0) ++cnt;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, cnt);
+}
+
 // ------------------------------------------------------------ PME
 
 struct GridGeom {
@@ -1244,6 +1418,8 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   } else {
     T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
   }
+  if ((terms & TERM_CHGTRN) && c.use_chgtrn) chgtrn_run(c, want_v);
+  if ((terms & TERM_DISP) && c.use_disp) disp_run(c, want_v);
   if (c.vdw_pending) {  // join the vdW pair kernel of the second stream
     T9_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     vdw_finish(c);
@@ -1313,7 +1489,7 @@ void energy_setup(Ctx& c, const System& s) {
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
   const long long nterm = (long long)(s.bond_kr.size() + s.angle_kt.size() + s.urey_kr.size()) / 2;
   // list loops: one row per 64 atoms; valence: one row per 256 terms
-  c.ecap = (int)(9LL * nb + 2LL * ceil_div(n, 64) + (nterm + 255) / 256 + lines + 16);
+  c.ecap = (int)(9LL * nb + 4LL * ceil_div(n, 64) + (nterm + 255) / 256 + lines + 16);
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1383,6 +1559,78 @@ void count_interactions(Ctx& c, long long* nem, long long* nep) {
   const long long self = c.ewald ? c.n : 0;
   *nem = c.opt.use_mpole ? (long long)h[0] + self : 0;
   *nep = c.opt.use_polar ? (long long)h[1] + self : 0;
+}
+
+namespace {
+HippoPairArgs hippo_args(Ctx& c, int kind) {
+  const auto& L = c.mlist;
+  HippoPairArgs A;
+  A.n = c.n;
+  A.nlist = L.nlist.as<int>();
+  A.woff = L.off.as<long long>();
+  A.entries = L.entries.as<uint32_t>();
+  A.perm = c.perm.as<int>();
+  const double cut = kind == 0 ? c.ct_cut : c.disp_cut, taper = kind == 0 ? c.ct_taper : c.disp_taper;
+  A.par = kind == 0 ? c.chgtrn_o.as<double>() : c.disp_o.as<double>();
+  for (int k = 0; k < kMaxCodes; ++k) A.sc[k] = k == 0 ? 1.0 : (kind == 0 ? c.scale.m[k] : c.scale.ds[k]);
+  A.off2 = cut * cut;
+  A.taper = taper;
+  A.inv_w = cut > taper ? 1.0 / (cut - taper) : 0.0;
+  A.slot = kind == 0 ? EP_ECT : EP_EDISP;
+  A.grad = c.egrad.as<double>();
+  A.rows = nullptr;
+  return A;
+}
+
+template <class real, int KIND>
+void hippo_run_t(Ctx& c, bool vir) {
+  HippoPairArgs A = hippo_args(c, KIND);
+  A.rows = energy_rows(c, c.nwave);
+  const PosRec<real>* pos = c.pos.as<PosRec<real>>();
+  const BoxT<real> box = make_boxT<real>(c);
+  if (vir)
+    hipLaunchKernelGGL((k_hippo_pair<real, true, KIND>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A, pos, box);
+  else
+    hipLaunchKernelGGL((k_hippo_pair<real, false, KIND>), dim3(c.nwave), dim3(64 * kPairParts), 0, c.stream, A, pos,
+                       box);
+  T9_LAUNCH_CHECK();
+}
+
+template <class real, int KIND>
+long long hippo_count_t(Ctx& c) {
+  DBuf out;
+  out.alloc(sizeof(unsigned long long));
+  T9_HIP(hipMemsetAsync(out.p, 0, sizeof(unsigned long long), c.stream));
+  hipLaunchKernelGGL((k_hippo_count<real, KIND>), dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream,
+                     hippo_args(c, KIND), c.pos.as<PosRec<real>>(), make_boxT<real>(c), out.as<unsigned long long>());
+  T9_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  T9_HIP(hipMemcpyAsync(&h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  return (long long)h;
+}
+}  // namespace
+
+// HIPPO charge transfer and dispersion after the electrostatic terms (which
+// set egrad).
+void chgtrn_run(Ctx& c, bool vir) {
+  if (c.prec == 32) hippo_run_t<float, 0>(c, vir);
+  else hippo_run_t<double, 0>(c, vir);
+}
+
+long long chgtrn_count(Ctx& c) {
+  if (!c.use_chgtrn) return 0;
+  return c.prec == 32 ? hippo_count_t<float, 0>(c) : hippo_count_t<double, 0>(c);
+}
+
+void disp_run(Ctx& c, bool vir) {
+  if (c.prec == 32) hippo_run_t<float, 1>(c, vir);
+  else hippo_run_t<double, 1>(c, vir);
+}
+
+long long disp_count(Ctx& c) {
+  if (!c.use_disp) return 0;
+  return c.prec == 32 ? hippo_count_t<float, 1>(c) : hippo_count_t<double, 1>(c);
 }
 
 void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig, bool grad_on_device) {

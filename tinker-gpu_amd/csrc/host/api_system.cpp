@@ -84,6 +84,21 @@ int t9_system_get_f64(const t9_system* sys, const char* name, double* out) {
     else if (k == "pcore") v = &s.pcore;
     else if (k == "pval") v = &s.pval;
     else if (k == "palpha") v = &s.palpha;
+    else if (k == "chgct") v = &s.chgct;
+    else if (k == "dmpct") v = &s.dmpct;
+    else if (k == "csix") v = &s.csix;
+    else if (k == "adisp") v = &s.adisp;
+    else if (k == "sizpr") v = &s.sizpr;
+    else if (k == "dmppr") v = &s.dmppr;
+    else if (k == "elepr") v = &s.elepr;
+    else if (k == "scale_rs") {  // repulsion scale of each pair of t9_system_get_scales
+      for (size_t j = 0; j < s.scales.size(); ++j) out[j] = s.scales[j].rs;
+      return T9_OK;
+    }
+    else if (k == "scale_ds") {  // dispersion scale of each pair of t9_system_get_scales
+      for (size_t j = 0; j < s.scales.size(); ++j) out[j] = s.scales[j].ds;
+      return T9_OK;
+    }
     else if (k == "mass") v = &s.mass;
     else if (k == "thlval") v = &s.thlval;
     else if (k == "vdw_rad") v = &s.vdw_rad;
@@ -163,6 +178,15 @@ int t9_system_get_scalar(const t9_system* sys, const char* name, double* out) {
     else if (k == "use_mpole") v = o.use_mpole;
     else if (k == "use_polar") v = o.use_polar;
     else if (k == "chgpen") v = s.chgpen;
+    else if (k == "chgtrn") v = s.chgtrn;
+    else if (k == "ct_cut") v = o.ct_cut;
+    else if (k == "ct_taper") v = o.ct_taper;
+    else if (k == "disp") v = s.disp;
+    else if (k == "disp_cut") v = o.disp_cut;
+    else if (k == "disp_taper") v = o.disp_taper;
+    else if (k == "repel") v = s.repel;
+    else if (k == "rep_cut") v = o.rep_cut;
+    else if (k == "rep_taper") v = o.rep_taper;
     else if (k == "mpole_cut") v = o.mpole_cut;
     else if (k == "vdw_cut") v = o.vdw_cut;
     else if (k == "vdw_taper") v = o.vdw_taper;

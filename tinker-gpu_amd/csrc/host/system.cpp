@@ -137,7 +137,8 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
     // that start with a record keyword (the text of a file's header) are not
     // records, and Tinker skips them too
     static const std::set<std::string> kRecords = {"atom", "vdw", "bond", "angle", "ureybrad", "multipole",
-                                                    "polarize", "polpair", "chgpen"};
+                                                    "polarize", "polpair", "chgpen", "chgtrn", "dispersion",
+                                                    "repulsion"};
     if (kRecords.count(key) && (tok.size() < 2 || !is_int(tok[1]))) continue;
     auto scale_kw = [&](const char* prefix, double* arr, int lo, int hi, const char* suffix) -> bool {
       for (int j = lo; j <= hi; ++j) {
@@ -155,7 +156,9 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
                scale_kw("polar-", ff.p_scale, 2, 5, "-scale") ||
                scale_kw("polar-", ff.p_iscale, 2, 5, "-intra") ||
                scale_kw("vdw-", ff.v_scale, 2, 5, "-scale") ||
-               scale_kw("induce-", ff.w_scale, 2, 5, "-scale")) {
+               scale_kw("induce-", ff.w_scale, 2, 5, "-scale") ||
+               scale_kw("disp-", ff.ds_scale, 2, 5, "-scale") ||
+               scale_kw("rep-", ff.rs_scale, 2, 5, "-scale")) {
     } else if (key.rfind("direct-1", 0) == 0 && key.size() == 15 && tok.size() >= 2) {
       int j = key[8] - '0';  // direct-1j-scale
       if (j >= 1 && j <= 4) ff.d_scale[j] = to_d(tok[1]);
@@ -249,6 +252,15 @@ void parse_prm_lines(const std::vector<std::string>& lines, ForceField& ff) {
       for (; j < tok.size(); ++j)
         if (is_int(tok[j])) p.group_types.push_back(std::stoi(tok[j]));
       ff.polarize[t] = p;
+    } else if (key == "repulsion" && tok.size() >= 5) {
+      // repulsion  class  size  alpha  valence
+      ff.repel[std::stoi(tok[1])] = std::make_tuple(to_d(tok[2]), to_d(tok[3]), to_d(tok[4]));
+    } else if (key == "dispersion" && tok.size() >= 4) {
+      // dispersion  class  C6  alpha
+      ff.disp[std::stoi(tok[1])] = {to_d(tok[2]), to_d(tok[3])};
+    } else if (key == "chgtrn" && tok.size() >= 4) {
+      // chgtrn  class  charge  alpha
+      ff.chgtrn[std::stoi(tok[1])] = {to_d(tok[2]), to_d(tok[3])};
     } else if (key == "chgpen" && tok.size() >= 4) {
       // chgpen  class  core-charge  alpha
       ff.chgpen[std::stoi(tok[1])] = {to_d(tok[2]), to_d(tok[3])};
@@ -304,7 +316,8 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
   std::istringstream in(text);
   std::string line;
   Options& o = s.opt;
-  bool mpole_only = false, polar_only = false, vdw_only = false, valence_only = false;
+  bool mpole_only = false, polar_only = false, vdw_only = false, valence_only = false, ct_only = false,
+       disp_only = false, rep_only = false;
   int other_off = 0, only_mask = 0;
   while (std::getline(in, line)) {
     auto tok = split(line);
@@ -370,6 +383,29 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
     } else if (k == "polarizeterm") {
       if (v1 == "only") polar_only = true;
       else o.use_polar = (v1 != "none");
+    } else if (k == "repulsionterm") {
+      if (v1 == "only") rep_only = true;
+      else o.use_repel = (v1 != "none");
+    } else if (k == "repuls-cutoff") {
+      o.rep_cut = to_d(tok[1]);
+    } else if (k == "repuls-taper") {
+      o.rep_taper = to_d(tok[1]);
+    } else if (k == "dispersionterm") {
+      if (v1 == "only") disp_only = true;
+      else o.use_disp = (v1 != "none");
+    } else if (k == "disp-cutoff") {
+      o.disp_cut = to_d(tok[1]);
+    } else if (k == "disp-taper") {
+      o.disp_taper = to_d(tok[1]);
+    } else if (k == "dewald") {
+      o.dewald = true;
+    } else if (k == "chgtrnterm") {
+      if (v1 == "only") ct_only = true;
+      else o.use_chgtrn = (v1 != "none");
+    } else if (k == "chgtrn-cutoff") {
+      o.ct_cut = to_d(tok[1]);
+    } else if (k == "chgtrn-taper") {
+      o.ct_taper = to_d(tok[1]);
     } else if (k == "vdwterm") {
       if (v1 == "only") vdw_only = true;
       else o.use_vdw = (v1 != "none");
@@ -392,18 +428,33 @@ void read_key_text(const std::string& text, System& s, std::string* prm_name) {
   }
   // stretch-bend, out-of-plane bending and the torsion family all named off
   if (other_off >= 7) o.other_valence_off = true;
-  if (vdw_only) { o.use_mpole = false; o.use_polar = false; o.use_vdw = true; }
+  if (vdw_only) {
+    o.use_mpole = false;
+    o.use_polar = false;
+    o.use_vdw = true;
+    o.use_chgtrn = o.use_disp = o.use_repel = false;
+  }
   if (valence_only) {
     o.other_valence_off = true;  // "only" switches every other term off
     o.use_mpole = false;
     o.use_polar = false;
     o.use_vdw = false;
+    o.use_chgtrn = o.use_disp = o.use_repel = false;
     o.use_bond = (only_mask & 1) != 0;
     o.use_angle = (only_mask & 2) != 0;
     o.use_urey = (only_mask & 4) != 0;
   }
   if (mpole_only) { o.use_mpole = true; o.use_polar = false; o.use_vdw = false; o.use_bond = o.use_angle = o.use_urey = false; }
   if (polar_only) { o.use_mpole = false; o.use_polar = true; o.use_vdw = false; o.use_bond = o.use_angle = o.use_urey = false; }
+  if (mpole_only || polar_only) o.use_chgtrn = o.use_disp = o.use_repel = false;
+  if (ct_only || disp_only || rep_only) {
+    o.other_valence_off = true;
+    o.use_mpole = o.use_polar = o.use_vdw = false;
+    o.use_bond = o.use_angle = o.use_urey = false;
+    o.use_chgtrn = ct_only;
+    o.use_disp = disp_only;
+    o.use_repel = rep_only;
+  }
   if (vdw_only) o.use_bond = o.use_angle = o.use_urey = false;
   o.use_valence = o.use_bond || o.use_angle || o.use_urey;
 }
@@ -640,6 +691,60 @@ void assign_chgpen(System& s) {
   }
 }
 
+// HIPPO charge transfer: charge and damping exponent of the atom's class
+// (src/hippo/echgtrn.cpp:58-66: a zero exponent becomes 1000).
+void assign_chgtrn(System& s) {
+  s.chgtrn = !s.ff.chgtrn.empty() && s.opt.use_chgtrn;
+  s.chgct.clear();
+  s.dmpct.clear();
+  if (!s.chgtrn) return;
+  s.chgct.assign(s.n, 0.0);
+  s.dmpct.assign(s.n, 1000.0);
+  for (int i = 0; i < s.n; ++i) {
+    auto it = s.ff.chgtrn.find(s.cls[i]);
+    if (it == s.ff.chgtrn.end()) continue;
+    s.chgct[i] = it->second.first;
+    if (it->second.second != 0) s.dmpct[i] = it->second.second;
+  }
+}
+
+// HIPPO dispersion: C6 and damping exponent of the atom's class.
+void assign_disp(System& s) {
+  s.disp = !s.ff.disp.empty() && s.opt.use_disp;
+  s.csix.clear();
+  s.adisp.clear();
+  if (!s.disp) return;
+  if (s.opt.dewald) throw std::runtime_error("Ewald dispersion (dewald) is not evaluated: use disp-cutoff");
+  s.csix.assign(s.n, 0.0);
+  s.adisp.assign(s.n, 0.0);
+  for (int i = 0; i < s.n; ++i) {
+    auto it = s.ff.disp.find(s.cls[i]);
+    if (it == s.ff.disp.end()) continue;
+    s.csix[i] = it->second.first;
+    s.adisp[i] = it->second.second;
+  }
+}
+
+// HIPPO Pauli repulsion: size, damping exponent and valence count of the
+// atom's class.
+void assign_repel(System& s) {
+  s.repel = !s.ff.repel.empty() && s.opt.use_repel;
+  s.sizpr.clear();
+  s.dmppr.clear();
+  s.elepr.clear();
+  if (!s.repel) return;
+  s.sizpr.assign(s.n, 0.0);
+  s.dmppr.assign(s.n, 0.0);
+  s.elepr.assign(s.n, 0.0);
+  for (int i = 0; i < s.n; ++i) {
+    auto it = s.ff.repel.find(s.cls[i]);
+    if (it == s.ff.repel.end()) continue;
+    s.sizpr[i] = std::get<0>(it->second);
+    s.dmppr[i] = std::get<1>(it->second);
+    s.elepr[i] = std::get<2>(it->second);
+  }
+}
+
 void build_scales(System& s) {
   const int n = s.n;
   const ForceField& ff = s.ff;
@@ -648,7 +753,7 @@ void build_scales(System& s) {
     std::map<int, ScalePair> row;
     auto get = [&](int k) -> ScalePair& {
       auto it = row.find(k);
-      if (it == row.end()) it = row.emplace(k, ScalePair{i, k, 1, 1, 1, 1}).first;
+      if (it == row.end()) it = row.emplace(k, ScalePair{i, k, 1, 1, 1, 1, 1, 1}).first;
       return it->second;
     };
     const std::vector<int>* cpl[6] = {nullptr, nullptr, &s.i12[i], &s.i13[i], &s.i14[i], &s.i15[i]};
@@ -656,6 +761,8 @@ void build_scales(System& s) {
       for (int k : *cpl[b]) {
         if (k <= i) continue;
         if (ff.m_scale[b] != 1) get(k).m = ff.m_scale[b];
+        if (s.disp && ff.ds_scale[b] != 1) get(k).ds = ff.ds_scale[b];
+        if (s.repel && ff.rs_scale[b] != 1) get(k).rs = ff.rs_scale[b];
         double pv = contains(s.ip11[i], k) ? ff.p_iscale[b] : ff.p_scale[b];
         if (ff.p_scale[b] != 1 || ff.p_iscale[b] != 1) get(k).p = pv;
         if (s.chgpen) {  // HIPPO: d = p (bond-based, intra-group variant), u = induce-1x
@@ -666,7 +773,7 @@ void build_scales(System& s) {
     if (s.chgpen) {
       for (auto& kv : row) {
         const ScalePair& p = kv.second;
-        if (p.m != 1 || p.d != 1 || p.p != 1 || p.u != 1) s.scales.push_back(p);
+        if (p.m != 1 || p.d != 1 || p.p != 1 || p.u != 1 || p.ds != 1 || p.rs != 1) s.scales.push_back(p);
       }
       continue;
     }
@@ -679,7 +786,7 @@ void build_scales(System& s) {
       }
     for (auto& kv : row) {
       const ScalePair& p = kv.second;
-      if (p.m != 1 || p.d != 1 || p.p != 1 || p.u != 1) s.scales.push_back(p);
+      if (p.m != 1 || p.d != 1 || p.p != 1 || p.u != 1 || p.ds != 1 || p.rs != 1) s.scales.push_back(p);
     }
   }
 }
@@ -808,6 +915,9 @@ void setup_system(System& s) {
   assign_multipoles(s);
   assign_polarize(s);
   assign_chgpen(s);
+  assign_chgtrn(s);
+  assign_disp(s);
+  assign_repel(s);
   build_scales(s);
   assign_vdw(s);
   build_vdw_tables(s);
@@ -825,6 +935,14 @@ void setup_system(System& s) {
             "or switch the valence terms off (bondterm none)");
   }
   if (o.vdw_taper <= 0) o.vdw_taper = 0.9 * o.vdw_cut;
+  // HIPPO switching distances: 0.9 of the cutoff by default, a value up to 1
+  // a fraction of the cutoff, a larger one a distance
+  for (auto tc : {std::make_pair(&o.ct_taper, o.ct_cut), std::make_pair(&o.disp_taper, o.disp_cut),
+                  std::make_pair(&o.rep_taper, o.rep_cut)}) {
+    double& t = *tc.first;
+    if (t <= 0) t = 0.9 * tc.second;
+    else if (t <= 1) t *= tc.second;
+  }
   if (o.ewald) {
     if (!s.periodic) throw std::runtime_error("ewald requires a periodic box");
     if (o.ewald_alpha <= 0) o.ewald_alpha = ewald_coeff(o.ewald_cut);

@@ -72,6 +72,8 @@ struct ForceField {
   double u_scale[5] = {0, 1, 1, 1, 1};
   double v_scale[6] = {0, 0, 0, 0, 1, 1};
   double w_scale[6] = {1, 1, 1, 1, 1, 1};       // induce-1x-scale (HIPPO mutual field)
+  double ds_scale[6] = {0, 0, 0, 0.4, 0.8, 1};  // disp-1x-scale (HIPPO dispersion)
+  double rs_scale[6] = {0, 0, 0, 1, 1, 1};      // rep-1x-scale (HIPPO repulsion)
   std::map<int, int> type_class;
   std::map<int, std::string> type_name;
   std::map<int, double> type_mass;
@@ -81,6 +83,13 @@ struct ForceField {
   // HIPPO charge penetration by class: core charge, valence damping alpha
   // (the chgpen records of a HIPPO parameter file)
   std::map<int, std::pair<double, double>> chgpen;
+  // HIPPO charge transfer by class: charge, damping alpha (chgtrn records)
+  std::map<int, std::pair<double, double>> chgtrn;
+  // HIPPO dispersion by class: C6 coefficient, damping alpha (dispersion records)
+  std::map<int, std::pair<double, double>> disp;
+  // HIPPO Pauli repulsion by class: size, damping alpha, valence count
+  // (repulsion records)
+  std::map<int, std::tuple<double, double, double>> repel;
   std::map<int, VdwParam> vdw;                  // by class
   std::map<std::pair<int, int>, BondParam> bond;  // by class pair (sorted)
   std::map<std::tuple<int, int, int>, AngleParam> angle;
@@ -96,6 +105,10 @@ struct Options {
   // every term on unless the keywords switch it off, as in Tinker
   bool use_mpole = true, use_polar = true, use_vdw = true, use_valence = true;
   bool use_bond = true, use_angle = true, use_urey = true;  // use_valence: any of the three
+  bool use_chgtrn = true;      // HIPPO charge transfer, where the parameters have chgtrn records
+  bool use_disp = true;        // HIPPO dispersion, where the parameters have dispersion records
+  bool dewald = false;         // Ewald summation of the dispersion (not evaluated here)
+  bool use_repel = true;       // HIPPO repulsion, where the parameters have repulsion records
   // valence terms this path does not evaluate (stretch-bend, out-of-plane,
   // torsions, ...): a system that would need them must switch them off
   bool other_valence_off = false;
@@ -115,6 +128,12 @@ struct Options {
   double list_buffer = 2.0;
   double vdw_cut = 9.0;
   double vdw_taper = 0;        // 0: 0.9 * vdw_cut (the reference's vdwtaper default)
+  double ct_cut = 6.0;         // chgtrn-cutoff
+  double ct_taper = 0;         // chgtrn-taper; 0: 0.9 * ct_cut
+  double disp_cut = 9.0;       // disp-cutoff
+  double disp_taper = 0;       // disp-taper; 0: 0.9 * disp_cut
+  double rep_cut = 6.0;        // repuls-cutoff
+  double rep_taper = 0;        // repuls-taper; 0: 0.9 * rep_cut
   bool mutual = true;
 };
 
@@ -126,6 +145,8 @@ struct Options {
 struct ScalePair {
   int i, k;  // i < k, 0-based original indices
   double m, d, p, u;
+  double ds = 1;  // HIPPO dispersion (disp-1x-scale)
+  double rs = 1;  // HIPPO repulsion (rep-1x-scale)
 };
 
 struct System {
@@ -151,6 +172,17 @@ struct System {
   // valence charge pval = q - pcore with damping exponent palpha
   bool chgpen = false;
   std::vector<double> pcore, pval, palpha;
+  // HIPPO charge transfer, per atom (empty unless the parameters have chgtrn
+  // records and the term is on): charge and damping exponent (a zero
+  // exponent stands for 1000, no transfer at any distance)
+  bool chgtrn = false;
+  std::vector<double> chgct, dmpct;
+  // HIPPO dispersion, per atom: C6 coefficient and damping exponent
+  bool disp = false;
+  std::vector<double> csix, adisp;
+  // HIPPO repulsion, per atom: size, damping exponent, valence count
+  bool repel = false;
+  std::vector<double> sizpr, dmppr, elepr;
   std::vector<double> thlval;   // [njpolar][njpolar] pair Thole values
   int njpolar = 0;
 

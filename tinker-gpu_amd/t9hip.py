@@ -37,7 +37,7 @@ EXPORTS = [
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL, CALC_ANALYZE = 1, 2, 4, 8
-TERM_ELEC, TERM_VDW, TERM_VALENCE, TERM_ALL = 1, 2, 4, 7
+TERM_ELEC, TERM_VDW, TERM_VALENCE, TERM_CHGTRN, TERM_DISP, TERM_ALL = 1, 2, 4, 8, 16, 31
 
 
 class T9Error(RuntimeError):
@@ -56,7 +56,9 @@ class EnergyOut(ctypes.Structure):
                 ("em_real", ctypes.c_double), ("em_recip", ctypes.c_double), ("em_self", ctypes.c_double),
                 ("vir", ctypes.c_double * 9), ("induce_iterations", ctypes.c_int), ("evdw", ctypes.c_double),
                 ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double),
-                ("nem", ctypes.c_int64), ("nep", ctypes.c_int64), ("nev", ctypes.c_int64)]
+                ("nem", ctypes.c_int64), ("nep", ctypes.c_int64), ("nev", ctypes.c_int64),
+                ("ect", ctypes.c_double), ("nct", ctypes.c_int64), ("edisp", ctypes.c_double),
+                ("ndisp", ctypes.c_int64)]
 
 
 class MdStats(ctypes.Structure):
@@ -249,6 +251,19 @@ class System:
         d["chgpen"] = bool(self.scalar("chgpen"))
         if d["chgpen"]:  # HIPPO charge penetration: core charge, valence charge, valence alpha
             d.update(pcore=self.f64("pcore", n), pval=self.f64("pval", n), palpha=self.f64("palpha", n))
+        d["chgtrn"] = bool(self.scalar("chgtrn"))
+        if d["chgtrn"]:  # HIPPO charge transfer: charge and damping exponent, cutoff and taper
+            d.update(chgct=self.f64("chgct", n), dmpct=self.f64("dmpct", n), ct_cut=self.scalar("ct_cut"),
+                     ct_taper=self.scalar("ct_taper"))
+        d["disp"] = bool(self.scalar("disp"))
+        if d["disp"]:  # HIPPO dispersion: C6, damping exponent, cutoff, taper, per-pair scales
+            d.update(csix=self.f64("csix", n), adisp=self.f64("adisp", n), disp_cut=self.scalar("disp_cut"),
+                     disp_taper=self.scalar("disp_taper"), scale_ds=self.f64("scale_ds", len(d["scale_ik"])))
+        d["repel"] = bool(self.scalar("repel"))
+        if d["repel"]:  # HIPPO repulsion: size, damping exponent, valence count, cutoff, taper, scales
+            d.update(sizpr=self.f64("sizpr", n), dmppr=self.f64("dmppr", n), elepr=self.f64("elepr", n),
+                     rep_cut=self.scalar("rep_cut"), rep_taper=self.scalar("rep_taper"),
+                     scale_rs=self.f64("scale_rs", len(d["scale_ik"])))
         # MD terms: vdW tables and valence lists
         nj = int(self.scalar("njvdw"))
         nvs, nb, na, nu = (int(self.scalar(k)) for k in ("nvscale", "nbond", "nangle", "nurey"))
@@ -374,21 +389,22 @@ class Context:
         check(lib().t9_energy(self.h, vers, ctypes.byref(out), _p(g) if g is not None else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
-                 eangle=out.eangle, eurey=out.eurey)
+                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp)
         if vers & CALC_ANALYZE:
-            e.update(nem=out.nem, nep=out.nep, nev=out.nev)
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     def energy_terms(self, terms, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL, grad=True):
-        """As energy(), for the selected term groups (TERM_ELEC | TERM_VDW | TERM_VALENCE)."""
+        """As energy(), for the selected term groups (TERM_ELEC | TERM_VDW | TERM_VALENCE | TERM_CHGTRN |
+        TERM_DISP)."""
         out = EnergyOut()
         g = np.zeros((self.n, 3)) if grad and (vers & CALC_GRAD) else None
         check(lib().t9_energy_terms(self.h, vers, terms, ctypes.byref(out), _p(g) if g is not None else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
-                 eangle=out.eangle, eurey=out.eurey)
+                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp)
         if vers & CALC_ANALYZE:
-            e.update(nem=out.nem, nep=out.nep, nev=out.nev)
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     def set_coords_device(self, d_xyz_ptr):
@@ -405,7 +421,7 @@ class Context:
         check(lib().t9_energy_device(self.h, vers, terms, ctypes.byref(out),
                                      ctypes.c_void_p(int(d_grad_ptr)) if d_grad_ptr else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, evdw=out.evdw, ebond=out.ebond, eangle=out.eangle,
-                 eurey=out.eurey, iterations=out.induce_iterations)
+                 eurey=out.eurey, ect=out.ect, edisp=out.edisp, iterations=out.induce_iterations)
         return e, np.array(out.vir[:]).reshape(3, 3)
 
     def _precond(self, fn, rd, rp):
