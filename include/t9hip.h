@@ -195,6 +195,11 @@ typedef struct {
    * ndisp with T9_CALC_ANALYZE */
   double edisp;
   int64_t ndisp;
+  /* HIPPO Pauli repulsion (erepel, src/hippo/erepel.cpp,
+   * src/acc/hippo/erepel.cpp:13-215) and its count nrep with
+   * T9_CALC_ANALYZE */
+  double erep;
+  int64_t nrep;
 } t9_energy_out;
 int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
 /* Term selection of one energy call, as energy_core selects its terms by the
@@ -203,11 +208,11 @@ int t9_energy(t9_ctx* ctx, int vers, t9_energy_out* out, double* grad);
  * dipole solve), T9_TERM_VDW the buffered 14-7 vdW (evdw), T9_TERM_VALENCE
  * bond / angle / Urey-Bradley (ebond, eangle, eurey), T9_TERM_CHGTRN the
  * HIPPO charge transfer (echgtrn), T9_TERM_DISP the HIPPO dispersion
- * (edisp).  A term the system's keywords switch off
+ * (edisp), T9_TERM_REPEL the HIPPO Pauli repulsion (erepel).  A term the system's keywords switch off
  * stays off.  t9_energy is all of them.  A host that keeps its own vdW and
  * valence terms asks for T9_TERM_ELEC only. */
 enum { T9_TERM_ELEC = 1, T9_TERM_VDW = 2, T9_TERM_VALENCE = 4, T9_TERM_CHGTRN = 8, T9_TERM_DISP = 16,
-       T9_TERM_ALL = 31 };
+       T9_TERM_REPEL = 32, T9_TERM_ALL = 63 };
 int t9_energy_terms(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* grad);
 
 /* Device-resident coordinates and gradient: d_xyz and d_grad are device

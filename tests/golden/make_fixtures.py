@@ -24,6 +24,9 @@ What it writes (numbers and data files only, no reference source text):
     (test/emhippo.cpp, test/ephippo.cpp, test/chgtrn.cpp, test/rephippo.cpp,
     test/ref/emhippo.{1,2}.txt, ephippo.{1,2}.txt, chgtrn.1.txt,
     rephippo.1.txt)
+  * tests/golden/h2o10.xyz, water19_records.prm, hippo_water_ref.json — the
+    HIPPO repulsion of ten waters (test/rephippo.cpp EREPEL-2,
+    test/ref/rephippo.2.txt)
   * tests/golden/hippo19_disp_subset.prm, hippo_disp_ref.json — the HIPPO
     dispersion tests (test/disp.cpp with its own test/file/disp/hippo19.prm,
     test/ref/disp.{1,2}.txt)
@@ -367,6 +370,33 @@ def write_hippo():
     return sorted(ref)
 
 
+def write_hippo_water():
+    """HIPPO repulsion of ten waters (test/rephippo.cpp EREPEL-2): the
+    coordinates of test/file/water10/h2o10.xyz, the records of
+    test/file/hippo/water19.prm (comment lines dropped), and the energy,
+    count, virial and gradient of test/ref/rephippo.2.txt."""
+    copy_file("test/file/water10/h2o10.xyz", "h2o10.xyz")
+    lines = open(os.path.join(REF, "test/file/hippo/water19.prm")).read().splitlines()
+    out = ["# Records of the HIPPO water parameter set (test/file/hippo/water19.prm): keywords and parameter records", ""]
+    recs = {"atom", "bond", "angle", "ureybrad", "opbend", "strbnd", "polarize", "chgpen", "chgtrn", "dispersion",
+            "repulsion", "vdw", "torsion"}
+    i = 0
+    while i < len(lines):
+        t = lines[i].split()
+        if t and t[0] == "multipole":
+            out.extend(lines[i:i + 5])
+            i += 5
+            continue
+        if t and (t[0] in recs or (re.match(r"^[a-z][a-z0-9-]*$", t[0]) and (len(t) == 2 or t[0] == "d-equals-p")
+                                   and not lines[i][0].isspace())):
+            out.append(lines[i])
+        i += 1
+    open(os.path.join(HERE, "water19_records.prm"), "w").write("\n".join(out) + "\n")
+    ref = {"rephippo_water": parse_ref_txt(os.path.join(REF, "test/ref/rephippo.2.txt"))}
+    json.dump(ref, open(os.path.join(HERE, "hippo_water_ref.json"), "w"), indent=0)
+    return sorted(ref)
+
+
 def write_hippo_disp():
     """HIPPO dispersion (test/disp.cpp): the parameter header and the atom and
     dispersion records of the c5h12acnh2 types from test/file/disp/hippo19.prm,
@@ -423,6 +453,7 @@ def main():
     print("nacl:", write_nacl())
     print("hippo:", write_hippo())
     print("hippo dispersion:", write_hippo_disp())
+    print("hippo water:", write_hippo_water())
     print("more:", write_more())
 
 

@@ -1,5 +1,5 @@
-"""HIPPO charge-penetration multipoles, polarization, charge transfer and
-dispersion on the GPU through the C ABI.
+"""HIPPO charge-penetration multipoles, polarization, charge transfer,
+dispersion and Pauli repulsion on the GPU through the C ABI.
 
 test/emhippo.cpp: pentane + acetamide (104 atoms, HIPPO-2019) in a
 35 x 30 x 25 Å box, Ewald at 7 Å (PME) and non-Ewald at 9 Å.  Against the
@@ -12,7 +12,9 @@ interaction count; and test/ephippo.cpp (HIPPO mutual polarization of 50
 DMSO molecules, Ewald and non-Ewald) against its goldens and the oracle's
 induced dipoles; test/chgtrn.cpp (HIPPO charge transfer of the same DMSO
 box) against test/ref/chgtrn.1.txt and the oracle; test/disp.cpp (HIPPO
-dispersion with a cutoff) against test/ref/disp.1.txt and the oracle."""
+dispersion with a cutoff) against test/ref/disp.1.txt and the oracle;
+test/rephippo.cpp (HIPPO Pauli repulsion of pentane + acetamide) against
+test/ref/rephippo.1.txt and the oracle."""
 import json
 import os
 
@@ -137,3 +139,33 @@ def test_disp_gpu(golden_dir, prec):
     assert np.abs(g - go).max() < tol[1] and np.abs(v - vo).max() < tol[1] * 10
     ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
     assert ea["ndisp"] == ref["count"] == co
+
+
+@pytest.mark.parametrize("case", ["rephippo", "rephippo_water"])
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_repel_gpu(golden_dir, prec, case):
+    """test/rephippo.cpp EREPEL-1 (pentane + acetamide) and EREPEL-2 (ten
+    waters), HIPPO Pauli repulsion at 7 Å: the reference's tolerances (energy
+    and gradient 1e-4, virial 1e-3, both precisions), the interaction count,
+    and the oracle (FP64: 1e-10 relative energy, 1e-8 gradient and virial;
+    FP32 reads the FP32 rotated multipoles and fixed-point positions: 1e-6
+    relative energy, 1e-4 gradient, 1e-3 virial)."""
+    print(f"case repel {case} prec={prec}", flush=True)
+    from test_oracle_hippo import hippo_rep_system, rep_ref
+    ref = rep_ref(golden_dir, case)
+    s = hippo_rep_system(golden_dir, case)
+    ctx = t9hip.Context(s, prec)
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    de, dg, dv = abs(e["esum"] - ref["energy"]), np.abs(g - rg).max(), np.abs(v - rv).max()
+    eo, go, vo, co = oracle.Oracle(s.arrays()).repel()
+    print(f"  E={e['esum']:.6f} erep={e['erep']:.6f} ref={ref['energy']} |dE|={de:.2e} max|dg|={dg:.2e} "
+          f"max|dv|={dv:.2e}; vs oracle {abs(e['erep'] - eo):.2e} {np.abs(g - go).max():.2e} "
+          f"{np.abs(v - vo).max():.2e}")
+    assert e["erep"] == e["esum"]
+    assert de < 1e-4 and dg < 1.5e-4 and dv < 1e-3
+    tol = (1e-10, 1e-8, 1e-8) if prec == t9hip.T9_FP64 else (1e-6, 1e-4, 1e-3)
+    assert abs(e["erep"] - eo) < tol[0] * abs(eo)
+    assert np.abs(g - go).max() < tol[1] and np.abs(v - vo).max() < tol[2]
+    ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
+    assert ea["nrep"] == ref["count"] == co

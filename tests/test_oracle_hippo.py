@@ -271,3 +271,64 @@ def test_disp_gradient_is_derivative(golden_dir):
             em = o.disp()[0]
             assert abs((ep - em) / (2 * h) - g[i, d]) < 1e-6 * max(1.0, abs(g[i, d]))
     o.set_xyz(x0)
+
+
+HIPPO_REP_KEY = "neighbor-list\na-axis 35.0\nb-axis 30.0\nc-axis 25.0\nrepulsionterm only\nrepuls-cutoff 7.0\n"
+
+
+HIPPO_REP_WATER_KEY = "neighbor-list\nrepulsionterm only\na-axis 25.0\nrepuls-cutoff 7.0\n"
+REP_CASES = {"rephippo": ("c5h12acnh2.xyz", "hippo19_subset.prm", HIPPO_REP_KEY, "hippo_ref.json"),
+             "rephippo_water": ("h2o10.xyz", "water19_records.prm", HIPPO_REP_WATER_KEY, "hippo_water_ref.json")}
+
+
+def hippo_rep_system(golden_dir, case="rephippo"):
+    xyz, prm, key, _ = REP_CASES[case]
+    return t9hip.System.from_files(os.path.join(golden_dir, xyz), prm=os.path.join(golden_dir, prm), key=key)
+
+
+def rep_ref(golden_dir, case):
+    return json.load(open(os.path.join(golden_dir, REP_CASES[case][3])))[case]
+
+
+@pytest.mark.parametrize("case", sorted(REP_CASES))
+def test_repel_goldens(golden_dir, case):
+    """test/rephippo.cpp EREPEL-1 (pentane + acetamide, HIPPO-2019) and
+    EREPEL-2 (ten waters, the HIPPO water set), 7 Å cutoff: HIPPO Pauli
+    repulsion against test/ref/rephippo.{1,2}.txt at the reference test's
+    tolerances (energy and gradient 1e-4, virial 1e-3), and the interaction
+    count."""
+    ref = rep_ref(golden_dir, case)
+    a = hippo_rep_system(golden_dir, case).arrays()
+    assert a["repel"] and a["rep_taper"] == pytest.approx(6.3)
+    e, g, v, cnt = oracle.Oracle(a).repel()
+    rg, rv = np.array(ref["grad"]), np.array(ref["virial"])
+    print(f"E={e:.6f} ref={ref['energy']} count={cnt} max|dg|={np.abs(g - rg).max():.2e} "
+          f"max|dv|={np.abs(v - rv).max():.2e}")
+    assert abs(e - ref["energy"]) < 1e-4
+    assert np.abs(g - rg).max() < 1e-4
+    assert np.abs(v - rv).max() < 1e-3
+    assert cnt == ref["count"]
+
+
+@pytest.mark.parametrize("r,ai,ak", [(1.3, 4.64, 4.64), (2.2, 4.64, 4.2256), (1.8, 3.1, 4.9), (2.6, 4.64, 4.6529)])
+def test_repel_kernel_is_orbital_overlap(r, ai, ak):
+    """sqrt(D_0) is the overlap of two normalized 1s Slater orbitals of
+    exponents ai/2, ak/2 at distance r (a numerical 2D integral), close
+    exponents included."""
+    za, zb = ai / 2, ak / 2
+    phi = lambda z, t: np.sqrt(z ** 3 / np.pi) * np.exp(-z * t)
+    f = lambda mu, t: 2 * np.pi * t * t * phi(za, t) * phi(zb, np.sqrt(max(t * t + r * r - 2 * t * r * mu, 0.0)))
+    S = integrate.dblquad(f, 0, 40.0 / min(za, zb), -1, 1, epsabs=1e-13, epsrel=1e-11)[0]
+    D = oracle.repel_series(r, ai, ak)
+    assert abs(np.sqrt(D[0]) - S) < 1e-9 * max(1.0, S)
+
+
+@pytest.mark.parametrize("ai,ak", [(4.64, 4.64), (4.64, 4.2256), (4.64, 4.6529)])
+def test_repel_derivative_series(ai, ak):
+    """D_(n+1) = -(1/r) dD_n/dr by central differences."""
+    for r in (1.1, 2.4, 4.0):
+        h = 1e-5 * r
+        Dp, Dm, D = oracle.repel_series(r + h, ai, ak), oracle.repel_series(r - h, ai, ak), oracle.repel_series(r, ai, ak)
+        for n in range(5):
+            d = (Dp[n] - Dm[n]) / (2 * h)
+            assert abs(-d / r - D[n + 1]) < 1e-7 * abs(D[n + 1]), (r, n)

@@ -95,7 +95,10 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     if (!(s.opt.ct_cut > 0) || s.opt.ct_taper > s.opt.ct_cut) throw ArgError("bad chgtrn-cutoff or chgtrn-taper");
     c.list_cut = std::max(c.list_cut, s.opt.ct_cut);
   }
-  if (s.repel) throw ArgError("HIPPO repulsion is not evaluated on the device yet: switch it off (repulsionterm none)");
+  if (s.repel) {
+    if (!(s.opt.rep_cut > 0) || s.opt.rep_taper > s.opt.rep_cut) throw ArgError("bad repuls-cutoff or repuls-taper");
+    c.list_cut = std::max(c.list_cut, s.opt.rep_cut);
+  }
   if (s.disp) {
     if (!(s.opt.disp_cut > 0) || s.opt.disp_taper > s.opt.disp_cut) throw ArgError("bad disp-cutoff or disp-taper");
     c.list_cut = std::max(c.list_cut, s.opt.disp_cut);
@@ -160,6 +163,19 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     upload(c.chgtrn_o, ct, st);
     T9_HIP(hipStreamSynchronize(st));  // ct is a local host buffer
   }
+  c.use_repel = s.repel;
+  if (c.use_repel) {
+    c.rep_cut = s.opt.rep_cut;
+    c.rep_taper = s.opt.rep_taper;
+    std::vector<double> rp(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      rp[3 * i] = s.sizpr[i];
+      rp[3 * i + 1] = s.dmppr[i];
+      rp[3 * i + 2] = s.elepr[i];
+    }
+    upload(c.repel_o, rp, st);
+    T9_HIP(hipStreamSynchronize(st));  // rp is a local host buffer
+  }
   c.use_disp = s.disp;
   if (c.use_disp) {
     c.disp_cut = s.opt.disp_cut;
@@ -185,11 +201,11 @@ void ctx_init(Ctx& c, const System& s, int precision) {
   }
 
   // ---- scale classes and the exclusion CSR (both directions)
-  std::map<std::tuple<double, double, double, double, double>, int> classes;
-  classes[{1.0, 1.0, 1.0, 1.0, 1.0}] = 0;
+  std::map<std::tuple<double, double, double, double, double, double>, int> classes;
+  classes[{1.0, 1.0, 1.0, 1.0, 1.0, 1.0}] = 0;
   std::vector<std::vector<std::pair<int, int>>> ex(n);
   for (const auto& p : s.scales) {
-    auto key = std::make_tuple(p.m, p.d, p.p, p.u, p.ds);
+    auto key = std::make_tuple(p.m, p.d, p.p, p.u, p.ds, p.rs);
     auto it = classes.find(key);
     int code;
     if (it == classes.end()) {
@@ -203,7 +219,8 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     ex[p.k].push_back({p.i, code});
   }
   c.ncodes = (int)classes.size();
-  for (int k = 0; k < kMaxCodes; ++k) c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = c.scale.ds[k] = 1.0;
+  for (int k = 0; k < kMaxCodes; ++k)
+    c.scale.m[k] = c.scale.d[k] = c.scale.p[k] = c.scale.u[k] = c.scale.ds[k] = c.scale.rs[k] = 1.0;
   for (auto& kv : classes) {
     int k = kv.second;
     c.scale.m[k] = std::get<0>(kv.first);
@@ -211,6 +228,7 @@ void ctx_init(Ctx& c, const System& s, int precision) {
     c.scale.p[k] = std::get<2>(kv.first);
     c.scale.u[k] = std::get<3>(kv.first);
     c.scale.ds[k] = std::get<4>(kv.first);
+    c.scale.rs[k] = std::get<5>(kv.first);
   }
   std::vector<int> eoff(n + 1, 0), ej, ecode;
   for (int i = 0; i < n; ++i) {
@@ -497,7 +515,7 @@ int evaluate(Ctx& c, int vers, int terms, double* e16, double* grad, bool grad_o
     T9_HIP(hipStreamSynchronize(c.side));
     c.vdw_pending = false;
   }
-  if (terms & (TERM_ELEC | TERM_VDW | TERM_CHGTRN | TERM_DISP)) {
+  if (terms & (TERM_ELEC | TERM_VDW | TERM_CHGTRN | TERM_DISP | TERM_REPEL)) {
     c.mlist.cut_valid = c.ulist.cut_valid = false;
     c.bins_valid = false;
     c.rpole_valid = false;
@@ -537,7 +555,9 @@ void fill_energy_out(const double* e, int vers, int iters, t9_energy_out* out) {
   out->eurey = e[EP_EUREY];
   out->ect = e[EP_ECT];
   out->edisp = e[EP_EDISP];
-  out->esum = out->em + out->ep + out->evdw + out->ebond + out->eangle + out->eurey + out->ect + out->edisp;
+  out->erep = e[EP_EREP];
+  out->esum = out->em + out->ep + out->evdw + out->ebond + out->eangle + out->eurey + out->ect + out->edisp +
+              out->erep;
   const double v6[6] = {e[EP_VXX], e[EP_VYY], e[EP_VZZ], e[EP_VXY], e[EP_VXZ], e[EP_VYZ]};
   const double v9[9] = {v6[0], v6[3], v6[4], v6[3], v6[1], v6[5], v6[4], v6[5], v6[2]};
   for (int k = 0; k < 9; ++k) out->vir[k] = (vers & T9_CALC_VIRIAL) ? v9[k] : 0.0;
@@ -820,14 +840,15 @@ int energy_common(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* 
   if (vers & ~(T9_CALC_ENERGY | T9_CALC_GRAD | T9_CALC_VIRIAL | T9_CALC_ANALYZE))
     throw t9::ArgError("unknown vers bits");
   if (terms & ~t9::TERM_ALL) throw t9::ArgError("unknown term bits");
-  double e[16];
+  double e[t9::kEnergyRow];
   const int iters = t9::evaluate(c, vers, terms, e, (vers & T9_CALC_GRAD) ? grad : nullptr, grad_on_device);
-  long long nem = 0, nep = 0, nev = 0, nct = 0, ndisp = 0;
+  long long nem = 0, nep = 0, nev = 0, nct = 0, ndisp = 0, nrep = 0;
   if (vers & T9_CALC_ANALYZE) {
     if (terms & t9::TERM_ELEC) t9::count_interactions(c, &nem, &nep);
     if (terms & t9::TERM_VDW) nev = t9::vdw_count(c);
     if (terms & t9::TERM_CHGTRN) nct = t9::chgtrn_count(c);
     if (terms & t9::TERM_DISP) ndisp = t9::disp_count(c);
+    if (terms & t9::TERM_REPEL) nrep = t9::repel_count(c);
   }
   if (out) {
     t9::fill_energy_out(e, vers, iters, out);
@@ -836,6 +857,7 @@ int energy_common(t9_ctx* ctx, int vers, int terms, t9_energy_out* out, double* 
     out->nev = nev;
     out->nct = nct;
     out->ndisp = ndisp;
+    out->nrep = nrep;
   }
   return T9_OK;
 }
@@ -1073,7 +1095,7 @@ int t9_md_run(t9_ctx* ctx, int nsteps, double dt, int nrespa, double bath_kelvin
     auto& c = ctx->c;
     if (nsteps < 0 || !(dt > 0) || nrespa < 1) throw t9::ArgError("bad nsteps, dt or nrespa");
     if (bath_kelvin > 0 && !(tau > 0)) throw t9::ArgError("the thermostat needs tau > 0");
-    double e[16] = {0};
+    double e[t9::kEnergyRow] = {0};
     long long iters = 0;
     const int builds0 = c.builds;
     if (!c.md_ready) {

@@ -62,4 +62,102 @@ __device__ __forceinline__ void chgpen_lambda(double r, double ai, double ak, do
   }
 }
 
+// HIPPO Pauli repulsion radial series D_0..D_5 (FP64): D_0 = S^2, S the
+// overlap of two normalized 1s Slater orbitals of exponents a = alpha_i / 2,
+// b = alpha_k / 2 at distance r, and D_(n+1) = -(1/r) dD_n/dr.  In
+// c = (a + b) r / 2, d = (b - a) r / 2:
+//   S = P^(3/2) e^(-c) (f1(d) (1 + c) + f2(d) c^2),  P = 4 a b / (a + b)^2,
+// f1(x) = sinh(x) / x = sum x^(2j) / (2j+1)!, f2(x) = (cosh x - f1) / x^2 =
+// sum (2j+2) x^(2j) / (2j+3)!, both entire, so close exponents need no
+// special case.  The derivatives come from Taylor arithmetic in u = r^2 / 2
+// (-(1/r) d/dr = -d/du): D_n = (-1)^n n! [u^n] D_0(u0 + u).  Replaces the
+// closed forms of damp_rep (include/seq/damp_hippo.h).
+struct RepJet {
+  double c[6];
+};
+__device__ __forceinline__ RepJet rj_mul(const RepJet& x, const RepJet& y) {
+  RepJet z;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    z.c[q] = 0;
+#pragma unroll
+    for (int p = 0; p <= q; ++p) z.c[q] = fma(x.c[p], y.c[q - p], z.c[q]);
+  }
+  return z;
+}
+
+__device__ __forceinline__ void repel_series(double r, double ai, double ak, double* D) {
+  const double a = 0.5 * ai, b = 0.5 * ak;
+  const double P = 4 * a * b / ((a + b) * (a + b));
+  const double P32 = P * sqrt(P);
+  // r(u) = sqrt(2 u) about u0 = r^2 / 2: the square-root recurrence on the
+  // jet of 2 u = r^2 + 2 e
+  RepJet rj;
+  rj.c[0] = r;
+  {
+    const double x[6] = {r * r, 2, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 1; q < 6; ++q) {
+      double t = x[q];
+#pragma unroll
+      for (int p = 1; p < q; ++p) t -= rj.c[p] * rj.c[q - p];
+      rj.c[q] = t / (2 * r);
+    }
+  }
+  RepJet c, d;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    c.c[q] = 0.5 * (a + b) * rj.c[q];
+    d.c[q] = 0.5 * (b - a) * rj.c[q];
+  }
+  const RepJet x2 = rj_mul(d, d);
+  // f1 and f2 by Horner in x^2; 24 terms cover x^2 up to ~40
+  constexpr int NT = 24;
+  RepJet f1 = {}, f2 = {};
+  {
+    double fact = 1;  // (2j+1)!
+    double k1[NT], k2[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      if (j > 0) fact *= (2.0 * j) * (2.0 * j + 1);
+      k1[j] = 1 / fact;
+      k2[j] = 1 / (fact * (2.0 * j + 3));
+    }
+    f1.c[0] = k1[NT - 1];
+    f2.c[0] = k2[NT - 1];
+    for (int j = NT - 2; j >= 0; --j) {
+      f1 = rj_mul(f1, x2);
+      f2 = rj_mul(f2, x2);
+      f1.c[0] += k1[j];
+      f2.c[0] += k2[j];
+    }
+  }
+  // e^(-c): z' = -c' z
+  RepJet ec;
+  ec.c[0] = exp(-c.c[0]);
+#pragma unroll
+  for (int q = 1; q < 6; ++q) {
+    double t = 0;
+#pragma unroll
+    for (int p = 1; p <= q; ++p) t -= p * c.c[p] * ec.c[q - p];
+    ec.c[q] = t / q;
+  }
+  RepJet c1 = c;
+  c1.c[0] += 1;
+  RepJet poly = rj_mul(f1, c1);
+  const RepJet f2c2 = rj_mul(f2, rj_mul(c, c));
+#pragma unroll
+  for (int q = 0; q < 6; ++q) poly.c[q] += f2c2.c[q];
+  RepJet S = rj_mul(ec, poly);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) S.c[q] *= P32;
+  const RepJet D0 = rj_mul(S, S);
+  double fn = 1;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    if (n > 0) fn *= n;
+    D[n] = ((n & 1) ? -fn : fn) * D0.c[n];
+  }
+}
+
 }  // namespace t9

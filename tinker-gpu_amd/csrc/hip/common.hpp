@@ -266,10 +266,10 @@ __device__ __forceinline__ PosRec<double> make_pos<double>(double x, double y, d
   return p;
 }
 
-// Per scale-class factors (m, d, p, u, and the HIPPO dispersion scale ds);
-// class 0 is the all-ones class.
+// Per scale-class factors (m, d, p, u, and the HIPPO dispersion and
+// repulsion scales ds, rs); class 0 is the all-ones class.
 struct ScaleTable {
-  double m[kMaxCodes], d[kMaxCodes], p[kMaxCodes], u[kMaxCodes], ds[kMaxCodes];
+  double m[kMaxCodes], d[kMaxCodes], p[kMaxCodes], u[kMaxCodes], ds[kMaxCodes], rs[kMaxCodes];
 };
 
 // Thole damping of the 1/r^3 and 1/r^5 (and 1/r^7) tensors:

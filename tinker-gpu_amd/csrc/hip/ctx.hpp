@@ -154,6 +154,12 @@ struct Ctx {
   bool use_disp = false;
   double disp_cut = 0, disp_taper = 0;
   DBuf disp_o;
+  // HIPPO Pauli repulsion: caller order [n][3] (size, damping exponent,
+  // valence count), over the multipole list within rep_cut, tapered from
+  // rep_taper
+  bool use_repel = false;
+  double rep_cut = 0, rep_taper = 0;
+  DBuf repel_o;
   double list_cut = 0;  // largest cutoff of the terms on the multipole list
   DBuf chgtrn_o;
   DBuf jpolar_o;                 // int [n]
@@ -296,9 +302,10 @@ struct Ctx {
 // Per-block partial rows of the energy/virial reductions.
 enum {
   EP_MREAL = 0, EP_MRECIP, EP_MSELF, EP_POLAR, EP_VXX, EP_VYY, EP_VZZ, EP_VXY, EP_VXZ, EP_VYZ,
-  EP_EVDW, EP_EBOND, EP_EANGLE, EP_EUREY, EP_ECT, EP_EDISP, EP_N
+  EP_EVDW, EP_EBOND, EP_EANGLE, EP_EUREY, EP_ECT, EP_EDISP, EP_EREP, EP_N
 };
-constexpr int kEnergyRow = 16;
+constexpr int kEnergyRow = 24;
+static_assert(EP_N <= kEnergyRow, "an energy row holds every EP_* slot");
 
 // ----- entry points implemented across the .hip files
 void ctx_init(Ctx& c, const System& s, int precision);
@@ -385,8 +392,8 @@ void vdw_finish(Ctx& c);                                             // site gra
 void valence_run(Ctx& c, double* gval_caller, bool energy);
 // Term groups of one evaluation: the electrostatics (multipole + polarization)
 // and vdW are the slow group of the RESPA split, valence the fast group.
-enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_CHGTRN = 8, TERM_DISP = 16, TERM_ALL = 31 };
-// vers bit 2: virial; out16[EP_*] energies / virial; sorted-order dE/dx of
+enum { TERM_ELEC = 1, TERM_VDW = 2, TERM_VALENCE = 4, TERM_CHGTRN = 8, TERM_DISP = 16, TERM_REPEL = 32, TERM_ALL = 63 };
+// vers bit 2: virial; out16[EP_*] (kEnergyRow values) energies / virial; sorted-order dE/dx of
 // the electrostatics and vdW in c.egrad, caller-order valence dE/dx in
 // c.g_fast; grad_orig (host, caller order, or null) receives their sum
 void energy_run(Ctx& c, int vers, int terms, double* out16, double* grad_orig, bool grad_on_device = false);
@@ -414,6 +421,9 @@ long long chgtrn_count(Ctx& c);
 // HIPPO dispersion (non-Ewald), the same contract
 void disp_run(Ctx& c, bool virial);
 long long disp_count(Ctx& c);
+// HIPPO repulsion: egrad and etq += (the torque kernel maps etq to forces)
+void repel_run(Ctx& c, bool virial);
+long long repel_count(Ctx& c);
 void respa_inner(Ctx& c, double* gfast, double h, int nrep);  // drift + nrep x (fast forces, kick, drift)
 bool respa_fused(const Ctx& c);
 void valence_inner(Ctx& c, double* gval, double h);  // valence gradient, kick by h, drift by h

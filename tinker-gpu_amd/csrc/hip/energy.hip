@@ -532,6 +532,153 @@ __global__ __launch_bounds__(64 * kEmplarParts) void k_empole_chgpen(EmplarArgs<
   block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
 }
 
+// Pauli repulsion (HIPPO; replaces src/acc/hippo/erepel.cpp:13-215 and
+// src/cu/hippo/erepel_cu1.cc with include/seq/pair_repel.h): the multipole
+// contraction of the two valence electron clouds (-v_i, d_i, Q_i) and
+// (-v_k, d_k, Q_k) (v the valence count, d and Q the rotated multipoles)
+// with the radial series D_n of repel_series, times K_i K_k s / r (sizes,
+// the pair's repulsion scale) and the fifth-order switch between taper and
+// cutoff; the radial factor F = Sw / r multiplies the energy, its potential
+// derivatives (torques) and, with F', the pair gradient.  Owner computes
+// over the buffered multipole list; egrad and etq are added to, the energy
+// and virial halved.  FP64 pair arithmetic in both precisions.
+struct RepelArgs {
+  int n;
+  const int* nlist;
+  const long long* woff;
+  const uint32_t* entries;
+  const int* perm;
+  const double* rp;  // caller order [n][3]: size, exponent, valence count
+  double sc[kMaxCodes];
+  double off2, taper, inv_w;
+  double* grad;      // sorted [n][3], +=
+  double* tq;        // sorted [n][9], +=
+  double* rows;
+};
+
+template <class real>
+__device__ __forceinline__ void repel_pair(const double* R, double r, const Mp<double>& mi, const Mp<double>& mk,
+                                           double ai, double ak, double kk, const RepelArgs& A,
+                                           PairOut<double>& o) {
+  const double z3[3] = {0, 0, 0};
+  double D[6];
+  repel_series(r, ai, ak, D);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) D[q] *= kk;
+  pair_terms<double, true, false>(R, D, nullptr, mi, mk, z3, z3, z3, z3, z3, z3, 1.0, o);
+  double Sw = 1, dSw = 0;
+  if (r > A.taper) {
+    const double x = (r - A.taper) * A.inv_w;
+    Sw = 1 - x * x * x * (10 + x * (6 * x - 15));
+    dSw = -30 * x * x * (1 - x) * (1 - x) * A.inv_w;
+  }
+  const double F = Sw / r, dF = dSw / r - Sw / (r * r);
+  const double e0 = o.em;
+  o.em *= F;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    o.de[t] = F * o.de[t] + e0 * dF * R[t] / r;
+    o.g[t] *= F;
+  }
+#pragma unroll
+  for (int t = 0; t < 6; ++t) o.h[t] *= F;
+}
+
+template <class real>
+__device__ __forceinline__ Mp<double> repel_mp(const real* __restrict__ rpole, int s, double v) {
+  Mp<double> m;
+  m.q = -v;  // the valence electrons
+  for (int t = 0; t < 3; ++t) m.d[t] = rpole[10 * (size_t)s + 1 + t];
+  for (int t = 0; t < 6; ++t) m.Q[t] = rpole[10 * (size_t)s + 4 + t];
+  return m;
+}
+
+template <class real>
+__global__ __launch_bounds__(64 * kEmplarParts) void k_repel(RepelArgs A, const PosRec<real>* __restrict__ pos,
+                                                             BoxT<real> box, const real* __restrict__ rpole) {
+  __shared__ double sh[parts_lds<kEmplarParts, 19>()];
+  const int part = threadIdx.x >> 6;
+  const int blk = xcd_block();
+  const int s = blk * 64 + (threadIdx.x & 63);
+  Acc<double> acc;
+  acc.zero();
+  if (s < A.n) {
+    const PosRec<real> pi = pos[s];
+    const int io = A.perm[s];
+    const double Ki = A.rp[3 * io], ai = A.rp[3 * io + 1];
+    const Mp<double> mi = repel_mp(rpole, s, A.rp[3 * io + 2]);
+    const int len = A.nlist[s];
+    const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
+    for (int kk = part; kk < len; kk += kEmplarParts) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      const int code = (int)(ent & (kMaxCodes - 1));
+      const double sc = A.sc[code];
+      if (sc == 0) continue;
+      double R[3];
+      pair_disp64(pos[k], pi, box, R);  // r_i - r_k
+      const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      if (!(r2 <= A.off2)) continue;
+      const int ko = A.perm[k];
+      const Mp<double> mk = repel_mp(rpole, k, A.rp[3 * ko + 2]);
+      PairOut<double> o;
+      repel_pair<real>(R, sqrt(r2), mi, mk, ai, A.rp[3 * ko + 1], Ki * A.rp[3 * ko] * sc, A, o);
+      acc.add(o, R);
+    }
+  }
+  double av[19];
+  acc.pack(av);
+  parts_reduce<kEmplarParts, 19>(av, sh);
+  double red[EP_N];
+  for (int q = 0; q < EP_N; ++q) red[q] = 0;
+  if (part == 0) {
+    acc.unpack(av);
+    if (s < A.n) {
+      for (int k = 0; k < 3; ++k) A.grad[3 * (size_t)s + k] += acc.gr[k];
+      for (int k = 0; k < 3; ++k) A.tq[9 * (size_t)s + k] += acc.tg[k];
+      for (int k = 0; k < 6; ++k) A.tq[9 * (size_t)s + 3 + k] += acc.th[k];
+    }
+    red[EP_EREP] = 0.5 * acc.e;
+    for (int k = 0; k < 6; ++k) red[EP_VXX + k] = 0.5 * acc.v[k];
+  }
+  block_row<EP_N>(red, A.rows + (size_t)blk * kRow);
+}
+
+// Analyze-mode count (the reference's nrep): pairs within the cutoff, each
+// once, whose unscaled energy is not zero, less the pairs of scale 0.
+template <class real>
+__global__ __launch_bounds__(256) void k_repel_count(RepelArgs A, const PosRec<real>* __restrict__ pos,
+                                                     BoxT<real> box, const real* __restrict__ rpole,
+                                                     unsigned long long* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cnt = 0;
+  if (s < A.n) {
+    const PosRec<real> pi = pos[s];
+    const int io = A.perm[s];
+    const Mp<double> mi = repel_mp(rpole, s, A.rp[3 * io + 2]);
+    const int len = A.nlist[s];
+    const uint32_t* en = A.entries + A.woff[s >> 6] + (s & 63);
+    for (int kk = 0; kk < len; ++kk) {
+      const uint32_t ent = en[(long long)kk * 64];
+      const int k = (int)(ent >> kCodeBits);
+      if (k <= s) continue;
+      const int code = (int)(ent & (kMaxCodes - 1));
+      if (A.sc[code] == 0) continue;
+      double R[3];
+      pair_disp64(pos[k], pi, box, R);
+      const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      if (!(r2 <= A.off2)) continue;
+      const int ko = A.perm[k];
+      PairOut<double> o;
+      repel_pair<real>(R, sqrt(r2), mi, repel_mp(rpole, k, A.rp[3 * ko + 2]), A.rp[3 * io + 1], A.rp[3 * ko + 1],
+                       A.rp[3 * io] * A.rp[3 * ko], A, o);
+      if (o.em != 0) ++cnt;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, cnt);
+}
+
 // Charge transfer and dispersion (HIPPO) share one pair-kernel skeleton:
 // owner computes over the buffered multipole list (both directions), each
 // atom sums its own gradient, the energy and virial are halved; FP64 pair
@@ -1398,6 +1545,7 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   const bool want_v = (vers & 4) != 0;
   const int n = c.n, nb = ceil_div(n, 256);
   if (!c.vdw_pending) c.erows = 0;  // else the rows after the pending vdW kernel's
+  const bool rep = (terms & TERM_REPEL) && c.use_repel;
   if (use_m || use_p) {
     real_space_t<real>(c, use_m, use_p);
     if (c.ewald) recip_t<real>(c, use_m, use_p, want_v);
@@ -1405,6 +1553,12 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
                        c.ewald ? 1 : 0, c.aewald, c.rpole.as<real>(), c.uind.as<real>(), c.udirp.as<real>(),
                        c.polinv.as<real>(), energy_rows(c, nb));
     T9_LAUNCH_CHECK();
+  } else {
+    T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
+    if (rep) T9_HIP(hipMemsetAsync(c.etq.p, 0, sizeof(double) * 9 * n, c.stream));
+  }
+  if (rep) repel_run(c, want_v);
+  if (use_m || use_p || rep) {
     const double lx = c.periodic ? c.box[0] : 1.0, ly = c.periodic ? c.box[1] : 1.0,
                  lz = c.periodic ? c.box[2] : 1.0;
     hipLaunchKernelGGL(k_torque<real>, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.xyz.as<double>(),
@@ -1415,8 +1569,6 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
     hipLaunchKernelGGL(k_frame_gather, dim3(nb), dim3(256), 0, c.stream, n, c.perm.as<int>(), c.iperm.as<int>(),
                        c.rev_off.as<int>(), c.rev_ent.as<int>(), c.eframe.as<double>(), c.egrad.as<double>());
     T9_LAUNCH_CHECK();
-  } else {
-    T9_HIP(hipMemsetAsync(c.egrad.p, 0, sizeof(double) * 3 * n, c.stream));
   }
   if ((terms & TERM_CHGTRN) && c.use_chgtrn) chgtrn_run(c, want_v);
   if ((terms & TERM_DISP) && c.use_disp) disp_run(c, want_v);
@@ -1430,15 +1582,15 @@ void energy_t(Ctx& c, int vers, int terms, double* out16) {
   if ((terms & TERM_VALENCE) && c.use_valence) valence_run(c, c.g_fast.as<double>(), true);
   // without CALC_ENERGY / CALC_VIRIAL nothing comes back to the host (no sync)
   if (c.erows == 0 || !(vers & 5)) {
-    for (int q = 0; q < 16; ++q) out16[q] = 0;
+    for (int q = 0; q < kEnergyRow; ++q) out16[q] = 0;
     return;
   }
   double* dsum = c.scalars.as<double>() + 16;
   hipLaunchKernelGGL(k_rows_reduce, dim3(1), dim3(256), 0, c.stream, c.erows, c.epart.as<double>(), dsum);
   T9_LAUNCH_CHECK();
-  T9_HIP(hipMemcpyAsync(c.h_scalars + 16, dsum, sizeof(double) * 16, hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipMemcpyAsync(c.h_scalars + 16, dsum, sizeof(double) * EP_N, hipMemcpyDeviceToHost, c.stream));
   T9_HIP(hipStreamSynchronize(c.stream));
-  for (int q = 0; q < 16; ++q) out16[q] = q < EP_N ? c.h_scalars[16 + q] : 0.0;
+  for (int q = 0; q < kEnergyRow; ++q) out16[q] = q < EP_N ? c.h_scalars[16 + q] : 0.0;
 }
 
 }  // namespace
@@ -1489,7 +1641,7 @@ void energy_setup(Ctx& c, const System& s) {
   if (c.ewald) lines = 3LL * ((c.grid[0] + 7) / 8) * c.grid[1] + 16;
   const long long nterm = (long long)(s.bond_kr.size() + s.angle_kt.size() + s.urey_kr.size()) / 2;
   // list loops: one row per 64 atoms; valence: one row per 256 terms
-  c.ecap = (int)(9LL * nb + 4LL * ceil_div(n, 64) + (nterm + 255) / 256 + lines + 16);
+  c.ecap = (int)(9LL * nb + 5LL * ceil_div(n, 64) + (nterm + 255) / 256 + lines + 16);
   c.epart.alloc(sizeof(double) * kRow * (size_t)c.ecap);
   T9_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1621,6 +1773,63 @@ void chgtrn_run(Ctx& c, bool vir) {
 long long chgtrn_count(Ctx& c) {
   if (!c.use_chgtrn) return 0;
   return c.prec == 32 ? hippo_count_t<float, 0>(c) : hippo_count_t<double, 0>(c);
+}
+
+namespace {
+template <class real>
+RepelArgs repel_args(Ctx& c) {
+  const auto& L = c.mlist;
+  RepelArgs A;
+  A.n = c.n;
+  A.nlist = L.nlist.as<int>();
+  A.woff = L.off.as<long long>();
+  A.entries = L.entries.as<uint32_t>();
+  A.perm = c.perm.as<int>();
+  A.rp = c.repel_o.as<double>();
+  for (int k = 0; k < kMaxCodes; ++k) A.sc[k] = k == 0 ? 1.0 : c.scale.rs[k];
+  A.off2 = c.rep_cut * c.rep_cut;
+  A.taper = c.rep_taper;
+  A.inv_w = c.rep_cut > c.rep_taper ? 1.0 / (c.rep_cut - c.rep_taper) : 0.0;
+  A.grad = c.egrad.as<double>();
+  A.tq = c.etq.as<double>();
+  A.rows = nullptr;
+  return A;
+}
+
+template <class real>
+void repel_run_t(Ctx& c) {
+  RepelArgs A = repel_args<real>(c);
+  A.rows = energy_rows(c, c.nwave);
+  hipLaunchKernelGGL(k_repel<real>, dim3(c.nwave), dim3(64 * kEmplarParts), 0, c.stream, A,
+                     c.pos.as<PosRec<real>>(), make_boxT<real>(c), c.rpole.as<real>());
+  T9_LAUNCH_CHECK();
+}
+
+template <class real>
+long long repel_count_t(Ctx& c) {
+  DBuf out;
+  out.alloc(sizeof(unsigned long long));
+  T9_HIP(hipMemsetAsync(out.p, 0, sizeof(unsigned long long), c.stream));
+  hipLaunchKernelGGL(k_repel_count<real>, dim3(ceil_div(c.n, 256)), dim3(256), 0, c.stream, repel_args<real>(c),
+                     c.pos.as<PosRec<real>>(), make_boxT<real>(c), c.rpole.as<real>(), out.as<unsigned long long>());
+  T9_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  T9_HIP(hipMemcpyAsync(&h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  T9_HIP(hipStreamSynchronize(c.stream));
+  return (long long)h;
+}
+}  // namespace
+
+// HIPPO repulsion (the virial rows are written in both forms; the torque
+// kernel adds the frame part).
+void repel_run(Ctx& c, bool /*vir*/) {
+  if (c.prec == 32) repel_run_t<float>(c);
+  else repel_run_t<double>(c);
+}
+
+long long repel_count(Ctx& c) {
+  if (!c.use_repel) return 0;
+  return c.prec == 32 ? repel_count_t<float>(c) : repel_count_t<double>(c);
 }
 
 void disp_run(Ctx& c, bool vir) {

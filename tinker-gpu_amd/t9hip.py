@@ -37,7 +37,7 @@ EXPORTS = [
 ]
 K_UFIELD, K_DFIELD, K_SPREAD, K_FFT, K_GATHER, K_PRECOND, K_PCG_VEC, K_EMPLAR, K_VDW, K_VALENCE = range(10)
 CALC_ENERGY, CALC_GRAD, CALC_VIRIAL, CALC_ANALYZE = 1, 2, 4, 8
-TERM_ELEC, TERM_VDW, TERM_VALENCE, TERM_CHGTRN, TERM_DISP, TERM_ALL = 1, 2, 4, 8, 16, 31
+TERM_ELEC, TERM_VDW, TERM_VALENCE, TERM_CHGTRN, TERM_DISP, TERM_REPEL, TERM_ALL = 1, 2, 4, 8, 16, 32, 63
 
 
 class T9Error(RuntimeError):
@@ -58,7 +58,7 @@ class EnergyOut(ctypes.Structure):
                 ("ebond", ctypes.c_double), ("eangle", ctypes.c_double), ("eurey", ctypes.c_double),
                 ("nem", ctypes.c_int64), ("nep", ctypes.c_int64), ("nev", ctypes.c_int64),
                 ("ect", ctypes.c_double), ("nct", ctypes.c_int64), ("edisp", ctypes.c_double),
-                ("ndisp", ctypes.c_int64)]
+                ("ndisp", ctypes.c_int64), ("erep", ctypes.c_double), ("nrep", ctypes.c_int64)]
 
 
 class MdStats(ctypes.Structure):
@@ -389,22 +389,22 @@ class Context:
         check(lib().t9_energy(self.h, vers, ctypes.byref(out), _p(g) if g is not None else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
-                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp)
+                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp, erep=out.erep)
         if vers & CALC_ANALYZE:
-            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp)
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp, nrep=out.nrep)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     def energy_terms(self, terms, vers=CALC_ENERGY | CALC_GRAD | CALC_VIRIAL, grad=True):
         """As energy(), for the selected term groups (TERM_ELEC | TERM_VDW | TERM_VALENCE | TERM_CHGTRN |
-        TERM_DISP)."""
+        TERM_DISP | TERM_REPEL)."""
         out = EnergyOut()
         g = np.zeros((self.n, 3)) if grad and (vers & CALC_GRAD) else None
         check(lib().t9_energy_terms(self.h, vers, terms, ctypes.byref(out), _p(g) if g is not None else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, em_real=out.em_real, em_recip=out.em_recip,
                  em_self=out.em_self, iterations=out.induce_iterations, evdw=out.evdw, ebond=out.ebond,
-                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp)
+                 eangle=out.eangle, eurey=out.eurey, ect=out.ect, edisp=out.edisp, erep=out.erep)
         if vers & CALC_ANALYZE:
-            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp)
+            e.update(nem=out.nem, nep=out.nep, nev=out.nev, nct=out.nct, ndisp=out.ndisp, nrep=out.nrep)
         return e, g, np.array(out.vir[:]).reshape(3, 3)
 
     def set_coords_device(self, d_xyz_ptr):
@@ -421,7 +421,7 @@ class Context:
         check(lib().t9_energy_device(self.h, vers, terms, ctypes.byref(out),
                                      ctypes.c_void_p(int(d_grad_ptr)) if d_grad_ptr else None))
         e = dict(esum=out.esum, em=out.em, ep=out.ep, evdw=out.evdw, ebond=out.ebond, eangle=out.eangle,
-                 eurey=out.eurey, ect=out.ect, edisp=out.edisp, iterations=out.induce_iterations)
+                 eurey=out.eurey, ect=out.ect, edisp=out.edisp, erep=out.erep, iterations=out.induce_iterations)
         return e, np.array(out.vir[:]).reshape(3, 3)
 
     def _precond(self, fn, rd, rp):
