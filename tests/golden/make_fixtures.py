@@ -327,8 +327,9 @@ def write_hippo():
     """HIPPO charge-penetration multipoles, polarization and charge transfer
     (test/emhippo.cpp, test/ephippo.cpp, test/chgtrn.cpp): the coordinates of
     test/file/c5h12acnh2/c5h12acnh2.xyz and test/file/dmso/dmso.xyz, the
-    parameter header and the atom, multipole, chgpen, chgtrn, repulsion and
-    polarize records of their types from test/file/hippo/hippo19.prm, and the
+    parameter header and the atom, multipole, chgpen, chgtrn, repulsion,
+    dispersion and polarize records of their types from
+    test/file/hippo/hippo19.prm, and the
     energy, count, virial and gradient of test/ref/emhippo.{1,2}.txt and
     ephippo.{1,2}.txt (Ewald at 7 Å and non-Ewald at 9 Å), chgtrn.1.txt and
     rephippo.1.txt."""
@@ -339,7 +340,7 @@ def write_hippo():
         types |= {int(ln.split()[5]) for ln in xyz[1:] if len(ln.split()) > 5 and ln.split()[0].isdigit()}
     lines = open(os.path.join(REF, "test/file/hippo/hippo19.prm")).read().splitlines()
     classes = {int(ln.split()[2]) for ln in lines if ln.split()[:1] == ["atom"] and int(ln.split()[1]) in types}
-    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole, chgpen, chgtrn, repulsion and polarize records of the c5h12acnh2 and dmso test systems", ""]
+    out = ["# Subset of the HIPPO-2019 parameter set (test/file/hippo/hippo19.prm): header, and the atom,", "# multipole, chgpen, chgtrn, repulsion, dispersion and polarize records of the c5h12acnh2 and dmso", "# test systems", ""]
     i, header = 0, True
     while i < len(lines):
         t = lines[i].split()
@@ -354,7 +355,7 @@ def write_hippo():
             out.extend(lines[i:i + 5])
             i += 5
             continue
-        elif t and t[0] in ("chgpen", "chgtrn", "repulsion") and int(t[1]) in classes:
+        elif t and t[0] in ("chgpen", "chgtrn", "repulsion", "dispersion") and int(t[1]) in classes:
             out.append(lines[i])
         elif t and t[0] == "polarize" and int(t[1]) in types:
             out.append(lines[i])

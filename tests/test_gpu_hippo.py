@@ -169,3 +169,81 @@ def test_repel_gpu(golden_dir, prec, case):
     assert np.abs(g - go).max() < tol[1] and np.abs(v - vo).max() < tol[2]
     ea, _, _ = ctx.energy(t9hip.CALC_ENERGY | t9hip.CALC_ANALYZE, grad=False)
     assert ea["nrep"] == ref["count"] == co
+
+
+HIPPO_ALL_KEY = ("neighbor-list\na-axis 25.0\nb-axis 25.0\nc-axis 30.0\newald\newald-cutoff 7.0\nchgtrn-cutoff 6.0\n"
+                 "repuls-cutoff 6.0\ndisp-cutoff 7.0\nvdwterm none\nbondterm none\nangleterm none\n"
+                 "ureybradterm none\nstrbndterm none\nopbendterm none\nopdistterm none\ntorsionterm none\n"
+                 "pitorsterm none\ntortorterm none\nangtorterm none\nstrtorterm none\nimproperterm none\n"
+                 "imptorsterm none\n")
+
+
+@pytest.mark.parametrize("prec", [t9hip.T9_FP64, t9hip.T9_FP32])
+def test_hippo_all_terms_gpu(golden_dir, prec):
+    """Every HIPPO nonbonded term at once on the 50-DMSO box (Ewald
+    multipoles and polarization, Pauli repulsion, dispersion with a cutoff,
+    charge transfer): the energy, gradient and virial of one call equal the
+    sums of the calls that select one term at a time (the accumulation order
+    of the gradient and torque arrays across the term kernels), and the sum
+    of the oracle's terms (FP64: 1e-6 relative energy, 2e-5 gradient, 1e-4
+    virial, the solve stopping at 1e-6 D; FP32: 1e-5, 1e-4, 1e-3)."""
+    print(f"case hippo all terms prec={prec}", flush=True)
+    s = t9hip.System.from_files(os.path.join(golden_dir, "dmso.xyz"),
+                                prm=os.path.join(golden_dir, "hippo19_subset.prm"), key=HIPPO_ALL_KEY)
+    a = s.arrays()
+    assert a["chgpen"] and a["repel"] and a["disp"] and a["chgtrn"]
+    ctx = t9hip.Context(s, prec)
+    e, g, v = ctx.energy_terms(t9hip.TERM_ALL)
+    parts = {t: ctx.energy_terms(t) for t in (t9hip.TERM_ELEC, t9hip.TERM_REPEL, t9hip.TERM_DISP,
+                                               t9hip.TERM_CHGTRN)}
+    es = sum(p[0]["esum"] for p in parts.values())
+    gs = sum(p[1] for p in parts.values())
+    vs = sum(p[2] for p in parts.values())
+    print(f"  E={e['esum']:.6f} (em {e['em']:.4f} ep {e['ep']:.4f} erep {e['erep']:.4f} edisp {e['edisp']:.4f} "
+          f"ect {e['ect']:.4f}) sum of parts {es:.6f} max|dg| {np.abs(g - gs).max():.2e} "
+          f"max|dv| {np.abs(v - vs).max():.2e}")
+    assert abs(e["erep"] - parts[t9hip.TERM_REPEL][0]["erep"]) < 1e-10 * abs(e["erep"])
+    assert abs(e["edisp"] - parts[t9hip.TERM_DISP][0]["edisp"]) < 1e-10 * abs(e["edisp"])
+    assert abs(e["ect"] - parts[t9hip.TERM_CHGTRN][0]["ect"]) < 1e-10 * abs(e["ect"])
+    assert abs(e["esum"] - es) < 1e-7 * abs(es)
+    assert np.abs(g - gs).max() < 1e-5 and np.abs(v - vs).max() < 1e-4
+    o = oracle.Oracle(a)
+    ud, up, _, _, _ = o.induce()
+    eo, go, vo = o.energy(ud, up, mpole=True, polar=True)
+    er, gr, vr, _ = o.repel()
+    ed, gd, vd, _ = o.disp()
+    ec, gc, vc, _ = o.chgtrn()
+    etot = eo["em"] + eo["ep"] + er + ed + ec
+    gtot, vtot = go + gr + gd + gc, vo + vr + vd + vc
+    print(f"  vs oracle: {abs(e['esum'] - etot):.2e} {np.abs(g - gtot).max():.2e} {np.abs(v - vtot).max():.2e}")
+    tol = (1e-6, 2e-5, 1e-4) if prec == t9hip.T9_FP64 else (1e-5, 1e-4, 1e-3)
+    assert abs(e["esum"] - etot) < tol[0] * abs(etot)
+    assert np.abs(g - gtot).max() < tol[1] and np.abs(v - vtot).max() < tol[2]
+
+
+HIPPO_WATER_MD_KEY = ("a-axis 25.0\newald\newald-cutoff 7.0\nrepuls-cutoff 6.0\ndisp-cutoff 7.0\nchgtrn-cutoff 6.0\n"
+                      "polar-eps 1e-8\nvdwterm none\n")
+
+
+def test_hippo_water_nve_gpu(golden_dir):
+    """HIPPO water in the MD loop (FP64): ten waters of the HIPPO water set
+    with every term on (valence, Ewald multipoles and polarization, Pauli
+    repulsion, dispersion, charge transfer), 1-fs RESPA steps (two 0.5-fs
+    valence steps), no thermostat.  The total energy stays within 0.02
+    kcal/mol of its start over 200 steps: the forces of all the terms are the
+    gradient of the energy that the same calls report."""
+    print("case hippo water nve", flush=True)
+    s = t9hip.System.from_files(os.path.join(golden_dir, "h2o10.xyz"),
+                                prm=os.path.join(golden_dir, "water19_records.prm"), key=HIPPO_WATER_MD_KEY)
+    a = s.arrays()
+    assert a["chgpen"] and a["repel"] and a["disp"] and a["chgtrn"]
+    ctx = t9hip.Context(s, t9hip.T9_FP64)
+    ctx.md_init(298.0, 11)
+    etot = []
+    for _ in range(20):
+        st = ctx.md_run(10, 0.001, 2, 0.0)
+        etot.append(st["epot"] + st["ekin"])
+    etot = np.array(etot)
+    print(f"  E_total first {etot[0]:.5f} last {etot[-1]:.5f} max deviation {np.abs(etot - etot[0]).max():.2e} "
+          f"T {st['temperature']:.1f} K")
+    assert np.abs(etot - etot[0]).max() < 0.02
