@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_spread(TileGeom g, const int* __restric
 // order at the end.  Every grid point is written once, by one lane, and its
 // sum runs in an order fixed by the atoms' positions alone: bit-reproducible,
 // with no atomics and no fixed point.
-constexpr int kRowTile = 8, kRowBin = 4, kRowChunk = 128;
+constexpr int kRowTile = 8, kRowBin = 4, kRowChunk = 64;
 
 template <class real, int MODE>
 struct RowAtom {
@@ -460,9 +460,12 @@ __global__ __launch_bounds__(256) void k_spread_rows(TileGeom g, const int* __re
   using RA = RowAtom<real, MODE>;
   constexpr int LV = RA::LV;
   constexpr int NACC = MODE == 1 ? 1 : 2;
-  // multipole atoms carry 3 derivative orders: chunks of 64 keep the staging
-  // area small enough for more workgroups per CU (measured faster)
-  constexpr int CH = MODE != 0 ? kRowChunk / 2 : kRowChunk;
+  // chunks of 64 atoms keep the staging area small enough for more
+  // workgroups per CU: measured faster for multipoles and for FP64 dipoles
+  // (FP64 step 6 % faster).  FP32 dipoles keep 128: with 64 the spread
+  // alone is faster (66 -> 57 us) but the FP32 step time is the same, the
+  // earlier spread only overlapping more of the FFT with the mutual field
+  constexpr int CH = (MODE != 0 || sizeof(real) == 8) ? kRowChunk : 2 * kRowChunk;
   static_assert(CH % 64 == 0, "the sub-tile lists are built by whole wavefronts");
   constexpr size_t kStage = sizeof(RA) * CH, kPart = sizeof(real) * 2 * 64 * NACC * 4;
   __shared__ __align__(16) char smem[kStage > kPart ? kStage : kPart];
