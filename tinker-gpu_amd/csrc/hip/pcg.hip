@@ -51,17 +51,25 @@ __global__ __launch_bounds__(kB) void k_aspc_guess(int n, const int* __restrict_
   if (i >= n) return;
   const int s = iperm[i];
   const size_t slot = 6 * (size_t)n;
-  for (int d = 0; d < 3; ++d) {
-    double a = 0, b = 0;
-    for (int k = 0; k < kMaxUalt; ++k) {
-      a += cf.c[k] * (double)hist[k * slot + 3 * (size_t)i + d];
-      b += cf.c[k] * (double)hist[k * slot + 3 * (size_t)n + 3 * (size_t)i + d];
+  // one pass over the history: each slot's three d and three p components
+  // of the atom are read together (the per-component sums keep slot order)
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  for (int k = 0; k < kMaxUalt; ++k) {
+    const real* hd = hist + k * slot + 3 * (size_t)i;
+    const real* hp = hd + 3 * (size_t)n;
+    const double c = cf.c[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      a[d] += c * (double)hd[d];
+      b[d] += c * (double)hp[d];
     }
-    ud[3 * s + d] = (real)a;
-    up[3 * s + d] = (real)b;
+  }
+  for (int d = 0; d < 3; ++d) {
+    ud[3 * s + d] = (real)a[d];
+    up[3 * s + d] = (real)b[d];
     if (keep) {  // a copy of the ASPC guess (d, then p) for the learned correction's fallback
-      keep[3 * s + d] = (real)a;
-      keep[3 * (size_t)n + 3 * s + d] = (real)b;
+      keep[3 * s + d] = (real)a[d];
+      keep[3 * (size_t)n + 3 * s + d] = (real)b[d];
     }
   }
 }
